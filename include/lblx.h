@@ -1,0 +1,163 @@
+/*
+ * lblx.h -- C ABI of the MI355X monochromatic path for LBLRTM.
+ *
+ * The reference has no plugin API: the hot path sits behind three Fortran
+ * subroutines with COMMON-block state, called from the driver XLAYER:
+ *   OPDPTH(MPTS)                  src/lblrtm.f90:6135  (CONTNM, LINF4, HIRAC1)
+ *   XMERGE(NPTS,LFILE,MFILE,JPATHL) src/xmerge.f90:25   (EMINIT/RADMRG)
+ *   FFTSCN(IFILE,JFILE)           src/fftscn_dbl.f90:25
+ * The entry points below replace those bodies.  Each takes explicitly what
+ * the Fortran routine reads from COMMON (/FILHDR/, /MANE/, /CNTSCL/) and
+ * returns in caller-owned memory what the Fortran routine writes to its
+ * unit (TAPE10 panels of layer optical depth, TAPE12 radiance and
+ * transmittance).  All wavenumbers are double.  Every call returns
+ * LBLX_OK (0) or a negative LBLX_E* code; the library never exits, the
+ * Fortran shim maps a code to the reference's STOP text.
+ *
+ * Threading: one context per host thread; calls are synchronous to the
+ * caller.  A context owns device memory, the resident line table and one
+ * HIP stream.
+ */
+#ifndef LBLX_H
+#define LBLX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* error codes                                                         */
+enum {
+  LBLX_OK = 0,
+  LBLX_E_ARG = -1,          /* bad argument (null pointer, size)           */
+  LBLX_E_NODEV = -2,        /* no HIP device / library built without GPU   */
+  LBLX_E_HIP = -3,          /* a HIP runtime call failed                   */
+  LBLX_E_NREC = -4,         /* 'RDLIN; NREC GT LIMIN' (oprop.f90:763)      */
+  LBLX_E_UNSUPPORTED = -5,  /* input outside the supported subset          */
+  LBLX_E_SIZE = -6,         /* output buffer too small                     */
+  LBLX_E_NOLINES = -7,      /* no line table loaded                        */
+  LBLX_E_DVRATIO = -8,      /* layer DV ratio above 2.5 (lblrtm.f90:5479)  */
+  LBLX_E_COARSER = -9       /* 'RADMRG; ITYPE LT 0' (xmerge.f90:2741)      */
+};
+
+/* Precision of the line-shape evaluation and panel accumulation.        */
+enum {
+  LBLX_FP64 = 0,            /* plain drop-in, everything in double         */
+  LBLX_FP32_SPLIT = 1       /* FP32 shape and accumulation, FP64 positions */
+};
+
+/* ------------------------------------------------------------------ */
+/* Line table in the LNFL TAPE3 block layout (src/struct_types.f90:28-44):
+ * nblk blocks of LBLX_LINREC slots; block b holds nrec[b] records in
+ * slots [b*LBLX_LINREC, b*LBLX_LINREC + nrec[b]).  vmin/vmax are the
+ * block-header wavenumbers RDLIN uses to skip blocks (oprop.f90:768).
+ * mol = molecule + 100*isotope, iflg < 0 marks a line-coupling record. */
+#define LBLX_LINREC 250
+typedef struct lblx_lines {
+  int32_t nblk;
+  const int32_t *nrec;              /* [nblk]                              */
+  const double *vmin, *vmax;        /* [nblk]                              */
+  const double *vnu;                /* [nblk*250] line centre, cm-1        */
+  const float *sp;                  /* intensity at 296 K, cm/molecule     */
+  const float *alfa;                /* air-broadened HWHM at 1 atm, 296 K  */
+  const float *epp;                 /* lower-state energy, cm-1            */
+  const int32_t *mol;               /* mol + 100*iso                       */
+  const float *hwhm;                /* self-broadened HWHM                 */
+  const float *tmpalf;              /* width temperature exponent          */
+  const float *pshift;              /* pressure shift, cm-1/atm            */
+  const int32_t *iflg;              /* coupling flag                       */
+} lblx_lines;
+
+/* Run-level TAPE5 record 1.2/1.3 values (src/lblrtm.f90:674-676,909-951)
+ * after the reference's defaulting (ALFAL0<=0 -> 0.04, SAMPLE<=0 -> 4,
+ * DPTMIN<0 -> 2e-4, DPTFAC<0 -> 1e-3).                                   */
+typedef struct lblx_run {
+  double v1, v2;            /* band, cm-1                                  */
+  double sample;            /* SAMPLE: average halfwidth / DV              */
+  double alfal0;            /* ALFAL0                                      */
+  double avmass;            /* AVMASS                                      */
+  double dptmin, dptfac;    /* line rejection thresholds                   */
+  int32_t ilblf4;           /* ILBLF4 0/1/2                                */
+  int32_t icntnm;           /* ICNTNM (0 = continuum off)                  */
+  int32_t irad;             /* IRAD != 0 -> JRAD = -1 (lblrtm.f90:832-833) */
+  int32_t precision;        /* LBLX_FP64 or LBLX_FP32_SPLIT                */
+} lblx_run;
+
+/* One layer, as /FILHDR/ holds it when OPDPTH is called.                */
+#define LBLX_MXMOL 64
+typedef struct lblx_layer {
+  double pave, tave;        /* mb, K                                       */
+  double wk[LBLX_MXMOL];    /* column amounts of molecules 1..nmol         */
+  double wbroad;            /* broadening column                           */
+  double dv;                /* layer fine-grid spacing (PATH, lblrtm.f90:5407-5500) */
+  double v1, v2;            /* layer band (normally the run band)          */
+  int32_t nmol;
+  int32_t jrad;             /* radiation flag carried across layers:
+                               1 initially, set to 0 by OPDPTH when
+                               V1/kT < 5 (lblrtm.f90:6211)                  */
+} lblx_layer;
+
+/* Counters HIRAC1/LBLF4 print on TAPE6 (oprop.f90:547-631), also the
+ * denominators of the roofline figures.                                   */
+typedef struct lblx_stats {
+  int64_t nlin;             /* lines read with non-zero strength (NLIN)    */
+  int64_t lincnt;           /* lines after rejection (LINCNT)              */
+  int64_t nlins4;           /* effective lines after SHRINK                */
+  int64_t ilin4;            /* far-wing lines after rejection (ILIN4)      */
+  int64_t npanel;           /* panels emitted                              */
+  int64_t npts;             /* output points                               */
+  int64_t k1_updates;       /* near-wing (R1,R2,R3) point updates          */
+  int64_t k3_updates;       /* far-wing R4 point updates                   */
+} lblx_stats;
+
+/* ------------------------------------------------------------------ */
+/* Context                                                              */
+typedef struct lblx_ctx lblx_ctx;
+
+int lblx_create(lblx_ctx **ctx, int device);
+void lblx_destroy(lblx_ctx *ctx);
+const char *lblx_strerror(int code);
+int lblx_device_count(void);
+
+/* Parse an LNFL TAPE3 (sequential unformatted, 4-byte record markers) and
+ * keep the line table resident on the device.  Replaces the per-layer
+ * re-read of TAPE3 in RDLIN/RDLNFL (oprop.f90:734-776, 3371-3507).       */
+int lblx_load_tape3(lblx_ctx *ctx, const char *path);
+/* Same from host arrays in the TAPE3 block layout.                        */
+int lblx_load_lines(lblx_ctx *ctx, const lblx_lines *lines);
+
+/* Layer DV selection of PATH (lblrtm.f90:5407-5500) for nlay layers:
+ * fills dv[nlay] and itype[nlay].  w* are per-layer column amounts
+ * (wk is [nlay][nmol], row-major).  iemit is IEMIT of record 1.2.        */
+int lblx_plan_dv(const lblx_run *run, int nlay, int nmol, const double *pave,
+                 const double *tave, const double *wk, const double *wbroad,
+                 int iemit, double *dv, int32_t *itype);
+
+/* Number of output points OPDPTH writes for a layer (sum of panel NLIM). */
+int lblx_od_size(const lblx_run *run, const lblx_layer *layer, int64_t *npts);
+
+/* OPDPTH for one layer (IHIRAC=1, IOD=0): continuum (ICNTNM) + LINF4 +
+ * HIRAC1/LBLF4 panels.  od receives npts values on the grid
+ * v1p + k*dv (v1p returned); double for LBLX_FP64, float for
+ * LBLX_FP32_SPLIT.  layer->jrad is updated as OPDPTH updates FSCDID(9).  */
+int lblx_od_layer(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layer,
+                  void *od, int64_t n_out, double *v1p, lblx_stats *st);
+
+/* As lblx_od_layer, with od in device memory (HBM) of the context's
+ * device; the call returns when the work is queued on the context stream.
+ * Used when the next stage (merge, scan) also runs on the device.        */
+int lblx_od_layer_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layer,
+                         void *d_od, int64_t n_out, double *v1p, lblx_stats *st);
+
+/* Wait for the context stream; the stream handle (hipStream_t) for callers
+ * that order their own device work after the library's.                  */
+int lblx_sync(lblx_ctx *ctx);
+void *lblx_stream(lblx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBLX_H */

@@ -1,0 +1,185 @@
+"""ctypes mirror of include/lblx.h (structs and error codes).
+
+The same structures are used to call the HIP library (liblblx.so) and, in
+tests only, the CPU oracle library.
+"""
+import ctypes as C
+
+import numpy as np
+
+LINREC = 250
+MXMOL = 64
+
+OK = 0
+E_ARG = -1
+E_NODEV = -2
+E_HIP = -3
+E_NREC = -4
+E_UNSUPPORTED = -5
+E_SIZE = -6
+E_NOLINES = -7
+E_DVRATIO = -8
+E_COARSER = -9
+
+FP64 = 0
+FP32_SPLIT = 1
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+
+class Lines(C.Structure):
+    _fields_ = [
+        ("nblk", C.c_int32),
+        ("nrec", _ip),
+        ("vmin", _dp),
+        ("vmax", _dp),
+        ("vnu", _dp),
+        ("sp", _fp),
+        ("alfa", _fp),
+        ("epp", _fp),
+        ("mol", _ip),
+        ("hwhm", _fp),
+        ("tmpalf", _fp),
+        ("pshift", _fp),
+        ("iflg", _ip),
+    ]
+
+
+class Run(C.Structure):
+    _fields_ = [
+        ("v1", C.c_double),
+        ("v2", C.c_double),
+        ("sample", C.c_double),
+        ("alfal0", C.c_double),
+        ("avmass", C.c_double),
+        ("dptmin", C.c_double),
+        ("dptfac", C.c_double),
+        ("ilblf4", C.c_int32),
+        ("icntnm", C.c_int32),
+        ("irad", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
+class Layer(C.Structure):
+    _fields_ = [
+        ("pave", C.c_double),
+        ("tave", C.c_double),
+        ("wk", C.c_double * MXMOL),
+        ("wbroad", C.c_double),
+        ("dv", C.c_double),
+        ("v1", C.c_double),
+        ("v2", C.c_double),
+        ("nmol", C.c_int32),
+        ("jrad", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("nlin", C.c_int64),
+        ("lincnt", C.c_int64),
+        ("nlins4", C.c_int64),
+        ("ilin4", C.c_int64),
+        ("npanel", C.c_int64),
+        ("npts", C.c_int64),
+        ("k1_updates", C.c_int64),
+        ("k3_updates", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class LineTable:
+    """Host line table in the TAPE3 block layout; owns the numpy arrays the
+    Lines struct points into."""
+
+    FIELDS = [("vnu", np.float64), ("sp", np.float32), ("alfa", np.float32),
+              ("epp", np.float32), ("mol", np.int32), ("hwhm", np.float32),
+              ("tmpalf", np.float32), ("pshift", np.float32), ("iflg", np.int32)]
+
+    def __init__(self, nrec, vmin, vmax, **arrays):
+        self.nrec = np.ascontiguousarray(nrec, dtype=np.int32)
+        self.vmin = np.ascontiguousarray(vmin, dtype=np.float64)
+        self.vmax = np.ascontiguousarray(vmax, dtype=np.float64)
+        nblk = len(self.nrec)
+        for name, dt in self.FIELDS:
+            a = np.ascontiguousarray(arrays[name], dtype=dt)
+            assert a.shape == (nblk * LINREC,), (name, a.shape)
+            setattr(self, name, a)
+        self.nblk = nblk
+
+    @classmethod
+    def from_lines(cls, vnu, sp, alfa, epp, mol, hwhm, tmpalf, pshift, iflg=None):
+        """Pack sorted per-line arrays into full 250-slot blocks."""
+        n = len(vnu)
+        nblk = max(1, (n + LINREC - 1) // LINREC)
+        tot = nblk * LINREC
+        out = {}
+        src = dict(vnu=vnu, sp=sp, alfa=alfa, epp=epp, mol=mol, hwhm=hwhm,
+                   tmpalf=tmpalf, pshift=pshift,
+                   iflg=np.zeros(n, np.int32) if iflg is None else iflg)
+        for name, dt in cls.FIELDS:
+            a = np.zeros(tot, dtype=dt)
+            a[:n] = np.asarray(src[name], dtype=dt)
+            out[name] = a
+        nrec = np.full(nblk, LINREC, np.int32)
+        nrec[-1] = n - (nblk - 1) * LINREC if n else 0
+        vmin = np.zeros(nblk)
+        vmax = np.zeros(nblk)
+        for b in range(nblk):
+            seg = out["vnu"][b * LINREC:b * LINREC + nrec[b]]
+            if len(seg):
+                vmin[b] = seg.min()
+                vmax[b] = seg.max()
+        return cls(nrec, vmin, vmax, **out)
+
+    def nlines(self):
+        return int(self.nrec.sum())
+
+    def struct(self):
+        s = Lines()
+        s.nblk = self.nblk
+        s.nrec = ptr(self.nrec, C.c_int32)
+        s.vmin = ptr(self.vmin, C.c_double)
+        s.vmax = ptr(self.vmax, C.c_double)
+        s.vnu = ptr(self.vnu, C.c_double)
+        s.sp = ptr(self.sp, C.c_float)
+        s.alfa = ptr(self.alfa, C.c_float)
+        s.epp = ptr(self.epp, C.c_float)
+        s.mol = ptr(self.mol, C.c_int32)
+        s.hwhm = ptr(self.hwhm, C.c_float)
+        s.tmpalf = ptr(self.tmpalf, C.c_float)
+        s.pshift = ptr(self.pshift, C.c_float)
+        s.iflg = ptr(self.iflg, C.c_int32)
+        self._s = s
+        return s
+
+
+def make_run(v1, v2, sample=4.0, alfal0=0.04, avmass=36.0, dptmin=2e-4,
+             dptfac=1e-3, ilblf4=1, icntnm=0, irad=0, precision=FP64):
+    r = Run()
+    r.v1, r.v2 = v1, v2
+    r.sample, r.alfal0, r.avmass = sample, alfal0, avmass
+    r.dptmin, r.dptfac = dptmin, dptfac
+    r.ilblf4, r.icntnm, r.irad, r.precision = ilblf4, icntnm, irad, precision
+    return r
+
+
+def make_layer(pave, tave, wk, wbroad, dv, v1, v2, jrad=1):
+    L = Layer()
+    L.pave, L.tave = pave, tave
+    for i, w in enumerate(wk):
+        L.wk[i] = w
+    L.nmol = len(wk)
+    L.wbroad = wbroad
+    L.dv, L.v1, L.v2 = dv, v1, v2
+    L.jrad = jrad
+    return L

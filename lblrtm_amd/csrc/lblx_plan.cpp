@@ -1,0 +1,401 @@
+// Host planner: see lblx_plan.h.  Reference lines are cited per block.
+#include "lblx_plan.h"
+
+#include <cmath>
+#include <cstring>
+
+#include "lblx_refdata.h"
+
+namespace lblx {
+
+static const double BOLTZ = 1.3806503E-16;   // phys_consts.f90:36
+static const double CLIGHT = 2.99792458E+10; // phys_consts.f90:37
+static const double AVOGAD = 6.02214199E+23; // phys_consts.f90:38
+
+// RADFN (oprop.f90:2235-2296): nu * tanh(nu / 2kT) with its three regimes.
+double radfn(double vi, double xkt) {
+  if (xkt > 0.0) {
+    double x = vi / xkt;
+    if (x <= 0.01) return 0.5 * x * vi;
+    if (x <= 10.0) {
+      double e = std::exp(-x);
+      return vi * (1. - e) / (1. + e);
+    }
+  }
+  return vi;
+}
+
+// RADFNI (oprop.f90:2297-2475): radiation term at vi and the next node
+// (*vinew); *rdel is the per-step slope of the chord to that node.
+double radfni(double vi, double dvi, double xkt, double *vinew, double *rdel,
+              double *rdlast) {
+  const double FACT1 = 3.0E-03;
+  if (*rdlast < 0.) *rdlast = radfn(vi, xkt);
+  double rad = *rdlast;
+  int n = 1;
+  double rdnext;
+  auto step = [&](double target) {
+    if (*vinew >= 0.0) {
+      *vinew = target;
+      n = fint((*vinew - vi) / dvi);
+      if (n < 1) n = 1;
+      *vinew = vi + dvi * (double)n;
+    } else {
+      *vinew = std::fabs(*vinew);
+      n = fint((*vinew - vi) / dvi);
+      if (n < 1) n = 1;
+    }
+  };
+  if (xkt > 0.0) {
+    double x = vi / xkt;
+    if (x <= 0.01) {
+      step(vi + FACT1 * 0.5 * vi);
+      double xn = *vinew / xkt;
+      rdnext = 0.5 * xn * *vinew;
+    } else if (x <= 10.0) {
+      double e = std::exp(-x);
+      double xm = 1. - e, xp = 1. + e;
+      double c = x * e;
+      step(vi + FACT1 * vi / (1. + (c / xm + c / xp)));
+      double en = std::exp(-(*vinew / xkt));
+      rdnext = *vinew * (1. - en) / (1. + en);
+    } else {
+      step(vi + (FACT1 * vi));
+      rdnext = *vinew;
+    }
+  } else {
+    step(vi + (FACT1 * vi));
+    rdnext = *vinew;
+  }
+  *rdel = (rdnext - rad) / (double)n;
+  *rdlast = rdnext;
+  return rad;
+}
+
+// TIPS_2003 with AtoB Lagrange interpolation (oprop.f90:5919-6295,
+// 11708-11798) for molecules 1..7.
+static double atob(double aa, const double *A, const double *B, int npt) {
+  auto d = [](double x) { return x == 0. ? 0.0001 : x; };
+  for (int i = 2; i <= npt; i++) {
+    if (A[i - 1] < aa) continue;
+    auto a = [&](int k) { return A[k - 1]; };
+    auto b = [&](int k) { return B[k - 1]; };
+    if (i < 3 || i == npt) {
+      int j = i < 3 ? 3 : npt;
+      double a0 = (aa - a(j - 1)) * (aa - a(j)) / (d(a(j - 2) - a(j - 1)) * d(a(j - 2) - a(j)));
+      double a1 = (aa - a(j - 2)) * (aa - a(j)) / (d(a(j - 1) - a(j - 2)) * d(a(j - 1) - a(j)));
+      double a2 = (aa - a(j - 2)) * (aa - a(j - 1)) / (d(a(j) - a(j - 2)) * d(a(j) - a(j - 1)));
+      return a0 * b(j - 2) + a1 * b(j - 1) + a2 * b(j);
+    }
+    int j = i;
+    double a0 = (aa - a(j - 1)) * (aa - a(j)) * (aa - a(j + 1));
+    a0 = a0 / (d(a(j - 2) - a(j - 1)) * d(a(j - 2) - a(j)) * d(a(j - 2) - a(j + 1)));
+    double a1 = (aa - a(j - 2)) * (aa - a(j)) * (aa - a(j + 1));
+    a1 = a1 / (d(a(j - 1) - a(j - 2)) * d(a(j - 1) - a(j)) * d(a(j - 1) - a(j + 1)));
+    double a2 = (aa - a(j - 2)) * (aa - a(j - 1)) * (aa - a(j + 1));
+    a2 = a2 / (d(a(j) - a(j - 2)) * d(a(j) - a(j - 1)) * d(a(j) - a(j + 1)));
+    double a3 = (aa - a(j - 2)) * (aa - a(j - 1)) * (aa - a(j));
+    a3 = a3 / (d(a(j + 1) - a(j - 2)) * d(a(j + 1) - a(j - 1)) * d(a(j + 1) - a(j)));
+    return a0 * b(j - 2) + a1 * b(j - 1) + a2 * b(j) + a3 * b(j + 1);
+  }
+  return 0.;
+}
+
+void tips_scor(double temp, double scor[7][10]) {
+  for (int m = 0; m < 7; m++)
+    for (int iso = 0; iso < 10; iso++) {
+      scor[m][iso] = 0.;
+      if (iso >= LBLX_ISO_MAX[m] || temp < 70. || temp > 3000.) continue;
+      double q296 = atob(296., LBLX_TDAT, LBLX_QOFT[m][iso], LBLX_NT_TIPS);
+      double qt = atob(temp, LBLX_TDAT, LBLX_QOFT[m][iso], LBLX_NT_TIPS);
+      scor[m][iso] = q296 / qt;
+    }
+}
+
+int plan_dv(const lblx_run *run, int nlay, int nmol, const double *pave,
+            const double *tave, const double *wk, const double *wbroad,
+            int iemit, double *dv_out, int32_t *itype_out) {
+  if (!run || nlay <= 0 || nmol <= 0 || !pave || !tave || !wk || !wbroad || !dv_out)
+    return LBLX_E_ARG;
+  double DV = 0.;
+  for (int l = 0; l < nlay; l++) {
+    double sumwk = 0.;
+    for (int m = 0; m < nmol; m++) sumwk = sumwk + wk[(size_t)l * nmol + m];
+    double wtotl = sumwk + wbroad[l];
+    double frh2o = wk[(size_t)l * nmol] / wtotl;
+    double alfcor = (pave[l] / P0) * std::sqrt(TEMP0 / tave[l]);
+    double h2oslf = (1. - frh2o + 5. * frh2o);
+    double albar = run->alfal0 * alfcor * h2oslf;
+    double adbar = 3.58115E-07 * (0.5 * (run->v1 + run->v2)) * std::sqrt(tave[l] / run->avmass);
+    double avbar = 0.5 * (albar + std::sqrt(albar * albar + 4. * adbar * adbar));
+    double olddv = DV;
+    DV = avbar / run->sample;
+    int itype = 99;
+    if (l == 0) {  // three significant figures, even (lblrtm.f90:5444-5457)
+      int iscal = fint(std::log10(DV) - 3.);
+      double scal = std::pow(10., (double)iscal);
+      int idv = fint((DV / scal) + 0.5);
+      if (idv % 2 > 0) idv = idv + 1;
+      DV = scal * (double)idv;
+    } else {     // DV ratio chaining (lblrtm.f90:5460-5498)
+      double type = olddv / DV;
+      if (type > 2.5) return LBLX_E_DVRATIO;
+      if (type >= 1.2) {
+        itype = fint(1. / (type - 1.) + 0.5);
+        if (itype == 3) itype = 2;
+        DV = olddv * (double)itype / (double)(itype + 1);
+      } else if (type >= 0.8) {
+        DV = olddv;
+        itype = 0;
+      } else {
+        DV = olddv;
+        itype = 0;
+        if (iemit == 0) {
+          itype = fint(type / (1. - type) + 0.5);
+          DV = DV * (double)(itype + 1) / (double)itype;
+          itype = -itype;
+        }
+      }
+    }
+    dv_out[l] = DV;
+    if (itype_out) itype_out[l] = itype;
+  }
+  return LBLX_OK;
+}
+
+// LBLF4 window geometry (oprop.f90:3688-3700) and, for JRAD = 1, the R4
+// radiation segments (oprop.f90:3733-3758).
+static void add_window(LayerPlan *P, double v1, double v2, int p_first) {
+  WinPlan w{};
+  w.v1r4 = v1;
+  w.v2r4 = v2;
+  w.nptr4 = fint((w.v2r4 - w.v1r4) / P->dvr4 + ONEPL);
+  if (w.nptr4 > LIMOUT4) w.nptr4 = LIMOUT4;
+  w.v2r4 = w.v1r4 + P->dvr4 * (double)(w.nptr4 - 1);
+  w.p_first = p_first;
+  w.seg0 = (int32_t)P->segs.size();
+  w.nseg = 0;
+  if (P->jrad == 1) {
+    double xkt = 1. / (RADCN2 / P->tave);  // LBLF4: XKT = 1/BETA, BETA = RADCN2/TAVE
+    double vitst = w.v1r4 - P->dvr4, rdlast = -1.;
+    int n1 = 0, n2 = 0;
+    do {
+      n1 = n2 + 1;
+      double vi = w.v1r4 + P->dvr4 * (double)(n1 - 1);
+      double rdel;
+      double radvi = radfni(vi, P->dvr4, xkt, &vitst, &rdel, &rdlast);
+      n2 = fint((vitst - w.v1r4) / P->dvr4 + 0.001);
+      if (n2 > w.nptr4) n2 = w.nptr4;
+      if (n2 >= n1) P->segs.push_back(RadSeg{n1, n2, radvi, rdel});
+      else if (n2 < n1) { /* empty segment: the Fortran loop does nothing */ }
+    } while (n2 < w.nptr4);
+    w.nseg = (int32_t)P->segs.size() - w.seg0;
+  }
+  P->wins.push_back(w);
+}
+
+int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P) {
+  if (!run || !Ly || !P) return LBLX_E_ARG;
+  if (Ly->nmol < 1 || Ly->nmol > 7) return LBLX_E_UNSUPPORTED;
+  if (run->ilblf4 < 1) return LBLX_E_UNSUPPORTED;
+  *P = LayerPlan{};
+  const double V1 = Ly->v1, V2 = Ly->v2, DV = Ly->dv;
+  const double SAMPLE = run->sample, ALFAL0 = run->alfal0;
+  if (!(DV > 0.) || !(V2 > V1)) return LBLX_E_ARG;
+  P->v1 = V1;
+  P->v2 = V2;
+  P->dv = DV;
+  // OPDPTH (lblrtm.f90:6202-6256)
+  double xkt_l = Ly->tave / RADCN2;
+  if ((V1 / xkt_l) < 5. && Ly->jrad != -1) Ly->jrad = 0;
+  P->jrad = Ly->jrad;
+  P->xkt = xkt_l;
+  P->tave = Ly->tave;
+  P->pave = Ly->pave;
+  int ipflag = (run->icntnm != 0 && Ly->pave <= 0.5) ? 1 : 0;
+  double alfav = SAMPLE * DV;
+  double alfav4 = 64. * alfav;
+  P->dvr4 = alfav4 / SAMPLE;
+  P->bound4 = 25.;
+  if (run->ilblf4 == 2 && ipflag == 1) P->bound4 = 5.;
+  P->v1r4st = V1 - 2. * P->dvr4;
+  P->v2r4st = V2 + 2. * P->dvr4;
+  P->v1l4 = P->v1r4st - P->bound4 - P->dvr4;
+  P->v2l4 = P->v2r4st + P->bound4 + 2 * P->dvr4;
+  if (P->v1l4 <= 25.) return LBLX_E_UNSUPPORTED;  // no VVW mirror / RSYM on device
+  // HIRAC1 (oprop.f90:223-315)
+  P->dptmn_hirac = run->dptmin;
+  if (P->jrad != 1) P->dptmn_hirac = run->dptmin / radfn(V2, Ly->tave / RADCN2);
+  P->dptfc = run->dptfac;
+  P->dptmn4 = run->dptmin / radfn(P->v2r4st, Ly->tave / RADCN2);  // LBLF4 uses its V2 argument
+  P->alfmax = 4 * SAMPLE * DV * 0.04 / ALFAL0;
+  P->nalfmax = fint(P->alfmax / DV);
+  P->nbound = fint((2. * 64.) * P->nalfmax + 0.01);
+  P->dvr2 = (0.008 / 0.002) * DV;
+  P->dvr3 = (0.032 / 0.002) * DV;
+  int max1 = NSHIFT + NLIM1 + NSHIFT + P->nbound / 2;
+  int max2 = max1 / 4, max3 = max1 / 16;
+  P->max1 = max1 + 4 * P->nalfmax;
+  P->max2 = max2 + 16 * P->nalfmax / 4 + 1;
+  P->max3 = max3 + 64 * P->nalfmax / 16 + 1;
+  P->bound = (double)P->nbound * DV / 2.;
+  P->vft0 = V1 - (double)NSHIFT * DV;
+  P->vbot = V1 - P->bound;
+  P->vtop = V2 + P->bound;
+  if (P->vft0 <= 0.) return LBLX_E_UNSUPPORTED;
+  // molecular quantities (MOLEC, LNCOR1 and LINF4 first-call constants)
+  P->rhorat = (Ly->pave / P0) * (TEMP0 / Ly->tave);
+  P->tratio = Ly->tave / TEMP0;
+  P->pavp0 = Ly->pave / P0;
+  P->xkt0 = TEMP0 / RADCN2;
+  P->betacr_l = (1. / xkt_l) - (1. / P->xkt0);
+  P->beta4 = RADCN2 / Ly->tave;
+  P->betacr4 = P->beta4 - RADCN2 / TEMP0;
+  tips_scor(Ly->tave, P->scor);
+  double fad = std::log(2.) * 2. * AVOGAD * BOLTZ / (CLIGHT * CLIGHT);
+  for (int m = 0; m < 7; m++)
+    for (int i = 0; i < 10; i++)
+      P->alfd1[m][i] = i < LBLX_ISO_MAX[m] ? std::sqrt(fad * Ly->tave / LBLX_SMASS[m][i]) : 0.;
+  double sumwk = 0.;
+  for (int m = 0; m < Ly->nmol; m++) sumwk = sumwk + Ly->wk[m];
+  double wtot = sumwk + Ly->wbroad;
+  for (int m = 0; m < LBLX_MXMOL; m++) P->rhoslf[m] = 0.;
+  for (int m = 0; m < Ly->nmol; m++) P->rhoslf[m] = P->rhorat * Ly->wk[m] / wtot;
+
+  // Panel walk (HIRAC1 label 80 / PANEL / LBLF4 refill).
+  add_window(P, P->v1r4st, P->v2r4st, 0);
+  double VFT = P->vft0;
+  const int NLO = NSHIFT + 1;
+  int64_t out = 0;
+  for (int p = 0;; p++) {
+    PanelPlan pp{};
+    pp.vft = VFT;
+    pp.w = (int32_t)P->wins.size() - 1;
+    int NHI = NLIM1 + NSHIFT - 1;
+    int istop = 0;
+    if ((VFT + (NHI - 1) * DV) > V2) {
+      NHI = fint((V2 - VFT) / DV + 1.);
+      double v2p = VFT + (double)(NHI - 1) * DV;
+      if (v2p < V2) NHI = NHI + 1;
+      istop = 1;
+    }
+    pp.nhi = NHI;
+    pp.istop = istop;
+    pp.out0 = out;
+    out += NHI - NLO + 1;
+    pp.seg0 = (int32_t)P->segs.size();
+    if (P->jrad == 0) {  // PANEL radiation (oprop.f90:1492-1519)
+      double v1p = VFT + (double)(NLO - 1) * DV;
+      double vitst = v1p - DV, rdlast = -1.;
+      int n1 = NLO - 1, n2 = NLO - 1;
+      do {
+        n1 = n2 + 1;
+        double vi = VFT + (double)(n1 - 1) * DV;
+        double rdel;
+        double radvi = radfni(vi, DV, Ly->tave / RADCN2, &vitst, &rdel, &rdlast);
+        n2 = fint((vitst - VFT) / DV + 0.001);
+        if (n2 > NHI) n2 = NHI;
+        if (n2 >= n1) P->segs.push_back(RadSeg{n1, n2, radvi, rdel});
+      } while (n2 < NHI);
+    }
+    pp.nseg = (int32_t)P->segs.size() - pp.seg0;
+    P->panels.push_back(pp);
+    VFT = VFT + (double)(NLIM1 - 1) * DV;
+    if (istop) break;
+    // LBLF4 refill (oprop.f90:525-540); N2R3 = 155 for every panel
+    const int N2R3 = NLIM3 + (NSHIFT / 16) - 1 + 3;
+    double vf1 = VFT - 2. * P->dvr4;
+    vf1 = V1 + std::floor((vf1 - V1) / P->dvr4) * P->dvr4;
+    double vf2 = VFT + 2. * P->dvr4 + (double)(N2R3 + 4) * P->dvr3;
+    const WinPlan &cw = P->wins.back();
+    if (vf2 > cw.v2r4 && cw.v2r4 != P->v2r4st) add_window(P, vf1, P->v2r4st, p + 1);
+    if (p > 100000000) return LBLX_E_SIZE;
+  }
+  int np = (int)P->panels.size();
+  int pl = np - 1;
+  const PanelPlan &last = P->panels.back();
+  P->npts = out;
+  P->g_last = (int64_t)2400 * pl + (last.nhi - 1);
+  // R2 range that receives L4(R3) in the last panel (PANEL loop 10 + the
+  // ISTOP extra pair, oprop.f90:1459-1477)
+  {
+    int limlo = (np == 1) ? 5 : 13;
+    int limhi = last.nhi / 4 + 1;
+    if (limlo <= limhi) {
+      int jl = limlo + 4 * ((limhi - limlo) / 4);
+      P->g2max = (int64_t)(jl + 5) - 1 + (int64_t)600 * pl;
+    } else {
+      P->g2max = (int64_t)600 * pl + 11;
+    }
+  }
+  P->g3max = 154 + (int64_t)150 * pl;
+  P->n1 = (int64_t)2400 * pl + P->max1 + 16;
+  P->n2 = (int64_t)600 * pl + P->max2 + 16;
+  P->n3 = (int64_t)150 * pl + P->max3 + 16;
+  return LBLX_OK;
+}
+
+}  // namespace lblx
+
+namespace lblx {
+
+// Line-shape sub-function tables on 2601 points (SHAPEL, SHAPEG, VERFN,
+// oprop.f90:1822-1991): F1/F2/F3 = Lorentz minus the quadratic matched at
+// 4, 16 and 64 half-widths (grid 0.002, 0.008, 0.032), FG = normalised
+// Gaussian, XVER = the zeta = 0.3 Voigt correction.  tab holds F1, F2, F3,
+// FG, XVER back to back, entry k = Fortran index k+1.
+void shape_tables_host(double *tab) {
+  const int NX = 2001, NF = 2601;
+  double *f1 = tab, *f2 = tab + NF, *f3 = tab + 2 * NF, *fg = tab + 3 * NF, *xv = tab + 4 * NF;
+  for (int i = 0; i < 5 * NF; i++) tab[i] = 0.;
+  auto A = [](double z) { return (1. + 2. * z * z) / ((1. + z * z) * (1. + z * z)); };
+  auto B = [](double z) { return -1. / ((1. + z * z) * (1. + z * z)); };
+  const double recpi = 1. / (2. * std::asin(1.));
+  const double a1 = A(4.), b1 = B(4.), a2 = A(16.), b2 = B(16.), a3 = A(64.), b3 = B(64.);
+  auto lor = [](double xsq) { return 1. / (1. + xsq); };
+  f1[0] = recpi * (lor(0.) - (a1 + b1 * 0.));
+  for (int jj = 2; jj <= NX; jj++) {
+    double x = (double)(jj - 1) * 0.002, xsq = x * x;
+    f1[jj - 1] = recpi * (lor(xsq) - (a1 + b1 * xsq));
+  }
+  f1[NX - 1] = 0.;
+  f2[0] = recpi * ((a1 + b1 * 0.) - (a2 + b2 * 0.));
+  int j1lim = fint(4. / 0.008 + 1.001);
+  for (int jj = 2; jj <= j1lim; jj++) {
+    double x = (double)(jj - 1) * 0.008, xsq = x * x;
+    f2[jj - 1] = recpi * ((a1 + b1 * xsq) - (a2 + b2 * xsq));
+  }
+  for (int jj = j1lim + 1; jj <= NX; jj++) {
+    double x = (double)(jj - 1) * 0.008, xsq = x * x;
+    f2[jj - 1] = recpi * (lor(xsq) - (a2 + b2 * xsq));
+  }
+  f2[NX - 1] = 0.;
+  f3[0] = recpi * ((a2 + b2 * 0.) - (a3 + b3 * 0.));
+  int j2lim = fint(16. / 0.032 + 1.001);
+  for (int jj = 2; jj <= j2lim; jj++) {
+    double x = (double)(jj - 1) * 0.032, xsq = x * x;
+    f3[jj - 1] = recpi * ((a2 + b2 * xsq) - (a3 + b3 * xsq));
+  }
+  for (int jj = j2lim + 1; jj <= NX; jj++) {
+    double x = (double)(jj - 1) * 0.032, xsq = x * x;
+    f3[jj - 1] = recpi * (lor(xsq) - (a3 + b3 * xsq));
+  }
+  const double fln2 = std::log(2.);
+  const double fgnorm = std::sqrt(fln2 * recpi);
+  fg[0] = fgnorm * std::exp(-fln2 * 0.);
+  for (int jj = 2; jj <= NX; jj++) {
+    double x = (double)(jj - 1) * 0.002, xsq = x * x;
+    fg[jj - 1] = fgnorm * std::exp(-fln2 * xsq);
+  }
+  fg[NX - 1] = 0.;
+  const double PI = 3.1415926535898, CEXP = 0.45, CE0 = 1., CE2 = -.20737285249,
+               CE4 = -.00872684335747;
+  const double se0 = std::sqrt(CEXP / PI), ae2 = 2. * CEXP, ae4 = ae2 * ae2 / 3.;
+  for (int i = 1; i <= NF; i++) {
+    double z = 0.002 * (double)(i - 1), z2 = z * z;
+    double xe0 = se0 * std::exp(-CEXP * z2);
+    xv[i - 1] = (1. / (CE0 + CE2 + CE4)) * (CE0 + CE2 * ae2 * z2 + CE4 * ae4 * z2 * z2) * xe0;
+  }
+}
+
+}  // namespace lblx

@@ -1,0 +1,87 @@
+// Host-side planner for one layer: the panel sequence, far-wing window
+// schedule and radiation-term node tables that HIRAC1/LBLF4 would walk
+// through sequentially (src/oprop.f90:223-540, 1418-1558, 3641-3766).
+// They depend only on the layer grid, never on line data, so the host
+// computes them once per layer in double precision, with the same
+// operation order as the Fortran, and the kernels index them.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "../../include/lblx.h"
+
+namespace lblx {
+
+// Physical and numerical constants of the dbl build.
+constexpr double RADCN2 = 1.4387752;       // phys_consts.f90:42
+constexpr double RADCN1 = 1.191042722E-12; // phys_consts.f90:41
+constexpr double ONEPL = 1.001, ONEMI = 0.999;  // lblrtm.f90:631-632
+constexpr double P0 = 1013.25, TEMP0 = 296.;    // lblrtm.f90:2115-2116
+constexpr int NSHIFT = 32, NLIM1 = 2401, NLIM2 = 601, NLIM3 = 151;
+constexpr int LIMOUT4 = 2500;   // R4 window points (LBLF4 LIMOUT)
+constexpr int R4STRIDE = 2504;  // device row length of one window
+
+inline int fint(double x) { return (int)x; }
+
+// One segment of the piecewise-linear radiation term (RADFNI).
+struct RadSeg {
+  int32_t i1, i2;   // local point range (1-based, inclusive)
+  double radvi;     // value at i1
+  double rdel;      // increment per point
+};
+
+struct PanelPlan {
+  double vft;       // VFT of this panel
+  int32_t w;        // far-wing window active at XINT time
+  int32_t nhi;      // NHI (2432 except the last panel)
+  int32_t istop;    // last panel
+  int32_t seg0, nseg;  // radiation segments (JRAD = 0)
+  int64_t out0;     // index of the panel's first point in the layer output
+};
+
+struct WinPlan {
+  double v1r4, v2r4;
+  int32_t nptr4;
+  int32_t seg0, nseg;  // radiation segments of R4 (JRAD = 1)
+  int32_t p_first;     // first panel that uses this window
+};
+
+struct LayerPlan {
+  // grid
+  double dv, dvr2, dvr3, dvr4, bound4, alfmax, bound;
+  int32_t nalfmax, nbound, max1, max2, max3;
+  double v1, v2, vft0, vbot, vtop;
+  double v1r4st, v2r4st, v1l4, v2l4;
+  // radiation and rejection
+  int32_t jrad;
+  double tave, pave;
+  double xkt, dptmn_hirac, dptfc, dptmn4;
+  // molecular quantities
+  double rhorat, tratio, pavp0, xkt0, betacr_l, beta4, betacr4;
+  double scor[7][10], alfd1[7][10], rhoslf[LBLX_MXMOL];
+  // plan tables
+  std::vector<PanelPlan> panels;
+  std::vector<WinPlan> wins;
+  std::vector<RadSeg> segs;
+  int64_t npts;       // output points
+  int64_t g_last;     // last global fine index written
+  int64_t g2max;      // last R2 index that receives L4(R3)
+  int64_t g3max;      // last R3 index that receives XINT
+  int64_t n1, n2, n3; // array sizes (fine, R2, R3 grids)
+};
+
+// Build the plan; layer->jrad is updated as OPDPTH does.  Returns LBLX_OK
+// or an LBLX_E_* code.
+int plan_layer(const lblx_run *run, lblx_layer *layer, LayerPlan *P);
+
+// PATH DV selection for a column (lblrtm.f90:5336-5500).
+int plan_dv(const lblx_run *run, int nlay, int nmol, const double *pave,
+            const double *tave, const double *wk, const double *wbroad,
+            int iemit, double *dv, int32_t *itype);
+
+double radfn(double vi, double xkt);
+double radfni(double vi, double dvi, double xkt, double *vinew, double *rdel,
+              double *rdlast);
+void tips_scor(double temp, double scor[7][10]);
+
+}  // namespace lblx

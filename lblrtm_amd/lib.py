@@ -1,0 +1,153 @@
+"""Python host mirror of the C ABI in include/lblx.h (ctypes over
+lblrtm_amd/liblblx.so).
+
+The reference drives the path through OPDPTH (src/lblrtm.f90:6135),
+XMERGE (src/xmerge.f90:25) and FFTSCN (src/fftscn_dbl.f90:25); the
+methods of Context carry the same names and argument meaning.  There is no
+CPU fallback: without the built library or a HIP device every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "liblblx.so")
+_LIB = None
+
+
+class LblxError(RuntimeError):
+    def __init__(self, code, where):
+        msg = "%s failed: %s (%d)" % (where, strerror(code), code)
+        super().__init__(msg)
+        self.code = code
+
+
+def load():
+    """Load liblblx.so (raises when it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError("liblblx.so is not built; run python -m lblrtm_amd.build")
+    L = C.CDLL(LIBPATH)
+    P = C.POINTER
+    sig = {
+        "lblx_strerror": (C.c_char_p, [C.c_int]),
+        "lblx_device_count": (C.c_int, []),
+        "lblx_create": (C.c_int, [P(C.c_void_p), C.c_int]),
+        "lblx_destroy": (None, [C.c_void_p]),
+        "lblx_load_tape3": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "lblx_load_lines": (C.c_int, [C.c_void_p, P(abi.Lines)]),
+        "lblx_plan_dv": (C.c_int, [P(abi.Run), C.c_int, C.c_int, P(C.c_double), P(C.c_double),
+                                   P(C.c_double), P(C.c_double), C.c_int, P(C.c_double),
+                                   P(C.c_int32)]),
+        "lblx_od_size": (C.c_int, [P(abi.Run), P(abi.Layer), P(C.c_int64)]),
+        "lblx_od_layer": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_void_p, C.c_int64,
+                                    P(C.c_double), P(abi.Stats)]),
+        "lblx_od_layer_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_void_p,
+                                           C.c_int64, P(C.c_double), P(abi.Stats)]),
+        "lblx_sync": (C.c_int, [C.c_void_p]),
+        "lblx_stream": (C.c_void_p, [C.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def strerror(code):
+    try:
+        return load().lblx_strerror(code).decode()
+    except Exception:
+        return "error %d" % code
+
+
+def _chk(rc, where):
+    if rc != 0:
+        raise LblxError(rc, where)
+
+
+def plan_dv(run, pave, tave, wk, wbroad, iemit=0):
+    """Layer DV chain of PATH (lblrtm.f90:5407-5500)."""
+    L = load()
+    pave = np.ascontiguousarray(pave, np.float64)
+    tave = np.ascontiguousarray(tave, np.float64)
+    wk = np.ascontiguousarray(np.atleast_2d(wk), np.float64)
+    wbroad = np.ascontiguousarray(wbroad, np.float64)
+    nlay, nmol = wk.shape
+    dv = np.zeros(nlay)
+    it = np.zeros(nlay, np.int32)
+    _chk(L.lblx_plan_dv(C.byref(run), nlay, nmol, abi.ptr(pave, C.c_double),
+                        abi.ptr(tave, C.c_double), abi.ptr(wk, C.c_double),
+                        abi.ptr(wbroad, C.c_double), iemit, abi.ptr(dv, C.c_double),
+                        abi.ptr(it, C.c_int32)), "lblx_plan_dv")
+    return dv, it
+
+
+def od_size(run, layer):
+    n = C.c_int64(0)
+    _chk(load().lblx_od_size(C.byref(run), C.byref(layer), C.byref(n)), "lblx_od_size")
+    return n.value
+
+
+class Context:
+    """One device context: resident line table, stream and work buffers."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = C.c_void_p()
+        _chk(L.lblx_create(C.byref(h), device), "lblx_create")
+        self._h = h
+        self._lib = L
+
+    def close(self):
+        if self._h:
+            self._lib.lblx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_lines(self, table):
+        s = table.struct()
+        _chk(self._lib.lblx_load_lines(self._h, C.byref(s)), "lblx_load_lines")
+
+    def load_tape3(self, path):
+        _chk(self._lib.lblx_load_tape3(self._h, str(path).encode()), "lblx_load_tape3")
+
+    def opdpth(self, run, layer):
+        """Layer optical depth (OPDPTH/HIRAC1) into host memory.
+        Returns (od, v1p, stats); layer.jrad is updated in place."""
+        n = od_size(run, layer)
+        dt = np.float32 if run.precision == abi.FP32_SPLIT else np.float64
+        out = np.zeros(n, dt)
+        v1p = C.c_double(0.0)
+        st = abi.Stats()
+        _chk(self._lib.lblx_od_layer(self._h, C.byref(run), C.byref(layer),
+                                     out.ctypes.data_as(C.c_void_p), n, C.byref(v1p),
+                                     C.byref(st)), "lblx_od_layer")
+        return out, v1p.value, st.as_dict()
+
+    def opdpth_device(self, run, layer, dptr, n, stats=False):
+        """Layer optical depth into device memory at dptr (n points)."""
+        v1p = C.c_double(0.0)
+        st = abi.Stats() if stats else None
+        _chk(self._lib.lblx_od_layer_device(self._h, C.byref(run), C.byref(layer),
+                                            C.c_void_p(dptr), n, C.byref(v1p),
+                                            C.byref(st) if stats else None),
+             "lblx_od_layer_device")
+        return v1p.value, (st.as_dict() if stats else None)
+
+    def sync(self):
+        _chk(self._lib.lblx_sync(self._h), "lblx_sync")
+
+    def stream(self):
+        return self._lib.lblx_stream(self._h)

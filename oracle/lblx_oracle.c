@@ -1,0 +1,1197 @@
+/*
+ * lblx_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, double-precision restatement of LBLRTM's layer optical-depth
+ * path (the linuxGNUdbl build: every REAL is REAL*8), written as the
+ * checker for the HIP implementation.  It follows the Fortran control flow
+ * routine by routine -- panel by panel, block by block, line by line -- so
+ * that every order-dependent decision of the reference (line rejection
+ * against the running far-wing array, SHRINK batching, far-wing window
+ * refills) happens exactly as there.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's CPU-baseline leg may load it; the product never does.
+ *
+ * Compile with -ffp-contract=off: the reference build (gfortran, generic
+ * x86-64) evaluates each expression without fused multiply-adds.
+ *
+ * Routines and the reference lines they follow:
+ *   tips_scor      TIPS_2003 + AtoB          src/oprop.f90:5919-6295, 11708-11798
+ *   molec_layer    MOLEC(2)                  src/oprop.f90:2476-2557
+ *   shape_tables   SHAPEL, SHAPEG, VERFN     src/oprop.f90:1822-1991
+ *   radfn, radfni  RADFN, RADFNI             src/oprop.f90:2235-2475
+ *   linf4          LINF4 + RDLNFL + SHRINK   src/oprop.f90:2934-3293, 3371-3640
+ *   lblf4          LBLF4 + RDLIN4 + CONVF4   src/oprop.f90:3641-4039
+ *   rdlin, lncor1  RDLIN, LNCOR1             src/oprop.f90:678-1197
+ *   cnvfnv         CNVFNV                    src/oprop.f90:1200-1370
+ *   panel          PANEL                     src/oprop.f90:1371-1561
+ *   xint           XINT                      src/oprop.f90:2201-2234
+ *   oracle_od_layer OPDPTH + HIRAC1          src/lblrtm.f90:6135-6285,
+ *                                            src/oprop.f90:25-668
+ *   oracle_plan_dv PATH DV selection         src/lblrtm.f90:5336-5500
+ *
+ * Parity pins (tests/test_oracle.py): shape-table closed forms and sum
+ * rules, SCOR(296 K) = 1, single-line sums against the Voigt profile,
+ * linearity in strength, symmetry, and the known-answer DV of the
+ * single-layer case.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/lblx.h"
+#include "../lblrtm_amd/csrc/lblx_refdata.h"
+
+#define NFPTS 2001
+#define NFMX 2601
+#define NLIMIN 250
+
+/* phys_consts.f90:34-42 (REAL parameters, 8-byte in the dbl build) */
+static const double PI_ = 3.1415926535898;
+static const double BOLTZ = 1.3806503E-16;
+static const double CLIGHT = 2.99792458E+10;
+static const double AVOGAD = 6.02214199E+23;
+static const double RADCN2 = 1.4387752;
+/* /LAMCHN/ set in the driver, lblrtm.f90:631-634 */
+static const double ONEPL = 1.001, ONEMI = 0.999;
+/* XLAYER, lblrtm.f90:2115-2116 */
+static const double P0 = 1013.25, TEMP0 = 296.;
+
+static int fint(double x) { return (int)x; } /* Fortran INT(): toward zero */
+
+/* ------------------------------------------------------------------ */
+/* TIPS_2003 / AtoB: SCOR(m,iso) = Q(296)/Q(T) for molecules 1..7.      */
+static double atob(double aa, const double *A, const double *B, int npt)
+{
+  /* Fortran 1-based views */
+#define a(k) A[(k) - 1]
+#define b(k) B[(k) - 1]
+  double bb = 0.;
+  for (int i = 2; i <= npt; i++) {
+    if (a(i) >= aa) {
+      if (i < 3 || i == npt) {
+        int j = i;
+        if (i < 3) j = 3;
+        if (i == npt) j = npt;
+        double a0d1 = a(j - 2) - a(j - 1); if (a0d1 == 0.) a0d1 = 0.0001;
+        double a0d2 = a(j - 2) - a(j);     if (a0d2 == 0.) a0d2 = 0.0001;
+        double a1d1 = a(j - 1) - a(j - 2); if (a1d1 == 0.) a1d1 = 0.0001;
+        double a1d2 = a(j - 1) - a(j);     if (a1d2 == 0.) a1d2 = 0.0001;
+        double a2d1 = a(j) - a(j - 2);     if (a2d1 == 0.) a2d1 = 0.0001;
+        double a2d2 = a(j) - a(j - 1);     if (a2d2 == 0.) a2d2 = 0.0001;
+        double a0 = (aa - a(j - 1)) * (aa - a(j)) / (a0d1 * a0d2);
+        double a1 = (aa - a(j - 2)) * (aa - a(j)) / (a1d1 * a1d2);
+        double a2 = (aa - a(j - 2)) * (aa - a(j - 1)) / (a2d1 * a2d2);
+        bb = a0 * b(j - 2) + a1 * b(j - 1) + a2 * b(j);
+      } else {
+        int j = i;
+        double a0d1 = a(j - 2) - a(j - 1); if (a0d1 == 0.) a0d1 = 0.0001;
+        double a0d2 = a(j - 2) - a(j);     if (a0d2 == 0.) a0d2 = 0.0001;
+        double a0d3 = a(j - 2) - a(j + 1); if (a0d3 == 0.) a0d3 = 0.0001;
+        double a1d1 = a(j - 1) - a(j - 2); if (a1d1 == 0.) a1d1 = 0.0001;
+        double a1d2 = a(j - 1) - a(j);     if (a1d2 == 0.) a1d2 = 0.0001;
+        double a1d3 = a(j - 1) - a(j + 1); if (a1d3 == 0.) a1d3 = 0.0001;
+        double a2d1 = a(j) - a(j - 2);     if (a2d1 == 0.) a2d1 = 0.0001;
+        double a2d2 = a(j) - a(j - 1);     if (a2d2 == 0.) a2d2 = 0.0001;
+        double a2d3 = a(j) - a(j + 1);     if (a2d3 == 0.) a2d3 = 0.0001;
+        double a3d1 = a(j + 1) - a(j - 2); if (a3d1 == 0.) a3d1 = 0.0001;
+        double a3d2 = a(j + 1) - a(j - 1); if (a3d2 == 0.) a3d2 = 0.0001;
+        double a3d3 = a(j + 1) - a(j);     if (a3d3 == 0.) a3d3 = 0.0001;
+        double a0 = (aa - a(j - 1)) * (aa - a(j)) * (aa - a(j + 1));
+        a0 = a0 / (a0d1 * a0d2 * a0d3);
+        double a1 = (aa - a(j - 2)) * (aa - a(j)) * (aa - a(j + 1));
+        a1 = a1 / (a1d1 * a1d2 * a1d3);
+        double a2 = (aa - a(j - 2)) * (aa - a(j - 1)) * (aa - a(j + 1));
+        a2 = a2 / (a2d1 * a2d2 * a2d3);
+        double a3 = (aa - a(j - 2)) * (aa - a(j - 1)) * (aa - a(j));
+        a3 = a3 / (a3d1 * a3d2 * a3d3);
+        bb = a0 * b(j - 2) + a1 * b(j - 1) + a2 * b(j) + a3 * b(j + 1);
+      }
+      return bb;
+    }
+  }
+  return bb;
+#undef a
+#undef b
+}
+
+static double qt_of(int mol, int iso, double t)
+{
+  if (t < 70. || t > 3000.) return -1.;
+  return atob(t, LBLX_TDAT, LBLX_QOFT[mol - 1][iso - 1], LBLX_NT_TIPS);
+}
+
+void tips_scor(double temp, double scor[7][10])
+{
+  for (int m = 1; m <= 7; m++)
+    for (int iso = 1; iso <= 10; iso++) {
+      if (iso > LBLX_ISO_MAX[m - 1]) { scor[m - 1][iso - 1] = 0.; continue; }
+      double q296 = qt_of(m, iso, 296.);
+      double qt = qt_of(m, iso, temp);
+      scor[m - 1][iso - 1] = q296 / qt;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Line-shape sub-function tables (1-based, NFMX entries).              */
+typedef struct {
+  double f1[NFMX + 1], f2[NFMX + 1], f3[NFMX + 1], fg[NFMX + 1], xver[NFMX + 1];
+} shapes_t;
+
+void shape_tables(double *f1, double *f2, double *f3, double *fg, double *xv)
+{
+  /* arrays of NFMX+1 entries, Fortran index i stored at [i] */
+  const double HWF1 = 4., DXF1 = 0.002, HWF2 = 16., DXF2 = 0.008, HWF3 = 64., DXF3 = 0.032;
+  const int NX1 = NFPTS, NX2 = NFPTS, NX3 = NFPTS;
+#define AZ(z0) ((1. + 2. * (z0) * (z0)) / (((1. + (z0) * (z0))) * ((1. + (z0) * (z0)))))
+#define BZ(z0) (-1. / (((1. + (z0) * (z0))) * ((1. + (z0) * (z0)))))
+  double recpi = 1. / (2. * asin(1.));
+  double a1 = AZ(HWF1), b1 = BZ(HWF1), a2 = AZ(HWF2), b2 = BZ(HWF2), a3 = AZ(HWF3), b3 = BZ(HWF3);
+#define XLORNZ(xsq) (1. / (1. + (xsq)))
+  for (int i = 1; i <= NFMX; i++) f1[i] = 0.;
+  f1[1] = recpi * (XLORNZ(0.) - (a1 + b1 * 0.));
+  for (int jj = 2; jj <= NX1; jj++) {
+    double x = (double)(jj - 1) * DXF1, xsq = x * x;
+    f1[jj] = recpi * (XLORNZ(xsq) - (a1 + b1 * xsq));
+  }
+  f1[NX1] = 0.;
+  for (int i = 1; i <= NFMX; i++) f2[i] = 0.;
+  f2[1] = recpi * ((a1 + b1 * 0.) - (a2 + b2 * 0.));
+  int j1lim = fint(HWF1 / DXF2 + 1.001);
+  for (int jj = 2; jj <= j1lim; jj++) {
+    double x = (double)(jj - 1) * DXF2, xsq = x * x;
+    f2[jj] = recpi * ((a1 + b1 * xsq) - (a2 + b2 * xsq));
+  }
+  for (int jj = j1lim + 1; jj <= NX2; jj++) {
+    double x = (double)(jj - 1) * DXF2, xsq = x * x;
+    f2[jj] = recpi * (XLORNZ(xsq) - (a2 + b2 * xsq));
+  }
+  f2[NX2] = 0.;
+  for (int i = 1; i <= NFMX; i++) f3[i] = 0.;
+  f3[1] = recpi * ((a2 + b2 * 0.) - (a3 + b3 * 0.));
+  int j2lim = fint(HWF2 / DXF3 + 1.001);
+  for (int jj = 2; jj <= j2lim; jj++) {
+    double x = (double)(jj - 1) * DXF3, xsq = x * x;
+    f3[jj] = recpi * ((a2 + b2 * xsq) - (a3 + b3 * xsq));
+  }
+  for (int jj = j2lim + 1; jj <= NX3; jj++) {
+    double x = (double)(jj - 1) * DXF3, xsq = x * x;
+    f3[jj] = recpi * (XLORNZ(xsq) - (a3 + b3 * xsq));
+  }
+  /* SHAPEG */
+  double fln2 = log(2.);
+  double fgnorm = sqrt(fln2 * recpi);
+  for (int i = 1; i <= NFMX; i++) fg[i] = 0.;
+  fg[1] = fgnorm * exp(-fln2 * 0.);
+  for (int jj = 2; jj <= NX1; jj++) {
+    double x = (double)(jj - 1) * DXF1, xsq = x * x;
+    fg[jj] = fgnorm * exp(-fln2 * xsq);
+  }
+  fg[NX1] = 0.;
+  /* VERFN (zeta = 0.3 correction, all NFMX entries) */
+  const double CEXP = 0.45, CE0 = 1., CE2 = -.20737285249, CE4 = -.00872684335747;
+  double se0 = sqrt(CEXP / PI_);
+  double ae2 = 2. * CEXP;
+  double ae4 = ae2 * ae2 / 3.;
+  for (int i = 1; i <= NFMX; i++) {
+    double z = DXF1 * (double)(i - 1), z2 = z * z;
+    double xe0 = se0 * exp(-CEXP * z2);
+    xv[i] = (1. / (CE0 + CE2 + CE4)) * (CE0 + CE2 * ae2 * z2 + CE4 * ae4 * z2 * z2) * xe0;
+  }
+#undef AZ
+#undef BZ
+#undef XLORNZ
+}
+
+/* ------------------------------------------------------------------ */
+double radfn(double vi, double xkt)
+{
+  double xvi = vi;
+  if (xkt > 0.0) {
+    double xviokt = xvi / xkt;
+    if (xviokt <= 0.01) return 0.5 * xviokt * xvi;
+    if (xviokt <= 10.0) {
+      double expvkt = exp(-xviokt);
+      return xvi * (1. - expvkt) / (1. + expvkt);
+    }
+    return xvi;
+  }
+  return xvi;
+}
+
+/* RADFNI: returns radiation term at VI; updates *vinew, *rdel, *rdlast.  */
+double radfni(double vi, double dvi, double xkt, double *vinew, double *rdel, double *rdlast)
+{
+  const double FACT1 = 3.0E-03;
+  double xvi = vi;
+  if (*rdlast < 0.) {
+    if (xkt > 0.0) {
+      double xviokt = xvi / xkt;
+      if (xviokt <= 0.01) *rdlast = 0.5 * xviokt * xvi;
+      else if (xviokt <= 10.0) {
+        double expvkt = exp(-xviokt);
+        *rdlast = xvi * (1. - expvkt) / (1. + expvkt);
+      } else *rdlast = xvi;
+    } else *rdlast = xvi;
+  }
+  double rad = *rdlast;
+  int intvls = 1;
+  double rdnext;
+  if (xkt > 0.0) {
+    double xviokt = xvi / xkt;
+    if (xviokt <= 0.01) {
+      if (*vinew >= 0.0) {
+        *vinew = vi + FACT1 * 0.5 * xvi;
+        intvls = fint((*vinew - vi) / dvi);
+        if (intvls < 1) intvls = 1;
+        *vinew = vi + dvi * (double)intvls;
+      } else {
+        *vinew = fabs(*vinew);
+        intvls = fint((*vinew - vi) / dvi);
+        if (intvls < 1) intvls = 1;
+      }
+      double xvinew = *vinew;
+      double xvinewokt = xvinew / xkt;
+      rdnext = 0.5 * xvinewokt * xvinew;
+    } else if (xviokt <= 10.0) {
+      double expvkt = exp(-xviokt);
+      double xminus = 1. - expvkt;
+      double xplus = 1. + expvkt;
+      if (*vinew >= 0.0) {
+        double cvikt = xviokt * expvkt;
+        *vinew = vi + FACT1 * xvi / (1. + (cvikt / xminus + cvikt / xplus));
+        intvls = fint((*vinew - vi) / dvi);
+        if (intvls < 1) intvls = 1;
+        *vinew = vi + dvi * (double)intvls;
+      } else {
+        *vinew = fabs(*vinew);
+        intvls = fint((*vinew - vi) / dvi);
+        if (intvls < 1) intvls = 1;
+      }
+      double xvinew = *vinew;
+      double xvinewokt = xvinew / xkt;
+      double expvnewkt = exp(-xvinewokt);
+      double xminusnew = 1. - expvnewkt;
+      double xplusnew = 1. + expvnewkt;
+      rdnext = xvinew * xminusnew / xplusnew;
+    } else {
+      if (*vinew >= 0.0) {
+        *vinew = vi + (FACT1 * xvi);
+        intvls = fint((*vinew - vi) / dvi);
+        if (intvls < 1) intvls = 1;
+        *vinew = vi + dvi * (double)intvls;
+      } else {
+        *vinew = fabs(*vinew);
+        intvls = fint((*vinew - vi) / dvi);
+        if (intvls < 1) intvls = 1;
+      }
+      rdnext = *vinew;
+    }
+  } else {
+    if (*vinew >= 0.0) {
+      *vinew = vi + (FACT1 * xvi);
+      intvls = fint((*vinew - vi) / dvi);
+      if (intvls < 1) intvls = 1;
+      *vinew = vi + dvi * (double)intvls;
+    } else {
+      *vinew = fabs(*vinew);
+      intvls = fint((*vinew - vi) / dvi);
+      if (intvls < 1) intvls = 1;
+    }
+    rdnext = *vinew;
+  }
+  *rdel = (rdnext - rad) / (double)intvls;
+  *rdlast = rdnext;
+  return rad;
+}
+
+/* ------------------------------------------------------------------ */
+/* Per-layer molecular quantities (MOLEC IND=1 and IND=2).              */
+typedef struct {
+  double scor[7][10], alfd1[7][10], rhoslf[LBLX_MXMOL];
+  double fad;
+} molec_t;
+
+static void molec_layer(const lblx_layer *L, molec_t *M)
+{
+  double fln2 = log(2.);
+  M->fad = fln2 * 2. * AVOGAD * BOLTZ / (CLIGHT * CLIGHT);
+  double rhorat = (L->pave / P0) * (TEMP0 / L->tave);
+  double wtot = L->wbroad;
+  /* WTOTL = SUMWK + WBRODL (lblrtm.f90:5360-5365): sum first, then add */
+  double sumwk = 0.;
+  for (int m = 0; m < L->nmol; m++) sumwk = sumwk + L->wk[m];
+  wtot = sumwk + L->wbroad;
+  tips_scor(L->tave, M->scor);
+  for (int m = 1; m <= 7; m++)
+    for (int iso = 1; iso <= 10; iso++)
+      M->alfd1[m - 1][iso - 1] = iso <= LBLX_ISO_MAX[m - 1]
+        ? sqrt(M->fad * L->tave / LBLX_SMASS[m - 1][iso - 1]) : 0.;
+  for (int m = 0; m < LBLX_MXMOL; m++) M->rhoslf[m] = 0.;
+  for (int m = 1; m <= L->nmol; m++) M->rhoslf[m - 1] = rhorat * L->wk[m - 1] / wtot;
+}
+
+/* ------------------------------------------------------------------ */
+/* Shrunk far-wing line list (TAPE9 in memory).                        */
+typedef struct {
+  int n, cap;
+  double *vnu, *sp, *alfa, *epp, *spp;
+  int *mol;
+  int nblk, blkcap;
+  int *bstart, *bcount;
+  double *bvlo, *bvhi;
+} tape9_t;
+
+static void t9_reserve(tape9_t *t, int n)
+{
+  if (n <= t->cap) return;
+  int c = t->cap ? t->cap : 4096;
+  while (c < n) c *= 2;
+  t->vnu = realloc(t->vnu, c * sizeof(double));
+  t->sp = realloc(t->sp, c * sizeof(double));
+  t->alfa = realloc(t->alfa, c * sizeof(double));
+  t->epp = realloc(t->epp, c * sizeof(double));
+  t->spp = realloc(t->spp, c * sizeof(double));
+  t->mol = realloc(t->mol, c * sizeof(int));
+  t->cap = c;
+}
+
+static void t9_free(tape9_t *t)
+{
+  free(t->vnu); free(t->sp); free(t->alfa); free(t->epp); free(t->spp); free(t->mol);
+  free(t->bstart); free(t->bcount); free(t->bvlo); free(t->bvhi);
+  memset(t, 0, sizeof(*t));
+}
+
+/* SHRINK on the 1250-slot buffer, entries [ilo, ihi] (1-based).
+ * Returns the new IHI (= number of entries after compaction).          */
+typedef struct {
+  double vnu[1251], sp[1251], alfa[1251], epp[1251], spp[1251];
+  int mol[1251];
+} shrunk_t;
+
+static int shrink(shrunk_t *S, int *ilo_io, int ihi, double dvr4)
+{
+  int ilo = *ilo_io;
+  int j = ilo - 1;
+  double dv = dvr4 / 2.;
+  double vlmt = (ilo >= 1 && ilo <= 1250) ? S->vnu[ilo] + dv : 0.;
+  double sumal = 0., sumad = 0., sums = 0., sumv = 0., sumc = 0.;
+  double sumal2 = 0., sumad2 = 0., sums2 = 0., sumv2 = 0., sumc2 = 0.;
+  for (int i = ilo; i <= ihi; i++) {
+    if (S->sp[i] < 1.0e-35) { S->sp[i] = 0.0; S->spp[i] = 0.0; }
+    if (S->spp[i] != 0.0) {
+      j = j + 1;
+      S->vnu[j] = S->vnu[i]; S->sp[j] = S->sp[i]; S->alfa[j] = S->alfa[i];
+      S->epp[j] = S->epp[i]; S->spp[j] = S->spp[i]; S->mol[j] = S->mol[i];
+      goto label10;
+    }
+    if (S->mol[i] != 2) {
+      sumv = sumv + S->vnu[i] * S->sp[i];
+      sums = sums + S->sp[i];
+      sumal = sumal + S->sp[i] * S->alfa[i];
+      sumad = sumad + S->sp[i] * S->epp[i];
+      sumc = sumc + S->spp[i];
+    } else {
+      sumv2 = sumv2 + S->vnu[i] * S->sp[i];
+      sums2 = sums2 + S->sp[i];
+      sumal2 = sumal2 + S->sp[i] * S->alfa[i];
+      sumad2 = sumad2 + S->sp[i] * S->epp[i];
+      sumc2 = sumc2 + S->spp[i];
+    }
+  label10:
+    if (i < ihi) {
+      if (S->vnu[i + 1] <= vlmt) continue;
+    }
+    vlmt = S->vnu[i] + dv;
+    if (sums > 0.) {
+      j = j + 1;
+      S->sp[j] = sums; S->alfa[j] = sumal / sums; S->epp[j] = sumad / sums;
+      S->vnu[j] = sumv / sums; S->spp[j] = sumc; S->mol[j] = 0;
+      sumal = 0.; sumad = 0.; sums = 0.; sumv = 0.; sumc = 0.;
+    }
+    if (sums2 > 0.) {
+      j = j + 1;
+      S->sp[j] = sums2; S->alfa[j] = sumal2 / sums2; S->epp[j] = sumad2 / sums2;
+      S->vnu[j] = sumv2 / sums2; S->mol[j] = 2; S->spp[j] = sumc2;
+      sumal2 = 0.; sumad2 = 0.; sums2 = 0.; sumv2 = 0.; sumc2 = 0.;
+    }
+  }
+  *ilo_io = j + 1;
+  return j;
+}
+
+/* Block reader over the in-memory TAPE3 (RDLNFL / RDLIN semantics).     */
+typedef struct {
+  const lblx_lines *L;
+  int next; /* next block index to read */
+} t3reader_t;
+
+/* Reads the next block with VMAX >= vlo; returns block index or -1 (EOF). */
+static int t3_next_block(t3reader_t *r, double vlo)
+{
+  while (r->next < r->L->nblk) {
+    int b = r->next++;
+    if (r->L->vmax[b] < vlo) continue;
+    return b;
+  }
+  return -1;
+}
+
+/* One line of a block, converted as RDLIN/RDLNFL do (oprop.f90:776-837). */
+typedef struct {
+  double vnu, sp, alfa, epp, hwhm, tmpalf, pshift;
+  int mol, iflg;
+} line_t;
+
+static void get_line(const lblx_lines *L, int b, int i /*0-based slot*/, line_t *o)
+{
+  size_t k = (size_t)b * LBLX_LINREC + i;
+  o->iflg = L->iflg[k];
+  o->vnu = L->vnu[k];
+  o->sp = L->sp[k];
+  o->alfa = L->alfa[k];
+  o->epp = L->epp[k];
+  o->mol = L->mol[k];
+  o->hwhm = L->hwhm[k];
+  o->tmpalf = L->tmpalf[k];
+  o->pshift = L->pshift[k];
+  int m = o->mol % 100;
+  if (m == 7 && o->iflg >= 0) {
+    double rvmr = 0.21;
+    o->alfa = (o->alfa - rvmr * o->hwhm) / (1.0 - rvmr);
+  }
+  if (m == 22 && o->iflg >= 0) {
+    double rvmr = 0.79;
+    o->alfa = (o->alfa - rvmr * o->hwhm) / (1.0 - rvmr);
+  }
+}
+
+/* LINF4: read, correct and shrink the lines for the far-wing function.  */
+static int linf4(const lblx_lines *L, const lblx_layer *Ly, const molec_t *M,
+                 double v1l4, double v2l4, double dvr4, tape9_t *T9, int64_t *nlin_out)
+{
+  const int LIMIN = 1000;
+  static shrunk_t S;
+  double pavp0 = Ly->pave / P0, pavp2 = pavp0 * pavp0;
+  (void)pavp2;
+  double vlo = v1l4, vhi = v2l4;
+  double tratio = Ly->tave / TEMP0;
+  double rhorat = (Ly->pave / P0) * (TEMP0 / Ly->tave);
+  double beta = RADCN2 / Ly->tave;
+  double beta0 = RADCN2 / TEMP0;
+  double betacr = beta - beta0;
+  t3reader_t rd = {L, 0};
+  int ij = 0, ilo = 1, ihi = 0, ieof = 0;
+  int64_t nlin = 0;
+  T9->n = 0; T9->nblk = 0;
+  for (;;) {
+    /* label 30 */
+    int b = t3_next_block(&rd, vlo);
+    if (b < 0) { ieof = 1; goto label60; }
+    if (L->nrec[b] > NLIMIN) return LBLX_E_NREC;
+    {
+      int nrec = L->nrec[b];
+      for (int i = 0; i < nrec; i++) {
+        line_t ln;
+        get_line(L, b, i, &ln);
+        if (ln.iflg < 0) continue;           /* IOUT holds IFLG >= 0 only */
+        if (ln.iflg != 0) return LBLX_E_UNSUPPORTED; /* line coupling */
+        int m = ln.mol % 100;
+        int iso = (ln.mol % 1000) / 100;
+        if (m > 7 || m < 1) continue;        /* outside the tabulated set */
+        if (iso > LBLX_ISO_MAX[m - 1] || iso < 1) continue;
+        if (m > Ly->nmol) continue;          /* W(M) = 0 -> SUI = 0 */
+        double sui = ln.sp * Ly->wk[m - 1];
+        if (sui == 0.) continue;
+        if (ln.vnu < vlo) continue;
+        ij = ij + 1;
+        S.vnu[ij] = ln.vnu + rhorat * ln.pshift;
+        S.alfa[ij] = ln.alfa;
+        S.epp[ij] = ln.epp;
+        S.mol[ij] = m;
+        if (S.vnu[ij] == 0.) sui = 2. * sui;
+        if (S.epp[ij] <= -1.001) S.epp[ij] = fabs(S.epp[ij]);
+        if (S.epp[ij] > -0.999)
+          sui = sui * M->scor[m - 1][iso - 1] * exp(-S.epp[ij] * betacr) *
+                (1. + exp(-S.vnu[ij] * beta));
+        double tmpcor = pow(tratio, (double)ln.tmpalf);
+        double alfa0i = S.alfa[ij] * tmpcor;
+        double hwhmsi = ln.hwhm * tmpcor;
+        S.alfa[ij] = alfa0i * (rhorat - M->rhoslf[m - 1]) + hwhmsi * M->rhoslf[m - 1];
+        S.epp[ij] = S.vnu[ij] * M->alfd1[m - 1][iso - 1];
+        nlin = nlin + 1;
+        S.sp[ij] = sui * (1. + 0. * pavp2);
+        S.spp[ij] = sui * 0. * pavp0;
+        if (S.vnu[ij] > vhi) { ieof = 1; goto label60; }
+      }
+    }
+    if (ij < LIMIN && ieof == 0) continue;
+  label60:
+    ihi = ij;
+    ihi = shrink(&S, &ilo, ihi, dvr4);
+    ij = ilo - 1;
+    if (ihi < LIMIN && ieof == 0) continue;
+    if (ihi > 0) {
+      t9_reserve(T9, T9->n + ihi);
+      if (T9->nblk == T9->blkcap) {
+        T9->blkcap = T9->blkcap ? 2 * T9->blkcap : 256;
+        T9->bstart = realloc(T9->bstart, T9->blkcap * sizeof(int));
+        T9->bcount = realloc(T9->bcount, T9->blkcap * sizeof(int));
+        T9->bvlo = realloc(T9->bvlo, T9->blkcap * sizeof(double));
+        T9->bvhi = realloc(T9->bvhi, T9->blkcap * sizeof(double));
+      }
+      T9->bstart[T9->nblk] = T9->n;
+      T9->bcount[T9->nblk] = ihi;
+      T9->bvlo[T9->nblk] = S.vnu[1];
+      T9->bvhi[T9->nblk] = S.vnu[ihi];
+      T9->nblk++;
+      for (int k = 1; k <= ihi; k++) {
+        int o = T9->n + k - 1;
+        T9->vnu[o] = S.vnu[k]; T9->sp[o] = S.sp[k]; T9->alfa[o] = S.alfa[k];
+        T9->epp[o] = S.epp[k]; T9->spp[o] = S.spp[k]; T9->mol[o] = S.mol[k];
+      }
+      T9->n += ihi;
+    }
+    if (ieof == 1) break;
+    ij = 0;
+    ilo = 1;
+  }
+  *nlin_out = nlin;
+  return LBLX_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Far-wing window (/LBLF/): R4 on V1R4 + (j-1)*DVR4, j = 1..NPTR4.      */
+typedef struct {
+  double v1r4, v2r4, dvr4, bound4;
+  int nptr4;
+  double r4[2503]; /* 1-based, 2502 used */
+} lblf_t;
+
+/* LBLF4(JRAD, V1, V2) with RDLIN4 + CONVF4 over the in-memory TAPE9.    */
+static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
+                  double v1, double v2, double dptmin, double dptfac,
+                  int64_t *ilin4, int64_t *k3upd)
+{
+  const int LIMOUT = 2500;
+  double dptmn = dptmin / radfn(v2, Ly->tave / RADCN2);
+  double dptfc = dptfac;
+  W->v1r4 = v1;
+  W->v2r4 = v2;
+  W->nptr4 = fint((W->v2r4 - W->v1r4) / W->dvr4 + ONEPL);
+  if (W->nptr4 > LIMOUT) W->nptr4 = LIMOUT;
+  W->v2r4 = W->v1r4 + W->dvr4 * (double)(W->nptr4 - 1);
+  for (int i = 1; i <= LIMOUT + 2; i++) W->r4[i] = 0.;
+  double beta = RADCN2 / Ly->tave;
+  double vlo = W->v1r4 - W->bound4;
+  /* CONVF4 constants (oprop.f90:3860-3877) */
+  const double ZBND = 64.;
+  double recpi = 1. / (2. * asin(1.));
+  double zsqbnd = ZBND * ZBND;
+  double A3 = (1. + 2. * zsqbnd) / ((1. + zsqbnd) * (1. + zsqbnd));
+  double B3 = -1. / ((1. + zsqbnd) * (1. + zsqbnd));
+  double bndsq = W->bound4 * W->bound4;
+  double r_bndsq = 1. / bndsq;
+  double vnulst = W->v2r4 + W->bound4;
+  double *r4 = W->r4;
+  int done = 0;
+  for (int b = 0; b < T9->nblk && !done; b++) {
+    if (T9->bvhi[b] < vlo) continue;            /* RDLIN4 skip */
+    /* RDLIN4 flags the block whose last line passes VHI (IEOF=1) but LBLF4
+       keeps reading after it: only VNULST (label 70) or EOF end the loop. */
+    for (int k = 0; k < T9->bcount[b]; k++) {
+      int i = T9->bstart[b] + k;
+      double sp = T9->sp[i], spp = T9->spp[i];
+      if (sp == 0. && spp == 0.) continue;
+      double alfadi = T9->epp[i];
+      double alfali = T9->alfa[i];
+      double zetai = alfali / (alfali + alfadi);
+      int iz = fint(100. * zetai + ONEPL);
+      double zetdif = 100. * zetai - (double)(iz - 1);
+      double alfavi = (LBLX_AVRAT[iz - 1] + zetdif * (LBLX_AVRAT[iz] - LBLX_AVRAT[iz - 1])) *
+                      (alfali + alfadi);
+      double ralfvi = 1. / alfavi;
+      double sil = sp * recpi * alfali;
+      double siv = (alfali * ralfvi) * sp * recpi * ralfvi;
+      double speak = A3 * fabs(siv);
+      int jj = fint((T9->vnu[i] - W->v1r4) / W->dvr4 + 1.);
+      if (jj < 1) jj = 1;
+      if (jj > W->nptr4) jj = W->nptr4;
+      if (speak <= (dptmn + dptfc * r4[jj]) && spp == 0.) continue;
+      *ilin4 += 1;
+      double vnui = T9->vnu[i];
+      double xnui = vnui - W->v1r4;
+      int jmin = fint((xnui - W->bound4) / W->dvr4 + 2.);
+      if (vnui >= vnulst) { done = 1; break; }
+      if (jmin > W->nptr4) continue;
+      if (jmin < 1) jmin = 1;
+      int jmax = fint((xnui + W->bound4) / W->dvr4 + 1.);
+      if (jmax < jmin) continue; /* label 50: VVW mirror only for 0<vnu<=25 */
+      if (jmax > W->nptr4) jmax = W->nptr4;
+      double alfli2 = alfali * alfali;
+      double alfvi2 = alfavi * alfavi;
+      double xjj = (double)(jmin - 1) * W->dvr4;
+      double f4bnd = sil / (alfli2 + bndsq);
+      double rec_alfvi2 = 1. / alfvi2;
+      double siv_a3 = siv * A3;
+      double siv_b3 = siv * B3;
+      *k3upd += jmax - jmin + 1;
+      if (T9->mol[i] == 2) {
+        for (int j = jmin; j <= jmax; j++) {
+          double xm = (xjj - xnui);
+          double xmsq = xm * xm;
+          double zvsq = xmsq * rec_alfvi2;
+          double fcnt_fn = (2. - (xmsq * r_bndsq)) * f4bnd;
+          double f4fn;
+          if (zvsq <= zsqbnd) f4fn = (siv_a3 + zvsq * siv_b3) - fcnt_fn;
+          else f4fn = sil / (alfli2 + xmsq) - fcnt_fn;
+          r4[j] = r4[j] + f4fn * 1.0; /* CHI == 1 (oprop.f90:4075) */
+          xjj = xjj + W->dvr4;
+        }
+      } else {
+        for (int j = jmin; j <= jmax; j++) {
+          double xm = (xjj - xnui);
+          double xmsq = xm * xm;
+          double zvsq = xmsq * rec_alfvi2;
+          double f4fn;
+          if (zvsq <= zsqbnd) f4fn = (siv_a3 + zvsq * siv_b3) - f4bnd;
+          else f4fn = sil / (alfli2 + xmsq) - f4bnd;
+          r4[j] = r4[j] + f4fn;
+          xjj = xjj + W->dvr4;
+        }
+      }
+    }
+  }
+  if (jrad == 1) {
+    double xkt = 1. / beta;
+    double vitst = W->v1r4 - W->dvr4;
+    double rdlast = -1.;
+    int n1 = 0, n2 = 0;
+    do {
+      n1 = n2 + 1;
+      double vi = W->v1r4 + W->dvr4 * (double)(n1 - 1);
+      double rdel;
+      double radvi = radfni(vi, W->dvr4, xkt, &vitst, &rdel, &rdlast);
+      n2 = fint((vitst - W->v1r4) / W->dvr4 + 0.001);
+      if (n2 > W->nptr4) n2 = W->nptr4;
+      for (int i = n1; i <= n2; i++) {
+        r4[i] = r4[i] * radvi;
+        radvi = radvi + rdel;
+      }
+    } while (n2 < W->nptr4);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+static void xint(double v1a, double v2a, double dva, const double *A /*1-based*/, double afact,
+                 double vft, double dvr3, double *R3 /*1-based*/, int n1r3, int n2r3)
+{
+  double recdva = 1. / dva;
+  int ilo = fint((v1a + dva - vft) / dvr3 + 1. + ONEMI);
+  if (ilo < n1r3) ilo = n1r3;
+  int ihi = fint((v2a - dva - vft) / dvr3 + ONEMI);
+  if (ihi > n2r3) ihi = n2r3;
+  for (int i = ilo; i <= ihi; i++) {
+    double vi = vft + dvr3 * (double)(i - 1);
+    int j = fint((vi - v1a) * recdva + ONEPL);
+    double vj = v1a + dva * (double)(j - 1);
+    double p = recdva * (vi - vj);
+    double c = (3. - 2. * p) * p * p;
+    double b = 0.5 * p * (1. - p);
+    double b1 = b * (1. - p);
+    double b2 = b * p;
+    double conti = -A[j - 1] * b1 + A[j] * (1. - c + b2) + A[j + 1] * (c + b1) - A[j + 2] * b2;
+    R3[i] = R3[i] + conti * afact;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* OPDPTH + HIRAC1 for one layer.                                       */
+typedef struct {
+  /* near-line block buffer (RDLIN) */
+  double vnu[251], sp[251], alfa0[251], epp[251], hwhms[251], tmpalf[251], pshift[251];
+  double sppsp[251], recalf[251], zetai[251];
+  int mol[251], iflg[251], izeta[251], iout[251];
+} blk_t;
+
+int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
+                    const double *absrb, double v1abs, double v2abs, double dvabs, int nptabs,
+                    double *od, int64_t n_out, int64_t *npts_out, double *v1p_out,
+                    lblx_stats *st)
+{
+  lblx_stats stats;
+  memset(&stats, 0, sizeof(stats));
+  const double V1 = Ly->v1, V2 = Ly->v2;
+  double DV = Ly->dv;
+  const double SAMPLE = run->sample, ALFAL0 = run->alfal0;
+  const double HWF3 = 64., DXF1 = 0.002, DXF2 = 0.008, DXF3 = 0.032, HWF1 = 4.;
+  const int NX1 = NFPTS, NX2 = NFPTS, NX3 = NFPTS;
+  int rc = LBLX_OK;
+
+  /* OPDPTH: radiation flag and far-wing grid (lblrtm.f90:6202-6256) */
+  double xkt_l = Ly->tave / RADCN2;
+  if ((V1 / xkt_l) < 5. && Ly->jrad != -1) Ly->jrad = 0;
+  int JRAD = Ly->jrad;
+  int icntnm = run->icntnm;
+  double DVR4 = 0., BOUND4 = 25.;
+  int ilblf4 = run->ilblf4;
+  int ipflag = (icntnm != 0 && Ly->pave <= 0.5) ? 1 : 0;
+  tape9_t T9;
+  memset(&T9, 0, sizeof(T9));
+  molec_t M;
+  molec_layer(Ly, &M);
+  lblf_t *W = calloc(1, sizeof(lblf_t));
+  double V1R4ST = 0., V2R4ST = 0.;
+  if (ilblf4 >= 1) {
+    double alfav = SAMPLE * DV;
+    double alfav4 = 64. * alfav;
+    DVR4 = alfav4 / SAMPLE;
+    BOUND4 = 25.;
+    if (ilblf4 == 2 && ipflag == 1) BOUND4 = 5.;
+    V1R4ST = V1 - 2. * DVR4;
+    V2R4ST = V2 + 2. * DVR4;
+    double v1l4 = V1R4ST - BOUND4 - DVR4;
+    double v2l4 = V2R4ST + BOUND4 + 2 * DVR4;
+    rc = linf4(L, Ly, &M, v1l4, v2l4, DVR4, &T9, &stats.nlin);
+    if (rc) goto done;
+    stats.nlins4 = T9.n;
+  }
+  W->dvr4 = DVR4;
+  W->bound4 = BOUND4;
+
+  /* HIRAC1 set-up (oprop.f90:223-315) */
+  double DPTMN = run->dptmin;
+  if (JRAD != 1) DPTMN = run->dptmin / radfn(V2, Ly->tave / RADCN2);
+  double DPTFC = run->dptfac;
+  const int NSHIFT = 32;
+  double ALFMAX = 4 * SAMPLE * DV * 0.04 / ALFAL0;
+  int nALFMAX = fint(ALFMAX / DV);
+  int NBOUND = fint((2. * HWF3) * nALFMAX + 0.01);
+  const int NLIM1 = 2401, NLIM2 = (NLIM1 / 4) + 1, NLIM3 = (NLIM2 / 4) + 1;
+  static double F1[NFMX + 1], F2[NFMX + 1], F3[NFMX + 1], FG[NFMX + 1], XVER[NFMX + 1];
+  static int shapes_done = 0;
+  if (!shapes_done) { shape_tables(F1, F2, F3, FG, XVER); shapes_done = 1; }
+  double DVP = DV;
+  double DVR2 = (DXF2 / DXF1) * DV;
+  double DVR3 = (DXF3 / DXF1) * DV;
+  int MAX1 = NSHIFT + NLIM1 + NSHIFT + NBOUND / 2;
+  int MAX2 = MAX1 / 4;
+  int MAX3 = MAX1 / 16;
+  MAX1 = MAX1 + 4 * nALFMAX;
+  MAX2 = MAX2 + 16 * nALFMAX / 4 + 1;
+  MAX3 = MAX3 + 64 * nALFMAX / 16 + 1;
+  double BOUND = (double)NBOUND * DV / 2.;
+  int NLO = NSHIFT + 1;
+  int NHI = NLIM1 + NSHIFT - 1;
+  /* R arrays with room below index 1 (RR1/RR2/RR3 of HIRAC1) */
+  int off1 = NBOUND + 8 * nALFMAX + 64, off2 = off1 / 2 + 64, off3 = off1 / 4 + 64;
+  double *RR1 = calloc(MAX1 + off1 + 64, sizeof(double));
+  double *RR2 = calloc(MAX2 + off2 + 64, sizeof(double));
+  double *RR3 = calloc(MAX3 + off3 + 64, sizeof(double));
+  double *R1 = RR1 + off1 - 1, *R2 = RR2 + off2 - 1, *R3 = RR3 + off3 - 1;
+
+  double VFT = V1 - (double)NSHIFT * DV;
+  double VBOT = V1 - BOUND;
+  double VTOP = V2 + BOUND;
+
+  if (ilblf4 >= 1)
+    lblf4(W, &T9, Ly, JRAD, V1R4ST, V2R4ST, run->dptmin, run->dptfac, &stats.ilin4, &stats.k3_updates);
+
+  /* LNCOR1 first-call constants (oprop.f90:970-993) */
+  double XKT0 = TEMP0 / RADCN2;
+  double XKT = Ly->tave / RADCN2;
+  double BETACR = (1. / XKT) - (1. / XKT0);
+  double RHORAT = (Ly->pave / P0) * (TEMP0 / Ly->tave);
+  double TRATIO = Ly->tave / TEMP0;
+  double PAVP0 = Ly->pave / P0;
+  double PAVP2 = PAVP0 * PAVP0;
+
+  blk_t *B = calloc(1, sizeof(blk_t));
+  double *absrb1 = NULL; /* ABSRB as a 1-based array with zero padding */
+  if (icntnm >= 1 && absrb) {
+    absrb1 = calloc((size_t)nptabs + 8, sizeof(double));
+    for (int k = 0; k < nptabs; k++) absrb1[k + 1] = absrb[k];
+  }
+  t3reader_t rd = {L, 0};
+  int IEOF = 0, ILO = 0, IHI = -999, IDATA = 0, IPANEL = 0, ISTOP = 0;
+  int IFPAN = 1;
+  int N1R2, N1R3, N2R2, N2R3;
+  int64_t nout = 0;
+  int first_panel_out = 1;
+  /* CNVFNV constants */
+  double WAVDXF = DV / DXF1;
+  double HWDXF = HWF1 / DXF1;
+  double CONF2 = DV / DVR2;
+  double CONF3 = DV / DVR3;
+  (void)NX2; (void)NX3; (void)N2R2;
+
+  for (;;) {
+    /* label 60: read a block of lines */
+    if (IEOF != 0) goto label80;
+    {
+      /* RDLIN (oprop.f90:734-885) */
+      IDATA = 0;
+      int b = t3_next_block(&rd, VBOT);
+      if (b < 0) { IEOF = 1; goto label80; }
+      int nrec = L->nrec[b];
+      if (nrec > NLIMIN) { rc = LBLX_E_NREC; goto done_hirac; }
+      for (int i = 1; i <= nrec; i++) {
+        line_t ln;
+        get_line(L, b, i - 1, &ln);
+        B->iflg[i] = ln.iflg; B->vnu[i] = ln.vnu; B->sp[i] = ln.sp;
+        B->alfa0[i] = ln.alfa; B->epp[i] = ln.epp; B->mol[i] = ln.mol;
+        B->hwhms[i] = ln.hwhm; B->tmpalf[i] = ln.tmpalf; B->pshift[i] = ln.pshift;
+        if (ln.iflg > 0) { rc = LBLX_E_UNSUPPORTED; goto done_hirac; }
+      }
+      ILO = 1;
+      int IJ = 0;
+      for (int i = 1; i <= nrec; i++)
+        if (B->iflg[i] >= 0) { IJ = IJ + 1; B->iout[IJ] = i; }
+      for (int i = IJ + 1; i <= 250; i++) B->iout[i] = nrec;
+      /* VMIN < VBOT branch: first record at or above VBOT */
+      for (int j = 1; j <= IJ; j++) {
+        ILO = j;
+        if (B->vnu[B->iout[j]] >= VBOT) break;
+      }
+      for (int j = ILO; j <= IJ; j++) {
+        int i = B->iout[j];
+        if (B->mol[i] > 0) {
+          IHI = j;
+          if (B->vnu[i] > VTOP) break;
+        }
+      }
+      if (IHI < IJ) IDATA = 1;
+    }
+    /* LNCOR1 (oprop.f90:995-1193) */
+    for (int j = ILO; j <= IHI; j++) {
+      int i = B->iout[j];
+      int m = B->mol[i] % 100;
+      int iso = (B->mol[i] % 1000) / 100;
+      double SUI;
+      if (m > 7 || m < 1 || iso > LBLX_ISO_MAX[m - 1] || iso < 1) goto rej25;
+      B->mol[i] = m;
+      SUI = (m <= Ly->nmol) ? B->sp[i] * Ly->wk[m - 1] : 0.;
+      if (JRAD == 1) SUI = SUI * radfn(B->vnu[i], XKT);
+      if (SUI == 0.) goto rej25;
+      stats.nlin++;
+      {
+        B->vnu[i] = B->vnu[i] + RHORAT * B->pshift[i];
+        double TMPCOR = pow(TRATIO, B->tmpalf[i]);
+        double ALFA0I = B->alfa0[i] * TMPCOR;
+        double HWHMSI = B->hwhms[i] * TMPCOR;
+        double ALFL = ALFA0I * (RHORAT - M.rhoslf[m - 1]) + HWHMSI * M.rhoslf[m - 1];
+        double ALFAD = B->vnu[i] * M.alfd1[m - 1][iso - 1];
+        double ZETA = ALFL / (ALFL + ALFAD);
+        B->zetai[i] = ZETA;
+        double FZETA = 100. * ZETA;
+        int IZ = fint(FZETA + ONEPL);
+        B->izeta[i] = IZ;
+        double ZETDIF = FZETA - (double)(IZ - 1);
+        double ALFV = (LBLX_AVRAT[IZ - 1] + ZETDIF * (LBLX_AVRAT[IZ] - LBLX_AVRAT[IZ - 1])) * (ALFL + ALFAD);
+        if (ALFV < DV) ALFV = DV;
+        if (ALFV > ALFMAX) ALFV = ALFMAX;
+        if (HWF3 * ALFV + B->vnu[i] < VFT) goto rej25;
+        B->recalf[i] = 1. / ALFV;
+        if (B->epp[i] <= -1.001) B->epp[i] = fabs(B->epp[i]);
+        if (B->epp[i] > -0.999)
+          SUI = SUI * M.scor[m - 1][iso - 1] * exp(-B->epp[i] * BETACR) * (1. + exp(-B->vnu[i] / XKT));
+        B->sp[i] = SUI * (1. + 0. * PAVP2);
+        double SPPI = SUI * 0. * PAVP0;
+        B->sppsp[i] = SPPI / B->sp[i];
+        double SPEAK = SUI * B->recalf[i];
+        if (DVR4 <= 0.) {
+          if (SPEAK <= DPTMN) goto rej25;
+        } else {
+          int JJ = fint((B->vnu[i] - W->v1r4) / DVR4 + 1.);
+          if (JJ < 1) JJ = 1;
+          if (JJ > W->nptr4) JJ = W->nptr4;
+          if (SPEAK <= (DPTMN + DPTFC * W->r4[JJ]) && B->sppsp[i] == 0.) goto rej25;
+        }
+        stats.lincnt++;
+        continue;
+      }
+    rej25:
+      B->sp[i] = 0.;
+      B->sppsp[i] = 0.;
+    }
+  label70:
+    /* CNVFNV (oprop.f90:1255-1368) */
+    {
+      int ILAST = ILO - 1;
+      if (ILO <= IHI) {
+        for (int j = ILO; j <= IHI; j++) {
+          int i = B->iout[j];
+          if (B->sp[i] == 0.) continue;
+          double DEPTHI = B->sp[i] * B->recalf[i];
+          int IZM = B->izeta[i];
+          double ZETDIF = 100. * B->zetai[i] - (double)(IZM - 1);
+#define ZI(T) (T[IZM - 1] + ZETDIF * (T[IZM] - T[IZM - 1]))
+          double STRF1 = DEPTHI * ZI(LBLX_CF1);
+          double STRF2 = DEPTHI * ZI(LBLX_CF2);
+          double STRF3 = DEPTHI * ZI(LBLX_CF3);
+          double STRD = DEPTHI * ZI(LBLX_CGAUSS);
+          double STRVER = DEPTHI * ZI(LBLX_CER);
+#undef ZI
+          double ZSLOPE = B->recalf[i] * WAVDXF;
+          double ZINT = (B->vnu[i] - VFT) / DV;
+          double BHWDXF = HWDXF / ZSLOPE;
+          int JMAX1 = fint(ZINT + BHWDXF + 1.5);
+          if (JMAX1 > MAX1) {
+            ILAST = j - 1;
+            IPANEL = 1;
+            goto label40;
+          }
+          int JMIN1 = fint(ZINT - BHWDXF + 1.5);
+          double RSHFT = 0.5;
+          if (ZINT < 0.0) RSHFT = -RSHFT;
+          int J2SHFT = fint(ZINT * (1. - CONF2) + RSHFT);
+          int J3SHFT = fint(ZINT * (1. - CONF3) + RSHFT);
+          int JMIN2 = JMIN1 - J2SHFT;
+          int JMIN3 = JMIN1 - J3SHFT;
+          double ZF1 = ((double)(JMIN1 - 2) - ZINT) * ZSLOPE;
+          double ZF2 = ((double)(JMIN2 - 2) - ZINT * CONF2) * ZSLOPE;
+          double ZF3 = ((double)(JMIN3 - 2) - ZINT * CONF3) * ZSLOPE;
+          stats.k1_updates += (int64_t)(JMAX1 - JMIN1 + 1);
+          for (int J1 = JMIN1; J1 <= JMAX1; J1++) {
+            int J2 = J1 - J2SHFT;
+            int J3 = J1 - J3SHFT;
+            ZF3 = ZF3 + ZSLOPE;
+            ZF2 = ZF2 + ZSLOPE;
+            ZF1 = ZF1 + ZSLOPE;
+            int IZ3 = fint(fabs(ZF3) + 1.5);
+            int IZ2 = fint(fabs(ZF2) + 1.5);
+            int IZ1 = fint(fabs(ZF1) + 1.5);
+            R3[J3] = R3[J3] + STRF3 * F3[IZ3];
+            R2[J2] = R2[J2] + STRF2 * F2[IZ2];
+            R1[J1] = R1[J1] + STRF1 * F1[IZ1] + STRD * FG[IZ1] + STRVER * XVER[IZ1];
+          }
+        }
+        ILAST = IHI;
+        IPANEL = IDATA;
+      } else {
+        IPANEL = 1;
+      }
+    label40:
+      ILO = ILAST + 1;
+    }
+    if (IPANEL == 0) continue; /* GO TO 60 */
+  label80:
+    {
+      int N1R1, N2R1;
+      (void)N1R1; (void)N2R1;
+      if (IFPAN == 1) {
+        IFPAN = 0;
+        N1R1 = 1; N1R2 = 1; N1R3 = 1;
+      } else {
+        N1R1 = NSHIFT + 1;
+        N1R2 = (NSHIFT / 4) + 1 + 4;
+        N1R3 = (NSHIFT / 16) + 1 + 3;
+      }
+      N2R1 = NLIM1 + NSHIFT - 1;
+      N2R2 = NLIM2 + (NSHIFT / 4) - 1 + 4;
+      N2R3 = NLIM3 + (NSHIFT / 16) - 1 + 3;
+      if (VFT <= 0.) { rc = LBLX_E_UNSUPPORTED; goto done_hirac; } /* RSYM path */
+      if (ilblf4 >= 1)
+        xint(W->v1r4, W->v2r4, DVR4, W->r4, 1.0, VFT, DVR3, R3, N1R3, N2R3);
+      if (icntnm >= 1 && absrb1)
+        xint(v1abs, v2abs, dvabs, absrb1, 1., VFT, DVR3, R3, N1R3, N2R3);
+      (void)v2abs; (void)nptabs;
+
+      /* PANEL (oprop.f90:1418-1558) */
+      const double X00 = -7. / 128., X01 = 105. / 128., X02 = 35. / 128., X03 = -5. / 128.;
+      const double X10 = -1. / 16., X11 = 9. / 16.;
+      double V1P, V2P;
+      ISTOP = 0;
+      if ((VFT + (NHI - 1) * DVP) > V2) {
+        NHI = fint((V2 - VFT) / DVP + 1.);
+        V2P = VFT + (double)(NHI - 1) * DVP;
+        if (V2P < V2) { V2P = V2P + DVP; NHI = NHI + 1; }
+        ISTOP = 1;
+      } else {
+        V2P = VFT + (double)(NHI - 1) * DV;
+      }
+      (void)V2P;
+      int NLIM = NHI - NLO + 1;
+      V1P = VFT + (double)(NLO - 1) * DV;
+      int LIMLO = N1R2;
+      if (N1R2 == 1) LIMLO = LIMLO + 4;
+      int LIMHI = (NHI / 4) + 1;
+      int J = LIMLO, J3 = 0;
+      for (J = LIMLO; J <= LIMHI; J += 4) {
+        J3 = (J - 1) / 4 + 1;
+        R2[J] = R2[J] + R3[J3];
+        R2[J + 1] = R2[J + 1] + X00 * R3[J3 - 1] + X01 * R3[J3] + X02 * R3[J3 + 1] + X03 * R3[J3 + 2];
+        R2[J + 2] = R2[J + 2] + X10 * (R3[J3 - 1] + R3[J3 + 2]) + X11 * (R3[J3] + R3[J3 + 1]);
+        R2[J + 3] = R2[J + 3] + X03 * R3[J3 - 1] + X02 * R3[J3] + X01 * R3[J3 + 1] + X00 * R3[J3 + 2];
+      }
+      if (LIMLO <= LIMHI && ISTOP == 1) {
+        J3 = J3 + 1;
+        R2[J] = R2[J] + R3[J3];
+        R2[J + 1] = R2[J + 1] + X00 * R3[J3 - 1] + X01 * R3[J3] + X02 * R3[J3 + 1] + X03 * R3[J3 + 2];
+      }
+      for (J = NLO; J <= NHI; J += 4) {
+        int J2 = (J - 1) / 4 + 1;
+        R1[J] = R1[J] + R2[J2];
+        R1[J + 1] = R1[J + 1] + X00 * R2[J2 - 1] + X01 * R2[J2] + X02 * R2[J2 + 1] + X03 * R2[J2 + 2];
+        R1[J + 2] = R1[J + 2] + X10 * (R2[J2 - 1] + R2[J2 + 2]) + X11 * (R2[J2] + R2[J2 + 1]);
+        R1[J + 3] = R1[J + 3] + X03 * R2[J2 - 1] + X02 * R2[J2] + X01 * R2[J2 + 1] + X00 * R2[J2 + 2];
+      }
+      if (JRAD == 0) {
+        double xkt = Ly->tave / RADCN2;
+        double VI = V1P - DV;
+        double VITST = VI;
+        double RDLAST = -1.;
+        int NPTSI1 = NLO - 1, NPTSI2 = NLO - 1;
+        do {
+          NPTSI1 = NPTSI2 + 1;
+          VI = VFT + (double)(NPTSI1 - 1) * DV;
+          double RDEL;
+          double RADVI = radfni(VI, DV, xkt, &VITST, &RDEL, &RDLAST);
+          NPTSI2 = fint((VITST - VFT) / DV + 0.001);
+          if (NPTSI2 > NHI) NPTSI2 = NHI;
+          for (int I = NPTSI1; I <= NPTSI2; I++) {
+            R1[I] = R1[I] * RADVI;
+            RADVI = RADVI + RDEL;
+          }
+        } while (NPTSI2 < NHI);
+      }
+      /* emit panel R1(NLO..NHI) */
+      if (first_panel_out) { if (v1p_out) *v1p_out = V1P; first_panel_out = 0; }
+      for (int k = 0; k < NLIM; k++) {
+        if (od && nout + k < n_out) od[nout + k] = R1[NLO + k];
+      }
+      nout += NLIM;
+      stats.npanel++;
+      VFT = VFT + (double)(NLIM1 - 1) * DV;
+      if (ISTOP != 1) {
+        for (J = NLIM1; J <= MAX1; J++) R1[J - NLIM1 + 1] = R1[J];
+        for (J = MAX1 - NLIM1 + 2; J <= MAX1; J++) R1[J] = 0.;
+        for (J = NLIM2; J <= MAX2; J++) R2[J - NLIM2 + 1] = R2[J];
+        for (J = MAX2 - NLIM2 + 2; J <= MAX2; J++) R2[J] = 0.;
+        for (J = NLIM3; J <= MAX3; J++) R3[J - NLIM3 + 1] = R3[J];
+        for (J = MAX3 - NLIM3 + 2; J <= MAX3; J++) R3[J] = 0.;
+        NLO = NSHIFT + 1;
+      }
+      if (ISTOP != 1) {
+        if (ilblf4 >= 1) {
+          double VF1 = VFT - 2. * DVR4;
+          VF1 = V1 + floor((VF1 - V1) / DVR4) * DVR4;
+          double VF2 = VFT + 2. * DVR4 + (double)(N2R3 + 4) * DVR3;
+          if (VF2 > W->v2r4 && W->v2r4 != V2R4ST)
+            lblf4(W, &T9, Ly, JRAD, VF1, V2R4ST, run->dptmin, run->dptfac, &stats.ilin4, &stats.k3_updates);
+        }
+        goto label70;
+      }
+      break;
+    }
+  }
+done_hirac:
+  free(RR1); free(RR2); free(RR3); free(B); free(absrb1);
+  stats.npts = nout;
+  if (npts_out) *npts_out = nout;
+  if (rc == LBLX_OK && od && nout > n_out) rc = LBLX_E_SIZE;
+done:
+  t9_free(&T9);
+  free(W);
+  if (st) *st = stats;
+  (void)NX1;
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* PATH layer DV selection (lblrtm.f90:5336-5500), vertical path.       */
+int oracle_plan_dv(const lblx_run *run, int nlay, int nmol, const double *pave,
+                   const double *tave, const double *wk, const double *wbroad,
+                   int iemit, double *dv_out, int32_t *itype_out)
+{
+  double DV = 0.;
+  double SAMPLE = run->sample, ALFAL0 = run->alfal0, AVMASS = run->avmass;
+  double V1 = run->v1, V2 = run->v2;
+  for (int l = 0; l < nlay; l++) {
+    double sumwk = 0.;
+    for (int m = 0; m < nmol; m++) sumwk = sumwk + wk[(size_t)l * nmol + m];
+    double wtotl = sumwk + wbroad[l];
+    double frh2o = wk[(size_t)l * nmol + 0] / wtotl;
+    double alfcor = (pave[l] / P0) * sqrt(TEMP0 / tave[l]);
+    double h2oslf = (1. - frh2o + 5. * frh2o);
+    double albar = ALFAL0 * alfcor * h2oslf;
+    double adbar = 3.58115E-07 * (0.5 * (V1 + V2)) * sqrt(tave[l] / AVMASS);
+    double avbar = 0.5 * (albar + sqrt(albar * albar + 4. * adbar * adbar));
+    double olddv = DV;
+    DV = avbar / SAMPLE;
+    int itype = 99;
+    if (l == 0) {
+      int iscal = fint(log10(DV) - 3.);
+      double scal = pow(10., (double)iscal);
+      int idv = fint((DV / scal) + 0.5);
+      if (idv % 2 > 0) idv = idv + 1;
+      DV = scal * (double)idv;
+    } else {
+      double type = olddv / DV;
+      const double TYPMAX = 2.5;
+      if (type > TYPMAX) {
+        return LBLX_E_DVRATIO;
+      } else if (type >= 1.2) {
+        DV = olddv;
+        itype = fint(1. / (type - 1.) + 0.5);
+        if (itype == 3) itype = 2;
+        DV = olddv * (double)itype / (double)(itype + 1);
+      } else if (type >= 0.8) {
+        DV = olddv;
+        itype = 0;
+      } else {
+        DV = olddv;
+        itype = 0;
+        if (iemit == 0) {
+          itype = fint(type / (1. - type) + 0.5);
+          DV = DV * (double)(itype + 1) / (double)itype;
+          itype = -itype;
+        }
+      }
+    }
+    dv_out[l] = DV;
+    if (itype_out) itype_out[l] = itype;
+  }
+  return LBLX_OK;
+}
+
+/* Table export for the checks in tests/test_oracle.py.                   */
+void oracle_shape_tables(double *f1, double *f2, double *f3, double *fg, double *xver)
+{
+  static double t[5][NFMX + 1];
+  shape_tables(t[0], t[1], t[2], t[3], t[4]);
+  for (int i = 0; i < NFMX; i++) {
+    f1[i] = t[0][i + 1]; f2[i] = t[1][i + 1]; f3[i] = t[2][i + 1]; fg[i] = t[3][i + 1]; xver[i] = t[4][i + 1];
+  }
+}
+
+/* SHRINK / LINF4 export: returns the number of shrunk lines written.     */
+int oracle_linf4(const lblx_run *run, const lblx_layer *Ly, const lblx_lines *L,
+                 double *vnu, double *sp, double *alfa, double *epp, int32_t *mol,
+                 int32_t *blk_count, int cap, int blkcap, int *nblk_out)
+{
+  (void)run;
+  molec_t M;
+  molec_layer(Ly, &M);
+  double DV = Ly->dv;
+  double alfav = run->sample * DV;
+  double DVR4 = 64. * alfav / run->sample;
+  double BOUND4 = 25.;
+  double V1R4ST = Ly->v1 - 2. * DVR4, V2R4ST = Ly->v2 + 2. * DVR4;
+  tape9_t T9;
+  memset(&T9, 0, sizeof(T9));
+  int64_t nlin = 0;
+  int rc = linf4(L, Ly, &M, V1R4ST - BOUND4 - DVR4, V2R4ST + BOUND4 + 2 * DVR4, DVR4, &T9, &nlin);
+  if (rc) { t9_free(&T9); return rc; }
+  int n = T9.n < cap ? T9.n : cap;
+  for (int i = 0; i < n; i++) {
+    vnu[i] = T9.vnu[i]; sp[i] = T9.sp[i]; alfa[i] = T9.alfa[i]; epp[i] = T9.epp[i]; mol[i] = T9.mol[i];
+  }
+  int nb = T9.nblk < blkcap ? T9.nblk : blkcap;
+  for (int b = 0; b < nb; b++) blk_count[b] = T9.bcount[b];
+  *nblk_out = T9.nblk;
+  int total = T9.n;
+  t9_free(&T9);
+  return total;
+}

@@ -1,0 +1,89 @@
+"""ctypes loader for the CPU oracle (oracle/liblblx_oracle.so) -- tests only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from lblrtm_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(ROOT, "oracle", "liblblx_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run make -C oracle")
+        L = C.CDLL(path)
+        L.oracle_od_layer.restype = C.c_int
+        L.oracle_od_layer.argtypes = [
+            C.POINTER(abi.Run), C.POINTER(abi.Layer), C.POINTER(abi.Lines),
+            C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_int,
+            C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64),
+            C.POINTER(C.c_double), C.POINTER(abi.Stats)]
+        L.oracle_plan_dv.restype = C.c_int
+        L.oracle_plan_dv.argtypes = [
+            C.POINTER(abi.Run), C.c_int, C.c_int, C.POINTER(C.c_double),
+            C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+            C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.oracle_shape_tables.restype = None
+        L.oracle_shape_tables.argtypes = [C.POINTER(C.c_double)] * 5
+        L.tips_scor.restype = None
+        L.tips_scor.argtypes = [C.c_double, C.POINTER(C.c_double)]
+        L.radfn.restype = C.c_double
+        L.radfn.argtypes = [C.c_double, C.c_double]
+        L.oracle_linf4.restype = C.c_int
+        L.oracle_linf4.argtypes = [
+            C.POINTER(abi.Run), C.POINTER(abi.Layer), C.POINTER(abi.Lines),
+            C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+            C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+            C.c_int, C.c_int, C.POINTER(C.c_int)]
+        _LIB = L
+    return _LIB
+
+
+def plan_dv(run, pave, tave, wk, wbroad, iemit=0):
+    pave = np.ascontiguousarray(pave, np.float64)
+    tave = np.ascontiguousarray(tave, np.float64)
+    wk = np.ascontiguousarray(np.atleast_2d(wk), np.float64)
+    wbroad = np.ascontiguousarray(wbroad, np.float64)
+    nlay, nmol = wk.shape
+    dv = np.zeros(nlay)
+    it = np.zeros(nlay, np.int32)
+    rc = lib().oracle_plan_dv(C.byref(run), nlay, nmol, abi.ptr(pave, C.c_double),
+                              abi.ptr(tave, C.c_double), abi.ptr(wk, C.c_double),
+                              abi.ptr(wbroad, C.c_double), iemit,
+                              abi.ptr(dv, C.c_double), abi.ptr(it, C.c_int32))
+    if rc:
+        raise RuntimeError("oracle_plan_dv rc=%d" % rc)
+    return dv, it
+
+
+def od_layer(run, layer, table, absrb=None, v1abs=0.0, v2abs=0.0, dvabs=0.0):
+    """Run the oracle for one layer; returns (od, v1p, stats, layer.jrad)."""
+    lines = table.struct()
+    n = C.c_int64(0)
+    v1p = C.c_double(0.0)
+    st = abi.Stats()
+    ab = None
+    nab = 0
+    if absrb is not None:
+        absrb = np.ascontiguousarray(absrb, np.float64)
+        ab = abi.ptr(absrb, C.c_double)
+        nab = len(absrb)
+    # size query, then the real run
+    rc = lib().oracle_od_layer(C.byref(run), C.byref(layer), C.byref(lines), ab, v1abs, v2abs,
+                               dvabs, nab, None, 0, C.byref(n), C.byref(v1p), C.byref(st))
+    if rc not in (0, abi.E_SIZE):
+        raise RuntimeError("oracle_od_layer rc=%d" % rc)
+    jrad_after = layer.jrad
+    out = np.zeros(n.value)
+    lay2 = abi.Layer.from_buffer_copy(layer)
+    rc = lib().oracle_od_layer(C.byref(run), C.byref(lay2), C.byref(lines), ab, v1abs, v2abs,
+                               dvabs, nab, abi.ptr(out, C.c_double), n.value, C.byref(n),
+                               C.byref(v1p), C.byref(st))
+    if rc:
+        raise RuntimeError("oracle_od_layer rc=%d" % rc)
+    return out, v1p.value, st.as_dict(), jrad_after
