@@ -151,10 +151,42 @@ int lblx_od_layer(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layer,
 int lblx_od_layer_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layer,
                          void *d_od, int64_t n_out, double *v1p, lblx_stats *st);
 
+/* OPDPTH for the nlay layers of one column in one batch: every stage is
+ * one launch for all layers.  Layer k's radiation flag is the one layer
+ * k-1 leaves (FSCDID(9) persists across XLAYER's layer loop); layer 0
+ * starts from layers[0].jrad.  Layer k's optical depth goes to
+ * d_od + od_off[k] elements (double, or float for LBLX_FP32_SPLIT).       */
+int lblx_od_column_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers,
+                          int nlay, void *d_od, const int64_t *od_off, double *v1p,
+                          lblx_stats *st);
+
 /* Wait for the context stream; the stream handle (hipStream_t) for callers
  * that order their own device work after the library's.                  */
 int lblx_sync(lblx_ctx *ctx);
 void *lblx_stream(lblx_ctx *ctx);
+
+/* Stage timers: HIP events on the context stream around each stage
+ * (linf4, shrink, convf4, lncor1, reject, scatter_rec, tile_r3, tile_r2,
+ * tile_r1, merge, contnm, scan), the device analogue of the TAPE6 timing
+ * table (oprop.f90:547-620).  lblx_timing(ctx, 1) clears and enables;
+ * lblx_timing_get returns the summed milliseconds and launch count of a
+ * stage index (0 .. LBLX_NSTAGE-1).                                       */
+#define LBLX_NSTAGE 12
+
+/* Diagnostics: the shrunk far-wing line list (TAPE9 content, LINF4 +
+ * SHRINK, oprop.f90:2934-3640) of job `job` of the last OPDPTH call.      */
+int lblx_debug_tape9(lblx_ctx *ctx, int job, double *vnu, double *sp, double *alfa,
+                     double *epp, int32_t *mol, int cap, int32_t *n_out);
+
+/* Device memory of the context's device for callers without their own
+ * allocator; lblx_copy kind: 0 host->device, 1 device->host, 2 device->
+ * device (ordered on the context stream, returns when complete).          */
+void *lblx_dev_alloc(lblx_ctx *ctx, int64_t bytes);
+int lblx_dev_free(lblx_ctx *ctx, void *p);
+int lblx_copy(lblx_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind);
+int lblx_timing(lblx_ctx *ctx, int enable);
+int lblx_timing_get(lblx_ctx *ctx, int stage, const char **name, double *ms,
+                    int64_t *count);
 
 #ifdef __cplusplus
 }

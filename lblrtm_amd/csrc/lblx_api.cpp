@@ -69,9 +69,57 @@ struct DevBuf {
 
 }  // namespace
 
+// Stage timers (the analogue of the TAPE6 TIME/CONVOLUTION/PANEL columns,
+// oprop.f90:547-620): HIP events recorded on the context stream around each
+// stage, summed when collected.
+enum Stage { ST_LINF4, ST_SHRINK, ST_CONVF4, ST_LNCOR1, ST_REJECT, ST_SCATTER, ST_TILE3, ST_TILE2,
+             ST_TILE1, ST_MERGE, ST_CONTNM, ST_SCAN, ST_N };
+static const char *kStageNames[ST_N] = {"linf4", "shrink", "convf4", "lncor1", "reject",
+                                        "scatter_rec", "tile_r3", "tile_r2", "tile_r1",
+                                        "merge", "contnm", "scan"};
+
 struct lblx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool timing = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+  std::vector<hipEvent_t> evpool;
+  double stage_ms[ST_N] = {0};
+  long stage_n[ST_N] = {0};
+  hipEvent_t ev_get() {
+    if (!evpool.empty()) {
+      hipEvent_t e = evpool.back();
+      evpool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  int tbegin(int st) {
+    if (!timing) return -1;
+    hipEvent_t a = ev_get(), b = ev_get();
+    (void)hipEventRecord(a, stream);
+    evs.push_back({st, {a, b}});
+    return (int)evs.size() - 1;
+  }
+  void tend(int h) {
+    if (h < 0) return;
+    (void)hipEventRecord(evs[h].second.second, stream);
+  }
+  void tcollect() {
+    (void)hipStreamSynchronize(stream);
+    for (auto &e : evs) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) {
+        stage_ms[e.first] += ms;
+        stage_n[e.first] += 1;
+      }
+      evpool.push_back(e.second.first);
+      evpool.push_back(e.second.second);
+    }
+    evs.clear();
+  }
   // host copy of the line table (flat, file order)
   std::vector<double> h_vnu, h_bvmax;
   std::vector<int32_t> h_bstart, h_blk;
@@ -82,10 +130,12 @@ struct lblx_ctx {
   DevBuf d_vnu, d_sp, d_alfa, d_epp, d_hwhm, d_tmpalf, d_pshift, d_mol, d_blk, d_bstart, d_bvmax;
   DevBuf d_tabs;  // F1 F2 F3 FG XVER
   // work buffers
-  DevBuf w_f4, w_stop, w_t9vnu, w_t9sp, w_t9alfa, w_t9epp, w_t9mol, w_t9n, w_t9bstart,
-      w_t9bcount, w_t9nblk, w_t9bvhi;
-  DevBuf w_pan, w_win, w_seg, w_r4, w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec;
-  DevBuf w_tlo, w_thi, w_r3, w_r2, w_cnt, w_od, w_absrb;
+  DevBuf w_dlay, w_pan, w_win, w_seg, w_r4, w_margin;
+  DevBuf w_cx, w_csp, w_cal, w_cad, w_cco2, w_pn, w_pc, w_g, w_jmp, w_wsum, w_bcnt, w_etab, w_jobi,
+      w_bat;
+  DevBuf w_t9vnu, w_t9sp, w_t9alfa, w_t9epp, w_t9mol, w_t9n, w_t9blk, w_t9bvhi;
+  DevBuf w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
+  DevBuf w_tmap[3];
   DevLines lines() const {
     DevLines L;
     L.n = nlines;
@@ -164,13 +214,17 @@ void lblx_destroy(lblx_ctx *c) {
   hipSetDevice(c->device);
   DevBuf *bufs[] = {&c->d_vnu, &c->d_sp, &c->d_alfa, &c->d_epp, &c->d_hwhm, &c->d_tmpalf,
                     &c->d_pshift, &c->d_mol, &c->d_blk, &c->d_bstart, &c->d_bvmax, &c->d_tabs,
-                    &c->w_f4, &c->w_stop, &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp,
-                    &c->w_t9mol, &c->w_t9n, &c->w_t9bstart, &c->w_t9bcount, &c->w_t9nblk,
-                    &c->w_t9bvhi, &c->w_pan, &c->w_win, &c->w_seg, &c->w_r4, &c->w_near,
-                    &c->w_keep, &c->w_blkmax, &c->w_pread, &c->w_changed, &c->w_rec, &c->w_tlo,
-                    &c->w_thi, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od, &c->w_absrb};
+                    &c->w_dlay, &c->w_pan, &c->w_win, &c->w_seg, &c->w_r4, &c->w_margin,
+                    &c->w_cx, &c->w_csp, &c->w_cal, &c->w_cad, &c->w_cco2, &c->w_pn, &c->w_pc,
+                    &c->w_g, &c->w_jmp, &c->w_wsum, &c->w_bcnt, &c->w_etab, &c->w_jobi, &c->w_bat,
+                    &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
+                    &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
+                    &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
+                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2]};
   for (DevBuf *b : bufs) b->release();
-  if (c->stream) hipStreamDestroy(c->stream);
+  c->tcollect();
+  for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -253,113 +307,200 @@ int lblx_od_size(const lblx_run *run, const lblx_layer *layer, int64_t *npts) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------
-static int run_layer(lblx_ctx *c, const lblx_run *run, lblx_layer *layer, void *d_od,
-                     int64_t n_out, bool f32out, double *v1p, lblx_stats *st) {
+// Run OPDPTH for a batch of jobs (layers).  Every stage is one launch for
+// all jobs.  od_base + od_off[j] receives job j's npts values.
+static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int njobs, bool chain_jrad,
+                    void *od_base, const int64_t *od_off, bool f32out, double *v1p, lblx_stats *st) {
   if (c->nlines <= 0) return LBLX_E_NOLINES;
-  LayerPlan P;
-  int rc = plan_layer(run, layer, &P);
-  if (rc) return rc;
   if (run->icntnm != 0) return LBLX_E_UNSUPPORTED;  // continuum: separate entry
-  if (n_out < P.npts) return LBLX_E_SIZE;
+  if (njobs <= 0) return LBLX_E_ARG;
   hipStream_t s = c->stream;
-  DevLayer Y{};
-  Y.dv = P.dv;
-  Y.dvr2 = P.dvr2;
-  Y.dvr3 = P.dvr3;
-  Y.dvr4 = P.dvr4;
-  Y.bound4 = P.bound4;
-  Y.alfmax = P.alfmax;
-  Y.vft0 = P.vft0;
-  Y.vbot = P.vbot;
-  Y.vtop = P.vtop;
-  Y.v1l4 = P.v1l4;
-  Y.v2l4 = P.v2l4;
-  Y.rhorat = P.rhorat;
-  Y.tratio = P.tratio;
-  Y.xkt = P.xkt;
-  Y.betacr_l = P.betacr_l;
-  Y.beta4 = P.beta4;
-  Y.betacr4 = P.betacr4;
-  Y.dptmn_hirac = P.dptmn_hirac;
-  Y.dptfc = P.dptfc;
-  Y.dptmn4 = P.dptmn4;
-  std::memcpy(Y.scor, P.scor, sizeof(Y.scor));
-  std::memcpy(Y.alfd1, P.alfd1, sizeof(Y.alfd1));
-  for (int m = 0; m < 8; m++) {
-    Y.rhoslf[m] = P.rhoslf[m];
-    Y.wk[m] = m < layer->nmol ? layer->wk[m] : 0.;
+  const int n = c->nlines, nblk = c->nblk;
+  std::vector<LayerPlan> plans(njobs);
+  std::vector<DevLayer> ys(njobs);
+  std::vector<DevPanel> pan;
+  std::vector<DevWin> win;
+  std::vector<DevSeg> seg;
+  std::vector<double> margins(3 * njobs);
+  int64_t r2tot = 0, r3tot = 0;
+  int maxnn = 0, maxnbk = 0;
+  int prev_jrad = layers[0].jrad;
+  for (int jb = 0; jb < njobs; jb++) {
+    lblx_layer *ly = &layers[jb];
+    if (chain_jrad && jb > 0) ly->jrad = prev_jrad;
+    LayerPlan &P = plans[jb];
+    int rc = plan_layer(run, ly, &P);
+    if (rc) return rc;
+    prev_jrad = ly->jrad;
+    DevLayer &Y = ys[jb];
+    Y = DevLayer{};
+    Y.dv = P.dv;
+    Y.dvr2 = P.dvr2;
+    Y.dvr3 = P.dvr3;
+    Y.dvr4 = P.dvr4;
+    Y.bound4 = P.bound4;
+    Y.alfmax = P.alfmax;
+    Y.vft0 = P.vft0;
+    Y.vbot = P.vbot;
+    Y.vtop = P.vtop;
+    Y.v1l4 = P.v1l4;
+    Y.v2l4 = P.v2l4;
+    Y.rhorat = P.rhorat;
+    Y.tratio = P.tratio;
+    Y.xkt = P.xkt;
+    Y.betacr_l = P.betacr_l;
+    Y.beta4 = P.beta4;
+    Y.betacr4 = P.betacr4;
+    Y.dptmn_hirac = P.dptmn_hirac;
+    Y.dptfc = P.dptfc;
+    Y.dptmn4 = P.dptmn4;
+    std::memcpy(Y.scor, P.scor, sizeof(Y.scor));
+    std::memcpy(Y.alfd1, P.alfd1, sizeof(Y.alfd1));
+    for (int m = 0; m < 8; m++) {
+      Y.rhoslf[m] = P.rhoslf[m];
+      Y.wk[m] = m < ly->nmol ? ly->wk[m] : 0.;
+    }
+    Y.nmol = ly->nmol;
+    Y.jrad = P.jrad;
+    Y.max1 = P.max1;
+    Y.npanel = (int)P.panels.size();
+    Y.nwin = (int)P.wins.size();
+    Y.g_last = P.g_last;
+    Y.g2max = P.g2max;
+    Y.g3max = P.g3max;
+    Y.n1 = P.n1;
+    Y.n2 = P.n2;
+    Y.n3 = P.n3;
+    Y.icntnm = 0;
+    Y.pan0 = (int)pan.size();
+    Y.win0 = (int)win.size();
+    Y.seg0 = (int)seg.size();
+    for (const PanelPlan &q : P.panels)
+      pan.push_back(DevPanel{q.vft, q.w, q.nhi, q.istop, q.seg0, q.nseg, 0, q.out0});
+    for (const WinPlan &w : P.wins)
+      win.push_back(DevWin{w.v1r4, w.v2r4, w.nptr4, Y.seg0 + w.seg0, w.nseg, jb});
+    for (const RadSeg &g : P.segs) seg.push_back(DevSeg{g.i1, g.i2, g.radvi, g.rdel});
+    // far-wing lines: first block with VMAX >= V1L4 (RDLNFL skip)
+    Y.b4 = (int)(std::lower_bound(c->h_bvmax.begin(), c->h_bvmax.end(), P.v1l4) - c->h_bvmax.begin());
+    // near range (RDLIN): first record >= VBOT in the first block read, up
+    // to and including the first record above VTOP
+    int bn = (int)(std::lower_bound(c->h_bvmax.begin(), c->h_bvmax.end(), P.vbot) - c->h_bvmax.begin());
+    int nlo = n, nhi = n - 1;
+    if (bn < nblk) {
+      int s0 = c->h_bstart[bn], e0 = c->h_bstart[bn + 1];
+      nlo = (int)(std::lower_bound(c->h_vnu.begin() + s0, c->h_vnu.begin() + e0, P.vbot) - c->h_vnu.begin());
+      if (nlo >= e0) nlo = e0 - 1;
+      int up = (int)(std::upper_bound(c->h_vnu.begin() + nlo, c->h_vnu.end(), P.vtop) - c->h_vnu.begin());
+      nhi = up < n ? up : n - 1;
+    }
+    Y.nlo = nlo;
+    Y.nn = nhi >= nlo ? nhi - nlo + 1 : 0;
+    Y.blk0 = Y.nn > 0 ? c->h_blk[nlo] : 0;
+    Y.nbk = Y.nn > 0 ? c->h_blk[nhi] - Y.blk0 + 1 : 0;
+    Y.od0 = od_off ? od_off[jb] : 0;
+    Y.r2o = r2tot;
+    Y.r3o = r3tot;
+    r2tot += P.n2;
+    r3tot += P.n3;
+    maxnn = std::max(maxnn, Y.nn);
+    maxnbk = std::max(maxnbk, Y.nbk);
+    double shiftm = c->max_pshift * P.rhorat + 2. * P.dv;
+    const double dvg[3] = {P.dv, P.dvr2, P.dvr3};
+    for (int G = 1; G <= 3; G++)
+      margins[3 * jb + G - 1] = (double)(4 * (P.nalfmax + 1) + 8) * dvg[G - 1] + shiftm;
   }
-  Y.nmol = layer->nmol;
-  Y.jrad = P.jrad;
-  Y.max1 = P.max1;
-  Y.npanel = (int)P.panels.size();
-  Y.nwin = (int)P.wins.size();
-  Y.g_last = P.g_last;
-  Y.g2max = P.g2max;
-  Y.g3max = P.g3max;
-  Y.n1 = P.n1;
-  Y.n2 = P.n2;
-  Y.n3 = P.n3;
-  Y.icntnm = 0;
-
-  // plan tables
-  std::vector<DevPanel> pan(P.panels.size());
-  for (size_t i = 0; i < P.panels.size(); i++) {
-    const PanelPlan &q = P.panels[i];
-    pan[i] = DevPanel{q.vft, q.w, q.nhi, q.istop, q.seg0, q.nseg, 0, q.out0};
+  if (seg.empty()) seg.push_back(DevSeg{0, 0, 0., 0.});
+  const int nwin = (int)win.size();
+  // tile maps
+  const int tiles[3] = {2048, 1024, 512};
+  std::vector<int32_t> tjob[3], tloc[3];
+  for (int jb = 0; jb < njobs; jb++) {
+    const LayerPlan &P = plans[jb];
+    const int64_t ng[3] = {P.g_last + 1, P.n2, P.n3};
+    for (int G = 0; G < 3; G++) {
+      int nt = (int)((ng[G] + tiles[G] - 1) / tiles[G]);
+      for (int t = 0; t < nt; t++) {
+        tjob[G].push_back(jb);
+        tloc[G].push_back(t);
+      }
+    }
   }
-  std::vector<DevWin> win(P.wins.size());
-  for (size_t i = 0; i < P.wins.size(); i++) {
-    const WinPlan &w = P.wins[i];
-    win[i] = DevWin{w.v1r4, w.v2r4, w.nptr4, w.seg0, w.nseg, w.p_first};
-  }
-  std::vector<DevSeg> seg(std::max<size_t>(P.segs.size(), 1));
-  for (size_t i = 0; i < P.segs.size(); i++)
-    seg[i] = DevSeg{P.segs[i].i1, P.segs[i].i2, P.segs[i].radvi, P.segs[i].rdel};
-
-  const int n = c->nlines;
-  // far-wing line range: first block with VMAX >= V1L4 (RDLNFL skip)
-  int b4 = (int)(std::lower_bound(c->h_bvmax.begin(), c->h_bvmax.end(), P.v1l4) - c->h_bvmax.begin());
-  int first4 = b4 < c->nblk ? c->h_bstart[b4] : n;
-  // near range (RDLIN): first record >= VBOT in the first block read, up
-  // to and including the first record above VTOP
-  int bn = (int)(std::lower_bound(c->h_bvmax.begin(), c->h_bvmax.end(), P.vbot) - c->h_bvmax.begin());
-  int nlo = n, nhi = n - 1;
-  if (bn < c->nblk) {
-    int s0 = c->h_bstart[bn], e0 = c->h_bstart[bn + 1];
-    nlo = (int)(std::lower_bound(c->h_vnu.begin() + s0, c->h_vnu.begin() + e0, P.vbot) -
-                c->h_vnu.begin());
-    if (nlo >= e0) nlo = e0 - 1;
-    int up = (int)(std::upper_bound(c->h_vnu.begin() + nlo, c->h_vnu.end(), P.vtop) - c->h_vnu.begin());
-    nhi = up < n ? up : n - 1;
-  }
-  const int nn = nhi >= nlo ? nhi - nlo + 1 : 0;
-  const int blk0 = nn > 0 ? c->h_blk[nlo] : 0;
-  const int nbk = nn > 0 ? c->h_blk[nhi] - blk0 + 1 : 0;
-
-  // buffers
-  const int nwin = Y.nwin;
-  if (c->w_f4.ensure((size_t)n * sizeof(F4Line)) || c->w_stop.ensure(16) ||
-      c->w_t9vnu.ensure((size_t)n * 8 + 64) || c->w_t9sp.ensure((size_t)n * 8 + 64) ||
-      c->w_t9alfa.ensure((size_t)n * 8 + 64) || c->w_t9epp.ensure((size_t)n * 8 + 64) ||
-      c->w_t9mol.ensure((size_t)n * 4 + 64) || c->w_t9n.ensure(16) ||
-      c->w_t9bstart.ensure((size_t)(n / 1000 + 16) * 4) ||
-      c->w_t9bcount.ensure((size_t)(n / 1000 + 16) * 4) || c->w_t9nblk.ensure(16) ||
-      c->w_t9bvhi.ensure((size_t)(n / 1000 + 16) * 8) ||
-      c->w_pan.ensure(pan.size() * sizeof(DevPanel)) || c->w_win.ensure(win.size() * sizeof(DevWin)) ||
-      c->w_seg.ensure(seg.size() * sizeof(DevSeg)) || c->w_r4.ensure((size_t)nwin * 2504 * 8) ||
-      c->w_near.ensure((size_t)std::max(nn, 1) * sizeof(NearLine)) ||
-      c->w_keep.ensure((size_t)std::max(nn, 1) * 4) || c->w_blkmax.ensure((size_t)(nbk + 1) * 4) ||
-      c->w_pread.ensure((size_t)(nbk + 1) * 4) || c->w_changed.ensure(16) ||
-      c->w_rec.ensure((size_t)std::max(nn, 1) * sizeof(ScatterRec)) ||
-      c->w_r3.ensure((size_t)P.n3 * 8) || c->w_r2.ensure((size_t)P.n2 * 8) || c->w_cnt.ensure(64)) {
+  Strides S;
+  S.line = ((int64_t)n + 64 + 63) / 64 * 64;
+  S.blk = nblk + 2;
+  S.t9blk = n / 1000 + 16;
+  const int64_t NJ = njobs;
+  auto need = [&](DevBuf &b, size_t bytes) { return b.ensure(bytes) != 0; };
+  if (need(c->w_dlay, NJ * sizeof(DevLayer)) || need(c->w_pan, pan.size() * sizeof(DevPanel)) ||
+      need(c->w_win, win.size() * sizeof(DevWin)) || need(c->w_seg, seg.size() * sizeof(DevSeg)) ||
+      need(c->w_r4, (size_t)nwin * R4ROW * 8) ||
+      need(c->w_cx, NJ * S.line * 8) || need(c->w_csp, NJ * S.line * 8) ||
+      need(c->w_cal, NJ * S.line * 8) || need(c->w_cad, NJ * S.line * 8) ||
+      need(c->w_cco2, NJ * S.line) || need(c->w_pn, NJ * S.line * 4) ||
+      need(c->w_pc, NJ * S.line * 4) || need(c->w_g, NJ * S.line * 4) ||
+      need(c->w_jmp, NJ * S.line * 4 * NLEV) || need(c->w_wsum, NJ * S.line * 2 * NLEV) ||
+      need(c->w_bcnt, NJ * S.blk * 4 * 7) || need(c->w_etab, NJ * S.blk * 4 * EBLK) ||
+      need(c->w_jobi, NJ * 4 * 4) || need(c->w_bat, NJ * S.line * 4 * 3) ||
+      need(c->w_t9vnu, NJ * S.line * 8) || need(c->w_t9sp, NJ * S.line * 8) ||
+      need(c->w_t9alfa, NJ * S.line * 8) || need(c->w_t9epp, NJ * S.line * 8) ||
+      need(c->w_t9mol, NJ * S.line * 4) || need(c->w_t9n, NJ * 8) ||
+      need(c->w_t9blk, NJ * S.t9blk * 16) || need(c->w_t9bvhi, NJ * S.t9blk * 8) ||
+      need(c->w_near, NJ * S.line * sizeof(NearLine)) || need(c->w_keep, NJ * S.line * 4) ||
+      need(c->w_blkmax, NJ * S.blk * 4) || need(c->w_pread, NJ * S.blk * 4) ||
+      need(c->w_changed, 16) || need(c->w_rec, NJ * S.line * sizeof(ScatterRec)) ||
+      need(c->w_r3, (size_t)r3tot * 8) || need(c->w_r2, (size_t)r2tot * 8) ||
+      need(c->w_cnt, NJ * 16) || need(c->w_margin, NJ * 3 * 8)) {
     set_err("work buffer allocation failed");
     return LBLX_E_HIP;
   }
+  for (int G = 0; G < 3; G++)
+    if (need(c->w_tmap[G], tjob[G].size() * 16)) return LBLX_E_HIP;
+  HIPCHK(hipMemcpyAsync(c->w_dlay.p, ys.data(), NJ * sizeof(DevLayer), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(c->w_pan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(c->w_win.p, win.data(), win.size() * sizeof(DevWin), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(c->w_seg.p, seg.data(), seg.size() * sizeof(DevSeg), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->w_margin.p, margins.data(), margins.size() * 8, hipMemcpyHostToDevice, s));
+  TileMap tm[3];
+  for (int G = 0; G < 3; G++) {
+    size_t nt = tjob[G].size();
+    int32_t *base = c->w_tmap[G].as<int32_t>();
+    HIPCHK(hipMemcpyAsync(base, tjob[G].data(), nt * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(base + nt, tloc[G].data(), nt * 4, hipMemcpyHostToDevice, s));
+    tm[G].job = base;
+    tm[G].loc = base + nt;
+    tm[G].lo = base + 2 * nt;
+    tm[G].hi = base + 3 * nt;
+  }
+  const DevLayer *dY = c->w_dlay.as<DevLayer>();
   DevLines L = c->lines();
+  ShrinkWork W;
+  W.cx = c->w_cx.as<double>();
+  W.csp = c->w_csp.as<double>();
+  W.cal = c->w_cal.as<double>();
+  W.cad = c->w_cad.as<double>();
+  W.cco2 = c->w_cco2.as<int8_t>();
+  W.pn = c->w_pn.as<int32_t>();
+  W.pc = c->w_pc.as<int32_t>();
+  W.g = c->w_g.as<int32_t>();
+  W.jmp = c->w_jmp.as<int32_t>();
+  W.wsum = c->w_wsum.as<int16_t>();
+  int32_t *bb = c->w_bcnt.as<int32_t>();
+  W.bcnt = bb;
+  W.bcntn = bb + NJ * S.blk;
+  W.bcntc = bb + 2 * NJ * S.blk;
+  W.bstop = bb + 3 * NJ * S.blk;
+  W.bp = bb + 4 * NJ * S.blk;
+  W.bpn = bb + 5 * NJ * S.blk;
+  W.bpc = bb + 6 * NJ * S.blk;
+  int32_t *ji = c->w_jobi.as<int32_t>();
+  W.npass = ji;
+  W.stop = ji + NJ;
+  W.nbat = ji + 2 * NJ;
+  W.etab = c->w_etab.as<int32_t>();
+  int32_t *bat = c->w_bat.as<int32_t>();
+  W.bat_s = bat;
+  W.bat_t = bat + NJ * S.line;
+  W.bat_off = bat + 2 * NJ * S.line;
   Tape9 T9;
   T9.vnu = c->w_t9vnu.as<double>();
   T9.sp = c->w_t9sp.as<double>();
@@ -367,129 +508,129 @@ static int run_layer(lblx_ctx *c, const lblx_run *run, lblx_layer *layer, void *
   T9.epp = c->w_t9epp.as<double>();
   T9.mol = c->w_t9mol.as<int32_t>();
   T9.n = c->w_t9n.as<int32_t>();
-  T9.bstart = c->w_t9bstart.as<int32_t>();
-  T9.bcount = c->w_t9bcount.as<int32_t>();
-  T9.nblk = c->w_t9nblk.as<int32_t>();
+  T9.nblk = c->w_t9n.as<int32_t>() + NJ;
+  T9.bstart = c->w_t9blk.as<int32_t>();
+  T9.bcount = c->w_t9blk.as<int32_t>() + NJ * S.t9blk;
   T9.bvhi = c->w_t9bvhi.as<double>();
-  unsigned long long *cnt = c->w_cnt.as<unsigned long long>();
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, c->w_stop.as<int>(), 1, INT_MAX);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int *)cnt, 8, 0);
-  // 1. LINF4 correction
-  if (first4 < n) {
-    int cnt4 = n - first4;
-    hipLaunchKernelGGL(k_linf4, dim3((cnt4 + 255) / 256), dim3(256), 0, s, L, Y, first4,
-                       c->w_f4.as<F4Line>(), c->w_stop.as<int>());
-  }
-  // 2. SHRINK into TAPE9
-  hipLaunchKernelGGL(k_shrink, dim3(1), dim3(256), 0, s, c->w_f4.as<F4Line>(), L, Y,
-                     c->w_stop.as<int>(), T9);
+  unsigned long long *cnt = c->w_cnt.as<unsigned long long>();  // [2][njobs]: k1, k3
+  hipLaunchKernelGGL(k_fill_i32, dim3((4 * NJ + 255) / 256), dim3(256), 0, s, (int *)cnt, 4 * NJ, 0);
+
+  // 1-2. LINF4 + SHRINK
+  int th = c->tbegin(ST_SHRINK);
+  const int lgrid = (int)((S.line + 255) / 256);
+  hipLaunchKernelGGL(k_shr_count, dim3(nblk, njobs), dim3(256), 0, s, L, dY, S, W);
+  hipLaunchKernelGGL(k_shr_scan, dim3(njobs), dim3(256), 0, s, nblk, S, W);
+  hipLaunchKernelGGL(k_shr_compact, dim3(nblk, njobs), dim3(256), 0, s, L, dY, S, W);
+  hipLaunchKernelGGL(k_shr_chain, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W);
+  for (int lev = 1; lev < NLEV; lev++)
+    hipLaunchKernelGGL(k_shr_lift, dim3(lgrid, njobs), dim3(256), 0, s, lev, S, W);
+  hipLaunchKernelGGL(k_shr_etab, dim3((nblk * EBLK + 255) / 256, njobs), dim3(256), 0, s, nblk, S, W, njobs);
+  hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), 0, s, dY, nblk, S, W, T9, njobs);
+  hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9, njobs);
+  hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);
+  c->tend(th);
   // 3. far-wing windows
-  hipLaunchKernelGGL(k_convf4, dim3(nwin), dim3(256), 0, s, T9, Y, c->w_win.as<DevWin>(),
-                     c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + 1);
-  if (nn > 0) {
+  th = c->tbegin(ST_CONVF4);
+  hipLaunchKernelGGL(k_convf4, dim3(nwin), dim3(256), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
+                     c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
+  c->tend(th);
+  if (maxnn > 0) {
+    const int ngrid = (maxnn + 255) / 256;
     // 4. LNCOR1 + first fit
-    hipLaunchKernelGGL(k_lncor1, dim3((nn + 255) / 256), dim3(256), 0, s, L, Y,
-                       c->w_pan.as<DevPanel>(), nlo, nn, c->w_near.as<NearLine>());
+    th = c->tbegin(ST_LNCOR1);
+    hipLaunchKernelGGL(k_lncor1, dim3(ngrid, njobs), dim3(256), 0, s, L, dY, S,
+                       c->w_pan.as<DevPanel>(), c->w_near.as<NearLine>());
+    c->tend(th);
     // 5. block-read panel fixpoint
-    hipLaunchKernelGGL(k_fill_i32, dim3((nbk + 255) / 256), dim3(256), 0, s, c->w_pread.as<int>(), nbk, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3((nbk + 255) / 256), dim3(256), 0, s, c->w_blkmax.as<int>(), nbk, 0);
-    int it = 0;
-    for (; it < 64; it++) {
+    th = c->tbegin(ST_REJECT);
+    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s,
+                       c->w_pread.as<int>(), NJ * S.blk, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s,
+                       c->w_blkmax.as<int>(), NJ * S.blk, 0);
+    for (int it = 0; it < 64; it++) {
       hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, c->w_changed.as<int>(), 1, 0);
-      hipLaunchKernelGGL(k_reject, dim3((nn + 255) / 256), dim3(256), 0, s, c->w_near.as<NearLine>(), nn,
-                         nlo, c->d_blk.as<int32_t>(), blk0, c->w_pread.as<int>(),
-                         c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(), c->w_r4.as<double>(), Y,
-                         c->w_keep.as<int>(), c->w_blkmax.as<int>());
-      hipLaunchKernelGGL(k_pread_scan, dim3(1), dim3(256), 0, s, c->w_blkmax.as<int>(), nbk,
+      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY,
+                         S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
+                         c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
+      hipLaunchKernelGGL(k_pread_scan, dim3(njobs), dim3(256), 0, s, dY, S, c->w_blkmax.as<int>(),
                          c->w_pread.as<int>(), c->w_changed.as<int>());
-      if (it >= 1) {
+      if (it >= 2) {
         int changed = 1;
         HIPCHK(hipMemcpyAsync(&changed, c->w_changed.p, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (!changed) break;
       }
     }
-    // the final pass left keep/pread consistent; rerun k_reject so that keep
-    // matches the converged pread exactly
-    hipLaunchKernelGGL(k_reject, dim3((nn + 255) / 256), dim3(256), 0, s, c->w_near.as<NearLine>(), nn,
-                       nlo, c->d_blk.as<int32_t>(), blk0, c->w_pread.as<int>(),
-                       c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(), c->w_r4.as<double>(), Y,
-                       c->w_keep.as<int>(), c->w_blkmax.as<int>());
+    // keep flags for the converged read panels
+    hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY, S,
+                       c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
+                       c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
+    c->tend(th);
     // 6. CNVFNV scatter records
-    hipLaunchKernelGGL(k_scatter_rec, dim3(nbk), dim3(256), 0, s, c->w_near.as<NearLine>(),
-                       c->w_keep.as<int>(), nlo, L, blk0, nbk, c->w_pread.as<int>(),
-                       c->w_pan.as<DevPanel>(), Y, nn, c->w_rec.as<ScatterRec>());
-    hipLaunchKernelGGL(k_sum_npts, dim3((nn + 255) / 256), dim3(256), 0, s, c->w_rec.as<ScatterRec>(),
-                       nn, cnt);
+    th = c->tbegin(ST_SCATTER);
+    hipLaunchKernelGGL(k_scatter_rec, dim3(maxnbk, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(),
+                       c->w_keep.as<int>(), L, dY, S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(),
+                       c->w_rec.as<ScatterRec>());
+    hipLaunchKernelGGL(k_sum_npts, dim3(ngrid, njobs), dim3(256), 0, s, dY, S, c->w_rec.as<ScatterRec>(), cnt);
+    c->tend(th);
   }
   // 7. tiles: R3, R2, R1 (+ PANEL combine)
-  const int tiles[3] = {2048, 1024, 512};  // grid 1, 2, 3
-  const int64_t ng[3] = {P.g_last + 1, P.n2, P.n3};
-  const double dvg[3] = {P.dv, P.dvr2, P.dvr3};
-  double shiftm = c->max_pshift * P.rhorat + 2. * P.dv;
+  const double *tabs = c->d_tabs.as<double>();
   for (int G = 3; G >= 1; G--) {
-    int T = tiles[G - 1];
-    int nt = (int)((ng[G - 1] + T - 1) / T);
-    if (c->w_tlo.ensure((size_t)nt * 4) || c->w_thi.ensure((size_t)nt * 4)) return LBLX_E_HIP;
-    double margin = (double)(4 * (P.nalfmax + 1) + 8) * dvg[G - 1] + shiftm;
-    if (nn > 0)
-      hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s,
-                         c->d_vnu.as<double>() + nlo, nn, P.vft0, dvg[G - 1], T, nt, margin,
-                         c->w_tlo.as<int>(), c->w_thi.as<int>());
-    else {
-      hipLaunchKernelGGL(k_fill_i32, dim3((nt + 255) / 256), dim3(256), 0, s, c->w_tlo.as<int>(), nt, 0);
-      hipLaunchKernelGGL(k_fill_i32, dim3((nt + 255) / 256), dim3(256), 0, s, c->w_thi.as<int>(), nt, 0);
-    }
+    const int T = tiles[G - 1];
+    const int nt = (int)tjob[G - 1].size();
+    int tht = c->tbegin(G == 3 ? ST_TILE3 : (G == 2 ? ST_TILE2 : ST_TILE1));
+    hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY,
+                       tm[G - 1], nt, T, G, c->w_margin.as<double>());
     size_t shm = (size_t)(T + (G == 1 ? 3 : 1) * 2601) * sizeof(double);
-    const double *tabs = c->d_tabs.as<double>();
     if (G == 3) {
       HIPCHK(hipFuncSetAttribute((const void *)k_tile<3, double>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
       hipLaunchKernelGGL((k_tile<3, double>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
-                         c->w_tlo.as<int>(), c->w_thi.as<int>(), T, tabs, Y, c->w_pan.as<DevPanel>(),
-                         c->w_win.as<DevWin>(), c->w_r4.as<double>(), (const double *)nullptr,
-                         c->w_seg.as<DevSeg>(), (const double *)nullptr, c->w_r3.as<double>(),
-                         (double *)nullptr);
+                         tm[2], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
+                         c->w_r4.as<double>(), (const double *)nullptr, c->w_seg.as<DevSeg>(),
+                         (const double *)nullptr, c->w_r3.as<double>(), (double *)nullptr);
     } else if (G == 2) {
       HIPCHK(hipFuncSetAttribute((const void *)k_tile<2, double>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
       hipLaunchKernelGGL((k_tile<2, double>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
-                         c->w_tlo.as<int>(), c->w_thi.as<int>(), T, tabs, Y, c->w_pan.as<DevPanel>(),
-                         c->w_win.as<DevWin>(), c->w_r4.as<double>(), (const double *)nullptr,
-                         c->w_seg.as<DevSeg>(), c->w_r3.as<double>(), c->w_r2.as<double>(),
-                         (double *)nullptr);
+                         tm[1], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
+                         c->w_r4.as<double>(), (const double *)nullptr, c->w_seg.as<DevSeg>(),
+                         c->w_r3.as<double>(), c->w_r2.as<double>(), (double *)nullptr);
     } else if (!f32out) {
       HIPCHK(hipFuncSetAttribute((const void *)k_tile<1, double>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
       hipLaunchKernelGGL((k_tile<1, double>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
-                         c->w_tlo.as<int>(), c->w_thi.as<int>(), T, tabs, Y, c->w_pan.as<DevPanel>(),
-                         c->w_win.as<DevWin>(), c->w_r4.as<double>(), (const double *)nullptr,
-                         c->w_seg.as<DevSeg>(), c->w_r2.as<double>(), (double *)nullptr,
-                         (double *)d_od);
+                         tm[0], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
+                         c->w_r4.as<double>(), (const double *)nullptr, c->w_seg.as<DevSeg>(),
+                         c->w_r2.as<double>(), (double *)nullptr, (double *)od_base);
     } else {
       HIPCHK(hipFuncSetAttribute((const void *)k_tile<1, float>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
       hipLaunchKernelGGL((k_tile<1, float>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
-                         c->w_tlo.as<int>(), c->w_thi.as<int>(), T, tabs, Y, c->w_pan.as<DevPanel>(),
-                         c->w_win.as<DevWin>(), c->w_r4.as<double>(), (const double *)nullptr,
-                         c->w_seg.as<DevSeg>(), c->w_r2.as<double>(), (double *)nullptr,
-                         (float *)d_od);
+                         tm[0], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
+                         c->w_r4.as<double>(), (const double *)nullptr, c->w_seg.as<DevSeg>(),
+                         c->w_r2.as<double>(), (double *)nullptr, (float *)od_base);
     }
+    c->tend(tht);
   }
   HIPCHK(hipGetLastError());
-  if (v1p) *v1p = P.panels[0].vft + (double)(NSHIFT) * P.dv;
+  if (v1p)
+    for (int jb = 0; jb < njobs; jb++) v1p[jb] = plans[jb].panels[0].vft + (double)NSHIFT * plans[jb].dv;
   if (st) {
-    unsigned long long h[2] = {0, 0};
-    int32_t t9n = 0;
-    HIPCHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&t9n, c->w_t9n.p, 4, hipMemcpyDeviceToHost, s));
+    std::vector<unsigned long long> h(2 * NJ);
+    std::vector<int32_t> t9n(NJ);
+    HIPCHK(hipMemcpyAsync(h.data(), cnt, 2 * NJ * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(t9n.data(), T9.n, NJ * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    std::memset(st, 0, sizeof(*st));
-    st->k1_updates = (int64_t)h[0];
-    st->k3_updates = (int64_t)h[1];
-    st->nlins4 = t9n;
-    st->npanel = (int64_t)P.panels.size();
-    st->npts = P.npts;
+    for (int jb = 0; jb < njobs; jb++) {
+      std::memset(&st[jb], 0, sizeof(lblx_stats));
+      st[jb].k1_updates = (int64_t)h[jb];
+      st[jb].k3_updates = (int64_t)h[NJ + jb];
+      st[jb].nlins4 = t9n[jb];
+      st[jb].npanel = (int64_t)plans[jb].panels.size();
+      st[jb].npts = plans[jb].npts;
+    }
   }
   return LBLX_OK;
 }
@@ -500,7 +641,12 @@ int lblx_od_layer_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layer, vo
                          int64_t n_out, double *v1p, lblx_stats *st) {
   if (!c || !run || !layer || !d_od) return LBLX_E_ARG;
   HIPCHK(hipSetDevice(c->device));
-  int rc = run_layer(c, run, layer, d_od, n_out, run->precision == LBLX_FP32_SPLIT, v1p, st);
+  int64_t npts = 0;
+  int rc = lblx_od_size(run, layer, &npts);
+  if (rc) return rc;
+  if (n_out < npts) return LBLX_E_SIZE;
+  int64_t off = 0;
+  rc = run_jobs(c, run, layer, 1, false, d_od, &off, run->precision == LBLX_FP32_SPLIT, v1p, st);
   if (rc) return rc;
   return LBLX_OK;
 }
@@ -515,10 +661,61 @@ int lblx_od_layer(lblx_ctx *c, const lblx_run *run, lblx_layer *layer, void *od,
   if (n_out < npts) return LBLX_E_SIZE;
   size_t esz = run->precision == LBLX_FP32_SPLIT ? 4 : 8;
   if (c->w_od.ensure((size_t)npts * esz)) return LBLX_E_HIP;
-  rc = run_layer(c, run, layer, c->w_od.p, npts, esz == 4, v1p, st);
+  int64_t off = 0;
+  rc = run_jobs(c, run, layer, 1, false, c->w_od.p, &off, esz == 4, v1p, st);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(od, c->w_od.p, (size_t)npts * esz, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return LBLX_OK;
+}
+
+int lblx_od_column_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
+                          void *d_od, const int64_t *od_off, double *v1p, lblx_stats *st) {
+  if (!c || !run || !layers || nlay <= 0 || !d_od || !od_off) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  return run_jobs(c, run, layers, nlay, true, d_od, od_off, run->precision == LBLX_FP32_SPLIT, v1p, st);
+}
+
+// Copy job `job`'s shrunk far-wing line list (TAPE9) of the last run to
+// host memory (diagnostics and tests).
+int lblx_debug_tape9(lblx_ctx *c, int job, double *vnu, double *sp, double *alfa, double *epp,
+                     int32_t *mol, int cap, int32_t *n_out) {
+  if (!c || !n_out || job < 0) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int64_t line = ((int64_t)c->nlines + 64 + 63) / 64 * 64;
+  int32_t n = 0;
+  HIPCHK(hipMemcpy(&n, c->w_t9n.as<int32_t>() + job, 4, hipMemcpyDeviceToHost));
+  *n_out = n;
+  int k = n < cap ? n : cap;
+  if (k > 0) {
+    int64_t o = job * line;
+    HIPCHK(hipMemcpy(vnu, c->w_t9vnu.as<double>() + o, k * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sp, c->w_t9sp.as<double>() + o, k * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(alfa, c->w_t9alfa.as<double>() + o, k * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(epp, c->w_t9epp.as<double>() + o, k * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(mol, c->w_t9mol.as<int32_t>() + o, k * 4, hipMemcpyDeviceToHost));
+  }
+  return LBLX_OK;
+}
+
+int lblx_timing(lblx_ctx *c, int enable) {
+  if (!c) return LBLX_E_ARG;
+  c->tcollect();
+  c->timing = enable != 0;
+  for (int i = 0; i < ST_N; i++) {
+    c->stage_ms[i] = 0.;
+    c->stage_n[i] = 0;
+  }
+  return LBLX_OK;
+}
+
+int lblx_timing_get(lblx_ctx *c, int stage, const char **name, double *ms, int64_t *count) {
+  if (!c || stage < 0 || stage >= ST_N) return LBLX_E_ARG;
+  c->tcollect();
+  if (name) *name = kStageNames[stage];
+  if (ms) *ms = c->stage_ms[stage];
+  if (count) *count = c->stage_n[stage];
   return LBLX_OK;
 }
 
@@ -529,6 +726,31 @@ int lblx_sync(lblx_ctx *c) {
 }
 
 void *lblx_stream(lblx_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+void *lblx_dev_alloc(lblx_ctx *c, int64_t bytes) {
+  if (!c || bytes <= 0) return nullptr;
+  if (hipSetDevice(c->device) != hipSuccess) return nullptr;
+  void *p = nullptr;
+  if (hipMalloc(&p, (size_t)bytes) != hipSuccess) return nullptr;
+  return p;
+}
+
+int lblx_dev_free(lblx_ctx *c, void *p) {
+  if (!c) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  if (p) HIPCHK(hipFree(p));
+  return LBLX_OK;
+}
+
+int lblx_copy(lblx_ctx *c, void *dst, const void *src, int64_t bytes, int kind) {
+  if (!c || bytes < 0) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                    : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, k, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return LBLX_OK;
+}
 
 }  // extern "C"
 

@@ -1,9 +1,14 @@
 // Device-side data layouts shared by the host orchestration and the
-// kernels of the layer optical-depth path.
+// kernels of the layer optical-depth path.  A "job" is one layer of one
+// profile; every stage launches once for all jobs of a batch.
 #pragma once
 #include <cstdint>
 
 namespace lblx {
+
+constexpr int NLEV = 11;   // pointer-doubling levels: SHRINK batches < 2^11 lines
+constexpr int EBLK = 8;    // batch lengths (in TAPE3 blocks) precomputed per block
+constexpr int R4ROW = 2504;
 
 // Resident line table (TAPE3 records in file order, struct-of-arrays).
 struct DevLines {
@@ -17,7 +22,7 @@ struct DevLines {
   const int32_t *blk;     // block of each record
 };
 
-// Layer constants the kernels need (a flattened subset of LayerPlan).
+// Per-job constants and offsets (a flattened LayerPlan).
 struct DevLayer {
   double dv, dvr2, dvr3, dvr4, bound4, alfmax;
   double vft0, vbot, vtop, v1l4, v2l4;
@@ -26,9 +31,18 @@ struct DevLayer {
   double scor[7][10], alfd1[7][10], rhoslf[8], wk[8];
   int32_t nmol, jrad, max1, npanel, nwin;
   int64_t g_last, g2max, g3max, n1, n2, n3;
-  // continuum grid (ICNTNM): absrb[0..nptabs) on v1abs + k*dvabs
+  // continuum grid (ICNTNM): absrb[1..nptabs] on v1abs + (k-1)*dvabs
   double v1abs, v2abs, dvabs;
   int32_t nptabs, icntnm;
+  int64_t abs0;           // offset of this job's ABSRB row (1-based entries)
+  // offsets into concatenated plan tables
+  int32_t pan0, win0, seg0;
+  // far-wing line range: first TAPE3 block with VMAX >= V1L4
+  int32_t b4;
+  // near-line range (RDLIN): records [nlo, nlo+nn), blocks [blk0, blk0+nbk)
+  int32_t nlo, nn, blk0, nbk;
+  // outputs and grids
+  int64_t od0, r2o, r3o;
 };
 
 struct DevPanel {   // mirror of PanelPlan
@@ -37,9 +51,9 @@ struct DevPanel {   // mirror of PanelPlan
   int64_t out0;
 };
 
-struct DevWin {     // mirror of WinPlan
+struct DevWin {     // mirror of WinPlan, plus its job
   double v1r4, v2r4;
-  int32_t nptr4, seg0, nseg, p_first;
+  int32_t nptr4, seg0, nseg, job;
 };
 
 struct DevSeg {
@@ -47,18 +61,33 @@ struct DevSeg {
   double radvi, rdel;
 };
 
-// Per-line output of the LINF4 correction (far-wing preparation).
-struct F4Line {
-  double x, sp, al, ad;  // shifted centre, strength, Lorentz and Doppler widths
-  int32_t mol;           // molecule (2 = CO2 handled separately)
-  int32_t pass;          // enters SHRINK
+// Strides of the per-job slices of the work arrays.
+struct Strides {
+  int64_t line;   // >= records + 64
+  int64_t blk;    // >= blocks + 2
+  int64_t t9blk;  // TAPE9 blocks per job
 };
 
-// Shrunk far-wing line list (TAPE9) and its block table.
+// SHRINK work arrays (compacted passing lines of LINF4 and the chain tables).
+struct ShrinkWork {
+  double *cx, *csp, *cal, *cad;  // centre, strength (0 below 1e-35), widths
+  int8_t *cco2;
+  int32_t *pn, *pc;       // [n+1] exclusive prefix of positive non-CO2 / CO2 lines
+  int32_t *g;             // first-group end from each start
+  int32_t *jmp;           // [NLEV][line] chain doubling
+  int16_t *wsum;          // [NLEV][line] emitted lines over the doubled chain
+  int32_t *bcnt, *bcntn, *bcntc, *bstop;  // per TAPE3 block
+  int32_t *bp, *bpn, *bpc;                // block prefixes
+  int32_t *npass, *stop;                  // per job
+  int32_t *etab;          // [blk][EBLK] emitted lines of a batch of k+1 blocks
+  int32_t *bat_s, *bat_t, *bat_off, *nbat;  // batches
+};
+
+// Shrunk far-wing line list (TAPE9) and its block table, per job.
 struct Tape9 {
   double *vnu, *sp, *alfa, *epp;
   int32_t *mol;
-  int32_t *n;            // [1] lines written
+  int32_t *n;            // [job] lines written
   int32_t *bstart, *bcount, *nblk;  // TAPE9 block table
   double *bvhi;
 };

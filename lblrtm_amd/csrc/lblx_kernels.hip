@@ -1,22 +1,25 @@
 // HIP kernels for the layer optical-depth path on MI355X (gfx950).
 //
 // The reference computes one layer panel by panel on one core (HIRAC1,
-// src/oprop.f90:25-668).  Here the whole band of a layer is resident in
-// HBM and each stage is one launch:
-//   k_linf4       LINF4 line correction            oprop.f90:3093-3235
-//   k_shrink      LINF4 batching + SHRINK          oprop.f90:3236-3290, 3508-3640
-//   k_convf4      LBLF4/CONVF4, one window per WG  oprop.f90:3641-4039
-//   k_lncor1      LNCOR1 line correction           oprop.f90:995-1193
-//   k_reject*     LNCOR1 rejection + block-read panel fixpoint
-//   k_scatter_rec CNVFNV per-line set-up           oprop.f90:1262-1305
-//   k_tile<G>     CNVFNV scatter into R3/R2/R1 tiles in LDS, with the
-//                 PANEL/XINT combine fused into the epilogue
-//                 oprop.f90:1307-1320, 1459-1519, 2201-2234
-// All order-dependent decisions of the reference (line rejection against
-// the running far-wing array, SHRINK batch boundaries, far-wing windows)
-// are reproduced exactly; see the kernel comments.
+// src/oprop.f90:25-668).  Here every layer ("job") of a batch is resident
+// in HBM over its whole band and each stage is one launch for all jobs:
+//   LINF4 + SHRINK   k_shr_count, k_shr_scan, k_shr_compact, k_shr_chain,
+//                    k_shr_lift, k_shr_etab, k_shr_batches, k_shr_emit
+//                    (oprop.f90:3093-3290, 3508-3640)
+//   LBLF4/CONVF4     k_convf4, one far-wing window per workgroup
+//                    (oprop.f90:3641-4039)
+//   LNCOR1           k_lncor1 + the block-read fixpoint k_reject/k_pread_scan
+//                    (oprop.f90:734-885, 995-1193)
+//   CNVFNV set-up    k_scatter_rec (oprop.f90:1262-1305)
+//   CNVFNV + PANEL   k_tile<G>: scatter into R3/R2/R1 tiles in LDS with the
+//                    XINT / Lagrange / radiation combine in the epilogue
+//                    (oprop.f90:1307-1320, 1459-1519, 2201-2234)
+// Every order-dependent decision of the reference (line rejection against
+// the running far-wing array, SHRINK batch boundaries, far-wing windows,
+// the panel at which a line block is read) is reproduced exactly.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 
 #include "lblx_kernels.h"
@@ -28,6 +31,7 @@ __constant__ int c_isomax[7];
 
 constexpr double ONEPL_D = 1.001, ONEMI_D = 0.999;
 constexpr int NFMX = 2601;
+constexpr int LIMIN4 = 1000;  // LINF4 batch size
 
 __device__ __forceinline__ int fint(double x) { return (int)x; }
 
@@ -43,8 +47,7 @@ __device__ __forceinline__ double d_radfn(double vi, double xkt) {
   return vi;
 }
 
-// ---------------------------------------------------------------------
-// Block-wide exclusive scan (sum) for 256 threads.
+// Block-wide exclusive scan (sum) for NT threads.
 template <int NT>
 __device__ int block_scan_excl(int v, int *buf, int *total) {
   int tid = threadIdx.x;
@@ -63,250 +66,417 @@ __device__ int block_scan_excl(int v, int *buf, int *total) {
   return incl - v;
 }
 
+__global__ void k_fill_i32(int *p, int64_t n, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // ---------------------------------------------------------------------
-// LINF4 correction of every record: shifted centre, temperature-corrected
-// strength, Lorentz and Doppler widths (oprop.f90:3093-3235).  Records the
-// first passing line above V2L4, where LINF4 stops reading (IEOF = 1).
-__global__ void k_linf4(DevLines L, DevLayer Y, int first, F4Line *out, int *stop_idx) {
-  int i = first + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= L.n) return;
-  F4Line o;
-  o.pass = 0;
-  o.x = 0.;
-  o.sp = 0.;
-  o.al = 0.;
-  o.ad = 0.;
-  o.mol = 0;
+// LINF4 correction of one record (oprop.f90:3093-3235).  Returns whether
+// the record enters SHRINK and fills the corrected centre, strength and
+// widths.  RDLNFL's O2 air->foreign width conversion (oprop.f90:3456-3462)
+// is applied first.
+struct F4Line {
+  double x, sp, al, ad;
+  int co2;
+};
+
+__device__ __forceinline__ bool linf4_line(const DevLines &L, const DevLayer &Y, int i, F4Line &o) {
   int b = L.blk[i];
   int mc = L.mol[i];
   int m = mc % 100, iso = (mc % 1000) / 100;
   double vraw = L.vnu[i];
-  if (L.bvmax[b] >= Y.v1l4 && m >= 1 && m <= Y.nmol && iso >= 1 && iso <= c_isomax[m - 1]) {
-    double sui = (double)L.sp[i] * Y.wk[m - 1];
-    if (sui != 0. && vraw >= Y.v1l4) {
-      double alfa = L.alfa[i], hwhm = L.hwhm[i];
-      if (m == 7) alfa = (alfa - 0.21 * hwhm) / (1.0 - 0.21);  // RDLNFL O2 (oprop.f90:3456-3462)
-      double x = vraw + Y.rhorat * (double)L.pshift[i];
-      double epp = L.epp[i];
-      if (epp <= -1.001) epp = fabs(epp);
-      if (epp > -0.999)
-        sui = sui * Y.scor[m - 1][iso - 1] * exp(-epp * Y.betacr4) * (1. + exp(-x * Y.beta4));
-      double tmpcor = pow(Y.tratio, (double)L.tmpalf[i]);
-      double alfa0i = alfa * tmpcor;
-      double hwhmsi = hwhm * tmpcor;
-      o.al = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
-      o.ad = x * Y.alfd1[m - 1][iso - 1];
-      o.x = x;
-      o.sp = sui;
-      o.mol = m;
-      o.pass = 1;
-      if (x > Y.v2l4) atomicMin(stop_idx, i);
-    }
-  }
-  out[i] = o;
+  if (L.bvmax[b] < Y.v1l4 || m < 1 || m > Y.nmol || iso < 1 || iso > c_isomax[m - 1]) return false;
+  double sui = (double)L.sp[i] * Y.wk[m - 1];
+  if (sui == 0. || vraw < Y.v1l4) return false;
+  double alfa = L.alfa[i], hwhm = L.hwhm[i];
+  if (m == 7) alfa = (alfa - 0.21 * hwhm) / (1.0 - 0.21);
+  double x = vraw + Y.rhorat * (double)L.pshift[i];
+  double epp = L.epp[i];
+  if (epp <= -1.001) epp = fabs(epp);
+  if (epp > -0.999) sui = sui * Y.scor[m - 1][iso - 1] * exp(-epp * Y.betacr4) * (1. + exp(-x * Y.beta4));
+  double tmpcor = pow(Y.tratio, (double)L.tmpalf[i]);
+  double alfa0i = alfa * tmpcor;
+  double hwhmsi = hwhm * tmpcor;
+  o.al = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
+  o.ad = x * Y.alfd1[m - 1][iso - 1];
+  o.x = x;
+  o.sp = sui < 1.0e-35 ? 0.0 : sui;  // SHRINK zeroes S < 1e-35 (oprop.f90:3546-3549)
+  o.co2 = m == 2;
+  return true;
 }
 
 // ---------------------------------------------------------------------
-// LINF4 batching and SHRINK for one layer in one workgroup.
+// SHRINK, in parallel.
 //
-// LINF4 reads 250-line TAPE3 blocks until the buffer holds >= 1000 lines
-// (shrunk lines carried from earlier batches plus new raw lines), then
-// SHRINK merges consecutive lines into groups: a group ends before the
-// first line whose centre exceeds VLMT, and VLMT is reset to (centre of
-// the group's last line) + DVR4/2.  Group boundaries therefore form a
-// chain e_{k+1} = f(e_k) with f(e) = (first k >= e+2, x_k > x_e + DVR4/2)
-// - 1; the chain of each batch is marked with pointer doubling in LDS.
-// Each group emits one non-CO2 and one CO2 effective line (strength-
-// weighted centre and widths) if it holds any positive strength of that
-// class.  TAPE9 blocks close when they hold >= 1000 lines.
-constexpr int SHR_T = 256;
-constexpr int SHR_MAX = 1280;
-constexpr int SHR_K = 11;
+// LINF4 reads 250-line TAPE3 blocks until its buffer holds >= 1000 lines
+// (shrunk lines kept from earlier batches of the current TAPE9 block plus
+// new raw lines) and then SHRINKs the new lines as one batch.  Inside a
+// batch a group ends before the first line whose centre exceeds VLMT;
+// VLMT starts at x_s + DVR4/2 and is reset to (centre of the group's last
+// line) + DVR4/2.  Group ends therefore form the chain
+//     e_1 = g(s),  e_{k+1} = f(e_k),
+//     f(e) = (first k >= e+2 with x_k > x_e + DVR4/2) - 1,
+//     g(s) = (first k >= s+1 with x_k > x_s + DVR4/2) - 1,
+// cut at the batch end.  A group emits one non-CO2 and one CO2 effective
+// line if it holds positive strength of that class.  The chain is doubled
+// (jmp, wsum) so that the number of lines a batch emits, and the group of
+// any line, are O(log) queries; the batch boundaries, which depend on
+// those counts, are walked by one thread per job over per-block tables.
+constexpr int SB_T = 256;
 
-__global__ __launch_bounds__(SHR_T) void k_shrink(const F4Line *f4, DevLines L, DevLayer Y,
-                                                  const int *stop_idx, Tape9 T9) {
-  __shared__ double bx[SHR_MAX], bsp[SHR_MAX], bal[SHR_MAX], bad[SHR_MAX];
-  __shared__ int8_t bco2[SHR_MAX];
-  __shared__ int16_t jmp[SHR_K][SHR_MAX + 1];
-  __shared__ int16_t gst[SHR_MAX];
-  __shared__ int scnt[SHR_MAX];
-  __shared__ int sbuf[SHR_T];
-  __shared__ int s_g;
-  __shared__ double s_lastx;
-  const int tid = threadIdx.x;
-  const double dv2 = Y.dvr4 / 2.;
-  const int stop_i = *stop_idx;
-  int J = 0, out_n = 0, t9s = 0, t9nb = 0, nb = 0;
-  bool stop = false;
+__device__ __forceinline__ int em_cnt(const int32_t *pn, const int32_t *pc, int a, int b) {
+  return (pn[b + 1] - pn[a] > 0 ? 1 : 0) + (pc[b + 1] - pc[a] > 0 ? 1 : 0);
+}
 
-  auto flush = [&]() {
-    if (J > 0 && tid == 0) {
-      T9.bstart[t9nb] = t9s;
-      T9.bcount[t9nb] = J;
-      T9.bvhi[t9nb] = s_lastx;
+// per TAPE3 block: passing lines, positive non-CO2 / CO2 lines, and the
+// block-local position of the first passing line above V2L4 (LINF4 stops
+// there with IEOF = 1, oprop.f90:3228-3231)
+__global__ __launch_bounds__(SB_T) void k_shr_count(DevLines L, const DevLayer *Ys, Strides S,
+                                                    ShrinkWork W) {
+  __shared__ int buf[SB_T];
+  __shared__ int s_stop;
+  const int job = blockIdx.y;
+  const DevLayer &Y = Ys[job];
+  const int b = blockIdx.x;
+  if (b >= L.nblk) return;
+  int i = L.bstart[b] + threadIdx.x;
+  int pass = 0, pn = 0, pc = 0, stp = 0;
+  if (b >= Y.b4 && i < L.bstart[b + 1]) {
+    F4Line o;
+    if (linf4_line(L, Y, i, o)) {
+      pass = 1;
+      pn = (o.sp > 0. && !o.co2);
+      pc = (o.sp > 0. && o.co2);
+      stp = o.x > Y.v2l4;
     }
-    if (J > 0) t9nb++;
-    t9s = out_n;
-    J = 0;
-  };
+  }
+  if (threadIdx.x == 0) s_stop = INT_MAX;
+  int tot, tn, tc;
+  int pos = block_scan_excl<SB_T>(pass, buf, &tot);
+  block_scan_excl<SB_T>(pn, buf, &tn);
+  block_scan_excl<SB_T>(pc, buf, &tc);
+  if (stp) atomicMin(&s_stop, pos);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t o = job * S.blk + b;
+    W.bcnt[o] = tot;
+    W.bcntn[o] = tn;
+    W.bcntc[o] = tc;
+    W.bstop[o] = s_stop == INT_MAX ? -1 : s_stop;
+  }
+}
 
-  auto process = [&](int n) -> int {
-    if (n == 0) return 0;
-    for (int i = tid; i < n; i += SHR_T)
-      if (bsp[i] < 1.0e-35) bsp[i] = 0.0;
-    for (int i = tid; i < n; i += SHR_T) {
-      double thr = bx[i] + dv2;
-      int k = i + 2;
-      while (k < n && !(bx[k] > thr)) k++;
-      int f = k - 1;
-      if (f > n - 1) f = n - 1;
-      jmp[0][i] = (int16_t)((i == n - 1) ? n : f);
+// per job: exclusive prefix over blocks of the three counts; passing
+// index of the stop line (or -1)
+__global__ __launch_bounds__(256) void k_shr_scan(int nblk, Strides S, ShrinkWork W) {
+  __shared__ int buf[256];
+  __shared__ int carry[3];
+  __shared__ int s_stop;
+  const int job = blockIdx.x;
+  const int64_t o = job * S.blk;
+  if (threadIdx.x < 3) carry[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_stop = INT_MAX;
+  __syncthreads();
+  for (int base = 0; base < nblk; base += 256) {
+    int b = base + threadIdx.x;
+    int v[3] = {0, 0, 0};
+    if (b < nblk) {
+      v[0] = W.bcnt[o + b];
+      v[1] = W.bcntn[o + b];
+      v[2] = W.bcntc[o + b];
     }
-    if (tid == 0) {
-      jmp[0][n] = (int16_t)n;
-      double thr = bx[0] + dv2;
-      int k = 1;
-      while (k < n && !(bx[k] > thr)) k++;
-      s_g = k - 1;
-    }
-    __syncthreads();
-    int nlev = 1;
-    while ((1 << nlev) <= n && nlev < SHR_K) nlev++;
-    for (int lev = 1; lev < nlev; lev++) {
-      for (int i = tid; i <= n; i += SHR_T) jmp[lev][i] = jmp[lev - 1][jmp[lev - 1][i]];
+    int32_t *dst[3] = {W.bp, W.bpn, W.bpc};
+    for (int q = 0; q < 3; q++) {
+      int tot;
+      int ex = block_scan_excl<256>(v[q], buf, &tot);
+      if (b < nblk) dst[q][o + b] = carry[q] + ex;
+      if (q == 0 && b < nblk && W.bstop[o + b] >= 0) atomicMin(&s_stop, carry[0] + ex + W.bstop[o + b]);
+      __syncthreads();
+      if (threadIdx.x == 0) carry[q] += tot;
       __syncthreads();
     }
-    const int g = s_g;
-    for (int i = tid; i < n; i += SHR_T) {
-      int ge, gs;
-      if (i <= g) {
-        ge = g;
-        gs = 0;
-      } else {
-        int cur = g;
-        for (int lev = nlev - 1; lev >= 0; lev--) {
-          int nx = jmp[lev][cur];
-          if (nx <= i - 1) cur = nx;
-        }
-        ge = jmp[0][cur];
-        gs = cur + 1;
-      }
-      gst[i] = (int16_t)((ge == i) ? gs : -1);
-    }
-    __syncthreads();
-    // emitted lines per group end
-    for (int i = tid; i < n; i += SHR_T) {
-      int c = 0;
-      int gs = gst[i];
-      if (gs >= 0) {
-        int a = 0, b = 0;
-        for (int k = gs; k <= i; k++) {
-          if (bsp[k] > 0.) {
-            if (bco2[k]) b = 1;
-            else a = 1;
-          }
-        }
-        c = a + b;
-      }
-      scnt[i] = c;
-    }
-    __syncthreads();
-    // exclusive scan of scnt over n entries (5 per thread)
-    constexpr int PER = (SHR_MAX + SHR_T - 1) / SHR_T;
-    int loc[PER];
-    int s = 0;
-    for (int q = 0; q < PER; q++) {
-      int i = tid * PER + q;
-      loc[q] = (i < n) ? scnt[i] : 0;
-      s += loc[q];
-    }
-    int total;
-    int base = block_scan_excl<SHR_T>(s, sbuf, &total);
-    __syncthreads();
-    for (int q = 0; q < PER; q++) {
-      int i = tid * PER + q;
-      if (i < n) scnt[i] = base;
-      base += loc[q];
-    }
-    __syncthreads();
-    for (int i = tid; i < n; i += SHR_T) {
-      int gs = gst[i];
-      if (gs < 0) continue;
-      double sumv = 0., sums = 0., sumal = 0., sumad = 0.;
-      double sumv2 = 0., sums2 = 0., sumal2 = 0., sumad2 = 0.;
-      for (int k = gs; k <= i; k++) {
-        double sp = bsp[k];
-        if (!bco2[k]) {
-          sumv = sumv + bx[k] * sp;
-          sums = sums + sp;
-          sumal = sumal + sp * bal[k];
-          sumad = sumad + sp * bad[k];
-        } else {
-          sumv2 = sumv2 + bx[k] * sp;
-          sums2 = sums2 + sp;
-          sumal2 = sumal2 + sp * bal[k];
-          sumad2 = sumad2 + sp * bad[k];
-        }
-      }
-      int o = out_n + scnt[i];
-      if (sums > 0.) {
-        T9.vnu[o] = sumv / sums;
-        T9.sp[o] = sums;
-        T9.alfa[o] = sumal / sums;
-        T9.epp[o] = sumad / sums;
-        T9.mol[o] = 0;
-        if (o == out_n + total - 1) s_lastx = sumv / sums;
-        o++;
-      }
-      if (sums2 > 0.) {
-        T9.vnu[o] = sumv2 / sums2;
-        T9.sp[o] = sums2;
-        T9.alfa[o] = sumal2 / sums2;
-        T9.epp[o] = sumad2 / sums2;
-        T9.mol[o] = 2;
-        if (o == out_n + total - 1) s_lastx = sumv2 / sums2;
-      }
-    }
-    __syncthreads();
-    return total;
-  };
+  }
+  if (threadIdx.x == 0) {
+    W.bp[o + nblk] = carry[0];
+    W.bpn[o + nblk] = carry[1];
+    W.bpc[o + nblk] = carry[2];
+    W.npass[job] = carry[0];
+    W.stop[job] = s_stop == INT_MAX ? -1 : s_stop;
+  }
+}
 
-  for (int b = 0; b < L.nblk && !stop; b++) {
-    if (L.bvmax[b] < Y.v1l4) continue;  // RDLNFL skips the block
-    int s = L.bstart[b], e = L.bstart[b + 1];
-    int i = s + tid;
-    int fl = 0;
-    if (i < e && i <= stop_i) fl = f4[i].pass;
-    int total;
-    int pos = block_scan_excl<SHR_T>(fl, sbuf, &total);
-    if (fl) {
-      int k = nb + pos;
-      bx[k] = f4[i].x;
-      bsp[k] = f4[i].sp;
-      bal[k] = f4[i].al;
-      bad[k] = f4[i].ad;
-      bco2[k] = (int8_t)(f4[i].mol == 2);
+// per TAPE3 block: write the passing lines compacted in file order, with
+// the exclusive prefixes of positive non-CO2 / CO2 lines
+__global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer *Ys, Strides S,
+                                                      ShrinkWork W) {
+  __shared__ int buf[SB_T];
+  const int job = blockIdx.y;
+  const DevLayer &Y = Ys[job];
+  const int b = blockIdx.x;
+  if (b >= L.nblk) return;
+  int i = L.bstart[b] + threadIdx.x;
+  F4Line o{};
+  int pass = 0, pn = 0, pc = 0;
+  if (b >= Y.b4 && i < L.bstart[b + 1] && linf4_line(L, Y, i, o)) {
+    pass = 1;
+    pn = (o.sp > 0. && !o.co2);
+    pc = (o.sp > 0. && o.co2);
+  }
+  int t0, t1, t2;
+  int pos = block_scan_excl<SB_T>(pass, buf, &t0);
+  int posn = block_scan_excl<SB_T>(pn, buf, &t1);
+  int posc = block_scan_excl<SB_T>(pc, buf, &t2);
+  const int64_t ob = job * S.blk + b;
+  if (pass) {
+    int64_t k = job * S.line + W.bp[ob] + pos;
+    W.cx[k] = o.x;
+    W.csp[k] = o.sp;
+    W.cal[k] = o.al;
+    W.cad[k] = o.ad;
+    W.cco2[k] = (int8_t)o.co2;
+    W.pn[k] = W.bpn[ob] + posn;
+    W.pc[k] = W.bpc[ob] + posc;
+  }
+  if (b == L.nblk - 1 && threadIdx.x == 0) {
+    int64_t k = job * S.line + W.bp[job * S.blk + L.nblk];
+    W.pn[k] = W.bpn[job * S.blk + L.nblk];
+    W.pc[k] = W.bpc[job * S.blk + L.nblk];
+  }
+}
+
+// per passing line e: g(e), f(e) and the first doubling level
+__global__ void k_shr_chain(const DevLayer *Ys, Strides S, ShrinkWork W) {
+  const int job = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = W.npass[job];
+  if (e > n) return;
+  const int64_t o = job * S.line;
+  if (e == n) {
+    W.jmp[o + e] = n;
+    W.wsum[o + e] = 0;
+    return;
+  }
+  const double dv2 = Ys[job].dvr4 / 2.;
+  const double *x = W.cx + o;
+  double thr = x[e] + dv2;
+  int k = e + 1;
+  while (k < n && !(x[k] > thr)) k++;
+  W.g[o + e] = k - 1;
+  if (e == n - 1) {
+    W.jmp[o + e] = n;
+    W.wsum[o + e] = 0;
+    return;
+  }
+  k = e + 2;
+  while (k < n && !(x[k] > thr)) k++;
+  int f = k - 1;
+  W.jmp[o + e] = f;
+  W.wsum[o + e] = (int16_t)em_cnt(W.pn + o, W.pc + o, e + 1, f);
+}
+
+// doubling level lev from lev-1
+__global__ void k_shr_lift(int lev, Strides S, ShrinkWork W) {
+  const int job = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = W.npass[job];
+  if (e > n) return;
+  const int64_t o = job * S.line;
+  const int32_t *j0 = W.jmp + (int64_t)(lev - 1) * S.line * gridDim.y + o;
+  const int16_t *w0 = W.wsum + (int64_t)(lev - 1) * S.line * gridDim.y + o;
+  int32_t *j1 = W.jmp + (int64_t)lev * S.line * gridDim.y + o;
+  int16_t *w1 = W.wsum + (int64_t)lev * S.line * gridDim.y + o;
+  int a = j0[e];
+  j1[e] = j0[a];
+  int w = (int)w0[e] + (int)w0[a];
+  w1[e] = (int16_t)(w > 32767 ? 32767 : w);
+}
+
+// Lines emitted by SHRINK for the batch [s, t] (passing indices); with
+// *last_node the last chain node below t (or s-1 when the first group is
+// cut) and *acc_before its emitted-line count.
+__device__ int shr_query(const ShrinkWork &W, int64_t o, int64_t lstride, int s, int t, int stop_at,
+                         int *cur_out, int *acc_out) {
+  const int32_t *pn = W.pn + o, *pc = W.pc + o;
+  int e1 = W.g[o + s];
+  if (e1 >= t || e1 >= stop_at) {
+    if (cur_out) *cur_out = s - 1;
+    if (acc_out) *acc_out = 0;
+    return em_cnt(pn, pc, s, t);
+  }
+  int acc = em_cnt(pn, pc, s, e1);
+  int cur = e1;
+  for (int lev = NLEV - 1; lev >= 0; lev--) {
+    int nx = W.jmp[(int64_t)lev * lstride + o + cur];
+    if (nx < t && nx <= stop_at) {
+      acc += W.wsum[(int64_t)lev * lstride + o + cur];
+      cur = nx;
     }
-    if (stop_i >= s && stop_i < e) stop = true;
-    nb += total;
-    __syncthreads();
-    if (!stop && J + nb < 1000) continue;
-    int em = process(nb);
+  }
+  if (cur_out) *cur_out = cur;
+  if (acc_out) *acc_out = acc;
+  return acc + em_cnt(pn, pc, cur + 1, t);
+}
+
+// per (job, block b, k): lines emitted by a batch of blocks b..b+k
+__global__ void k_shr_etab(int nblk, Strides S, ShrinkWork W, int njobs) {
+  const int job = blockIdx.y;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nblk * EBLK) return;
+  const int b = id / EBLK, k = id % EBLK;
+  const int64_t ob = job * S.blk;
+  int v = 0;
+  if (b + k + 1 <= nblk) {
+    int s = W.bp[ob + b], t = W.bp[ob + b + k + 1] - 1;
+    if (t >= s) v = shr_query(W, job * S.line, S.line * njobs, s, t, INT_MAX, nullptr, nullptr);
+  }
+  W.etab[ob * EBLK + id] = v;
+}
+
+// one thread per job: LINF4's read loop over blocks (oprop.f90:3130-3290)
+// deciding batch boundaries and TAPE9 blocks from the emitted counts
+__global__ void k_shr_batches(const DevLayer *Ys, int nblk, Strides S, ShrinkWork W, Tape9 T9,
+                              int njobs) {
+  const int job = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  const int64_t ob = job * S.blk;
+  const int64_t o = job * S.line;
+  const int64_t ot = job * S.t9blk;
+  const int stop_p = W.stop[job];
+  int b = Ys[job].b4;
+  int J = 0, out = 0, nbat = 0, nt9 = 0, t9s = 0;
+  for (;;) {
+    int bs = b;
+    int raw = 0;
+    bool stop = false, eof = false;
+    for (;;) {
+      if (b >= nblk) {
+        eof = true;
+        break;
+      }
+      raw += W.bcnt[ob + b];
+      b++;
+      if (stop_p >= 0 && stop_p < W.bp[ob + b]) {
+        stop = true;
+        break;
+      }
+      if (J + raw >= LIMIN4) break;
+    }
+    int s = W.bp[ob + bs];
+    int t = stop ? stop_p : W.bp[ob + b] - 1;
+    int em = 0;
+    if (t >= s) {
+      int nb = b - bs;
+      if (!stop && nb >= 1 && nb <= EBLK) em = W.etab[(ob + bs) * EBLK + nb - 1];
+      else em = shr_query(W, o, S.line * njobs, s, t, INT_MAX, nullptr, nullptr);
+      W.bat_s[o + nbat] = s;
+      W.bat_t[o + nbat] = t;
+      W.bat_off[o + nbat] = out;
+      nbat++;
+    }
+    out += em;
     J += em;
-    out_n += em;
-    nb = 0;
-    if (J >= 1000 || stop) flush();
+    if (J >= LIMIN4 || stop || eof) {
+      if (J > 0) {
+        T9.bstart[ot + nt9] = t9s;
+        T9.bcount[ot + nt9] = J;
+        nt9++;
+      }
+      t9s = out;
+      J = 0;
+    }
+    if (stop || eof) break;
   }
-  if (!stop) {  // end of file: shrink what was read and close the block
-    int em = process(nb);
-    J += em;
-    out_n += em;
-    flush();
+  W.nbat[job] = nbat;
+  T9.n[job] = out;
+  T9.nblk[job] = nt9;
+}
+
+// per passing line: its group within its batch; the group's last line
+// writes the effective lines (strength-weighted centre and widths, sums in
+// line order as SHRINK does, oprop.f90:3560-3631)
+__global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9, int njobs) {
+  const int job = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nbat = W.nbat[job];
+  const int64_t o = job * S.line;
+  if (nbat == 0) return;
+  if (i < W.bat_s[o] || i > W.bat_t[o + nbat - 1]) return;
+  int lo = 0, hi = nbat - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (W.bat_s[o + mid] <= i) lo = mid;
+    else hi = mid - 1;
   }
-  if (tid == 0) {
-    *T9.n = out_n;
-    *T9.nblk = t9nb;
+  const int s = W.bat_s[o + lo], t = W.bat_t[o + lo], off = W.bat_off[o + lo];
+  if (i > t) return;
+  const int64_t ls = S.line * njobs;
+  int gs, ge, acc;
+  int e1 = W.g[o + s];
+  if (e1 >= t) {
+    gs = s;
+    ge = t;
+    acc = 0;
+  } else if (i <= e1) {
+    gs = s;
+    ge = e1;
+    acc = 0;
+  } else {
+    int cur = e1;
+    acc = em_cnt(W.pn + o, W.pc + o, s, e1);
+    for (int lev = NLEV - 1; lev >= 0; lev--) {
+      int nx = W.jmp[(int64_t)lev * ls + o + cur];
+      if (nx <= i - 1) {
+        acc += W.wsum[(int64_t)lev * ls + o + cur];
+        cur = nx;
+      }
+    }
+    gs = cur + 1;
+    ge = W.jmp[o + cur];
+    if (ge > t) ge = t;
   }
+  if (ge != i) return;
+  double sumv = 0., sums = 0., sumal = 0., sumad = 0.;
+  double sumv2 = 0., sums2 = 0., sumal2 = 0., sumad2 = 0.;
+  for (int k = gs; k <= ge; k++) {
+    double sp = W.csp[o + k];
+    if (!W.cco2[o + k]) {
+      sumv = sumv + W.cx[o + k] * sp;
+      sums = sums + sp;
+      sumal = sumal + sp * W.cal[o + k];
+      sumad = sumad + sp * W.cad[o + k];
+    } else {
+      sumv2 = sumv2 + W.cx[o + k] * sp;
+      sums2 = sums2 + sp;
+      sumal2 = sumal2 + sp * W.cal[o + k];
+      sumad2 = sumad2 + sp * W.cad[o + k];
+    }
+  }
+  int64_t q = o + off + acc;
+  if (sums > 0.) {
+    T9.vnu[q] = sumv / sums;
+    T9.sp[q] = sums;
+    T9.alfa[q] = sumal / sums;
+    T9.epp[q] = sumad / sums;
+    T9.mol[q] = 0;
+    q++;
+  }
+  if (sums2 > 0.) {
+    T9.vnu[q] = sumv2 / sums2;
+    T9.sp[q] = sums2;
+    T9.alfa[q] = sumal2 / sums2;
+    T9.epp[q] = sumad2 / sums2;
+    T9.mol[q] = 2;
+  }
+}
+
+// VMAX of each TAPE9 block header: centre of its last line (oprop.f90:3272-3277)
+__global__ void k_t9_vhi(Strides S, Tape9 T9) {
+  const int job = blockIdx.y;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= T9.nblk[job]) return;
+  const int64_t ot = job * S.t9blk;
+  T9.bvhi[ot + b] = T9.vnu[job * S.line + T9.bstart[ot + b] + T9.bcount[ot + b] - 1];
 }
 
 // ---------------------------------------------------------------------
@@ -337,14 +507,18 @@ __device__ __forceinline__ double f4_at(const F4Par &p, double xjj, double r_bnd
   return p.sil / (p.alfli2 + xmsq) - sub;
 }
 
-__global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const DevWin *wins,
-                                                  const DevSeg *segs, double *r4out,
-                                                  unsigned long long *k3upd) {
-  __shared__ double r4[2504];
+__global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S,
+                                                  const DevWin *wins, const DevSeg *segs,
+                                                  double *r4out, unsigned long long *k3upd) {
+  __shared__ double r4[R4ROW];
   __shared__ F4Par par[64];
   __shared__ int s_nacc, s_done, s_b;
   const int tid = threadIdx.x;
   const DevWin W = wins[blockIdx.x];
+  const int job = W.job;
+  const DevLayer &Y = Ys[job];
+  const int64_t o = job * S.line;
+  const int64_t ot = job * S.t9blk;
   const double dvr4 = Y.dvr4, bound4 = Y.bound4;
   const double recpi = 1. / (2. * asin(1.));
   const double zsqbnd = 64. * 64.;
@@ -355,18 +529,19 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
   const double vnulst = W.v2r4 + bound4;
   const double vlo = W.v1r4 - bound4;
   const int nptr4 = W.nptr4;
-  const int nl = *T9.n;
-  const int nb9 = *T9.nblk;
-  for (int j = tid; j < 2504; j += CF4_T) r4[j] = 0.;
+  const int nl = T9.n[job];
+  const int nb9 = T9.nblk[job];
+  for (int j = tid; j < R4ROW; j += CF4_T) r4[j] = 0.;
   if (tid == 0) {
     s_b = nb9;
     s_done = 0;
   }
   __syncthreads();
   for (int b = tid; b < nb9; b += CF4_T)
-    if (T9.bvhi[b] >= vlo) atomicMin(&s_b, b);
+    if (T9.bvhi[ot + b] >= vlo) atomicMin(&s_b, b);
   __syncthreads();
-  int pos = (s_b < nb9) ? T9.bstart[s_b] : nl;
+  int pos = (s_b < nb9) ? T9.bstart[ot + s_b] : nl;
+  unsigned long long nupd = 0;
   while (pos < nl) {
     if (tid < 64) {
       const int lane = tid;
@@ -376,13 +551,14 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
       p.co2 = 0;
       p.jmin = 1;
       p.jmax = 0;
+      p.xnui = 0.;
       double speak = 0., partial = 0., vnui = 0.;
       bool nz = false;
       int jj = 1;
       if (valid) {
-        double sp = T9.sp[i];
+        double sp = T9.sp[o + i];
         nz = sp != 0.;
-        double alfadi = T9.epp[i], alfali = T9.alfa[i];
+        double alfadi = T9.epp[o + i], alfali = T9.alfa[o + i];
         double zetai = alfali / (alfali + alfadi);
         int iz = fint(100. * zetai + ONEPL_D);
         iz = iz < 1 ? 1 : (iz > 101 ? 101 : iz);  // zeta in [0,1] keeps iz in range
@@ -392,9 +568,9 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
         p.sil = sp * recpi * alfali;
         double siv = (alfali * ralfvi) * sp * recpi * ralfvi;
         speak = A3 * fabs(siv);
-        jj = fint((T9.vnu[i] - W.v1r4) / dvr4 + 1.);
+        vnui = T9.vnu[o + i];
+        jj = fint((vnui - W.v1r4) / dvr4 + 1.);
         jj = jj < 1 ? 1 : (jj > nptr4 ? nptr4 : jj);
-        vnui = T9.vnu[i];
         p.xnui = vnui - W.v1r4;
         int jmin = fint((p.xnui - bound4) / dvr4 + 2.);
         int jmax = fint((p.xnui + bound4) / dvr4 + 1.);
@@ -406,7 +582,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
         p.rec_alfvi2 = 1. / (alfavi * alfavi);
         p.siv_a3 = siv * A3;
         p.siv_b3 = siv * B3;
-        p.co2 = T9.mol[i] == 2;
+        p.co2 = T9.mol[o + i] == 2;
         partial = r4[jj];
       }
       const double xjj_me = (double)(jj - 1) * dvr4;
@@ -437,7 +613,6 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
           if (lane == j && p.jmax >= p.jmin) par[nacc] = p;
           if (q.jmax >= q.jmin) nacc++;
         }
-        // lanes at or below j are final; a rejected lane j simply adds nothing
       }
       if (lane == 0) {
         s_nacc = nacc;
@@ -446,14 +621,9 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
     }
     __syncthreads();
     const int nacc = s_nacc;
-    if (tid == 0) {
-      unsigned long long u = 0;
-      for (int a = 0; a < nacc; a++) u += (unsigned long long)(par[a].jmax - par[a].jmin + 1);
-      atomicAdd(k3upd, u);
-    }
     for (int a = 0; a < nacc; a++) {
       F4Par q = par[a];
-      // owned points j = 1 + tid + k*CF4_T
+      if (tid == 0) nupd += (unsigned long long)(q.jmax - q.jmin + 1);
       int j0 = q.jmin - 1;
       int r = j0 % CF4_T;
       int first = j0 - r + tid + (tid < r ? CF4_T : 0) + 1;
@@ -466,6 +636,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
     if (s_done) break;
     pos += 64;
   }
+  if (tid == 0 && nupd) atomicAdd(&k3upd[job], nupd);
   __syncthreads();
   // JRAD = 1: radiation term as a piecewise-linear factor (oprop.f90:3733-3758)
   if (W.nseg > 0) {
@@ -481,8 +652,8 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
     }
     __syncthreads();
   }
-  double *row = r4out + (size_t)blockIdx.x * 2504;
-  for (int j = tid; j < 2504; j += CF4_T) row[j] = (j >= 1 && j <= nptr4) ? r4[j] : 0.;
+  double *row = r4out + (size_t)blockIdx.x * R4ROW;
+  for (int j = tid; j < R4ROW; j += CF4_T) row[j] = (j >= 1 && j <= nptr4) ? r4[j] : 0.;
 }
 
 // ---------------------------------------------------------------------
@@ -490,11 +661,14 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, DevLayer Y, const De
 // that depend on the panel at which the block is read (the VFT test and
 // the R4 rejection); those are applied in k_reject.  Also computes the
 // first panel whose R1 buffer can hold the line (CNVFNV JMAX1 <= MAX1).
-__global__ void k_lncor1(DevLines L, DevLayer Y, const DevPanel *pan, int nlo, int nn,
+__global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPanel *pans,
                          NearLine *out) {
+  const int job = blockIdx.y;
+  const DevLayer &Y = Ys[job];
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nn) return;
-  int i = nlo + t;
+  if (t >= Y.nn) return;
+  const DevPanel *pan = pans + Y.pan0;
+  int i = Y.nlo + t;
   NearLine o;
   o.vnu = L.vnu[i];
   o.recalf = 0.;
@@ -554,50 +728,57 @@ __global__ void k_lncor1(DevLines L, DevLayer Y, const DevPanel *pan, int nlo, i
       o.ff = p;  // == npanel: never processed before the last panel
     }
   }
-  out[t] = o;
+  out[job * S.line + t] = o;
 }
 
 // One step of the block-read fixpoint: with the current guess of the panel
 // at which each block is read, apply LNCOR1's VFT test and R4 rejection
 // (oprop.f90:1120, 1153-1177) and reduce the first-fit panel of the kept
 // lines per block.
-__global__ void k_reject(const NearLine *nl, int nn, int nlo, const int32_t *blk, int blk0,
-                         const int *pread, const DevPanel *pan, const DevWin *wins,
-                         const double *r4, DevLayer Y, int *keep_out, int *blkmax) {
+__global__ void k_reject(const NearLine *nl, DevLines L, const DevLayer *Ys, Strides S,
+                         const int *pread, const DevPanel *pans, const DevWin *wins,
+                         const double *r4, int *keep_out, int *blkmax) {
+  const int job = blockIdx.y;
+  const DevLayer &Y = Ys[job];
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nn) return;
-  NearLine o = nl[t];
+  if (t >= Y.nn) return;
+  NearLine o = nl[job * S.line + t];
   int keep = o.keep;
-  int k = blk[nlo + t] - blk0;
+  int k = L.blk[Y.nlo + t] - Y.blk0;
   if (keep) {
-    int pr = pread[k];
+    int pr = pread[job * S.blk + k];
     int pq = pr < Y.npanel ? pr : Y.npanel - 1;
-    double vft = pan[pq].vft;
+    const DevPanel &pp = pans[Y.pan0 + pq];
     double alfv = 1. / o.recalf;
-    if (64. * alfv + o.vnu < vft) keep = 0;
+    if (64. * alfv + o.vnu < pp.vft) keep = 0;
     if (keep) {
-      DevWin W = wins[pan[pq].w];
+      int w = Y.win0 + pp.w;
+      DevWin W = wins[w];
       int jj = fint((o.vnu - W.v1r4) / Y.dvr4 + 1.);
       jj = jj < 1 ? 1 : (jj > W.nptr4 ? W.nptr4 : jj);
-      double r = r4[(size_t)pan[pq].w * 2504 + jj];
+      double r = r4[(size_t)w * R4ROW + jj];
       if (o.speak <= (Y.dptmn_hirac + Y.dptfc * r)) keep = 0;
     }
   }
-  keep_out[t] = keep;
-  if (keep) atomicMax(&blkmax[k], o.ff);
+  keep_out[job * S.line + t] = keep;
+  if (keep) atomicMax(&blkmax[job * S.blk + k], o.ff);
 }
 
 // Exclusive prefix max over blocks: panel at which each block is read.
-// Sets *changed when the guess moved; resets blkmax for the next pass.
-__global__ __launch_bounds__(256) void k_pread_scan(int *blkmax, int nbk, int *pread, int *changed) {
+// Raises *changed when the guess moved; resets blkmax for the next pass.
+__global__ __launch_bounds__(256) void k_pread_scan(const DevLayer *Ys, Strides S, int *blkmax,
+                                                    int *pread, int *changed) {
   __shared__ int buf[256];
   __shared__ int carry;
+  const int job = blockIdx.x;
+  const int nbk = Ys[job].nbk;
+  const int64_t ob = job * S.blk;
   int tid = threadIdx.x;
   if (tid == 0) carry = 0;
   __syncthreads();
   for (int base = 0; base < nbk; base += 256) {
     int k = base + tid;
-    int v = (k < nbk) ? blkmax[k] : 0;
+    int v = (k < nbk) ? blkmax[ob + k] : 0;
     buf[tid] = v;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
@@ -608,9 +789,9 @@ __global__ __launch_bounds__(256) void k_pread_scan(int *blkmax, int nbk, int *p
     }
     int excl = max(carry, tid > 0 ? buf[tid - 1] : 0);
     if (k < nbk) {
-      if (pread[k] != excl) *changed = 1;
-      pread[k] = excl;
-      blkmax[k] = 0;
+      if (pread[ob + k] != excl) *changed = 1;
+      pread[ob + k] = excl;
+      blkmax[ob + k] = 0;
     }
     __syncthreads();
     if (tid == 0) carry = max(carry, buf[255]);
@@ -622,24 +803,26 @@ __global__ __launch_bounds__(256) void k_pread_scan(int *blkmax, int nbk, int *p
 // raised by the first-fit panels of kept lines before it in the block
 // (CNVFNV stops at the first line that does not fit, oprop.f90:1287-1292).
 // Then the scatter record of CNVFNV (oprop.f90:1262-1305).
-__global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const int *keep, int nlo,
-                                                     DevLines L, int blk0, int nbk, const int *pread,
-                                                     const DevPanel *pan, DevLayer Y, int nhi_t,
-                                                     ScatterRec *rec) {
+__global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const int *keep, DevLines L,
+                                                     const DevLayer *Ys, Strides S, const int *pread,
+                                                     const DevPanel *pans, ScatterRec *recs) {
   __shared__ int buf[256];
-  int k = blockIdx.x;  // near block
-  int b = blk0 + k;
-  int s = L.bstart[b] - nlo, e = L.bstart[b + 1] - nlo;
+  const int job = blockIdx.y;
+  const DevLayer &Y = Ys[job];
+  const int k = blockIdx.x;  // near block
+  if (k >= Y.nbk) return;
+  const int b = Y.blk0 + k;
+  int s = L.bstart[b] - Y.nlo, e = L.bstart[b + 1] - Y.nlo;
   if (s < 0) s = 0;
-  if (e > nhi_t) e = nhi_t;
-  int tid = threadIdx.x;
-  int t = s + tid;
-  bool valid = t < e;
-  int kp = valid ? keep[t] : 0;
-  NearLine o;
-  if (valid) o = nl[t];
-  int v = (valid && kp) ? o.ff : 0;
-  buf[tid] = v;
+  if (e > Y.nn) e = Y.nn;
+  const int tid = threadIdx.x;
+  const int t = s + tid;
+  const bool valid = t < e;
+  const int64_t o = job * S.line;
+  int kp = valid ? keep[o + t] : 0;
+  NearLine n;
+  if (valid) n = nl[o + t];
+  buf[tid] = (valid && kp) ? n.ff : 0;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
     int q = tid >= off ? buf[tid - off] : 0;
@@ -655,19 +838,19 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const i
   r.s2 = r.s3 = 0;
   r.zfl1 = r.zfl2 = r.zfl3 = r.zslope = 0.;
   r.strf1 = r.strf2 = r.strf3 = r.strd = r.strver = 0.;
-  int P = max(pread[k], buf[tid]);
+  int P = max(pread[job * S.blk + k], buf[tid]);
   if (kp && P < Y.npanel) {
-    double vft = pan[P].vft;
-    double depthi = o.sp * o.recalf;
-    int izm = o.izeta;
-    double zetdif = 100. * o.zetai - (double)(izm - 1);
+    double vft = pans[Y.pan0 + P].vft;
+    double depthi = n.sp * n.recalf;
+    int izm = n.izeta;
+    double zetdif = 100. * n.zetai - (double)(izm - 1);
     r.strf1 = depthi * (c_cf1[izm - 1] + zetdif * (c_cf1[izm] - c_cf1[izm - 1]));
     r.strf2 = depthi * (c_cf2[izm - 1] + zetdif * (c_cf2[izm] - c_cf2[izm - 1]));
     r.strf3 = depthi * (c_cf3[izm - 1] + zetdif * (c_cf3[izm] - c_cf3[izm - 1]));
     r.strd = depthi * (c_cgauss[izm - 1] + zetdif * (c_cgauss[izm] - c_cgauss[izm - 1]));
     r.strver = depthi * (c_cer[izm - 1] + zetdif * (c_cer[izm] - c_cer[izm - 1]));
-    double zslope = o.recalf * (Y.dv / 0.002);
-    double zint = (o.vnu - vft) / Y.dv;
+    double zslope = n.recalf * (Y.dv / 0.002);
+    double zint = (n.vnu - vft) / Y.dv;
     double bhwdxf = (4. / 0.002) / zslope;
     int jmax1 = fint(zint + bhwdxf + 1.5);
     int jmin1 = fint(zint - bhwdxf + 1.5);
@@ -685,47 +868,56 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const i
     r.s3 = (int64_t)j3shft + (int64_t)2250 * P;
     r.npts = jmax1 - jmin1 + 1;
   }
-  rec[t] = r;
+  recs[o + t] = r;
 }
 
-__global__ void k_fill_i32(int *p, int n, int v) {
+// Near-wing point updates per job (sum of JMAX1-JMIN1+1 over kept lines).
+__global__ void k_sum_npts(const DevLayer *Ys, Strides S, const ScatterRec *rec,
+                           unsigned long long *acc) {
+  const int job = blockIdx.y;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-// Near-wing point updates of one grid (sum of JMAX1-JMIN1+1 over kept lines).
-__global__ void k_sum_npts(const ScatterRec *rec, int n, unsigned long long *acc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long v = (i < n && rec[i].npts > 0) ? (unsigned long long)rec[i].npts : 0ull;
+  unsigned long long v = 0ull;
+  if (i < Ys[job].nn && rec[job * S.line + i].npts > 0) v = (unsigned long long)rec[job * S.line + i].npts;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc, v);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&acc[job], v);
 }
+
+// Tile table of one grid level: tile -> (job, local tile).
+struct TileMap {
+  const int32_t *job, *loc;
+  int32_t *lo, *hi;  // line range per tile
+};
 
 // Line range [lo, hi) whose scatter extent may touch each tile of grid G.
-__global__ void k_tile_ranges(const double *vraw, int nn, double vft0, double dvg, int tile,
-                              int ntiles, double margin, int *lo, int *hi) {
+__global__ void k_tile_ranges(const double *vraw, const DevLayer *Ys, TileMap M, int ntiles,
+                              int tile, int G, const double *margin) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  double a = vft0 + (double)t * tile * dvg - margin;
-  double b = vft0 + (double)(t + 1) * tile * dvg + margin;
+  const int job = M.job[t];
+  const DevLayer &Y = Ys[job];
+  const double dvg = G == 1 ? Y.dv : (G == 2 ? Y.dvr2 : Y.dvr3);
+  const double *v = vraw + Y.nlo;
+  const int nn = Y.nn;
+  double a = Y.vft0 + (double)M.loc[t] * tile * dvg - margin[job * 3 + G - 1];
+  double b = Y.vft0 + (double)(M.loc[t] + 1) * tile * dvg + margin[job * 3 + G - 1];
   int l = 0, h = nn;
   while (l < h) {
     int m = (l + h) >> 1;
-    if (vraw[m] < a) l = m + 1;
+    if (v[m] < a) l = m + 1;
     else h = m;
   }
   int l2 = l, h2 = nn;
   while (l2 < h2) {
     int m = (l2 + h2) >> 1;
-    if (vraw[m] <= b) l2 = m + 1;
+    if (v[m] <= b) l2 = m + 1;
     else h2 = m;
   }
-  lo[t] = l;
-  hi[t] = l2;
+  M.lo[t] = l;
+  M.hi[t] = l2;
 }
 
-// XINT of a coarse array into R3 point G3 (oprop.f90:2219-2230) for the
-// panel that owns G3.
+// XINT of a coarse array into R3 point I of the panel that owns it
+// (oprop.f90:2219-2230); A is the 1-based coarse array.
 __device__ __forceinline__ double xint_at(double v1a, double v2a, double dva, const double *A,
                                           double vft, double dvr3, int I, int n1r3, int n2r3) {
   double recdva = 1. / dva;
@@ -747,14 +939,14 @@ __device__ __forceinline__ double xint_at(double v1a, double v2a, double dva, co
 
 // Near-line scatter into one tile of grid G (3: DVR3, 2: DVR2, 1: DV) with
 // the PANEL combine in the epilogue.
-//   G = 3: R3 = lines(F3) + XINT(R4) + XINT(ABSRB)           -> r3 (global)
-//   G = 2: R2 = lines(F2) + Lagrange4(R3)                      -> r2 (global)
+//   G = 3: R3 = lines(F3) + XINT(R4) + XINT(ABSRB)            -> r3
+//   G = 2: R2 = lines(F2) + Lagrange4(R3)                       -> r2
 //   G = 1: OD = (lines(F1,FG,XVER) + Lagrange4(R2)) * radiation -> od
 template <int G, typename OUT>
-__global__ __launch_bounds__(256) void k_tile(const ScatterRec *rec, const int *tlo, const int *thi,
-                                              int tile, const double *tabs, DevLayer Y,
-                                              const DevPanel *pan, const DevWin *wins,
-                                              const double *r4, const double *absrb1,
+__global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M, int tile,
+                                              const double *tabs, const DevLayer *Ys, Strides S,
+                                              const DevPanel *pans, const DevWin *wins,
+                                              const double *r4, const double *absrb,
                                               const DevSeg *segs, const double *rin, double *rout,
                                               OUT *od) {
   extern __shared__ double sm[];
@@ -763,6 +955,9 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *rec, const int *
   double *tg = t1 + NFMX;           // FG   (G = 1)
   double *tv = tg + NFMX;           // XVER (G = 1)
   const int tid = threadIdx.x;
+  const int job = M.job[blockIdx.x];
+  const DevLayer &Y = Ys[job];
+  const ScatterRec *rec = recs + job * S.line;
   for (int k = tid; k < tile; k += 256) acc[k] = 0.;
   const double *src = tabs + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NFMX;
   for (int k = tid; k < NFMX; k += 256) t1[k] = src[k];
@@ -773,8 +968,8 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *rec, const int *
     }
   }
   __syncthreads();
-  const int64_t T0 = (int64_t)blockIdx.x * tile;
-  const int lo = tlo[blockIdx.x], hi = thi[blockIdx.x];
+  const int64_t T0 = (int64_t)M.loc[blockIdx.x] * tile;
+  const int lo = M.lo[blockIdx.x], hi = M.hi[blockIdx.x];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   for (int li = lo + wave; li < hi; li += 4) {
@@ -800,6 +995,7 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *rec, const int *
   __syncthreads();
   const double X00 = -7. / 128., X01 = 105. / 128., X02 = 35. / 128., X03 = -5. / 128.;
   const double X10 = -1. / 16., X11 = 9. / 16.;
+  const DevPanel *pan = pans + Y.pan0;
   for (int k = tid; k < tile; k += 256) {
     int64_t g = T0 + k;
     if (G == 3) {
@@ -811,32 +1007,35 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *rec, const int *
         int I = (int)(g - (int64_t)150 * p) + 1;
         int n1r3 = p == 0 ? 1 : 6;
         DevPanel pp = pan[p];
-        DevWin W = wins[pp.w];
-        v = v + xint_at(W.v1r4, W.v2r4, Y.dvr4, r4 + (size_t)pp.w * 2504, pp.vft, Y.dvr3, I, n1r3,
+        int w = Y.win0 + pp.w;
+        DevWin W = wins[w];
+        v = v + xint_at(W.v1r4, W.v2r4, Y.dvr4, r4 + (size_t)w * R4ROW, pp.vft, Y.dvr3, I, n1r3,
                         155) * 1.0;
         if (Y.icntnm >= 1)
-          v = v + xint_at(Y.v1abs, Y.v2abs, Y.dvabs, absrb1, pp.vft, Y.dvr3, I, n1r3, 155) * 1.;
+          v = v + xint_at(Y.v1abs, Y.v2abs, Y.dvabs, absrb + Y.abs0, pp.vft, Y.dvr3, I, n1r3, 155) * 1.;
       }
-      rout[g] = v;
+      rout[Y.r3o + g] = v;
     } else if (G == 2) {
       if (g >= Y.n2) continue;
       double v = acc[k];
       if (g >= 4 && g <= Y.g2max) {
+        const double *r3 = rin + Y.r3o;
         int64_t q = g >> 2;
         int ph = (int)(g & 3);
-        double a0 = rin[q - 1], a1 = rin[q], a2 = rin[q + 1], a3 = rin[q + 2];
+        double a0 = r3[q - 1], a1 = r3[q], a2 = r3[q + 1], a3 = r3[q + 2];
         if (ph == 0) v = v + a1;
         else if (ph == 1) v = v + X00 * a0 + X01 * a1 + X02 * a2 + X03 * a3;
         else if (ph == 2) v = v + X10 * (a0 + a3) + X11 * (a1 + a2);
         else v = v + X03 * a0 + X02 * a1 + X01 * a2 + X00 * a3;
       }
-      rout[g] = v;
+      rout[Y.r2o + g] = v;
     } else {
       if (g < 32 || g > Y.g_last) continue;
+      const double *r2 = rin + Y.r2o;
       double v = acc[k];
       int64_t q = g >> 2;
       int ph = (int)(g & 3);
-      double a0 = rin[q - 1], a1 = rin[q], a2 = rin[q + 1], a3 = rin[q + 2];
+      double a0 = r2[q - 1], a1 = r2[q], a2 = r2[q + 1], a3 = r2[q + 2];
       if (ph == 0) v = v + a1;
       else if (ph == 1) v = v + X00 * a0 + X01 * a1 + X02 * a2 + X03 * a3;
       else if (ph == 2) v = v + X10 * (a0 + a3) + X11 * (a1 + a2);
@@ -849,13 +1048,13 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *rec, const int *
         int lo2 = 0, hi2 = pp.nseg - 1;
         while (lo2 < hi2) {
           int mid = (lo2 + hi2 + 1) >> 1;
-          if (segs[pp.seg0 + mid].i1 <= I) lo2 = mid;
+          if (segs[Y.seg0 + pp.seg0 + mid].i1 <= I) lo2 = mid;
           else hi2 = mid - 1;
         }
-        DevSeg sg = segs[pp.seg0 + lo2];
+        DevSeg sg = segs[Y.seg0 + pp.seg0 + lo2];
         v = v * (sg.radvi + (double)(I - sg.i1) * sg.rdel);
       }
-      od[g - 32] = (OUT)v;
+      od[Y.od0 + g - 32] = (OUT)v;
     }
   }
 }

@@ -16,6 +16,7 @@ from . import abi
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "liblblx.so")
 _LIB = None
+NSTAGE = 12
 
 
 class LblxError(RuntimeError):
@@ -50,7 +51,19 @@ def load():
         "lblx_od_layer_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_void_p,
                                            C.c_int64, P(C.c_double), P(abi.Stats)]),
         "lblx_sync": (C.c_int, [C.c_void_p]),
+        "lblx_od_column_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int,
+                                            C.c_void_p, P(C.c_int64), P(C.c_double),
+                                            P(abi.Stats)]),
+        "lblx_debug_tape9": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double),
+                                       P(C.c_double), P(C.c_double), P(C.c_int32), C.c_int,
+                                       P(C.c_int32)]),
+        "lblx_timing": (C.c_int, [C.c_void_p, C.c_int]),
+        "lblx_timing_get": (C.c_int, [C.c_void_p, C.c_int, P(C.c_char_p), P(C.c_double),
+                                      P(C.c_int64)]),
         "lblx_stream": (C.c_void_p, [C.c_void_p]),
+        "lblx_dev_alloc": (C.c_void_p, [C.c_void_p, C.c_int64]),
+        "lblx_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "lblx_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -146,8 +159,94 @@ class Context:
              "lblx_od_layer_device")
         return v1p.value, (st.as_dict() if stats else None)
 
+    def opdpth_column_device(self, run, layers, dptr, offsets, stats=False):
+        """OPDPTH for all layers of a column in one batch; layer k's optical
+        depth at element offsets[k] of the device buffer dptr.  The radiation
+        flag is chained from layer to layer as XLAYER does.  Returns
+        (v1p per layer, stats per layer or None)."""
+        n = len(layers)
+        arr = (abi.Layer * n)(*layers)
+        off = np.ascontiguousarray(offsets, np.int64)
+        v1p = (C.c_double * n)()
+        st = (abi.Stats * n)() if stats else None
+        _chk(self._lib.lblx_od_column_device(self._h, C.byref(run), arr, n, C.c_void_p(dptr),
+                                             abi.ptr(off, C.c_int64), v1p, st),
+             "lblx_od_column_device")
+        for k in range(n):
+            layers[k].jrad = arr[k].jrad
+        return list(v1p), ([x.as_dict() for x in st] if stats else None)
+
+    def debug_tape9(self, job=0, cap=2000000):
+        vnu = np.zeros(cap)
+        sp = np.zeros(cap)
+        al = np.zeros(cap)
+        ad = np.zeros(cap)
+        mol = np.zeros(cap, np.int32)
+        n = C.c_int32(0)
+        _chk(self._lib.lblx_debug_tape9(self._h, job, abi.ptr(vnu, C.c_double), abi.ptr(sp, C.c_double),
+                                        abi.ptr(al, C.c_double), abi.ptr(ad, C.c_double),
+                                        abi.ptr(mol, C.c_int32), cap, C.byref(n)), "lblx_debug_tape9")
+        k = min(n.value, cap)
+        return vnu[:k], sp[:k], al[:k], ad[:k], mol[:k]
+
+    def timing(self, enable=True):
+        _chk(self._lib.lblx_timing(self._h, 1 if enable else 0), "lblx_timing")
+
+    def timings(self):
+        """{stage: (ms_total, launches)} since timing was enabled."""
+        out = {}
+        for i in range(NSTAGE):
+            nm = C.c_char_p()
+            ms = C.c_double()
+            cnt = C.c_int64()
+            _chk(self._lib.lblx_timing_get(self._h, i, C.byref(nm), C.byref(ms), C.byref(cnt)),
+                 "lblx_timing_get")
+            out[nm.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def alloc(self, nbytes):
+        """Device buffer (DeviceBuffer) of nbytes on the context's device."""
+        return DeviceBuffer(self, nbytes)
+
     def sync(self):
         _chk(self._lib.lblx_sync(self._h), "lblx_sync")
 
     def stream(self):
         return self._lib.lblx_stream(self._h)
+
+
+class DeviceBuffer:
+    """Device memory owned through the library (no second HIP runtime)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctx._lib.lblx_dev_alloc(ctx._h, self.nbytes)
+        if not p:
+            raise RuntimeError("lblx_dev_alloc(%d) failed" % self.nbytes)
+        self.ptr = p
+
+    def to_host(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        n = self.nbytes // dtype.itemsize if count is None else int(count)
+        out = np.empty(n, dtype)
+        _chk(self.ctx._lib.lblx_copy(self.ctx._h, out.ctypes.data_as(C.c_void_p),
+                                     C.c_void_p(self.ptr), n * dtype.itemsize, 1), "lblx_copy")
+        return out
+
+    def from_host(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _chk(self.ctx._lib.lblx_copy(self.ctx._h, C.c_void_p(self.ptr),
+                                     arr.ctypes.data_as(C.c_void_p), arr.nbytes, 0), "lblx_copy")
+
+    def free(self):
+        if self.ptr:
+            self.ctx._lib.lblx_dev_free(self.ctx._h, C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
