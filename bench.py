@@ -1,0 +1,146 @@
+"""Throughput of the MI355X monochromatic path on the 50-layer column.
+
+Workload (one step = one spectrum): the 50-layer atmosphere, 500,000
+seeded synthetic lines over 500-2500 cm-1 in 7 molecules, every layer's
+optical depth (OPDPTH: LINF4, SHRINK, LBLF4 far wing, HIRAC1 near wing and
+panel combine) on the layer DV grids of PATH, FP64.  Inputs (line table,
+layer parameters) are resident on the device before the timed region;
+per-layer optical depths stay in HBM.
+
+Prints one JSON line (the bench contract) with the dominant kernel's
+roofline, measured with the library's HIP-event stage timers over the
+timed steps, and the CPU oracle timed on this host on the same spectrum.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "spectra/s FP32-mixed vs plain FP64 at ≤1e-4 rel; Jacobian/forward time"
+HBM_PEAK_GBS = 8000.0
+REC_BYTES = 120      # ScatterRec
+T9_BYTES = 36        # shrunk line: centre, strength, two widths, molecule
+LINE_BYTES = 36      # TAPE3 line read per layer (centre f64, six f32, mol)
+
+
+def stage_bytes(stage, stats, sizes, nwin):
+    """Algorithmic HBM bytes of one step for a stage (sum over layers)."""
+    npts = sum(sizes)
+    if stage == "tile_r1":
+        # write OD, read R2 (npts/4) and the scatter record of every kept line
+        return 8 * npts + 8 * npts / 4 + REC_BYTES * sum(s["lincnt"] for s in stats)
+    if stage == "tile_r2":
+        return 8 * npts / 4 + 8 * npts / 16 + REC_BYTES * sum(s["lincnt"] for s in stats)
+    if stage == "tile_r3":
+        return 8 * npts / 16 + REC_BYTES * sum(s["lincnt"] for s in stats) + 2504 * 8 * nwin
+    if stage == "convf4":
+        return T9_BYTES * sum(s["nlins4"] for s in stats) + 2504 * 8 * nwin
+    if stage == "shrink":
+        return LINE_BYTES * 500000 * len(sizes) + T9_BYTES * sum(s["nlins4"] for s in stats)
+    return LINE_BYTES * 500000 * len(sizes)
+
+
+def cpu_baseline(run, layers, tab):
+    """The oracle (single-threaded C restatement of OPDPTH) on the whole
+    spectrum: every layer of the column, one core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from lblrtm_amd import abi
+    t0 = time.perf_counter()
+    jrad = 1
+    for L in layers:
+        lay = abi.Layer.from_buffer_copy(L)
+        lay.jrad = jrad
+        _, _, _, jrad = O.od_layer(run, lay, tab)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "spectra/s", "cores": 1, "kind": "port",
+            "sample": "one full spectrum (all 50 layers, 500k lines) with the C oracle on one "
+                      "host core, %.1f s" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
+                                   "(see DESIGN.md for how the path would shard)" % a.gpus}))
+        sys.exit(2)
+    from lblrtm_amd import abi, lib, synth
+
+    tab = synth.config1_lines()
+    atm = synth.atmosphere()
+    run = abi.make_run(500.0, 2500.0)
+    dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    nlay = len(dv)
+    layers = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]),
+                             atm["wbroad"][l], dv[l], 500.0, 2500.0) for l in range(nlay)]
+    sizes = [lib.od_size(run, L) for L in layers]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+
+    ctx = lib.Context(0)
+    ctx.load_lines(tab)
+    buf = ctx.alloc(8 * int(sum(sizes)))
+
+    def step(stats=False):
+        lays = [abi.Layer.from_buffer_copy(x) for x in layers]
+        return ctx.opdpth_column_device(run, lays, buf.ptr, offs, stats=stats)
+
+    _, stats = step(stats=True)
+    for _ in range(a.warmup):
+        step()
+    ctx.sync()
+    ctx.timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    ctx.sync()
+    t1 = time.perf_counter()
+    tim = ctx.timings()
+    ctx.timing(False)
+    ms = (t1 - t0) * 1000.0 / a.steps
+    per = {k: v[0] / a.steps for k, v in tim.items() if v[1] > 0}
+    dom = max(per, key=per.get)
+    nwin = sum(s["nwin4"] for s in stats)
+    nbytes = stage_bytes(dom, stats, sizes, nwin)
+    achieved = nbytes / (per[dom] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms,
+        "unit": "spectra/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "50-layer column optical depth, 500k lines 500-2500 cm-1, 7 "
+                               "molecules, LBLF4 panels, FP64 (continuum and merge not in this step)",
+                   "layers": nlay, "lines": 500000, "points": int(sum(sizes)),
+                   "stage_ms": {k: round(v, 4) for k, v in per.items()}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(run, layers, tab)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -110,6 +110,7 @@ typedef struct lblx_stats {
   int64_t npts;             /* output points                               */
   int64_t k1_updates;       /* near-wing (R1,R2,R3) point updates          */
   int64_t k3_updates;       /* far-wing R4 point updates                   */
+  int64_t nwin4;            /* far-wing (LBLF4) windows computed           */
 } lblx_stats;
 
 /* ------------------------------------------------------------------ */
@@ -180,7 +181,9 @@ int lblx_debug_tape9(lblx_ctx *ctx, int job, double *vnu, double *sp, double *al
 
 /* Device memory of the context's device for callers without their own
  * allocator; lblx_copy kind: 0 host->device, 1 device->host, 2 device->
- * device (ordered on the context stream, returns when complete).          */
+ * device, 3 device->device through a plain copy kernel (the reference
+ * kernel for HBM traffic counters); ordered on the context stream,
+ * returns when complete.                                                  */
 void *lblx_dev_alloc(lblx_ctx *ctx, int64_t bytes);
 int lblx_dev_free(lblx_ctx *ctx, void *p);
 int lblx_copy(lblx_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind);

@@ -87,6 +87,7 @@ class Stats(C.Structure):
         ("npts", C.c_int64),
         ("k1_updates", C.c_int64),
         ("k3_updates", C.c_int64),
+        ("nwin4", C.c_int64),
     ]
 
     def as_dict(self):

@@ -449,7 +449,7 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
       need(c->w_blkmax, NJ * S.blk * 4) || need(c->w_pread, NJ * S.blk * 4) ||
       need(c->w_changed, 16) || need(c->w_rec, NJ * S.line * sizeof(ScatterRec)) ||
       need(c->w_r3, (size_t)r3tot * 8) || need(c->w_r2, (size_t)r2tot * 8) ||
-      need(c->w_cnt, NJ * 16) || need(c->w_margin, NJ * 3 * 8)) {
+      need(c->w_cnt, NJ * 24) || need(c->w_margin, NJ * 3 * 8)) {
     set_err("work buffer allocation failed");
     return LBLX_E_HIP;
   }
@@ -512,8 +512,8 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   T9.bstart = c->w_t9blk.as<int32_t>();
   T9.bcount = c->w_t9blk.as<int32_t>() + NJ * S.t9blk;
   T9.bvhi = c->w_t9bvhi.as<double>();
-  unsigned long long *cnt = c->w_cnt.as<unsigned long long>();  // [2][njobs]: k1, k3
-  hipLaunchKernelGGL(k_fill_i32, dim3((4 * NJ + 255) / 256), dim3(256), 0, s, (int *)cnt, 4 * NJ, 0);
+  unsigned long long *cnt = c->w_cnt.as<unsigned long long>();  // [3][njobs]: k1, k3, lines
+  hipLaunchKernelGGL(k_fill_i32, dim3((6 * NJ + 255) / 256), dim3(256), 0, s, (int *)cnt, 6 * NJ, 0);
 
   // 1-2. LINF4 + SHRINK
   int th = c->tbegin(ST_SHRINK);
@@ -571,7 +571,8 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
     hipLaunchKernelGGL(k_scatter_rec, dim3(maxnbk, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(),
                        c->w_keep.as<int>(), L, dY, S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(),
                        c->w_rec.as<ScatterRec>());
-    hipLaunchKernelGGL(k_sum_npts, dim3(ngrid, njobs), dim3(256), 0, s, dY, S, c->w_rec.as<ScatterRec>(), cnt);
+    hipLaunchKernelGGL(k_sum_npts, dim3(ngrid, njobs), dim3(256), 0, s, dY, S, c->w_rec.as<ScatterRec>(), cnt,
+                       cnt + 2 * NJ);
     c->tend(th);
   }
   // 7. tiles: R3, R2, R1 (+ PANEL combine)
@@ -618,18 +619,20 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   if (v1p)
     for (int jb = 0; jb < njobs; jb++) v1p[jb] = plans[jb].panels[0].vft + (double)NSHIFT * plans[jb].dv;
   if (st) {
-    std::vector<unsigned long long> h(2 * NJ);
+    std::vector<unsigned long long> h(3 * NJ);
     std::vector<int32_t> t9n(NJ);
-    HIPCHK(hipMemcpyAsync(h.data(), cnt, 2 * NJ * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h.data(), cnt, 3 * NJ * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(t9n.data(), T9.n, NJ * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     for (int jb = 0; jb < njobs; jb++) {
       std::memset(&st[jb], 0, sizeof(lblx_stats));
       st[jb].k1_updates = (int64_t)h[jb];
       st[jb].k3_updates = (int64_t)h[NJ + jb];
+      st[jb].lincnt = (int64_t)h[2 * NJ + jb];
       st[jb].nlins4 = t9n[jb];
       st[jb].npanel = (int64_t)plans[jb].panels.size();
       st[jb].npts = plans[jb].npts;
+      st[jb].nwin4 = (int64_t)plans[jb].wins.size();
     }
   }
   return LBLX_OK;
@@ -745,6 +748,14 @@ int lblx_dev_free(lblx_ctx *c, void *p) {
 int lblx_copy(lblx_ctx *c, void *dst, const void *src, int64_t bytes, int kind) {
   if (!c || bytes < 0) return LBLX_E_ARG;
   HIPCHK(hipSetDevice(c->device));
+  if (kind == 3) {  // device copy through a kernel (traffic calibration)
+    int64_t n = bytes / 8;
+    hipLaunchKernelGGL(k_copy_f64, dim3(2048), dim3(256), 0, c->stream, (double *)dst,
+                       (const double *)src, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return LBLX_OK;
+  }
   hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
                     : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
   HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, k, c->stream));

@@ -871,15 +871,32 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const i
   recs[o + t] = r;
 }
 
-// Near-wing point updates per job (sum of JMAX1-JMIN1+1 over kept lines).
+// Near-wing point updates and convolved lines per job (sum of
+// JMAX1-JMIN1+1 and count over kept lines; LINCNT of the TAPE6 table).
 __global__ void k_sum_npts(const DevLayer *Ys, Strides S, const ScatterRec *rec,
-                           unsigned long long *acc) {
+                           unsigned long long *acc, unsigned long long *lines) {
   const int job = blockIdx.y;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long v = 0ull;
-  if (i < Ys[job].nn && rec[job * S.line + i].npts > 0) v = (unsigned long long)rec[job * S.line + i].npts;
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&acc[job], v);
+  unsigned long long v = 0ull, c = 0ull;
+  if (i < Ys[job].nn && rec[job * S.line + i].npts > 0) {
+    v = (unsigned long long)rec[job * S.line + i].npts;
+    c = 1ull;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    v += __shfl_down(v, off);
+    c += __shfl_down(c, off);
+  }
+  if ((threadIdx.x & 63) == 0 && v) {
+    atomicAdd(&acc[job], v);
+    atomicAdd(&lines[job], c);
+  }
+}
+
+// Plain device copy (the calibration kernel for HBM traffic counters).
+__global__ void k_copy_f64(double *dst, const double *src, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
 }
 
 // Tile table of one grid level: tile -> (job, local tile).

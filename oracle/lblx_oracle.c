@@ -795,8 +795,10 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
   double VBOT = V1 - BOUND;
   double VTOP = V2 + BOUND;
 
-  if (ilblf4 >= 1)
+  if (ilblf4 >= 1) {
     lblf4(W, &T9, Ly, JRAD, V1R4ST, V2R4ST, run->dptmin, run->dptfac, &stats.ilin4, &stats.k3_updates);
+    stats.nwin4++;
+  }
 
   /* LNCOR1 first-call constants (oprop.f90:970-993) */
   double XKT0 = TEMP0 / RADCN2;
@@ -1078,8 +1080,10 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
           double VF1 = VFT - 2. * DVR4;
           VF1 = V1 + floor((VF1 - V1) / DVR4) * DVR4;
           double VF2 = VFT + 2. * DVR4 + (double)(N2R3 + 4) * DVR3;
-          if (VF2 > W->v2r4 && W->v2r4 != V2R4ST)
+          if (VF2 > W->v2r4 && W->v2r4 != V2R4ST) {
             lblf4(W, &T9, Ly, JRAD, VF1, V2R4ST, run->dptmin, run->dptfac, &stats.ilin4, &stats.k3_updates);
+            stats.nwin4++;
+          }
         }
         goto label70;
       }

@@ -1,0 +1,150 @@
+"""GPU parity of the layer optical-depth path (OPDPTH/HIRAC1/LBLF4) with
+the CPU oracle on the same seeded inputs, through the C ABI."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from lblrtm_amd import abi, lib, synth, tape
+from parity import assert_od_close
+
+pytestmark = pytest.mark.gpu
+
+
+def _layer0():
+    p, t, wk, wb = synth.config0_layer()
+    return p, t, wk, wb
+
+
+def _run_both(ctx, run, lay, tab):
+    ref, v1r, st_r, _ = O.od_layer(run, abi.Layer.from_buffer_copy(lay), tab)
+    od, v1p, st = ctx.opdpth(run, abi.Layer.from_buffer_copy(lay))
+    assert v1p == v1r
+    return od, ref, st, st_r
+
+
+def test_single_layer_workload(gpu_ctx):
+    """20,000 lines, 600-1100 cm-1, one layer (the CPU-runnable case)."""
+    tab = synth.config0_lines()
+    gpu_ctx.load_lines(tab)
+    p, t, wk, wb = _layer0()
+    run = abi.make_run(600.0, 1100.0)
+    dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+    lay = abi.make_layer(p, t, wk, wb, dv[0], 600.0, 1100.0)
+    od, ref, st, st_r = _run_both(gpu_ctx, run, lay, tab)
+    print("single layer: npts", len(od), "max rel", np.max(np.abs(od - ref) / np.abs(ref)), flush=True)
+    assert_od_close(od, ref, "single layer")
+    for k in ("npanel", "npts", "k1_updates", "k3_updates", "nlins4"):
+        assert st[k] == st_r[k], k
+    # the shrunk far-wing list (TAPE9) agrees line by line
+    cap = 100000
+    arrs = [np.zeros(cap) for _ in range(4)] + [np.zeros(cap, np.int32)]
+    nb = C.c_int(0)
+    blk = np.zeros(1000, np.int32)
+    n_o = O.lib().oracle_linf4(C.byref(run), C.byref(abi.Layer.from_buffer_copy(lay)), C.byref(tab.struct()),
+                               *[abi.ptr(a, C.c_double) for a in arrs[:4]], abi.ptr(arrs[4], C.c_int32),
+                               abi.ptr(blk, C.c_int32), cap, 1000, C.byref(nb))
+    gv, gs, ga, ge, gm = gpu_ctx.debug_tape9(0)
+    assert len(gv) == n_o
+    np.testing.assert_array_equal(gm, arrs[4][:n_o])
+    np.testing.assert_allclose(gv, arrs[0][:n_o], rtol=1e-14)
+    np.testing.assert_allclose(gs, arrs[1][:n_o], rtol=1e-13)
+    np.testing.assert_allclose(ga, arrs[2][:n_o], rtol=1e-13)
+    np.testing.assert_allclose(ge, arrs[3][:n_o], rtol=1e-13)
+
+
+def test_column_workload_layers(gpu_ctx):
+    """50-layer column, 500,000 lines over 500-2500 cm-1: every layer in one
+    batch on the GPU; five layers across the column checked point by point."""
+    tab = synth.config1_lines()
+    gpu_ctx.load_lines(tab)
+    atm = synth.atmosphere()
+    run = abi.make_run(500.0, 2500.0)
+    dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    layers = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l],
+                             dv[l], 500.0, 2500.0) for l in range(len(dv))]
+    sizes = [lib.od_size(run, L) for L in layers]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    buf = gpu_ctx.alloc(8 * int(sum(sizes)))
+    lays = [abi.Layer.from_buffer_copy(x) for x in layers]
+    _, st = gpu_ctx.opdpth_column_device(run, lays, buf.ptr, offs, stats=True)
+    allod = buf.to_host(np.float64)
+    buf.free()
+    jrad = 1
+    for l in range(len(dv)):
+        inp = abi.Layer.from_buffer_copy(layers[l])
+        inp.jrad = jrad
+        jrad = lays[l].jrad
+        if l not in (0, 12, 25, 37, 49):
+            continue
+        ref, _, st_r, _ = O.od_layer(run, inp, tab)
+        od = allod[offs[l]:offs[l] + sizes[l]]
+        print("layer", l, "npts", len(od), "max rel", np.max(np.abs(od - ref) / np.abs(ref)), flush=True)
+        assert_od_close(od, ref, "layer %d" % l)
+        for k in ("npanel", "npts", "k1_updates", "k3_updates", "nlins4"):
+            assert st[l][k] == st_r[k], (l, k)
+
+
+def test_radiation_in_line_strengths(gpu_ctx):
+    """JRAD = 1 (V1/kT >= 5): radiation term in the strengths and in R4."""
+    tab = synth.synth_lines(20000, 1950.0, 2350.0, [1, 2, 3], 31)
+    gpu_ctx.load_lines(tab)
+    p, t, w, wb = 300.0, 200.0, [1e21, 5e21, 1e19], 1.0e24
+    run = abi.make_run(2000.0, 2300.0)
+    dv, _ = lib.plan_dv(run, [p], [t], [w], [wb])
+    lay = abi.make_layer(p, t, w, wb, dv[0], 2000.0, 2300.0)
+    od, ref, st, st_r = _run_both(gpu_ctx, run, lay, tab)
+    assert st_r["k3_updates"] > 0
+    assert_od_close(od, ref, "jrad=1")
+
+
+def test_no_rejection_and_ragged_blocks(gpu_ctx):
+    tab = synth.synth_lines(1234, 600.0, 700.0, [1, 2, 7], 11)
+    gpu_ctx.load_lines(tab)
+    p, t, wk, wb = 800.0, 280.0, [1e22, 3e21, 2e24], 1.0e25
+    for dmin, dfac in ((0.0, 0.0), (2e-4, 1e-3)):
+        run = abi.make_run(610.0, 690.0, dptmin=dmin, dptfac=dfac)
+        dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+        lay = abi.make_layer(p, t, wk, wb, dv[0], 610.0, 690.0)
+        od, ref, st, st_r = _run_both(gpu_ctx, run, lay, tab)
+        assert_od_close(od, ref, "dptmin=%g" % dmin)
+
+
+def test_band_without_lines(gpu_ctx):
+    tab = synth.synth_lines(500, 600.0, 650.0, [1], 5)
+    gpu_ctx.load_lines(tab)
+    p, t, wk, wb = _layer0()
+    run = abi.make_run(900.0, 950.0)
+    dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+    lay = abi.make_layer(p, t, wk, wb, dv[0], 900.0, 950.0)
+    od, ref, st, st_r = _run_both(gpu_ctx, run, lay, tab)
+    assert np.all(ref == 0.0) and np.all(od == 0.0)
+
+
+def test_tape3_file_and_errors(gpu_ctx, tmp_path):
+    tab = synth.synth_lines(3000, 700.0, 760.0, [1], 17)
+    path = tmp_path / "TAPE3"
+    tape.write_tape3(path, tab)
+    gpu_ctx.load_tape3(path)
+    p, t, wk, wb = _layer0()
+    run = abi.make_run(705.0, 755.0)
+    dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+    lay = abi.make_layer(p, t, wk, wb, dv[0], 705.0, 755.0)
+    od, ref, st, st_r = _run_both(gpu_ctx, run, lay, tab)
+    assert_od_close(od, ref, "tape3 file")
+    # line-coupling records are outside the supported subset
+    bad = abi.LineTable(tab.nrec, tab.vmin, tab.vmax, vnu=tab.vnu, sp=tab.sp, alfa=tab.alfa,
+                        epp=tab.epp, mol=tab.mol, hwhm=tab.hwhm, tmpalf=tab.tmpalf,
+                        pshift=tab.pshift, iflg=np.where(np.arange(len(tab.iflg)) == 3, -1, 0))
+    with pytest.raises(lib.LblxError) as e:
+        gpu_ctx.load_lines(bad)
+    assert e.value.code == abi.E_UNSUPPORTED
+    nrec = tab.nrec.copy()
+    nrec[0] = 251
+    bad2 = abi.LineTable(nrec, tab.vmin, tab.vmax, vnu=tab.vnu, sp=tab.sp, alfa=tab.alfa,
+                         epp=tab.epp, mol=tab.mol, hwhm=tab.hwhm, tmpalf=tab.tmpalf,
+                         pshift=tab.pshift, iflg=tab.iflg)
+    with pytest.raises(lib.LblxError) as e:
+        gpu_ctx.load_lines(bad2)
+    assert e.value.code == abi.E_NREC
