@@ -161,6 +161,30 @@ int lblx_od_column_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers
                           int nlay, void *d_od, const int64_t *od_off, double *v1p,
                           lblx_stats *st);
 
+/* XMERGE for one layer, IEMIT=1 upwelling (IPATHL=1), LTE, no aerosols
+ * (xmerge.f90:25-167 -> EMINIT/RADMRG, EMIN, RADNN).  layer_old == NULL:
+ * the layer at the surface (EMINIT), R = E + T*emis*B(tbnd).  Otherwise
+ * RADMRG: R = T*L4(R_old) + E, T = T*L4(T_old), where d_rold/d_told hold
+ * the merged result of the layers below on layer_old's grid.  tzu is the
+ * layer's upper boundary temperature (TZU).  d_od: the layer's optical
+ * depth as lblx_od_layer_device writes it (double, or float for
+ * FP32_SPLIT).  Outputs on layer's grid (lblx_merge_size points).        */
+int lblx_merge_size(const lblx_run *run, const lblx_layer *layer_old,
+                    const lblx_layer *layer, int64_t *n);
+int lblx_xmerge_device(lblx_ctx *ctx, const lblx_run *run, const lblx_layer *layer_old,
+                       const lblx_layer *layer, double tzu, double tbnd, double emis,
+                       const void *d_od, const double *d_rold, const double *d_told,
+                       double *d_rnew, double *d_tnew);
+
+/* The spectrum of a column: OPDPTH for all nlay layers (surface first) in
+ * one batch, then XMERGE layer by layer; TOA upwelling radiance and
+ * transmittance on the top layer's grid in d_rad/d_tra (n_out >= its
+ * size).  tzu[nlay]: upper boundary temperatures; tbnd, emis: surface
+ * temperature and emissivity (record 1.4).                                */
+int lblx_spectrum_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay,
+                         const double *tzu, double tbnd, double emis, double *d_rad,
+                         double *d_tra, int64_t n_out, double *v1p_top, lblx_stats *st);
+
 /* Wait for the context stream; the stream handle (hipStream_t) for callers
  * that order their own device work after the library's.                  */
 int lblx_sync(lblx_ctx *ctx);

@@ -136,6 +136,7 @@ struct lblx_ctx {
   DevBuf w_t9vnu, w_t9sp, w_t9alfa, w_t9epp, w_t9mol, w_t9n, w_t9blk, w_t9bvhi;
   DevBuf w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
   DevBuf w_tmap[3];
+  DevBuf w_mpan, w_rt;
   DevLines lines() const {
     DevLines L;
     L.n = nlines;
@@ -220,7 +221,7 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
                     &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
-                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2]};
+                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -699,6 +700,212 @@ int lblx_debug_tape9(lblx_ctx *c, int job, double *vnu, double *sp, double *alfa
     HIPCHK(hipMemcpy(epp, c->w_t9epp.as<double>() + o, k * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(mol, c->w_t9mol.as<int32_t>() + o, k * 4, hipMemcpyDeviceToHost));
   }
+  return LBLX_OK;
+}
+
+// Replays RADMRG's panel reads (xmerge.f90:2683-2712) and RADNN's slot
+// walk (xmerge.f90:3105-3130) over the panel headers of the old (merged)
+// and new layers: base[p] = old point held in carry-over slot 1 while new
+// panel p is merged.  Grids on which RADNN would read a slot that holds no
+// old point yet (or the end padding while old panels remain) are refused.
+static int merge_bases(const LayerPlan &O, const LayerPlan &K, int itype, double v2,
+                       std::vector<int64_t> &base) {
+  const int LL = itype + 1, LLM1 = std::max(itype, 1);
+  const int npo = (int)O.panels.size();
+  int64_t g0 = 0, npe = 0;
+  double v2po = 0., dvpo = 0.;
+  int po = 0;
+  base.resize(K.panels.size());
+  for (size_t pk = 0; pk < K.panels.size(); pk++) {
+    const int nlim = K.panels[pk].nhi - NSHIFT;
+    const double v2p = K.panels[pk].v2p;
+    if (v2po <= v2p + dvpo) {
+      while (po < npo) {
+        npe += O.panels[po].nhi - NSHIFT;
+        v2po = O.panels[po].v2p;
+        dvpo = O.dv;
+        po++;
+        if (!(v2po <= v2p + dvpo)) break;
+      }
+    }
+    const bool padded = v2po + dvpo >= v2;
+    const int64_t lim = (padded && po == npo) ? npe + 2 : npe;
+    int64_t ipl = 0;
+    for (int nl = 1; nl <= LL; nl++) {
+      ipl = nl - LLM1 - (nl > 1 ? 1 : 0);
+      int64_t cnt = nlim >= nl ? (nlim - nl) / LL + 1 : 0;
+      ipl += cnt * LLM1;
+      if (cnt > 0 && ipl + (nl > 1 ? 2 : 0) > lim) return LBLX_E_UNSUPPORTED;
+    }
+    base[pk] = g0;
+    const int64_t npl = ipl;   // RADO(NPL) becomes RADO(0)
+    g0 += npl;
+    npe -= npl;
+  }
+  return LBLX_OK;
+}
+
+static int merge_itype(double dvl, double dvk, int *itype) {
+  if (dvl == dvk) *itype = 0;
+  else if (dvl > dvk) *itype = fint(dvk / (dvl - dvk) + 0.5);
+  else return LBLX_E_COARSER;
+  if (*itype > 10) return LBLX_E_UNSUPPORTED;
+  return LBLX_OK;
+}
+
+// Panel table of one merge launch: VFT and the carry-over base per panel.
+static void merge_panels(const LayerPlan &K, const std::vector<int64_t> *base,
+                         std::vector<DevPanel> &out) {
+  for (size_t i = 0; i < K.panels.size(); i++)
+    out.push_back(DevPanel{K.panels[i].vft, 0, 0, 0, 0, 0, 0, base ? (*base)[i] : 0});
+}
+
+// XMERGE (IEMIT=1, IPATHL=1) for one layer on device buffers.
+static int merge_launch(lblx_ctx *c, const LayerPlan &K, const LayerPlan *O, int itype, double tave,
+                        double tzu, double tbnd, double emis, const void *d_od, bool od32,
+                        const double *d_rold, const double *d_told, double *d_rnew, double *d_tnew,
+                        const DevPanel *d_pan) {
+  MergeArgs A{};
+  A.dv = K.dv;
+  A.xkt = tave / RADCN2;
+  A.xkta = tzu / RADCN2;
+  A.xktbnd = tbnd / RADCN2;
+  A.emis = emis;
+  A.n = K.npts;
+  A.nold = O ? O->npts : 0;
+  A.first = O == nullptr;
+  A.itype = itype;
+  double ap = 1.0 / ((double)itype + 1.0);
+  for (int jpg = 1; jpg <= itype; jpg++) {  // RADMRG (xmerge.f90:2788-2796)
+    double pp = 1.0 - (ap * (double)jpg);
+    int ipl = jpg + 1;
+    A.a1[ipl] = -pp * (pp - 1.0) * (pp - 2.0) / 6.0;
+    A.a2[ipl] = (pp * pp - 1.0) * (pp - 2.0) * 0.5;
+    A.a3[ipl] = -pp * (pp + 1.0) * (pp - 2.0) * 0.5;
+    A.a4[ipl] = pp * (pp * pp - 1.0) / 6.0;
+  }
+  int th = c->tbegin(ST_MERGE);
+  dim3 grid((unsigned)((A.n + 255) / 256));
+  if (od32)
+    hipLaunchKernelGGL(k_merge<float>, grid, dim3(256), 0, c->stream, A, d_pan, (const float *)d_od,
+                       d_rold, d_told, d_rnew, d_tnew);
+  else
+    hipLaunchKernelGGL(k_merge<double>, grid, dim3(256), 0, c->stream, A, d_pan, (const double *)d_od,
+                       d_rold, d_told, d_rnew, d_tnew);
+  c->tend(th);
+  HIPCHK(hipGetLastError());
+  return LBLX_OK;
+}
+
+int lblx_merge_size(const lblx_run *run, const lblx_layer *layer_old, const lblx_layer *layer,
+                    int64_t *n) {
+  if (!run || !layer || !n) return LBLX_E_ARG;
+  lblx_layer tmp = *layer;
+  LayerPlan K;
+  int rc = plan_layer(run, &tmp, &K);
+  if (rc) return rc;
+  if (layer_old) {
+    lblx_layer t2 = *layer_old;
+    LayerPlan O;
+    if ((rc = plan_layer(run, &t2, &O))) return rc;
+    int itype;
+    if ((rc = merge_itype(O.dv, K.dv, &itype))) return rc;
+    std::vector<int64_t> base;
+    if ((rc = merge_bases(O, K, itype, run->v2, base))) return rc;
+  }
+  *n = K.npts;
+  return LBLX_OK;
+}
+
+int lblx_xmerge_device(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer_old,
+                       const lblx_layer *layer, double tzu, double tbnd, double emis,
+                       const void *d_od, const double *d_rold, const double *d_told, double *d_rnew,
+                       double *d_tnew) {
+  if (!c || !run || !layer || !d_od || !d_rnew || !d_tnew) return LBLX_E_ARG;
+  if (layer_old && (!d_rold || !d_told)) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  lblx_layer tmp = *layer;
+  LayerPlan K, O;
+  int rc = plan_layer(run, &tmp, &K);
+  if (rc) return rc;
+  std::vector<int64_t> base;
+  int itype = 0;
+  if (layer_old) {
+    lblx_layer t2 = *layer_old;
+    if ((rc = plan_layer(run, &t2, &O))) return rc;
+    if ((rc = merge_itype(O.dv, K.dv, &itype))) return rc;
+    if ((rc = merge_bases(O, K, itype, run->v2, base))) return rc;
+  }
+  std::vector<DevPanel> pan;
+  merge_panels(K, layer_old ? &base : nullptr, pan);
+  if (c->w_mpan.ensure(pan.size() * sizeof(DevPanel))) return LBLX_E_HIP;
+  HIPCHK(hipMemcpyAsync(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice,
+                        c->stream));
+  return merge_launch(c, K, layer_old ? &O : nullptr, itype, layer->tave, tzu,
+                      layer_old ? 0. : tbnd, emis, d_od, run->precision == LBLX_FP32_SPLIT, d_rold,
+                      d_told, d_rnew, d_tnew, c->w_mpan.as<DevPanel>());
+}
+
+// OPDPTH for every layer of a column in one batch, then XMERGE layer by
+// layer from the surface up: upwelling radiance and transmittance at the
+// top of the atmosphere on the top layer's grid.
+int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
+                         const double *tzu, double tbnd, double emis, double *d_rad, double *d_tra,
+                         int64_t n_out, double *v1p_top, lblx_stats *st) {
+  if (!c || !run || !layers || nlay <= 0 || !tzu || !d_rad || !d_tra) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<LayerPlan> plans(nlay);
+  std::vector<int64_t> off(nlay);
+  int64_t tot = 0, nmax = 0;
+  int jr = layers[0].jrad;
+  for (int l = 0; l < nlay; l++) {
+    lblx_layer tmp = layers[l];
+    tmp.jrad = jr;
+    int rc = plan_layer(run, &tmp, &plans[l]);
+    if (rc) return rc;
+    jr = tmp.jrad;
+    off[l] = tot;
+    tot += plans[l].npts;
+    nmax = std::max(nmax, plans[l].npts);
+  }
+  if (n_out < plans[nlay - 1].npts) return LBLX_E_SIZE;
+  // merge tables first: a refused grid fails before any device work
+  std::vector<int> itype(nlay, 0);
+  std::vector<DevPanel> pan;
+  std::vector<int> pan0(nlay);
+  for (int l = 0; l < nlay; l++) {
+    pan0[l] = (int)pan.size();
+    std::vector<int64_t> base;
+    if (l > 0) {
+      int rc = merge_itype(plans[l - 1].dv, plans[l].dv, &itype[l]);
+      if (rc) return rc;
+      if ((rc = merge_bases(plans[l - 1], plans[l], itype[l], run->v2, base))) return rc;
+    }
+    merge_panels(plans[l], l > 0 ? &base : nullptr, pan);
+  }
+  const bool f32 = run->precision == LBLX_FP32_SPLIT;
+  const size_t esz = f32 ? 4 : 8;
+  if (c->w_od.ensure((size_t)tot * esz) || c->w_rt.ensure((size_t)nmax * 8 * 4) ||
+      c->w_mpan.ensure(pan.size() * sizeof(DevPanel)))
+    return LBLX_E_HIP;
+  int rc = run_jobs(c, run, layers, nlay, true, c->w_od.p, off.data(), f32, nullptr, st);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice,
+                        c->stream));
+  double *rt = c->w_rt.as<double>();
+  double *R[2] = {rt, rt + nmax}, *T[2] = {rt + 2 * nmax, rt + 3 * nmax};
+  for (int l = 0; l < nlay; l++) {
+    const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+    double *rn = (l == nlay - 1) ? d_rad : R[l & 1];
+    double *tn = (l == nlay - 1) ? d_tra : T[l & 1];
+    const double *ro = l > 0 ? R[(l - 1) & 1] : nullptr;
+    const double *to = l > 0 ? T[(l - 1) & 1] : nullptr;
+    rc = merge_launch(c, plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+                      l == 0 ? tbnd : 0., emis, od, f32, ro, to, rn, tn,
+                      c->w_mpan.as<DevPanel>() + pan0[l]);
+    if (rc) return rc;
+  }
+  if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
   return LBLX_OK;
 }
 

@@ -1076,4 +1076,84 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
   }
 }
 
+// ---------------------------------------------------------------------
+// XMERGE for upwelling radiance (IEMIT=1, IPATHL=1, LTE, no aerosols).
+// EMIN (xmerge.f90:1850-1879): T = exp(-tau), E = (1-T)(B(Tave) + (B(Tzu)
+// - B(Tave)) F(tau)) with F = tau/6 for tau <= 0.06.  First layer (EMINIT,
+// xmerge.f90:2470-2480): R = E + T eps B(Tbnd).  Later layers (RADMRG/RADNN,
+// xmerge.f90:2991-2995, 3118-3130): R = T L4(R_old) + E, T = T L4(T_old).
+// RADNN walks each new panel in LL = ITYPE+1 phases: local point
+// i = m*LL + (NL-1) reads carry-over slot IPL = 1 + m*max(ITYPE,1) (NL = 1)
+// or the four slots IPL-1..IPL+2 around IPL = NL-1 + m*ITYPE with the
+// Lagrange weights of P = 1 - (NL-1)/LL.  Slot j of new panel p holds old
+// point base_p + j - 1 (base_p from the host's replay of RADMRG's panel
+// reads, in DevPanel::out0); RADMRG duplicates the first old point below
+// slot 1 and the last one past the end, which the index clamp reproduces.
+struct MergeArgs {
+  double dv, xkt, xkta, xktbnd, emis;
+  int64_t n, nold;
+  int32_t first, itype;
+  double a1[12], a2[12], a3[12], a4[12];
+};
+
+__device__ __forceinline__ double d_planck(double vi, double xkt) {
+  double v = 0.;
+  if (xkt > 0.0) {
+    double x = vi / xkt;
+    if (x < 1e-2) v = (vi * vi) * xkt / (1. + 0.5 * x);
+    else if (x <= 80.) {
+      double e = exp(-x);
+      v = (vi * vi * vi) * e / (1. - e);
+    }
+  }
+  return v * 1.191042722E-12;
+}
+
+template <typename IN>
+__global__ void k_merge(MergeArgs A, const DevPanel *pan, const IN *od, const double *rold,
+                        const double *told, double *rnew, double *tnew) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.n) return;
+  const int p = (int)(g / 2400);
+  const int k = (int)(g - (int64_t)2400 * p);
+  const double vi = (pan[p].vft + 32. * A.dv) + (double)k * A.dv;
+  const double od_lo = 0.06;
+  double bb = d_planck(vi, A.xkt);
+  double bb_dif = A.xkta > 0. ? d_planck(vi, A.xkta) - bb : 0.;
+  double odvi = (double)od[g];
+  if (odvi < -od_lo) odvi = -od_lo;
+  double tr = exp(-odvi);
+  double f = odvi <= od_lo ? (1. / 6.) * odvi : 1. - 2. * (tr / (tr - 1.) + 1. / odvi);
+  double em = (1. - tr) * (bb + bb_dif * f);
+  if (A.first) {
+    if (A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
+    rnew[g] = em;
+    tnew[g] = tr;
+    return;
+  }
+  const int LL = A.itype + 1;
+  const int64_t base = pan[p].out0 - 1;   // old index of carry-over slot 0
+  const int m = k / LL;
+  const int r = k - m * LL;
+  const int64_t last = A.nold - 1;
+  double ro, to;
+  if (r == 0) {
+    int64_t q = base + 1 + (int64_t)m * (A.itype > 0 ? A.itype : 1);
+    q = q < 0 ? 0 : (q > last ? last : q);
+    ro = rold[q];
+    to = told[q];
+  } else {
+    int64_t q = base + r + (int64_t)m * A.itype;   // slot IPL = r + m*ITYPE
+    int64_t i0 = q - 1 < 0 ? 0 : (q - 1 > last ? last : q - 1);
+    int64_t i1 = q < 0 ? 0 : (q > last ? last : q);
+    int64_t i2 = q + 1 > last ? last : q + 1;
+    int64_t i3 = q + 2 > last ? last : q + 2;
+    const int nl = r + 1;
+    ro = A.a1[nl] * rold[i0] + A.a2[nl] * rold[i1] + A.a3[nl] * rold[i2] + A.a4[nl] * rold[i3];
+    to = A.a1[nl] * told[i0] + A.a2[nl] * told[i1] + A.a3[nl] * told[i2] + A.a4[nl] * told[i3];
+  }
+  rnew[g] = tr * ro + em;
+  tnew[g] = tr * to;
+}
+
 }  // namespace lblx

@@ -276,8 +276,11 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P) {
     if ((VFT + (NHI - 1) * DV) > V2) {
       NHI = fint((V2 - VFT) / DV + 1.);
       double v2p = VFT + (double)(NHI - 1) * DV;
-      if (v2p < V2) NHI = NHI + 1;
+      if (v2p < V2) { v2p = v2p + DV; NHI = NHI + 1; }
+      pp.v2p = v2p;
       istop = 1;
+    } else {
+      pp.v2p = VFT + (double)(NHI - 1) * DV;
     }
     pp.nhi = NHI;
     pp.istop = istop;

@@ -32,6 +32,7 @@ struct RadSeg {
 
 struct PanelPlan {
   double vft;       // VFT of this panel
+  double v2p;       // V2P of the panel header
   int32_t w;        // far-wing window active at XINT time
   int32_t nhi;      // NHI (2432 except the last panel)
   int32_t istop;    // last panel

@@ -64,6 +64,13 @@ def load():
         "lblx_dev_alloc": (C.c_void_p, [C.c_void_p, C.c_int64]),
         "lblx_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
         "lblx_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
+        "lblx_merge_size": (C.c_int, [P(abi.Run), P(abi.Layer), P(abi.Layer), P(C.c_int64)]),
+        "lblx_xmerge_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer),
+                                         C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "lblx_spectrum_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int,
+                                           P(C.c_double), C.c_double, C.c_double, C.c_void_p,
+                                           C.c_void_p, C.c_int64, P(C.c_double), P(abi.Stats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -105,6 +112,15 @@ def plan_dv(run, pave, tave, wk, wbroad, iemit=0):
 def od_size(run, layer):
     n = C.c_int64(0)
     _chk(load().lblx_od_size(C.byref(run), C.byref(layer), C.byref(n)), "lblx_od_size")
+    return n.value
+
+
+def merge_size(run, layer_old, layer):
+    """Points of the merged result on layer's grid; raises when RADMRG's
+    panel walk over these two grids is not supported."""
+    n = C.c_int64(0)
+    _chk(load().lblx_merge_size(C.byref(run), C.byref(layer_old) if layer_old is not None else None,
+                                C.byref(layer), C.byref(n)), "lblx_merge_size")
     return n.value
 
 
@@ -175,6 +191,44 @@ class Context:
         for k in range(n):
             layers[k].jrad = arr[k].jrad
         return list(v1p), ([x.as_dict() for x in st] if stats else None)
+
+    def xmerge_device(self, run, layer_old, layer, tzu, tbnd, emis, d_od, d_rold, d_told,
+                      d_rnew, d_tnew):
+        """XMERGE of one layer (IEMIT=1, IPATHL=1) on device pointers;
+        layer_old None for the layer at the surface (EMINIT)."""
+        _chk(self._lib.lblx_xmerge_device(
+            self._h, C.byref(run), C.byref(layer_old) if layer_old is not None else None,
+            C.byref(layer), tzu, tbnd, emis, C.c_void_p(d_od), C.c_void_p(d_rold or 0),
+            C.c_void_p(d_told or 0), C.c_void_p(d_rnew), C.c_void_p(d_tnew)), "lblx_xmerge_device")
+
+    def spectrum_device(self, run, layers, tzu, tbnd, emis, d_rad, d_tra, n_out, stats=False):
+        """OPDPTH for every layer (surface first) then XMERGE up the column:
+        TOA upwelling radiance/transmittance at device pointers d_rad/d_tra.
+        Returns (v1p of the top grid, stats per layer or None)."""
+        n = len(layers)
+        arr = (abi.Layer * n)(*layers)
+        tz = np.ascontiguousarray(tzu, np.float64)
+        v1p = C.c_double(0.0)
+        st = (abi.Stats * n)() if stats else None
+        _chk(self._lib.lblx_spectrum_device(self._h, C.byref(run), arr, n, abi.ptr(tz, C.c_double),
+                                            tbnd, emis, C.c_void_p(d_rad), C.c_void_p(d_tra),
+                                            n_out, C.byref(v1p), st), "lblx_spectrum_device")
+        for k in range(n):
+            layers[k].jrad = arr[k].jrad
+        return v1p.value, ([x.as_dict() for x in st] if stats else None)
+
+    def spectrum(self, run, layers, tzu, tbnd, emis):
+        """Host-side convenience over spectrum_device: returns (radiance,
+        transmittance, v1p, dv) of the top of the atmosphere."""
+        n = od_size(run, layers[-1])
+        rad = self.alloc(8 * n)
+        tra = self.alloc(8 * n)
+        try:
+            v1p, _ = self.spectrum_device(run, layers, tzu, tbnd, emis, rad.ptr, tra.ptr, n)
+            return rad.to_host(np.float64), tra.to_host(np.float64), v1p, layers[-1].dv
+        finally:
+            rad.free()
+            tra.free()
 
     def debug_tape9(self, job=0, cap=2000000):
         vnu = np.zeros(cap)

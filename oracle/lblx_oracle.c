@@ -715,6 +715,19 @@ typedef struct {
   int mol[251], iflg[251], izeta[251], iout[251];
 } blk_t;
 
+/* Panel headers of the last oracle_od_layer call (V1P and NLIM per panel),
+ * the panel structure XMERGE reads back from the optical-depth file.     */
+static double *g_pan_v1p = NULL, *g_pan_v2p = NULL;
+static int *g_pan_nlim = NULL;
+static int g_pan_n = 0, g_pan_cap = 0;
+
+int oracle_panels(double *v1p, double *v2p, int *nlim, int cap)
+{
+  int n = g_pan_n < cap ? g_pan_n : cap;
+  for (int i = 0; i < n; i++) { v1p[i] = g_pan_v1p[i]; v2p[i] = g_pan_v2p[i]; nlim[i] = g_pan_nlim[i]; }
+  return g_pan_n;
+}
+
 int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
                     const double *absrb, double v1abs, double v2abs, double dvabs, int nptabs,
                     double *od, int64_t n_out, int64_t *npts_out, double *v1p_out,
@@ -1013,7 +1026,6 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
       } else {
         V2P = VFT + (double)(NHI - 1) * DV;
       }
-      (void)V2P;
       int NLIM = NHI - NLO + 1;
       V1P = VFT + (double)(NLO - 1) * DV;
       int LIMLO = N1R2;
@@ -1059,7 +1071,17 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
         } while (NPTSI2 < NHI);
       }
       /* emit panel R1(NLO..NHI) */
-      if (first_panel_out) { if (v1p_out) *v1p_out = V1P; first_panel_out = 0; }
+      if (first_panel_out) { if (v1p_out) *v1p_out = V1P; first_panel_out = 0; g_pan_n = 0; }
+      if (g_pan_n == g_pan_cap) {
+        g_pan_cap = g_pan_cap ? 2 * g_pan_cap : 1024;
+        g_pan_v1p = realloc(g_pan_v1p, g_pan_cap * sizeof(double));
+        g_pan_nlim = realloc(g_pan_nlim, g_pan_cap * sizeof(int));
+        g_pan_v2p = realloc(g_pan_v2p, g_pan_cap * sizeof(double));
+      }
+      g_pan_v1p[g_pan_n] = V1P;
+      g_pan_v2p[g_pan_n] = V2P;
+      g_pan_nlim[g_pan_n] = NLIM;
+      g_pan_n++;
       for (int k = 0; k < NLIM; k++) {
         if (od && nout + k < n_out) od[nout + k] = R1[NLO + k];
       }
@@ -1198,4 +1220,177 @@ int oracle_linf4(const lblx_run *run, const lblx_layer *Ly, const lblx_lines *L,
   int total = T9.n;
   t9_free(&T9);
   return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* Layer merge for upwelling radiance (IEMIT=1, IPATHL=1, no aerosols,
+ * LTE): EMIN + EMINIT for the first layer, EMIN + RADMRG/RADNN for the
+ * others (src/xmerge.f90:1684-1900, 2245-2586, 2587-3280), panel by panel
+ * with RADMRG's RADO/TRAO carry-over buffer.                              */
+static const double RADCN1_ = 1.191042722E-12;
+
+double oracle_planck(double vi, double xkt)
+{
+  double vplanck = 0.;
+  if (xkt > 0.0) {
+    double vkt = xkt;
+    double viokt = vi / vkt;
+    if (viokt < 1e-2) vplanck = (vi * vi) * vkt / (1. + 0.5 * viokt);
+    else if (viokt <= 80.) {
+      double e = exp(-viokt);
+      vplanck = (vi * vi * vi) * e / (1. - e);
+    }
+  }
+  return vplanck * RADCN1_;
+}
+
+/* EMIN (LTE branch, XKTB <= 0): optical depth -> layer transmittance tr
+ * and emission em with the linear-in-tau source of od_lo = 0.06.         */
+static void emin_panel(double v1p, double dvp, int nlim, const double *od, double xkt,
+                       double xkta, double *em, double *tr)
+{
+  const double od_lo = 0.06;
+  double rec_6 = 1. / 6.;
+  double vi = v1p;
+  for (int i = 0; i < nlim; i++) {
+    double bb = oracle_planck(vi, xkt);
+    double bb_dif = 0.;
+    if (xkta > 0.) bb_dif = oracle_planck(vi, xkta) - bb;
+    double odvi = od[i];
+    if (odvi < -od_lo) odvi = -od_lo;
+    double tr_i = exp(-odvi);
+    double f_i;
+    if (odvi <= od_lo) f_i = rec_6 * odvi;
+    else f_i = 1. - 2. * (tr_i / (tr_i - 1.) + 1. / odvi);
+    tr[i] = tr_i;
+    em[i] = (1. - tr_i) * (bb + bb_dif * f_i);
+    vi = vi + dvp;
+  }
+}
+
+/* First layer (EMINIT, IPATHL=1, TBND>0): R = E + T*eps*B(TBND), T = T_l.
+ * od holds the layer's panels back to back (npan panels, v1p[]/nlim[]). */
+int oracle_merge_first(int npan, const double *v1p, const int *nlim, double dv, const double *od,
+                       double tave, double tzu, double tbnd, double emis, double *rad, double *tra)
+{
+  double xkt = tave / RADCN2, xkta = tzu / RADCN2, xktbnd = tbnd / RADCN2;
+  int64_t o = 0;
+  for (int p = 0; p < npan; p++) {
+    int n = nlim[p];
+    emin_panel(v1p[p], dv, n, od + o, xkt, xkta, rad + o, tra + o);
+    if (tbnd > 0.) {
+      double vi = v1p[p];
+      for (int j = 0; j < n; j++) {
+        double bb = oracle_planck(vi, xktbnd);
+        rad[o + j] = rad[o + j] + tra[o + j] * bb * emis;
+        vi = vi + dv;
+      }
+    }
+    o += n;
+  }
+  return 0;
+}
+
+/* RADMRG + RADNN, IPATH_FLG = 1: the accumulated radiance/transmittance of
+ * the layers below (old panels, grid dvo) merged with this layer's
+ * optical depth panels (grid dvk).                                       */
+static int64_t g_merge_quirks = 0;
+
+/* RADNN reads of the carry-over buffer beyond the points read so far: the
+ * stale tail of the buffer, or the end padding while old panels remain.
+ * Zero for grids whose merge equals the point-wise interpolation.         */
+int64_t oracle_merge_quirks(void) { return g_merge_quirks; }
+
+int oracle_merge_next(int npo, const double *v1po, const double *v2po, const int *nlimo, double dvo,
+                      const double *rado_in, const double *trao_in, int npk, const double *v1pk,
+                      const double *v2pk, const int *nlimk, double dvk, const double *od, double tave,
+                      double tzu, double v2, double *rad, double *tra)
+{
+  g_merge_quirks = 0;
+  int itype;
+  if (dvo == dvk) itype = 0;
+  else if (dvo > dvk) itype = fint(dvk / (dvo - dvk) + 0.5);
+  else return LBLX_E_COARSER;
+  double xkt = tave / RADCN2, xkta = tzu / RADCN2;
+  double A1[12], A2[12], A3[12], A4[12];
+  double atype = itype;
+  int LL = itype + 1;
+  double ap = 1.0 / (atype + 1.0);
+  for (int jpg = 1; jpg <= itype; jpg++) {
+    double apg = jpg;
+    int ipl = jpg + 1;
+    double P = 1.0 - (ap * apg);
+    A1[ipl] = -P * (P - 1.0) * (P - 2.0) / 6.0;
+    A2[ipl] = (P * P - 1.0) * (P - 2.0) * 0.5;
+    A3[ipl] = -P * (P + 1.0) * (P - 2.0) * 0.5;
+    A4[ipl] = P * (P * P - 1.0) / 6.0;
+  }
+  static double RADO[20005], TRAO[20005];
+  int npe = 0;
+  RADO[0] = 0.0;
+  TRAO[0] = 0.0;
+  double V2PO = 0.0, DVPO = 0.0;
+  int po = 0;         /* next old panel to read */
+  int64_t oo = 0;     /* offset of that panel in rado_in */
+  int64_t ok = 0;     /* offset of the current new panel */
+  double *radn = malloc(2410 * sizeof(double)), *tran = malloc(2410 * sizeof(double));
+  for (int pk = 0; pk < npk; pk++) {
+    double V1P = v1pk[pk];
+    int NLIM = nlimk[pk];
+    double V2P = v2pk[pk];
+    emin_panel(V1P, dvk, NLIM, od + ok, xkt, xkta, radn, tran);
+    if (V2PO <= V2P + DVPO) {
+      while (po < npo) {
+        int n = nlimo[po];
+        for (int j = 0; j < n; j++) { RADO[npe + 1 + j] = rado_in[oo + j]; TRAO[npe + 1 + j] = trao_in[oo + j]; }
+        V2PO = v2po[po];
+        DVPO = dvo;
+        oo += n;
+        po++;
+        npe = n + npe;
+        if (!(V2PO <= V2P + DVPO)) break;
+      }
+    }
+    if (RADO[0] == 0.0 && TRAO[0] == 0.0) { RADO[0] = RADO[1]; TRAO[0] = TRAO[1]; }
+    int padded = 0;
+    if (V2PO + DVPO >= v2) {
+      RADO[npe + 1] = RADO[npe]; RADO[npe + 2] = RADO[npe];
+      TRAO[npe + 1] = TRAO[npe]; TRAO[npe + 2] = TRAO[npe];
+      padded = 1;
+    }
+    int lim_ok = (padded && po == npo) ? npe + 2 : npe;
+    int npl = 1;
+    /* RADNN, IPATH_FLG = 1 */
+    int LLM1 = LL - 1;
+    if (LLM1 < 1) LLM1 = 1;
+    int IPL = 0;
+    for (int NL = 1; NL <= LL; NL++) {
+      IPL = (npl + NL - 1) - LLM1;
+      if (NL > 1) IPL = IPL - 1;
+      if (NL == 1) {
+        for (int I = NL; I <= NLIM; I += LL) {
+          IPL = IPL + LLM1;
+          if (IPL > lim_ok) g_merge_quirks++;
+          radn[I - 1] = tran[I - 1] * RADO[IPL] + radn[I - 1];
+          tran[I - 1] = tran[I - 1] * TRAO[IPL];
+        }
+      } else {
+        double a1 = A1[NL], a2 = A2[NL], a3 = A3[NL], a4 = A4[NL];
+        for (int I = NL; I <= NLIM; I += LL) {
+          IPL = IPL + LLM1;
+          if (IPL + 2 > lim_ok) g_merge_quirks++;
+          radn[I - 1] = tran[I - 1] * (a1 * RADO[IPL - 1] + a2 * RADO[IPL] + a3 * RADO[IPL + 1] + a4 * RADO[IPL + 2]) + radn[I - 1];
+          tran[I - 1] = tran[I - 1] * (a1 * TRAO[IPL - 1] + a2 * TRAO[IPL] + a3 * TRAO[IPL + 1] + a4 * TRAO[IPL + 2]);
+        }
+      }
+    }
+    npl = IPL;
+    for (int j = 0; j < NLIM; j++) { rad[ok + j] = radn[j]; tra[ok + j] = tran[j]; }
+    ok += NLIM;
+    int ip = -1;
+    for (int nl = npl; nl <= npe; nl++) { ip++; RADO[ip] = RADO[nl]; TRAO[ip] = TRAO[nl]; }
+    npe = ip;
+  }
+  free(radn); free(tran);
+  return 0;
 }

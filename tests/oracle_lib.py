@@ -40,6 +40,21 @@ def lib():
             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
             C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
             C.c_int, C.c_int, C.POINTER(C.c_int)]
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.oracle_panels.restype = C.c_int
+        L.oracle_panels.argtypes = [dp, dp, ip, C.c_int]
+        L.oracle_planck.restype = C.c_double
+        L.oracle_planck.argtypes = [C.c_double, C.c_double]
+        L.oracle_merge_first.restype = C.c_int
+        L.oracle_merge_first.argtypes = [C.c_int, dp, ip, C.c_double, dp, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, dp, dp]
+        L.oracle_merge_next.restype = C.c_int
+        L.oracle_merge_next.argtypes = [C.c_int, dp, dp, ip, C.c_double, dp, dp, C.c_int, dp, dp,
+                                        ip, C.c_double, dp, C.c_double, C.c_double, C.c_double,
+                                        dp, dp]
+        L.oracle_merge_quirks.restype = C.c_int64
+        L.oracle_merge_quirks.argtypes = []
         _LIB = L
     return _LIB
 
@@ -87,3 +102,59 @@ def od_layer(run, layer, table, absrb=None, v1abs=0.0, v2abs=0.0, dvabs=0.0):
     if rc:
         raise RuntimeError("oracle_od_layer rc=%d" % rc)
     return out, v1p.value, st.as_dict(), jrad_after
+
+
+def panels():
+    """Panel headers (v1p, v2p, nlim) of the last od_layer run."""
+    cap = 100000
+    v1 = np.zeros(cap)
+    v2 = np.zeros(cap)
+    nl = np.zeros(cap, np.int32)
+    n = lib().oracle_panels(abi.ptr(v1, C.c_double), abi.ptr(v2, C.c_double), abi.ptr(nl, C.c_int), cap)
+    return v1[:n].copy(), v2[:n].copy(), nl[:n].copy()
+
+
+class Merged:
+    """Accumulated XMERGE result: panels of the grid and the R/T arrays."""
+
+    def __init__(self, pan, dv, rad, tra):
+        self.pan, self.dv, self.rad, self.tra = pan, dv, rad, tra
+
+
+def _d(a):
+    return abi.ptr(np.ascontiguousarray(a, np.float64), C.c_double)
+
+
+def merge_first(layer_od, pan, dv, tave, tzu, tbnd, emis):
+    v1, v2, nl = pan
+    n = int(nl.sum())
+    rad = np.zeros(n)
+    tra = np.zeros(n)
+    od = np.ascontiguousarray(layer_od, np.float64)
+    nlc = np.ascontiguousarray(nl, np.int32)
+    rc = lib().oracle_merge_first(len(v1), _d(v1), abi.ptr(nlc, C.c_int), dv, abi.ptr(od, C.c_double),
+                                  tave, tzu, tbnd, emis, abi.ptr(rad, C.c_double), abi.ptr(tra, C.c_double))
+    assert rc == 0
+    return Merged(pan, dv, rad, tra)
+
+
+def merge_next(old, layer_od, pan, dv, tave, tzu, v2run):
+    """Returns (Merged, quirks) -- quirks counts RADNN reads past the old
+    points read so far."""
+    v1o, v2o, nlo = old.pan
+    v1, v2, nl = pan
+    n = int(nl.sum())
+    rad = np.zeros(n)
+    tra = np.zeros(n)
+    od = np.ascontiguousarray(layer_od, np.float64)
+    nloc = np.ascontiguousarray(nlo, np.int32)
+    nlc = np.ascontiguousarray(nl, np.int32)
+    ro = np.ascontiguousarray(old.rad)
+    to = np.ascontiguousarray(old.tra)
+    rc = lib().oracle_merge_next(len(v1o), _d(v1o), _d(v2o), abi.ptr(nloc, C.c_int), old.dv,
+                                 abi.ptr(ro, C.c_double), abi.ptr(to, C.c_double), len(v1), _d(v1),
+                                 _d(v2), abi.ptr(nlc, C.c_int), dv, abi.ptr(od, C.c_double), tave, tzu,
+                                 v2run, abi.ptr(rad, C.c_double), abi.ptr(tra, C.c_double))
+    if rc:
+        raise RuntimeError("oracle_merge_next rc=%d" % rc)
+    return Merged(pan, dv, rad, tra), int(lib().oracle_merge_quirks())

@@ -1,0 +1,115 @@
+"""GPU parity of the layer merge (XMERGE, IEMIT=1 upwelling) and of the
+whole spectrum (OPDPTH for every layer, then XMERGE up the column) with
+the CPU oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from lblrtm_amd import abi, lib, synth
+
+pytestmark = pytest.mark.gpu
+
+RADCN1 = 1.191042722E-12
+RADCN2 = 1.4387752
+BT_TOL = 0.01   # K, brightness temperature
+
+
+def bt(rad, v):
+    return RADCN2 * v / np.log1p(RADCN1 * v ** 3 / rad)
+
+
+def _layers(v1, v2):
+    atm = synth.atmosphere()
+    run = abi.make_run(v1, v2)
+    dv, it = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    lays = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l],
+                           dv[l], v1, v2) for l in range(len(dv))]
+    return run, lays, atm, it
+
+
+@pytest.mark.parametrize("pair", [(0, 1), (1, 2), (5, 6), (10, 11)])
+def test_xmerge_layer_pairs(gpu_ctx, pair):
+    """One merge step on the grids of two adjacent layers (ITYPE 0, 2, 4, 5)
+    with the same seeded optical depth and carry-over radiance on both."""
+    a, b = pair
+    tab = synth.config0_lines()
+    run, lays, atm, it = _layers(600.0, 1100.0)
+    La, Lb = lays[a], lays[b]
+    print("xmerge pair", pair, "itype", int(it[b]), flush=True)
+    O.od_layer(run, abi.Layer.from_buffer_copy(La), tab)
+    pan_a = O.panels()
+    O.od_layer(run, abi.Layer.from_buffer_copy(Lb), tab)
+    pan_b = O.panels()
+    na, nb = int(pan_a[2].sum()), int(pan_b[2].sum())
+    assert nb == lib.merge_size(run, La, Lb)
+    rng = np.random.default_rng(100 + b)
+    od = rng.exponential(0.4, nb)
+    ro = 1e-6 * (1.0 + rng.random(na))
+    to = rng.random(na)
+    bufs = [gpu_ctx.alloc(8 * n) for n in (nb, na, na, nb, nb)]
+    for buf, x in zip(bufs[:3], (od, ro, to)):
+        buf.from_host(x)
+    tzu = float(atm["tlev"][b + 1])
+    gpu_ctx.xmerge_device(run, La, Lb, tzu, 0.0, 0.0, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                          bufs[3].ptr, bufs[4].ptr)
+    rg, tg = bufs[3].to_host(np.float64), bufs[4].to_host(np.float64)
+    ref, q = O.merge_next(O.Merged(pan_a, La.dv, ro, to), od, pan_b, Lb.dv, Lb.tave, tzu, 1100.0)
+    assert q == 0
+    np.testing.assert_allclose(rg, ref.rad, rtol=1e-12)
+    np.testing.assert_allclose(tg, ref.tra, rtol=1e-12, atol=1e-300)
+    # the layer at the surface (EMINIT)
+    gpu_ctx.xmerge_device(run, None, Lb, tzu, 288.0, 0.98, bufs[0].ptr, None, None,
+                          bufs[3].ptr, bufs[4].ptr)
+    rg, tg = bufs[3].to_host(np.float64), bufs[4].to_host(np.float64)
+    ref = O.merge_first(od, pan_b, Lb.dv, Lb.tave, tzu, 288.0, 0.98)
+    np.testing.assert_allclose(rg, ref.rad, rtol=1e-12)
+    np.testing.assert_allclose(tg, ref.tra, rtol=1e-14)
+    for buf in bufs:
+        buf.free()
+
+
+def test_spectrum_column_workload(gpu_ctx):
+    """The 50-layer column, 500,000 lines over 500-2500 cm-1: TOA upwelling
+    radiance within 0.01 K brightness temperature of the oracle's OPDPTH +
+    XMERGE chain, and within 1e-10 of the oracle's merge of the GPU's own
+    layer optical depths."""
+    tab = synth.config1_lines()
+    gpu_ctx.load_lines(tab)
+    run, lays, atm, _ = _layers(500.0, 2500.0)
+    tzu = np.asarray(atm["tlev"][1:], np.float64)
+    tbnd, emis = float(atm["tlev"][0]), 0.98
+    work = [abi.Layer.from_buffer_copy(x) for x in lays]
+    rad, tra, v1p, dv = gpu_ctx.spectrum(run, work, tzu, tbnd, emis)
+    print("spectrum: top grid points", len(rad), "v1p", v1p, flush=True)
+    # the GPU's per-layer optical depths
+    sizes = [lib.od_size(run, L) for L in lays]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    buf = gpu_ctx.alloc(8 * int(sum(sizes)))
+    gpu_ctx.opdpth_column_device(run, [abi.Layer.from_buffer_copy(x) for x in lays], buf.ptr, offs)
+    allod = buf.to_host(np.float64)
+    buf.free()
+    jrad = 1
+    m_ref = m_mix = None
+    for l, L in enumerate(lays):
+        inp = abi.Layer.from_buffer_copy(L)
+        inp.jrad = jrad
+        od_ref, _, _, jrad = O.od_layer(run, inp, tab)
+        pan = O.panels()
+        od_gpu = allod[offs[l]:offs[l] + sizes[l]]
+        assert len(od_ref) == len(od_gpu)
+        if l == 0:
+            m_ref = O.merge_first(od_ref, pan, L.dv, L.tave, tzu[l], tbnd, emis)
+            m_mix = O.merge_first(od_gpu, pan, L.dv, L.tave, tzu[l], tbnd, emis)
+        else:
+            m_ref, q1 = O.merge_next(m_ref, od_ref, pan, L.dv, L.tave, tzu[l], 2500.0)
+            m_mix, q2 = O.merge_next(m_mix, od_gpu, pan, L.dv, L.tave, tzu[l], 2500.0)
+            assert q1 == 0 and q2 == 0
+    assert len(rad) == len(m_ref.rad)
+    np.testing.assert_allclose(rad, m_mix.rad, rtol=1e-10)
+    np.testing.assert_allclose(tra, m_mix.tra, rtol=1e-10, atol=1e-300)
+    v = v1p + np.arange(len(rad)) * dv
+    d = np.abs(bt(rad, v) - bt(m_ref.rad, v))
+    print("spectrum: max |dBT| %.3e K, max rel dR %.3e" % (d.max(), np.max(np.abs(rad - m_ref.rad) / m_ref.rad)),
+          flush=True)
+    assert d.max() <= BT_TOL
+    np.testing.assert_allclose(tra, m_ref.tra, rtol=1e-4, atol=1e-12)

@@ -1,0 +1,177 @@
+"""Pins for the oracle's layer merge (XMERGE, IEMIT=1 upwelling) and the
+host replay of RADMRG's panel walk -- CPU only.
+
+The merge is fixed by closed forms: an isothermal black column radiates
+B(T) whatever its optical depth; a transparent layer passes the radiance
+below unchanged; an opaque layer radiates B(Tave) + (B(TZU) - B(Tave))(1 - 2/tau); RADNN's 4-point Lagrange
+interpolation between grids reproduces a cubic exactly, which pins the
+phase walk (a slot off by one or a swapped weight breaks it); on equal
+grids the transmittance is the product of the layer transmittances.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from lblrtm_amd import abi, lib, synth
+
+RADCN1 = 1.191042722E-12
+RADCN2 = 1.4387752
+
+
+def planck(v, t):
+    return RADCN1 * v ** 3 / np.expm1(RADCN2 * v / t)
+
+
+def grid(v1, dv, n, npan=2400):
+    """Panel headers of a layer grid starting at v1 (panels of npan points)."""
+    v1p, v2p, nl = [], [], []
+    k = 0
+    while k < n:
+        m = min(npan, n - k)
+        v1p.append(v1 + k * dv)
+        v2p.append(v1 + (k + m - 1) * dv)
+        nl.append(m)
+        k += m
+    return np.array(v1p), np.array(v2p), np.array(nl, np.int32)
+
+
+def test_planck_matches_closed_form():
+    v = np.linspace(500.0, 2500.0, 101)
+    for t in (200.0, 250.0, 300.0):
+        got = np.array([O.lib().oracle_planck(x, t / RADCN2) for x in v])
+        np.testing.assert_allclose(got, planck(v, t), rtol=1e-13)
+
+
+def test_isothermal_black_column_radiates_planck():
+    """Tave = TZU = Tbnd, emissivity 1: R = B(T) for any optical depth, on
+    every layer of a merge chain across grids of ratio 3/2 and 5/4."""
+    t = 260.0
+    rng = np.random.default_rng(5)
+    v1, n0 = 800.0, 3 * 2400 + 77
+    dvs = [0.009, 0.006, 0.0048]
+    span = (n0 - 1) * dvs[0]
+    pans = [grid(v1, dv, int(round(span / dv)) + 1) for dv in dvs]
+    od = rng.exponential(0.5, int(pans[0][2].sum()))
+    m = O.merge_first(od, pans[0], dvs[0], t, t, t, 1.0)
+    v = v1 + np.arange(len(m.rad)) * dvs[0]
+    np.testing.assert_allclose(m.rad, planck(v, t), rtol=1e-12)
+    for k in (1, 2):
+        od = rng.exponential(0.5, int(pans[k][2].sum()))
+        m, q = O.merge_next(m, od, pans[k], dvs[k], t, t, v1 + span)
+        assert q == 0
+        v = v1 + np.arange(len(m.rad)) * dvs[k]
+        # interior: the 4-point interpolation of B(v) on the coarser grid
+        np.testing.assert_allclose(m.rad[8:-8], planck(v, t)[8:-8], rtol=1e-9)
+
+
+def test_transparent_and_opaque_layers():
+    v1, dv, n = 1000.0, 0.004, 5000
+    pan = grid(v1, dv, n)
+    v = v1 + np.arange(n) * dv
+    m = O.merge_first(np.zeros(n), pan, dv, 250.0, 240.0, 290.0, 0.9)
+    np.testing.assert_allclose(m.rad, 0.9 * planck(v, 290.0), rtol=1e-13)
+    np.testing.assert_array_equal(m.tra, 1.0)
+    m2, q = O.merge_next(m, np.zeros(n), pan, dv, 230.0, 220.0, v[-1])
+    assert q == 0
+    np.testing.assert_allclose(m2.rad, m.rad, rtol=1e-15)
+    tau = 800.0
+    m3, _ = O.merge_next(m2, np.full(n, tau), pan, dv, 230.0, 220.0, v[-1])
+    # opaque: linear-in-tau source, F = 1 - 2/tau once exp(-tau) underflows
+    ba, bu = planck(v, 230.0), planck(v, 220.0)
+    np.testing.assert_allclose(m3.rad, ba + (bu - ba) * (1.0 - 2.0 / tau), rtol=1e-12)
+    assert np.all(m3.tra == 0.0)
+
+
+def test_equal_grids_transmittance_product():
+    rng = np.random.default_rng(11)
+    v1, dv, n = 700.0, 0.002, 2400 * 2 + 13
+    pan = grid(v1, dv, n)
+    ods = [rng.exponential(0.3, n) for _ in range(3)]
+    m = O.merge_first(ods[0], pan, dv, 280.0, 275.0, 285.0, 0.95)
+    for od in ods[1:]:
+        m, q = O.merge_next(m, od, pan, dv, 260.0, 255.0, v1 + (n - 1) * dv)
+        assert q == 0
+    np.testing.assert_allclose(m.tra, np.exp(-sum(ods)), rtol=1e-13)
+
+
+@pytest.mark.parametrize("itype", [1, 2, 3, 4, 5])
+def test_radnn_interpolates_cubics_exactly(itype):
+    """Old radiance = cubic in the old point index, new layer transparent:
+    the merged radiance is the cubic at the new points' positions."""
+    LL = itype + 1
+    dvo = 0.012
+    dvk = dvo * itype / LL
+    v1 = 900.0
+    no = 2400 * 3 + 500
+    span = (no - 1) * dvo
+    nk = int(round(span / dvk)) + 1
+    po, pk = grid(v1, dvo, no), grid(v1, dvk, nk)
+    c = lambda j: 3.0 + 0.01 * j - 2e-6 * j ** 2 + 1e-10 * j ** 3
+    old = O.Merged(po, dvo, c(np.arange(no, dtype=float)), np.ones(no))
+    m, q = O.merge_next(old, np.zeros(nk), pk, dvk, 250.0, 250.0, v1 + span)
+    assert q == 0
+    i = np.arange(nk, dtype=float)
+    want = c(i * itype / LL)
+    np.testing.assert_allclose(m.rad[LL:-3 * LL], want[LL:-3 * LL], rtol=1e-12)
+    np.testing.assert_allclose(m.tra[LL:-3 * LL], 1.0, rtol=1e-14)
+
+
+def _col_layers(v1, v2, sel=None):
+    atm = synth.atmosphere()
+    run = abi.make_run(v1, v2)
+    dv, it = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    idx = range(len(dv)) if sel is None else sel
+    lays = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l],
+                           dv[l], v1, v2) for l in idx]
+    return run, lays, atm, it
+
+
+def test_merge_plan_accepts_the_column():
+    run, lays, _, it = _col_layers(500.0, 2500.0)
+    assert set(int(x) for x in it[1:]) <= {0, 2, 4, 5}
+    for a, b in zip(lays[:-1], lays[1:]):
+        assert lib.merge_size(run, a, b) == lib.od_size(run, b)
+
+
+def test_merge_plan_agrees_with_oracle_walk():
+    """The host replay of RADMRG refuses exactly the grids on which the
+    oracle's RADNN reads past the old points read (layers of unequal
+    extent), and accepts the column's own layer pairs."""
+    tab = synth.config0_lines()
+    run, lays, atm, _ = _col_layers(600.0, 1100.0, sel=range(7))
+    jrad = 1
+    prev = None
+    for l, L in enumerate(lays):
+        inp = abi.Layer.from_buffer_copy(L)
+        inp.jrad = jrad
+        od, v1p, _, jrad = O.od_layer(run, inp, tab)
+        pan = O.panels()
+        assert pan[0][0] == v1p and int(pan[2].sum()) == len(od)
+        if prev is None:
+            m = O.merge_first(od, pan, L.dv, L.tave, atm["tlev"][1], atm["tlev"][0], 0.98)
+        else:
+            m, q = O.merge_next(m, od, pan, L.dv, L.tave, atm["tlev"][l + 1], 1100.0)
+            assert q == 0, l
+            assert lib.merge_size(run, prev, L) == len(od)
+        prev = L
+    # an old layer that stops short of the new one: RADNN would read stale slots
+    short = abi.Layer.from_buffer_copy(lays[1])
+    short.v2 = 900.0
+    with pytest.raises(lib.LblxError) as e:
+        lib.merge_size(run, short, lays[2])
+    assert e.value.code == abi.E_UNSUPPORTED
+    v1o, v2o, nlo = grid(600.0, lays[1].dv, int(300.0 / lays[1].dv) + 1)
+    no = int(nlo.sum())
+    old = O.Merged((v1o, v2o, nlo), lays[1].dv, np.ones(no), np.ones(no))
+    nk = int(500.0 / lays[2].dv) + 1
+    _, q = O.merge_next(old, np.zeros(nk), grid(600.0, lays[2].dv, nk), lays[2].dv, 250.0, 250.0, 1100.0)
+    assert q > 0
+
+
+def test_coarser_layer_is_refused():
+    run, lays, _, _ = _col_layers(500.0, 2500.0, sel=[0, 2])
+    with pytest.raises(lib.LblxError) as e:
+        lib.merge_size(run, lays[1], lays[0])
+    assert e.value.code == abi.E_COARSER
