@@ -1,11 +1,13 @@
 """Throughput of the MI355X monochromatic path on the 50-layer column.
 
 Workload (one step = one spectrum): the 50-layer atmosphere, 500,000
-seeded synthetic lines over 500-2500 cm-1 in 7 molecules, every layer's
-optical depth (OPDPTH: LINF4, SHRINK, LBLF4 far wing, HIRAC1 near wing and
-panel combine) on the layer DV grids of PATH, FP64.  Inputs (line table,
-layer parameters) are resident on the device before the timed region;
-per-layer optical depths stay in HBM.
+seeded synthetic lines over 500-2500 cm-1 in 7 molecules, continuum on
+(ICNTNM=1), every layer's optical depth (OPDPTH: CONTNM, LINF4, SHRINK,
+LBLF4 far wing, HIRAC1 near wing and panel combine) on the layer DV grids
+of PATH, then XMERGE up the column to the upwelling radiance and
+transmittance at the top (surface 295 K, emissivity 0.98), FP64.  Inputs
+(line table, layer parameters) are resident on the device before the
+timed region; per-layer optical depths stay in HBM.
 
 Prints one JSON line (the bench contract) with the dominant kernel's
 roofline, measured with the library's HIP-event stage timers over the
@@ -34,6 +36,11 @@ LINE_BYTES = 36      # TAPE3 line read per layer (centre f64, six f32, mol)
 def stage_bytes(stage, stats, sizes, nwin):
     """Algorithmic HBM bytes of one step for a stage (sum over layers)."""
     npts = sum(sizes)
+    if stage == "merge":
+        # read the layer OD, read R and T of the layer below, write R and T
+        return 8 * npts + 16 * sum(sizes[:-1]) + 16 * npts
+    if stage == "contnm":
+        return 8 * 2007 * len(sizes)
     if stage == "tile_r1":
         # write OD, read R2 (npts/4) and the scatter record of every kept line
         return 8 * npts + 8 * npts / 4 + REC_BYTES * sum(s["lincnt"] for s in stats)
@@ -48,22 +55,28 @@ def stage_bytes(stage, stats, sizes, nwin):
     return LINE_BYTES * 500000 * len(sizes)
 
 
-def cpu_baseline(run, layers, tab):
-    """The oracle (single-threaded C restatement of OPDPTH) on the whole
-    spectrum: every layer of the column, one core."""
+def cpu_baseline(run, layers, tab, tzu, tbnd, emis):
+    """The oracle (single-threaded C restatement of OPDPTH and XMERGE) on
+    the whole spectrum: every layer of the column, one core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from lblrtm_amd import abi
     t0 = time.perf_counter()
     jrad = 1
-    for L in layers:
+    m = None
+    for l, L in enumerate(layers):
         lay = abi.Layer.from_buffer_copy(L)
         lay.jrad = jrad
-        _, _, _, jrad = O.od_layer(run, lay, tab)
+        od, _, _, jrad = O.od_layer(run, lay, tab)
+        pan = O.panels()
+        if m is None:
+            m = O.merge_first(od, pan, L.dv, L.tave, tzu[l], tbnd, emis)
+        else:
+            m, _ = O.merge_next(m, od, pan, L.dv, L.tave, tzu[l], run.v2)
     dt = time.perf_counter() - t0
     return {"value": 1.0 / dt, "unit": "spectra/s", "cores": 1, "kind": "port",
-            "sample": "one full spectrum (all 50 layers, 500k lines) with the C oracle on one "
-                      "host core, %.1f s" % dt}
+            "sample": "one full spectrum (all 50 layers, 500k lines, continuum, merge) with the "
+                      "C oracle on one host core, %.1f s" % dt}
 
 
 def main():
@@ -81,21 +94,24 @@ def main():
 
     tab = synth.config1_lines()
     atm = synth.atmosphere()
-    run = abi.make_run(500.0, 2500.0)
+    run = abi.make_run(500.0, 2500.0, icntnm=1)
     dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    tzu = np.asarray(atm["tlev"][1:], np.float64)
+    tbnd, emis = 295.0, 0.98
     nlay = len(dv)
     layers = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]),
                              atm["wbroad"][l], dv[l], 500.0, 2500.0) for l in range(nlay)]
     sizes = [lib.od_size(run, L) for L in layers]
-    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    ntop = sizes[-1]
 
     ctx = lib.Context(0)
     ctx.load_lines(tab)
-    buf = ctx.alloc(8 * int(sum(sizes)))
+    rad = ctx.alloc(8 * ntop)
+    tra = ctx.alloc(8 * ntop)
 
     def step(stats=False):
         lays = [abi.Layer.from_buffer_copy(x) for x in layers]
-        return ctx.opdpth_column_device(run, lays, buf.ptr, offs, stats=stats)
+        return ctx.spectrum_device(run, lays, tzu, tbnd, emis, rad.ptr, tra.ptr, ntop, stats=stats)
 
     _, stats = step(stats=True)
     for _ in range(a.warmup):
@@ -129,16 +145,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "50-layer column optical depth, 500k lines 500-2500 cm-1, 7 "
-                               "molecules, LBLF4 panels, FP64 (continuum and merge not in this step)",
-                   "layers": nlay, "lines": 500000, "points": int(sum(sizes)),
+        "config": {"workload": "50-layer column upwelling TOA spectrum: 500k lines 500-2500 cm-1, "
+                               "7 molecules, LBLF4 panels, continuum on, XMERGE radiance, FP64",
+                   "layers": nlay, "lines": 500000, "points": int(sum(sizes)), "top_points": ntop,
                    "stage_ms": {k: round(v, 4) for k, v in per.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(run, layers, tab)
+        out["cpu_baseline"] = cpu_baseline(run, layers, tab, tzu, tbnd, emis)
     print(json.dumps(out), flush=True)
 
 

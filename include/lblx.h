@@ -80,9 +80,13 @@ typedef struct lblx_run {
   double avmass;            /* AVMASS                                      */
   double dptmin, dptfac;    /* line rejection thresholds                   */
   int32_t ilblf4;           /* ILBLF4 0/1/2                                */
-  int32_t icntnm;           /* ICNTNM (0 = continuum off)                  */
+  int32_t icntnm;           /* ICNTNM 0..6 (lblrtm.f90:688-760)            */
   int32_t irad;             /* IRAD != 0 -> JRAD = -1 (lblrtm.f90:832-833) */
   int32_t precision;        /* LBLX_FP64 or LBLX_FP32_SPLIT                */
+  double cntscl[7];         /* XSELF XFRGN XCO2C XO3CN XO2CN XN2CN XRAYL,
+                               read when icntnm = 6 (record 1.2a)          */
+  int32_t ifrgnx;           /* IFRGNX = 1: MT_CKD foreign closure table    */
+  int32_t iaersl;           /* IAERSL (Rayleigh only for 0)                */
 } lblx_run;
 
 /* One layer, as /FILHDR/ holds it when OPDPTH is called.                */
@@ -160,6 +164,16 @@ int lblx_od_layer_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layer,
 int lblx_od_column_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers,
                           int nlay, void *d_od, const int64_t *od_off, double *v1p,
                           lblx_stats *st);
+
+/* CONTNM (src/contnm.f90:20-990) for one layer: ABSRB(1..nptabs) on the
+ * 1 cm-1 grid v1abs + (k-1)*dvabs of OPDPTH (src/lblrtm.f90:6214-6226),
+ * with the radiation flag OPDPTH sets (layer->jrad is updated).  The same
+ * ABSRB enters every lblx_od_* call with run->icntnm != 0.  Continua of
+ * 0..4340 cm-1 (MT_CKD H2O, CO2, O2 and N2 fundamentals, Rayleigh); bands
+ * reaching the N2 rotational (< 350 cm-1) or any continuum above 4340
+ * cm-1 return LBLX_E_UNSUPPORTED.                                        */
+int lblx_contnm(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layer, double *absrb,
+                int cap, int *nptabs, double *v1abs, double *dvabs);
 
 /* XMERGE for one layer, IEMIT=1 upwelling (IPATHL=1), LTE, no aerosols
  * (xmerge.f90:25-167 -> EMINIT/RADMRG, EMIN, RADNN).  layer_old == NULL:

@@ -60,6 +60,9 @@ class Run(C.Structure):
         ("icntnm", C.c_int32),
         ("irad", C.c_int32),
         ("precision", C.c_int32),
+        ("cntscl", C.c_double * 7),
+        ("ifrgnx", C.c_int32),
+        ("iaersl", C.c_int32),
     ]
 
 
@@ -165,12 +168,16 @@ class LineTable:
 
 
 def make_run(v1, v2, sample=4.0, alfal0=0.04, avmass=36.0, dptmin=2e-4,
-             dptfac=1e-3, ilblf4=1, icntnm=0, irad=0, precision=FP64):
+             dptfac=1e-3, ilblf4=1, icntnm=0, irad=0, precision=FP64, cntscl=None,
+             ifrgnx=0, iaersl=0):
     r = Run()
     r.v1, r.v2 = v1, v2
     r.sample, r.alfal0, r.avmass = sample, alfal0, avmass
     r.dptmin, r.dptfac = dptmin, dptfac
     r.ilblf4, r.icntnm, r.irad, r.precision = ilblf4, icntnm, irad, precision
+    for i, x in enumerate(cntscl if cntscl is not None else [0.0] * 7):
+        r.cntscl[i] = x
+    r.ifrgnx, r.iaersl = ifrgnx, iaersl
     return r
 
 

@@ -17,6 +17,7 @@
 #include "../../include/lblx.h"
 #include "lblx_kernels.h"
 #include "lblx_plan.h"
+#include "lblx_contdata.h"
 #include "lblx_refdata.h"
 
 // The kernels are compiled in this translation unit (one HIP module).
@@ -129,6 +130,9 @@ struct lblx_ctx {
   // device line table
   DevBuf d_vnu, d_sp, d_alfa, d_epp, d_hwhm, d_tmpalf, d_pshift, d_mol, d_blk, d_bstart, d_bvmax;
   DevBuf d_tabs;  // F1 F2 F3 FG XVER
+  DevBuf d_cont;  // continuum tables (ContTabs)
+  ContTabs cont_tabs{};
+  DevBuf w_cps;   // ContPlan per job
   // work buffers
   DevBuf w_dlay, w_pan, w_win, w_seg, w_r4, w_margin;
   DevBuf w_cx, w_csp, w_cal, w_cad, w_cco2, w_pn, w_pc, w_g, w_jmp, w_wsum, w_bcnt, w_etab, w_jobi,
@@ -206,6 +210,33 @@ int lblx_create(lblx_ctx **out, int device) {
     return LBLX_E_HIP;
   }
   HIPCHK(hipMemcpy(c->d_tabs.p, tabs.data(), tabs.size() * sizeof(double), hipMemcpyHostToDevice));
+  // continuum tables, concatenated in ContTabs order
+  {
+    const std::pair<const double *, size_t> parts[] = {
+        {LBLX_MTCKD_WVN, LBLX_MTCKD_N}, {LBLX_MTCKD_SELF, LBLX_MTCKD_N},
+        {LBLX_MTCKD_FOR, LBLX_MTCKD_N}, {LBLX_MTCKD_FORCL, LBLX_MTCKD_N},
+        {LBLX_MTCKD_TEXP, LBLX_MTCKD_N}, {LBLX_FCO2, 5003}, {LBLX_XFACCO2, 500},
+        {LBLX_CO2_TDEP, 25}, {LBLX_O2F_XO2, 103}, {LBLX_O2F_XO2T, 103},
+        {LBLX_N2F_272, 228}, {LBLX_N2F_228, 228}, {LBLX_N2F_AH2O, 228}};
+    std::vector<double> all;
+    std::vector<size_t> off;
+    for (auto &q : parts) {
+      off.push_back(all.size());
+      all.insert(all.end(), q.first, q.first + q.second);
+    }
+    if (c->d_cont.ensure(all.size() * sizeof(double))) {
+      delete c;
+      return LBLX_E_HIP;
+    }
+    HIPCHK(hipMemcpy(c->d_cont.p, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+    const double *b = c->d_cont.as<double>();
+    const double **dst[] = {&c->cont_tabs.wvn, &c->cont_tabs.self, &c->cont_tabs.forr,
+                            &c->cont_tabs.forcl, &c->cont_tabs.texp, &c->cont_tabs.fco2,
+                            &c->cont_tabs.xfac, &c->cont_tabs.tdep, &c->cont_tabs.o2x,
+                            &c->cont_tabs.o2t, &c->cont_tabs.n2a, &c->cont_tabs.n2b,
+                            &c->cont_tabs.n2h};
+    for (size_t k = 0; k < off.size(); k++) *dst[k] = b + off[k];
+  }
   *out = c;
   return LBLX_OK;
 }
@@ -214,7 +245,7 @@ void lblx_destroy(lblx_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   DevBuf *bufs[] = {&c->d_vnu, &c->d_sp, &c->d_alfa, &c->d_epp, &c->d_hwhm, &c->d_tmpalf,
-                    &c->d_pshift, &c->d_mol, &c->d_blk, &c->d_bstart, &c->d_bvmax, &c->d_tabs,
+                    &c->d_pshift, &c->d_mol, &c->d_blk, &c->d_bstart, &c->d_bvmax, &c->d_tabs, &c->d_cont, &c->w_cps,
                     &c->w_dlay, &c->w_pan, &c->w_win, &c->w_seg, &c->w_r4, &c->w_margin,
                     &c->w_cx, &c->w_csp, &c->w_cal, &c->w_cad, &c->w_cco2, &c->w_pn, &c->w_pc,
                     &c->w_g, &c->w_jmp, &c->w_wsum, &c->w_bcnt, &c->w_etab, &c->w_jobi, &c->w_bat,
@@ -313,7 +344,6 @@ int lblx_od_size(const lblx_run *run, const lblx_layer *layer, int64_t *npts) {
 static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int njobs, bool chain_jrad,
                     void *od_base, const int64_t *od_off, bool f32out, double *v1p, lblx_stats *st) {
   if (c->nlines <= 0) return LBLX_E_NOLINES;
-  if (run->icntnm != 0) return LBLX_E_UNSUPPORTED;  // continuum: separate entry
   if (njobs <= 0) return LBLX_E_ARG;
   hipStream_t s = c->stream;
   const int n = c->nlines, nblk = c->nblk;
@@ -324,7 +354,8 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   std::vector<DevSeg> seg;
   std::vector<double> margins(3 * njobs);
   int64_t r2tot = 0, r3tot = 0;
-  int maxnn = 0, maxnbk = 0;
+  int maxnn = 0, maxnbk = 0, abs_stride = 0;
+  std::vector<ContPlan> cps(njobs);
   int prev_jrad = layers[0].jrad;
   for (int jb = 0; jb < njobs; jb++) {
     lblx_layer *ly = &layers[jb];
@@ -372,7 +403,13 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
     Y.n1 = P.n1;
     Y.n2 = P.n2;
     Y.n3 = P.n3;
-    Y.icntnm = 0;
+    cps[jb] = P.cont;
+    Y.icntnm = P.cont.on ? 1 : 0;
+    Y.v1abs = P.cont.v1abs;
+    Y.v2abs = P.cont.v2abs;
+    Y.dvabs = P.cont.dvabs;
+    Y.nptabs = P.cont.nptabs;
+    if (P.cont.on) abs_stride = std::max(abs_stride, P.cont.nptabs + 4);
     Y.pan0 = (int)pan.size();
     Y.win0 = (int)win.size();
     Y.seg0 = (int)seg.size();
@@ -412,6 +449,8 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   }
   if (seg.empty()) seg.push_back(DevSeg{0, 0, 0., 0.});
   const int nwin = (int)win.size();
+  abs_stride = (abs_stride + 63) / 64 * 64;
+  for (int jb = 0; jb < njobs; jb++) ys[jb].abs0 = (int64_t)jb * abs_stride;
   // tile maps
   const int tiles[3] = {2048, 1024, 512};
   std::vector<int32_t> tjob[3], tloc[3];
@@ -450,7 +489,9 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
       need(c->w_blkmax, NJ * S.blk * 4) || need(c->w_pread, NJ * S.blk * 4) ||
       need(c->w_changed, 16) || need(c->w_rec, NJ * S.line * sizeof(ScatterRec)) ||
       need(c->w_r3, (size_t)r3tot * 8) || need(c->w_r2, (size_t)r2tot * 8) ||
-      need(c->w_cnt, NJ * 24) || need(c->w_margin, NJ * 3 * 8)) {
+      need(c->w_cnt, NJ * 24) || need(c->w_margin, NJ * 3 * 8) ||
+      (abs_stride > 0 && (need(c->w_absrb, NJ * abs_stride * 8) ||
+                          need(c->w_cps, NJ * sizeof(ContPlan))))) {
     set_err("work buffer allocation failed");
     return LBLX_E_HIP;
   }
@@ -461,6 +502,16 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   HIPCHK(hipMemcpyAsync(c->w_win.p, win.data(), win.size() * sizeof(DevWin), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(c->w_seg.p, seg.data(), seg.size() * sizeof(DevSeg), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(c->w_margin.p, margins.data(), margins.size() * 8, hipMemcpyHostToDevice, s));
+  if (abs_stride > 0) {  // CONTNM: ABSRB rows of every job
+    HIPCHK(hipMemcpyAsync(c->w_cps.p, cps.data(), NJ * sizeof(ContPlan), hipMemcpyHostToDevice, s));
+    int thc = c->tbegin(ST_CONTNM);
+    int64_t nth = NJ * abs_stride;
+    hipLaunchKernelGGL(k_contnm, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s,
+                       c->w_cps.as<ContPlan>(), c->cont_tabs, abs_stride, njobs,
+                       c->w_absrb.as<double>());
+    c->tend(thc);
+    HIPCHK(hipGetLastError());
+  }
   TileMap tm[3];
   for (int G = 0; G < 3; G++) {
     size_t nt = tjob[G].size();
@@ -590,7 +641,8 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
       hipLaunchKernelGGL((k_tile<3, double>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
                          tm[2], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                         c->w_r4.as<double>(), (const double *)nullptr, c->w_seg.as<DevSeg>(),
+                         c->w_r4.as<double>(), (const double *)(abs_stride > 0 ? c->w_absrb.p : nullptr),
+                         c->w_seg.as<DevSeg>(),
                          (const double *)nullptr, c->w_r3.as<double>(), (double *)nullptr);
     } else if (G == 2) {
       HIPCHK(hipFuncSetAttribute((const void *)k_tile<2, double>,
@@ -700,6 +752,33 @@ int lblx_debug_tape9(lblx_ctx *c, int job, double *vnu, double *sp, double *alfa
     HIPCHK(hipMemcpy(epp, c->w_t9epp.as<double>() + o, k * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(mol, c->w_t9mol.as<int32_t>() + o, k * 4, hipMemcpyDeviceToHost));
   }
+  return LBLX_OK;
+}
+
+int lblx_contnm(lblx_ctx *c, const lblx_run *run, lblx_layer *layer, double *absrb, int cap,
+                int *nptabs, double *v1abs, double *dvabs) {
+  if (!c || !run || !layer || !absrb || !nptabs) return LBLX_E_ARG;
+  if (run->icntnm == 0) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  LayerPlan P;
+  int rc = plan_layer(run, layer, &P);
+  if (rc) return rc;
+  const ContPlan &C = P.cont;
+  if (cap < C.nptabs) return LBLX_E_SIZE;
+  int stride = (C.nptabs + 4 + 63) / 64 * 64;
+  if (c->w_cps.ensure(sizeof(ContPlan)) || c->w_absrb.ensure((size_t)stride * 8)) return LBLX_E_HIP;
+  HIPCHK(hipMemcpyAsync(c->w_cps.p, &C, sizeof(ContPlan), hipMemcpyHostToDevice, c->stream));
+  int th = c->tbegin(ST_CONTNM);
+  hipLaunchKernelGGL(k_contnm, dim3((stride + 255) / 256), dim3(256), 0, c->stream, c->w_cps.as<ContPlan>(),
+                     c->cont_tabs, stride, 1, c->w_absrb.as<double>());
+  c->tend(th);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(absrb, c->w_absrb.as<double>() + 1, (size_t)C.nptabs * 8, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *nptabs = C.nptabs;
+  if (v1abs) *v1abs = C.v1abs;
+  if (dvabs) *dvabs = C.dvabs;
   return LBLX_OK;
 }
 

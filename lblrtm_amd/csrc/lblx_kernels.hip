@@ -1077,6 +1077,156 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
 }
 
 // ---------------------------------------------------------------------
+// CONTNM: ABSRB(1..NPTABS) of every job on its 1 cm-1 grid, one thread
+// per point.  Each continuum is XINT (Keys cubic convolution) of its
+// coarse array, whose four needed values are evaluated from the tables
+// in place (FRNCO2 / o2_ver_1 / n2_ver_1 / mt_ckd_h2o_absco); the terms
+// are added in CONTNM's order: H2O self and foreign, CO2, O2, N2,
+// Rayleigh (src/contnm.f90:296-977).
+struct ContTabs {
+  const double *wvn, *self, *forr, *forcl, *texp;  // MT_CKD, LBLX_MTCKD_N
+  const double *fco2, *xfac, *tdep;                // 5003, 500, 25
+  const double *o2x, *o2t;                         // 103
+  const double *n2a, *n2b, *n2h;                   // 228
+};
+
+__device__ __forceinline__ double d_mt_radfn(double vi, double xkt) {  // myradfn
+  double x = vi / xkt;
+  if (x <= 0.01) return 0.5 * x * vi;
+  if (x <= 10.) {
+    double e = exp(-x);
+    return vi * (1. - e) / (1. + e);
+  }
+  return vi;
+}
+
+// XINT / myxint at ABSRB point I with the coarse values from A(j).
+template <class F>
+__device__ __forceinline__ double xint_fn(double v1a, double v2a, double dva, F A, double vft,
+                                          double dvr3, int I, int n1r3, int n2r3) {
+  double recdva = 1. / dva;
+  int ilo = fint((v1a + dva - vft) / dvr3 + 1. + ONEMI_D);
+  if (ilo < n1r3) ilo = n1r3;
+  int ihi = fint((v2a - dva - vft) / dvr3 + ONEMI_D);
+  if (ihi > n2r3) ihi = n2r3;
+  if (I < ilo || I > ihi) return 0.;
+  double vi = vft + dvr3 * (double)(I - 1);
+  int j = fint((vi - v1a) * recdva + ONEPL_D);
+  double vj = v1a + dva * (double)(j - 1);
+  double p = recdva * (vi - vj);
+  double c = (3. - 2. * p) * p * p;
+  double b = 0.5 * p * (1. - p);
+  double b1 = b * (1. - p);
+  double b2 = b * p;
+  return -A(j - 1) * b1 + A(j) * (1. - c + b2) + A(j + 1) * (c + b1) - A(j + 2) * b2;
+}
+
+__global__ __launch_bounds__(256) void k_contnm(const ContPlan *cps, ContTabs TB, int stride,
+                                                int njobs, double *absrb) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)stride * njobs) return;
+  const int job = (int)(t / stride);
+  const int i = (int)(t - (int64_t)job * stride);
+  const ContPlan &C = cps[job];
+  double ab = 0.;
+  if (!C.on || i < 1 || i > C.nptabs) {
+    absrb[t] = 0.;
+    return;
+  }
+  const double xkt = C.xkt;
+  const int jrad = C.jrad;
+  if (C.h2o.on) {
+    const ContTerm &H = C.h2o;
+    const double *fortab = C.forcl ? TB.forcl : TB.forr;
+    auto self = [&](int k) -> double {
+      if (k < 1 || k > H.nptc) return 0.;
+      int q = H.i1 + k - 2;
+      double v = TB.self[q] * pow(H.f2, TB.texp[q]);
+      v = v * H.f0 * H.f1;
+      if (H.f3 != 0.) v = v * d_mt_radfn(TB.wvn[q], xkt);
+      return v;
+    };
+    auto forn = [&](int k) -> double {
+      if (k < 1 || k > H.nptc) return 0.;
+      int q = H.i1 + k - 2;
+      double v = fortab[q] * (1 - H.f0) * H.f1;
+      if (H.f3 != 0.) v = v * d_mt_radfn(TB.wvn[q], xkt);
+      return v;
+    };
+    double sh = 0. + xint_fn(H.v1c, H.v2c, H.dvc, self, C.v1abs, C.dvabs, i, H.ist, H.last) * 1.0;
+    double fh = 0. + xint_fn(H.v1c, H.v2c, H.dvc, forn, C.v1abs, C.dvabs, i, H.ist, H.last) * 1.0;
+    double cself = C.wk1 * sh * C.xself;
+    double cfor = C.wk1 * fh * C.xfrgn;
+    ab = ab + cself + cfor;
+  }
+  if (C.co2.on) {
+    const ContTerm &K = C.co2;
+    auto co2 = [&](int J) -> double {
+      if (J < 1 || J > K.nptc) return 0.;
+      int I = K.i1 + (J - 1);
+      double f = 0.;
+      if (I >= 1 && I <= 5003) {
+        double tcor = 1.;
+        if (I >= 1196 && I <= 1220) tcor = pow(K.f1, TB.tdep[I - 1196]);
+        f = tcor * TB.fco2[I - 1];
+      }
+      double VJ = K.v1c + K.dvc * (double)(J - 1);
+      double cfac = 1.;
+      if (VJ >= 2000. && VJ <= 2998.) cfac = TB.xfac[fint((VJ - 1998.) / 2. + 0.00001) - 1];
+      f = cfac * f;
+      double c = f * K.f0;
+      if (jrad == 1) c = c * d_radfn(VJ, xkt);
+      return c;
+    };
+    ab = ab + xint_fn(K.v1c, K.v2c, K.dvc, co2, C.v1abs, C.dvabs, i, K.ist, K.last) * 1.0;
+  }
+  if (C.o2.on) {
+    const ContTerm &K = C.o2;
+    auto o2 = [&](int J) -> double {
+      if (J < 1 || J > K.nptc) return 0.;
+      int I = K.i1 + (J - 1);
+      double VJ = K.v1c + K.dvc * (double)(J - 1);
+      double c0 = 0.;
+      if (I >= 1 && I <= 103) c0 = K.f2 * TB.o2x[I - 1] * exp(TB.o2t[I - 1] * K.f1) / VJ;
+      double c = K.f0 * c0;
+      if (jrad == 1) c = c * d_radfn(VJ, xkt);
+      return c;
+    };
+    ab = ab + xint_fn(K.v1c, K.v2c, K.dvc, o2, C.v1abs, C.dvabs, i, K.ist, K.last) * 1.0;
+  }
+  if (C.n2.on) {
+    const ContTerm &K = C.n2;
+    auto n2 = [&](int J) -> double {
+      if (J < 1 || J > K.nptc) return 0.;
+      int I = K.i1 + (J - 1);
+      double VJ = K.v1c + K.dvc * (double)(J - 1);
+      double c0 = 0., c1 = 0., c2 = 0.;
+      if (I >= 1 && I <= 228) {
+        double a = TB.n2a[I - 1], b = TB.n2b[I - 1];
+        if (a > 0. && b > 0.) c0 = a * pow(b / a, K.f1);
+        else c0 = a + (b - a) * K.f2;
+        c0 = c0 / VJ;
+        c1 = K.f3 * c0;
+        c2 = (9. / 7.) * TB.n2h[I - 1] * c0;
+      }
+      double c = K.f0 * (K.f4 * c0 + K.f5 * c1 + K.f6 * c2);
+      if (jrad == 1) c = c * d_radfn(VJ, xkt);
+      return c;
+    };
+    ab = ab + xint_fn(K.v1c, K.v2c, K.dvc, n2, C.v1abs, C.dvabs, i, K.ist, K.last) * 1.0;
+  }
+  if (C.ray_on) {
+    double vr = C.v1abs + (double)(i - 1) * C.dvabs;
+    double x = vr / 1.e4;
+    double ray = (x * x * x / (9.38076E2 - 10.8426 * (x * x))) * C.ray_f;
+    if (jrad == 1) ray = ray * x;
+    else if (jrad == 0) ray = ray * x / d_radfn(vr, xkt);
+    ab = ab + ray;
+  }
+  absrb[t] = ab;
+}
+
+// ---------------------------------------------------------------------
 // XMERGE for upwelling radiance (IEMIT=1, IPATHL=1, LTE, no aerosols).
 // EMIN (xmerge.f90:1850-1879): T = exp(-tau), E = (1-T)(B(Tave) + (B(Tzu)
 // - B(Tave)) F(tau)) with F = tau/6 for tau <= 0.06.  First layer (EMINIT,

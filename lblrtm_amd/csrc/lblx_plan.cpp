@@ -1,9 +1,11 @@
 // Host planner: see lblx_plan.h.  Reference lines are cited per block.
 #include "lblx_plan.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
+#include "lblx_contdata.h"
 #include "lblx_refdata.h"
 
 namespace lblx {
@@ -194,6 +196,138 @@ static void add_window(LayerPlan *P, double v1, double v2, int p_first) {
   P->wins.push_back(w);
 }
 
+// pre_xint (src/contnm.f90:994-1012)
+static void pre_xint(double v1ss, double v2ss, double v1abs, double dvabs, int nptabs, int *ist,
+                     int *last) {
+  int nb1 = fint(2. + (v1ss - v1abs) / dvabs + 1.e-5);
+  *ist = std::max(1, nb1);
+  int nb2 = fint(1. + (v2ss - v1abs) / dvabs + 1.e-5);
+  *last = std::min(nptabs, nb2);
+}
+
+// Coarse walk shared by FRNCO2 / o2_ver_1 / n2_ver_1 (contnm.f90:1312-1326).
+static void cont_walk(const ContPlan &C, const double *hdr, ContTerm *T) {
+  const double v1s = hdr[0], v2s = hdr[1], dvs = hdr[2];
+  const int npts = (int)hdr[3];
+  double V1C = C.v1abs - dvs, V2C = C.v2abs + dvs;
+  int I1 = V1C < v1s ? -1 : fint((V1C - v1s) / dvs + 0.01);
+  V1C = v1s + dvs * (double)(I1 - 1);
+  int I2 = fint((V2C - v1s) / dvs + 0.01);
+  int NPTC = I2 - I1 + 3;
+  if (NPTC > npts) NPTC = npts + 4;
+  T->on = 1;
+  T->i1 = I1;
+  T->nptc = NPTC;
+  T->v1c = V1C;
+  T->v2c = V1C + dvs * (double)(NPTC - 1);
+  T->dvc = dvs;
+  pre_xint(v1s, v2s, C.v1abs, C.dvabs, C.nptabs, &T->ist, &T->last);
+}
+
+// CONTNM scalars for the continua of 0..4340 cm-1 (H2O MT_CKD, CO2,
+// O2 and N2 fundamentals, Rayleigh).
+static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPlan *C) {
+  *C = ContPlan{};
+  double X[7] = {1, 1, 1, 1, 1, 1, 1};
+  switch (run->icntnm) {  // lblrtm.f90:705-760
+    case 0: return LBLX_OK;
+    case 1: break;
+    case 2: X[0] = 0.; break;
+    case 3: X[1] = 0.; break;
+    case 4: X[0] = X[1] = 0.; break;
+    case 5: X[6] = 0.; break;
+    case 6: for (int k = 0; k < 7; k++) X[k] = run->cntscl[k]; break;
+    default: return LBLX_E_ARG;
+  }
+  const double xself = X[0], xfrgn = X[1], xco2c = X[2], xo3cn = X[3], xo2cn = X[4],
+               xn2cn = X[5], xrayl = X[6];
+  const double V1 = Ly->v1, V2 = Ly->v2, PAVE = Ly->pave, TAVE = Ly->tave;
+  if ((V2 > -10.0 && V1 < 350. && xn2cn > 0.) ||
+      (V2 > 4340.0 && (xn2cn > 0. || xo2cn > 0. || xo3cn > 0.)))
+    return LBLX_E_UNSUPPORTED;
+  C->on = 1;
+  // OPDPTH ABSRB grid (lblrtm.f90:6216-6221)
+  C->dvabs = 1.;
+  double a = (double)fint(V1);
+  if (V1 < 0.) a = a - 1.;
+  C->v1abs = a - 3. * C->dvabs;
+  C->v2abs = (double)fint(V2 + 3. * C->dvabs + 0.5);
+  C->nptabs = fint((C->v2abs - C->v1abs) / C->dvabs + 1.5);
+  if (C->nptabs + 4 > 5050) return LBLX_E_SIZE;  // n_absrb (lblparams.f90:36)
+  C->jrad = jrad;
+  C->tave = TAVE;
+  const double P0c = 1013., T0c = 296., XLOSMT = 2.68675E+19;
+  const double RHOAVE = (PAVE / P0c) * (T0c / TAVE);
+  C->xkt = TAVE / RADCN2;
+  const double amagat = (PAVE / P0c) * (273. / TAVE);
+  double WTOT = Ly->wbroad;
+  for (int m = 0; m < Ly->nmol; m++) WTOT = WTOT + Ly->wk[m];
+  const double wk1 = Ly->nmol >= 1 ? Ly->wk[0] : 0., wk2 = Ly->nmol >= 2 ? Ly->wk[1] : 0.;
+  const double wk7 = Ly->nmol >= 7 ? Ly->wk[6] : 0.;
+  const double x_vmr_h2o = wk1 / WTOT, x_vmr_o2 = wk7 / WTOT;
+  const double x_vmr_n2 = 1. - x_vmr_h2o - x_vmr_o2;
+  const double wn2 = x_vmr_n2 * WTOT;
+  // H2O (contnm.f90:296-327, mt_ckd_h2o_module.f90:68-176)
+  if (V2 > -20.0 && V1 < 20000. && (xself > 0. || xfrgn > 0.)) {
+    ContTerm &T = C->h2o;
+    const double *W = LBLX_MTCKD_WVN;
+    const double dvc = W[1] - W[0];
+    int i = 1;
+    while (W[i - 1] <= C->v1abs - 2 * dvc) i++;
+    int i1 = i - 1;
+    if (i1 == 0) i1 = 1;
+    while (W[i - 1] < C->v2abs + 2 * dvc) i++;
+    T.on = 1;
+    T.i1 = i1;                 // 1-based table index of coefficient 1
+    T.nptc = i - i1 + 1;       // ncoeff
+    T.v1c = W[i1 - 1];
+    T.v2c = W[i - 1];
+    T.dvc = dvc;
+    int npt_l = fint((C->v2abs - C->v1abs) / C->dvabs + 1);
+    pre_xint(W[0], W[LBLX_MTCKD_N - 1], C->v1abs, C->dvabs, npt_l, &T.ist, &T.last);
+    T.f0 = wk1 / WTOT;                                                 // h2o_vmr
+    T.f1 = (PAVE / LBLX_MTCKD_REF_P) * (LBLX_MTCKD_REF_T / TAVE);      // rho_rat
+    T.f2 = LBLX_MTCKD_REF_T / TAVE;
+    T.f3 = jrad != 0 ? 1. : 0.;                                        // radflag
+    C->xself = xself;
+    C->xfrgn = xfrgn;
+    C->wk1 = wk1;
+    C->forcl = run->ifrgnx == 1;
+  }
+  // CO2 (contnm.f90:330-353)
+  if (V2 > -20.0 && V1 < 10000. && xco2c > 0.) {
+    cont_walk(*C, LBLX_FCO2_HDR, &C->co2);
+    C->co2.f0 = wk2 * RHOAVE * 1.0E-20 * xco2c;  // WCO2
+    C->co2.f1 = TAVE / 246.;                     // trat (FRNCO2 t_eff)
+  }
+  // O2 fundamental (contnm.f90:583-597)
+  if (V2 > 1340.0 && V1 < 1850. && xo2cn > 0.) {
+    cont_walk(*C, LBLX_O2F_HDR, &C->o2);
+    C->o2.f0 = xo2cn * wk7 * 1.e-20 * amagat;    // tau_fac
+    C->o2.f1 = (1. / 296.) - (1. / TAVE);        // xktfac
+    C->o2.f2 = (1.e+20 / XLOSMT);                // factor
+  }
+  // N2 fundamental (contnm.f90:734-753)
+  if (V2 > 2001.77 && V1 < 2897.59 && xn2cn > 0.) {
+    cont_walk(*C, LBLX_N2F_HDR, &C->n2);
+    const double T = TAVE, T_272 = 272., T_228 = 228.;
+    C->n2.f0 = xn2cn * (wn2 / XLOSMT) * amagat;  // tau_fac
+    C->n2.f1 = ((1. / T) - (1. / T_272)) / ((1. / T_228) - (1. / T_272));  // xtfac
+    C->n2.f2 = (T - T_272) / (T_228 - T_272);    // xt_lin
+    C->n2.f3 = 1.294 - 0.4545 * T / 296.;         // a_o2
+    C->n2.f4 = x_vmr_n2;
+    C->n2.f5 = x_vmr_o2;
+    C->n2.f6 = x_vmr_h2o;
+  }
+  // Rayleigh (contnm.f90:955-977)
+  if (run->iaersl == 0 && V2 >= 820. && xrayl > 0.) {
+    C->ray_on = 1;
+    double conv = xrayl * 1.E-20 / (2.68675e-1 * 1.e5);
+    C->ray_f = WTOT * conv;
+  }
+  return LBLX_OK;
+}
+
 int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P) {
   if (!run || !Ly || !P) return LBLX_E_ARG;
   if (Ly->nmol < 1 || Ly->nmol > 7) return LBLX_E_UNSUPPORTED;
@@ -212,6 +346,10 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P) {
   P->xkt = xkt_l;
   P->tave = Ly->tave;
   P->pave = Ly->pave;
+  if (run->icntnm != 0) {
+    int rc = plan_cont(run, Ly, P->jrad, &P->cont);
+    if (rc) return rc;
+  }
   int ipflag = (run->icntnm != 0 && Ly->pave <= 0.5) ? 1 : 0;
   double alfav = SAMPLE * DV;
   double alfav4 = 64. * alfav;

@@ -47,6 +47,28 @@ struct WinPlan {
   int32_t p_first;     // first panel that uses this window
 };
 
+// CONTNM set-up of one layer (src/contnm.f90:20-990, OPDPTH grid
+// src/lblrtm.f90:6214-6226): the ABSRB grid and, per continuum, the
+// coarse-grid walk (V1C, NPTC, first table index I1), the pre_xint clip
+// [ist, last] and the layer scalars that multiply the tables.
+struct ContTerm {
+  int32_t on;
+  int32_t i1, nptc, ist, last;  // I1 (table index of coarse point 1), NPTC, clip
+  double v1c, v2c, dvc;
+  double f0, f1, f2, f3, f4, f5, f6, f7;  // term scalars (see plan_cont)
+};
+struct ContPlan {
+  int32_t on;
+  double v1abs, v2abs, dvabs;
+  int32_t nptabs, jrad;
+  double xkt, tave;
+  ContTerm h2o, co2, o2, n2;
+  int32_t ray_on;
+  double ray_f;      // WTOT * conv_cm2mol
+  double xself, xfrgn, wk1;
+  int32_t forcl;     // MT_CKD foreign closure table
+};
+
 struct LayerPlan {
   // grid
   double dv, dvr2, dvr3, dvr4, bound4, alfmax, bound;
@@ -69,6 +91,7 @@ struct LayerPlan {
   int64_t g2max;      // last R2 index that receives L4(R3)
   int64_t g3max;      // last R3 index that receives XINT
   int64_t n1, n2, n3; // array sizes (fine, R2, R3 grids)
+  ContPlan cont;      // ICNTNM != 0
 };
 
 // Build the plan; layer->jrad is updated as OPDPTH does.  Returns LBLX_OK

@@ -64,6 +64,8 @@ def load():
         "lblx_dev_alloc": (C.c_void_p, [C.c_void_p, C.c_int64]),
         "lblx_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
         "lblx_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
+        "lblx_contnm": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(C.c_double), C.c_int,
+                                  P(C.c_int), P(C.c_double), P(C.c_double)]),
         "lblx_merge_size": (C.c_int, [P(abi.Run), P(abi.Layer), P(abi.Layer), P(C.c_int64)]),
         "lblx_xmerge_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer),
                                          C.c_double, C.c_double, C.c_double, C.c_void_p,
@@ -191,6 +193,18 @@ class Context:
         for k in range(n):
             layers[k].jrad = arr[k].jrad
         return list(v1p), ([x.as_dict() for x in st] if stats else None)
+
+    def contnm(self, run, layer):
+        """CONTNM for one layer: (absrb, v1abs, dvabs); layer.jrad is
+        updated as OPDPTH does."""
+        cap = 5050
+        out = np.zeros(cap)
+        n = C.c_int(0)
+        v1 = C.c_double(0.0)
+        dv = C.c_double(0.0)
+        _chk(self._lib.lblx_contnm(self._h, C.byref(run), C.byref(layer), abi.ptr(out, C.c_double), cap,
+                                   C.byref(n), C.byref(v1), C.byref(dv)), "lblx_contnm")
+        return out[:n.value].copy(), v1.value, dv.value
 
     def xmerge_device(self, run, layer_old, layer, tzu, tbnd, emis, d_od, d_rold, d_told,
                       d_rnew, d_tnew):

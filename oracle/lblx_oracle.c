@@ -41,6 +41,7 @@
 
 #include "../include/lblx.h"
 #include "../lblrtm_amd/csrc/lblx_refdata.h"
+#include "../lblrtm_amd/csrc/lblx_contdata.h"
 
 #define NFPTS 2001
 #define NFMX 2601
@@ -707,6 +708,259 @@ static void xint(double v1a, double v2a, double dva, const double *A /*1-based*/
 }
 
 /* ------------------------------------------------------------------ */
+/* CONTNM (src/contnm.f90:20-990) for ABSRB on the 1 cm-1 grid of OPDPTH
+ * (src/lblrtm.f90:6214-6226).  Continua of 0..4340 cm-1: MT_CKD H2O self
+ * and foreign (src/mt_ckd_h2o_module.f90:68-176), CO2 (FRNCO2 with the
+ * XFACCO2 scale, contnm.f90:330-353, 1295-1351), O2 fundamental (o2_ver_1,
+ * contnm.f90:583-597, 7254-7318), N2 fundamental (n2_ver_1,
+ * contnm.f90:734-753, 2668-2754) and Rayleigh (contnm.f90:955-977).
+ * Bands that reach the N2 rotational (< 350) or any continuum above
+ * 4340 cm-1 return LBLX_E_UNSUPPORTED.                                    */
+#define N_ABSRB 5050
+
+/* pre_xint (contnm.f90:994-1012) */
+static void pre_xint(double v1ss, double v2ss, double v1abs, double dvabs, int nptabs, int *ist,
+                     int *last)
+{
+  int nb1 = fint(2. + (v1ss - v1abs) / dvabs + 1.e-5);
+  *ist = nb1 > 1 ? nb1 : 1;
+  int nb2 = fint(1. + (v2ss - v1abs) / dvabs + 1.e-5);
+  *last = nptabs < nb2 ? nptabs : nb2;
+}
+
+/* The common index walk of FRNCO2/o2_ver_1/n2_ver_1: coarse points
+ * J = 1..NPTC at V1C + (J-1)*DVS cover table entries I = I1 + J - 1.     */
+static void cont_grid(double v1abs, double v2abs, double v1s, double dvs, int npts, double *v1c,
+                      double *v2c, int *nptc, int *i1)
+{
+  double V1C = v1abs - dvs, V2C = v2abs + dvs;
+  int I1 = V1C < v1s ? -1 : fint((V1C - v1s) / dvs + 0.01);
+  V1C = v1s + dvs * (double)(I1 - 1);
+  int I2 = fint((V2C - v1s) / dvs + 0.01);
+  int NPTC = I2 - I1 + 3;
+  if (NPTC > npts) NPTC = npts + 4;
+  *v1c = V1C;
+  *v2c = V1C + dvs * (double)(NPTC - 1);
+  *nptc = NPTC;
+  *i1 = I1;
+}
+
+/* myradfn (mt_ckd_h2o_module.f90:178-204) */
+static double mt_radfn(double vi, double xkt)
+{
+  double x = vi / xkt;
+  if (x <= 0.01) return 0.5 * x * vi;
+  if (x <= 10.) {
+    double e = exp(-x);
+    return vi * (1. - e) / (1. + e);
+  }
+  return vi;
+}
+
+int oracle_cont_grid(const lblx_layer *Ly, double *v1abs, double *v2abs, double *dvabs, int *nptabs)
+{
+  double V1 = Ly->v1, V2 = Ly->v2;
+  double d = 1.;
+  double a = (double)fint(V1);
+  if (V1 < 0.) a = a - 1.;
+  a = a - 3. * d;
+  double b = (double)fint(V2 + 3. * d + 0.5);
+  *v1abs = a;
+  *v2abs = b;
+  *dvabs = d;
+  *nptabs = fint((b - a) / d + 1.5);
+  return *nptabs + 4 > N_ABSRB ? LBLX_E_SIZE : LBLX_OK;
+}
+
+/* ABSRB(1..nptabs) in absrb[0..nptabs-1]; jrad as OPDPTH passes it.     */
+int oracle_contnm(const lblx_run *run, const lblx_layer *Ly, int jrad, double *absrb, int cap,
+                  int *nptabs_out)
+{
+  double v1abs, v2abs, dvabs;
+  int nptabs;
+  int rc = oracle_cont_grid(Ly, &v1abs, &v2abs, &dvabs, &nptabs);
+  if (rc) return rc;
+  if (cap < nptabs) return LBLX_E_SIZE;
+  *nptabs_out = nptabs;
+  double X[7] = {1, 1, 1, 1, 1, 1, 1};
+  switch (run->icntnm) {
+    case 0: for (int k = 0; k < 7; k++) X[k] = 0.; break;
+    case 1: break;
+    case 2: X[0] = 0.; break;
+    case 3: X[1] = 0.; break;
+    case 4: X[0] = 0.; X[1] = 0.; break;
+    case 5: X[6] = 0.; break;
+    case 6: for (int k = 0; k < 7; k++) X[k] = run->cntscl[k]; break;
+    default: return LBLX_E_ARG;
+  }
+  const double xself = X[0], xfrgn = X[1], xco2c = X[2], xo3cn = X[3], xo2cn = X[4], xn2cn = X[5],
+               xrayl = X[6];
+  const double V1 = Ly->v1, V2 = Ly->v2, PAVE = Ly->pave, TAVE = Ly->tave;
+  if ((V2 > -10.0 && V1 < 350. && xn2cn > 0.) || (V2 > 4340.0 && (xn2cn > 0. || xo2cn > 0. || xo3cn > 0.)))
+    return LBLX_E_UNSUPPORTED;
+  double *AB = calloc(N_ABSRB + 8, sizeof(double)); /* 1-based ABSRB */
+  double *C = calloc(N_ABSRB + 8, sizeof(double));  /* 1-based coarse array */
+  const double P0c = 1013., T0c = 296., XLOSMT = 2.68675E+19;
+  double RHOAVE = (PAVE / P0c) * (T0c / TAVE);
+  double XKT = TAVE / RADCN2;
+  double amagat = (PAVE / P0c) * (273. / TAVE);
+  double WTOT = Ly->wbroad;
+  for (int m = 0; m < Ly->nmol; m++) WTOT = WTOT + Ly->wk[m];
+  double wk1 = Ly->nmol >= 1 ? Ly->wk[0] : 0., wk2 = Ly->nmol >= 2 ? Ly->wk[1] : 0.;
+  double wk7 = Ly->nmol >= 7 ? Ly->wk[6] : 0.;
+  double x_vmr_h2o = wk1 / WTOT, x_vmr_o2 = wk7 / WTOT;
+  double x_vmr_n2 = 1. - x_vmr_h2o - x_vmr_o2;
+  double wn2 = x_vmr_n2 * WTOT;
+  int ist, last;
+
+  /* H2O (contnm.f90:296-327; mt_ckd_h2o_absco) */
+  double h2o_fac = wk1 / WTOT;
+  if (V2 > -20.0 && V1 < 20000. && (xself > 0. || xfrgn > 0.)) {
+    const double *W = LBLX_MTCKD_WVN;
+    const double *FOR = run->ifrgnx == 1 ? LBLX_MTCKD_FORCL : LBLX_MTCKD_FOR;
+    int radflag = jrad != 0;
+    double dvc = W[1] - W[0];
+    int i = 1;
+    while (W[i - 1] <= v1abs - 2 * dvc) i++;
+    int i1 = i - 1;
+    if (i1 == 0) i1 = 1;
+    while (W[i - 1] < v2abs + 2 * dvc) i++;
+    int i2 = i;
+    int nco = i2 - i1 + 1;
+    double xkt = TAVE / RADCN2;
+    double rho_rat = (PAVE / LBLX_MTCKD_REF_P) * (LBLX_MTCKD_REF_T / TAVE);
+    double *sc = calloc(nco + 4, sizeof(double)), *fc = calloc(nco + 4, sizeof(double));
+    for (int k = 1; k <= nco; k++) {
+      int q = i1 + k - 2;
+      double sv = LBLX_MTCKD_SELF[q] * pow(LBLX_MTCKD_REF_T / TAVE, LBLX_MTCKD_TEXP[q]);
+      sv = sv * h2o_fac * rho_rat;
+      double fv = FOR[q] * (1 - h2o_fac) * rho_rat;
+      if (radflag) {
+        double rad = mt_radfn(W[q], xkt);
+        sv = sv * rad;
+        fv = fv * rad;
+      }
+      sc[k] = sv;
+      fc[k] = fv;
+    }
+    int npt_l = fint((v2abs - v1abs) / dvabs + 1);
+    int lst;
+    pre_xint(W[0], W[LBLX_MTCKD_N - 1], v1abs, dvabs, npt_l, &ist, &lst);
+    double *sh = calloc(N_ABSRB + 8, sizeof(double)), *fh = calloc(N_ABSRB + 8, sizeof(double));
+    xint(W[i1 - 1], W[i2 - 1], dvc, sc, 1.0, v1abs, dvabs, sh, ist, lst);
+    xint(W[i1 - 1], W[i2 - 1], dvc, fc, 1.0, v1abs, dvabs, fh, ist, lst);
+    for (int k = 1; k <= N_ABSRB; k++) {
+      double cself = wk1 * sh[k] * xself;
+      double cfor = wk1 * fh[k] * xfrgn;
+      AB[k] = AB[k] + cself + cfor;
+    }
+    free(sc); free(fc); free(sh); free(fh);
+  }
+
+  /* CO2 (contnm.f90:330-353, FRNCO2 contnm.f90:1295-1351) */
+  if (V2 > -20.0 && V1 < 10000. && xco2c > 0.) {
+    const double *H = LBLX_FCO2_HDR;
+    double V1C, V2C;
+    int NPTC, I1;
+    cont_grid(v1abs, v2abs, H[0], H[2], (int)H[3], &V1C, &V2C, &NPTC, &I1);
+    double WCO2 = wk2 * RHOAVE * 1.0E-20 * xco2c;
+    double trat = TAVE / 246.;
+    for (int J = 1; J <= NPTC; J++) {
+      int I = I1 + (J - 1);
+      double f = 0.;
+      if (I >= 1 && I <= (int)H[3]) {
+        double tcor = 1.;
+        if (I >= LBLX_CO2_TDEP_I0 && I <= LBLX_CO2_TDEP_I0 + 24)
+          tcor = pow(trat, LBLX_CO2_TDEP[I - LBLX_CO2_TDEP_I0]);
+        f = tcor * LBLX_FCO2[I - 1];
+      }
+      double VJ = V1C + H[2] * (double)(J - 1);
+      double CFAC = 1.;
+      if (VJ >= 2000. && VJ <= 2998.) {
+        int JFAC = fint((VJ - 1998.) / 2. + 0.00001);
+        CFAC = LBLX_XFACCO2[JFAC - 1];
+      }
+      f = CFAC * f;
+      C[J] = f * WCO2;
+      if (jrad == 1) C[J] = C[J] * radfn(VJ, XKT);
+    }
+    C[NPTC + 1] = C[NPTC + 2] = 0.;
+    pre_xint(H[0], H[1], v1abs, dvabs, nptabs, &ist, &last);
+    xint(V1C, V2C, H[2], C, 1.0, v1abs, dvabs, AB, ist, last);
+  }
+
+  /* O2 fundamental 1340-1850 (contnm.f90:583-597, o2_ver_1 :7254-7318) */
+  if (V2 > 1340.0 && V1 < 1850. && xo2cn > 0.) {
+    const double *H = LBLX_O2F_HDR;
+    double tau_fac = xo2cn * wk7 * 1.e-20 * amagat;
+    double V1C, V2C;
+    int NPTC, I1;
+    cont_grid(v1abs, v2abs, H[0], H[2], (int)H[3], &V1C, &V2C, &NPTC, &I1);
+    double xktfac = (1. / 296.) - (1. / TAVE);
+    double factor = (1.e+20 / XLOSMT);
+    for (int J = 1; J <= NPTC; J++) {
+      int I = I1 + (J - 1);
+      double VJ = V1C + H[2] * (double)(J - 1);
+      double c0 = 0.;
+      if (I >= 1 && I <= (int)H[3]) c0 = factor * LBLX_O2F_XO2[I - 1] * exp(LBLX_O2F_XO2T[I - 1] * xktfac) / VJ;
+      C[J] = tau_fac * c0;
+      if (jrad == 1) C[J] = C[J] * radfn(VJ, XKT);
+    }
+    C[NPTC + 1] = C[NPTC + 2] = 0.;
+    pre_xint(H[0], H[1], v1abs, dvabs, nptabs, &ist, &last);
+    xint(V1C, V2C, H[2], C, 1.0, v1abs, dvabs, AB, ist, last);
+  }
+
+  /* N2 fundamental (contnm.f90:734-753, n2_ver_1 :2668-2754) */
+  if (V2 > 2001.77 && V1 < 2897.59 && xn2cn > 0.) {
+    const double *H = LBLX_N2F_HDR;
+    double tau_fac = xn2cn * (wn2 / XLOSMT) * amagat;
+    double V1C, V2C;
+    int NPTC, I1;
+    cont_grid(v1abs, v2abs, H[0], H[2], (int)H[3], &V1C, &V2C, &NPTC, &I1);
+    const double T = TAVE, T_272 = 272., T_228 = 228.;
+    double xtfac = ((1. / T) - (1. / T_272)) / ((1. / T_228) - (1. / T_272));
+    double xt_lin = (T - T_272) / (T_228 - T_272);
+    double a_o2 = 1.294 - 0.4545 * T / 296.;
+    for (int J = 1; J <= NPTC; J++) {
+      int I = I1 + (J - 1);
+      double c0 = 0., c1 = 0., c2 = 0.;
+      double VJ = V1C + H[2] * (double)(J - 1);
+      if (I >= 1 && I <= (int)H[3]) {
+        double a = LBLX_N2F_272[I - 1], b = LBLX_N2F_228[I - 1];
+        if (a > 0. && b > 0.) c0 = a * pow(b / a, xtfac);
+        else c0 = a + (b - a) * xt_lin;
+        c0 = c0 / VJ;
+        c1 = a_o2 * c0;
+        c2 = (9. / 7.) * LBLX_N2F_AH2O[I - 1] * c0;
+      }
+      C[J] = tau_fac * (x_vmr_n2 * c0 + x_vmr_o2 * c1 + x_vmr_h2o * c2);
+      if (jrad == 1) C[J] = C[J] * radfn(VJ, XKT);
+    }
+    C[NPTC + 1] = C[NPTC + 2] = 0.;
+    pre_xint(H[0], H[1], v1abs, dvabs, nptabs, &ist, &last);
+    xint(V1C, V2C, H[2], C, 1.0, v1abs, dvabs, AB, ist, last);
+  }
+
+  /* Rayleigh (contnm.f90:955-977) */
+  if (run->iaersl == 0 && V2 >= 820. && xrayl > 0.) {
+    double conv = xrayl * 1.E-20 / (2.68675e-1 * 1.e5);
+    for (int i = 1; i <= nptabs; i++) {
+      double vr = v1abs + (double)(i - 1) * dvabs;
+      double x = vr / 1.e4;
+      double ray = (x * x * x / (9.38076E2 - 10.8426 * (x * x))) * (WTOT * conv);
+      if (jrad == 1) ray = ray * x;
+      else if (jrad == 0) ray = ray * x / radfn(vr, XKT);
+      AB[i] = AB[i] + ray;
+    }
+  }
+  for (int i = 0; i < nptabs; i++) absrb[i] = AB[i + 1];
+  free(AB);
+  free(C);
+  return LBLX_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* OPDPTH + HIRAC1 for one layer.                                       */
 typedef struct {
   /* near-line block buffer (RDLIN) */
@@ -747,6 +1001,14 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
   if ((V1 / xkt_l) < 5. && Ly->jrad != -1) Ly->jrad = 0;
   int JRAD = Ly->jrad;
   int icntnm = run->icntnm;
+  /* CONTNM with the radiation flag OPDPTH has just set (lblrtm.f90:6214-6226) */
+  double *absrb_own = NULL;
+  if (icntnm >= 1 && !absrb) {
+    absrb_own = calloc(N_ABSRB + 8, sizeof(double));
+    rc = oracle_cont_grid(Ly, &v1abs, &v2abs, &dvabs, &nptabs);
+    if (!rc) rc = oracle_contnm(run, Ly, JRAD, absrb_own + 1, N_ABSRB, &nptabs);
+    if (rc) { free(absrb_own); return rc; }
+  }
   double DVR4 = 0., BOUND4 = 25.;
   int ilblf4 = run->ilblf4;
   int ipflag = (icntnm != 0 && Ly->pave <= 0.5) ? 1 : 0;
@@ -827,6 +1089,9 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
   if (icntnm >= 1 && absrb) {
     absrb1 = calloc((size_t)nptabs + 8, sizeof(double));
     for (int k = 0; k < nptabs; k++) absrb1[k + 1] = absrb[k];
+  } else if (icntnm >= 1) {
+    absrb1 = absrb_own;
+    absrb_own = NULL;
   }
   t3reader_t rd = {L, 0};
   int IEOF = 0, ILO = 0, IHI = -999, IDATA = 0, IPANEL = 0, ISTOP = 0;

@@ -53,6 +53,10 @@ def lib():
         L.oracle_merge_next.argtypes = [C.c_int, dp, dp, ip, C.c_double, dp, dp, C.c_int, dp, dp,
                                         ip, C.c_double, dp, C.c_double, C.c_double, C.c_double,
                                         dp, dp]
+        L.oracle_cont_grid.restype = C.c_int
+        L.oracle_cont_grid.argtypes = [C.POINTER(abi.Layer), dp, dp, dp, ip]
+        L.oracle_contnm.restype = C.c_int
+        L.oracle_contnm.argtypes = [C.POINTER(abi.Run), C.POINTER(abi.Layer), C.c_int, dp, C.c_int, ip]
         L.oracle_merge_quirks.restype = C.c_int64
         L.oracle_merge_quirks.argtypes = []
         _LIB = L
@@ -158,3 +162,18 @@ def merge_next(old, layer_od, pan, dv, tave, tzu, v2run):
     if rc:
         raise RuntimeError("oracle_merge_next rc=%d" % rc)
     return Merged(pan, dv, rad, tra), int(lib().oracle_merge_quirks())
+
+
+def contnm(run, layer, jrad):
+    """ABSRB of CONTNM for one layer: (absrb, v1abs, dvabs)."""
+    a, b, d = C.c_double(), C.c_double(), C.c_double()
+    n = C.c_int()
+    rc = lib().oracle_cont_grid(C.byref(layer), C.byref(a), C.byref(b), C.byref(d), C.byref(n))
+    if rc:
+        raise RuntimeError("oracle_cont_grid rc=%d" % rc)
+    out = np.zeros(n.value)
+    rc = lib().oracle_contnm(C.byref(run), C.byref(layer), jrad, abi.ptr(out, C.c_double), n.value,
+                             C.byref(n))
+    if rc:
+        raise RuntimeError("oracle_contnm rc=%d" % rc)
+    return out, a.value, d.value

@@ -18,9 +18,9 @@ def bt(rad, v):
     return RADCN2 * v / np.log1p(RADCN1 * v ** 3 / rad)
 
 
-def _layers(v1, v2):
+def _layers(v1, v2, icntnm=0):
     atm = synth.atmosphere()
-    run = abi.make_run(v1, v2)
+    run = abi.make_run(v1, v2, icntnm=icntnm)
     dv, it = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
     lays = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l],
                            dv[l], v1, v2) for l in range(len(dv))]
@@ -69,13 +69,13 @@ def test_xmerge_layer_pairs(gpu_ctx, pair):
 
 
 def test_spectrum_column_workload(gpu_ctx):
-    """The 50-layer column, 500,000 lines over 500-2500 cm-1: TOA upwelling
-    radiance within 0.01 K brightness temperature of the oracle's OPDPTH +
-    XMERGE chain, and within 1e-10 of the oracle's merge of the GPU's own
-    layer optical depths."""
+    """The 50-layer column, 500,000 lines over 500-2500 cm-1, continuum on:
+    TOA upwelling radiance within 0.01 K brightness temperature of the
+    oracle's OPDPTH + XMERGE chain, and within 1e-10 of the oracle's merge
+    of the GPU's own layer optical depths."""
     tab = synth.config1_lines()
     gpu_ctx.load_lines(tab)
-    run, lays, atm, _ = _layers(500.0, 2500.0)
+    run, lays, atm, _ = _layers(500.0, 2500.0, icntnm=1)
     tzu = np.asarray(atm["tlev"][1:], np.float64)
     tbnd, emis = float(atm["tlev"][0]), 0.98
     work = [abi.Layer.from_buffer_copy(x) for x in lays]
