@@ -599,6 +599,7 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
                        c->w_pread.as<int>(), NJ * S.blk, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s,
                        c->w_blkmax.as<int>(), NJ * S.blk, 0);
+    bool converged = false;
     for (int it = 0; it < 64; it++) {
       hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, c->w_changed.as<int>(), 1, 0);
       hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY,
@@ -606,25 +607,30 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
                          c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
       hipLaunchKernelGGL(k_pread_scan, dim3(njobs), dim3(256), 0, s, dY, S, c->w_blkmax.as<int>(),
                          c->w_pread.as<int>(), c->w_changed.as<int>());
-      if (it >= 2) {
+      if (it >= 1) {
         int changed = 1;
         HIPCHK(hipMemcpyAsync(&changed, c->w_changed.p, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        if (!changed) break;
+        if (!changed) {
+          converged = true;  // this pass already used the final read panels
+          break;
+        }
       }
     }
     // keep flags for the converged read panels
-    hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY, S,
-                       c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                       c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
+    if (!converged)
+      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY, S,
+                         c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
+                         c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
     c->tend(th);
     // 6. CNVFNV scatter records
     th = c->tbegin(ST_SCATTER);
     hipLaunchKernelGGL(k_scatter_rec, dim3(maxnbk, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(),
                        c->w_keep.as<int>(), L, dY, S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(),
                        c->w_rec.as<ScatterRec>());
-    hipLaunchKernelGGL(k_sum_npts, dim3(ngrid, njobs), dim3(256), 0, s, dY, S, c->w_rec.as<ScatterRec>(), cnt,
-                       cnt + 2 * NJ);
+    if (st)  // TAPE6 counters only
+      hipLaunchKernelGGL(k_sum_npts, dim3(ngrid, njobs), dim3(256), 0, s, dY, S, c->w_rec.as<ScatterRec>(),
+                         cnt, cnt + 2 * NJ);
     c->tend(th);
   }
   // 7. tiles: R3, R2, R1 (+ PANEL combine)

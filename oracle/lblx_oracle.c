@@ -27,11 +27,23 @@
  *   oracle_od_layer OPDPTH + HIRAC1          src/lblrtm.f90:6135-6285,
  *                                            src/oprop.f90:25-668
  *   oracle_plan_dv PATH DV selection         src/lblrtm.f90:5336-5500
+ *   oracle_contnm  CONTNM (H2O MT_CKD, CO2,  src/contnm.f90:20-990, 1295-1351,
+ *                  O2/N2 fundamentals,       2668-2754, 7254-7318,
+ *                  Rayleigh) + pre_xint      src/mt_ckd_h2o_module.f90:68-176
+ *   oracle_merge_* EMIN, EMINIT, RADMRG,     src/xmerge.f90:968-1017, 1684-2241,
+ *                  RADNN (IEMIT=1, IPATHL=1) 2245-3280
  *
- * Parity pins (tests/test_oracle.py): shape-table closed forms and sum
- * rules, SCOR(296 K) = 1, single-line sums against the Voigt profile,
- * linearity in strength, symmetry, and the known-answer DV of the
- * single-layer case.
+ * Parity pins: tests/test_oracle.py (shape-table closed forms and sum
+ * rules, SCOR(296 K) = 1, a single line against the Voigt profile,
+ * linearity in strength, the known-answer DV of the single-layer case),
+ * tests/test_oracle_shrink.py (SHRINK conservation and known grouping,
+ * LINF4 batches), tests/test_contnm.py (continua against Keys cubic
+ * convolution of the tables at their identity temperatures, Rayleigh,
+ * scale-factor linearity, continuum-only optical depth), tests/test_merge.py
+ * (isothermal, transparent and opaque columns, cubic reproduction of RADNN,
+ * the RADMRG panel walk).  Partly pinned (invariants only): the exact
+ * line-rejection choices of CONVF4 (src/oprop.f90:3900-3917) and LNCOR1
+ * (src/oprop.f90:1153-1177); no executable reference exists to check them.
  */
 #include <math.h>
 #include <stdint.h>
