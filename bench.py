@@ -55,6 +55,11 @@ def stage_bytes(stage, stats, sizes, nwin):
     return LINE_BYTES * 500000 * len(sizes)
 
 
+def bt(rad, v):
+    """Brightness temperature of radiance rad (W/(cm2 sr cm-1)) at v."""
+    return 1.4387752 * v / np.log1p(1.191042722e-12 * v ** 3 / rad)
+
+
 def cpu_baseline(run, layers, tab, tzu, tbnd, emis):
     """The oracle (single-threaded C restatement of OPDPTH and XMERGE) on
     the whole spectrum: every layer of the column, one core."""
@@ -85,6 +90,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="first run the plain copy kernel over this many MB (the reference "
+                         "dispatch for scaling HBM traffic counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -105,29 +113,45 @@ def main():
     ntop = sizes[-1]
 
     ctx = lib.Context(0)
+    if a.calib_copy_mb > 0:
+        nb = a.calib_copy_mb << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy_kernel(dst, src, nb)
+        ctx.sync()
+        src.free()
+        dst.free()
     ctx.load_lines(tab)
     rad = ctx.alloc(8 * ntop)
     tra = ctx.alloc(8 * ntop)
 
-    def step(stats=False):
+    def step(r, stats=False):
         lays = [abi.Layer.from_buffer_copy(x) for x in layers]
-        return ctx.spectrum_device(run, lays, tzu, tbnd, emis, rad.ptr, tra.ptr, ntop, stats=stats)
+        return ctx.spectrum_device(r, lays, tzu, tbnd, emis, rad.ptr, tra.ptr, ntop, stats=stats)
 
-    _, stats = step(stats=True)
-    for _ in range(a.warmup):
-        step()
-    ctx.sync()
-    ctx.timing(True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step()
-    ctx.sync()
-    t1 = time.perf_counter()
-    tim = ctx.timings()
-    ctx.timing(False)
-    ms = (t1 - t0) * 1000.0 / a.steps
-    per = {k: v[0] / a.steps for k, v in tim.items() if v[1] > 0}
+    def timed(r):
+        for _ in range(a.warmup):
+            step(r)
+        ctx.sync()
+        ctx.timing(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            step(r)
+        ctx.sync()
+        t1 = time.perf_counter()
+        tim = ctx.timings()
+        ctx.timing(False)
+        return (t1 - t0) * 1000.0 / a.steps, {k: v[0] / a.steps for k, v in tim.items() if v[1] > 0}
+
+    v1top, stats = step(run, stats=True)
+    ms, per = timed(run)
+    r64 = rad.to_host(np.float64)
+    # FP32 split form of the same spectrum, timed the same way
+    run32 = abi.make_run(500.0, 2500.0, icntnm=1, precision=abi.FP32_SPLIT)
+    ms32, per32 = timed(run32)
+    r32 = rad.to_host(np.float64)
+    v = v1top + np.arange(ntop) * layers[-1].dv
+    dbt = np.abs(bt(r32, v) - bt(r64, v))
     dom = max(per, key=per.get)
     nwin = sum(s["nwin4"] for s in stats)
     nbytes = stage_bytes(dom, stats, sizes, nwin)
@@ -149,6 +173,9 @@ def main():
                                "7 molecules, LBLF4 panels, continuum on, XMERGE radiance, FP64",
                    "layers": nlay, "lines": 500000, "points": int(sum(sizes)), "top_points": ntop,
                    "stage_ms": {k: round(v, 4) for k, v in per.items()}},
+        "fp32_split": {"value": 1000.0 / ms32, "ms_per_step": ms32, "ratio_vs_fp64": ms / ms32,
+                       "max_dbt_K": float(dbt.max()), "rms_dbt_K": float(np.sqrt(np.mean(dbt ** 2))),
+                       "stage_ms": {k: round(v, 4) for k, v in per32.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,

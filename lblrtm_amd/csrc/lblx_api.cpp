@@ -345,6 +345,9 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
                     void *od_base, const int64_t *od_off, bool f32out, double *v1p, lblx_stats *st) {
   if (c->nlines <= 0) return LBLX_E_NOLINES;
   if (njobs <= 0) return LBLX_E_ARG;
+  // FP32 split: line shapes evaluated and accumulated in FP32, positions,
+  // grid offsets and the panel combine in FP64; OD written as float.
+  const bool split = run->precision == LBLX_FP32_SPLIT;
   hipStream_t s = c->stream;
   const int n = c->nlines, nblk = c->nblk;
   std::vector<LayerPlan> plans(njobs);
@@ -583,8 +586,12 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   c->tend(th);
   // 3. far-wing windows
   th = c->tbegin(ST_CONVF4);
-  hipLaunchKernelGGL(k_convf4, dim3(nwin), dim3(256), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
-                     c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
+  if (split)
+    hipLaunchKernelGGL(k_convf4<float>, dim3(nwin), dim3(256), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
+                       c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
+  else
+    hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(256), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
+                       c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
   c->tend(th);
   if (maxnn > 0) {
     const int ngrid = (maxnn + 255) / 256;
@@ -635,43 +642,41 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   }
   // 7. tiles: R3, R2, R1 (+ PANEL combine)
   const double *tabs = c->d_tabs.as<double>();
+  const double *d_abs = abs_stride > 0 ? c->w_absrb.as<double>() : nullptr;
   for (int G = 3; G >= 1; G--) {
     const int T = tiles[G - 1];
     const int nt = (int)tjob[G - 1].size();
     int tht = c->tbegin(G == 3 ? ST_TILE3 : (G == 2 ? ST_TILE2 : ST_TILE1));
     hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY,
                        tm[G - 1], nt, T, G, c->w_margin.as<double>());
-    size_t shm = (size_t)(T + (G == 1 ? 3 : 1) * 2601) * sizeof(double);
+    const size_t esz = split ? sizeof(float) : sizeof(double);
+    size_t shm = (size_t)(T + (G == 1 ? 3 : 1) * 2601) * esz;
+    const ScatterRec *rec = c->w_rec.as<ScatterRec>();
+    const DevPanel *dp = c->w_pan.as<DevPanel>();
+    const DevWin *dw = c->w_win.as<DevWin>();
+    const DevSeg *dg = c->w_seg.as<DevSeg>();
+    const double *r4 = c->w_r4.as<double>();
+    double *r3 = c->w_r3.as<double>(), *r2 = c->w_r2.as<double>();
+#define LBLX_TILE(GG, OUTT, ACCT, RIN, ROUT, ODP, ABS)                                                 \
+  do {                                                                                             \
+    HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT>,                               \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));             \
+    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT>), dim3(nt), dim3(256), shm, s, rec, tm[GG - 1], T, \
+                       tabs, dY, S, dp, dw, r4, (const double *)(ABS), dg, (const double *)(RIN), \
+                       (double *)(ROUT), (OUTT *)(ODP));                                         \
+  } while (0)
     if (G == 3) {
-      HIPCHK(hipFuncSetAttribute((const void *)k_tile<3, double>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      hipLaunchKernelGGL((k_tile<3, double>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
-                         tm[2], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                         c->w_r4.as<double>(), (const double *)(abs_stride > 0 ? c->w_absrb.p : nullptr),
-                         c->w_seg.as<DevSeg>(),
-                         (const double *)nullptr, c->w_r3.as<double>(), (double *)nullptr);
+      if (split) LBLX_TILE(3, double, float, nullptr, r3, nullptr, d_abs);
+      else LBLX_TILE(3, double, double, nullptr, r3, nullptr, d_abs);
     } else if (G == 2) {
-      HIPCHK(hipFuncSetAttribute((const void *)k_tile<2, double>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      hipLaunchKernelGGL((k_tile<2, double>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
-                         tm[1], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                         c->w_r4.as<double>(), (const double *)nullptr, c->w_seg.as<DevSeg>(),
-                         c->w_r3.as<double>(), c->w_r2.as<double>(), (double *)nullptr);
+      if (split) LBLX_TILE(2, double, float, r3, r2, nullptr, nullptr);
+      else LBLX_TILE(2, double, double, r3, r2, nullptr, nullptr);
     } else if (!f32out) {
-      HIPCHK(hipFuncSetAttribute((const void *)k_tile<1, double>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      hipLaunchKernelGGL((k_tile<1, double>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
-                         tm[0], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                         c->w_r4.as<double>(), (const double *)nullptr, c->w_seg.as<DevSeg>(),
-                         c->w_r2.as<double>(), (double *)nullptr, (double *)od_base);
+      LBLX_TILE(1, double, double, r2, nullptr, od_base, nullptr);
     } else {
-      HIPCHK(hipFuncSetAttribute((const void *)k_tile<1, float>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      hipLaunchKernelGGL((k_tile<1, float>), dim3(nt), dim3(256), shm, s, c->w_rec.as<ScatterRec>(),
-                         tm[0], T, tabs, dY, S, c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                         c->w_r4.as<double>(), (const double *)nullptr, c->w_seg.as<DevSeg>(),
-                         c->w_r2.as<double>(), (double *)nullptr, (float *)od_base);
+      LBLX_TILE(1, float, float, r2, nullptr, od_base, nullptr);
     }
+#undef LBLX_TILE
     c->tend(tht);
   }
   HIPCHK(hipGetLastError());

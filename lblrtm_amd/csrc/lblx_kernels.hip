@@ -493,25 +493,35 @@ __global__ void k_t9_vhi(Strides S, Tape9 T9) {
 // without atomics.
 constexpr int CF4_T = 256;
 
+// T = double: the dbl build.  T = float (FP32 split): the far-wing shape is
+// evaluated and accumulated in FP32; the line centre relative to the R4
+// grid (xnui) and the grid positions stay FP64.
+template <typename T>
 struct F4Par {
-  double xnui, sil, siv_a3, siv_b3, f4bnd, rec_alfvi2, alfli2;
+  double xnui;
+  T sil, siv_a3, siv_b3, f4bnd, rec_alfvi2, alfli2;
   int jmin, jmax, co2;
 };
 
-__device__ __forceinline__ double f4_at(const F4Par &p, double xjj, double r_bndsq, double zsqbnd) {
-  double xm = xjj - p.xnui;
-  double xmsq = xm * xm;
-  double zvsq = xmsq * p.rec_alfvi2;
-  double sub = p.co2 ? (2. - (xmsq * r_bndsq)) * p.f4bnd : p.f4bnd;
+template <typename T>
+__device__ __forceinline__ T f4_at(const F4Par<T> &p, double xjj, T r_bndsq, T zsqbnd) {
+  T xm = (T)(xjj - p.xnui);
+  T xmsq = xm * xm;
+  T zvsq = xmsq * p.rec_alfvi2;
+  T sub = p.co2 ? ((T)2. - (xmsq * r_bndsq)) * p.f4bnd : p.f4bnd;
   if (zvsq <= zsqbnd) return (p.siv_a3 + zvsq * p.siv_b3) - sub;
-  return p.sil / (p.alfli2 + xmsq) - sub;
+  if constexpr (sizeof(T) == 4)  // FP32 split: hardware reciprocal (1 ulp)
+    return p.sil * __builtin_amdgcn_rcpf(p.alfli2 + xmsq) - sub;
+  else
+    return p.sil / (p.alfli2 + xmsq) - sub;
 }
 
+template <typename T>
 __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S,
                                                   const DevWin *wins, const DevSeg *segs,
                                                   double *r4out, unsigned long long *k3upd) {
-  __shared__ double r4[R4ROW];
-  __shared__ F4Par par[64];
+  __shared__ T r4[R4ROW];
+  __shared__ F4Par<T> par[64];
   __shared__ int s_nacc, s_done, s_b;
   const int tid = threadIdx.x;
   const DevWin W = wins[blockIdx.x];
@@ -525,13 +535,14 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
   const double A3 = (1. + 2. * zsqbnd) / ((1. + zsqbnd) * (1. + zsqbnd));
   const double B3 = -1. / ((1. + zsqbnd) * (1. + zsqbnd));
   const double bndsq = bound4 * bound4;
-  const double r_bndsq = 1. / bndsq;
+  const T r_bndsq = (T)(1. / bndsq);
+  const T zsqbnd_t = (T)zsqbnd;
   const double vnulst = W.v2r4 + bound4;
   const double vlo = W.v1r4 - bound4;
   const int nptr4 = W.nptr4;
   const int nl = T9.n[job];
   const int nb9 = T9.nblk[job];
-  for (int j = tid; j < R4ROW; j += CF4_T) r4[j] = 0.;
+  for (int j = tid; j < R4ROW; j += CF4_T) r4[j] = (T)0.;
   if (tid == 0) {
     s_b = nb9;
     s_done = 0;
@@ -547,12 +558,13 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       const int lane = tid;
       int i = pos + lane;
       bool valid = i < nl;
-      F4Par p;
+      F4Par<T> p;
       p.co2 = 0;
       p.jmin = 1;
       p.jmax = 0;
       p.xnui = 0.;
-      double speak = 0., partial = 0., vnui = 0.;
+      T speak = (T)0., partial = (T)0.;
+      double vnui = 0.;
       bool nz = false;
       int jj = 1;
       if (valid) {
@@ -565,9 +577,10 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         double zetdif = 100. * zetai - (double)(iz - 1);
         double alfavi = (c_avrat[iz - 1] + zetdif * (c_avrat[iz] - c_avrat[iz - 1])) * (alfali + alfadi);
         double ralfvi = 1. / alfavi;
-        p.sil = sp * recpi * alfali;
+        double sil = sp * recpi * alfali;
+        p.sil = (T)sil;
         double siv = (alfali * ralfvi) * sp * recpi * ralfvi;
-        speak = A3 * fabs(siv);
+        speak = (T)(A3 * fabs(siv));
         vnui = T9.vnu[o + i];
         jj = fint((vnui - W.v1r4) / dvr4 + 1.);
         jj = jj < 1 ? 1 : (jj > nptr4 ? nptr4 : jj);
@@ -577,11 +590,12 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         p.jmin = jmin < 1 ? 1 : jmin;
         p.jmax = jmax > nptr4 ? nptr4 : jmax;
         if (jmin > nptr4) p.jmax = 0;  // accepted but nothing to add (GO TO 60)
-        p.alfli2 = alfali * alfali;
-        p.f4bnd = p.sil / (p.alfli2 + bndsq);
-        p.rec_alfvi2 = 1. / (alfavi * alfavi);
-        p.siv_a3 = siv * A3;
-        p.siv_b3 = siv * B3;
+        double alfli2 = alfali * alfali;
+        p.alfli2 = (T)alfli2;
+        p.f4bnd = (T)(sil / (alfli2 + bndsq));
+        p.rec_alfvi2 = (T)(1. / (alfavi * alfavi));
+        p.siv_a3 = (T)(siv * A3);
+        p.siv_b3 = (T)(siv * B3);
         p.co2 = T9.mol[o + i] == 2;
         partial = r4[jj];
       }
@@ -589,7 +603,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       int nacc = 0;
       int done = 0;
       for (int j = 0; j < 64; j++) {
-        bool acc = valid && nz && !(speak <= (Y.dptmn4 + Y.dptfc * partial));
+        bool acc = valid && nz && !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial));
         bool stp = acc && (vnui >= vnulst);
         int accj = __shfl(acc ? 1 : 0, j);
         int stpj = __shfl(stp ? 1 : 0, j);
@@ -598,7 +612,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
           break;
         }
         if (accj) {
-          F4Par q;
+          F4Par<T> q;
           q.xnui = __shfl(p.xnui, j);
           q.sil = __shfl(p.sil, j);
           q.siv_a3 = __shfl(p.siv_a3, j);
@@ -609,7 +623,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
           q.jmin = __shfl(p.jmin, j);
           q.jmax = __shfl(p.jmax, j);
           q.co2 = __shfl(p.co2, j);
-          if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me, r_bndsq, zsqbnd);
+          if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me, r_bndsq, zsqbnd_t);
           if (lane == j && p.jmax >= p.jmin) par[nacc] = p;
           if (q.jmax >= q.jmin) nacc++;
         }
@@ -622,14 +636,14 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
     __syncthreads();
     const int nacc = s_nacc;
     for (int a = 0; a < nacc; a++) {
-      F4Par q = par[a];
+      F4Par<T> q = par[a];
       if (tid == 0) nupd += (unsigned long long)(q.jmax - q.jmin + 1);
       int j0 = q.jmin - 1;
       int r = j0 % CF4_T;
       int first = j0 - r + tid + (tid < r ? CF4_T : 0) + 1;
       for (int j = first; j <= q.jmax; j += CF4_T) {
         double xjj = (double)(j - 1) * dvr4;
-        r4[j] = r4[j] + f4_at(q, xjj, r_bndsq, zsqbnd);
+        r4[j] = r4[j] + f4_at(q, xjj, r_bndsq, zsqbnd_t);
       }
     }
     __syncthreads();
@@ -648,12 +662,12 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         else hi = mid - 1;
       }
       DevSeg sg = segs[W.seg0 + lo];
-      if (j >= sg.i1 && j <= sg.i2) r4[j] = r4[j] * (sg.radvi + (double)(j - sg.i1) * sg.rdel);
+      if (j >= sg.i1 && j <= sg.i2) r4[j] = (T)((double)r4[j] * (sg.radvi + (double)(j - sg.i1) * sg.rdel));
     }
     __syncthreads();
   }
   double *row = r4out + (size_t)blockIdx.x * R4ROW;
-  for (int j = tid; j < R4ROW; j += CF4_T) row[j] = (j >= 1 && j <= nptr4) ? r4[j] : 0.;
+  for (int j = tid; j < R4ROW; j += CF4_T) row[j] = (j >= 1 && j <= nptr4) ? (double)r4[j] : 0.;
 }
 
 // ---------------------------------------------------------------------
@@ -954,34 +968,76 @@ __device__ __forceinline__ double xint_at(double v1a, double v2a, double dva, co
   return -A[j - 1] * b1 + A[j] * (1. - c + b2) + A[j + 1] * (c + b1) - A[j + 2] * b2;
 }
 
+__device__ __forceinline__ double rl_f64(double v, int j) {
+  int2 w = __builtin_bit_cast(int2, v);
+  w.x = __builtin_amdgcn_readlane(w.x, j);
+  w.y = __builtin_amdgcn_readlane(w.y, j);
+  return __builtin_bit_cast(double, w);
+}
+__device__ __forceinline__ int64_t rl_i64(int64_t v, int j) {
+  int2 w = __builtin_bit_cast(int2, v);
+  w.x = __builtin_amdgcn_readlane(w.x, j);
+  w.y = __builtin_amdgcn_readlane(w.y, j);
+  return __builtin_bit_cast(int64_t, w);
+}
+
+// The fields of a scatter record that grid G needs, one line per lane.
+template <int G>
+struct TileLine {
+  int64_t glo;
+  double zfl, zslope, str, strd, strver;
+  int npts;
+  __device__ __forceinline__ void load(const ScatterRec *rec, int li, int hi) {
+    if (li < hi) {
+      const ScatterRec &r = rec[li];
+      npts = r.npts;
+      glo = G == 1 ? r.g1lo : (G == 2 ? r.g1lo - r.s2 : r.g1lo - r.s3);
+      zfl = G == 1 ? r.zfl1 : (G == 2 ? r.zfl2 : r.zfl3);
+      zslope = r.zslope;
+      str = G == 1 ? r.strf1 : (G == 2 ? r.strf2 : r.strf3);
+      if (G == 1) {
+        strd = r.strd;
+        strver = r.strver;
+      }
+    } else {
+      npts = 0;
+      glo = 0;
+      zfl = zslope = str = 0.;
+    }
+    if (G != 1) strd = strver = 0.;
+    if (G == 1 && li >= hi) strd = strver = 0.;
+  }
+};
+
 // Near-line scatter into one tile of grid G (3: DVR3, 2: DVR2, 1: DV) with
 // the PANEL combine in the epilogue.
 //   G = 3: R3 = lines(F3) + XINT(R4) + XINT(ABSRB)            -> r3
 //   G = 2: R2 = lines(F2) + Lagrange4(R3)                       -> r2
 //   G = 1: OD = (lines(F1,FG,XVER) + Lagrange4(R2)) * radiation -> od
-template <int G, typename OUT>
+template <int G, typename OUT, typename ACC>
 __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M, int tile,
                                               const double *tabs, const DevLayer *Ys, Strides S,
                                               const DevPanel *pans, const DevWin *wins,
                                               const double *r4, const double *absrb,
                                               const DevSeg *segs, const double *rin, double *rout,
                                               OUT *od) {
-  extern __shared__ double sm[];
-  double *acc = sm;                 // [tile]
-  double *t1 = sm + tile;           // F_G table
-  double *tg = t1 + NFMX;           // FG   (G = 1)
-  double *tv = tg + NFMX;           // XVER (G = 1)
+  extern __shared__ double sm_d[];
+  ACC *sm = reinterpret_cast<ACC *>(sm_d);
+  ACC *acc = sm;                    // [tile]
+  ACC *t1 = sm + tile;              // F_G table
+  ACC *tg = t1 + NFMX;              // FG   (G = 1)
+  ACC *tv = tg + NFMX;              // XVER (G = 1)
   const int tid = threadIdx.x;
   const int job = M.job[blockIdx.x];
   const DevLayer &Y = Ys[job];
   const ScatterRec *rec = recs + job * S.line;
-  for (int k = tid; k < tile; k += 256) acc[k] = 0.;
+  for (int k = tid; k < tile; k += 256) acc[k] = (ACC)0.;
   const double *src = tabs + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NFMX;
-  for (int k = tid; k < NFMX; k += 256) t1[k] = src[k];
+  for (int k = tid; k < NFMX; k += 256) t1[k] = (ACC)src[k];
   if (G == 1) {
     for (int k = tid; k < NFMX; k += 256) {
-      tg[k] = tabs[3 * NFMX + k];
-      tv[k] = tabs[4 * NFMX + k];
+      tg[k] = (ACC)tabs[3 * NFMX + k];
+      tv[k] = (ACC)tabs[4 * NFMX + k];
     }
   }
   __syncthreads();
@@ -989,25 +1045,40 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
   const int lo = M.lo[blockIdx.x], hi = M.hi[blockIdx.x];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
-  for (int li = lo + wave; li < hi; li += 4) {
-    const ScatterRec r = rec[li];
-    if (r.npts <= 0) continue;
-    int64_t glo = G == 1 ? r.g1lo : (G == 2 ? r.g1lo - r.s2 : r.g1lo - r.s3);
-    int64_t a = glo > T0 ? glo : T0;
-    int64_t b = glo + r.npts - 1;
-    if (b > T0 + tile - 1) b = T0 + tile - 1;
-    const double zfl = G == 1 ? r.zfl1 : (G == 2 ? r.zfl2 : r.zfl3);
-    const double str = G == 1 ? r.strf1 : (G == 2 ? r.strf2 : r.strf3);
-    for (int64_t k = a + lane; k <= b; k += 64) {
-      int off = (int)(k - glo);
-      double zf = zfl + (double)(off + 1) * r.zslope;
-      int iz = fint(fabs(zf) + 1.5) - 1;
-      iz = iz > NFMX - 1 ? NFMX - 1 : iz;  // |zf| <= 2500 by construction
-      double v;
-      if (G == 1) v = str * t1[iz] + r.strd * tg[iz] + r.strver * tv[iz];
-      else v = str * t1[iz];
-      atomicAdd(&acc[k - T0], v);
+  // Each wave takes chunks of 64 consecutive lines: lane i loads the
+  // record of line c0 + i with vector loads, then the wave walks the chunk
+  // line by line, broadcasting each line's fields with readlane.  The next
+  // chunk's records are loaded before the current chunk is scattered.
+  TileLine<G> cur, nxt;
+  int c0 = lo + wave * 64;
+  if (c0 < hi) cur.load(rec, c0 + lane, hi);
+  for (; c0 < hi; c0 += 256) {
+    const int c1 = c0 + 256;
+    if (c1 < hi) nxt.load(rec, c1 + lane, hi);
+    const int nj = hi - c0 < 64 ? hi - c0 : 64;
+    for (int j = 0; j < nj; j++) {
+      const int npts = __builtin_amdgcn_readlane(cur.npts, j);
+      if (npts <= 0) continue;
+      const int64_t glo = rl_i64(cur.glo, j);
+      const double zfl = rl_f64(cur.zfl, j), zslope = rl_f64(cur.zslope, j);
+      const ACC str = (ACC)rl_f64(cur.str, j);
+      const ACC strd = G == 1 ? (ACC)rl_f64(cur.strd, j) : (ACC)0.;
+      const ACC strver = G == 1 ? (ACC)rl_f64(cur.strver, j) : (ACC)0.;
+      int64_t a = glo > T0 ? glo : T0;
+      int64_t b = glo + npts - 1;
+      if (b > T0 + tile - 1) b = T0 + tile - 1;
+      for (int64_t k = a + lane; k <= b; k += 64) {
+        int off = (int)(k - glo);
+        double zf = zfl + (double)(off + 1) * zslope;
+        int iz = fint(fabs(zf) + 1.5) - 1;
+        iz = iz > NFMX - 1 ? NFMX - 1 : iz;  // |zf| <= 2500 by construction
+        ACC v;
+        if (G == 1) v = str * t1[iz] + strd * tg[iz] + strver * tv[iz];
+        else v = str * t1[iz];
+        atomicAdd(&acc[k - T0], v);
+      }
     }
+    cur = nxt;
   }
   __syncthreads();
   const double X00 = -7. / 128., X01 = 105. / 128., X02 = 35. / 128., X03 = -5. / 128.;
@@ -1017,7 +1088,7 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
     int64_t g = T0 + k;
     if (G == 3) {
       if (g >= Y.n3) continue;
-      double v = acc[k];
+      double v = (double)acc[k];
       if (g <= Y.g3max) {
         int p = g < 155 ? 0 : (int)((g - 5) / 150);
         if (p > Y.npanel - 1) p = Y.npanel - 1;
@@ -1034,7 +1105,7 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
       rout[Y.r3o + g] = v;
     } else if (G == 2) {
       if (g >= Y.n2) continue;
-      double v = acc[k];
+      double v = (double)acc[k];
       if (g >= 4 && g <= Y.g2max) {
         const double *r3 = rin + Y.r3o;
         int64_t q = g >> 2;
@@ -1049,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
     } else {
       if (g < 32 || g > Y.g_last) continue;
       const double *r2 = rin + Y.r2o;
-      double v = acc[k];
+      double v = (double)acc[k];
       int64_t q = g >> 2;
       int ph = (int)(g & 3);
       double a0 = r2[q - 1], a1 = r2[q], a2 = r2[q + 1], a3 = r2[q + 2];

@@ -276,6 +276,11 @@ class Context:
         """Device buffer (DeviceBuffer) of nbytes on the context's device."""
         return DeviceBuffer(self, nbytes)
 
+    def copy_kernel(self, dst, src, nbytes):
+        """Plain device-to-device copy by a kernel (lblx_copy kind 3)."""
+        _chk(self._lib.lblx_copy(self._h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), int(nbytes), 3),
+             "lblx_copy")
+
     def sync(self):
         _chk(self._lib.lblx_sync(self._h), "lblx_sync")
 
