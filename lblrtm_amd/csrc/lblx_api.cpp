@@ -587,10 +587,10 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   // 3. far-wing windows
   th = c->tbegin(ST_CONVF4);
   if (split)
-    hipLaunchKernelGGL(k_convf4<float>, dim3(nwin), dim3(256), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
+    hipLaunchKernelGGL(k_convf4<float>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
                        c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
   else
-    hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(256), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
+    hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
                        c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
   c->tend(th);
   if (maxnn > 0) {

@@ -493,6 +493,25 @@ __global__ void k_t9_vhi(Strides S, Tape9 T9) {
 // without atomics.
 constexpr int CF4_T = 256;
 
+// Broadcast lane j's value to the wave (j wave-uniform).
+__device__ __forceinline__ double rl_f64(double v, int j) {
+  int2 w = __builtin_bit_cast(int2, v);
+  w.x = __builtin_amdgcn_readlane(w.x, j);
+  w.y = __builtin_amdgcn_readlane(w.y, j);
+  return __builtin_bit_cast(double, w);
+}
+__device__ __forceinline__ float rl_f32(float v, int j) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+}
+__device__ __forceinline__ int64_t rl_i64(int64_t v, int j) {
+  int2 w = __builtin_bit_cast(int2, v);
+  w.x = __builtin_amdgcn_readlane(w.x, j);
+  w.y = __builtin_amdgcn_readlane(w.y, j);
+  return __builtin_bit_cast(int64_t, w);
+}
+__device__ __forceinline__ double rl_t(double v, int j) { return rl_f64(v, j); }
+__device__ __forceinline__ float rl_t(float v, int j) { return rl_f32(v, j); }
+
 // T = double: the dbl build.  T = float (FP32 split): the far-wing shape is
 // evaluated and accumulated in FP32; the line centre relative to the R4
 // grid (xnui) and the grid positions stay FP64.
@@ -553,6 +572,19 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
   __syncthreads();
   int pos = (s_b < nb9) ? T9.bstart[ot + s_b] : nl;
   unsigned long long nupd = 0;
+  // raw TAPE9 fields of the chunk wave 0 resolves next (prefetched)
+  double f_sp = 0., f_al = 0., f_ad = 0., f_v = 0.;
+  int f_mol = 0;
+  if (tid < 64 && pos + tid < nl) {
+    f_sp = T9.sp[o + pos + tid];
+    f_al = T9.alfa[o + pos + tid];
+    f_ad = T9.epp[o + pos + tid];
+    f_v = T9.vnu[o + pos + tid];
+    f_mol = T9.mol[o + pos + tid];
+  }
+  // margin that sends lines within rounding of the rejection bound to the
+  // exact sequential test
+  const T lb_margin = sizeof(T) == 4 ? (T)1e-5 : (T)1e-12;
   while (pos < nl) {
     if (tid < 64) {
       const int lane = tid;
@@ -563,14 +595,15 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       p.jmin = 1;
       p.jmax = 0;
       p.xnui = 0.;
+      p.f4bnd = (T)0.;
       T speak = (T)0., partial = (T)0.;
       double vnui = 0.;
       bool nz = false;
       int jj = 1;
       if (valid) {
-        double sp = T9.sp[o + i];
+        double sp = f_sp;
         nz = sp != 0.;
-        double alfadi = T9.epp[o + i], alfali = T9.alfa[o + i];
+        double alfadi = f_ad, alfali = f_al;
         double zetai = alfali / (alfali + alfadi);
         int iz = fint(100. * zetai + ONEPL_D);
         iz = iz < 1 ? 1 : (iz > 101 ? 101 : iz);  // zeta in [0,1] keeps iz in range
@@ -581,7 +614,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         p.sil = (T)sil;
         double siv = (alfali * ralfvi) * sp * recpi * ralfvi;
         speak = (T)(A3 * fabs(siv));
-        vnui = T9.vnu[o + i];
+        vnui = f_v;
         jj = fint((vnui - W.v1r4) / dvr4 + 1.);
         jj = jj < 1 ? 1 : (jj > nptr4 ? nptr4 : jj);
         p.xnui = vnui - W.v1r4;
@@ -596,33 +629,50 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         p.rec_alfvi2 = (T)(1. / (alfavi * alfavi));
         p.siv_a3 = (T)(siv * A3);
         p.siv_b3 = (T)(siv * B3);
-        p.co2 = T9.mol[o + i] == 2;
+        p.co2 = f_mol == 2;
         partial = r4[jj];
       }
+      const bool cand = valid && nz;
+      // Lower bound of the running R4 at this line: every earlier line adds
+      // at least -2 F4BND (shape >= 0, subtraction <= 2 F4BND), so a line
+      // rejected against that bound is rejected whatever the earlier lines
+      // decide; only the others go through the sequential test.
+      T neg = cand ? (T)2. * p.f4bnd : (T)0.;
+      T pre = neg;
+      for (int d = 1; d < 64; d <<= 1) {
+        T t = __shfl_up(pre, d);
+        if (lane >= d) pre += t;
+      }
+      const T lbv = partial - (pre - neg);
+      const T thr_lb = (T)Y.dptmn4 + (T)Y.dptfc * lbv;
+      const bool surely_rej = speak < thr_lb - fabs(thr_lb) * lb_margin;
+      uint64_t todo = __ballot(cand && !surely_rej);
       const double xjj_me = (double)(jj - 1) * dvr4;
       int nacc = 0;
       int done = 0;
-      for (int j = 0; j < 64; j++) {
-        bool acc = valid && nz && !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial));
+      while (todo) {
+        const int j = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        bool acc = cand && !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial));
         bool stp = acc && (vnui >= vnulst);
-        int accj = __shfl(acc ? 1 : 0, j);
-        int stpj = __shfl(stp ? 1 : 0, j);
+        const int accj = __builtin_amdgcn_readlane(acc ? 1 : 0, j);
+        const int stpj = __builtin_amdgcn_readlane(stp ? 1 : 0, j);
         if (stpj) {
           done = 1;
           break;
         }
         if (accj) {
           F4Par<T> q;
-          q.xnui = __shfl(p.xnui, j);
-          q.sil = __shfl(p.sil, j);
-          q.siv_a3 = __shfl(p.siv_a3, j);
-          q.siv_b3 = __shfl(p.siv_b3, j);
-          q.f4bnd = __shfl(p.f4bnd, j);
-          q.rec_alfvi2 = __shfl(p.rec_alfvi2, j);
-          q.alfli2 = __shfl(p.alfli2, j);
-          q.jmin = __shfl(p.jmin, j);
-          q.jmax = __shfl(p.jmax, j);
-          q.co2 = __shfl(p.co2, j);
+          q.xnui = rl_f64(p.xnui, j);
+          q.sil = rl_t(p.sil, j);
+          q.siv_a3 = rl_t(p.siv_a3, j);
+          q.siv_b3 = rl_t(p.siv_b3, j);
+          q.f4bnd = rl_t(p.f4bnd, j);
+          q.rec_alfvi2 = rl_t(p.rec_alfvi2, j);
+          q.alfli2 = rl_t(p.alfli2, j);
+          q.jmin = __builtin_amdgcn_readlane(p.jmin, j);
+          q.jmax = __builtin_amdgcn_readlane(p.jmax, j);
+          q.co2 = __builtin_amdgcn_readlane(p.co2, j);
           if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me, r_bndsq, zsqbnd_t);
           if (lane == j && p.jmax >= p.jmin) par[nacc] = p;
           if (q.jmax >= q.jmin) nacc++;
@@ -631,6 +681,15 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       if (lane == 0) {
         s_nacc = nacc;
         s_done = done;
+      }
+      // prefetch the next chunk while the group scatters this one
+      const int in = pos + 64 + lane;
+      if (!done && in < nl) {
+        f_sp = T9.sp[o + in];
+        f_al = T9.alfa[o + in];
+        f_ad = T9.epp[o + in];
+        f_v = T9.vnu[o + in];
+        f_mol = T9.mol[o + in];
       }
     }
     __syncthreads();
@@ -968,18 +1027,6 @@ __device__ __forceinline__ double xint_at(double v1a, double v2a, double dva, co
   return -A[j - 1] * b1 + A[j] * (1. - c + b2) + A[j + 1] * (c + b1) - A[j + 2] * b2;
 }
 
-__device__ __forceinline__ double rl_f64(double v, int j) {
-  int2 w = __builtin_bit_cast(int2, v);
-  w.x = __builtin_amdgcn_readlane(w.x, j);
-  w.y = __builtin_amdgcn_readlane(w.y, j);
-  return __builtin_bit_cast(double, w);
-}
-__device__ __forceinline__ int64_t rl_i64(int64_t v, int j) {
-  int2 w = __builtin_bit_cast(int2, v);
-  w.x = __builtin_amdgcn_readlane(w.x, j);
-  w.y = __builtin_amdgcn_readlane(w.y, j);
-  return __builtin_bit_cast(int64_t, w);
-}
 
 // The fields of a scatter record that grid G needs, one line per lane.
 template <int G>
