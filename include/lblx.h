@@ -199,6 +199,34 @@ int lblx_spectrum_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers,
                          const double *tzu, double tbnd, double emis, double *d_rad,
                          double *d_tra, int64_t n_out, double *v1p_top, lblx_stats *st);
 
+/* FFTSCN (src/fftscn_dbl.f90:25-789): record 10.1 of TAPE5.             */
+typedef struct lblx_scan {
+  double hwhm;              /* HWHM field: OPD in cm when jfnin < 0        */
+  double v1, v2;            /* band of the scanned output                  */
+  double dvout;             /* output spacing (INTPDR); 0: boxcar grid     */
+  double param;             /* record 10.1 param (0: none)                 */
+  int32_t jemit;            /* 0 transmittance, 1 radiance                 */
+  int32_t jfnin;            /* scanning function: 0 boxcar; +-1,2,3,4,5,6  */
+  int32_t mratin;           /* boxcar ratio (0: 12, < 0: no boxcar)        */
+  int32_t ivx;              /* 0 auto, 1 interferogram domain              */
+  int32_t nofix;            /* 1: no boxcar deconvolution                  */
+  int32_t pad;
+} lblx_scan;
+
+/* Output size of FFTSCN for an input spectrum on v1_in + k*dv_in,
+ * k < n_in (the TAPE12 the scan reads).                                  */
+int lblx_scan_size(const lblx_scan *scan, double v1_in, double dv_in, int64_t n_in,
+                   int64_t *n_out, double *v1_out, double *dv_out);
+
+/* FFTSCN on device buffers: load V1Z..V2Z, boxcar by M, real FFT of the
+ * padded record set, apodization x boxcar deconvolution, inverse FFT, then
+ * INTPDR's four-point interpolation onto dvout.  Supported: JEMIT 0/1,
+ * |JFNIN| in {0..6} with the interferogram-domain functions (IVX = 1),
+ * input spacing below 2 HWHM / 12.                                       */
+int lblx_fftscn_device(lblx_ctx *ctx, const lblx_scan *scan, const double *d_in, double v1_in,
+                       double dv_in, int64_t n_in, double *d_out, int64_t cap, int64_t *n_out,
+                       double *v1_out, double *dv_out);
+
 /* Wait for the context stream; the stream handle (hipStream_t) for callers
  * that order their own device work after the library's.                  */
 int lblx_sync(lblx_ctx *ctx);

@@ -97,6 +97,30 @@ class Stats(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class Scan(C.Structure):
+    _fields_ = [
+        ("hwhm", C.c_double),
+        ("v1", C.c_double),
+        ("v2", C.c_double),
+        ("dvout", C.c_double),
+        ("param", C.c_double),
+        ("jemit", C.c_int32),
+        ("jfnin", C.c_int32),
+        ("mratin", C.c_int32),
+        ("ivx", C.c_int32),
+        ("nofix", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+def make_scan(hwhm, v1, v2, jemit=1, jfnin=-6, mratin=0, dvout=0.0, ivx=0, nofix=0, param=0.0):
+    """TAPE5 record 10.1 of FFTSCN (src/fftscn_dbl.f90:313-315)."""
+    s = Scan()
+    s.hwhm, s.v1, s.v2, s.dvout, s.param = hwhm, v1, v2, dvout, param
+    s.jemit, s.jfnin, s.mratin, s.ivx, s.nofix = jemit, jfnin, mratin, ivx, nofix
+    return s
+
+
 def ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 

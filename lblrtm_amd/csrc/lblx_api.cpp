@@ -22,6 +22,7 @@
 
 // The kernels are compiled in this translation unit (one HIP module).
 #include "lblx_kernels.hip"
+#include "lblx_scan.hip"
 
 namespace lblx {
 void shape_tables_host(double *tab /* 5 x 2601: F1 F2 F3 FG XVER */);
@@ -141,6 +142,8 @@ struct lblx_ctx {
   DevBuf w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
   DevBuf w_tmap[3];
   DevBuf w_mpan, w_rt;
+  DevBuf s_z[2], s_tw, s_g, s_ip, s_cof;  // FFTSCN
+  int64_t s_twN = 0;
   DevLines lines() const {
     DevLines L;
     L.n = nlines;
@@ -252,7 +255,7 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
                     &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
-                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt};
+                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -996,6 +999,91 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     if (rc) return rc;
   }
   if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
+  return LBLX_OK;
+}
+
+int lblx_scan_size(const lblx_scan *scan, double v1_in, double dv_in, int64_t n_in, int64_t *n_out,
+                   double *v1_out, double *dv_out) {
+  if (!scan || !n_out) return LBLX_E_ARG;
+  ScanPlan P;
+  int rc = plan_scan(scan, v1_in, dv_in, n_in, &P);
+  if (rc) return rc;
+  *n_out = P.nout;
+  if (v1_out) *v1_out = P.v1out;
+  if (dv_out) *dv_out = P.dvout;
+  return LBLX_OK;
+}
+
+int lblx_fftscn_device(lblx_ctx *c, const lblx_scan *scan, const double *d_in, double v1_in,
+                       double dv_in, int64_t n_in, double *d_out, int64_t cap, int64_t *n_out,
+                       double *v1_out, double *dv_out) {
+  if (!c || !scan || !d_in || !d_out) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  ScanPlan P;
+  int rc = plan_scan(scan, v1_in, dv_in, n_in, &P);
+  if (rc) return rc;
+  if (cap < P.nout) return LBLX_E_SIZE;
+  const int64_t N = P.N;
+  hipStream_t s = c->stream;
+  if (c->s_z[0].ensure(N * sizeof(double2)) || c->s_z[1].ensure(N * sizeof(double2))) return LBLX_E_HIP;
+  if (c->s_twN != N) {  // twiddles exp(-2 pi i m / N), m < N/2
+    std::vector<double2> tw(N / 2);
+    const double two_pi = 4.0 * std::asin(1.0);
+    for (int64_t m = 0; m < N / 2; m++) {
+      double ang = two_pi * (double)m / (double)N;
+      tw[m] = make_double2(std::cos(ang), -std::sin(ang));
+    }
+    if (c->s_tw.ensure(tw.size() * sizeof(double2))) return LBLX_E_HIP;
+    HIPCHK(hipMemcpy(c->s_tw.p, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+    c->s_twN = N;
+  }
+  int th = c->tbegin(ST_SCAN);
+  double2 *z0 = c->s_z[0].as<double2>(), *z1 = c->s_z[1].as<double2>();
+  const unsigned gN = (unsigned)((N + 255) / 256), gH = (unsigned)((N / 2 + 255) / 256);
+  hipLaunchKernelGGL(k_scan_load, dim3(gN), dim3(256), 0, s, d_in, P.n0, P.M, P.ktot, N, P.fill, z0);
+  double2 *cur = z0, *oth = z1;
+  if (P.jfn != 0) {
+    for (int64_t ns = 1; ns < N; ns <<= 1) {
+      hipLaunchKernelGGL(k_fft_pass, dim3(gH), dim3(256), 0, s, cur, oth, c->s_tw.as<double2>(), N, ns, -1);
+      std::swap(cur, oth);
+    }
+    ScanFilter F{N, P.k4, P.DX, P.A, P.dvb, P.r4, P.jfn, P.fix};
+    hipLaunchKernelGGL(k_scan_filter, dim3((unsigned)((N / 2 + 1 + 255) / 256)), dim3(256), 0, s, F, cur);
+    for (int64_t ns = 1; ns < N; ns <<= 1) {
+      hipLaunchKernelGGL(k_fft_pass, dim3(gH), dim3(256), 0, s, cur, oth, c->s_tw.as<double2>(), N, ns, +1);
+      std::swap(cur, oth);
+    }
+  }
+  const double rn = P.jfn != 0 ? 1.0 / (double)N : 1.0;
+  const int64_t off = P.n1 - 1;
+  if (scan->dvout > 0) {
+    std::vector<double> cof(4 * 203);
+    const double xnumcf = 201.;
+    for (int ip = 0; ip <= 202; ip++) {  // INTERP coefficients (postsub.f90:2966-2974)
+      double p = ((double)ip - 1.0) / (xnumcf - 1.0), pp = p * p;
+      cof[ip] = -p / 2.0 * (1 - p) * (1 - p);
+      cof[203 + ip] = 1.0 - pp * (3.0 - 2.0 * p) + pp / 2.0 * (1.0 - p);
+      cof[406 + ip] = pp * (3.0 - 2.0 * p) + p / 2.0 * (1.0 - p) * (1.0 - p);
+      cof[609 + ip] = -pp / 2 * (1.0 - p);
+    }
+    if (c->s_g.ensure(P.g.size() * 8) || c->s_ip.ensure(P.ip.size() * 4) || c->s_cof.ensure(cof.size() * 8))
+      return LBLX_E_HIP;
+    HIPCHK(hipMemcpyAsync(c->s_g.p, P.g.data(), P.g.size() * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->s_ip.p, P.ip.data(), P.ip.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->s_cof.p, cof.data(), cof.size() * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_scan_out, dim3((unsigned)((P.nout + 255) / 256)), dim3(256), 0, s, cur, off, P.ntot,
+                       rn, P.nout, c->s_g.as<int64_t>(), c->s_ip.as<int32_t>(), c->s_cof.as<double>(), d_out);
+    HIPCHK(hipStreamSynchronize(s));  // host vectors above are freed on return
+  } else {
+    hipLaunchKernelGGL(k_scan_out, dim3((unsigned)((P.nout + 255) / 256)), dim3(256), 0, s, cur, off, P.ntot,
+                       rn, P.nout, (const int64_t *)nullptr, (const int32_t *)nullptr, (const double *)nullptr,
+                       d_out);
+  }
+  c->tend(th);
+  HIPCHK(hipGetLastError());
+  if (n_out) *n_out = P.nout;
+  if (v1_out) *v1_out = P.v1out;
+  if (dv_out) *dv_out = P.dvout;
   return LBLX_OK;
 }
 

@@ -540,3 +540,215 @@ void shape_tables_host(double *tab) {
 }
 
 }  // namespace lblx
+
+namespace lblx {
+
+namespace {
+const double SC_C[14] = {0.0, 2.0, 0.849322, 2.257609, 3.314800, 2.100669, 2.195676,
+                         2.0, 2.570274, 2.367714, 2.071759, -3.3, -2.5, 3.314800};
+const double SC_CRATIO[14] = {0.0, 40., 10., 40., 160., 20., 20., 20., 40., 40., 20., 100., 10., -1.};
+const double SC_CLIMIT[14] = {0., 2., 3., 40., 160., 20., 20., 20., 40., 40., 20., 100., 10., 160.};
+constexpr int64_t LPTSMX = 65536;
+
+// Records of 65536 points that FFTSCN uses for npts data points: four
+// times the data records, rounded up to a power of two (Loadsp, Boxcar).
+int scan_records(int64_t npts) {
+  double ar = (double)npts / (double)LPTSMX;
+  int rd = (ar - (int)ar) == 0.0 ? (int)ar : (int)ar + 1;
+  double pw = std::log((double)rd) / std::log(2.0);
+  return (pw - (int)pw) == 0 ? 1 << (int)(pw + 2) : 1 << ((int)pw + 3);
+}
+}  // namespace
+
+int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan *P) {
+  if (!S || !P || !(dv > 0.) || n_in < 4) return LBLX_E_ARG;
+  *P = ScanPlan{};
+  double HWHM = S->hwhm;
+  const double V1 = S->v1, V2 = S->v2;
+  const int jfnin = S->jfnin, jfn = jfnin < 0 ? -jfnin : jfnin;
+  if (HWHM <= 0 || S->jemit < 0 || S->jemit > 1 || jfn > 6) return LBLX_E_UNSUPPORTED;
+  double A = 0.;
+  if (jfnin > 0) {
+    A = SC_C[jfn] * HWHM;
+  } else if (jfnin < 0) {
+    A = 1. / HWHM;  // PATHL = HWHM field
+    HWHM = S->param == 0. ? A / std::fabs(SC_C[jfn]) : S->param;
+  }
+  if (jfn == 2 && S->param != 0.) return LBLX_E_UNSUPPORTED;
+  const int nobox = S->mratin < 0;
+  const int mratio = S->mratin > 0 ? S->mratin : 12;
+  const double V2C = v1c + dv * (double)(n_in - 1);
+  if (V1 >= V2 || V1 >= V2C || V2 <= v1c) return LBLX_E_ARG;
+  // FFTSCN 442-470
+  double V1Z = V1 - HWHM * SC_CLIMIT[jfn], V2Z = V2 + HWHM * SC_CLIMIT[jfn];
+  V1Z = v1c + (double)fint((V1Z - v1c) / dv + 1.01) * dv;
+  V2Z = v1c + (double)fint((V2Z - v1c) / dv + 1.01) * dv;
+  if (V1Z < v1c) V1Z = v1c;
+  if (V2Z > V2C) V2Z = V2C;
+  if ((V2Z - V1Z) / HWHM < SC_CLIMIT[jfn]) return LBLX_E_ARG;
+  if (dv > 2. * HWHM / 12) return LBLX_E_UNSUPPORTED;  // INTPDR onto 2 HWHM / 12 first
+  int ivx = S->ivx;
+  if (ivx == 0) ivx = (V2Z - V1Z) / HWHM > SC_CRATIO[jfn] ? 1 : -1;
+  if (jfn == 0) ivx = 0;
+  if (ivx == -1) return LBLX_E_UNSUPPORTED;  // spectral-domain instrument function
+  // Loadsp
+  P->n0 = (int64_t)std::llround((V1Z - v1c) / dv);
+  P->ltot = (int64_t)((V2Z - V1Z) / dv + 1.1);
+  V2Z = V1Z + dv * (double)(P->ltot - 1);
+  if (P->n0 < 0 || P->n0 + P->ltot > n_in) return LBLX_E_ARG;
+  int lrec = scan_records(P->ltot);
+  P->fill = S->jemit == 0 ? 1.0 : 0.0;
+  int M = 1;
+  if (jfn == 0) {
+    M = (int)(2.0 * HWHM / dv);
+    if (M < 2) return LBLX_E_ARG;
+  } else if (!nobox) {
+    M = (int)(2.0 * HWHM / (mratio * dv));
+  }
+  if (M >= 2 && !nobox) {
+    V1Z = V1Z + dv * (M - 1) / 2.0;
+    P->ktot = P->ltot / M;
+    dv = M * dv;
+    V2Z = V1Z + dv * (double)(P->ktot - 1);
+    P->N = (int64_t)scan_records(P->ktot) * LPTSMX;
+    P->M = M;
+  } else {
+    P->M = 1;
+    P->ktot = P->ltot;
+    P->N = (int64_t)lrec * LPTSMX;
+  }
+  P->jfn = jfn;
+  P->A = A;
+  P->hwhm = HWHM;
+  P->dvb = dv;
+  P->DX = 1.0 / ((double)P->N * dv);
+  P->fix = (P->M >= 2 && !nobox && S->nofix == 0) ? 1 : 0;
+  P->k4 = -1;
+  if (jfn == 4) {  // Scnfnt label 40: the sample straddling 1/A
+    double alen = 1.0 / A;
+    int64_t nn = (int64_t)(alen / P->DX) + 1;
+    double r = alen / P->DX + 1 - (double)nn;
+    if (r <= 0.5) {
+      r = r + 1.0;
+      nn = nn - 1;
+    }
+    if (nn >= 1) {
+      P->k4 = nn;
+      P->r4 = r - 0.5;
+    }
+  }
+  // label 95: points handed to Wrtspc
+  int64_t N1 = (int64_t)((V1 - V1Z) / dv);
+  double V1S = V1Z + (double)(N1 - 1) * dv;
+  if (V1S < V1Z) {
+    N1 = 1;
+    V1S = V1Z;
+  }
+  int64_t N2 = (int64_t)((V2 - V1Z) / dv + 3.01);
+  double V2S = V1Z + (double)(N2 - 1) * dv;
+  if (V2S > V2Z) N2 = (int64_t)((V2Z - V1Z) / dv + 1.01);
+  P->n1 = N1;
+  P->ntot = N2 - N1 + 1;
+  P->v1s = V1S;
+  if (!(S->dvout > 0)) {
+    P->v1out = V1S;
+    P->dvout = dv;
+    P->nout = P->ntot;
+    return LBLX_OK;
+  }
+  // INTPDR / INTERP four-point walk over Wrtspc's 2400-point panels
+  const int64_t nz = P->ntot;
+  const double dvi = dv, dvo = S->dvout;
+  const int npan = (int)((nz + 2399) / 2400);
+  std::vector<double> pv1(npan), pv2(npan);
+  std::vector<int> pn(npan);
+  double v1p = V1S, v2p = 0.;
+  for (int k = 0; k < npan; k++) {
+    int np = (int)std::min<int64_t>(2400, nz - 2400LL * k);
+    if (k > 0) v1p = v2p + dvi;
+    v2p = v1p + dvi * (double)(np - 1);
+    pv1[k] = v1p;
+    pv2[k] = v2p;
+    pn[k] = np;
+  }
+  const double xnumcf = 201.;
+  int pk = 0;
+  while (pk < npan && pv2[pk] < V1 - 4.0 * dvo) pk++;
+  if (pk >= npan) return LBLX_E_ARG;
+  double V1I = pv1[pk] - 4.0 * dvi, V2I = pv2[pk];
+  int NNI = pn[pk] + 4;
+  int64_t base = 2400LL * pk - 5;  // global index of T(0)
+  bool hit_eof = false;
+  double V1J = V1, DVJ = dvo, VJ = V1J;
+  const double vratio = DVJ / dvi;
+  if (V1J < V1I) return LBLX_E_UNSUPPORTED;
+  int J2 = 0;
+  std::vector<int64_t> gpan;
+  std::vector<int32_t> ippan;
+  auto flush = [&](int n) {
+    for (int j = 0; j < n; j++) {
+      P->g.push_back(gpan[j]);
+      P->ip.push_back(ippan[j]);
+    }
+  };
+  gpan.assign(2401, 0);
+  ippan.assign(2401, -1);
+  for (;;) {
+    int I = fint((VJ - V1I) / dvi + 1.00001);
+    double VI = V1I + dvi * (double)(I - 1);
+    double Pf = (VJ - VI) / dvi;
+    int J1 = fint((VJ - V1J) / DVJ + 1.001);
+    J2 = std::min(fint((V2 - V1J) / DVJ + 1.001), fint((V2I - dvi - V1J) / DVJ + 1.));
+    if (J2 > 2400) J2 = 2400;
+    for (int J = J1; J <= J2; J++) {
+      gpan[J - 1] = base + I;
+      ippan[J - 1] = fint(Pf * xnumcf + 1.00001);
+      Pf = Pf + vratio;
+      if (Pf >= 1.0) {
+        I = fint((double)I + Pf);
+        Pf = Pf - (double)fint(Pf);
+      }
+    }
+    VJ = V1J + DVJ * J2;
+    if (J2 >= 2400 || VJ >= V2) {
+      flush(J2);
+      V1J = V1J + DVJ * (J2 - 1) + DVJ;
+      J2 = 0;
+    }
+    if (VJ >= V2) break;
+    if (hit_eof) {  // label 80: zeros to the end of the band
+      for (;;) {
+        int J1b = J2 + 1;
+        int J2b = std::min(fint((V2 - V1J) / DVJ + 1.0001), 2400);
+        for (int J = J1b; J <= J2b; J++) ippan[J - 1] = -1;
+        J2 = J2b;
+        VJ = V1J + DVJ * J2;
+        if (J2 >= 2400 || VJ >= V2) {
+          flush(J2);
+          V1J = V1J + DVJ * (J2 - 1) + DVJ;
+          J2 = 0;
+        }
+        if (VJ >= V2) break;
+      }
+      break;
+    }
+    if (I >= NNI - 2) {
+      if (pk + 1 < npan) {
+        pk++;
+        V1I = pv1[pk] - 4.0 * dvi;
+        V2I = pv2[pk];
+        NNI = pn[pk] + 4;
+        base = 2400LL * pk - 5;
+      } else {
+        hit_eof = true;  // T(NNI+1) extrapolated, T(NNI+2) = 0
+        V2I = V2I + dvi;
+      }
+    }
+  }
+  P->v1out = V1;
+  P->dvout = dvo;
+  P->nout = (int64_t)P->g.size();
+  return LBLX_OK;
+}
+
+}  // namespace lblx

@@ -94,6 +94,34 @@ struct LayerPlan {
   ContPlan cont;      // ICNTNM != 0
 };
 
+// FFTSCN set-up (src/fftscn_dbl.f90:25-789) for an input spectrum on
+// v1c + k*dv, k < n_in: the loaded range, the boxcar, the FFT length, the
+// transform-domain filter parameters, the extracted range and, for
+// DVOUT > 0, INTPDR's four-point walk (src/postsub.f90:2940-3100) as one
+// (input index, coefficient index) pair per output point.
+struct ScanPlan {
+  int jfn;                // |JFNIN|
+  double A;               // apodization parameter (1/OPD)
+  double hwhm;
+  int64_t n0, ltot;       // first loaded input point, points loaded
+  int M;                  // boxcar width (1: none)
+  int64_t ktot;           // points after the boxcar
+  double dvb;             // grid after the boxcar
+  int64_t N;              // FFT length
+  double fill;            // padding (1 transmittance, 0 radiance)
+  double DX;              // OPD step of the transform bins
+  int fix;                // boxcar deconvolution Z / sin Z
+  int64_t k4;             // JFN 4: bin holding the fractional edge value
+  double r4;
+  int64_t n1, ntot;       // extracted points N1 (1-based) .. N1+ntot-1
+  double v1s;             // first extracted wavenumber
+  double v1out, dvout;
+  int64_t nout;
+  std::vector<int64_t> g; // INTPDR: input index of T(I) per output point
+  std::vector<int32_t> ip;  // coefficient index (0..202), -1: zero fill
+};
+int plan_scan(const lblx_scan *scan, double v1c, double dv, int64_t n_in, ScanPlan *P);
+
 // Build the plan; layer->jrad is updated as OPDPTH does.  Returns LBLX_OK
 // or an LBLX_E_* code.
 int plan_layer(const lblx_run *run, lblx_layer *layer, LayerPlan *P);

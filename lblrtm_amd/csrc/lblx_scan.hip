@@ -1,0 +1,134 @@
+// FFTSCN on the device: load + boxcar, radix-2 Stockham complex FFT of the
+// padded record set, the interferogram-domain filter (apodization x boxcar
+// deconvolution), inverse FFT, and INTPDR's four-point interpolation.
+// The algorithm's steps are those of src/fftscn_dbl.f90 (FFTSCN, Boxcar,
+// Scntrn/Scnfnt, Multrn, Wrtspc) and src/postsub.f90 (INTERP).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lblx {
+
+// Boxcar (fftscn_dbl.f90:844-1043) of M input points per output point,
+// padded with FILL to the FFT length; M = 1 is Loadsp's plain copy.
+__global__ void k_scan_load(const double *in, int64_t n0, int M, int64_t ktot, int64_t N,
+                            double fill, double2 *z) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  double v = fill;
+  if (k < ktot) {
+    if (M == 1) {
+      v = in[n0 + k];
+    } else {
+      const double *p = in + n0 + k * (int64_t)M;
+      double sum = 0.0;
+      for (int mm = 0; mm < M; mm++) sum = sum + p[mm];
+      v = sum / (double)M;
+    }
+  }
+  z[k] = make_double2(v, 0.0);
+}
+
+// One radix-2 Stockham pass: sub-transforms of length ns merge into 2 ns.
+// tw[m] = exp(-2 pi i m / N); sign > 0 takes the conjugate (inverse).
+__global__ void k_fft_pass(const double2 *src, double2 *dst, const double2 *tw, int64_t N,
+                           int64_t ns, int sign) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t h = N >> 1;
+  if (i >= h) return;
+  const int64_t k = i & (ns - 1);
+  double2 w = tw[k * (h / ns)];
+  if (sign > 0) w.y = -w.y;
+  const double2 a = src[i], b0 = src[i + h];
+  const double2 b = make_double2(b0.x * w.x - b0.y * w.y, b0.x * w.y + b0.y * w.x);
+  const int64_t o = ((i - k) << 1) + k;
+  dst[o] = make_double2(a.x + b.x, a.y + b.y);
+  dst[o + ns] = make_double2(a.x - b.x, a.y - b.y);
+}
+
+// Scnfnt interferogram-domain functions (IAPSC = 1) at OPD x.
+__device__ __forceinline__ double scan_apod(int jfn, double a, double x) {
+  const double pi = 3.141592653589793;
+  switch (jfn) {
+    case 1: {
+      if (x == 0) return 1.0;
+      double u = sin(pi * x * a) / (pi * x * a);
+      return u * u;
+    }
+    case 2: {
+      double u = pi * x * a;
+      return exp(-fmin(2.0 * (u * u), 34.0));
+    }
+    case 3: return x < 1 / a ? 1.0 - x * a : 0.0;
+    case 4: return x < 1.0 / a ? 1.0 : 0.0;
+    case 5: {
+      double u = 1.0 - (x * a) * (x * a);
+      return x < 1 / a ? u * u : 0.0;
+    }
+    case 6: return x < 1 / a ? 0.53835685 * (1.0 + 0.857504 * cos(pi * x * a)) : 0.0;
+  }
+  return 0.0;
+}
+
+struct ScanFilter {
+  int64_t N, k4;
+  double DX, A, dvb, r4;
+  int jfn, fix;
+};
+
+// Multrn with Scntrn's functions: bins k and N-k times H(k DX); the
+// Nyquist bin is multiplied by the zero imaginary slot of the packed
+// record, i.e. cleared.
+__global__ void k_scan_filter(ScanFilter F, double2 *z) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t h = F.N >> 1;
+  if (k > h) return;
+  double v = 0.;
+  if (k < h) {
+    const double x = (double)k * F.DX;
+    v = scan_apod(F.jfn, F.A, x);
+    if (k == F.k4) v = F.r4;
+    if (F.fix) {
+      const double zz = 2.0 * 3.141592653589793 * x * (F.dvb / 2.0);
+      v = v * (zz != 0 ? zz / sin(zz) : 1.0);
+    }
+  }
+  double2 a = z[k];
+  z[k] = make_double2(a.x * v, a.y * v);
+  if (k > 0 && k < h) {
+    double2 b = z[F.N - k];
+    z[F.N - k] = make_double2(b.x * v, b.y * v);
+  }
+}
+
+// Scanned spectrum point h of the extracted range (Wrtspc), with INTERP's
+// end handling: T(4) = 2 T(5) - T(6) before the first point, the EOF
+// extrapolation one point past the last, zeros beyond.
+__device__ __forceinline__ double scan_pt(const double2 *z, int64_t off, int64_t nz, double rn,
+                                          int64_t h) {
+  if (h >= 0 && h < nz) return z[off + h].x * rn;
+  if (h == -1) return 2.0 * (z[off].x * rn) - z[off + 1].x * rn;
+  if (h == nz) return 2.0 * (z[off + nz - 1].x * rn) - z[off + nz - 2].x * rn;
+  return 0.0;
+}
+
+// Output: the extracted points themselves (DVOUT = 0) or INTPDR's
+// four-point interpolation with the 203-entry coefficient tables.
+__global__ void k_scan_out(const double2 *z, int64_t off, int64_t nz, double rn, int64_t nout,
+                           const int64_t *g, const int32_t *ip, const double *cof, double *out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nout) return;
+  if (!g) {
+    out[t] = z[off + t].x * rn;
+    return;
+  }
+  const int c = ip[t];
+  if (c < 0) {
+    out[t] = 0.0;
+    return;
+  }
+  const int64_t q = g[t];  // extracted-point index of T(I)
+  out[t] = cof[c] * scan_pt(z, off, nz, rn, q - 1) + cof[203 + c] * scan_pt(z, off, nz, rn, q) +
+           cof[406 + c] * scan_pt(z, off, nz, rn, q + 1) + cof[609 + c] * scan_pt(z, off, nz, rn, q + 2);
+}
+
+}  // namespace lblx

@@ -66,6 +66,11 @@ def load():
         "lblx_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
         "lblx_contnm": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(C.c_double), C.c_int,
                                   P(C.c_int), P(C.c_double), P(C.c_double)]),
+        "lblx_scan_size": (C.c_int, [P(abi.Scan), C.c_double, C.c_double, C.c_int64, P(C.c_int64),
+                                     P(C.c_double), P(C.c_double)]),
+        "lblx_fftscn_device": (C.c_int, [C.c_void_p, P(abi.Scan), C.c_void_p, C.c_double, C.c_double,
+                                         C.c_int64, C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_double),
+                                         P(C.c_double)]),
         "lblx_merge_size": (C.c_int, [P(abi.Run), P(abi.Layer), P(abi.Layer), P(C.c_int64)]),
         "lblx_xmerge_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer),
                                          C.c_double, C.c_double, C.c_double, C.c_void_p,
@@ -124,6 +129,16 @@ def merge_size(run, layer_old, layer):
     _chk(load().lblx_merge_size(C.byref(run), C.byref(layer_old) if layer_old is not None else None,
                                 C.byref(layer), C.byref(n)), "lblx_merge_size")
     return n.value
+
+
+def scan_size(scan, v1_in, dv_in, n_in):
+    """(points, v1, dv) of FFTSCN's output for an input on v1_in + k dv_in."""
+    n = C.c_int64(0)
+    v1 = C.c_double(0.0)
+    dv = C.c_double(0.0)
+    _chk(load().lblx_scan_size(C.byref(scan), v1_in, dv_in, n_in, C.byref(n), C.byref(v1), C.byref(dv)),
+         "lblx_scan_size")
+    return n.value, v1.value, dv.value
 
 
 class Context:
@@ -243,6 +258,29 @@ class Context:
         finally:
             rad.free()
             tra.free()
+
+    def fftscn_device(self, scan, d_in, v1_in, dv_in, n_in, d_out, cap):
+        """FFTSCN on device pointers; returns (points, v1, dv) of the output."""
+        n = C.c_int64(0)
+        v1 = C.c_double(0.0)
+        dv = C.c_double(0.0)
+        _chk(self._lib.lblx_fftscn_device(self._h, C.byref(scan), C.c_void_p(d_in), v1_in, dv_in, n_in,
+                                          C.c_void_p(d_out), cap, C.byref(n), C.byref(v1), C.byref(dv)),
+             "lblx_fftscn_device")
+        return n.value, v1.value, dv.value
+
+    def fftscn(self, scan, spectrum, v1_in, dv_in):
+        """Host-side convenience: scan a spectrum held in host memory."""
+        x = np.ascontiguousarray(spectrum, np.float64)
+        n, _, _ = scan_size(scan, v1_in, dv_in, len(x))
+        bi, bo = self.alloc(x.nbytes), self.alloc(8 * max(n, 1))
+        try:
+            bi.from_host(x)
+            n, v1, dv = self.fftscn_device(scan, bi.ptr, v1_in, dv_in, len(x), bo.ptr, n)
+            return bo.to_host(np.float64, n), v1, dv
+        finally:
+            bi.free()
+            bo.free()
 
     def debug_tape9(self, job=0, cap=2000000):
         vnu = np.zeros(cap)
