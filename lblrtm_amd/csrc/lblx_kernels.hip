@@ -528,11 +528,15 @@ __device__ __forceinline__ T f4_at(const F4Par<T> &p, double xjj, T r_bndsq, T z
   T xmsq = xm * xm;
   T zvsq = xmsq * p.rec_alfvi2;
   T sub = p.co2 ? ((T)2. - (xmsq * r_bndsq)) * p.f4bnd : p.f4bnd;
-  if (zvsq <= zsqbnd) return (p.siv_a3 + zvsq * p.siv_b3) - sub;
+  // both branches evaluated and selected: the inner region covers a few
+  // points per line, so a divergent branch would only add exec-mask work
+  T near = p.siv_a3 + zvsq * p.siv_b3;
+  T far;
   if constexpr (sizeof(T) == 4)  // FP32 split: hardware reciprocal (1 ulp)
-    return p.sil * __builtin_amdgcn_rcpf(p.alfli2 + xmsq) - sub;
+    far = p.sil * __builtin_amdgcn_rcpf(p.alfli2 + xmsq);
   else
-    return p.sil / (p.alfli2 + xmsq) - sub;
+    far = p.sil / (p.alfli2 + xmsq);
+  return (zvsq <= zsqbnd ? near : far) - sub;
 }
 
 template <typename T>
