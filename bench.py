@@ -90,6 +90,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run the plain copy kernel over this many MB (the reference "
                          "dispatch for scaling HBM traffic counters)")
@@ -169,6 +170,22 @@ def main():
     dbt_scan = np.abs(bt(s32, vs) - bt(s64, vs))
     v = v1top + np.arange(ntop) * layers[-1].dv
     dbt = np.abs(bt(r32, v) - bt(r64, v))
+    # Jacobians of the FP64 spectrum: T and H2O/CO2/O3 at every layer
+    jac_ms = None
+    if not a.no_jacobian:
+        nj = 4 * nlay * ntop
+        jbuf = ctx.alloc(8 * nj)
+        ctx.jacobian_device(run, [abi.Layer.from_buffer_copy(x) for x in layers], tzu, tbnd, emis, [1, 2, 3],
+                            1e-3, 1e-4, rad.ptr, jbuf.ptr, ntop)
+        ctx.sync()
+        t0 = time.perf_counter()
+        nrep = max(1, a.steps // 10)
+        for _ in range(nrep):
+            ctx.jacobian_device(run, [abi.Layer.from_buffer_copy(x) for x in layers], tzu, tbnd, emis, [1, 2, 3],
+                                1e-3, 1e-4, rad.ptr, jbuf.ptr, ntop)
+        ctx.sync()
+        jac_ms = (time.perf_counter() - t0) * 1000.0 / nrep
+        jbuf.free()
     dom = max(per, key=per.get)
     nwin = sum(s["nwin4"] for s in stats)
     nbytes = stage_bytes(dom, stats, sizes, nwin)
@@ -196,6 +213,9 @@ def main():
                                 "rms_dbt_K": float(np.sqrt(np.mean(dbt_scan ** 2))),
                                 "function": "Hamming, 1 cm OPD, 0.1 cm-1 output"},
                        "stage_ms": {k: round(v, 4) for k, v in per32.items()}},
+        "jacobian": None if jac_ms is None else {
+            "ms": jac_ms, "ratio_vs_forward": jac_ms / ms, "columns": 4 * nlay,
+            "parameters": "T, ln q of H2O, CO2, O3 at every layer; OD forward differences, exact merge tangents"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,

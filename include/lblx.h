@@ -199,6 +199,22 @@ int lblx_spectrum_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers,
                          const double *tzu, double tbnd, double emis, double *d_rad,
                          double *d_tra, int64_t n_out, double *v1p_top, lblx_stats *st);
 
+/* Jacobians of the TOA upwelling radiance (the forward/backward pair of
+ * lblx_spectrum_device) with respect to every layer's average temperature
+ * and the log column amounts of nspec molecules (species[] 1-based ids).
+ * Each layer's optical-depth derivative is a forward difference of OPDPTH
+ * (steps dT kelvin, dlnq in ln amount; the perturbed layers run in one
+ * batch with the column); the radiative part is exact: dR_l/dx on the
+ * layer grid, carried to the top through the merge of every layer above.
+ * d_rad: TOA radiance (n_out >= top-grid points); d_jac: (1 + nspec) x
+ * nlay planes of n_out, plane (q, l) at (q * nlay + l) * n_out, q = 0
+ * temperature (per K), q >= 1 species q (per unit ln amount).  Comparison
+ * role of the reference's analytic Jacobians (xmerge.f90:3792-6147).   */
+int lblx_jacobian_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay,
+                         const double *tzu, double tbnd, double emis, const int *species,
+                         int nspec, double dT, double dlnq, double *d_rad, double *d_jac,
+                         int64_t n_out, double *v1p_top);
+
 /* FFTSCN (src/fftscn_dbl.f90:25-789): record 10.1 of TAPE5.             */
 typedef struct lblx_scan {
   double hwhm;              /* HWHM field: OPD in cm when jfnin < 0        */

@@ -143,6 +143,7 @@ struct lblx_ctx {
   DevBuf w_tmap[3];
   DevBuf w_mpan, w_rt;
   DevBuf s_z[2], s_tw, s_g, s_ip, s_cof;  // FFTSCN
+  DevBuf j_rall, j_a, j_b;                // Jacobians
   int64_t s_twN = 0;
   DevLines lines() const {
     DevLines L;
@@ -255,7 +256,7 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
                     &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
-                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof};
+                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -853,11 +854,9 @@ static void merge_panels(const LayerPlan &K, const std::vector<int64_t> *base,
     out.push_back(DevPanel{K.panels[i].vft, 0, 0, 0, 0, 0, 0, base ? (*base)[i] : 0});
 }
 
-// XMERGE (IEMIT=1, IPATHL=1) for one layer on device buffers.
-static int merge_launch(lblx_ctx *c, const LayerPlan &K, const LayerPlan *O, int itype, double tave,
-                        double tzu, double tbnd, double emis, const void *d_od, bool od32,
-                        const double *d_rold, const double *d_told, double *d_rnew, double *d_tnew,
-                        const DevPanel *d_pan) {
+// MergeArgs of one layer: grid, temperatures and RADMRG's weights.
+static MergeArgs merge_args(const LayerPlan &K, const LayerPlan *O, int itype, double tave, double tzu,
+                            double tbnd, double emis) {
   MergeArgs A{};
   A.dv = K.dv;
   A.xkt = tave / RADCN2;
@@ -877,6 +876,15 @@ static int merge_launch(lblx_ctx *c, const LayerPlan &K, const LayerPlan *O, int
     A.a3[ipl] = -pp * (pp + 1.0) * (pp - 2.0) * 0.5;
     A.a4[ipl] = pp * (pp * pp - 1.0) / 6.0;
   }
+  return A;
+}
+
+// XMERGE (IEMIT=1, IPATHL=1) for one layer on device buffers.
+static int merge_launch(lblx_ctx *c, const LayerPlan &K, const LayerPlan *O, int itype, double tave,
+                        double tzu, double tbnd, double emis, const void *d_od, bool od32,
+                        const double *d_rold, const double *d_told, double *d_rnew, double *d_tnew,
+                        const DevPanel *d_pan) {
+  MergeArgs A = merge_args(K, O, itype, tave, tzu, tbnd, emis);
   int th = c->tbegin(ST_MERGE);
   dim3 grid((unsigned)((A.n + 255) / 256));
   if (od32)
@@ -1084,6 +1092,140 @@ int lblx_fftscn_device(lblx_ctx *c, const lblx_scan *scan, const double *d_in, d
   if (n_out) *n_out = P.nout;
   if (v1_out) *v1_out = P.v1out;
   if (dv_out) *dv_out = P.dvout;
+  return LBLX_OK;
+}
+
+// Jacobians of the TOA radiance with respect to each layer's temperature
+// and the log amounts of chosen molecules.  Each layer's optical-depth
+// derivative is a forward difference of OPDPTH itself (all perturbed
+// layers in one batch with the base column); the radiative part is exact:
+// dR_l/dx on the layer grid, carried to the top through every layer above
+// as J <- T_k L4(J).
+int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
+                         const double *tzu, double tbnd, double emis, const int *species, int nspec,
+                         double dT, double dlnq, double *d_rad, double *d_jac, int64_t n_out,
+                         double *v1p_top) {
+  if (!c || !run || !layers || nlay <= 0 || !tzu || !d_rad || !d_jac) return LBLX_E_ARG;
+  if (nspec < 0 || nspec > 7 || (nspec > 0 && !species) || !(dT > 0.) || !(dlnq > 0.)) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  const int P = 1 + nspec;
+  std::vector<LayerPlan> plans(nlay);
+  std::vector<lblx_layer> jobs;
+  jobs.reserve((size_t)nlay * (1 + P));
+  int jr = layers[0].jrad;
+  for (int l = 0; l < nlay; l++) {
+    lblx_layer tmp = layers[l];
+    tmp.jrad = jr;
+    int rc = plan_layer(run, &tmp, &plans[l]);
+    if (rc) return rc;
+    jr = tmp.jrad;
+    layers[l].jrad = tmp.jrad;
+    jobs.push_back(tmp);
+  }
+  for (int l = 0; l < nlay; l++)
+    for (int q = 0; q < P; q++) {
+      lblx_layer t = jobs[l];
+      if (q == 0) t.tave = t.tave + dT;
+      else {
+        int m = species[q - 1];
+        if (m < 1 || m > t.nmol) return LBLX_E_ARG;
+        t.wk[m - 1] = t.wk[m - 1] * std::exp(dlnq);
+      }
+      jobs.push_back(t);
+    }
+  const int nj = (int)jobs.size();
+  std::vector<int64_t> off(nj);
+  int64_t tot = 0, nmax = 0, tot_base = 0;
+  for (int j = 0; j < nj; j++) {
+    const LayerPlan &K = plans[j < nlay ? j : (j - nlay) / P];
+    off[j] = tot;
+    tot += K.npts;
+    if (j < nlay) tot_base += K.npts;
+    nmax = std::max(nmax, K.npts);
+  }
+  const int64_t ntop = plans[nlay - 1].npts;
+  if (n_out < ntop) return LBLX_E_SIZE;
+  // merge tables (as lblx_spectrum_device)
+  std::vector<int> itype(nlay, 0), pan0(nlay);
+  std::vector<DevPanel> pan;
+  for (int l = 0; l < nlay; l++) {
+    pan0[l] = (int)pan.size();
+    std::vector<int64_t> base;
+    if (l > 0) {
+      int rc = merge_itype(plans[l - 1].dv, plans[l].dv, &itype[l]);
+      if (rc) return rc;
+      if ((rc = merge_bases(plans[l - 1], plans[l], itype[l], run->v2, base))) return rc;
+    }
+    merge_panels(plans[l], l > 0 ? &base : nullptr, pan);
+  }
+  const bool f32 = run->precision == LBLX_FP32_SPLIT;
+  const size_t esz = f32 ? 4 : 8;
+  const int64_t plane = nmax;
+  const size_t jbytes = (size_t)P * nlay * plane * 8;
+  if (c->w_od.ensure((size_t)tot * esz) || c->w_rt.ensure((size_t)nmax * 8 * 4) ||
+      c->w_mpan.ensure(pan.size() * sizeof(DevPanel)) || c->j_rall.ensure((size_t)tot_base * 8) ||
+      c->j_a.ensure(jbytes) || c->j_b.ensure(jbytes))
+    return LBLX_E_HIP;
+  // 1. OPDPTH for the column and every perturbed layer, one batch
+  int rc = run_jobs(c, run, jobs.data(), nj, false, c->w_od.p, off.data(), f32, nullptr, nullptr);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice, s));
+  const DevPanel *dpan = c->w_mpan.as<DevPanel>();
+  double *rall = c->j_rall.as<double>();
+  double *tt[2] = {c->w_rt.as<double>(), c->w_rt.as<double>() + nmax};
+  // 2. XMERGE up the column, keeping every layer's radiance
+  for (int l = 0; l < nlay; l++) {
+    const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+    rc = merge_launch(c, plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+                      l == 0 ? tbnd : 0., emis, od, f32, l > 0 ? rall + off[l - 1] : nullptr,
+                      l > 0 ? tt[(l - 1) & 1] : nullptr, rall + off[l], tt[l & 1], dpan + pan0[l]);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(d_rad, rall + off[nlay - 1], (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
+  // 3. sources and tangent propagation
+  int th = c->tbegin(ST_MERGE);
+  double *ja = c->j_a.as<double>(), *jb = c->j_b.as<double>();
+  for (int l = 0; l < nlay; l++) {
+    MergeArgs A = merge_args(plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+                             l == 0 ? tbnd : 0., emis);
+    const dim3 grid((unsigned)((A.n + 255) / 256));
+    const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+    if (l > 0) {
+      if (f32)
+        hipLaunchKernelGGL(k_jac_prop<float>, grid, dim3(256), 0, s, A, dpan + pan0[l], (const float *)od,
+                           (const double *)ja, plane, jb, plane, l * P);
+      else
+        hipLaunchKernelGGL(k_jac_prop<double>, grid, dim3(256), 0, s, A, dpan + pan0[l], (const double *)od,
+                           (const double *)ja, plane, jb, plane, l * P);
+      std::swap(ja, jb);
+    }
+    JacSrc S{};
+    S.A = A;
+    S.tave = layers[l].tave;
+    S.stride = plane;
+    S.nparam = P;
+    for (int q = 0; q < P; q++) {
+      S.rh[q] = q == 0 ? 1.0 / dT : 1.0 / dlnq;
+      S.odp[q] = off[nlay + l * P + q] - off[l];  // relative to the layer's own optical depth
+    }
+    double *dst = ja + (int64_t)l * P * plane;
+    const double *rold = l > 0 ? rall + off[l - 1] : nullptr;
+    if (f32)
+      hipLaunchKernelGGL(k_jac_src<float>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const float *)od,
+                         (const float *)od, rold, dst);
+    else
+      hipLaunchKernelGGL(k_jac_src<double>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const double *)od,
+                         (const double *)od, rold, dst);
+  }
+  // 4. planes (layer l, parameter q) -> d_jac[(q * nlay + l) * n_out + g]
+  for (int l = 0; l < nlay; l++)
+    for (int q = 0; q < P; q++)
+      HIPCHK(hipMemcpyAsync(d_jac + ((int64_t)q * nlay + l) * n_out, ja + ((int64_t)l * P + q) * plane,
+                            (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
+  c->tend(th);
+  HIPCHK(hipGetLastError());
+  if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
   return LBLX_OK;
 }
 

@@ -1381,6 +1381,41 @@ __device__ __forceinline__ double d_planck(double vi, double xkt) {
   return v * 1.191042722E-12;
 }
 
+// The four old points and weights that RADNN combines for local point k
+// of a new panel whose carry-over slot 1 holds old point base1.  Phase
+// r = 0 reads one point (weights 1,0,0,0).
+struct MergeTap {
+  int64_t i0, i1, i2, i3;
+  double w0, w1, w2, w3;
+};
+__device__ __forceinline__ MergeTap merge_tap(const MergeArgs &A, int64_t base1, int k) {
+  const int LL = A.itype + 1;
+  const int64_t base = base1 - 1;  // old index of carry-over slot 0
+  const int m = k / LL;
+  const int r = k - m * LL;
+  const int64_t last = A.nold - 1;
+  MergeTap t;
+  if (r == 0) {
+    int64_t q = base + 1 + (int64_t)m * (A.itype > 0 ? A.itype : 1);
+    q = q < 0 ? 0 : (q > last ? last : q);
+    t.i0 = t.i1 = t.i2 = t.i3 = q;
+    t.w0 = 1.;
+    t.w1 = t.w2 = t.w3 = 0.;
+  } else {
+    int64_t q = base + r + (int64_t)m * A.itype;  // slot IPL = r + m*ITYPE
+    t.i0 = q - 1 < 0 ? 0 : (q - 1 > last ? last : q - 1);
+    t.i1 = q < 0 ? 0 : (q > last ? last : q);
+    t.i2 = q + 1 > last ? last : q + 1;
+    t.i3 = q + 2 > last ? last : q + 2;
+    const int nl = r + 1;
+    t.w0 = A.a1[nl];
+    t.w1 = A.a2[nl];
+    t.w2 = A.a3[nl];
+    t.w3 = A.a4[nl];
+  }
+  return t;
+}
+
 template <typename IN>
 __global__ void k_merge(MergeArgs A, const DevPanel *pan, const IN *od, const double *rold,
                         const double *told, double *rnew, double *tnew) {
@@ -1403,29 +1438,94 @@ __global__ void k_merge(MergeArgs A, const DevPanel *pan, const IN *od, const do
     tnew[g] = tr;
     return;
   }
-  const int LL = A.itype + 1;
-  const int64_t base = pan[p].out0 - 1;   // old index of carry-over slot 0
-  const int m = k / LL;
-  const int r = k - m * LL;
-  const int64_t last = A.nold - 1;
-  double ro, to;
-  if (r == 0) {
-    int64_t q = base + 1 + (int64_t)m * (A.itype > 0 ? A.itype : 1);
-    q = q < 0 ? 0 : (q > last ? last : q);
-    ro = rold[q];
-    to = told[q];
-  } else {
-    int64_t q = base + r + (int64_t)m * A.itype;   // slot IPL = r + m*ITYPE
-    int64_t i0 = q - 1 < 0 ? 0 : (q - 1 > last ? last : q - 1);
-    int64_t i1 = q < 0 ? 0 : (q > last ? last : q);
-    int64_t i2 = q + 1 > last ? last : q + 1;
-    int64_t i3 = q + 2 > last ? last : q + 2;
-    const int nl = r + 1;
-    ro = A.a1[nl] * rold[i0] + A.a2[nl] * rold[i1] + A.a3[nl] * rold[i2] + A.a4[nl] * rold[i3];
-    to = A.a1[nl] * told[i0] + A.a2[nl] * told[i1] + A.a3[nl] * told[i2] + A.a4[nl] * told[i3];
-  }
+  MergeTap t = merge_tap(A, pan[p].out0, k);
+  double ro = t.w0 * rold[t.i0] + t.w1 * rold[t.i1] + t.w2 * rold[t.i2] + t.w3 * rold[t.i3];
+  double to = t.w0 * told[t.i0] + t.w1 * told[t.i1] + t.w2 * told[t.i2] + t.w3 * told[t.i3];
   rnew[g] = tr * ro + em;
   tnew[g] = tr * to;
+}
+
+
+// ---------------------------------------------------------------------
+// Jacobians of the TOA radiance.  With R_l = T_l L4(R_{l-1}) + E_l the
+// TOA radiance is a composition of per-layer maps, so a parameter x of
+// layer l enters through dR_l/dx on grid l and reaches the top as the
+// tangent J <- T_k L4(J) through every layer k above (the same carry-over
+// interpolation as the radiance itself).
+//   dR_l/dx = D_l dtau_l/dx [+ (1 - T_l)(1 - F) dB(Tave)/dT for x = Tave]
+//   D_l = dR_l/dtau_l = -T_l L4(R_{l-1}) + T_l (B + dB F) + (1 - T_l) dB F'
+// (L4(R_{l-1}) -> emis B(Tbnd) at the surface layer), F as in EMIN.
+struct JacSrc {
+  MergeArgs A;
+  double tave;
+  int64_t stride;        // between parameter planes of the tangent buffer
+  int32_t nparam;
+  double rh[8];          // 1 / step of each parameter's optical-depth difference
+  int64_t odp[8];        // element offset of each perturbed optical depth
+};
+
+template <typename IN>
+__global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN *odall,
+                          const double *rold, double *jout) {
+  const MergeArgs &A = S.A;
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.n) return;
+  const int p = (int)(g / 2400);
+  const int k = (int)(g - (int64_t)2400 * p);
+  const double vi = (pan[p].vft + 32. * A.dv) + (double)k * A.dv;
+  const double od_lo = 0.06;
+  const double bb = d_planck(vi, A.xkt);
+  const double bb_dif = A.xkta > 0. ? d_planck(vi, A.xkta) - bb : 0.;
+  double odvi = (double)od[g];
+  const bool clamped = odvi < -od_lo;
+  if (clamped) odvi = -od_lo;
+  const double tr = exp(-odvi);
+  double f, fp;
+  if (odvi <= od_lo) {
+    f = (1. / 6.) * odvi;
+    fp = 1. / 6.;
+  } else {
+    const double q = tr - 1.;
+    f = 1. - 2. * (tr / q + 1. / odvi);
+    fp = -2. * (tr / (q * q) - 1. / (odvi * odvi));
+  }
+  double below;
+  if (A.first) {
+    below = A.xktbnd > 0. ? d_planck(vi, A.xktbnd) * A.emis : 0.;
+  } else {
+    MergeTap t = merge_tap(A, pan[p].out0, k);
+    below = t.w0 * rold[t.i0] + t.w1 * rold[t.i1] + t.w2 * rold[t.i2] + t.w3 * rold[t.i3];
+  }
+  double D = -tr * below + tr * (bb + bb_dif * f) + (1. - tr) * bb_dif * fp;
+  if (clamped) D = 0.;
+  // dB/dT of PLANCK at Tave (zero where PLANCK is cut off)
+  const double x = vi / A.xkt;
+  const double dbdt = (x >= 1e-2 && x <= 80.) ? bb * (x / S.tave) / (1. - exp(-x)) : 0.;
+  for (int q = 0; q < S.nparam; q++) {
+    double dtau = ((double)odall[S.odp[q] + g] - (double)od[g]) * S.rh[q];
+    double v = D * dtau;
+    if (q == 0) v = v + (1. - tr) * (1. - f) * dbdt;
+    jout[(int64_t)q * S.stride + g] = v;
+  }
+}
+
+// One layer of tangent propagation for nv vectors (plane stride istr on
+// the old grid, ostr on the new one): J_new = T_k L4(J_old).
+template <typename IN>
+__global__ void k_jac_prop(MergeArgs A, const DevPanel *pan, const IN *od, const double *jin,
+                           int64_t istr, double *jout, int64_t ostr, int nv) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.n) return;
+  const int p = (int)(g / 2400);
+  const int k = (int)(g - (int64_t)2400 * p);
+  double odvi = (double)od[g];
+  if (odvi < -0.06) odvi = -0.06;
+  const double tr = exp(-odvi);
+  MergeTap t = merge_tap(A, pan[p].out0, k);
+  for (int v = 0; v < nv; v++) {
+    const double *a = jin + (int64_t)v * istr;
+    jout[(int64_t)v * ostr + g] = tr * (t.w0 * a[t.i0] + t.w1 * a[t.i1] + t.w2 * a[t.i2] + t.w3 * a[t.i3]);
+  }
 }
 
 }  // namespace lblx

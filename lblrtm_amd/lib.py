@@ -71,6 +71,10 @@ def load():
         "lblx_fftscn_device": (C.c_int, [C.c_void_p, P(abi.Scan), C.c_void_p, C.c_double, C.c_double,
                                          C.c_int64, C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_double),
                                          P(C.c_double)]),
+        "lblx_jacobian_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_double),
+                                           C.c_double, C.c_double, P(C.c_int), C.c_int, C.c_double,
+                                           C.c_double, C.c_void_p, C.c_void_p, C.c_int64,
+                                           P(C.c_double)]),
         "lblx_merge_size": (C.c_int, [P(abi.Run), P(abi.Layer), P(abi.Layer), P(C.c_int64)]),
         "lblx_xmerge_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer),
                                          C.c_double, C.c_double, C.c_double, C.c_void_p,
@@ -258,6 +262,35 @@ class Context:
         finally:
             rad.free()
             tra.free()
+
+    def jacobian_device(self, run, layers, tzu, tbnd, emis, species, dT, dlnq, d_rad, d_jac, n_out):
+        """TOA radiance and its Jacobians (lblx_jacobian_device); returns v1p
+        of the top grid."""
+        n = len(layers)
+        arr = (abi.Layer * n)(*layers)
+        tz = np.ascontiguousarray(tzu, np.float64)
+        sp = (C.c_int * max(1, len(species)))(*species)
+        v1p = C.c_double(0.0)
+        _chk(self._lib.lblx_jacobian_device(self._h, C.byref(run), arr, n, abi.ptr(tz, C.c_double), tbnd, emis,
+                                            sp, len(species), dT, dlnq, C.c_void_p(d_rad), C.c_void_p(d_jac),
+                                            n_out, C.byref(v1p)), "lblx_jacobian_device")
+        for k in range(n):
+            layers[k].jrad = arr[k].jrad
+        return v1p.value
+
+    def jacobian(self, run, layers, tzu, tbnd, emis, species, dT=1e-3, dlnq=1e-4):
+        """Host-side convenience: (radiance, jac[q, layer, point], v1p, dv)."""
+        n = od_size(run, layers[-1])
+        P = 1 + len(species)
+        rad = self.alloc(8 * n)
+        jac = self.alloc(8 * n * P * len(layers))
+        try:
+            v1p = self.jacobian_device(run, layers, tzu, tbnd, emis, species, dT, dlnq, rad.ptr, jac.ptr, n)
+            return (rad.to_host(np.float64), jac.to_host(np.float64).reshape(P, len(layers), n), v1p,
+                    layers[-1].dv)
+        finally:
+            rad.free()
+            jac.free()
 
     def fftscn_device(self, scan, d_in, v1_in, dv_in, n_in, d_out, cap):
         """FFTSCN on device pointers; returns (points, v1, dv) of the output."""
