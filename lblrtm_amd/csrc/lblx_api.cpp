@@ -1183,15 +1183,47 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     if (rc) return rc;
   }
   HIPCHK(hipMemcpyAsync(d_rad, rall + off[nlay - 1], (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
-  // 3. sources and tangent propagation
+  // 3. sources and tangent propagation.  The layers at the top whose merge
+  // is the identity map (ITYPE 0, same panels: J <- T_k J pointwise) are
+  // handled from the top down with a running transmittance product; below
+  // them every tangent is carried layer by layer.
+  int K0 = nlay;
+  for (int l = nlay - 1; l >= 1; l--) {
+    bool ident = itype[l] == 0 && plans[l].npts == plans[l - 1].npts;
+    for (size_t p = 0; ident && p < plans[l].panels.size(); p++)
+      ident = pan[pan0[l] + p].out0 == (int64_t)2400 * (int64_t)p;
+    if (!ident) break;
+    K0 = l;
+  }
   int th = c->tbegin(ST_MERGE);
   double *ja = c->j_a.as<double>(), *jb = c->j_b.as<double>();
-  for (int l = 0; l < nlay; l++) {
+  auto src_launch = [&](int l, double *dst, int64_t stride, const double *scale) {
     MergeArgs A = merge_args(plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
                              l == 0 ? tbnd : 0., emis);
     const dim3 grid((unsigned)((A.n + 255) / 256));
     const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+    JacSrc S{};
+    S.A = A;
+    S.tave = layers[l].tave;
+    S.stride = stride;
+    S.nparam = P;
+    for (int q = 0; q < P; q++) {
+      S.rh[q] = q == 0 ? 1.0 / dT : 1.0 / dlnq;
+      S.odp[q] = off[nlay + l * P + q] - off[l];  // relative to the layer's own optical depth
+    }
+    const double *rold = l > 0 ? rall + off[l - 1] : nullptr;
+    if (f32)
+      hipLaunchKernelGGL(k_jac_src<float>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const float *)od,
+                         (const float *)od, rold, dst, scale);
+    else
+      hipLaunchKernelGGL(k_jac_src<double>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const double *)od,
+                         (const double *)od, rold, dst, scale);
+  };
+  for (int l = 0; l < K0; l++) {
     if (l > 0) {
+      MergeArgs A = merge_args(plans[l], &plans[l - 1], itype[l], layers[l].tave, tzu[l], 0., emis);
+      const dim3 grid((unsigned)((A.n + 255) / 256));
+      const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
       if (f32)
         hipLaunchKernelGGL(k_jac_prop<float>, grid, dim3(256), 0, s, A, dpan + pan0[l], (const float *)od,
                            (const double *)ja, plane, jb, plane, l * P);
@@ -1200,29 +1232,33 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
                            (const double *)ja, plane, jb, plane, l * P);
       std::swap(ja, jb);
     }
-    JacSrc S{};
-    S.A = A;
-    S.tave = layers[l].tave;
-    S.stride = plane;
-    S.nparam = P;
-    for (int q = 0; q < P; q++) {
-      S.rh[q] = q == 0 ? 1.0 / dT : 1.0 / dlnq;
-      S.odp[q] = off[nlay + l * P + q] - off[l];  // relative to the layer's own optical depth
-    }
-    double *dst = ja + (int64_t)l * P * plane;
-    const double *rold = l > 0 ? rall + off[l - 1] : nullptr;
-    if (f32)
-      hipLaunchKernelGGL(k_jac_src<float>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const float *)od,
-                         (const float *)od, rold, dst);
-    else
-      hipLaunchKernelGGL(k_jac_src<double>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const double *)od,
-                         (const double *)od, rold, dst);
+    src_launch(l, ja + (int64_t)l * P * plane, plane, nullptr);
   }
-  // 4. planes (layer l, parameter q) -> d_jac[(q * nlay + l) * n_out + g]
-  for (int l = 0; l < nlay; l++)
-    for (int q = 0; q < P; q++)
-      HIPCHK(hipMemcpyAsync(d_jac + ((int64_t)q * nlay + l) * n_out, ja + ((int64_t)l * P + q) * plane,
-                            (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
+  const dim3 gtop((unsigned)((ntop + 255) / 256));
+  const int64_t qstr = (int64_t)nlay * n_out;  // between parameter planes of d_jac
+  if (K0 < nlay) {
+    double *tt = jb;  // running product of the transmittances above
+    hipLaunchKernelGGL(k_fill_f64, gtop, dim3(256), 0, s, tt, ntop, 1.0);
+    for (int l = nlay - 1; l >= K0; l--) {
+      src_launch(l, d_jac + (int64_t)l * n_out, qstr, tt);
+      const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+      if (f32) hipLaunchKernelGGL(k_jac_trans<float>, gtop, dim3(256), 0, s, (const float *)od, ntop, tt);
+      else hipLaunchKernelGGL(k_jac_trans<double>, gtop, dim3(256), 0, s, (const double *)od, ntop, tt);
+    }
+    // layers below the identity region: their tangents on grid K0-1 times
+    // the product of every transmittance above
+    for (int l = 0; l < K0; l++)
+      for (int q = 0; q < P; q++)
+        hipLaunchKernelGGL(k_jac_scale, gtop, dim3(256), 0, s, (const double *)(ja + ((int64_t)l * P + q) * plane),
+                           plane, (const double *)tt, ntop, d_jac + (int64_t)q * qstr + (int64_t)l * n_out,
+                           n_out, 1);
+  } else {
+    // 4. planes (layer l, parameter q) -> d_jac[(q * nlay + l) * n_out + g]
+    for (int l = 0; l < nlay; l++)
+      for (int q = 0; q < P; q++)
+        HIPCHK(hipMemcpyAsync(d_jac + (int64_t)q * qstr + (int64_t)l * n_out, ja + ((int64_t)l * P + q) * plane,
+                              (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
+  }
   c->tend(th);
   HIPCHK(hipGetLastError());
   if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;

@@ -1466,7 +1466,7 @@ struct JacSrc {
 
 template <typename IN>
 __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN *odall,
-                          const double *rold, double *jout) {
+                          const double *rold, double *jout, const double *scale) {
   const MergeArgs &A = S.A;
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.n) return;
@@ -1505,6 +1505,7 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
     double dtau = ((double)odall[S.odp[q] + g] - (double)od[g]) * S.rh[q];
     double v = D * dtau;
     if (q == 0) v = v + (1. - tr) * (1. - f) * dbdt;
+    if (scale) v = v * scale[g];
     jout[(int64_t)q * S.stride + g] = v;
   }
 }
@@ -1526,6 +1527,31 @@ __global__ void k_jac_prop(MergeArgs A, const DevPanel *pan, const IN *od, const
     const double *a = jin + (int64_t)v * istr;
     jout[(int64_t)v * ostr + g] = tr * (t.w0 * a[t.i0] + t.w1 * a[t.i1] + t.w2 * a[t.i2] + t.w3 * a[t.i3]);
   }
+}
+
+
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Running transmittance product of the layers above: tt *= exp(-tau).
+template <typename IN>
+__global__ void k_jac_trans(const IN *od, int64_t n, double *tt) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  double odvi = (double)od[g];
+  if (odvi < -0.06) odvi = -0.06;
+  tt[g] = tt[g] * exp(-odvi);
+}
+
+// dst plane v = tt * src plane v (nv planes, strides istr / ostr).
+__global__ void k_jac_scale(const double *src, int64_t istr, const double *tt, int64_t n, double *dst,
+                            int64_t ostr, int nv) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const double t = tt[g];
+  for (int v = 0; v < nv; v++) dst[(int64_t)v * ostr + g] = t * src[(int64_t)v * istr + g];
 }
 
 }  // namespace lblx
