@@ -215,6 +215,25 @@ int lblx_jacobian_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers,
                          int nspec, double dT, double dlnq, double *d_rad, double *d_jac,
                          int64_t n_out, double *v1p_top);
 
+/* A batch of independent profiles (ragged: nlay[p] layers each, surface
+ * first, concatenated in layers/tzu; each profile's own band in its
+ * layers' v1/v2).  Layer optical depths run in batches of whole profiles
+ * sized to the device memory, and the merges of a batch run one launch per
+ * layer index over all its profiles, the launch chain captured as a HIP
+ * graph and replayed on later calls of the same shape.  Profile p's TOA
+ * radiance/transmittance at d_rad/d_tra + out_off[p] (its top-grid size,
+ * lblx_od_size of its last layer); v1p_top[p] its first wavenumber.      */
+int lblx_spectra_batch_device(lblx_ctx *ctx, const lblx_run *run, int nprof, const int *nlay,
+                              lblx_layer *layers, const double *tzu, const double *tbnd,
+                              const double *emis, double *d_rad, double *d_tra,
+                              const int64_t *out_off, double *v1p_top);
+
+/* Tile widths (points) of the R1, R2, R3 near-wing scatter: chosen by
+ * timing candidates on the given layers (lblx_autotune), or set directly
+ * (multiples of 64 up to 8192).  They change speed, not results.         */
+int lblx_autotune(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay, int *tiles_out);
+int lblx_set_tiles(lblx_ctx *ctx, const int *tiles);
+
 /* FFTSCN (src/fftscn_dbl.f90:25-789): record 10.1 of TAPE5.             */
 typedef struct lblx_scan {
   double hwhm;              /* HWHM field: OPD in cm when jfnin < 0        */

@@ -145,6 +145,13 @@ struct lblx_ctx {
   DevBuf s_z[2], s_tw, s_g, s_ip, s_cof;  // FFTSCN
   DevBuf j_rall, j_a, j_b;                // Jacobians
   int64_t s_twN = 0;
+  int tiles[3] = {2048, 1024, 512};       // R1, R2, R3 tile widths (autotuned)
+  DevBuf b_jobs, b_rt;                    // batched merges
+  struct GraphEntry {
+    std::vector<int64_t> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<GraphEntry> graphs;         // instantiated merge chains
   DevLines lines() const {
     DevLines L;
     L.n = nlines;
@@ -248,6 +255,8 @@ int lblx_create(lblx_ctx **out, int device) {
 void lblx_destroy(lblx_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
+  for (auto &g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   DevBuf *bufs[] = {&c->d_vnu, &c->d_sp, &c->d_alfa, &c->d_epp, &c->d_hwhm, &c->d_tmpalf,
                     &c->d_pshift, &c->d_mol, &c->d_blk, &c->d_bstart, &c->d_bvmax, &c->d_tabs, &c->d_cont, &c->w_cps,
                     &c->w_dlay, &c->w_pan, &c->w_win, &c->w_seg, &c->w_r4, &c->w_margin,
@@ -256,7 +265,7 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
                     &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
-                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b};
+                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -459,7 +468,7 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   abs_stride = (abs_stride + 63) / 64 * 64;
   for (int jb = 0; jb < njobs; jb++) ys[jb].abs0 = (int64_t)jb * abs_stride;
   // tile maps
-  const int tiles[3] = {2048, 1024, 512};
+  const int tiles[3] = {c->tiles[0], c->tiles[1], c->tiles[2]};
   std::vector<int32_t> tjob[3], tloc[3];
   for (int jb = 0; jb < njobs; jb++) {
     const LayerPlan &P = plans[jb];
@@ -912,7 +921,7 @@ int lblx_merge_size(const lblx_run *run, const lblx_layer *layer_old, const lblx
     int itype;
     if ((rc = merge_itype(O.dv, K.dv, &itype))) return rc;
     std::vector<int64_t> base;
-    if ((rc = merge_bases(O, K, itype, run->v2, base))) return rc;
+    if ((rc = merge_bases(O, K, itype, K.v2, base))) return rc;
   }
   *n = K.npts;
   return LBLX_OK;
@@ -935,7 +944,7 @@ int lblx_xmerge_device(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer
     lblx_layer t2 = *layer_old;
     if ((rc = plan_layer(run, &t2, &O))) return rc;
     if ((rc = merge_itype(O.dv, K.dv, &itype))) return rc;
-    if ((rc = merge_bases(O, K, itype, run->v2, base))) return rc;
+    if ((rc = merge_bases(O, K, itype, K.v2, base))) return rc;
   }
   std::vector<DevPanel> pan;
   merge_panels(K, layer_old ? &base : nullptr, pan);
@@ -980,7 +989,7 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     if (l > 0) {
       int rc = merge_itype(plans[l - 1].dv, plans[l].dv, &itype[l]);
       if (rc) return rc;
-      if ((rc = merge_bases(plans[l - 1], plans[l], itype[l], run->v2, base))) return rc;
+      if ((rc = merge_bases(plans[l - 1], plans[l], itype[l], plans[l].v2, base))) return rc;
     }
     merge_panels(plans[l], l > 0 ? &base : nullptr, pan);
   }
@@ -1154,7 +1163,7 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     if (l > 0) {
       int rc = merge_itype(plans[l - 1].dv, plans[l].dv, &itype[l]);
       if (rc) return rc;
-      if ((rc = merge_bases(plans[l - 1], plans[l], itype[l], run->v2, base))) return rc;
+      if ((rc = merge_bases(plans[l - 1], plans[l], itype[l], plans[l].v2, base))) return rc;
     }
     merge_panels(plans[l], l > 0 ? &base : nullptr, pan);
   }
@@ -1262,6 +1271,229 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
   c->tend(th);
   HIPCHK(hipGetLastError());
   if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
+  return LBLX_OK;
+}
+
+// Per-job device bytes of run_jobs (line-table-sized work arrays).
+static size_t job_bytes(const lblx_ctx *c) {
+  const size_t line = ((size_t)c->nlines + 127) / 64 * 64;
+  return line * (32 + 1 + 12 + 4 * NLEV + 2 * NLEV + 12 + 36 + sizeof(NearLine) + 8 + sizeof(ScatterRec)) +
+         (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 2) + (size_t)(c->nlines / 1000 + 16) * 24;
+}
+
+// Many profiles in one call: OPDPTH for the layers of a group of profiles
+// in one batch, then XMERGE for all profiles of the group with one launch
+// per layer index (ragged columns), the chain of merge launches captured
+// once as a HIP graph and replayed on later calls of the same shape.
+int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const int *nlay,
+                              lblx_layer *layers, const double *tzu, const double *tbnd,
+                              const double *emis, double *d_rad, double *d_tra, const int64_t *out_off,
+                              double *v1p_top) {
+  if (!c || !run || nprof <= 0 || !nlay || !layers || !tzu || !tbnd || !emis || !d_rad || !d_tra || !out_off)
+    return LBLX_E_ARG;
+  if (c->nlines <= 0) return LBLX_E_NOLINES;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  std::vector<int> first(nprof + 1, 0);
+  for (int p = 0; p < nprof; p++) {
+    if (nlay[p] <= 0) return LBLX_E_ARG;
+    first[p + 1] = first[p] + nlay[p];
+  }
+  const int ntot = first[nprof];
+  std::vector<LayerPlan> plans(ntot);
+  std::vector<int> itype(ntot, 0), pan0(ntot, 0);
+  std::vector<DevPanel> pan;
+  for (int p = 0; p < nprof; p++) {
+    int jr = layers[first[p]].jrad;
+    for (int l = first[p]; l < first[p + 1]; l++) {
+      layers[l].jrad = jr;
+      int rc = plan_layer(run, &layers[l], &plans[l]);
+      if (rc) return rc;
+      jr = layers[l].jrad;
+      pan0[l] = (int)pan.size();
+      std::vector<int64_t> base;
+      if (l > first[p]) {
+        rc = merge_itype(plans[l - 1].dv, plans[l].dv, &itype[l]);
+        if (rc) return rc;
+        if ((rc = merge_bases(plans[l - 1], plans[l], itype[l], plans[l].v2, base))) return rc;
+      }
+      merge_panels(plans[l], l > first[p] ? &base : nullptr, pan);
+    }
+    if (v1p_top) {
+      const LayerPlan &T = plans[first[p + 1] - 1];
+      v1p_top[p] = T.panels[0].vft + (double)NSHIFT * T.dv;
+    }
+  }
+  if (c->w_mpan.ensure(pan.size() * sizeof(DevPanel))) return LBLX_E_HIP;
+  HIPCHK(hipMemcpy(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice));
+  const DevPanel *dpan = c->w_mpan.as<DevPanel>();
+  // groups of whole profiles within the device-memory budget
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  const size_t budget = std::min<size_t>(free_b / 2, (size_t)96 << 30);
+  const size_t jb = std::max<size_t>(job_bytes(c), 1);
+  const int max_jobs = (int)std::max<size_t>(1, budget / jb);
+  const bool f32 = run->precision == LBLX_FP32_SPLIT;
+  const size_t esz = f32 ? 4 : 8;
+  int p0 = 0, gidx = 0;
+  while (p0 < nprof) {
+    int p1 = p0, njobs = 0;
+    while (p1 < nprof && (njobs == 0 || njobs + nlay[p1] <= max_jobs)) njobs += nlay[p1++];
+    // 1. optical depths of every layer of profiles p0..p1-1
+    std::vector<int64_t> off(njobs);
+    int64_t tot = 0;
+    std::vector<int64_t> nmax(p1 - p0, 0), rt_off(p1 - p0, 0);
+    int64_t rt_tot = 0;
+    int maxl = 0;
+    for (int p = p0; p < p1; p++) {
+      for (int l = first[p]; l < first[p + 1]; l++) {
+        off[l - first[p0]] = tot;
+        tot += plans[l].npts;
+        nmax[p - p0] = std::max(nmax[p - p0], plans[l].npts);
+      }
+      rt_off[p - p0] = rt_tot;
+      rt_tot += 4 * nmax[p - p0];
+      maxl = std::max(maxl, nlay[p]);
+    }
+    if (c->w_od.ensure((size_t)tot * esz) || c->b_rt.ensure((size_t)rt_tot * 8)) return LBLX_E_HIP;
+    int rc = run_jobs(c, run, layers + first[p0], njobs, false, c->w_od.p, off.data(), f32, nullptr, nullptr);
+    if (rc) return rc;
+    // 2. merge descriptors for every layer index, uploaded once
+    std::vector<MergeJob> mj;
+    std::vector<int> kstart(maxl + 1, 0), kcount(maxl, 0);
+    std::vector<int64_t> key = {(int64_t)gidx, (int64_t)f32, (int64_t)maxl};
+    for (int k = 0; k < maxl; k++) {
+      kstart[k] = (int)mj.size();
+      int64_t nk = 0;
+      for (int p = p0; p < p1; p++) {
+        if (k >= nlay[p]) continue;
+        const int l = first[p] + k;
+        double *rt = c->b_rt.as<double>() + rt_off[p - p0];
+        const int64_t m = nmax[p - p0];
+        double *R[2] = {rt, rt + m}, *T[2] = {rt + 2 * m, rt + 3 * m};
+        const bool last = k == nlay[p] - 1;
+        MergeJob J;
+        J.A = merge_args(plans[l], k > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+                         k == 0 ? tbnd[p] : 0., emis[p]);
+        J.pan = dpan + pan0[l];
+        J.od = (const char *)c->w_od.p + (size_t)off[l - first[p0]] * esz;
+        J.rold = k > 0 ? R[(k - 1) & 1] : nullptr;
+        J.told = k > 0 ? T[(k - 1) & 1] : nullptr;
+        J.rnew = last ? d_rad + out_off[p] : R[k & 1];
+        J.tnew = last ? d_tra + out_off[p] : T[k & 1];
+        mj.push_back(J);
+        nk = std::max(nk, plans[l].npts);
+      }
+      kcount[k] = (int)mj.size() - kstart[k];
+      key.push_back(kcount[k]);
+      key.push_back(nk);
+    }
+    if (c->b_jobs.ensure(mj.size() * sizeof(MergeJob))) return LBLX_E_HIP;
+    key.push_back((int64_t)(intptr_t)c->b_jobs.p);  // the graph holds this address
+    HIPCHK(hipMemcpyAsync(c->b_jobs.p, mj.data(), mj.size() * sizeof(MergeJob), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));  // descriptors in place before the graph runs
+    // 3. the chain of merge launches as a graph (instantiated once per shape)
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : c->graphs)
+      if (g.key == key) exec = g.exec;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const MergeJob *dj = c->b_jobs.as<MergeJob>();
+      for (int k = 0; k < maxl; k++) {
+        const int64_t nk = key[3 + 2 * k + 1];
+        dim3 grid((unsigned)((nk + 255) / 256), (unsigned)kcount[k]);
+        if (f32) hipLaunchKernelGGL(k_merge_batch<float>, grid, dim3(256), 0, s, dj + kstart[k]);
+        else hipLaunchKernelGGL(k_merge_batch<double>, grid, dim3(256), 0, s, dj + kstart[k]);
+      }
+      HIPCHK(hipStreamEndCapture(s, &graph));
+      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      if (c->graphs.size() >= 64) {  // bounded cache
+        (void)hipGraphExecDestroy(c->graphs.front().exec);
+        c->graphs.erase(c->graphs.begin());
+      }
+      c->graphs.push_back(lblx_ctx::GraphEntry{key, exec});
+    }
+    int th = c->tbegin(ST_MERGE);
+    HIPCHK(hipGraphLaunch(exec, s));
+    c->tend(th);
+    // the next group reuses the descriptor buffer
+    HIPCHK(hipStreamSynchronize(s));
+    p0 = p1;
+    gidx++;
+  }
+  HIPCHK(hipGetLastError());
+  return LBLX_OK;
+}
+
+// Tile widths of the near-wing scatter, chosen by timing the three tile
+// stages on the given layers for each candidate (coordinate search).
+int lblx_autotune(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay, int *tiles_out) {
+  if (!c || !run || !layers || nlay <= 0) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<int64_t> off(nlay);
+  int64_t tot = 0;
+  for (int l = 0; l < nlay; l++) {
+    lblx_layer t = layers[l];
+    LayerPlan P;
+    int rc = plan_layer(run, &t, &P);
+    if (rc) return rc;
+    off[l] = tot;
+    tot += P.npts;
+  }
+  const size_t esz = run->precision == LBLX_FP32_SPLIT ? 4 : 8;
+  if (c->w_od.ensure((size_t)tot * esz)) return LBLX_E_HIP;
+  const bool was = c->timing;
+  auto trial = [&](int stage) -> double {
+    std::vector<lblx_layer> tmp(layers, layers + nlay);
+    c->tcollect();
+    c->timing = true;
+    for (int i = 0; i < ST_N; i++) {
+      c->stage_ms[i] = 0.;
+      c->stage_n[i] = 0;
+    }
+    double best = 1e30;
+    for (int rep = 0; rep < 2; rep++) {  // first pass grows buffers
+      std::vector<lblx_layer> t2 = tmp;
+      if (run_jobs(c, run, t2.data(), nlay, true, c->w_od.p, off.data(), esz == 4, nullptr, nullptr)) return 1e30;
+      c->tcollect();
+      best = rep ? std::min(best, c->stage_ms[stage]) : best;
+      c->stage_ms[stage] = 0.;
+    }
+    return best;
+  };
+  const int cand[3][4] = {{1024, 2048, 4096, 8192}, {512, 1024, 2048, 4096}, {256, 512, 1024, 2048}};
+  const int stage[3] = {ST_TILE1, ST_TILE2, ST_TILE3};
+  for (int G = 0; G < 3; G++) {
+    int bestw = c->tiles[G];
+    double bt = 1e30;
+    for (int w : cand[G]) {
+      c->tiles[G] = w;
+      double t = trial(stage[G]);
+      if (t < bt) {
+        bt = t;
+        bestw = w;
+      }
+    }
+    c->tiles[G] = bestw;
+  }
+  c->timing = was;
+  for (int i = 0; i < ST_N; i++) {
+    c->stage_ms[i] = 0.;
+    c->stage_n[i] = 0;
+  }
+  if (tiles_out)
+    for (int G = 0; G < 3; G++) tiles_out[G] = c->tiles[G];
+  return LBLX_OK;
+}
+
+int lblx_set_tiles(lblx_ctx *c, const int *tiles) {
+  if (!c || !tiles) return LBLX_E_ARG;
+  const int lo[3] = {256, 128, 64}, hi[3] = {8192, 8192, 8192};
+  for (int G = 0; G < 3; G++)
+    if (tiles[G] < lo[G] || tiles[G] > hi[G] || (tiles[G] & 63)) return LBLX_E_ARG;
+  for (int G = 0; G < 3; G++) c->tiles[G] = tiles[G];
   return LBLX_OK;
 }
 

@@ -1530,6 +1530,45 @@ __global__ void k_jac_prop(MergeArgs A, const DevPanel *pan, const IN *od, const
 }
 
 
+// One layer merge of many profiles in one launch (blockIdx.y = profile).
+struct MergeJob {
+  MergeArgs A;
+  const DevPanel *pan;
+  const void *od;        // double, or float for FP32 split
+  const double *rold, *told;
+  double *rnew, *tnew;
+};
+
+template <typename IN>
+__global__ void k_merge_batch(const MergeJob *jobs) {
+  const MergeJob &J = jobs[blockIdx.y];
+  const MergeArgs &A = J.A;
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.n) return;
+  const int p = (int)(g / 2400);
+  const int k = (int)(g - (int64_t)2400 * p);
+  const double vi = (J.pan[p].vft + 32. * A.dv) + (double)k * A.dv;
+  const double od_lo = 0.06;
+  double bb = d_planck(vi, A.xkt);
+  double bb_dif = A.xkta > 0. ? d_planck(vi, A.xkta) - bb : 0.;
+  double odvi = (double)((const IN *)J.od)[g];
+  if (odvi < -od_lo) odvi = -od_lo;
+  double tr = exp(-odvi);
+  double f = odvi <= od_lo ? (1. / 6.) * odvi : 1. - 2. * (tr / (tr - 1.) + 1. / odvi);
+  double em = (1. - tr) * (bb + bb_dif * f);
+  if (A.first) {
+    if (A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
+    J.rnew[g] = em;
+    J.tnew[g] = tr;
+    return;
+  }
+  MergeTap t = merge_tap(A, J.pan[p].out0, k);
+  double ro = t.w0 * J.rold[t.i0] + t.w1 * J.rold[t.i1] + t.w2 * J.rold[t.i2] + t.w3 * J.rold[t.i3];
+  double to = t.w0 * J.told[t.i0] + t.w1 * J.told[t.i1] + t.w2 * J.told[t.i2] + t.w3 * J.told[t.i3];
+  J.rnew[g] = tr * ro + em;
+  J.tnew[g] = tr * to;
+}
+
 __global__ void k_fill_f64(double *p, int64_t n, double v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

@@ -75,6 +75,11 @@ def load():
                                            C.c_double, C.c_double, P(C.c_int), C.c_int, C.c_double,
                                            C.c_double, C.c_void_p, C.c_void_p, C.c_int64,
                                            P(C.c_double)]),
+        "lblx_spectra_batch_device": (C.c_int, [C.c_void_p, P(abi.Run), C.c_int, P(C.c_int), P(abi.Layer),
+                                                P(C.c_double), P(C.c_double), P(C.c_double), C.c_void_p,
+                                                C.c_void_p, P(C.c_int64), P(C.c_double)]),
+        "lblx_autotune": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int)]),
+        "lblx_set_tiles": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lblx_merge_size": (C.c_int, [P(abi.Run), P(abi.Layer), P(abi.Layer), P(C.c_int64)]),
         "lblx_xmerge_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer),
                                          C.c_double, C.c_double, C.c_double, C.c_void_p,
@@ -291,6 +296,37 @@ class Context:
         finally:
             rad.free()
             jac.free()
+
+    def spectra_batch_device(self, run, profiles, d_rad, d_tra, out_off):
+        """Batch of profiles: profiles is a list of (layers, tzu, tbnd, emis).
+        Returns v1p of each profile's top grid."""
+        nl = [len(p[0]) for p in profiles]
+        lays = [abi.Layer.from_buffer_copy(x) for p in profiles for x in p[0]]
+        arr = (abi.Layer * len(lays))(*lays)
+        tz = np.ascontiguousarray(np.concatenate([np.asarray(p[1], np.float64) for p in profiles]))
+        tb = np.ascontiguousarray([p[2] for p in profiles], np.float64)
+        em = np.ascontiguousarray([p[3] for p in profiles], np.float64)
+        oo = np.ascontiguousarray(out_off, np.int64)
+        v1p = np.zeros(len(profiles))
+        nla = (C.c_int * len(nl))(*nl)
+        _chk(self._lib.lblx_spectra_batch_device(self._h, C.byref(run), len(profiles), nla, arr,
+                                                 abi.ptr(tz, C.c_double), abi.ptr(tb, C.c_double),
+                                                 abi.ptr(em, C.c_double), C.c_void_p(d_rad), C.c_void_p(d_tra),
+                                                 abi.ptr(oo, C.c_int64), abi.ptr(v1p, C.c_double)),
+             "lblx_spectra_batch_device")
+        return v1p
+
+    def autotune(self, run, layers):
+        """Time the near-wing tile widths on these layers; returns (R1, R2, R3)."""
+        n = len(layers)
+        arr = (abi.Layer * n)(*[abi.Layer.from_buffer_copy(x) for x in layers])
+        t = (C.c_int * 3)()
+        _chk(self._lib.lblx_autotune(self._h, C.byref(run), arr, n, t), "lblx_autotune")
+        return tuple(t)
+
+    def set_tiles(self, tiles):
+        t = (C.c_int * 3)(*tiles)
+        _chk(self._lib.lblx_set_tiles(self._h, t), "lblx_set_tiles")
 
     def fftscn_device(self, scan, d_in, v1_in, dv_in, n_in, d_out, cap):
         """FFTSCN on device pointers; returns (points, v1, dv) of the output."""

@@ -90,3 +90,21 @@ def atmosphere(nlev=51, psfc=1013.0, ptop=0.1, tsfc=295.0, seed=SEED0 + 1, nmol=
         wk[:, m] = vmr * col
     wbroad = col - wk.sum(axis=1)
     return dict(pave=pave, tave=tave, wk=wk, wbroad=wbroad, z=z, plev=p, tlev=tl)
+
+
+def batch_profiles(nprof=256, seed=SEED0 + 4, v1=500.0, v2=2500.0):
+    """Ragged batch of independent profiles: layer counts uniform 20-80,
+    surface pressure uniform 600-1050 hPa, band windows random 200-800 cm-1
+    sub-ranges of [v1, v2].  Returns a list of dicts (atmosphere() fields
+    plus 'v1', 'v2', 'nlay')."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(nprof):
+        nlay = int(rng.integers(20, 81))
+        psfc = float(rng.uniform(600.0, 1050.0))
+        width = float(rng.uniform(200.0, 800.0))
+        a = float(rng.uniform(v1, v2 - width))
+        atm = atmosphere(nlev=nlay + 1, psfc=psfc, seed=seed * 1000 + k)
+        atm.update(v1=round(a, 3), v2=round(a + width, 3), nlay=nlay)
+        out.append(atm)
+    return out

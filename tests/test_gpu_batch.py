@@ -1,0 +1,70 @@
+"""Batched profiles on the GPU (lblx_spectra_batch_device): ragged columns
+with their own surface pressures and band windows, each equal to the same
+profile run alone through lblx_spectrum_device (itself pinned to the
+oracle); the merge graph replayed; autotuned tile widths leave results
+unchanged."""
+import numpy as np
+import pytest
+
+from lblrtm_amd import abi, lib, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _profile_layers(run, atm):
+    dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    return [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l], dv[l],
+                           atm["v1"], atm["v2"]) for l in range(len(dv))]
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_SPLIT])
+def test_batch_equals_single_profiles(gpu_ctx, precision):
+    tab = synth.config1_lines()
+    gpu_ctx.load_lines(tab)
+    profs = synth.batch_profiles(6, seed=77)
+    run = abi.make_run(500.0, 2500.0, icntnm=1, precision=precision)
+    items = []
+    for atm in profs:
+        r = abi.make_run(atm["v1"], atm["v2"], icntnm=1, precision=precision)
+        items.append((_profile_layers(r, atm), atm["tlev"][1:], atm["tlev"][0], 0.98))
+    sizes = [lib.od_size(run, it[0][-1]) for it in items]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    rad = gpu_ctx.alloc(8 * int(sum(sizes)))
+    tra = gpu_ctx.alloc(8 * int(sum(sizes)))
+    for rep in range(3):  # first call captures the merge graph, later ones replay it
+        v1p = gpu_ctx.spectra_batch_device(run, items, rad.ptr, tra.ptr, offs)
+        rb = rad.to_host(np.float64)
+        tb = tra.to_host(np.float64)
+        for k, it in enumerate(items):
+            r = abi.make_run(profs[k]["v1"], profs[k]["v2"], icntnm=1, precision=precision)
+            rs, ts, v1s, _ = gpu_ctx.spectrum(r, [abi.Layer.from_buffer_copy(x) for x in it[0]], it[1], it[2], it[3])
+            print("rep %d profile %d: %d layers %.1f-%.1f cm-1, %d points" %
+                  (rep, k, len(it[0]), profs[k]["v1"], profs[k]["v2"], len(rs)), flush=True)
+            assert v1p[k] == v1s
+            # FP64: same kernels, only the order of LDS atomic adds differs;
+            # FP32 split: float atomics in a different order, ~1e-9 relative
+            tol = 1e-12 if precision == abi.FP64 else 1e-6
+            np.testing.assert_allclose(rb[offs[k]:offs[k] + sizes[k]], rs, rtol=tol)
+            np.testing.assert_allclose(tb[offs[k]:offs[k] + sizes[k]], ts, rtol=tol, atol=1e-300)
+    rad.free()
+    tra.free()
+
+
+def test_autotune_keeps_results(gpu_ctx):
+    tab = synth.config1_lines()
+    gpu_ctx.load_lines(tab)
+    atm = synth.atmosphere()
+    run = abi.make_run(900.0, 1000.0, icntnm=1)
+    dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    layers = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l], dv[l],
+                             900.0, 1000.0) for l in range(len(dv))]
+    tz = atm["tlev"][1:]
+    r0 = gpu_ctx.spectrum(run, [abi.Layer.from_buffer_copy(x) for x in layers], tz, 295.0, 0.98)[0]
+    tiles = gpu_ctx.autotune(run, layers[::5])
+    print("autotuned tiles", tiles, flush=True)
+    assert all(t % 64 == 0 for t in tiles)
+    r1 = gpu_ctx.spectrum(run, [abi.Layer.from_buffer_copy(x) for x in layers], tz, 295.0, 0.98)[0]
+    np.testing.assert_allclose(r1, r0, rtol=1e-12)
+    gpu_ctx.set_tiles((2048, 1024, 512))
+    with pytest.raises(lib.LblxError):
+        gpu_ctx.set_tiles((100, 1024, 512))
