@@ -270,10 +270,11 @@ void *lblx_stream(lblx_ctx *ctx);
 /* Stage timers: HIP events on the context stream around each stage
  * (linf4, shrink, convf4, lncor1, reject, scatter_rec, tile_r3, tile_r2,
  * tile_r1, merge, contnm, scan), the device analogue of the TAPE6 timing
- * table (oprop.f90:547-620).  lblx_timing(ctx, 1) clears and enables;
+ * table (oprop.f90:547-620), and "host": wall milliseconds the host spends
+ * planning layers and staging descriptors.  lblx_timing(ctx, 1) clears and enables;
  * lblx_timing_get returns the summed milliseconds and launch count of a
  * stage index (0 .. LBLX_NSTAGE-1).                                       */
-#define LBLX_NSTAGE 12
+#define LBLX_NSTAGE 13
 
 /* Diagnostics: the shrunk far-wing line list (TAPE9 content, LINF4 +
  * SHRINK, oprop.f90:2934-3640) of job `job` of the last OPDPTH call.      */

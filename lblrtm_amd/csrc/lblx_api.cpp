@@ -7,11 +7,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/lblx.h"
@@ -49,7 +52,10 @@ struct DevBuf {
   size_t bytes = 0;
   int ensure(size_t n) {
     if (n <= bytes) return 0;
-    if (p) hipFree(p);
+    if (p) {  // work queued on any stream may still use the old buffer
+      (void)hipDeviceSynchronize();
+      hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
     size_t want = std::max(n, (size_t)256);
@@ -75,10 +81,10 @@ struct DevBuf {
 // oprop.f90:547-620): HIP events recorded on the context stream around each
 // stage, summed when collected.
 enum Stage { ST_LINF4, ST_SHRINK, ST_CONVF4, ST_LNCOR1, ST_REJECT, ST_SCATTER, ST_TILE3, ST_TILE2,
-             ST_TILE1, ST_MERGE, ST_CONTNM, ST_SCAN, ST_N };
+             ST_TILE1, ST_MERGE, ST_CONTNM, ST_SCAN, ST_HOST, ST_N };
 static const char *kStageNames[ST_N] = {"linf4", "shrink", "convf4", "lncor1", "reject",
                                         "scatter_rec", "tile_r3", "tile_r2", "tile_r1",
-                                        "merge", "contnm", "scan"};
+                                        "merge", "contnm", "scan", "host"};
 
 struct lblx_ctx {
   int device = 0;
@@ -109,6 +115,16 @@ struct lblx_ctx {
     if (h < 0) return;
     (void)hipEventRecord(evs[h].second.second, stream);
   }
+  // host wall time (ST_HOST), summed only while timing is on
+  std::chrono::steady_clock::time_point h0;
+  void hbegin() {
+    if (timing) h0 = std::chrono::steady_clock::now();
+  }
+  void hend() {
+    if (!timing) return;
+    stage_ms[ST_HOST] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    stage_n[ST_HOST]++;
+  }
   void tcollect() {
     (void)hipStreamSynchronize(stream);
     for (auto &e : evs) {
@@ -125,6 +141,10 @@ struct lblx_ctx {
   // host copy of the line table (flat, file order)
   std::vector<double> h_vnu, h_bvmax;
   std::vector<int32_t> h_bstart, h_blk;
+  std::vector<float> h_sp;
+  std::vector<int32_t> h_mol;
+  std::vector<int32_t> h_molidx[7];  // lines of molecule m+1 with a valid isotope and S != 0
+  int64_t last_line_stride = 0;      // per-job line stride of the last OD run
   double max_pshift = 0.;
   int nlines = 0, nblk = 0;
   bool sorted = true;
@@ -323,6 +343,14 @@ int lblx_load_lines(lblx_ctx *c, const lblx_lines *L) {
     set_err("line table upload failed");
     return LBLX_E_HIP;
   }
+  for (int m = 0; m < 7; m++) c->h_molidx[m].clear();
+  for (int i = 0; i < n; i++) {
+    int m = mol[i] % 100, iso = (mol[i] % 1000) / 100;
+    if (m >= 1 && m <= 7 && iso >= 1 && iso <= LBLX_ISO_MAX[m - 1] && sp[i] != 0.f)
+      c->h_molidx[m - 1].push_back(i);
+  }
+  c->h_sp = std::move(sp);
+  c->h_mol = std::move(mol);
   c->h_vnu = std::move(vnu);
   c->h_bvmax = std::move(bvmax);
   c->h_bstart = std::move(bstart);
@@ -352,33 +380,118 @@ int lblx_od_size(const lblx_run *run, const lblx_layer *layer, int64_t *npts) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------
-// Run OPDPTH for a batch of jobs (layers).  Every stage is one launch for
-// all jobs.  od_base + od_off[j] receives job j's npts values.
-static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int njobs, bool chain_jrad,
-                    void *od_base, const int64_t *od_off, bool f32out, double *v1p, lblx_stats *st) {
-  if (c->nlines <= 0) return LBLX_E_NOLINES;
-  if (njobs <= 0) return LBLX_E_ARG;
-  // FP32 split: line shapes evaluated and accumulated in FP32, positions,
-  // grid offsets and the panel combine in FP64; OD written as float.
-  const bool split = run->precision == LBLX_FP32_SPLIT;
-  hipStream_t s = c->stream;
+// Line ranges of one layer in the flat table.
+//   far wing (LINF4): blocks [b4, b4 + nb4): from the first block with
+//     VMAX >= V1L4 to the block of the first line that passes LINF4 and
+//     lies above V2L4 even after the largest pressure shift, so the stop
+//     line (the first passing line above V2L4) is inside (oprop.f90:3228);
+//   near lines (RDLIN): records [nlo, nlo + nn).
+// Returns the number of records the job's per-line work arrays must hold.
+struct JobLines {
+  int b4, nb4, nlo, nn;
+};
+static int64_t job_lines(const lblx_ctx *c, const LayerPlan &P, const lblx_layer *ly, JobLines *J) {
   const int n = c->nlines, nblk = c->nblk;
-  std::vector<LayerPlan> plans(njobs);
-  std::vector<DevLayer> ys(njobs);
+  J->b4 = (int)(std::lower_bound(c->h_bvmax.begin(), c->h_bvmax.end(), P.v1l4) - c->h_bvmax.begin());
+  const double vlim = P.v2l4 + c->max_pshift * P.rhorat + 1e-6 * (1. + std::fabs(P.v2l4));
+  int i = (int)(std::upper_bound(c->h_vnu.begin(), c->h_vnu.end(), vlim) - c->h_vnu.begin());
+  int bend = nblk;
+  for (;;) {
+    int best = n, bm = -1;
+    for (int m = 0; m < 7 && m < ly->nmol; m++) {
+      if (ly->wk[m] == 0.) continue;
+      const std::vector<int32_t> &ix = c->h_molidx[m];
+      auto it = std::lower_bound(ix.begin(), ix.end(), i);
+      if (it != ix.end() && *it < best) {
+        best = *it;
+        bm = m;
+      }
+    }
+    if (best >= n) break;
+    if ((double)c->h_sp[best] * ly->wk[bm] != 0.) {
+      bend = c->h_blk[best] + 1;
+      break;
+    }
+    i = best + 1;
+  }
+  J->nb4 = std::max(0, bend - J->b4);
+  // near range (RDLIN): first record >= VBOT in the first block read, up
+  // to and including the first record above VTOP
+  int bn = (int)(std::lower_bound(c->h_bvmax.begin(), c->h_bvmax.end(), P.vbot) - c->h_bvmax.begin());
+  int nlo = n, nhi = n - 1;
+  if (bn < nblk) {
+    int s0 = c->h_bstart[bn], e0 = c->h_bstart[bn + 1];
+    nlo = (int)(std::lower_bound(c->h_vnu.begin() + s0, c->h_vnu.begin() + e0, P.vbot) - c->h_vnu.begin());
+    if (nlo >= e0) nlo = e0 - 1;
+    int up = (int)(std::upper_bound(c->h_vnu.begin() + nlo, c->h_vnu.end(), P.vtop) - c->h_vnu.begin());
+    nhi = up < n ? up : n - 1;
+  }
+  J->nlo = nlo;
+  J->nn = nhi >= nlo ? nhi - nlo + 1 : 0;
+  const int64_t n4 = J->nb4 > 0 ? c->h_bstart[J->b4 + J->nb4] - c->h_bstart[J->b4] : 0;
+  return std::max<int64_t>(n4, J->nn);
+}
+
+// Host side of an OPDPTH batch: the jobs' descriptors and plan tables, tile
+// maps and the strides of the per-job work arrays.
+struct JobsPrep {
+  int njobs = 0;
+  std::vector<LayerPlan> own;
+  std::vector<const LayerPlan *> plans;
+  std::vector<DevLayer> ys;
   std::vector<DevPanel> pan;
   std::vector<DevWin> win;
   std::vector<DevSeg> seg;
-  std::vector<double> margins(3 * njobs);
+  std::vector<double> margins;
+  std::vector<ContPlan> cps;
+  std::vector<int32_t> tjob[3], tloc[3];
+  int tiles[3] = {0, 0, 0};
+  int64_t r2tot = 0, r3tot = 0, maxline = 0;
+  int maxnn = 0, maxnbk = 0, abs_stride = 0, maxnb4 = 0;
+  Strides S{};
+};
+
+// Plan a batch of jobs (layers) on the host.  Reads only the context's line
+// table and settings, so it may run beside launches of another batch.
+// Layer plans are computed here, or handed in (pre, with the layers' JRAD
+// already chained).
+static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int njobs, bool chain_jrad,
+                     const int64_t *od_off, const LayerPlan *pre, JobsPrep &J) {
+  if (c->nlines <= 0) return LBLX_E_NOLINES;
+  if (njobs <= 0) return LBLX_E_ARG;
+  const int n = c->nlines, nblk = c->nblk;
+  J.njobs = njobs;
+  J.own.assign(pre ? 0 : njobs, LayerPlan{});
+  J.plans.assign(njobs, nullptr);
+  J.ys.assign(njobs, DevLayer{});
+  J.pan.clear();
+  J.win.clear();
+  J.seg.clear();
+  J.margins.assign(3 * njobs, 0.);
+  J.cps.assign(njobs, ContPlan{});
+  auto &own = J.own;
+  auto &plans = J.plans;
+  auto &ys = J.ys;
+  auto &pan = J.pan;
+  auto &win = J.win;
+  auto &seg = J.seg;
+  auto &margins = J.margins;
+  auto &cps = J.cps;
   int64_t r2tot = 0, r3tot = 0;
-  int maxnn = 0, maxnbk = 0, abs_stride = 0;
-  std::vector<ContPlan> cps(njobs);
+  int maxnn = 0, maxnbk = 0, abs_stride = 0, maxnb4 = 0;
+  int64_t maxline = 0;
   int prev_jrad = layers[0].jrad;
   for (int jb = 0; jb < njobs; jb++) {
     lblx_layer *ly = &layers[jb];
     if (chain_jrad && jb > 0) ly->jrad = prev_jrad;
-    LayerPlan &P = plans[jb];
-    int rc = plan_layer(run, ly, &P);
-    if (rc) return rc;
+    if (pre) {
+      plans[jb] = &pre[jb];
+    } else {
+      int rc = plan_layer(run, ly, &own[jb]);
+      if (rc) return rc;
+      plans[jb] = &own[jb];
+    }
+    const LayerPlan &P = *plans[jb];
     prev_jrad = ly->jrad;
     DevLayer &Y = ys[jb];
     Y = DevLayer{};
@@ -434,21 +547,14 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
     for (const WinPlan &w : P.wins)
       win.push_back(DevWin{w.v1r4, w.v2r4, w.nptr4, Y.seg0 + w.seg0, w.nseg, jb});
     for (const RadSeg &g : P.segs) seg.push_back(DevSeg{g.i1, g.i2, g.radvi, g.rdel});
-    // far-wing lines: first block with VMAX >= V1L4 (RDLNFL skip)
-    Y.b4 = (int)(std::lower_bound(c->h_bvmax.begin(), c->h_bvmax.end(), P.v1l4) - c->h_bvmax.begin());
-    // near range (RDLIN): first record >= VBOT in the first block read, up
-    // to and including the first record above VTOP
-    int bn = (int)(std::lower_bound(c->h_bvmax.begin(), c->h_bvmax.end(), P.vbot) - c->h_bvmax.begin());
-    int nlo = n, nhi = n - 1;
-    if (bn < nblk) {
-      int s0 = c->h_bstart[bn], e0 = c->h_bstart[bn + 1];
-      nlo = (int)(std::lower_bound(c->h_vnu.begin() + s0, c->h_vnu.begin() + e0, P.vbot) - c->h_vnu.begin());
-      if (nlo >= e0) nlo = e0 - 1;
-      int up = (int)(std::upper_bound(c->h_vnu.begin() + nlo, c->h_vnu.end(), P.vtop) - c->h_vnu.begin());
-      nhi = up < n ? up : n - 1;
-    }
-    Y.nlo = nlo;
-    Y.nn = nhi >= nlo ? nhi - nlo + 1 : 0;
+    JobLines JL;
+    maxline = std::max(maxline, job_lines(c, P, ly, &JL));
+    Y.b4 = JL.b4;
+    Y.nb4 = JL.nb4;
+    maxnb4 = std::max(maxnb4, JL.nb4);
+    const int nlo = JL.nlo, nhi = JL.nlo + JL.nn - 1;
+    Y.nlo = JL.nlo;
+    Y.nn = JL.nn;
     Y.blk0 = Y.nn > 0 ? c->h_blk[nlo] : 0;
     Y.nbk = Y.nn > 0 ? c->h_blk[nhi] - Y.blk0 + 1 : 0;
     Y.od0 = od_off ? od_off[jb] : 0;
@@ -469,9 +575,15 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   for (int jb = 0; jb < njobs; jb++) ys[jb].abs0 = (int64_t)jb * abs_stride;
   // tile maps
   const int tiles[3] = {c->tiles[0], c->tiles[1], c->tiles[2]};
-  std::vector<int32_t> tjob[3], tloc[3];
+  auto &tjob = J.tjob;
+  auto &tloc = J.tloc;
+  for (int G = 0; G < 3; G++) {
+    J.tiles[G] = tiles[G];
+    tjob[G].clear();
+    tloc[G].clear();
+  }
   for (int jb = 0; jb < njobs; jb++) {
-    const LayerPlan &P = plans[jb];
+    const LayerPlan &P = *plans[jb];
     const int64_t ng[3] = {P.g_last + 1, P.n2, P.n3};
     for (int G = 0; G < 3; G++) {
       int nt = (int)((ng[G] + tiles[G] - 1) / tiles[G]);
@@ -481,10 +593,45 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
       }
     }
   }
-  Strides S;
-  S.line = ((int64_t)n + 64 + 63) / 64 * 64;
+  Strides &S = J.S;
+  S.line = (maxline + 64 + 63) / 64 * 64;
   S.blk = nblk + 2;
   S.t9blk = n / 1000 + 16;
+  J.r2tot = r2tot;
+  J.r3tot = r3tot;
+  J.maxline = maxline;
+  J.maxnn = maxnn;
+  J.maxnbk = maxnbk;
+  J.abs_stride = abs_stride;
+  J.maxnb4 = maxnb4;
+  return LBLX_OK;
+}
+
+// Allocate, upload and launch a planned batch.  od_base + od_off[j]
+// receives job j's npts values.
+static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void *od_base, bool f32out,
+                       double *v1p, lblx_stats *st) {
+  // FP32 split: line shapes evaluated and accumulated in FP32, positions,
+  // grid offsets and the panel combine in FP64; OD written as float.
+  const bool split = run->precision == LBLX_FP32_SPLIT;
+  hipStream_t s = c->stream;
+  const int njobs = J.njobs;
+  const int nblk = c->nblk;
+  const auto &plans = J.plans;
+  const auto &ys = J.ys;
+  const auto &pan = J.pan;
+  const auto &win = J.win;
+  const auto &seg = J.seg;
+  const auto &margins = J.margins;
+  const auto &cps = J.cps;
+  const auto &tjob = J.tjob;
+  const auto &tloc = J.tloc;
+  const int *tiles = J.tiles;
+  const Strides S = J.S;
+  const int64_t r2tot = J.r2tot, r3tot = J.r3tot, maxline = J.maxline;
+  const int maxnn = J.maxnn, maxnbk = J.maxnbk, abs_stride = J.abs_stride, maxnb4 = J.maxnb4;
+  const int nwin = (int)win.size();
+  c->last_line_stride = S.line;
   const int64_t NJ = njobs;
   auto need = [&](DevBuf &b, size_t bytes) { return b.ensure(bytes) != 0; };
   if (need(c->w_dlay, NJ * sizeof(DevLayer)) || need(c->w_pan, pan.size() * sizeof(DevPanel)) ||
@@ -585,15 +732,16 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
 
   // 1-2. LINF4 + SHRINK
   int th = c->tbegin(ST_SHRINK);
-  const int lgrid = (int)((S.line + 255) / 256);
-  hipLaunchKernelGGL(k_shr_count, dim3(nblk, njobs), dim3(256), 0, s, L, dY, S, W);
-  hipLaunchKernelGGL(k_shr_scan, dim3(njobs), dim3(256), 0, s, nblk, S, W);
-  hipLaunchKernelGGL(k_shr_compact, dim3(nblk, njobs), dim3(256), 0, s, L, dY, S, W);
+  const int lgrid = (int)((maxline + 1 + 255) / 256);
+  const int bgrid = std::max(maxnb4, 1);
+  hipLaunchKernelGGL(k_shr_count, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
+  hipLaunchKernelGGL(k_shr_scan, dim3(njobs), dim3(256), 0, s, dY, S, W);
+  hipLaunchKernelGGL(k_shr_compact, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
   hipLaunchKernelGGL(k_shr_chain, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W);
   for (int lev = 1; lev < NLEV; lev++)
     hipLaunchKernelGGL(k_shr_lift, dim3(lgrid, njobs), dim3(256), 0, s, lev, S, W);
-  hipLaunchKernelGGL(k_shr_etab, dim3((nblk * EBLK + 255) / 256, njobs), dim3(256), 0, s, nblk, S, W, njobs);
-  hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), 0, s, dY, nblk, S, W, T9, njobs);
+  hipLaunchKernelGGL(k_shr_etab, dim3((bgrid * EBLK + 255) / 256, njobs), dim3(256), 0, s, dY, S, W, njobs);
+  hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), 0, s, dY, S, W, T9, njobs);
   hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9, njobs);
   hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);
   c->tend(th);
@@ -694,7 +842,7 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   }
   HIPCHK(hipGetLastError());
   if (v1p)
-    for (int jb = 0; jb < njobs; jb++) v1p[jb] = plans[jb].panels[0].vft + (double)NSHIFT * plans[jb].dv;
+    for (int jb = 0; jb < njobs; jb++) v1p[jb] = plans[jb]->panels[0].vft + (double)NSHIFT * plans[jb]->dv;
   if (st) {
     std::vector<unsigned long long> h(3 * NJ);
     std::vector<int32_t> t9n(NJ);
@@ -707,12 +855,24 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
       st[jb].k3_updates = (int64_t)h[NJ + jb];
       st[jb].lincnt = (int64_t)h[2 * NJ + jb];
       st[jb].nlins4 = t9n[jb];
-      st[jb].npanel = (int64_t)plans[jb].panels.size();
-      st[jb].npts = plans[jb].npts;
-      st[jb].nwin4 = (int64_t)plans[jb].wins.size();
+      st[jb].npanel = (int64_t)plans[jb]->panels.size();
+      st[jb].npts = plans[jb]->npts;
+      st[jb].nwin4 = (int64_t)plans[jb]->wins.size();
     }
   }
   return LBLX_OK;
+}
+
+// OPDPTH for a batch of jobs (layers): every stage one launch for all jobs.
+static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int njobs, bool chain_jrad,
+                    void *od_base, const int64_t *od_off, bool f32out, double *v1p, lblx_stats *st,
+                    const LayerPlan *pre = nullptr) {
+  c->hbegin();
+  JobsPrep J;
+  int rc = prep_jobs(c, run, layers, njobs, chain_jrad, od_off, pre, J);
+  c->hend();
+  if (rc) return rc;
+  return launch_jobs(c, run, J, od_base, f32out, v1p, st);
 }
 
 extern "C" {
@@ -763,7 +923,7 @@ int lblx_debug_tape9(lblx_ctx *c, int job, double *vnu, double *sp, double *alfa
   if (!c || !n_out || job < 0) return LBLX_E_ARG;
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
-  int64_t line = ((int64_t)c->nlines + 64 + 63) / 64 * 64;
+  const int64_t line = c->last_line_stride;
   int32_t n = 0;
   HIPCHK(hipMemcpy(&n, c->w_t9n.as<int32_t>() + job, 4, hipMemcpyDeviceToHost));
   *n_out = n;
@@ -1275,8 +1435,10 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
 }
 
 // Per-job device bytes of run_jobs (line-table-sized work arrays).
-static size_t job_bytes(const lblx_ctx *c) {
-  const size_t line = ((size_t)c->nlines + 127) / 64 * 64;
+// Device bytes one job takes in run_jobs at a per-job line stride of
+// about nline records.
+static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
+  const size_t line = ((size_t)nline + 127) / 64 * 64;
   return line * (32 + 1 + 12 + 4 * NLEV + 2 * NLEV + 12 + 36 + sizeof(NearLine) + 8 + sizeof(ScatterRec)) +
          (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 2) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
@@ -1294,6 +1456,7 @@ int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const
   if (c->nlines <= 0) return LBLX_E_NOLINES;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
+  c->hbegin();
   std::vector<int> first(nprof + 1, 0);
   for (int p = 0; p < nprof; p++) {
     if (nlay[p] <= 0) return LBLX_E_ARG;
@@ -1301,29 +1464,57 @@ int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const
   }
   const int ntot = first[nprof];
   std::vector<LayerPlan> plans(ntot);
+  std::vector<int64_t> jlines(ntot, 0);
   std::vector<int> itype(ntot, 0), pan0(ntot, 0);
-  std::vector<DevPanel> pan;
-  for (int p = 0; p < nprof; p++) {
+  std::vector<std::vector<DevPanel>> ppan(nprof);
+  std::vector<int> prc(nprof, 0);
+  // host planning of one profile: layer plans with JRAD chained up the
+  // column, line ranges, and the merge panels of each layer
+  auto plan_prof = [&](int p) -> int {
     int jr = layers[first[p]].jrad;
     for (int l = first[p]; l < first[p + 1]; l++) {
       layers[l].jrad = jr;
       int rc = plan_layer(run, &layers[l], &plans[l]);
       if (rc) return rc;
+      JobLines JL;
+      jlines[l] = job_lines(c, plans[l], &layers[l], &JL);
       jr = layers[l].jrad;
-      pan0[l] = (int)pan.size();
+      pan0[l] = (int)ppan[p].size();
       std::vector<int64_t> base;
       if (l > first[p]) {
         rc = merge_itype(plans[l - 1].dv, plans[l].dv, &itype[l]);
         if (rc) return rc;
         if ((rc = merge_bases(plans[l - 1], plans[l], itype[l], plans[l].v2, base))) return rc;
       }
-      merge_panels(plans[l], l > first[p] ? &base : nullptr, pan);
+      merge_panels(plans[l], l > first[p] ? &base : nullptr, ppan[p]);
     }
     if (v1p_top) {
       const LayerPlan &T = plans[first[p + 1] - 1];
       v1p_top[p] = T.panels[0].vft + (double)NSHIFT * T.dv;
     }
+    return 0;
+  };
+  {  // profiles are independent: plan them on host threads
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = (int)std::min<unsigned>((unsigned)nprof, std::min(16u, hw));
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+      for (int p = next++; p < nprof; p = next++) prc[p] = plan_prof(p);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; t++) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
   }
+  for (int p = 0; p < nprof; p++)
+    if (prc[p]) return prc[p];
+  std::vector<DevPanel> pan;
+  for (int p = 0; p < nprof; p++) {
+    const int off = (int)pan.size();
+    for (int l = first[p]; l < first[p + 1]; l++) pan0[l] += off;
+    pan.insert(pan.end(), ppan[p].begin(), ppan[p].end());
+  }
+  c->hend();
   if (c->w_mpan.ensure(pan.size() * sizeof(DevPanel))) return LBLX_E_HIP;
   HIPCHK(hipMemcpy(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice));
   const DevPanel *dpan = c->w_mpan.as<DevPanel>();
@@ -1331,98 +1522,177 @@ int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const
   size_t free_b = 0, total_b = 0;
   HIPCHK(hipMemGetInfo(&free_b, &total_b));
   const size_t budget = std::min<size_t>(free_b / 2, (size_t)96 << 30);
-  const size_t jb = std::max<size_t>(job_bytes(c), 1);
-  const int max_jobs = (int)std::max<size_t>(1, budget / jb);
   const bool f32 = run->precision == LBLX_FP32_SPLIT;
   const size_t esz = f32 ? 4 : 8;
-  int p0 = 0, gidx = 0;
-  while (p0 < nprof) {
+  struct Group {
+    int p0, p1, njobs, maxl;
+    std::vector<int64_t> off, nmax, rt_off;
+    int64_t tot, rt_tot;
+  };
+  std::vector<Group> groups;
+  int64_t max_tot = 0, max_rt = 0;
+  int max_jobs = 0;
+  for (int p0 = 0; p0 < nprof;) {
+    // whole profiles while njobs x (bytes per job at the group's widest
+    // line range) fits the budget
+    Group G{};
     int p1 = p0, njobs = 0;
-    while (p1 < nprof && (njobs == 0 || njobs + nlay[p1] <= max_jobs)) njobs += nlay[p1++];
-    // 1. optical depths of every layer of profiles p0..p1-1
-    std::vector<int64_t> off(njobs);
-    int64_t tot = 0;
-    std::vector<int64_t> nmax(p1 - p0, 0), rt_off(p1 - p0, 0);
-    int64_t rt_tot = 0;
-    int maxl = 0;
+    int64_t wl = 0;
+    while (p1 < nprof) {
+      int64_t w = wl;
+      for (int l = first[p1]; l < first[p1 + 1]; l++) w = std::max(w, jlines[l]);
+      if (njobs > 0 && (size_t)(njobs + nlay[p1]) * job_bytes(c, w) > budget) break;
+      wl = w;
+      njobs += nlay[p1++];
+    }
+    G.p0 = p0;
+    G.p1 = p1;
+    G.njobs = njobs;
+    G.off.resize(njobs);
+    G.nmax.assign(p1 - p0, 0);
+    G.rt_off.assign(p1 - p0, 0);
+    G.tot = G.rt_tot = 0;
+    G.maxl = 0;
     for (int p = p0; p < p1; p++) {
       for (int l = first[p]; l < first[p + 1]; l++) {
-        off[l - first[p0]] = tot;
-        tot += plans[l].npts;
-        nmax[p - p0] = std::max(nmax[p - p0], plans[l].npts);
+        G.off[l - first[p0]] = G.tot;
+        G.tot += plans[l].npts;
+        G.nmax[p - p0] = std::max(G.nmax[p - p0], plans[l].npts);
       }
-      rt_off[p - p0] = rt_tot;
-      rt_tot += 4 * nmax[p - p0];
-      maxl = std::max(maxl, nlay[p]);
+      G.rt_off[p - p0] = G.rt_tot;
+      G.rt_tot += 4 * G.nmax[p - p0];
+      G.maxl = std::max(G.maxl, nlay[p]);
     }
-    if (c->w_od.ensure((size_t)tot * esz) || c->b_rt.ensure((size_t)rt_tot * 8)) return LBLX_E_HIP;
-    int rc = run_jobs(c, run, layers + first[p0], njobs, false, c->w_od.p, off.data(), f32, nullptr, nullptr);
-    if (rc) return rc;
-    // 2. merge descriptors for every layer index, uploaded once
+    max_tot = std::max(max_tot, G.tot);
+    max_rt = std::max(max_rt, G.rt_tot);
+    max_jobs = std::max(max_jobs, njobs);
+    groups.push_back(std::move(G));
+    p0 = p1;
+  }
+  // the buffers every group's descriptors point into, sized once
+  if (c->w_od.ensure((size_t)max_tot * esz) || c->b_rt.ensure((size_t)max_rt * 8) ||
+      c->b_jobs.ensure((size_t)max_jobs * sizeof(MergeJob)))
+    return LBLX_E_HIP;
+  // Host plan of one group: its OPDPTH jobs and the merge descriptors of
+  // every layer index (kstart/kcount: the launch of layer index k).
+  struct GroupPlan {
+    JobsPrep J;
     std::vector<MergeJob> mj;
-    std::vector<int> kstart(maxl + 1, 0), kcount(maxl, 0);
-    std::vector<int64_t> key = {(int64_t)gidx, (int64_t)f32, (int64_t)maxl};
-    for (int k = 0; k < maxl; k++) {
-      kstart[k] = (int)mj.size();
+    std::vector<int> kstart, kcount;
+    std::vector<int64_t> key;
+    int rc = 0;
+  };
+  auto plan_group = [&](int g, GroupPlan &P) {
+    const Group &G = groups[g];
+    P.rc = prep_jobs(c, run, layers + first[G.p0], G.njobs, false, G.off.data(), plans.data() + first[G.p0],
+                     P.J);
+    if (P.rc) return;
+    P.mj.clear();
+    P.kstart.assign(G.maxl + 1, 0);
+    P.kcount.assign(G.maxl, 0);
+    P.key = {(int64_t)g, (int64_t)f32, (int64_t)G.maxl, (int64_t)(intptr_t)c->b_jobs.p};
+    for (int k = 0; k < G.maxl; k++) {
+      P.kstart[k] = (int)P.mj.size();
       int64_t nk = 0;
-      for (int p = p0; p < p1; p++) {
+      for (int p = G.p0; p < G.p1; p++) {
         if (k >= nlay[p]) continue;
         const int l = first[p] + k;
-        double *rt = c->b_rt.as<double>() + rt_off[p - p0];
-        const int64_t m = nmax[p - p0];
+        double *rt = c->b_rt.as<double>() + G.rt_off[p - G.p0];
+        const int64_t m = G.nmax[p - G.p0];
         double *R[2] = {rt, rt + m}, *T[2] = {rt + 2 * m, rt + 3 * m};
         const bool last = k == nlay[p] - 1;
-        MergeJob J;
-        J.A = merge_args(plans[l], k > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+        MergeJob M;
+        M.A = merge_args(plans[l], k > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
                          k == 0 ? tbnd[p] : 0., emis[p]);
-        J.pan = dpan + pan0[l];
-        J.od = (const char *)c->w_od.p + (size_t)off[l - first[p0]] * esz;
-        J.rold = k > 0 ? R[(k - 1) & 1] : nullptr;
-        J.told = k > 0 ? T[(k - 1) & 1] : nullptr;
-        J.rnew = last ? d_rad + out_off[p] : R[k & 1];
-        J.tnew = last ? d_tra + out_off[p] : T[k & 1];
-        mj.push_back(J);
+        M.pan = dpan + pan0[l];
+        M.od = (const char *)c->w_od.p + (size_t)G.off[l - first[G.p0]] * esz;
+        M.rold = k > 0 ? R[(k - 1) & 1] : nullptr;
+        M.told = k > 0 ? T[(k - 1) & 1] : nullptr;
+        M.rnew = last ? d_rad + out_off[p] : R[k & 1];
+        M.tnew = last ? d_tra + out_off[p] : T[k & 1];
+        P.mj.push_back(M);
         nk = std::max(nk, plans[l].npts);
       }
-      kcount[k] = (int)mj.size() - kstart[k];
-      key.push_back(kcount[k]);
-      key.push_back(nk);
+      P.kcount[k] = (int)P.mj.size() - P.kstart[k];
+      P.key.push_back(P.kcount[k]);
+      P.key.push_back(nk);
     }
-    if (c->b_jobs.ensure(mj.size() * sizeof(MergeJob))) return LBLX_E_HIP;
-    key.push_back((int64_t)(intptr_t)c->b_jobs.p);  // the graph holds this address
-    HIPCHK(hipMemcpyAsync(c->b_jobs.p, mj.data(), mj.size() * sizeof(MergeJob), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));  // descriptors in place before the graph runs
-    // 3. the chain of merge launches as a graph (instantiated once per shape)
-    hipGraphExec_t exec = nullptr;
-    for (auto &g : c->graphs)
-      if (g.key == key) exec = g.exec;
-    if (!exec) {
-      hipGraph_t graph = nullptr;
-      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const MergeJob *dj = c->b_jobs.as<MergeJob>();
-      for (int k = 0; k < maxl; k++) {
-        const int64_t nk = key[3 + 2 * k + 1];
-        dim3 grid((unsigned)((nk + 255) / 256), (unsigned)kcount[k]);
-        if (f32) hipLaunchKernelGGL(k_merge_batch<float>, grid, dim3(256), 0, s, dj + kstart[k]);
-        else hipLaunchKernelGGL(k_merge_batch<double>, grid, dim3(256), 0, s, dj + kstart[k]);
-      }
-      HIPCHK(hipStreamEndCapture(s, &graph));
-      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
-      if (c->graphs.size() >= 64) {  // bounded cache
-        (void)hipGraphExecDestroy(c->graphs.front().exec);
-        c->graphs.erase(c->graphs.begin());
-      }
-      c->graphs.push_back(lblx_ctx::GraphEntry{key, exec});
+  };
+  // Group g + 1 is planned on a host thread while group g runs.  A plan
+  // slot is rewritten only after the uploads made from it have completed
+  // (event after them).
+  GroupPlan gp[2];
+  hipEvent_t up_done[2] = {c->ev_get(), c->ev_get()};
+  auto put_events = [&]() {
+    c->evpool.push_back(up_done[0]);
+    c->evpool.push_back(up_done[1]);
+  };
+  c->hbegin();
+  plan_group(0, gp[0]);
+  c->hend();
+  const int ng = (int)groups.size();
+  int rc = LBLX_OK;
+  for (int g = 0; g < ng && rc == LBLX_OK; g++) {
+    GroupPlan &P = gp[g & 1];
+    std::thread helper;
+    if (g + 1 < ng) {
+      GroupPlan &Q = gp[(g + 1) & 1];
+      hipEvent_t ev = up_done[(g + 1) & 1];
+      const bool used = g >= 1;
+      helper = std::thread([&, g, ev, used]() {
+        if (used) (void)hipEventSynchronize(ev);
+        plan_group(g + 1, Q);
+      });
     }
-    int th = c->tbegin(ST_MERGE);
-    HIPCHK(hipGraphLaunch(exec, s));
-    c->tend(th);
-    // the next group reuses the descriptor buffer
-    HIPCHK(hipStreamSynchronize(s));
-    p0 = p1;
-    gidx++;
+    auto body = [&]() -> int {
+      if (P.rc) return P.rc;
+      // 1. optical depths of every layer of the group's profiles
+      int r = launch_jobs(c, run, P.J, c->w_od.p, f32, nullptr, nullptr);
+      if (r) return r;
+      // 2. merge descriptors, then the chain of merge launches as a graph
+      // (instantiated once per shape)
+      HIPCHK(hipMemcpyAsync(c->b_jobs.p, P.mj.data(), P.mj.size() * sizeof(MergeJob), hipMemcpyHostToDevice,
+                            s));
+      HIPCHK(hipEventRecord(up_done[g & 1], s));
+      hipGraphExec_t exec = nullptr;
+      for (auto &e : c->graphs)
+        if (e.key == P.key) exec = e.exec;
+      if (!exec) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const MergeJob *dj = c->b_jobs.as<MergeJob>();
+        const int maxl = groups[g].maxl;
+        for (int k = 0; k < maxl; k++) {
+          const int64_t nk = P.key[4 + 2 * k + 1];
+          dim3 grid((unsigned)((nk + 255) / 256), (unsigned)P.kcount[k]);
+          if (f32) hipLaunchKernelGGL(k_merge_batch<float>, grid, dim3(256), 0, s, dj + P.kstart[k]);
+          else hipLaunchKernelGGL(k_merge_batch<double>, grid, dim3(256), 0, s, dj + P.kstart[k]);
+        }
+        HIPCHK(hipStreamEndCapture(s, &graph));
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ie);
+        if (c->graphs.size() >= 64) {  // bounded cache
+          (void)hipGraphExecDestroy(c->graphs.front().exec);
+          c->graphs.erase(c->graphs.begin());
+        }
+        c->graphs.push_back(lblx_ctx::GraphEntry{P.key, exec});
+      }
+      int th = c->tbegin(ST_MERGE);
+      HIPCHK(hipGraphLaunch(exec, s));
+      c->tend(th);
+      return LBLX_OK;
+    };
+    rc = body();
+    c->hbegin();  // host time the device waits for the next group's plan
+    if (helper.joinable()) helper.join();
+    c->hend();
   }
+  // uploads and merges done before the host plans are released
+  hipError_t se = hipStreamSynchronize(s);
+  put_events();
+  if (rc) return rc;
+  HIPCHK(se);
   HIPCHK(hipGetLastError());
   return LBLX_OK;
 }
@@ -1463,7 +1733,7 @@ int lblx_autotune(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay
     }
     return best;
   };
-  const int cand[3][4] = {{1024, 2048, 4096, 8192}, {512, 1024, 2048, 4096}, {256, 512, 1024, 2048}};
+  const int cand[3][5] = {{512, 1024, 2048, 4096, 8192}, {256, 512, 1024, 2048, 4096}, {128, 256, 512, 1024, 2048}};
   const int stage[3] = {ST_TILE1, ST_TILE2, ST_TILE3};
   for (int G = 0; G < 3; G++) {
     int bestw = c->tiles[G];

@@ -37,8 +37,10 @@ struct DevLayer {
   int64_t abs0;           // offset of this job's ABSRB row (1-based entries)
   // offsets into concatenated plan tables
   int32_t pan0, win0, seg0;
-  // far-wing line range: first TAPE3 block with VMAX >= V1L4
-  int32_t b4;
+  // far-wing line range: first TAPE3 block with VMAX >= V1L4, and the
+  // number of blocks from there that can hold the stop line (the first
+  // passing line above V2L4; LINF4 reads no block after it)
+  int32_t b4, nb4;
   // near-line range (RDLIN): records [nlo, nlo+nn), blocks [blk0, blk0+nbk)
   int32_t nlo, nn, blk0, nbk;
   // outputs and grids

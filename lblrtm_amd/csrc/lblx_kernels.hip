@@ -138,11 +138,12 @@ __global__ __launch_bounds__(SB_T) void k_shr_count(DevLines L, const DevLayer *
   __shared__ int s_stop;
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
-  const int b = blockIdx.x;
-  if (b >= L.nblk) return;
+  const int bl = blockIdx.x;  // block index relative to Y.b4
+  if (bl >= Y.nb4) return;
+  const int b = Y.b4 + bl;
   int i = L.bstart[b] + threadIdx.x;
   int pass = 0, pn = 0, pc = 0, stp = 0;
-  if (b >= Y.b4 && i < L.bstart[b + 1]) {
+  if (i < L.bstart[b + 1]) {
     F4Line o;
     if (linf4_line(L, Y, i, o)) {
       pass = 1;
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(SB_T) void k_shr_count(DevLines L, const DevLayer *
   if (stp) atomicMin(&s_stop, pos);
   __syncthreads();
   if (threadIdx.x == 0) {
-    int64_t o = job * S.blk + b;
+    int64_t o = job * S.blk + bl;
     W.bcnt[o] = tot;
     W.bcntn[o] = tn;
     W.bcntc[o] = tc;
@@ -167,13 +168,15 @@ __global__ __launch_bounds__(SB_T) void k_shr_count(DevLines L, const DevLayer *
   }
 }
 
-// per job: exclusive prefix over blocks of the three counts; passing
-// index of the stop line (or -1)
-__global__ __launch_bounds__(256) void k_shr_scan(int nblk, Strides S, ShrinkWork W) {
+// per job: exclusive prefix over its blocks of the three counts; passing
+// index of the stop line (or -1); the closing entries of the positive-line
+// prefixes
+__global__ __launch_bounds__(256) void k_shr_scan(const DevLayer *Ys, Strides S, ShrinkWork W) {
   __shared__ int buf[256];
   __shared__ int carry[3];
   __shared__ int s_stop;
   const int job = blockIdx.x;
+  const int nblk = Ys[job].nb4;
   const int64_t o = job * S.blk;
   if (threadIdx.x < 3) carry[threadIdx.x] = 0;
   if (threadIdx.x == 0) s_stop = INT_MAX;
@@ -203,6 +206,8 @@ __global__ __launch_bounds__(256) void k_shr_scan(int nblk, Strides S, ShrinkWor
     W.bpc[o + nblk] = carry[2];
     W.npass[job] = carry[0];
     W.stop[job] = s_stop == INT_MAX ? -1 : s_stop;
+    W.pn[job * S.line + carry[0]] = carry[1];
+    W.pc[job * S.line + carry[0]] = carry[2];
   }
 }
 
@@ -213,12 +218,13 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
   __shared__ int buf[SB_T];
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
-  const int b = blockIdx.x;
-  if (b >= L.nblk) return;
+  const int bl = blockIdx.x;
+  if (bl >= Y.nb4) return;
+  const int b = Y.b4 + bl;
   int i = L.bstart[b] + threadIdx.x;
   F4Line o{};
   int pass = 0, pn = 0, pc = 0;
-  if (b >= Y.b4 && i < L.bstart[b + 1] && linf4_line(L, Y, i, o)) {
+  if (i < L.bstart[b + 1] && linf4_line(L, Y, i, o)) {
     pass = 1;
     pn = (o.sp > 0. && !o.co2);
     pc = (o.sp > 0. && o.co2);
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
   int pos = block_scan_excl<SB_T>(pass, buf, &t0);
   int posn = block_scan_excl<SB_T>(pn, buf, &t1);
   int posc = block_scan_excl<SB_T>(pc, buf, &t2);
-  const int64_t ob = job * S.blk + b;
+  const int64_t ob = job * S.blk + bl;
   if (pass) {
     int64_t k = job * S.line + W.bp[ob] + pos;
     W.cx[k] = o.x;
@@ -237,11 +243,6 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
     W.cco2[k] = (int8_t)o.co2;
     W.pn[k] = W.bpn[ob] + posn;
     W.pc[k] = W.bpc[ob] + posc;
-  }
-  if (b == L.nblk - 1 && threadIdx.x == 0) {
-    int64_t k = job * S.line + W.bp[job * S.blk + L.nblk];
-    W.pn[k] = W.bpn[job * S.blk + L.nblk];
-    W.pc[k] = W.bpc[job * S.blk + L.nblk];
   }
 }
 
@@ -319,9 +320,11 @@ __device__ int shr_query(const ShrinkWork &W, int64_t o, int64_t lstride, int s,
 }
 
 // per (job, block b, k): lines emitted by a batch of blocks b..b+k
-__global__ void k_shr_etab(int nblk, Strides S, ShrinkWork W, int njobs) {
+// (block indices relative to the job's first far-wing block)
+__global__ void k_shr_etab(const DevLayer *Ys, Strides S, ShrinkWork W, int njobs) {
   const int job = blockIdx.y;
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nblk = Ys[job].nb4;
   if (id >= nblk * EBLK) return;
   const int b = id / EBLK, k = id % EBLK;
   const int64_t ob = job * S.blk;
@@ -334,16 +337,18 @@ __global__ void k_shr_etab(int nblk, Strides S, ShrinkWork W, int njobs) {
 }
 
 // one thread per job: LINF4's read loop over blocks (oprop.f90:3130-3290)
-// deciding batch boundaries and TAPE9 blocks from the emitted counts
-__global__ void k_shr_batches(const DevLayer *Ys, int nblk, Strides S, ShrinkWork W, Tape9 T9,
-                              int njobs) {
+// deciding batch boundaries and TAPE9 blocks from the emitted counts.
+// Blocks run from the job's first far-wing block; its last one holds the
+// stop line unless the file ends first.
+__global__ void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9, int njobs) {
   const int job = blockIdx.x;
   if (threadIdx.x != 0) return;
   const int64_t ob = job * S.blk;
   const int64_t o = job * S.line;
   const int64_t ot = job * S.t9blk;
   const int stop_p = W.stop[job];
-  int b = Ys[job].b4;
+  const int nblk = Ys[job].nb4;
+  int b = 0;
   int J = 0, out = 0, nbat = 0, nt9 = 0, t9s = 0;
   for (;;) {
     int bs = b;

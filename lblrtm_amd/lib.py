@@ -16,7 +16,7 @@ from . import abi
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "liblblx.so")
 _LIB = None
-NSTAGE = 12
+NSTAGE = 13
 
 
 class LblxError(RuntimeError):
