@@ -87,6 +87,12 @@ typedef struct lblx_run {
                                read when icntnm = 6 (record 1.2a)          */
   int32_t ifrgnx;           /* IFRGNX = 1: MT_CKD foreign closure table    */
   int32_t iaersl;           /* IAERSL (Rayleigh only for 0)                */
+  int32_t voigt;            /* 0: HIRAC1 of src/oprop.f90 (approximate
+                               Voigt: F1/F2/F3 + FG + XVER with VOICON
+                               weights); 1: the tabulated exact-Voigt build
+                               src/oprop_voigt.f90 (voigt_init tables over
+                               101 zeta, CNVFNV:1207-1393, CONVF4:3807-3999) */
+  int32_t pad0;
 } lblx_run;
 
 /* One layer, as /FILHDR/ holds it when OPDPTH is called.                */
@@ -275,6 +281,12 @@ void *lblx_stream(lblx_ctx *ctx);
  * lblx_timing_get returns the summed milliseconds and launch count of a
  * stage index (0 .. LBLX_NSTAGE-1).                                       */
 #define LBLX_NSTAGE 13
+
+/* Tables of the exact-Voigt build (voigt_init, src/oprop_voigt.f90:
+ * 11739-12008) as this library computes them on the host: f = F1, F2, F3,
+ * each [101 zeta columns][2601] (entry = Fortran IZ - 1); coef = a_1 a_2
+ * a_3 b_1 b_2 b_3, 101 each.  Needs no device.                          */
+int lblx_voigt_tables(double *f, double *coef);
 
 /* Diagnostics: the shrunk far-wing line list (TAPE9 content, LINF4 +
  * SHRINK, oprop.f90:2934-3640) of job `job` of the last OPDPTH call.      */

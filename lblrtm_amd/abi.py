@@ -63,6 +63,8 @@ class Run(C.Structure):
         ("cntscl", C.c_double * 7),
         ("ifrgnx", C.c_int32),
         ("iaersl", C.c_int32),
+        ("voigt", C.c_int32),
+        ("pad0", C.c_int32),
     ]
 
 
@@ -193,7 +195,7 @@ class LineTable:
 
 def make_run(v1, v2, sample=4.0, alfal0=0.04, avmass=36.0, dptmin=2e-4,
              dptfac=1e-3, ilblf4=1, icntnm=0, irad=0, precision=FP64, cntscl=None,
-             ifrgnx=0, iaersl=0):
+             ifrgnx=0, iaersl=0, voigt=0):
     r = Run()
     r.v1, r.v2 = v1, v2
     r.sample, r.alfal0, r.avmass = sample, alfal0, avmass
@@ -202,6 +204,7 @@ def make_run(v1, v2, sample=4.0, alfal0=0.04, avmass=36.0, dptmin=2e-4,
     for i, x in enumerate(cntscl if cntscl is not None else [0.0] * 7):
         r.cntscl[i] = x
     r.ifrgnx, r.iaersl = ifrgnx, iaersl
+    r.voigt = voigt
     return r
 
 

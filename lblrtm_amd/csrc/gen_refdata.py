@@ -1,7 +1,9 @@
 """Extract the numeric tables the monochromatic path needs from LBLRTM's
 Fortran sources (read as text) and write them as a C header.
 
-The tables are physical data, not algorithm: the Voigt-width/shape
+The tables are physical data, not algorithm: the exact-Voigt build's
+halfwidth ratios AVC and the Armstrong/Hummer coefficients
+(src/oprop_voigt.f90), the Voigt-width/shape
 coefficient table (BLOCK DATA VOICON, src/oprop.f90:1993-2144), the TIPS
 partition sums QofT(iso, T) for molecules 1-7 (QT_H2O..QT_O2,
 src/oprop.f90:6588-8231) on the 119-point temperature grid Tdat
@@ -140,6 +142,19 @@ def extract(src_dir):
         for k in range(1, n + 1):
             assert k in rows and len(rows[k]) == 119, (qname, k)
         qt[mi] = [rows[k] for k in range(1, n + 1)]
+
+    # ---- exact-Voigt build: AVC of voigt_init, armstrong's Chebyshev
+    # coefficients C and BLOCK DATA ARMCOEFF's W, T (oprop_voigt.f90)
+    vst = _join_continuations(open(src_dir + "/oprop_voigt.f90").read())
+    voigt = {}
+    for key, pat, n in [("AVC", r"\s*DATA\s+AVC\s*/", 102), ("ARM_C", r"\s*data\s+C\s*/", 34),
+                        ("ARM_W", r"\s*DATA\s+W\s*/", 10), ("ARM_T", r"\s*DATA\s+T\s*/", 10)]:
+        for st in vst:
+            if re.match(pat, st, re.I):
+                voigt[key] = _data_values(st)
+                break
+        assert key in voigt and len(voigt[key]) == n, (key, len(voigt.get(key, [])))
+    tables["VOIGT"] = voigt
     return tables, tdat, iso_max, smass, qt
 
 
@@ -148,13 +163,20 @@ def emit(tables, tdat, iso_max, smass, qt, out):
     w("/* Generated by gen_refdata.py from LBLRTM src/oprop.f90 DATA statements.\n"
       "   Numeric tables only: VOICON (oprop.f90:1993-2144), TIPS QofT for\n"
       "   molecules 1-7 (oprop.f90:6588-8231), Tdat (oprop.f90:6296-6316),\n"
-      "   ISO_MAX and SMASS (oprop.f90:2697-2895).  Do not edit. */\n")
+      "   ISO_MAX and SMASS (oprop.f90:2697-2895); the exact-Voigt build's AVC,\n"
+      "   armstrong C and ARMCOEFF W, T (oprop_voigt.f90:11797-11818,\n"
+      "   12058-12068, 12136-12150).  Do not edit. */\n")
     w("#ifndef LBLX_REFDATA_H\n#define LBLX_REFDATA_H\n\n")
     w("#define LBLX_NMOL_TIPS %d\n#define LBLX_NT_TIPS 119\n#define LBLX_MAXISO 10\n\n" % NMOL)
     for k in ["AVRAT", "CGAUSS", "CF1", "CF2", "CF3", "CER"]:
         w("static const double LBLX_%s[102] = {\n" % k)
         v = tables[k]
         for i in range(0, 102, 6):
+            w("  " + ", ".join(repr(x) for x in v[i:i + 6]) + ",\n")
+        w("};\n")
+    for k, v in tables["VOIGT"].items():
+        w("static const double LBLX_VOIGT_%s[%d] = {\n" % (k, len(v)))
+        for i in range(0, len(v), 6):
             w("  " + ", ".join(repr(x) for x in v[i:i + 6]) + ",\n")
         w("};\n")
     w("static const double LBLX_TDAT[119] = {\n")

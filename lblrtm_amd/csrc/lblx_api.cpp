@@ -10,6 +10,7 @@
 #include <atomic>
 #include <chrono>
 #include <climits>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +30,7 @@
 
 namespace lblx {
 void shape_tables_host(double *tab /* 5 x 2601: F1 F2 F3 FG XVER */);
+void voigt_tables_host(double *f /* (3 x 101 + 1) x 2601 */, double *coef /* 6 x 101 */);
 }
 
 using namespace lblx;
@@ -151,6 +153,8 @@ struct lblx_ctx {
   // device line table
   DevBuf d_vnu, d_sp, d_alfa, d_epp, d_hwhm, d_tmpalf, d_pshift, d_mol, d_blk, d_bstart, d_bvmax;
   DevBuf d_tabs;  // F1 F2 F3 FG XVER
+  DevBuf d_vtab, d_vtabf;  // exact-Voigt build tables, FP64 and FP32 (built on first use)
+  bool voigt_ready = false;
   DevBuf d_cont;  // continuum tables (ContTabs)
   ContTabs cont_tabs{};
   DevBuf w_cps;   // ContPlan per job
@@ -432,6 +436,41 @@ static int64_t job_lines(const lblx_ctx *c, const LayerPlan &P, const lblx_layer
   return std::max<int64_t>(n4, J->nn);
 }
 
+// Tables of the exact-Voigt build, computed once per process on the host
+// and uploaded to the context's device on first use.
+static const std::vector<double> &voigt_host(std::vector<double> **coef_out) {
+  static std::vector<double> f, coef;
+  static std::once_flag once;
+  std::call_once(once, []() {
+    f.assign((size_t)(3 * 101 + 1) * 2601, 0.);
+    coef.assign(6 * 101, 0.);
+    voigt_tables_host(f.data(), coef.data());
+  });
+  if (coef_out) *coef_out = &coef;
+  return f;
+}
+
+static int ensure_voigt(lblx_ctx *c) {
+  if (c->voigt_ready) return LBLX_OK;
+  std::vector<double> *coef = nullptr;
+  const std::vector<double> &f = voigt_host(&coef);
+  std::vector<float> ff(f.begin(), f.end());
+  double a3[102], b3[102];
+  for (int i = 0; i < 101; i++) {
+    a3[i] = (*coef)[2 * 101 + i];
+    b3[i] = (*coef)[5 * 101 + i];
+  }
+  a3[101] = (*coef)[3 * 101];  // a_3(101) is b_1(0) in COMMON /voigt_cf/
+  b3[101] = b3[100];
+  if (c->d_vtab.ensure(f.size() * 8) || c->d_vtabf.ensure(ff.size() * 4)) return LBLX_E_HIP;
+  HIPCHK(hipMemcpy(c->d_vtab.p, f.data(), f.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_vtabf.p, ff.data(), ff.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(lblx::c_va3), a3, sizeof(a3)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(lblx::c_vb3), b3, sizeof(b3)));
+  c->voigt_ready = true;
+  return LBLX_OK;
+}
+
 // Host side of an OPDPTH batch: the jobs' descriptors and plan tables, tile
 // maps and the strides of the per-job work arrays.
 struct JobsPrep {
@@ -522,6 +561,7 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
       Y.wk[m] = m < ly->nmol ? ly->wk[m] : 0.;
     }
     Y.nmol = ly->nmol;
+    Y.voigt = run->voigt ? 1 : 0;
     Y.jrad = P.jrad;
     Y.max1 = P.max1;
     Y.npanel = (int)P.panels.size();
@@ -632,6 +672,11 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   const int maxnn = J.maxnn, maxnbk = J.maxnbk, abs_stride = J.abs_stride, maxnb4 = J.maxnb4;
   const int nwin = (int)win.size();
   c->last_line_stride = S.line;
+  const bool voigt = run->voigt != 0;
+  if (voigt) {
+    int rv = ensure_voigt(c);
+    if (rv) return rv;
+  }
   const int64_t NJ = njobs;
   auto need = [&](DevBuf &b, size_t bytes) { return b.ensure(bytes) != 0; };
   if (need(c->w_dlay, NJ * sizeof(DevLayer)) || need(c->w_pan, pan.size() * sizeof(DevPanel)) ||
@@ -811,20 +856,29 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY,
                        tm[G - 1], nt, T, G, c->w_margin.as<double>());
     const size_t esz = split ? sizeof(float) : sizeof(double);
-    size_t shm = (size_t)(T + (G == 1 ? 3 : 1) * 2601) * esz;
+    size_t shm = (size_t)(T + (voigt ? 0 : (G == 1 ? 3 : 1) * 2601)) * esz;
     const ScatterRec *rec = c->w_rec.as<ScatterRec>();
     const DevPanel *dp = c->w_pan.as<DevPanel>();
     const DevWin *dw = c->w_win.as<DevWin>();
     const DevSeg *dg = c->w_seg.as<DevSeg>();
     const double *r4 = c->w_r4.as<double>();
     double *r3 = c->w_r3.as<double>(), *r2 = c->w_r2.as<double>();
-#define LBLX_TILE(GG, OUTT, ACCT, RIN, ROUT, ODP, ABS)                                                 \
+#define LBLX_TILE_V(GG, OUTT, ACCT, VV, VTAB, RIN, ROUT, ODP, ABS)                                \
   do {                                                                                             \
-    HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT>,                               \
+    HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT, VV>,                           \
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));             \
-    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT>), dim3(nt), dim3(256), shm, s, rec, tm[GG - 1], T, \
-                       tabs, dY, S, dp, dw, r4, (const double *)(ABS), dg, (const double *)(RIN), \
-                       (double *)(ROUT), (OUTT *)(ODP));                                         \
+    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT, VV>), dim3(nt), dim3(256), shm, s, rec, tm[GG - 1], \
+                       T, tabs, (const ACCT *)(VTAB), dY, S, dp, dw, r4, (const double *)(ABS), dg, \
+                       (const double *)(RIN), (double *)(ROUT), (OUTT *)(ODP));                   \
+  } while (0)
+#define LBLX_TILE(GG, OUTT, ACCT, RIN, ROUT, ODP, ABS)                                             \
+  do {                                                                                             \
+    if (voigt)                                                                                     \
+      LBLX_TILE_V(GG, OUTT, ACCT, true, (sizeof(ACCT) == 4 ? c->d_vtabf.p : c->d_vtab.p), RIN,     \
+                  ROUT, ODP, ABS);                                                                 \
+    else                                                                                           \
+      LBLX_TILE_V(GG, OUTT, ACCT, false, (sizeof(ACCT) == 4 ? c->d_vtabf.p : c->d_vtab.p), RIN,    \
+                  ROUT, ODP, ABS);                                                                 \
   } while (0)
     if (G == 3) {
       if (split) LBLX_TILE(3, double, float, nullptr, r3, nullptr, d_abs);
@@ -838,6 +892,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       LBLX_TILE(1, float, float, r2, nullptr, od_base, nullptr);
     }
 #undef LBLX_TILE
+#undef LBLX_TILE_V
     c->tend(tht);
   }
   HIPCHK(hipGetLastError());
@@ -1755,6 +1810,15 @@ int lblx_autotune(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay
   }
   if (tiles_out)
     for (int G = 0; G < 3; G++) tiles_out[G] = c->tiles[G];
+  return LBLX_OK;
+}
+
+int lblx_voigt_tables(double *f, double *coef) {
+  if (!f || !coef) return LBLX_E_ARG;
+  std::vector<double> *cf = nullptr;
+  const std::vector<double> &t = voigt_host(&cf);
+  std::memcpy(f, t.data(), (size_t)3 * 101 * 2601 * 8);
+  std::memcpy(coef, cf->data(), 6 * 101 * 8);
   return LBLX_OK;
 }
 

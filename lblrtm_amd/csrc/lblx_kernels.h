@@ -30,6 +30,7 @@ struct DevLayer {
   double dptmn_hirac, dptfc, dptmn4;
   double scor[7][10], alfd1[7][10], rhoslf[8], wk[8];
   int32_t nmol, jrad, max1, npanel, nwin;
+  int32_t voigt;          // exact-Voigt build (oprop_voigt.f90) tables and CONVF4
   int64_t g_last, g2max, g3max, n1, n2, n3;
   // continuum grid (ICNTNM): absrb[1..nptabs] on v1abs + (k-1)*dvabs
   double v1abs, v2abs, dvabs;
@@ -114,7 +115,7 @@ struct ScatterRec {
   int64_t g1lo;    // global fine index of JMIN1
   int64_t s2, s3;  // global shifts: G2 = G1 - s2, G3 = G1 - s3
   int32_t npts;    // JMAX1 - JMIN1 + 1 (0 = no contribution)
-  int32_t pad;
+  int32_t izm;     // exact-Voigt build: zeta column IZM (strd holds ZETDIF)
 };
 
 }  // namespace lblx

@@ -28,6 +28,11 @@ namespace lblx {
 
 __constant__ double c_avrat[102], c_cf1[102], c_cf2[102], c_cf3[102], c_cgauss[102], c_cer[102];
 __constant__ int c_isomax[7];
+// exact-Voigt build: a_3, b_3 of voigt_init over zeta index 0..100, and the
+// value CONVF4 reads one past the end (a_3(101) = b_1(0) in COMMON
+// /voigt_cf/; b_3(101) taken as b_3(100))
+__constant__ double c_va3[102], c_vb3[102];
+constexpr int NZETA = 101;
 
 constexpr double ONEPL_D = 1.001, ONEMI_D = 0.999;
 constexpr int NFMX = 2601;
@@ -639,6 +644,21 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         p.siv_a3 = (T)(siv * A3);
         p.siv_b3 = (T)(siv * B3);
         p.co2 = f_mol == 2;
+        if (Y.voigt) {  // oprop_voigt.f90:3875-3892, 3929-3931: zeta-dependent
+                        // a_3, b_3; CO2 only through CHI (= 1)
+          const int izx = iz - 1;  // 0..100
+          const double a3x = c_va3[izx] + zetdif * (c_va3[izx + 1] - c_va3[izx]);
+          const double b3x = c_vb3[izx] + zetdif * (c_vb3[izx + 1] - c_vb3[izx]);
+          const double f4_64x = (a3x + b3x * zsqbnd);
+          const double sivv = sp * ralfvi;
+          speak = (T)(a3x * fabs(sivv));
+          const double siv_64 = sivv * f4_64x * (alfli2 + (alfavi * alfavi) * zsqbnd);
+          p.sil = (T)siv_64;
+          p.f4bnd = (T)(siv_64 / (alfli2 + bndsq));
+          p.siv_a3 = (T)(sivv * a3x);
+          p.siv_b3 = (T)(sivv * b3x);
+          p.co2 = 0;
+        }
         partial = r4[jj];
       }
       const bool cand = valid && nz;
@@ -915,7 +935,7 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const i
   if (!valid) return;
   ScatterRec r;
   r.npts = 0;
-  r.pad = 0;
+  r.izm = 1;
   r.g1lo = 0;
   r.s2 = r.s3 = 0;
   r.zfl1 = r.zfl2 = r.zfl3 = r.zslope = 0.;
@@ -926,11 +946,17 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const i
     double depthi = n.sp * n.recalf;
     int izm = n.izeta;
     double zetdif = 100. * n.zetai - (double)(izm - 1);
-    r.strf1 = depthi * (c_cf1[izm - 1] + zetdif * (c_cf1[izm] - c_cf1[izm - 1]));
-    r.strf2 = depthi * (c_cf2[izm - 1] + zetdif * (c_cf2[izm] - c_cf2[izm - 1]));
-    r.strf3 = depthi * (c_cf3[izm - 1] + zetdif * (c_cf3[izm] - c_cf3[izm - 1]));
-    r.strd = depthi * (c_cgauss[izm - 1] + zetdif * (c_cgauss[izm] - c_cgauss[izm - 1]));
-    r.strver = depthi * (c_cer[izm - 1] + zetdif * (c_cer[izm] - c_cer[izm - 1]));
+    if (Y.voigt) {  // oprop_voigt.f90:1273-1276: DEPTHI and the zeta column
+      r.strf1 = r.strf2 = r.strf3 = depthi;
+      r.strd = zetdif;
+      r.izm = izm < 1 ? 1 : (izm > NZETA ? NZETA : izm);
+    } else {
+      r.strf1 = depthi * (c_cf1[izm - 1] + zetdif * (c_cf1[izm] - c_cf1[izm - 1]));
+      r.strf2 = depthi * (c_cf2[izm - 1] + zetdif * (c_cf2[izm] - c_cf2[izm - 1]));
+      r.strf3 = depthi * (c_cf3[izm - 1] + zetdif * (c_cf3[izm] - c_cf3[izm - 1]));
+      r.strd = depthi * (c_cgauss[izm - 1] + zetdif * (c_cgauss[izm] - c_cgauss[izm - 1]));
+      r.strver = depthi * (c_cer[izm - 1] + zetdif * (c_cer[izm] - c_cer[izm - 1]));
+    }
     double zslope = n.recalf * (Y.dv / 0.002);
     double zint = (n.vnu - vft) / Y.dv;
     double bhwdxf = (4. / 0.002) / zslope;
@@ -1038,12 +1064,14 @@ __device__ __forceinline__ double xint_at(double v1a, double v2a, double dva, co
 
 
 // The fields of a scatter record that grid G needs, one line per lane.
-template <int G>
+template <int G, bool V>
 struct TileLine {
   int64_t glo;
   double zfl, zslope, str, strd, strver;
-  int npts;
+  int npts, izm;
   __device__ __forceinline__ void load(const ScatterRec *rec, int li, int hi) {
+    strd = strver = 0.;
+    izm = 1;
     if (li < hi) {
       const ScatterRec &r = rec[li];
       npts = r.npts;
@@ -1051,7 +1079,10 @@ struct TileLine {
       zfl = G == 1 ? r.zfl1 : (G == 2 ? r.zfl2 : r.zfl3);
       zslope = r.zslope;
       str = G == 1 ? r.strf1 : (G == 2 ? r.strf2 : r.strf3);
-      if (G == 1) {
+      if (V) {  // ZETDIF and IZM of the exact-Voigt build
+        strd = r.strd;
+        izm = r.izm;
+      } else if (G == 1) {
         strd = r.strd;
         strver = r.strver;
       }
@@ -1060,8 +1091,6 @@ struct TileLine {
       glo = 0;
       zfl = zslope = str = 0.;
     }
-    if (G != 1) strd = strver = 0.;
-    if (G == 1 && li >= hi) strd = strver = 0.;
   }
 };
 
@@ -1070,10 +1099,14 @@ struct TileLine {
 //   G = 3: R3 = lines(F3) + XINT(R4) + XINT(ABSRB)            -> r3
 //   G = 2: R2 = lines(F2) + Lagrange4(R3)                       -> r2
 //   G = 1: OD = (lines(F1,FG,XVER) + Lagrange4(R2)) * radiation -> od
-template <int G, typename OUT, typename ACC>
+//   V (exact-Voigt build, oprop_voigt.f90:1305-1336): the line shape is
+//   DEPTHI ((1 - A) F_G(IZ, IZM) + A F_G(IZ, IZM + 1)) from the zeta tables
+//   in vtab (read through L2: 101 columns per table do not fit LDS), with
+//   no FG or XVER term.
+template <int G, typename OUT, typename ACC, bool V>
 __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M, int tile,
-                                              const double *tabs, const DevLayer *Ys, Strides S,
-                                              const DevPanel *pans, const DevWin *wins,
+                                              const double *tabs, const ACC *vtab, const DevLayer *Ys,
+                                              Strides S, const DevPanel *pans, const DevWin *wins,
                                               const double *r4, const double *absrb,
                                               const DevSeg *segs, const double *rin, double *rout,
                                               OUT *od) {
@@ -1088,14 +1121,17 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
   const DevLayer &Y = Ys[job];
   const ScatterRec *rec = recs + job * S.line;
   for (int k = tid; k < tile; k += 256) acc[k] = (ACC)0.;
-  const double *src = tabs + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NFMX;
-  for (int k = tid; k < NFMX; k += 256) t1[k] = (ACC)src[k];
-  if (G == 1) {
-    for (int k = tid; k < NFMX; k += 256) {
-      tg[k] = (ACC)tabs[3 * NFMX + k];
-      tv[k] = (ACC)tabs[4 * NFMX + k];
+  if (!V) {
+    const double *src = tabs + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NFMX;
+    for (int k = tid; k < NFMX; k += 256) t1[k] = (ACC)src[k];
+    if (G == 1) {
+      for (int k = tid; k < NFMX; k += 256) {
+        tg[k] = (ACC)tabs[3 * NFMX + k];
+        tv[k] = (ACC)tabs[4 * NFMX + k];
+      }
     }
   }
+  const ACC *vt = vtab + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NZETA * NFMX;
   __syncthreads();
   const int64_t T0 = (int64_t)M.loc[blockIdx.x] * tile;
   const int lo = M.lo[blockIdx.x], hi = M.hi[blockIdx.x];
@@ -1105,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
   // record of line c0 + i with vector loads, then the wave walks the chunk
   // line by line, broadcasting each line's fields with readlane.  The next
   // chunk's records are loaded before the current chunk is scattered.
-  TileLine<G> cur, nxt;
+  TileLine<G, V> cur, nxt;
   int c0 = lo + wave * 64;
   if (c0 < hi) cur.load(rec, c0 + lane, hi);
   for (; c0 < hi; c0 += 256) {
@@ -1118,8 +1154,11 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
       const int64_t glo = rl_i64(cur.glo, j);
       const double zfl = rl_f64(cur.zfl, j), zslope = rl_f64(cur.zslope, j);
       const ACC str = (ACC)rl_f64(cur.str, j);
-      const ACC strd = G == 1 ? (ACC)rl_f64(cur.strd, j) : (ACC)0.;
-      const ACC strver = G == 1 ? (ACC)rl_f64(cur.strver, j) : (ACC)0.;
+      const ACC strd = (G == 1 && !V) ? (ACC)rl_f64(cur.strd, j) : (ACC)0.;
+      const ACC strver = (G == 1 && !V) ? (ACC)rl_f64(cur.strver, j) : (ACC)0.;
+      const ACC za = V ? (ACC)rl_f64(cur.strd, j) : (ACC)0.;
+      const ACC *ca = vt;
+      if (V) ca = vt + (size_t)(__builtin_amdgcn_readlane(cur.izm, j) - 1) * NFMX;
       int64_t a = glo > T0 ? glo : T0;
       int64_t b = glo + npts - 1;
       if (b > T0 + tile - 1) b = T0 + tile - 1;
@@ -1129,7 +1168,8 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
         int iz = fint(fabs(zf) + 1.5) - 1;
         iz = iz > NFMX - 1 ? NFMX - 1 : iz;  // |zf| <= 2500 by construction
         ACC v;
-        if (G == 1) v = str * t1[iz] + strd * tg[iz] + strver * tv[iz];
+        if (V) v = str * ((((ACC)1. - za) * ca[iz]) + (za * ca[iz + NFMX]));
+        else if (G == 1) v = str * t1[iz] + strd * tg[iz] + strver * tv[iz];
         else v = str * t1[iz];
         atomicAdd(&acc[k - T0], v);
       }

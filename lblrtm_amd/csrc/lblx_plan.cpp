@@ -751,4 +751,112 @@ int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan 
   return LBLX_OK;
 }
 
+// K(x, y) = Re w(x + iy) by Armstrong's three-region method, as the
+// exact-Voigt build evaluates it (armstrong, src/oprop_voigt.f90:12010-12134):
+// Dawson's function from Hummer's Chebyshev series (asymptotic beyond
+// x = 5) plus the y power series near the axis; 10-point Gauss-Hermite
+// quadrature of the Voigt integral (and of its x-antiderivative for
+// moderate y) elsewhere.
+static double voigt_k(double x, double y) {
+  const double y2 = y * y;
+  const double *W = LBLX_VOIGT_ARM_W, *T = LBLX_VOIGT_ARM_T;
+  if ((y < 1.0 && x < 4.0) || y < 1.8 / (x + 1.0)) {
+    const double u1 = (x * x - y2 > 70.0) ? 0.0 : std::exp(-(x * x) + y2) * std::cos(2.0 * x * y);
+    double d1, d0;  // D_n and D_{n-1} of the recurrence
+    if (x > 5.0) {
+      const double p2 = x * x, p4 = p2 * p2, p8 = p4 * p4;
+      d1 = -(.5 / p2 + .75 / p4 + 1.875 / (p4 * p2) + 6.5625 / p8 + 29.53125 / (p8 * p2) +
+             162.4218 / (p8 * p4) + 1055.7421 / ((p8 * p4) * p2));
+      d0 = (1.0 - d1) / (2.0 * x);
+    } else {
+      const double t = x / 5.0, c = 4.0 * t * t - 2.0;
+      double b0 = 0., b1 = 0., b2 = 0.;
+      for (int k = 33; k >= 0; k--) {
+        b0 = c * b1 - b2 + LBLX_VOIGT_ARM_C[k];
+        b2 = b1;
+        b1 = b0;
+      }
+      d0 = t * (b0 - b2);
+      d1 = 1.0 - 2.0 * x * d0;
+    }
+    double sum = y * d1;
+    if (y > 1.0e-08) {
+      double sgn = 1.0, yp = y;
+      for (int n = 2; n <= 50; n++) {
+        const double dn = (x * d1 + d0) * (-2.) / (double)n;
+        d0 = d1;
+        d1 = dn;
+        if (n & 1) {
+          sgn = -sgn;
+          yp = yp * y2;
+          const double term = dn * yp;
+          sum = sum + sgn * term;
+          if (std::fabs(term / sum) <= 1.0e-08) break;
+        }
+      }
+    }
+    return u1 - 1.12837917 * sum;
+  }
+  double g = 0.;
+  if (y < 2.5 && x < 4.0) {
+    for (int i = 0; i < 10; i++) {
+      const double r = T[i] - x, q = T[i] + x;
+      g = g + (4.0 * T[i] * T[i] - 2.0) *
+                  (r * std::atan(r / y) + q * std::atan(q / y) - 0.5 * y * (std::log(y2 + r * r) + std::log(y2 + q * q))) *
+                  W[i];
+    }
+    return 0.318309886 * g;
+  }
+  for (int i = 0; i < 10; i++)
+    g = g + (1.0 / ((x - T[i]) * (x - T[i]) + y2) + 1.0 / ((x + T[i]) * (x + T[i]) + y2)) * W[i];
+  return 0.318309886 * y * g;
+}
+
+// Tables of the exact-Voigt build (voigt_init, src/oprop_voigt.f90:11739-12008).
+// For zeta = 0, 0.01, .., 1 (Doppler HWHM 1, Lorentz zeta/(1-zeta)) the
+// Voigt profile in units of its halfwidth AV = AVC(zeta)(AL + AD) is
+// split over the domains 4, 16, 64 halfwidths (2001-point grids):
+//   F1 = V - q1 on [0,4);  F2 = q1 - q2 on [0,4], V - q2 on (4,16);
+//   F3 = q2 - q3 on [0,16], V - q3 on (16,64),
+// q_k = a_k + b_k z^2 matching V and its central-difference slope at the
+// domain end.  f: 3 tables x 101 columns x 2601 (column = zeta index,
+// entry = Fortran IZ - 1), then one zero column, in COMMON /FNSH/ order so
+// that column 101 of a table is column 0 of the next.  coef: a_1 a_2 a_3
+// b_1 b_2 b_3, 101 each.
+void voigt_tables_host(double *f, double *coef) {
+  const int NZ = 101, NX = 2001, NF = 2601;
+  std::fill(f, f + (size_t)(3 * NZ + 1) * NF, 0.);
+  const double dom[3] = {4.0, 16.0, 64.0};
+  const double pi = 2.0 * std::asin(1.0);
+  const double cl = std::sqrt(std::log(2.0));
+  std::vector<double> zg(NX + 1), v(NX + 1);
+  double *a = coef, *b = coef + 3 * NZ;  // a[k*NZ + iz], b[k*NZ + iz]
+  for (int iz = 0; iz < NZ; iz++) {
+    const double zeta = (double)iz / (double)(NZ - 1);
+    const bool lorentz = !(zeta < 1.0);
+    const double ad = lorentz ? 0. : 1., al = lorentz ? 1.0 : zeta / (1 - zeta);
+    const double av = LBLX_VOIGT_AVC[iz] * (ad + al);
+    const double norm = cl / std::sqrt(pi) * av;
+    const double y = lorentz ? 100.0 : cl * zeta / (1. - zeta);
+    for (int d = 0; d < 3; d++) {
+      const double step = dom[d] / 2000.;
+      const double dx = ad != 0.0 ? (cl * av) / ad * step : (cl * av) * step;
+      for (int i = 0; i <= NX; i++) {
+        zg[i] = (double)i * dom[d] / 2000.;
+        v[i] = lorentz ? (1. / pi) * (1. / (1. + zg[i] * zg[i])) : voigt_k((double)i * dx, y) * norm;
+      }
+      const double slope = .5 * (v[NX] - v[NX - 2]) / step;
+      b[d * NZ + iz] = slope / (2 * dom[d]);
+      a[d * NZ + iz] = v[NX - 1] - b[d * NZ + iz] * (dom[d] * dom[d]);
+      double *col = f + (size_t)(d * NZ + iz) * NF;
+      for (int i = 0; i <= NX - 2; i++) {
+        const double z2 = zg[i] * zg[i];
+        const double q_lo = d > 0 ? a[(d - 1) * NZ + iz] + b[(d - 1) * NZ + iz] * z2 : 0.;
+        const double q = a[d * NZ + iz] + b[d * NZ + iz] * z2;
+        col[i] = (d > 0 && zg[i] <= dom[d - 1]) ? q_lo - q : v[i] - q;
+      }
+    }
+  }
+}
+
 }  // namespace lblx

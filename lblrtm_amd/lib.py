@@ -80,6 +80,7 @@ def load():
                                                 C.c_void_p, P(C.c_int64), P(C.c_double)]),
         "lblx_autotune": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int)]),
         "lblx_set_tiles": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lblx_voigt_tables": (C.c_int, [P(C.c_double), P(C.c_double)]),
         "lblx_merge_size": (C.c_int, [P(abi.Run), P(abi.Layer), P(abi.Layer), P(C.c_int64)]),
         "lblx_xmerge_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer),
                                          C.c_double, C.c_double, C.c_double, C.c_void_p,
@@ -138,6 +139,15 @@ def merge_size(run, layer_old, layer):
     _chk(load().lblx_merge_size(C.byref(run), C.byref(layer_old) if layer_old is not None else None,
                                 C.byref(layer), C.byref(n)), "lblx_merge_size")
     return n.value
+
+
+def voigt_tables():
+    """The library's exact-Voigt tables (host build): f [3, 101, 2601] and
+    coefficients a_1 a_2 a_3 b_1 b_2 b_3 [6, 101]."""
+    f = np.zeros(3 * 101 * 2601)
+    c = np.zeros(6 * 101)
+    _chk(load().lblx_voigt_tables(abi.ptr(f, C.c_double), abi.ptr(c, C.c_double)), "lblx_voigt_tables")
+    return f.reshape(3, 101, 2601), c.reshape(6, 101)
 
 
 def scan_size(scan, v1_in, dv_in, n_in):

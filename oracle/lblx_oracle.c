@@ -17,6 +17,9 @@
  *   tips_scor      TIPS_2003 + AtoB          src/oprop.f90:5919-6295, 11708-11798
  *   molec_layer    MOLEC(2)                  src/oprop.f90:2476-2557
  *   shape_tables   SHAPEL, SHAPEG, VERFN     src/oprop.f90:1822-1991
+ *   voigt_init,    exact-Voigt tables        src/oprop_voigt.f90:11739-12150
+ *   oracle_armstrong (run->voigt: CNVFNV     src/oprop_voigt.f90:1207-1393,
+ *                  and CONVF4 variants)      3807-3999
  *   radfn, radfni  RADFN, RADFNI             src/oprop.f90:2235-2475
  *   linf4          LINF4 + RDLNFL + SHRINK   src/oprop.f90:2934-3293, 3371-3640
  *   lblf4          LBLF4 + RDLIN4 + CONVF4   src/oprop.f90:3641-4039
@@ -214,6 +217,191 @@ void shape_tables(double *f1, double *f2, double *f3, double *fg, double *xv)
 #undef AZ
 #undef BZ
 #undef XLORNZ
+}
+
+/* ------------------------------------------------------------------ */
+/* Exact-Voigt build (src/oprop_voigt.f90).  armstrong (:12010-12134) with
+ * the ARMCOEFF Gauss-Hermite nodes (:12136-12150) and Hummer's Chebyshev
+ * coefficients for Dawson's function; voigt_init (:11739-12008) builds
+ * F1/F2/F3 over 101 zeta values: the normalised Voigt minus the quadratic
+ * matching its value and central-difference slope at 4, 16 and 64
+ * halfwidths.  Tables are kept in the order of COMMON /FNSH/: column
+ * c = 101 k + (IZM - 1) of table k, so that F_k(IZ, 102) reads the first
+ * column of the next table (F3's is FG, all zero in this build), as the
+ * reference's F_k(IZ, IZM + 1) does at IZM = 101. */
+#define NZETA 101
+static int g_voigt = 0; /* set per call by oracle_od_layer from lblx_run.voigt */
+static double VF[3 * NZETA + 1][NFMX + 1];        /* [column][IZ], IZ 1-based */
+static double VCOEF[6][NZETA];                    /* a_1 a_2 a_3 b_1 b_2 b_3 */
+static int voigt_done = 0;
+
+static const double HUM_C[34] = {
+    0.1999999999972224, -0.1840000000029998, 0.1558399999965025, -0.1216640000043988,
+    0.0877081599940391, -0.0585141248086907, 0.0362157301623914, -0.0208497654398036,
+    0.0111960116346270, -0.56231896167109e-2, 0.26487634172265e-2, -0.11732670757704e-2,
+    0.4899519978088e-3, -0.1933630801528e-3, 0.722877446788e-4, -0.256555124979e-4,
+    0.86620736841e-5, -0.27876379719e-5, 0.8566873627e-6, -0.2518433784e-6,
+    0.709360221e-7, -0.191732257e-7, 0.49801256e-9, -0.12447734e-8,
+    0.2997777e-9, -0.696450e-10, 0.156262e-10, -0.33897e-11,
+    0.7116e-12, -0.1447e-12, 0.285e-13, -0.55e-14, 0.10e-14, -0.2e-15};
+static const double ARM_W[10] = {4.62243670E-1, 2.86675505E-1, 1.09017206E-1, 2.48105209E-2,
+                                 3.24377334E-3, 2.28338636E-4, 7.80255648E-6, 1.08606937E-7,
+                                 4.39934099E-10, 2.22939365E-13};
+static const double ARM_T[10] = {0.245340708, 0.737473729, 1.23407622, 1.73853771, 2.25497400,
+                                 2.78880606, 3.34785457, 3.94476404, 4.60368245, 5.38748089};
+
+/* K(x, y) = Re w(x + iy), Armstrong's three regions */
+double oracle_armstrong(double x, double y)
+{
+  double y2 = y * y;
+  if ((y < 1.0 && x < 4.0) || y < 1.8 / (x + 1.0)) {
+    double u1 = (x * x - y2 > 70.0) ? 0.0 : exp(-(x * x) + y2) * cos(2.0 * x * y);
+    double dn01, dn02;
+    if (x > 5.0) {
+      double x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4;
+      double x10 = x8 * x2, x12 = x8 * x4, x14 = x12 * x2;
+      dn01 = -(.5 / x2 + .75 / x4 + 1.875 / x6 + 6.5625 / x8 + 29.53125 / x10 + 162.4218 / x12 +
+               1055.7421 / x14);
+      dn02 = (1.0 - dn01) / (2.0 * x);
+    } else { /* Dawson's function by Clenshaw's recurrence */
+      double bn = 0., bn01 = 0., bn02 = 0.;
+      double x1 = x / 5.0;
+      double coef = 4.0 * x1 * x1 - 2.0;
+      for (int i = 1; i <= 34; i++) {
+        bn = coef * bn01 - bn02 + HUM_C[35 - i - 1];
+        bn02 = bn01;
+        bn01 = bn;
+      }
+      double f = x1 * (bn - bn02);
+      dn01 = 1.0 - 2.0 * x * f;
+      dn02 = f;
+    }
+    double funct = y * dn01;
+    if (y > 1.0e-08) {
+      double q = 1.0, yn = y;
+      for (int i = 2; i <= 50; i++) {
+        double dn = (x * dn01 + dn02) * (-2.) / (double)i;
+        dn02 = dn01;
+        dn01 = dn;
+        if (i % 2 == 1) {
+          q = -q;
+          yn = yn * y2;
+          double g = dn * yn;
+          funct = funct + q * g;
+          if (fabs(g / funct) <= 1.0e-08) break;
+        }
+      }
+    }
+    return u1 - 1.12837917 * funct;
+  } else if (y < 2.5 && x < 4.0) {
+    double g = 0.;
+    for (int i = 0; i < 10; i++) {
+      double r = ARM_T[i] - x, s = ARM_T[i] + x;
+      g = g + (4.0 * ARM_T[i] * ARM_T[i] - 2.0) *
+                  (r * atan(r / y) + s * atan(s / y) - 0.5 * y * (log(y2 + r * r) + log(y2 + s * s))) *
+                  ARM_W[i];
+    }
+    return 0.318309886 * g;
+  } else {
+    double g = 0.;
+    for (int i = 0; i < 10; i++)
+      g = g + (1.0 / ((x - ARM_T[i]) * (x - ARM_T[i]) + y2) + 1.0 / ((x + ARM_T[i]) * (x + ARM_T[i]) + y2)) *
+                  ARM_W[i];
+    return 0.318309886 * y * g;
+  }
+}
+
+static void voigt_init(void)
+{
+  /* AVC: Voigt halfwidth over (AL + AD) at zeta = 0, 0.01, .. 1 */
+  static const double AVC[NZETA + 1] = {
+      .10000E+01, .99535E+00, .99073E+00, .98613E+00, .98155E+00, .97700E+00, .97247E+00,
+      .96797E+00, .96350E+00, .95905E+00, .95464E+00, .95025E+00, .94589E+00, .94156E+00,
+      .93727E+00, .93301E+00, .92879E+00, .92460E+00, .92045E+00, .91634E+00, .91227E+00,
+      .90824E+00, .90425E+00, .90031E+00, .89641E+00, .89256E+00, .88876E+00, .88501E+00,
+      .88132E+00, .87768E+00, .87410E+00, .87058E+00, .86712E+00, .86372E+00, .86039E+00,
+      .85713E+00, .85395E+00, .85083E+00, .84780E+00, .84484E+00, .84197E+00, .83919E+00,
+      .83650E+00, .83390E+00, .83141E+00, .82901E+00, .82672E+00, .82454E+00, .82248E+00,
+      .82053E+00, .81871E+00, .81702E+00, .81547E+00, .81405E+00, .81278E+00, .81166E+00,
+      .81069E+00, .80989E+00, .80925E+00, .80879E+00, .80851E+00, .80842E+00, .80852E+00,
+      .80882E+00, .80932E+00, .81004E+00, .81098E+00, .81214E+00, .81353E+00, .81516E+00,
+      .81704E+00, .81916E+00, .82154E+00, .82418E+00, .82708E+00, .83025E+00, .83370E+00,
+      .83742E+00, .84143E+00, .84572E+00, .85029E+00, .85515E+00, .86030E+00, .86573E+00,
+      .87146E+00, .87747E+00, .88376E+00, .89035E+00, .89721E+00, .90435E+00, .91176E+00,
+      .91945E+00, .92741E+00, .93562E+00, .94409E+00, .95282E+00, .96179E+00, .97100E+00,
+      .98044E+00, .99011E+00, .10000E+01, .10000E+01};
+  const double dom[3] = {4.0, 16.0, 64.0};
+  const int nx = NFPTS;
+  double pi = 2.0 * asin(1.0);
+  double cst = sqrt(log(2.0));
+  static double z[NFPTS + 1], fv[NFPTS + 1];
+  memset(VF, 0, sizeof(VF));
+  for (int izet = 0; izet < NZETA; izet++) {
+    double zeta = (double)izet / (double)(NZETA - 1);
+    double ad, al;
+    if (zeta < 1.0) {
+      ad = 1.;
+      al = zeta / (1 - zeta);
+    } else {
+      ad = 0.;
+      al = 1.0;
+    }
+    double av = AVC[izet] * (ad + al);
+    double cnorm1 = cst / sqrt(pi) * av;
+    double y = (zeta != 1.0) ? cst * zeta / (1. - zeta) : 100.0;
+    for (int id = 0; id < 3; id++) {
+      double dx = (ad != 0.0) ? (cst * av) / ad * (dom[id] / 2000.) : (cst * av) * (dom[id] / 2000.);
+      double dz = dom[id] / 2000.;
+      for (int i = 0; i <= nx; i++) {
+        z[i] = (double)i * dom[id] / 2000.;
+        if (zeta < 1.0) fv[i] = oracle_armstrong((double)i * dx, y) * cnorm1;
+        else fv[i] = (1. / pi) * (1. / (1. + z[i] * z[i]));
+      }
+      double dfv = .5 * (fv[nx] - fv[nx - 2]) / dz;
+      double b = dfv / (2 * dom[id]);
+      VCOEF[3 + id][izet] = b;
+      VCOEF[id][izet] = fv[nx - 1] - b * (dom[id] * dom[id]);
+      for (int iz = 0; iz <= nx - 2; iz++) {
+        double zz = z[iz] * z[iz];
+        double q1 = VCOEF[0][izet] + VCOEF[3][izet] * zz;
+        double q2 = VCOEF[1][izet] + VCOEF[4][izet] * zz;
+        double q3 = VCOEF[2][izet] + VCOEF[5][izet] * zz;
+        double f;
+        if (id == 0) f = fv[iz] - q1;
+        else if (id == 1) f = (z[iz] <= dom[0]) ? q1 - q2 : fv[iz] - q2;
+        else f = (z[iz] <= dom[1]) ? q2 - q3 : fv[iz] - q3;
+        VF[NZETA * id + izet][iz + 1] = f;
+      }
+    }
+  }
+  voigt_done = 1;
+}
+
+/* f: [3][101][NFMX] (table, IZM-1, IZ-1); coef: a_1 a_2 a_3 b_1 b_2 b_3 [6][101] */
+void oracle_voigt_tables(double *f, double *coef)
+{
+  if (!voigt_done) voigt_init();
+  for (int c = 0; c < 3 * NZETA; c++)
+    for (int i = 1; i <= NFMX; i++) f[(size_t)c * NFMX + i - 1] = VF[c][i];
+  for (int k = 0; k < 6; k++)
+    for (int j = 0; j < NZETA; j++) coef[k * NZETA + j] = VCOEF[k][j];
+}
+
+/* CONVF4's a_3/b_3 interpolated to zeta (src/oprop_voigt.f90:3875-3886).
+ * At IZ = 101 the reference reads a_3(101), the next word of COMMON
+ * /voigt_cf/ (b_1(0)), and b_3(101), past the block; the second read is
+ * taken as b_3(100) here.  IZ = 101 needs alpha_D / alpha_L < 1e-5. */
+static void voigt_a3b3(int iz, double zetdif, double *a3x, double *b3x)
+{
+  int izx = iz - 1;
+  if (izx == 101) {
+    izx = 100;
+    zetdif = 1.0;
+  }
+  double a_hi = izx + 1 <= NZETA - 1 ? VCOEF[2][izx + 1] : VCOEF[3][0];
+  double b_hi = izx + 1 <= NZETA - 1 ? VCOEF[5][izx + 1] : VCOEF[5][NZETA - 1];
+  *a3x = VCOEF[2][izx] + zetdif * (a_hi - VCOEF[2][izx]);
+  *b3x = VCOEF[5][izx] + zetdif * (b_hi - VCOEF[5][izx]);
 }
 
 /* ------------------------------------------------------------------ */
@@ -628,6 +816,13 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
       double sil = sp * recpi * alfali;
       double siv = (alfali * ralfvi) * sp * recpi * ralfvi;
       double speak = A3 * fabs(siv);
+      double a3x = 0., b3x = 0., f4_64x = 0.;
+      if (g_voigt) { /* src/oprop_voigt.f90:3875-3892 */
+        voigt_a3b3(iz, zetdif, &a3x, &b3x);
+        f4_64x = (a3x + b3x * zsqbnd);
+        siv = sp * ralfvi;
+        speak = a3x * fabs(siv);
+      }
       int jj = fint((T9->vnu[i] - W->v1r4) / W->dvr4 + 1.);
       if (jj < 1) jj = 1;
       if (jj > W->nptr4) jj = W->nptr4;
@@ -650,7 +845,22 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
       double siv_a3 = siv * A3;
       double siv_b3 = siv * B3;
       *k3upd += jmax - jmin + 1;
-      if (T9->mol[i] == 2) {
+      if (g_voigt) { /* src/oprop_voigt.f90:3929-3971: CO2 only through CHI (= 1) */
+        double siv_a = siv * a3x, siv_b = siv * b3x;
+        double siv_64 = siv * f4_64x * (alfli2 + alfvi2 * zsqbnd);
+        double f4bnd_v = siv_64 / (alfli2 + bndsq);
+        for (int j = jmin; j <= jmax; j++) {
+          double xm = (xjj - xnui);
+          double xmsq = xm * xm;
+          double zvsq = xmsq * rec_alfvi2;
+          double f4fn;
+          if (zvsq <= zsqbnd) f4fn = (siv_a + zvsq * siv_b) - f4bnd_v;
+          else f4fn = siv_64 / (alfli2 + xmsq) - f4bnd_v;
+          if (T9->mol[i] == 2) r4[j] = r4[j] + f4fn * 1.0;
+          else r4[j] = r4[j] + f4fn;
+          xjj = xjj + W->dvr4;
+        }
+      } else if (T9->mol[i] == 2) {
         for (int j = jmin; j <= jmax; j++) {
           double xm = (xjj - xnui);
           double xmsq = xm * xm;
@@ -1059,6 +1269,8 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
   static double F1[NFMX + 1], F2[NFMX + 1], F3[NFMX + 1], FG[NFMX + 1], XVER[NFMX + 1];
   static int shapes_done = 0;
   if (!shapes_done) { shape_tables(F1, F2, F3, FG, XVER); shapes_done = 1; }
+  g_voigt = run->voigt != 0;
+  if (g_voigt && !voigt_done) voigt_init();
   double DVP = DV;
   double DVR2 = (DXF2 / DXF1) * DV;
   double DVR3 = (DXF3 / DXF1) * DV;
@@ -1218,6 +1430,49 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
           double DEPTHI = B->sp[i] * B->recalf[i];
           int IZM = B->izeta[i];
           double ZETDIF = 100. * B->zetai[i] - (double)(IZM - 1);
+          if (g_voigt) { /* src/oprop_voigt.f90:1273-1336 (no line coupling) */
+            double A = ZETDIF;
+            double ZSLOPE = B->recalf[i] * WAVDXF;
+            double ZINT = (B->vnu[i] - VFT) / DV;
+            double BHWDXF = HWDXF / ZSLOPE;
+            int JMAX1 = fint(ZINT + BHWDXF + 1.5);
+            if (JMAX1 > MAX1) {
+              ILAST = j - 1;
+              IPANEL = 1;
+              goto label40;
+            }
+            int JMIN1 = fint(ZINT - BHWDXF + 1.5);
+            double RSHFT = 0.5;
+            if (ZINT < 0.0) RSHFT = -RSHFT;
+            int J2SHFT = fint(ZINT * (1. - CONF2) + RSHFT);
+            int J3SHFT = fint(ZINT * (1. - CONF3) + RSHFT);
+            int JMIN2 = JMIN1 - J2SHFT;
+            int JMIN3 = JMIN1 - J3SHFT;
+            double ZF1 = ((double)(JMIN1 - 2) - ZINT) * ZSLOPE;
+            double ZF2 = ((double)(JMIN2 - 2) - ZINT * CONF2) * ZSLOPE;
+            double ZF3 = ((double)(JMIN3 - 2) - ZINT * CONF3) * ZSLOPE;
+            const double *f1a = VF[IZM - 1], *f1b = VF[IZM];
+            const double *f2a = VF[NZETA + IZM - 1], *f2b = VF[NZETA + IZM];
+            const double *f3a = VF[2 * NZETA + IZM - 1], *f3b = VF[2 * NZETA + IZM];
+            stats.k1_updates += (int64_t)(JMAX1 - JMIN1 + 1);
+            for (int J1 = JMIN1; J1 <= JMAX1; J1++) {
+              int J2 = J1 - J2SHFT;
+              int J3 = J1 - J3SHFT;
+              ZF3 = ZF3 + ZSLOPE;
+              ZF2 = ZF2 + ZSLOPE;
+              ZF1 = ZF1 + ZSLOPE;
+              int IZ3 = fint(fabs(ZF3) + 1.5);
+              int IZ2 = fint(fabs(ZF2) + 1.5);
+              int IZ1 = fint(fabs(ZF1) + 1.5);
+              double x3 = DEPTHI * ((1. - A) * f3a[IZ3] + A * f3b[IZ3]);
+              double x2 = DEPTHI * ((1. - A) * f2a[IZ2] + A * f2b[IZ2]);
+              double x1 = DEPTHI * ((1. - A) * f1a[IZ1] + A * f1b[IZ1]);
+              R3[J3] = R3[J3] + x3;
+              R2[J2] = R2[J2] + x2;
+              R1[J1] = R1[J1] + x1;
+            }
+            continue;
+          }
 #define ZI(T) (T[IZM - 1] + ZETDIF * (T[IZM] - T[IZM - 1]))
           double STRF1 = DEPTHI * ZI(LBLX_CF1);
           double STRF2 = DEPTHI * ZI(LBLX_CF2);

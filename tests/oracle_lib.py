@@ -28,6 +28,10 @@ def lib():
             C.POINTER(abi.Run), C.c_int, C.c_int, C.POINTER(C.c_double),
             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
             C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.oracle_armstrong.restype = C.c_double
+        L.oracle_armstrong.argtypes = [C.c_double, C.c_double]
+        L.oracle_voigt_tables.restype = None
+        L.oracle_voigt_tables.argtypes = [C.POINTER(C.c_double)] * 2
         L.oracle_shape_tables.restype = None
         L.oracle_shape_tables.argtypes = [C.POINTER(C.c_double)] * 5
         L.tips_scor.restype = None
@@ -193,3 +197,12 @@ def fftscn(scan, spectrum, v1_in, dv_in, cap=4000000):
     if rc:
         raise RuntimeError("oracle_fftscn rc=%d" % rc)
     return out[:n.value].copy(), v1.value, dv.value
+
+
+def voigt_tables():
+    """Exact-Voigt build tables: f [3, 101, 2601] (table, zeta index, IZ-1)
+    and coefficients a_1 a_2 a_3 b_1 b_2 b_3 [6, 101]."""
+    f = np.zeros(3 * 101 * 2601)
+    c = np.zeros(6 * 101)
+    lib().oracle_voigt_tables(f.ctypes.data_as(C.POINTER(C.c_double)), c.ctypes.data_as(C.POINTER(C.c_double)))
+    return f.reshape(3, 101, 2601), c.reshape(6, 101)
