@@ -20,6 +20,10 @@
 
 #include "../../include/lblx.h"
 #include "lblx_kernels.h"
+
+#ifndef LBLX_FP32_LDS64
+#define LBLX_FP32_LDS64 0xC  // bits G = 2, 3
+#endif
 #include "lblx_plan.h"
 #include "lblx_contdata.h"
 #include "lblx_refdata.h"
@@ -856,20 +860,28 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY,
                        tm[G - 1], nt, T, G, c->w_margin.as<double>());
     const size_t esz = split ? sizeof(float) : sizeof(double);
-    size_t shm = (size_t)(T + (voigt ? 0 : (G == 1 ? 3 : 1) * 2601)) * esz;
+    // FP32 shapes: accumulator in FP64 LDS for the grids in LBLX_FP32_LDS64
+    const bool l64 = !split || ((LBLX_FP32_LDS64 >> G) & 1);
+    size_t shm = (size_t)T * (l64 ? 8 : esz) + (size_t)(voigt ? 0 : (G == 1 ? 3 : 1) * 2601) * esz;
     const ScatterRec *rec = c->w_rec.as<ScatterRec>();
     const DevPanel *dp = c->w_pan.as<DevPanel>();
     const DevWin *dw = c->w_win.as<DevWin>();
     const DevSeg *dg = c->w_seg.as<DevSeg>();
     const double *r4 = c->w_r4.as<double>();
     double *r3 = c->w_r3.as<double>(), *r2 = c->w_r2.as<double>();
+#define LBLX_TILE_L(GG, OUTT, ACCT, VV, LACCT, VTAB, RIN, ROUT, ODP, ABS)                         \
+  do {                                                                                             \
+    HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT, VV, LACCT>,                    \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));             \
+    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT, VV, LACCT>), dim3(nt), dim3(256), shm, s, rec,      \
+                       tm[GG - 1], T, tabs, (const ACCT *)(VTAB), dY, S, dp, dw, r4,               \
+                       (const double *)(ABS), dg, (const double *)(RIN), (double *)(ROUT),         \
+                       (OUTT *)(ODP));                                                            \
+  } while (0)
 #define LBLX_TILE_V(GG, OUTT, ACCT, VV, VTAB, RIN, ROUT, ODP, ABS)                                \
   do {                                                                                             \
-    HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT, VV>,                           \
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));             \
-    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT, VV>), dim3(nt), dim3(256), shm, s, rec, tm[GG - 1], \
-                       T, tabs, (const ACCT *)(VTAB), dY, S, dp, dw, r4, (const double *)(ABS), dg, \
-                       (const double *)(RIN), (double *)(ROUT), (OUTT *)(ODP));                   \
+    if (l64) LBLX_TILE_L(GG, OUTT, ACCT, VV, double, VTAB, RIN, ROUT, ODP, ABS);                   \
+    else LBLX_TILE_L(GG, OUTT, ACCT, VV, ACCT, VTAB, RIN, ROUT, ODP, ABS);                         \
   } while (0)
 #define LBLX_TILE(GG, OUTT, ACCT, RIN, ROUT, ODP, ABS)                                             \
   do {                                                                                             \
@@ -893,6 +905,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     }
 #undef LBLX_TILE
 #undef LBLX_TILE_V
+#undef LBLX_TILE_L
     c->tend(tht);
   }
   HIPCHK(hipGetLastError());

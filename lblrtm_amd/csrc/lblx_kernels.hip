@@ -610,9 +610,9 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       p.jmax = 0;
       p.xnui = 0.;
       p.f4bnd = (T)0.;
-      T speak = (T)0., partial = (T)0.;
+      T speak = (T)0., partial = (T)0., pkb = (T)0.;
       double vnui = 0.;
-      bool nz = false;
+      bool nz = false, bnd_ok = false;
       int jj = 1;
       if (valid) {
         double sp = f_sp;
@@ -659,23 +659,44 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
           p.siv_b3 = (T)(sivv * b3x);
           p.co2 = 0;
         }
+        // peak bound of the shape: near branch <= siv_a3 (siv_b3 < 0), far
+        // branch <= sil / (alfli2 + zsqbnd alfvi2) (it applies beyond
+        // zsqbnd halfwidths squared); margins cover rounding in T
+        bnd_ok = sp > 0.;
+        const double pkn = (double)p.siv_a3;
+        const double pkf = (double)p.sil / ((double)p.alfli2 + zsqbnd * (alfavi * alfavi) * (1. - 1e-6));
+        pkb = (T)((pkn > pkf ? pkn : pkf) * (1. + 1e-5));
         partial = r4[jj];
       }
       const bool cand = valid && nz;
-      // Lower bound of the running R4 at this line: every earlier line adds
-      // at least -2 F4BND (shape >= 0, subtraction <= 2 F4BND), so a line
-      // rejected against that bound is rejected whatever the earlier lines
-      // decide; only the others go through the sequential test.
+      // Bounds of the running R4 at this line over the undecided earlier
+      // lines of the chunk: each adds at least -2 F4BND (shape >= 0,
+      // subtraction <= 2 F4BND) and at most its peak bound pk.  A line
+      // rejected against the lower bound is rejected, and one accepted
+      // against the upper bound accepted, whatever the earlier lines
+      // decide.  Only lines up to the last undecided one are taken in order
+      // (their exact running R4 is needed); the rest are accepted in bulk.
       T neg = cand ? (T)2. * p.f4bnd : (T)0.;
-      T pre = neg;
+      T pk = cand ? pkb : (T)0.;
+      T pre = neg, preu = pk;
       for (int d = 1; d < 64; d <<= 1) {
         T t = __shfl_up(pre, d);
-        if (lane >= d) pre += t;
+        T u = __shfl_up(preu, d);
+        if (lane >= d) {
+          pre += t;
+          preu += u;
+        }
       }
-      const T lbv = partial - (pre - neg);
-      const T thr_lb = (T)Y.dptmn4 + (T)Y.dptfc * lbv;
-      const bool surely_rej = speak < thr_lb - fabs(thr_lb) * lb_margin;
-      uint64_t todo = __ballot(cand && !surely_rej);
+      const T below = pre - neg, above = preu - pk;
+      const T slack = (T)Y.dptfc * (fabs(partial) + below + above) * lb_margin + fabs((T)Y.dptmn4) * lb_margin;
+      const T thr_lb = (T)Y.dptmn4 + (T)Y.dptfc * (partial - below);
+      const T thr_ub = (T)Y.dptmn4 + (T)Y.dptfc * (partial + above);
+      const bool surely_rej = speak < thr_lb - slack;
+      const bool surely_acc = cand && bnd_ok && speak > thr_ub + slack;
+      const uint64_t undm = __ballot(cand && !surely_rej && !surely_acc);
+      const int lastund = undm ? 63 - __builtin_clzll(undm) : -1;
+      const uint64_t head = lastund < 0 ? 0ull : (lastund == 63 ? ~0ull : ((1ull << (lastund + 1)) - 1));
+      uint64_t todo = __ballot(cand && !surely_rej) & head;
       const double xjj_me = (double)(jj - 1) * dvr4;
       int nacc = 0;
       int done = 0;
@@ -706,6 +727,19 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
           if (lane == j && p.jmax >= p.jmin) par[nacc] = p;
           if (q.jmax >= q.jmin) nacc++;
         }
+      }
+      if (!done) {  // lanes after the last undecided one: accepted in bulk
+        uint64_t rest = __ballot(surely_acc) & ~head;
+        const uint64_t stpm = __ballot(surely_acc && vnui >= vnulst) & rest;
+        if (stpm) {  // the first accepted line past VNULST ends the window
+          const int js = __builtin_ctzll(stpm);
+          rest &= (1ull << js) - 1;
+          done = 1;
+        }
+        const uint64_t addm = rest & __ballot(p.jmax >= p.jmin);
+        const uint64_t below_me = lane == 0 ? 0ull : (addm & ((1ull << lane) - 1));
+        if ((addm >> lane) & 1) par[nacc + __builtin_popcountll(below_me)] = p;
+        nacc += __builtin_popcountll(addm);
       }
       if (lane == 0) {
         s_nacc = nacc;
@@ -1103,7 +1137,9 @@ struct TileLine {
 //   DEPTHI ((1 - A) F_G(IZ, IZM) + A F_G(IZ, IZM + 1)) from the zeta tables
 //   in vtab (read through L2: 101 columns per table do not fit LDS), with
 //   no FG or XVER term.
-template <int G, typename OUT, typename ACC, bool V>
+//   LACC: type of the LDS accumulator (ACC, or double under FP32 shapes
+//   where LDS float atomics stall).
+template <int G, typename OUT, typename ACC, bool V, typename LACC>
 __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M, int tile,
                                               const double *tabs, const ACC *vtab, const DevLayer *Ys,
                                               Strides S, const DevPanel *pans, const DevWin *wins,
@@ -1111,16 +1147,15 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
                                               const DevSeg *segs, const double *rin, double *rout,
                                               OUT *od) {
   extern __shared__ double sm_d[];
-  ACC *sm = reinterpret_cast<ACC *>(sm_d);
-  ACC *acc = sm;                    // [tile]
-  ACC *t1 = sm + tile;              // F_G table
+  LACC *acc = reinterpret_cast<LACC *>(sm_d);        // [tile]
+  ACC *t1 = reinterpret_cast<ACC *>(acc + tile);     // F_G table
   ACC *tg = t1 + NFMX;              // FG   (G = 1)
   ACC *tv = tg + NFMX;              // XVER (G = 1)
   const int tid = threadIdx.x;
   const int job = M.job[blockIdx.x];
   const DevLayer &Y = Ys[job];
   const ScatterRec *rec = recs + job * S.line;
-  for (int k = tid; k < tile; k += 256) acc[k] = (ACC)0.;
+  for (int k = tid; k < tile; k += 256) acc[k] = (LACC)0.;
   if (!V) {
     const double *src = tabs + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NFMX;
     for (int k = tid; k < NFMX; k += 256) t1[k] = (ACC)src[k];
@@ -1171,7 +1206,7 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
         if (V) v = str * ((((ACC)1. - za) * ca[iz]) + (za * ca[iz + NFMX]));
         else if (G == 1) v = str * t1[iz] + strd * tg[iz] + strver * tv[iz];
         else v = str * t1[iz];
-        atomicAdd(&acc[k - T0], v);
+        atomicAdd(&acc[k - T0], (LACC)v);
       }
     }
     cur = nxt;
