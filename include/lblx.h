@@ -240,6 +240,21 @@ int lblx_spectra_batch_device(lblx_ctx *ctx, const lblx_run *run, int nprof, con
 int lblx_autotune(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay, int *tiles_out);
 int lblx_set_tiles(lblx_ctx *ctx, const int *tiles);
 
+/* IEMIT = 0: layer optical depths summed up the column (XMERGE ->
+ * ABINIT for the first layer, ABSMRG after, src/xmerge.f90:270-372,
+ * 437-685, IPATHL = 1).  lblx_absmrg_device merges one layer: d_od is the
+ * layer's OD (double, or float under LBLX_FP32_SPLIT) on its grid,
+ * d_acc_old the accumulated OD on layer_old's grid (NULL with layer_old
+ * NULL: ABINIT), d_acc_new (double) receives the sum on the layer's grid;
+ * a coarser layer grid returns LBLX_E_COARSER (ABSMRG's STOP).
+ * lblx_column_od_device runs OPDPTH for every layer in one batch and the
+ * merge chain, writing the column OD on the top layer's grid.            */
+int lblx_absmrg_device(lblx_ctx *ctx, const lblx_run *run, const lblx_layer *layer_old,
+                       const lblx_layer *layer, const void *d_od, const double *d_acc_old,
+                       double *d_acc_new);
+int lblx_column_od_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay,
+                          double *d_od, int64_t n_out, double *v1p_top, lblx_stats *st);
+
 /* FFTSCN (src/fftscn_dbl.f90:25-789): record 10.1 of TAPE5.             */
 typedef struct lblx_scan {
   double hwhm;              /* HWHM field: OPD in cm when jfnin < 0        */

@@ -529,6 +529,7 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
     if (chain_jrad && jb > 0) ly->jrad = prev_jrad;
     if (pre) {
       plans[jb] = &pre[jb];
+      ly->jrad = pre[jb].jrad;
     } else {
       int rc = plan_layer(run, ly, &own[jb]);
       if (rc) return rc;
@@ -1226,7 +1227,7 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
   if (c->w_od.ensure((size_t)tot * esz) || c->w_rt.ensure((size_t)nmax * 8 * 4) ||
       c->w_mpan.ensure(pan.size() * sizeof(DevPanel)))
     return LBLX_E_HIP;
-  int rc = run_jobs(c, run, layers, nlay, true, c->w_od.p, off.data(), f32, nullptr, st);
+  int rc = run_jobs(c, run, layers, nlay, true, c->w_od.p, off.data(), f32, nullptr, st, plans.data());
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice,
                         c->stream));
@@ -1242,6 +1243,160 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
                       l == 0 ? tbnd : 0., emis, od, f32, ro, to, rn, tn,
                       c->w_mpan.as<DevPanel>() + pan0[l]);
     if (rc) return rc;
+  }
+  if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
+  return LBLX_OK;
+}
+
+// ABSMRG's walk for a merge of layer K's OD onto the accumulated OD on layer
+// O's grid (xmerge.f90:437-685): ITYPE, the alignment search (labels
+// 100-120), and the new point at which the old file ends.
+static int absmrg_plan(const LayerPlan &O, const LayerPlan &K, AbsArgs *A) {
+  *A = AbsArgs{};
+  A->n = K.npts;
+  A->n_old = O.npts;
+  A->t_eof = -1;
+  int itype = 0;
+  int rc = merge_itype(O.dv, K.dv, &itype);
+  if (rc) return rc;
+  A->itype = itype;
+  if (itype == 0) {  // panel by panel (labels 40-70)
+    if (O.panels.size() != K.panels.size() || O.npts != K.npts) return LBLX_E_UNSUPPORTED;
+    for (size_t p = 0; p < K.panels.size(); p++)
+      if (O.panels[p].nhi != K.panels[p].nhi) return LBLX_E_UNSUPPORTED;
+    return LBLX_OK;
+  }
+  const double ap = 1.0 / ((double)itype + 1.0);
+  for (int jpg = 1; jpg <= itype; jpg++) {
+    const double P = 1.0 - (ap * (double)jpg);
+    A->a1[jpg] = -P * (P - 1.0) * (P - 2.0) / 6.0;
+    A->a2[jpg] = (P * P - 1.0) * (P - 2.0) * 0.5;
+    A->a3[jpg] = -P * (P + 1.0) * (P - 2.0) * 0.5;
+    A->a4[jpg] = P * (P * P - 1.0) / 6.0;
+  }
+  const double v1po = O.panels[0].vft + (double)NSHIFT * O.dv, dvpo = O.dv;
+  const double v1p = K.panels[0].vft + (double)NSHIFT * K.dv, dvp = K.dv;
+  const int nlim0 = K.panels[0].nhi - NSHIFT, nlimo0 = O.panels[0].nhi - NSHIFT;
+  const double dif = dvp * 0.01;
+  int64_t nvs = 1, ii = 1;
+  if (!(std::fabs(v1po - v1p) < dif)) {
+    for (;;) {
+      nvs = nvs + 1;
+      const double v1pn = v1po + dvpo * (double)(nvs - 1);
+      if (std::fabs(v1pn - v1p) < dif) break;
+      if (v1pn < v1p) {
+        if (nvs > nlimo0) return LBLX_E_UNSUPPORTED;
+        continue;
+      }
+      bool found = false;
+      for (;;) {
+        ii = ii + 1;
+        const double v1pp = v1p + dvp * (double)(ii - 1);
+        if (std::fabs(v1pn - v1pp) < dif) {
+          found = true;
+          break;
+        }
+        if (!(v1pp < v1pn)) break;
+      }
+      if (found) break;
+      if (nvs > nlimo0 || ii > nlim0) return LBLX_E_UNSUPPORTED;
+    }
+  }
+  A->g0 = ii - 1;
+  A->nvs0 = nvs;
+  const int LL = itype + 1;
+  // NVS after the walk's increment at point t >= 1 is nvs(t) + 1 with
+  // nvs(t) = NVS0 + (t - 1) - (t - 1) / LL; the old file ends when it
+  // passes N_old - 2
+  auto nvs_at = [&](int64_t t) { return A->nvs0 + (t - 1) - (t - 1) / LL; };
+  const int64_t tmax = A->n - 1 - A->g0;
+  if (tmax >= 1 && nvs_at(tmax) >= A->n_old - 2) {
+    int64_t lo = 1, hi = tmax;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (nvs_at(mid) >= A->n_old - 2) hi = mid;
+      else lo = mid + 1;
+    }
+    A->t_eof = lo;
+    // the fill covers the rest of that new panel; a later panel would be
+    // merged from stale buffer contents
+    const int64_t g = A->g0 + lo;
+    const int64_t p = g / (NLIM1 - 1);
+    if ((p + 1) * (NLIM1 - 1) < A->n) return LBLX_E_UNSUPPORTED;
+  }
+  if (A->nvs0 + 2 > A->n_old) return LBLX_E_UNSUPPORTED;
+  return LBLX_OK;
+}
+
+static int absmrg_launch(lblx_ctx *c, const AbsArgs &A, const void *od, bool f32, const double *old,
+                         double *out) {
+  const unsigned nb = (unsigned)((A.n + 255) / 256);
+  int th = c->tbegin(ST_MERGE);
+  if (f32) hipLaunchKernelGGL(k_absmrg<float>, dim3(nb), dim3(256), 0, c->stream, A, (const float *)od, old, out);
+  else hipLaunchKernelGGL(k_absmrg<double>, dim3(nb), dim3(256), 0, c->stream, A, (const double *)od, old, out);
+  c->tend(th);
+  HIPCHK(hipGetLastError());
+  return LBLX_OK;
+}
+
+int lblx_absmrg_device(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer_old,
+                       const lblx_layer *layer, const void *d_od, const double *d_acc_old,
+                       double *d_acc_new) {
+  if (!c || !run || !layer || !d_od || !d_acc_new || (layer_old && !d_acc_old)) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  LayerPlan K, O;
+  lblx_layer tk = *layer;
+  int rc = plan_layer(run, &tk, &K);
+  if (rc) return rc;
+  AbsArgs A{};
+  if (layer_old) {
+    lblx_layer to = *layer_old;
+    if ((rc = plan_layer(run, &to, &O))) return rc;
+    if ((rc = absmrg_plan(O, K, &A))) return rc;
+  } else {
+    A.n = K.npts;
+    A.first = 1;
+  }
+  return absmrg_launch(c, A, d_od, run->precision == LBLX_FP32_SPLIT, d_acc_old, d_acc_new);
+}
+
+int lblx_column_od_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay, double *d_od,
+                          int64_t n_out, double *v1p_top, lblx_stats *st) {
+  if (!c || !run || !layers || nlay <= 0 || !d_od) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<LayerPlan> plans(nlay);
+  std::vector<int64_t> off(nlay);
+  int64_t tot = 0, nmax = 0;
+  int jr = layers[0].jrad;
+  for (int l = 0; l < nlay; l++) {
+    lblx_layer tmp = layers[l];
+    tmp.jrad = jr;
+    int rc = plan_layer(run, &tmp, &plans[l]);
+    if (rc) return rc;
+    jr = tmp.jrad;
+    off[l] = tot;
+    tot += plans[l].npts;
+    nmax = std::max(nmax, plans[l].npts);
+  }
+  if (n_out < plans[nlay - 1].npts) return LBLX_E_SIZE;
+  std::vector<AbsArgs> A(nlay);
+  A[0] = AbsArgs{};
+  A[0].n = plans[0].npts;
+  A[0].first = 1;
+  for (int l = 1; l < nlay; l++) {
+    int rc = absmrg_plan(plans[l - 1], plans[l], &A[l]);
+    if (rc) return rc;
+  }
+  const bool f32 = run->precision == LBLX_FP32_SPLIT;
+  const size_t esz = f32 ? 4 : 8;
+  if (c->w_od.ensure((size_t)tot * esz) || c->w_rt.ensure((size_t)nmax * 8 * 2)) return LBLX_E_HIP;
+  int rc = run_jobs(c, run, layers, nlay, true, c->w_od.p, off.data(), f32, nullptr, st, plans.data());
+  if (rc) return rc;
+  double *acc[2] = {c->w_rt.as<double>(), c->w_rt.as<double>() + nmax};
+  for (int l = 0; l < nlay; l++) {
+    const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+    double *dst = (l == nlay - 1) ? d_od : acc[l & 1];
+    if ((rc = absmrg_launch(c, A[l], od, f32, l > 0 ? acc[(l - 1) & 1] : nullptr, dst))) return rc;
   }
   if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
   return LBLX_OK;

@@ -1673,4 +1673,56 @@ __global__ void k_jac_scale(const double *src, int64_t istr, const double *tt, i
   for (int v = 0; v < nv; v++) dst[(int64_t)v * ostr + g] = t * src[(int64_t)v * istr + g];
 }
 
+
+// ---------------------------------------------------------------------
+// IEMIT = 0 merge of layer optical depths: ABINIT / ABSMRG
+// (xmerge.f90:270-685), IPATHL = 1.  One thread per point of the new
+// layer's grid.  ABSMRG's walk, after its alignment at new point g0 / old
+// point NVS0 (1-based), is periodic: new point g0 + t, t = c LL + j
+// (LL = ITYPE + 1), is the old point NVS0 + c ITYPE when j = 0, and for
+// j = 1..ITYPE the 4-point Lagrange value A(j) at taps NVS - 1 .. NVS + 2,
+// NVS = NVS0 + c ITYPE + j - 1 (the tap before the first old point is the
+// first point).  Past the walk's end of the old file (t > t_eof) the rest
+// of that new panel takes the mean of the old file's last two points.
+struct AbsArgs {
+  int64_t n, n_old, g0, nvs0, t_eof;
+  int32_t itype, first;
+  double a1[12], a2[12], a3[12], a4[12];
+};
+
+template <typename IN>
+__global__ __launch_bounds__(256) void k_absmrg(AbsArgs A, const IN *od, const double *old, double *out) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= A.n) return;
+  const double nw = (double)od[g];
+  double v;
+  if (A.first) {
+    v = nw;  // ABINIT
+  } else if (A.itype == 0) {
+    v = nw + old[g];
+  } else {
+    const int64_t t = g - A.g0;
+    if (t < 0) {
+      v = nw;
+    } else if (t == 0) {
+      v = 1.0 * nw + old[A.nvs0 - 1];
+    } else if (A.t_eof >= 0 && t > A.t_eof) {
+      v = 1.0 * nw + (old[A.n_old - 1] + old[A.n_old - 2]) * 0.5;
+    } else {
+      const int LL = A.itype + 1;
+      const int64_t c = t / LL;
+      const int j = (int)(t - c * LL);
+      if (j == 0) {
+        v = nw + old[A.nvs0 + c * A.itype - 1];
+      } else {
+        const int64_t nvs = A.nvs0 + c * A.itype + j - 1;
+        const double y = nvs == 1 ? old[0] : old[nvs - 2];
+        const double oi = A.a1[j] * y + A.a2[j] * old[nvs - 1] + A.a3[j] * old[nvs] + A.a4[j] * old[nvs + 1];
+        v = nw + oi;
+      }
+    }
+  }
+  out[g] = v;
+}
+
 }  // namespace lblx

@@ -81,6 +81,10 @@ def load():
         "lblx_autotune": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int)]),
         "lblx_set_tiles": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lblx_voigt_tables": (C.c_int, [P(C.c_double), P(C.c_double)]),
+        "lblx_absmrg_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer), C.c_void_p,
+                                         C.c_void_p, C.c_void_p]),
+        "lblx_column_od_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, C.c_void_p,
+                                            C.c_int64, P(C.c_double), C.c_void_p]),
         "lblx_merge_size": (C.c_int, [P(abi.Run), P(abi.Layer), P(abi.Layer), P(C.c_int64)]),
         "lblx_xmerge_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer),
                                          C.c_double, C.c_double, C.c_double, C.c_void_p,
@@ -306,6 +310,25 @@ class Context:
         finally:
             rad.free()
             jac.free()
+
+    def absmrg_device(self, run, layer_old, layer, d_od, d_acc_old, d_acc_new):
+        """One IEMIT = 0 merge step (ABINIT when layer_old is None)."""
+        _chk(self._lib.lblx_absmrg_device(self._h, C.byref(run), None if layer_old is None else C.byref(layer_old),
+                                          C.byref(layer), C.c_void_p(d_od),
+                                          C.c_void_p(d_acc_old) if d_acc_old else None, C.c_void_p(d_acc_new)),
+             "lblx_absmrg_device")
+
+    def column_od_device(self, run, layers, d_od, n_out):
+        """Column optical depth (OPDPTH + ABINIT/ABSMRG) on the top layer's
+        grid; returns its first wavenumber."""
+        n = len(layers)
+        arr = (abi.Layer * n)(*layers)
+        v1p = C.c_double()
+        _chk(self._lib.lblx_column_od_device(self._h, C.byref(run), arr, n, C.c_void_p(d_od), n_out,
+                                             C.byref(v1p), None), "lblx_column_od_device")
+        for i in range(n):
+            layers[i].jrad = arr[i].jrad
+        return v1p.value
 
     def spectra_batch_device(self, run, profiles, d_rad, d_tra, out_off):
         """Batch of profiles: profiles is a list of (layers, tzu, tbnd, emis).

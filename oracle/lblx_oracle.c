@@ -33,6 +33,7 @@
  *   oracle_contnm  CONTNM (H2O MT_CKD, CO2,  src/contnm.f90:20-990, 1295-1351,
  *                  O2/N2 fundamentals,       2668-2754, 7254-7318,
  *                  Rayleigh) + pre_xint      src/mt_ckd_h2o_module.f90:68-176
+ *   oracle_absmrg  ABINIT, ABSMRG (IEMIT=0)  src/xmerge.f90:270-372, 437-685
  *   oracle_merge_* EMIN, EMINIT, RADMRG,     src/xmerge.f90:968-1017, 1684-2241,
  *                  RADNN (IEMIT=1, IPATHL=1) 2245-3280
  *
@@ -402,6 +403,155 @@ static void voigt_a3b3(int iz, double zetdif, double *a3x, double *b3x)
   double b_hi = izx + 1 <= NZETA - 1 ? VCOEF[5][izx + 1] : VCOEF[5][NZETA - 1];
   *a3x = VCOEF[2][izx] + zetdif * (a_hi - VCOEF[2][izx]);
   *b3x = VCOEF[5][izx] + zetdif * (b_hi - VCOEF[5][izx]);
+}
+
+/* ------------------------------------------------------------------ */
+/* IEMIT = 0 merge of layer optical depths: ABINIT (layer 1: copy) and
+ * ABSMRG (src/xmerge.f90:270-372, 437-685), IPATHL = 1 (FACT = 1).  The
+ * accumulated OD (old: npo panels v1po, nlimo on grid dvo) is added to the
+ * new layer's OD (npk panels v1pk, nlimk on grid dvk), interpolating the
+ * old panels with 4-point Lagrange weights when the new grid is finer
+ * (ITYPE = DVK/(DVL-DVK)+0.5).  The walk below is the reference's: its
+ * alignment search, the NVS/II/JJ indices, the OLDR1 sliding buffer with
+ * three points carried across panel reads, and the fill with the mean of
+ * the last two old points when the old file ends.  Returns the number of
+ * points written, LBLX_E_COARSER for a coarser new grid, or
+ * LBLX_E_UNSUPPORTED when the walk would read past the end of the old file
+ * for more than the new file's last panel. */
+int oracle_absmrg(int npo, const double *v1po, const int *nlimo, double dvo, const double *old_in,
+                  int npk, const double *v1pk, const int *nlimk, double dvk, const double *od,
+                  double *out)
+{
+  int itype;
+  if (dvo == dvk) itype = 0;
+  else if (dvo > dvk) itype = fint(dvk / (dvo - dvk) + 0.5);
+  else return LBLX_E_COARSER;
+  const double FACT = 1.;
+  int64_t ko = 0, oo = 0, nout = 0;
+  if (itype == 0) { /* labels 40-70: panel by panel */
+    int po = 0;
+    for (int pk = 0; pk < npk; pk++) {
+      if (po >= npo || nlimo[po] < nlimk[pk]) return LBLX_E_UNSUPPORTED;
+      for (int k = 0; k < nlimk[pk]; k++) out[ko + k] = od[ko + k] + old_in[oo + k];
+      oo += nlimo[po++];
+      ko += nlimk[pk];
+    }
+    return (int)ko;
+  }
+  double A1[12], A2[12], A3[12], A4[12];
+  double atype = itype;
+  double ap = 1.0 / (atype + 1.0);
+  int LL = itype + 1;
+  for (int jpg = 1; jpg <= itype; jpg++) {
+    double apg = jpg;
+    double P = 1.0 - (ap * apg);
+    A1[jpg] = -P * (P - 1.0) * (P - 2.0) / 6.0;
+    A2[jpg] = (P * P - 1.0) * (P - 2.0) * 0.5;
+    A3[jpg] = -P * (P + 1.0) * (P - 2.0) * 0.5;
+    A4[jpg] = P * (P * P - 1.0) / 6.0;
+  }
+  static double OLDR1[2420], R1[2420];
+  double SAVOR1[6] = {0., 0., 0., 0., 0., 0.};
+  int po = 0, pk = 0;
+  /* BUFIN of the first old panel */
+  double V1PO = v1po[0], DVPO = dvo;
+  int NLIMO = nlimo[0];
+  for (int k = 0; k < NLIMO; k++) OLDR1[1 + k] = old_in[k];
+  oo = NLIMO;
+  po = 1;
+  int MAXLF = NLIMO, NVS = 1;
+  if (npk < 1) return 0;
+  double V1P = v1pk[0], DVP = dvk;
+  int NLIM = nlimk[0];
+  for (int k = 0; k < NLIM; k++) R1[1 + k] = od[k];
+  int II = 1;
+  double DIF = DVP * 0.01;
+  double V1PN, V1PP;
+  if (!(fabs(V1PO - V1P) < DIF)) {
+    for (;;) { /* labels 100/110 */
+      NVS = NVS + 1;
+      V1PN = V1PO + DVPO * (NVS - 1);
+      if (fabs(V1PN - V1P) < DIF) break;
+      if (V1PN < V1P) continue;
+      int found = 0;
+      for (;;) {
+        II = II + 1;
+        V1PP = V1P + DVP * (II - 1);
+        if (fabs(V1PN - V1PP) < DIF) { found = 1; break; }
+        if (!(V1PP < V1PN)) break;
+      }
+      if (found) break;
+      if (NVS > MAXLF || II > NLIM) return LBLX_E_UNSUPPORTED;
+    }
+  }
+  /* label 120 */
+  R1[II] = FACT * R1[II] + OLDR1[NVS];
+  II = II + 1;
+  int eof = 0;
+  for (;;) { /* label 130 */
+    int JJ = 1;
+    for (int JPG = 1; JPG <= LL; JPG++) {
+      if (JPG == LL) {
+        R1[II] = R1[II] + OLDR1[NVS];
+      } else {
+        double OLDR1Y;
+        if (NVS == 1) OLDR1Y = (SAVOR1[4] == 0.0) ? OLDR1[1] : SAVOR1[4];
+        else OLDR1Y = OLDR1[NVS - 1];
+        double OLDR1I = A1[JJ] * OLDR1Y + A2[JJ] * OLDR1[NVS] + A3[JJ] * OLDR1[NVS + 1] + A4[JJ] * OLDR1[NVS + 2];
+        R1[II] = R1[II] + OLDR1I;
+      }
+      /* label 190 */
+      NVS = NVS + 1;
+      int fill = 0;
+      if (!(NVS <= MAXLF - 2)) {
+        SAVOR1[1] = OLDR1[NVS - 1];
+        SAVOR1[2] = OLDR1[NVS];
+        SAVOR1[3] = OLDR1[NVS + 1];
+        SAVOR1[4] = OLDR1[NVS - 2];
+        if (po >= npo) {
+          fill = 1; /* label 210 */
+        } else {
+          NLIMO = nlimo[po];
+          for (int k = 0; k < NLIMO; k++) OLDR1[4 + k] = old_in[oo + k];
+          oo += NLIMO;
+          po++;
+          MAXLF = NLIMO + 3;
+          OLDR1[1] = SAVOR1[1];
+          OLDR1[2] = SAVOR1[2];
+          OLDR1[3] = SAVOR1[3];
+          NVS = 2;
+        }
+      }
+      int flush = 0;
+      if (fill) {
+        if (eof) return LBLX_E_UNSUPPORTED; /* walks on with stale buffer contents */
+        eof = 1;
+        II = II + 1;
+        double AVRG = (SAVOR1[3] + SAVOR1[2]) * 0.5;
+        while (II <= NLIM) {
+          R1[II] = FACT * R1[II] + AVRG;
+          II = II + 1;
+        }
+        flush = 1;
+      } else {
+        II = II + 1; /* label 200 */
+        JJ = JJ + 1;
+        if (II > NLIM) flush = 1;
+      }
+      if (flush) { /* label 230: ABSOUT, next new panel */
+        for (int k = 0; k < NLIM; k++) out[ko + k] = R1[1 + k];
+        ko += NLIM;
+        nout = ko;
+        pk++;
+        if (pk >= npk) return (int)nout;
+        if (eof) return LBLX_E_UNSUPPORTED;
+        NLIM = nlimk[pk];
+        for (int k = 0; k < NLIM; k++) R1[1 + k] = od[ko + k];
+        II = 1;
+      }
+    }
+    NVS = NVS - 1;
+  }
 }
 
 /* ------------------------------------------------------------------ */

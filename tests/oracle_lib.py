@@ -206,3 +206,25 @@ def voigt_tables():
     c = np.zeros(6 * 101)
     lib().oracle_voigt_tables(f.ctypes.data_as(C.POINTER(C.c_double)), c.ctypes.data_as(C.POINTER(C.c_double)))
     return f.reshape(3, 101, 2601), c.reshape(6, 101)
+
+
+def absmrg(old_pan, old_od, dvo, pan, layer_od, dvk):
+    """ABINIT/ABSMRG (IEMIT = 0) of the oracle: accumulated OD on the new
+    layer's panels.  old_pan / pan: (v1p, v2p, nlim) arrays."""
+    v1o, _, nlo = old_pan
+    v1, _, nl = pan
+    n = int(np.sum(nl))
+    out = np.zeros(n)
+    nloc = np.ascontiguousarray(nlo, np.int32)
+    nlc = np.ascontiguousarray(nl, np.int32)
+    oo = np.ascontiguousarray(old_od, np.float64)
+    od = np.ascontiguousarray(layer_od, np.float64)
+    L = lib()
+    L.oracle_absmrg.restype = C.c_int
+    rc = L.oracle_absmrg(len(v1o), _d(np.ascontiguousarray(v1o, np.float64)), abi.ptr(nloc, C.c_int),
+                         C.c_double(dvo), abi.ptr(oo, C.c_double), len(v1),
+                         _d(np.ascontiguousarray(v1, np.float64)), abi.ptr(nlc, C.c_int), C.c_double(dvk),
+                         abi.ptr(od, C.c_double), abi.ptr(out, C.c_double))
+    if rc < 0:
+        raise RuntimeError("oracle_absmrg rc=%d" % rc)
+    return out
