@@ -240,6 +240,15 @@ int lblx_spectra_batch_device(lblx_ctx *ctx, const lblx_run *run, int nprof, con
 int lblx_autotune(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay, int *tiles_out);
 int lblx_set_tiles(lblx_ctx *ctx, const int *tiles);
 
+/* OPDPTH with DVOUT (IOD = 1 or 4, src/lblrtm.f90:1062-1076): the layer's
+ * optical depth regridded by PNLINT (src/oprop.f90:1564-1806) onto a DVOUT
+ * grid starting at the layer's first point, in output panels of 2400
+ * points, the last ending at or above V2.  lblx_od_dvout_size gives the
+ * number of output points (needs no device).                            */
+int lblx_od_dvout_size(const lblx_run *run, const lblx_layer *layer, double dvout, int64_t *n);
+int lblx_od_layer_dvout_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layer, double dvout,
+                               double *d_out, int64_t cap, int64_t *n_out, double *v1_out);
+
 /* IEMIT = 0: layer optical depths summed up the column (XMERGE ->
  * ABINIT for the first layer, ABSMRG after, src/xmerge.f90:270-372,
  * 437-685, IPATHL = 1).  lblx_absmrg_device merges one layer: d_od is the

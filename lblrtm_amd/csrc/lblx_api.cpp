@@ -175,6 +175,7 @@ struct lblx_ctx {
   int64_t s_twN = 0;
   int tiles[3] = {2048, 1024, 512};       // R1, R2, R3 tile widths (autotuned)
   DevBuf b_jobs, b_rt;                    // batched merges
+  DevBuf w_dvo;                           // PNLINT runs
   struct GraphEntry {
     std::vector<int64_t> key;
     hipGraphExec_t exec = nullptr;
@@ -1399,6 +1400,169 @@ int lblx_column_od_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, 
     if ((rc = absmrg_launch(c, A[l], od, f32, l > 0 ? acc[(l - 1) & 1] : nullptr, dst))) return rc;
   }
   if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
+  return LBLX_OK;
+}
+
+// PNLINT's control flow over the panels of one layer (oprop.f90:1596-1800),
+// without the values: the output panels and, for every run of output
+// points computed in one call, the input panel and FJ1DIF.
+struct DvoPlan {
+  DvoArgs A;
+  std::vector<DvoSeg> segs;
+  double v1out = 0.;
+};
+static int dvout_plan(const LayerPlan &P, double v2, double dvout, DvoPlan *D) {
+  D->A = DvoArgs{};
+  D->segs.clear();
+  DvoArgs &A = D->A;
+  A.n_in = P.npts;
+  if (!(dvout > 0.)) return LBLX_E_ARG;
+  const double DVP = P.dv;
+  double atype = 9.999E09;
+  if (DVP == dvout) atype = 0.;
+  if (dvout > DVP) atype = DVP / (dvout - DVP) + 0.5;
+  if (dvout < DVP) atype = -dvout / (DVP - dvout) - 0.5;
+  if (std::fabs(dvout - DVP) < 1.E-8) atype = 0.;
+  D->v1out = P.panels[0].vft + (double)NSHIFT * P.dv;
+  if (atype == 0.) {
+    A.pass = 1;
+    A.n_out = P.npts;
+    return LBLX_OK;
+  }
+  for (int jp = 0; jp <= 100; jp++) {
+    const double p = 0.01 * (double)jp;
+    A.a1[jp] = -p * (p - 1.0) * (p - 2.0) / 6.0;
+    A.a2[jp] = (p * p - 1.0) * (p - 2.0) * 0.5;
+    A.a3[jp] = -p * (p + 1.0) * (p - 2.0) * 0.5;
+    A.a4[jp] = p * (p * p - 1.0) / 6.0;
+  }
+  const int LIMOUT = 2400;
+  A.ratdv = dvout / DVP;
+  double v1po = 0., v2po = 0.;
+  int nlim1 = 1, nlim2 = 0, ilast = 0;
+  int64_t emitted = 0;
+  const int np = (int)P.panels.size();
+  for (int q = 0; q < np; q++) {
+    const PanelPlan &Q = P.panels[q];
+    const double V1P = Q.vft + (double)NSHIFT * P.dv;
+    double V2P = Q.v2p;
+    const int istop = q == np - 1;
+    if (istop != Q.istop) return LBLX_E_UNSUPPORTED;
+    int npp = 1;
+    if (v1po == 0.0) {
+      v1po = V1P;
+      nlim1 = 1;
+    }
+    if (istop) V2P = V2P + 2. * DVP;
+    const int64_t gq = (int64_t)q * (NLIM1 - 1);
+    for (;;) {
+      v2po = v1po + (double)LIMOUT * dvout;
+      if (V2P <= v2po + DVP && ilast == 0 && npp <= 0) break;
+      if ((v1po + (LIMOUT - 1) * dvout) > v2) {
+        nlim2 = fint((v2 - v1po) / dvout + 1.);
+        v2po = v1po + (double)(nlim2 - 1) * dvout;
+        if (v2po < v2) {
+          v2po = v2po + dvout;
+          nlim2 = nlim2 + 1;
+        }
+        ilast = 1;
+        if (!istop && v2po > V2P - DVP) {
+          nlim2 = fint(((V2P - DVP - v1po) / dvout) + 1.);
+          v2po = v1po + (double)(nlim2 - 1) * dvout;
+          if (v2po + dvout < V2P - DVP) {
+            nlim2 = nlim2 + 1;
+            v2po = v2po + dvout;
+          }
+          ilast = 0;
+        }
+      } else {
+        nlim2 = LIMOUT;
+        v2po = v1po + (double)(nlim2 - 1) * dvout;
+        if (v2po > V2P - DVP) {
+          nlim2 = fint(((V2P - DVP - v1po) / dvout) + 1.);
+          v2po = v1po + (double)(nlim2 - 1) * dvout;
+          if (v2po + dvout < V2P - DVP) {
+            nlim2 = nlim2 + 1;
+            v2po = v2po + dvout;
+          }
+        }
+        ilast = 0;
+      }
+      if (nlim2 > LIMOUT || nlim2 < 0) return LBLX_E_UNSUPPORTED;
+      if (nlim1 <= nlim2) {
+        DvoSeg S;
+        S.obase = emitted;
+        S.o0 = emitted + nlim1 - 1;
+        S.gq = gq;
+        S.fj1 = (v1po - V1P) / DVP + 1. + 2.;
+        if (!D->segs.empty() && D->segs.back().o0 >= S.o0) return LBLX_E_UNSUPPORTED;
+        D->segs.push_back(S);
+      }
+      if (nlim2 == LIMOUT || ilast == 1) {
+        emitted += nlim2;
+        nlim1 = 1;
+        npp = 0;
+        v1po = v2po + dvout;
+        if ((v1po + (double)LIMOUT * dvout) > (V2P - DVP)) npp = 1;
+      } else {
+        nlim1 = nlim2 + 1;
+        npp = -1;
+      }
+      if (ilast != 1) continue;
+      v1po = 0.0;
+      break;
+    }
+    if (ilast == 1 && !istop) return LBLX_E_UNSUPPORTED;  // output ended before the layer
+  }
+  if (D->segs.empty()) return LBLX_E_UNSUPPORTED;
+  A.n_out = emitted;
+  A.nseg = (int)D->segs.size();
+  return LBLX_OK;
+}
+
+int lblx_od_dvout_size(const lblx_run *run, const lblx_layer *layer, double dvout, int64_t *n) {
+  if (!run || !layer || !n) return LBLX_E_ARG;
+  LayerPlan P;
+  lblx_layer t = *layer;
+  int rc = plan_layer(run, &t, &P);
+  if (rc) return rc;
+  DvoPlan D;
+  if ((rc = dvout_plan(P, layer->v2, dvout, &D))) return rc;
+  *n = D.A.n_out;
+  return LBLX_OK;
+}
+
+int lblx_od_layer_dvout_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layer, double dvout,
+                               double *d_out, int64_t cap, int64_t *n_out, double *v1_out) {
+  if (!c || !run || !layer || !d_out) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  LayerPlan P;
+  lblx_layer t = *layer;
+  int rc = plan_layer(run, &t, &P);
+  if (rc) return rc;
+  DvoPlan D;
+  if ((rc = dvout_plan(P, layer->v2, dvout, &D))) return rc;
+  if (cap < D.A.n_out) return LBLX_E_SIZE;
+  const bool f32 = run->precision == LBLX_FP32_SPLIT;
+  const size_t esz = f32 ? 4 : 8;
+  if (c->w_od.ensure((size_t)P.npts * esz) || c->w_dvo.ensure(std::max<size_t>(D.segs.size(), 1) * sizeof(DvoSeg)))
+    return LBLX_E_HIP;
+  int64_t off = 0;
+  rc = run_jobs(c, run, layer, 1, false, c->w_od.p, &off, f32, nullptr, nullptr, &P);
+  if (rc) return rc;
+  if (!D.segs.empty())
+    HIPCHK(hipMemcpyAsync(c->w_dvo.p, D.segs.data(), D.segs.size() * sizeof(DvoSeg), hipMemcpyHostToDevice,
+                          c->stream));
+  const unsigned nb = (unsigned)((D.A.n_out + 255) / 256);
+  if (f32)
+    hipLaunchKernelGGL(k_dvout<float>, dim3(nb), dim3(256), 0, c->stream, D.A, c->w_dvo.as<DvoSeg>(),
+                       c->w_od.as<float>(), d_out);
+  else
+    hipLaunchKernelGGL(k_dvout<double>, dim3(nb), dim3(256), 0, c->stream, D.A, c->w_dvo.as<DvoSeg>(),
+                       c->w_od.as<double>(), d_out);
+  HIPCHK(hipGetLastError());
+  if (n_out) *n_out = D.A.n_out;
+  if (v1_out) *v1_out = D.v1out;
   return LBLX_OK;
 }
 

@@ -1725,4 +1725,55 @@ __global__ __launch_bounds__(256) void k_absmrg(AbsArgs A, const IN *od, const d
   out[g] = v;
 }
 
+
+// ---------------------------------------------------------------------
+// PNLINT (oprop.f90:1564-1806): the layer OD regridded onto DVOUT.  The
+// reference interpolates each DV panel as PANEL emits it; the host replays
+// that control flow and records, per run of output points computed from
+// one input panel, FJ1DIF = (V1PO - V1P)/DVP + 1 + 2 and the input panel's
+// first global point.  One thread per output point: FJJ = FJ1DIF + RATDV
+// (II - 1), JJ = INT(FJJ) - 2, JP = INT((FJJ - JJ) 100 - 199.5), value =
+// 4-point Lagrange with weights tabulated at phase JP/100; taps before the
+// first point and past the last repeat the end points (PNLINT's pads).
+struct DvoSeg {
+  int64_t o0;     // global output index of this run's first point
+  int64_t obase;  // global output index of the output panel's II = 1
+  int64_t gq;     // global index of the input panel's first point
+  double fj1;     // FJ1DIF
+};
+struct DvoArgs {
+  int64_t n_out, n_in;
+  int32_t nseg, pass;
+  double ratdv;
+  double a1[101], a2[101], a3[101], a4[101];
+};
+
+template <typename IN>
+__global__ __launch_bounds__(256) void k_dvout(DvoArgs A, const DvoSeg *segs, const IN *od, double *out) {
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= A.n_out) return;
+  if (A.pass) {  // DVOUT = DV: the panels themselves
+    out[o] = (double)od[o];
+    return;
+  }
+  int lo = 0, hi = A.nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].o0 <= o) lo = mid;
+    else hi = mid - 1;
+  }
+  const DvoSeg S = segs[lo];
+  const int64_t ii = o - S.obase + 1;
+  const double fjj = S.fj1 + A.ratdv * (double)(ii - 1);
+  const int jj = fint(fjj) - 2;
+  const int jp = fint((fjj - (double)jj) * 100. - 199.5);
+  double t[4];
+  for (int k = 0; k < 4; k++) {
+    int64_t g = S.gq + jj - 2 + k;
+    g = g < 0 ? 0 : (g > A.n_in - 1 ? A.n_in - 1 : g);
+    t[k] = (double)od[g];
+  }
+  out[o] = A.a1[jp] * t[0] + A.a2[jp] * t[1] + A.a3[jp] * t[2] + A.a4[jp] * t[3];
+}
+
 }  // namespace lblx

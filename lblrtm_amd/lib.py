@@ -81,6 +81,9 @@ def load():
         "lblx_autotune": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int)]),
         "lblx_set_tiles": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lblx_voigt_tables": (C.c_int, [P(C.c_double), P(C.c_double)]),
+        "lblx_od_dvout_size": (C.c_int, [P(abi.Run), P(abi.Layer), C.c_double, P(C.c_int64)]),
+        "lblx_od_layer_dvout_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_double, C.c_void_p,
+                                                 C.c_int64, P(C.c_int64), P(C.c_double)]),
         "lblx_absmrg_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer), C.c_void_p,
                                          C.c_void_p, C.c_void_p]),
         "lblx_column_od_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, C.c_void_p,
@@ -152,6 +155,12 @@ def voigt_tables():
     c = np.zeros(6 * 101)
     _chk(load().lblx_voigt_tables(abi.ptr(f, C.c_double), abi.ptr(c, C.c_double)), "lblx_voigt_tables")
     return f.reshape(3, 101, 2601), c.reshape(6, 101)
+
+
+def od_dvout_size(run, layer, dvout):
+    n = C.c_int64()
+    _chk(load().lblx_od_dvout_size(C.byref(run), C.byref(layer), dvout, C.byref(n)), "lblx_od_dvout_size")
+    return n.value
 
 
 def scan_size(scan, v1_in, dv_in, n_in):
@@ -310,6 +319,20 @@ class Context:
         finally:
             rad.free()
             jac.free()
+
+    def od_layer_dvout(self, run, layer, dvout):
+        """OPDPTH + PNLINT onto DVOUT: (od, v1) on the host."""
+        n = od_dvout_size(run, layer, dvout)
+        buf = self.alloc(8 * n)
+        no, v1 = C.c_int64(), C.c_double()
+        try:
+            _chk(self._lib.lblx_od_layer_dvout_device(self._h, C.byref(run), C.byref(layer), dvout,
+                                                      C.c_void_p(buf.ptr), n, C.byref(no), C.byref(v1)),
+                 "lblx_od_layer_dvout_device")
+            out = buf.to_host(np.float64, no.value)
+        finally:
+            buf.free()
+        return out, v1.value
 
     def absmrg_device(self, run, layer_old, layer, d_od, d_acc_old, d_acc_new):
         """One IEMIT = 0 merge step (ABINIT when layer_old is None)."""

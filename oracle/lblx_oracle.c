@@ -26,6 +26,7 @@
  *   rdlin, lncor1  RDLIN, LNCOR1             src/oprop.f90:678-1197
  *   cnvfnv         CNVFNV                    src/oprop.f90:1200-1370
  *   panel          PANEL                     src/oprop.f90:1371-1561
+ *   pnlint         PNLINT (DVOUT regrid)     src/oprop.f90:1564-1806
  *   xint           XINT                      src/oprop.f90:2201-2234
  *   oracle_od_layer OPDPTH + HIRAC1          src/lblrtm.f90:6135-6285,
  *                                            src/oprop.f90:25-668
@@ -1354,6 +1355,153 @@ int oracle_panels(double *v1p, double *v2p, int *nlim, int cap)
   return g_pan_n;
 }
 
+/* PNLINT (src/oprop.f90:1564-1806): each panel R1(1..NLIM) of the layer
+ * (Rp[1..NLIM], with the buffer's earlier points at Rp[0], Rp[-1] and the
+ * next points at Rp[NLIM+1], Rp[NLIM+2]) is interpolated onto DVOUT with
+ * 4-point Lagrange weights tabulated at 101 phases and written in output
+ * panels of LIMOUT = 2400 points.  State across panels as the reference
+ * keeps it (V1PO, NLIM1 saved; NLIM2, V2PO in COMMON /XPANO/). */
+static double g_dvout = 0.;
+static double *g_pn_out = NULL;              /* output values */
+static int64_t g_pn_n = 0, g_pn_cap = 0;
+static double *g_pn_v1 = NULL, *g_pn_v2 = NULL;
+static int *g_pn_nlim = NULL;
+static int g_pn_np = 0, g_pn_pcap = 0;
+static double pn_V1PO = 0., pn_V1PO_SAVE = 0., pn_V2PO = 0.;
+static int pn_NLIM1 = 1, pn_NLIM2 = 0, pn_ILAST = 0;
+static double pn_R1OUT[2410];
+
+static void pnlint(double *Rp, int NLIM, double V1P, double V2P, double DVP, int ISTOP, double V2)
+{
+  const int LIMOUT = 2400;
+  double DVOUT = g_dvout;
+  if (pn_V1PO_SAVE != 0.0) pn_V1PO = pn_V1PO_SAVE;
+  int NPPANL = 1;
+  double ATYPE = 9.999E09;
+  if (DVP == DVOUT) ATYPE = 0.;
+  if (DVOUT > DVP) ATYPE = DVP / (DVOUT - DVP) + 0.5;
+  if (DVOUT < DVP) ATYPE = -DVOUT / (DVP - DVOUT) - 0.5;
+  if (fabs(DVOUT - DVP) < 1.E-8) ATYPE = 0.;
+  if (ATYPE == 0.) { /* the panel itself */
+    if (g_pn_np == g_pn_pcap) {
+      g_pn_pcap = g_pn_pcap ? 2 * g_pn_pcap : 1024;
+      g_pn_v1 = realloc(g_pn_v1, g_pn_pcap * sizeof(double));
+      g_pn_v2 = realloc(g_pn_v2, g_pn_pcap * sizeof(double));
+      g_pn_nlim = realloc(g_pn_nlim, g_pn_pcap * sizeof(int));
+    }
+    g_pn_v1[g_pn_np] = V1P; g_pn_v2[g_pn_np] = V2P; g_pn_nlim[g_pn_np] = NLIM; g_pn_np++;
+    for (int k = 1; k <= NLIM; k++) if (g_pn_n < g_pn_cap) g_pn_out[g_pn_n++] = Rp[k];
+    pn_V1PO_SAVE = pn_V1PO;
+    return;
+  }
+  double A1[101], A2[101], A3[101], A4[101];
+  for (int JP = 0; JP <= 100; JP++) {
+    double APG = JP;
+    double P = 0.01 * APG;
+    A1[JP] = -P * (P - 1.0) * (P - 2.0) / 6.0;
+    A2[JP] = (P * P - 1.0) * (P - 2.0) * 0.5;
+    A3[JP] = -P * (P + 1.0) * (P - 2.0) * 0.5;
+    A4[JP] = P * (P * P - 1.0) / 6.0;
+  }
+  if (pn_V1PO == 0.0) {
+    Rp[-1] = Rp[1];
+    Rp[0] = Rp[1];
+    pn_V1PO = V1P;
+    pn_NLIM1 = 1;
+  }
+  if (ISTOP == 1) {
+    Rp[NLIM + 1] = Rp[NLIM];
+    Rp[NLIM + 2] = Rp[NLIM];
+    NLIM = NLIM + 2;
+    V2P = V2P + 2. * DVP;
+  }
+  for (;;) { /* label 20 */
+    pn_V2PO = pn_V1PO + (double)LIMOUT * DVOUT;
+    if (V2P <= pn_V2PO + DVP && pn_ILAST == 0 && NPPANL <= 0) break;
+    if ((pn_V1PO + (LIMOUT - 1) * DVOUT) > V2) {
+      pn_NLIM2 = fint((V2 - pn_V1PO) / DVOUT + 1.);
+      pn_V2PO = pn_V1PO + (double)(pn_NLIM2 - 1) * DVOUT;
+      if (pn_V2PO < V2) { pn_V2PO = pn_V2PO + DVOUT; pn_NLIM2 = pn_NLIM2 + 1; }
+      pn_ILAST = 1;
+      if (ISTOP != 1 && pn_V2PO > V2P - DVP) {
+        pn_NLIM2 = fint(((V2P - DVP - pn_V1PO) / DVOUT) + 1.);
+        pn_V2PO = pn_V1PO + (double)(pn_NLIM2 - 1) * DVOUT;
+        if (pn_V2PO + DVOUT < V2P - DVP) { pn_NLIM2 = pn_NLIM2 + 1; pn_V2PO = pn_V2PO + DVOUT; }
+        pn_ILAST = 0;
+      }
+    } else {
+      pn_NLIM2 = LIMOUT;
+      pn_V2PO = pn_V1PO + (double)(pn_NLIM2 - 1) * DVOUT;
+      if (pn_V2PO > V2P - DVP) {
+        pn_NLIM2 = fint(((V2P - DVP - pn_V1PO) / DVOUT) + 1.);
+        pn_V2PO = pn_V1PO + (double)(pn_NLIM2 - 1) * DVOUT;
+        if (pn_V2PO + DVOUT < V2P - DVP) { pn_NLIM2 = pn_NLIM2 + 1; pn_V2PO = pn_V2PO + DVOUT; }
+      }
+      pn_ILAST = 0;
+    }
+    double RATDV = DVOUT / DVP;
+    double FJ1DIF = (pn_V1PO - V1P) / DVP + 1. + 2.;
+    for (int II = pn_NLIM1; II <= pn_NLIM2; II++) {
+      double FJJ = FJ1DIF + RATDV * (double)(II - 1);
+      int JJ = fint(FJJ) - 2;
+      int JP = fint((FJJ - (double)JJ) * 100. - 199.5);
+      pn_R1OUT[II] = A1[JP] * Rp[JJ - 1] + A2[JP] * Rp[JJ] + A3[JP] * Rp[JJ + 1] + A4[JP] * Rp[JJ + 2];
+    }
+    if (pn_NLIM2 == LIMOUT || pn_ILAST == 1) {
+      if (g_pn_np == g_pn_pcap) {
+        g_pn_pcap = g_pn_pcap ? 2 * g_pn_pcap : 1024;
+        g_pn_v1 = realloc(g_pn_v1, g_pn_pcap * sizeof(double));
+        g_pn_v2 = realloc(g_pn_v2, g_pn_pcap * sizeof(double));
+        g_pn_nlim = realloc(g_pn_nlim, g_pn_pcap * sizeof(int));
+      }
+      g_pn_v1[g_pn_np] = pn_V1PO; g_pn_v2[g_pn_np] = pn_V2PO; g_pn_nlim[g_pn_np] = pn_NLIM2; g_pn_np++;
+      for (int k = 1; k <= pn_NLIM2; k++) if (g_pn_n < g_pn_cap) g_pn_out[g_pn_n++] = pn_R1OUT[k];
+      pn_NLIM1 = 1;
+      NPPANL = 0;
+      pn_V1PO = pn_V2PO + DVOUT;
+      if ((pn_V1PO + (double)LIMOUT * DVOUT) > (V2P - DVP)) NPPANL = 1;
+    } else {
+      pn_NLIM1 = pn_NLIM2 + 1;
+      NPPANL = -1;
+    }
+    if (pn_ILAST != 1) continue;
+    pn_V1PO = 0.0;
+    NPPANL = 1;
+    break;
+  }
+  pn_V1PO_SAVE = pn_V1PO;
+}
+
+/* OPDPTH with DVOUT (IOD = 1/4): the layer's panels through PNLINT.  out
+ * receives up to cap values; returns the number of output panels, their
+ * headers in v1o/v2o/nlimo (up to pcap). */
+int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
+                    const double *absrb, double v1abs, double v2abs, double dvabs, int nptabs,
+                    double *od, int64_t n_out, int64_t *npts_out, double *v1p_out,
+                    lblx_stats *st);
+int oracle_od_layer_dvout(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L, double dvout,
+                          double *out, int64_t cap, int64_t *n_written, double *v1o, double *v2o,
+                          int *nlimo, int pcap)
+{
+  g_dvout = dvout;
+  g_pn_out = out;
+  g_pn_cap = cap;
+  g_pn_n = 0;
+  g_pn_np = 0;
+  pn_V1PO = pn_V1PO_SAVE = pn_V2PO = 0.;
+  pn_NLIM1 = 1;
+  pn_NLIM2 = 0;
+  pn_ILAST = 0;
+  int64_t np = 0;
+  double v1p = 0.;
+  int rc = oracle_od_layer(run, Ly, L, NULL, 0., 0., 0., 0, NULL, 0, &np, &v1p, NULL);
+  g_dvout = 0.;
+  if (rc) return rc;
+  *n_written = g_pn_n;
+  for (int k = 0; k < g_pn_np && k < pcap; k++) { v1o[k] = g_pn_v1[k]; v2o[k] = g_pn_v2[k]; nlimo[k] = g_pn_nlim[k]; }
+  return g_pn_np;
+}
+
 int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
                     const double *absrb, double v1abs, double v2abs, double dvabs, int nptabs,
                     double *od, int64_t n_out, int64_t *npts_out, double *v1p_out,
@@ -1767,6 +1915,7 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
       for (int k = 0; k < NLIM; k++) {
         if (od && nout + k < n_out) od[nout + k] = R1[NLO + k];
       }
+      if (g_dvout != 0.) pnlint(&R1[NLO - 1], NLIM, V1P, V2P, DVP, ISTOP, V2);
       nout += NLIM;
       stats.npanel++;
       VFT = VFT + (double)(NLIM1 - 1) * DV;

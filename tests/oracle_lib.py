@@ -228,3 +228,22 @@ def absmrg(old_pan, old_od, dvo, pan, layer_od, dvk):
     if rc < 0:
         raise RuntimeError("oracle_absmrg rc=%d" % rc)
     return out
+
+
+def od_layer_dvout(run, layer, table, dvout, cap=20000000):
+    """OPDPTH with PNLINT onto DVOUT: (values, (v1o, v2o, nlimo))."""
+    out = np.zeros(cap)
+    pc = 20000
+    v1o, v2o = np.zeros(pc), np.zeros(pc)
+    nl = np.zeros(pc, np.int32)
+    nw = C.c_int64()
+    L = lib()
+    L.oracle_od_layer_dvout.restype = C.c_int
+    lay = abi.Layer.from_buffer_copy(layer)
+    lines = table.struct()
+    rc = L.oracle_od_layer_dvout(C.byref(run), C.byref(lay), C.byref(lines), C.c_double(dvout),
+                                 abi.ptr(out, C.c_double), C.c_int64(cap), C.byref(nw), abi.ptr(v1o, C.c_double),
+                                 abi.ptr(v2o, C.c_double), abi.ptr(nl, C.c_int), pc)
+    if rc < 0:
+        raise RuntimeError("oracle_od_layer_dvout rc=%d" % rc)
+    return out[:nw.value], (v1o[:rc], v2o[:rc], nl[:rc])
