@@ -167,6 +167,7 @@ struct lblx_ctx {
   DevBuf w_cx, w_csp, w_cal, w_cad, w_cco2, w_pn, w_pc, w_g, w_jmp, w_wsum, w_bcnt, w_etab, w_jobi,
       w_bat;
   DevBuf w_t9vnu, w_t9sp, w_t9alfa, w_t9epp, w_t9mol, w_t9n, w_t9blk, w_t9bvhi;
+  DevBuf w_dirty;  // blocks whose read panel moved in the last fixpoint step
   DevBuf w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
   DevBuf w_tmap[3];
   DevBuf w_mpan, w_rt;
@@ -701,6 +702,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_t9blk, NJ * S.t9blk * 16) || need(c->w_t9bvhi, NJ * S.t9blk * 8) ||
       need(c->w_near, NJ * S.line * sizeof(NearLine)) || need(c->w_keep, NJ * S.line * 4) ||
       need(c->w_blkmax, NJ * S.blk * 4) || need(c->w_pread, NJ * S.blk * 4) ||
+      need(c->w_dirty, NJ * S.blk * 4) ||
       need(c->w_changed, 16) || need(c->w_rec, NJ * S.line * sizeof(ScatterRec)) ||
       need(c->w_r3, (size_t)r3tot * 8) || need(c->w_r2, (size_t)r2tot * 8) ||
       need(c->w_cnt, NJ * 24) || need(c->w_margin, NJ * 3 * 8) ||
@@ -818,14 +820,17 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
                        c->w_pread.as<int>(), NJ * S.blk, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s,
                        c->w_blkmax.as<int>(), NJ * S.blk, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s,
+                       c->w_dirty.as<int>(), NJ * S.blk, 1);
     bool converged = false;
     for (int it = 0; it < 64; it++) {
       hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, c->w_changed.as<int>(), 1, 0);
       hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY,
                          S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                         c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
+                         c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>(),
+                         c->w_dirty.as<int>());
       hipLaunchKernelGGL(k_pread_scan, dim3(njobs), dim3(256), 0, s, dY, S, c->w_blkmax.as<int>(),
-                         c->w_pread.as<int>(), c->w_changed.as<int>());
+                         c->w_pread.as<int>(), c->w_changed.as<int>(), c->w_dirty.as<int>());
       if (it >= 1) {
         int changed = 1;
         HIPCHK(hipMemcpyAsync(&changed, c->w_changed.p, 4, hipMemcpyDeviceToHost, s));
@@ -840,7 +845,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     if (!converged)
       hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY, S,
                          c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                         c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
+                         c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>(),
+                         c->w_dirty.as<int>());
     c->tend(th);
     // 6. CNVFNV scatter records
     th = c->tbegin(ST_SCATTER);
@@ -1827,7 +1833,7 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
 static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
   const size_t line = ((size_t)nline + 127) / 64 * 64;
   return line * (32 + 1 + 12 + 4 * NLEV + 2 * NLEV + 12 + 36 + sizeof(NearLine) + 8 + sizeof(ScatterRec)) +
-         (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 2) + (size_t)(c->nlines / 1000 + 16) * 24;
+         (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 3) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
 
 // Many profiles in one call: OPDPTH for the layers of a group of profiles

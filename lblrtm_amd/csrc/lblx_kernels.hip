@@ -870,40 +870,58 @@ __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPan
 // One step of the block-read fixpoint: with the current guess of the panel
 // at which each block is read, apply LNCOR1's VFT test and R4 rejection
 // (oprop.f90:1120, 1153-1177) and reduce the first-fit panel of the kept
-// lines per block.
-__global__ void k_reject(const NearLine *nl, DevLines L, const DevLayer *Ys, Strides S,
-                         const int *pread, const DevPanel *pans, const DevWin *wins,
-                         const double *r4, int *keep_out, int *blkmax) {
+// lines per block.  Only blocks whose read panel changed in the last step
+// (dirty) are redone; the others keep their flags and block maxima.  The
+// per-block maximum is reduced across the wave before the atomic.
+__global__ __launch_bounds__(256) void k_reject(const NearLine *nl, DevLines L, const DevLayer *Ys, Strides S,
+                                                const int *pread, const DevPanel *pans, const DevWin *wins,
+                                                const double *r4, int *keep_out, int *blkmax, const int *dirty) {
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= Y.nn) return;
-  NearLine o = nl[job * S.line + t];
-  int keep = o.keep;
-  int k = L.blk[Y.nlo + t] - Y.blk0;
-  if (keep) {
-    int pr = pread[job * S.blk + k];
-    int pq = pr < Y.npanel ? pr : Y.npanel - 1;
-    const DevPanel &pp = pans[Y.pan0 + pq];
-    double alfv = 1. / o.recalf;
-    if (64. * alfv + o.vnu < pp.vft) keep = 0;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = t < Y.nn;
+  const int k = in ? L.blk[Y.nlo + t] - Y.blk0 : -1;
+  const bool act = in && dirty[job * S.blk + k] != 0;
+  int val = -1;
+  if (act) {
+    const NearLine &o = nl[job * S.line + t];
+    int keep = o.keep;
     if (keep) {
-      int w = Y.win0 + pp.w;
-      DevWin W = wins[w];
-      int jj = fint((o.vnu - W.v1r4) / Y.dvr4 + 1.);
-      jj = jj < 1 ? 1 : (jj > W.nptr4 ? W.nptr4 : jj);
-      double r = r4[(size_t)w * R4ROW + jj];
-      if (o.speak <= (Y.dptmn_hirac + Y.dptfc * r)) keep = 0;
+      int pr = pread[job * S.blk + k];
+      int pq = pr < Y.npanel ? pr : Y.npanel - 1;
+      const DevPanel &pp = pans[Y.pan0 + pq];
+      double alfv = 1. / o.recalf;
+      if (64. * alfv + o.vnu < pp.vft) keep = 0;
+      if (keep) {
+        int w = Y.win0 + pp.w;
+        DevWin W = wins[w];
+        int jj = fint((o.vnu - W.v1r4) / Y.dvr4 + 1.);
+        jj = jj < 1 ? 1 : (jj > W.nptr4 ? W.nptr4 : jj);
+        double r = r4[(size_t)w * R4ROW + jj];
+        if (o.speak <= (Y.dptmn_hirac + Y.dptfc * r)) keep = 0;
+      }
     }
+    keep_out[job * S.line + t] = keep;
+    if (keep) val = o.ff;
   }
-  keep_out[job * S.line + t] = keep;
-  if (keep) atomicMax(&blkmax[job * S.blk + k], o.ff);
+  const int lane = threadIdx.x & 63;
+  uint64_t pend = __ballot(val >= 0);
+  while (pend) {
+    const int leader = __builtin_ctzll(pend);
+    const int kl = __builtin_amdgcn_readlane(k, leader);
+    const bool mine = val >= 0 && k == kl;
+    int m = mine ? val : -1;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    if (lane == leader) atomicMax(&blkmax[job * S.blk + kl], m);
+    pend &= ~__ballot(mine);
+  }
 }
 
 // Exclusive prefix max over blocks: panel at which each block is read.
-// Raises *changed when the guess moved; resets blkmax for the next pass.
+// Raises *changed when the guess moved and marks the blocks whose read
+// panel moved dirty (their block maximum is reset for the next pass).
 __global__ __launch_bounds__(256) void k_pread_scan(const DevLayer *Ys, Strides S, int *blkmax,
-                                                    int *pread, int *changed) {
+                                                    int *pread, int *changed, int *dirty) {
   __shared__ int buf[256];
   __shared__ int carry;
   const int job = blockIdx.x;
@@ -924,10 +942,15 @@ __global__ __launch_bounds__(256) void k_pread_scan(const DevLayer *Ys, Strides 
       __syncthreads();
     }
     int excl = max(carry, tid > 0 ? buf[tid - 1] : 0);
+    __syncthreads();
     if (k < nbk) {
-      if (pread[ob + k] != excl) *changed = 1;
+      const int moved = pread[ob + k] != excl;
+      if (moved) {
+        *changed = 1;
+        blkmax[ob + k] = 0;
+      }
+      dirty[ob + k] = moved;
       pread[ob + k] = excl;
-      blkmax[ob + k] = 0;
     }
     __syncthreads();
     if (tid == 0) carry = max(carry, buf[255]);
@@ -967,6 +990,7 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const i
     __syncthreads();
   }
   if (!valid) return;
+  int P = max(pread[job * S.blk + k], buf[tid]);
   ScatterRec r;
   r.npts = 0;
   r.izm = 1;
@@ -974,7 +998,6 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const i
   r.s2 = r.s3 = 0;
   r.zfl1 = r.zfl2 = r.zfl3 = r.zslope = 0.;
   r.strf1 = r.strf2 = r.strf3 = r.strd = r.strver = 0.;
-  int P = max(pread[job * S.blk + k], buf[tid]);
   if (kp && P < Y.npanel) {
     double vft = pans[Y.pan0 + P].vft;
     double depthi = n.sp * n.recalf;
