@@ -174,7 +174,7 @@ struct lblx_ctx {
   DevBuf s_z[2], s_tw, s_g, s_ip, s_cof;  // FFTSCN
   DevBuf j_rall, j_a, j_b;                // Jacobians
   int64_t s_twN = 0;
-  int tiles[3] = {2048, 1024, 512};       // R1, R2, R3 tile widths (autotuned)
+  int tiles[3] = {8192, 4096, 2048};      // R1, R2, R3 tile widths (autotuned)
   DevBuf b_jobs, b_rt;                    // batched merges
   DevBuf w_dvo;                           // PNLINT runs
   struct GraphEntry {
@@ -881,7 +881,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   do {                                                                                             \
     HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT, VV, LACCT>,                    \
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));             \
-    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT, VV, LACCT>), dim3(nt), dim3(256), shm, s, rec,      \
+    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT, VV, LACCT>), dim3(nt), dim3(TILE_T), shm, s, rec,   \
                        tm[GG - 1], T, tabs, (const ACCT *)(VTAB), dY, S, dp, dw, r4,               \
                        (const double *)(ABS), dg, (const double *)(RIN), (double *)(ROUT),         \
                        (OUTT *)(ODP));                                                            \
@@ -2126,7 +2126,7 @@ int lblx_autotune(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay
     }
     return best;
   };
-  const int cand[3][5] = {{512, 1024, 2048, 4096, 8192}, {256, 512, 1024, 2048, 4096}, {128, 256, 512, 1024, 2048}};
+  const int cand[3][4] = {{1024, 2048, 4096, 8192}, {1024, 2048, 4096, 8192}, {512, 1024, 2048, 4096}};
   const int stage[3] = {ST_TILE1, ST_TILE2, ST_TILE3};
   for (int G = 0; G < 3; G++) {
     int bestw = c->tiles[G];

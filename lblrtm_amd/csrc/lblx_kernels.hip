@@ -1151,6 +1151,13 @@ struct TileLine {
   }
 };
 
+// Threads per tile workgroup: the tables and the accumulator in LDS are
+// shared by TILE_T / 64 waves.
+#ifndef LBLX_TILE_T
+#define LBLX_TILE_T 1024
+#endif
+constexpr int TILE_T = LBLX_TILE_T;
+
 // Near-line scatter into one tile of grid G (3: DVR3, 2: DVR2, 1: DV) with
 // the PANEL combine in the epilogue.
 //   G = 3: R3 = lines(F3) + XINT(R4) + XINT(ABSRB)            -> r3
@@ -1163,7 +1170,7 @@ struct TileLine {
 //   LACC: type of the LDS accumulator (ACC, or double under FP32 shapes
 //   where LDS float atomics stall).
 template <int G, typename OUT, typename ACC, bool V, typename LACC>
-__global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M, int tile,
+__global__ __launch_bounds__(TILE_T) void k_tile(const ScatterRec *recs, TileMap M, int tile,
                                               const double *tabs, const ACC *vtab, const DevLayer *Ys,
                                               Strides S, const DevPanel *pans, const DevWin *wins,
                                               const double *r4, const double *absrb,
@@ -1178,12 +1185,12 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
   const int job = M.job[blockIdx.x];
   const DevLayer &Y = Ys[job];
   const ScatterRec *rec = recs + job * S.line;
-  for (int k = tid; k < tile; k += 256) acc[k] = (LACC)0.;
+  for (int k = tid; k < tile; k += TILE_T) acc[k] = (LACC)0.;
   if (!V) {
     const double *src = tabs + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NFMX;
-    for (int k = tid; k < NFMX; k += 256) t1[k] = (ACC)src[k];
+    for (int k = tid; k < NFMX; k += TILE_T) t1[k] = (ACC)src[k];
     if (G == 1) {
-      for (int k = tid; k < NFMX; k += 256) {
+      for (int k = tid; k < NFMX; k += TILE_T) {
         tg[k] = (ACC)tabs[3 * NFMX + k];
         tv[k] = (ACC)tabs[4 * NFMX + k];
       }
@@ -1202,8 +1209,8 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
   TileLine<G, V> cur, nxt;
   int c0 = lo + wave * 64;
   if (c0 < hi) cur.load(rec, c0 + lane, hi);
-  for (; c0 < hi; c0 += 256) {
-    const int c1 = c0 + 256;
+  for (; c0 < hi; c0 += TILE_T) {
+    const int c1 = c0 + TILE_T;
     if (c1 < hi) nxt.load(rec, c1 + lane, hi);
     const int nj = hi - c0 < 64 ? hi - c0 : 64;
     for (int j = 0; j < nj; j++) {
@@ -1238,7 +1245,7 @@ __global__ __launch_bounds__(256) void k_tile(const ScatterRec *recs, TileMap M,
   const double X00 = -7. / 128., X01 = 105. / 128., X02 = 35. / 128., X03 = -5. / 128.;
   const double X10 = -1. / 16., X11 = 9. / 16.;
   const DevPanel *pan = pans + Y.pan0;
-  for (int k = tid; k < tile; k += 256) {
+  for (int k = tid; k < tile; k += TILE_T) {
     int64_t g = T0 + k;
     if (G == 3) {
       if (g >= Y.n3) continue;

@@ -65,6 +65,6 @@ def test_autotune_keeps_results(gpu_ctx):
     assert all(t % 64 == 0 for t in tiles)
     r1 = gpu_ctx.spectrum(run, [abi.Layer.from_buffer_copy(x) for x in layers], tz, 295.0, 0.98)[0]
     np.testing.assert_allclose(r1, r0, rtol=1e-12)
-    gpu_ctx.set_tiles((2048, 1024, 512))
+    gpu_ctx.set_tiles((8192, 4096, 2048))
     with pytest.raises(lib.LblxError):
         gpu_ctx.set_tiles((100, 1024, 512))
