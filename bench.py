@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "spectra/s FP32-mixed vs plain FP64 at ≤1e-4 rel; Jacobian/forward time"
 HBM_PEAK_GBS = 8000.0
-REC_BYTES = 120      # ScatterRec
+REC_BYTES = 104      # ScatterRec
 T9_BYTES = 36        # shrunk line: centre, strength, two widths, molecule
 LINE_BYTES = 36      # TAPE3 line read per layer (centre f64, six f32, mol)
 
@@ -53,6 +53,35 @@ def stage_bytes(stage, stats, sizes, nwin):
     if stage == "shrink":
         return LINE_BYTES * 500000 * len(sizes) + T9_BYTES * sum(s["nlins4"] for s in stats)
     return LINE_BYTES * 500000 * len(sizes)
+
+
+# stage -> kernel of the FP64 column (names as rocprofv3 lists them)
+STAGE_KERNEL = {
+    "convf4": "k_convf4<double>",
+    "tile_r1": "k_tile<1, double, double, false, double>",
+    "tile_r2": "k_tile<2, double, double, false, double>",
+    "tile_r3": "k_tile<3, double, double, false, double>",
+    "merge": "k_merge<double>",
+    "scatter_rec": "k_scatter_rec",
+    "lncor1": "k_lncor1",
+}
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_column50.md")
+
+
+def traffic_of(stage):
+    """HBM bytes per dispatch of the stage's kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated), or None."""
+    k = STAGE_KERNEL.get(stage)
+    if k is None or not os.path.exists(TRAFFIC_TABLE):
+        return None
+    for line in open(TRAFFIC_TABLE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].strip("`") == k:
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
+    return None
 
 
 def bt(rad, v):
@@ -262,7 +291,7 @@ def main():
             "parameters": "T, ln q of H2O, CO2, O3 at every layer; OD forward differences, exact merge tangents"},
         "batch": None if batch is None else dict(batch, spectra_ratio_vs_fp64=batch["value"] * ms / 1000.0),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(dom)},
         "cpu_baseline": None,
     }
     if not a.no_cpu:
