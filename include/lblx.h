@@ -221,6 +221,17 @@ int lblx_jacobian_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers,
                          int nspec, double dT, double dlnq, double *d_rad, double *d_jac,
                          int64_t n_out, double *v1p_top);
 
+/* The same outputs in the reference's analytic-Jacobian formulation (AJ
+ * path, src/lblrtm.f90:2930-2975, src/xmerge.f90:5786-6147): the layer
+ * temperature derivative from a +1 K OPDPTH run, and d tau / d ln q of
+ * molecule m from a single-species run (every other amount zero, the O2
+ * amount added to the broadener, CONTNM's ICFLG = m: no N2 continuum and
+ * the H2O continuum's ln q derivative term), whose optical depth is the
+ * derivative itself.  Same layout as lblx_jacobian_device.               */
+int lblx_jacobian_aj_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay,
+                            const double *tzu, double tbnd, double emis, const int *species, int nspec,
+                            double *d_rad, double *d_jac, int64_t n_out, double *v1p_top);
+
 /* A batch of independent profiles (ragged: nlay[p] layers each, surface
  * first, concatenated in layers/tzu; each profile's own band in its
  * layers' v1/v2).  Layer optical depths run in batches of whole profiles

@@ -1663,16 +1663,22 @@ int lblx_fftscn_device(lblx_ctx *c, const lblx_scan *scan, const double *d_in, d
 // layers in one batch with the base column); the radiative part is exact:
 // dR_l/dx on the layer grid, carried to the top through every layer above
 // as J <- T_k L4(J).
-int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
-                         const double *tzu, double tbnd, double emis, const int *species, int nspec,
-                         double dT, double dlnq, double *d_rad, double *d_jac, int64_t n_out,
-                         double *v1p_top) {
+// Jacobians of the TOA radiance.  aj = 0: forward differences of OPDPTH
+// (T + dT, q e^dlnq).  aj = 1: the reference's analytic-Jacobian runs
+// (lblrtm.f90:2930-2975): T + 1 K, and for molecule m a single-species run
+// (every other amount zero, O2's amount moved to the broadener, CONTNM with
+// ICFLG = m) whose optical depth is dtau/dln q itself.  The radiative part
+// (sources and merge tangents) is the same exact tangent in both.
+static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay, const double *tzu,
+                        double tbnd, double emis, const int *species, int nspec, double dT, double dlnq,
+                        int aj, double *d_rad, double *d_jac, int64_t n_out, double *v1p_top) {
   if (!c || !run || !layers || nlay <= 0 || !tzu || !d_rad || !d_jac) return LBLX_E_ARG;
   if (nspec < 0 || nspec > 7 || (nspec > 0 && !species) || !(dT > 0.) || !(dlnq > 0.)) return LBLX_E_ARG;
   HIPCHK(hipSetDevice(c->device));
   const int P = 1 + nspec;
   std::vector<LayerPlan> plans(nlay);
   std::vector<lblx_layer> jobs;
+  std::vector<int> icflg;
   jobs.reserve((size_t)nlay * (1 + P));
   int jr = layers[0].jrad;
   for (int l = 0; l < nlay; l++) {
@@ -1680,22 +1686,42 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     tmp.jrad = jr;
     int rc = plan_layer(run, &tmp, &plans[l]);
     if (rc) return rc;
-    jr = tmp.jrad;
     layers[l].jrad = tmp.jrad;
+    tmp.jrad = jr;  // the job plans again from its input flag
+    jr = layers[l].jrad;
     jobs.push_back(tmp);
+    icflg.push_back(0);
   }
   for (int l = 0; l < nlay; l++)
     for (int q = 0; q < P; q++) {
       lblx_layer t = jobs[l];
-      if (q == 0) t.tave = t.tave + dT;
-      else {
-        int m = species[q - 1];
+      int flg = 0;
+      if (q == 0) {
+        t.tave = t.tave + dT;
+      } else {
+        const int m = species[q - 1];
         if (m < 1 || m > t.nmol) return LBLX_E_ARG;
-        t.wk[m - 1] = t.wk[m - 1] * std::exp(dlnq);
+        if (aj) {
+          const double wsave = t.wk[m - 1], w7 = t.nmol >= 7 ? t.wk[6] : 0.;
+          for (int k = 0; k < t.nmol; k++) t.wk[k] = 0.;
+          t.wk[m - 1] = wsave;
+          t.wbroad = t.wbroad + w7;
+          flg = m;
+        } else {
+          t.wk[m - 1] = t.wk[m - 1] * std::exp(dlnq);
+        }
       }
       jobs.push_back(t);
+      icflg.push_back(flg);
     }
   const int nj = (int)jobs.size();
+  // every job's plan (the species runs with their ICFLG)
+  std::vector<LayerPlan> jplans(nj);
+  for (int j = 0; j < nj; j++) {
+    lblx_layer t = jobs[j];
+    int rc = plan_layer(run, &t, &jplans[j], icflg[j]);
+    if (rc) return rc;
+  }
   std::vector<int64_t> off(nj);
   int64_t tot = 0, nmax = 0, tot_base = 0;
   for (int j = 0; j < nj; j++) {
@@ -1729,7 +1755,8 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
       c->j_a.ensure(jbytes) || c->j_b.ensure(jbytes))
     return LBLX_E_HIP;
   // 1. OPDPTH for the column and every perturbed layer, one batch
-  int rc = run_jobs(c, run, jobs.data(), nj, false, c->w_od.p, off.data(), f32, nullptr, nullptr);
+  int rc = run_jobs(c, run, jobs.data(), nj, false, c->w_od.p, off.data(), f32, nullptr, nullptr,
+                    jplans.data());
   if (rc) return rc;
   hipStream_t s = c->stream;
   HIPCHK(hipMemcpyAsync(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice, s));
@@ -1770,7 +1797,8 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     S.stride = stride;
     S.nparam = P;
     for (int q = 0; q < P; q++) {
-      S.rh[q] = q == 0 ? 1.0 / dT : 1.0 / dlnq;
+      S.direct[q] = (aj && q > 0) ? 1 : 0;
+      S.rh[q] = q == 0 ? 1.0 / dT : (aj ? 1.0 : 1.0 / dlnq);
       S.odp[q] = off[nlay + l * P + q] - off[l];  // relative to the layer's own optical depth
     }
     const double *rold = l > 0 ? rall + off[l - 1] : nullptr;
@@ -1825,6 +1853,21 @@ int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
   HIPCHK(hipGetLastError());
   if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
   return LBLX_OK;
+}
+
+int lblx_jacobian_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
+                         const double *tzu, double tbnd, double emis, const int *species, int nspec,
+                         double dT, double dlnq, double *d_rad, double *d_jac, int64_t n_out,
+                         double *v1p_top) {
+  return jacobian_run(c, run, layers, nlay, tzu, tbnd, emis, species, nspec, dT, dlnq, 0, d_rad, d_jac,
+                      n_out, v1p_top);
+}
+
+int lblx_jacobian_aj_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
+                            const double *tzu, double tbnd, double emis, const int *species, int nspec,
+                            double *d_rad, double *d_jac, int64_t n_out, double *v1p_top) {
+  return jacobian_run(c, run, layers, nlay, tzu, tbnd, emis, species, nspec, 1.0, 1.0, 1, d_rad, d_jac,
+                      n_out, v1p_top);
 }
 
 // Per-job device bytes of run_jobs (line-table-sized work arrays).

@@ -1390,6 +1390,11 @@ __global__ __launch_bounds__(256) void k_contnm(const ContPlan *cps, ContTabs TB
     double cself = C.wk1 * sh * C.xself;
     double cfor = C.wk1 * fh * C.xfrgn;
     ab = ab + cself + cfor;
+    if (C.aj) {  // ln q derivative of the H2O continuum (contnm.f90:314-322)
+      const double h = H.f0;
+      const double cfaj = h >= 1.0 ? 0. : C.wk1 * fh * h / (1.0 - h);
+      ab = ab + (cself - cfaj);
+    }
   }
   if (C.co2.on) {
     const ContTerm &K = C.co2;
@@ -1572,6 +1577,7 @@ struct JacSrc {
   int32_t nparam;
   double rh[8];          // 1 / step of each parameter's optical-depth difference
   int64_t odp[8];        // element offset of each perturbed optical depth
+  int32_t direct[8];     // the perturbed run's optical depth is the derivative itself
 };
 
 template <typename IN>
@@ -1612,7 +1618,8 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
   const double x = vi / A.xkt;
   const double dbdt = (x >= 1e-2 && x <= 80.) ? bb * (x / S.tave) / (1. - exp(-x)) : 0.;
   for (int q = 0; q < S.nparam; q++) {
-    double dtau = ((double)odall[S.odp[q] + g] - (double)od[g]) * S.rh[q];
+    double dtau = S.direct[q] ? (double)odall[S.odp[q] + g] * S.rh[q]
+                              : ((double)odall[S.odp[q] + g] - (double)od[g]) * S.rh[q];
     double v = D * dtau;
     if (q == 0) v = v + (1. - tr) * (1. - f) * dbdt;
     if (scale) v = v * scale[g];

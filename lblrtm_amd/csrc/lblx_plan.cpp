@@ -226,7 +226,7 @@ static void cont_walk(const ContPlan &C, const double *hdr, ContTerm *T) {
 
 // CONTNM scalars for the continua of 0..4340 cm-1 (H2O MT_CKD, CO2,
 // O2 and N2 fundamentals, Rayleigh).
-static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPlan *C) {
+static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPlan *C, int icflg) {
   *C = ContPlan{};
   double X[7] = {1, 1, 1, 1, 1, 1, 1};
   switch (run->icntnm) {  // lblrtm.f90:705-760
@@ -266,7 +266,8 @@ static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPl
   const double wk7 = Ly->nmol >= 7 ? Ly->wk[6] : 0.;
   const double x_vmr_h2o = wk1 / WTOT, x_vmr_o2 = wk7 / WTOT;
   const double x_vmr_n2 = 1. - x_vmr_h2o - x_vmr_o2;
-  const double wn2 = x_vmr_n2 * WTOT;
+  // an analytic-Jacobian species run has no N2 continuum (contnm.f90:226)
+  const double wn2 = icflg > 0 ? 0. : x_vmr_n2 * WTOT;
   // H2O (contnm.f90:296-327, mt_ckd_h2o_module.f90:68-176)
   if (V2 > -20.0 && V1 < 20000. && (xself > 0. || xfrgn > 0.)) {
     ContTerm &T = C->h2o;
@@ -293,6 +294,7 @@ static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPl
     C->xfrgn = xfrgn;
     C->wk1 = wk1;
     C->forcl = run->ifrgnx == 1;
+    C->aj = icflg == 1 ? 1 : 0;
   }
   // CO2 (contnm.f90:330-353)
   if (V2 > -20.0 && V1 < 10000. && xco2c > 0.) {
@@ -328,7 +330,7 @@ static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPl
   return LBLX_OK;
 }
 
-int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P) {
+int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   if (!run || !Ly || !P) return LBLX_E_ARG;
   if (Ly->nmol < 1 || Ly->nmol > 7) return LBLX_E_UNSUPPORTED;
   if (run->ilblf4 < 1) return LBLX_E_UNSUPPORTED;
@@ -347,7 +349,7 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P) {
   P->tave = Ly->tave;
   P->pave = Ly->pave;
   if (run->icntnm != 0) {
-    int rc = plan_cont(run, Ly, P->jrad, &P->cont);
+    int rc = plan_cont(run, Ly, P->jrad, &P->cont, icflg);
     if (rc) return rc;
   }
   int ipflag = (run->icntnm != 0 && Ly->pave <= 0.5) ? 1 : 0;

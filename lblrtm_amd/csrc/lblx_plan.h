@@ -67,6 +67,7 @@ struct ContPlan {
   double ray_f;      // WTOT * conv_cm2mol
   double xself, xfrgn, wk1;
   int32_t forcl;     // MT_CKD foreign closure table
+  int32_t aj;        // analytic-Jacobian species run: H2O derivative term added
 };
 
 struct LayerPlan {
@@ -124,7 +125,10 @@ int plan_scan(const lblx_scan *scan, double v1c, double dv, int64_t n_in, ScanPl
 
 // Build the plan; layer->jrad is updated as OPDPTH does.  Returns LBLX_OK
 // or an LBLX_E_* code.
-int plan_layer(const lblx_run *run, lblx_layer *layer, LayerPlan *P);
+// icflg > 0: the layer is an analytic-Jacobian single-species run (CONTNM's
+// /CDERIV/ ICFLG, contnm.f90:221-239, 314-322): no N2 continuum, and the
+// H2O continuum carries its ln q derivative term.
+int plan_layer(const lblx_run *run, lblx_layer *layer, LayerPlan *P, int icflg = 0);
 
 // PATH DV selection for a column (lblrtm.f90:5336-5500).
 int plan_dv(const lblx_run *run, int nlay, int nmol, const double *pave,

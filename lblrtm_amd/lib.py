@@ -81,6 +81,9 @@ def load():
         "lblx_autotune": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int)]),
         "lblx_set_tiles": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lblx_voigt_tables": (C.c_int, [P(C.c_double), P(C.c_double)]),
+        "lblx_jacobian_aj_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_double),
+                                              C.c_double, C.c_double, P(C.c_int), C.c_int, C.c_void_p,
+                                              C.c_void_p, C.c_int64, P(C.c_double)]),
         "lblx_od_dvout_size": (C.c_int, [P(abi.Run), P(abi.Layer), C.c_double, P(C.c_int64)]),
         "lblx_od_layer_dvout_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_double, C.c_void_p,
                                                  C.c_int64, P(C.c_int64), P(C.c_double)]),
@@ -306,14 +309,32 @@ class Context:
             layers[k].jrad = arr[k].jrad
         return v1p.value
 
-    def jacobian(self, run, layers, tzu, tbnd, emis, species, dT=1e-3, dlnq=1e-4):
-        """Host-side convenience: (radiance, jac[q, layer, point], v1p, dv)."""
+    def jacobian_aj_device(self, run, layers, tzu, tbnd, emis, species, d_rad, d_jac, n_out):
+        """The reference's analytic-Jacobian formulation (lblx_jacobian_aj_device)."""
+        n = len(layers)
+        arr = (abi.Layer * n)(*layers)
+        tz = np.ascontiguousarray(tzu, np.float64)
+        sp = (C.c_int * max(1, len(species)))(*species)
+        v1p = C.c_double(0.0)
+        _chk(self._lib.lblx_jacobian_aj_device(self._h, C.byref(run), arr, n, abi.ptr(tz, C.c_double), tbnd,
+                                               emis, sp, len(species), C.c_void_p(d_rad), C.c_void_p(d_jac),
+                                               n_out, C.byref(v1p)), "lblx_jacobian_aj_device")
+        for k in range(n):
+            layers[k].jrad = arr[k].jrad
+        return v1p.value
+
+    def jacobian(self, run, layers, tzu, tbnd, emis, species, dT=1e-3, dlnq=1e-4, aj=False):
+        """Host-side convenience: (radiance, jac[q, layer, point], v1p, dv);
+        aj selects the reference's analytic-Jacobian formulation."""
         n = od_size(run, layers[-1])
         P = 1 + len(species)
         rad = self.alloc(8 * n)
         jac = self.alloc(8 * n * P * len(layers))
         try:
-            v1p = self.jacobian_device(run, layers, tzu, tbnd, emis, species, dT, dlnq, rad.ptr, jac.ptr, n)
+            if aj:
+                v1p = self.jacobian_aj_device(run, layers, tzu, tbnd, emis, species, rad.ptr, jac.ptr, n)
+            else:
+                v1p = self.jacobian_device(run, layers, tzu, tbnd, emis, species, dT, dlnq, rad.ptr, jac.ptr, n)
             return (rad.to_host(np.float64), jac.to_host(np.float64).reshape(P, len(layers), n), v1p,
                     layers[-1].dv)
         finally:

@@ -233,6 +233,8 @@ void shape_tables(double *f1, double *f2, double *f3, double *fg, double *xv)
  * reference's F_k(IZ, IZM + 1) does at IZM = 101. */
 #define NZETA 101
 static int g_voigt = 0; /* set per call by oracle_od_layer from lblx_run.voigt */
+static int g_icflg = 0; /* CONTNM's /CDERIV/ ICFLG: > 0 for an analytic-Jacobian species run */
+void oracle_set_icflg(int icflg) { g_icflg = icflg; }
 static double VF[3 * NZETA + 1][NFMX + 1];        /* [column][IZ], IZ 1-based */
 static double VCOEF[6][NZETA];                    /* a_1 a_2 a_3 b_1 b_2 b_3 */
 static int voigt_done = 0;
@@ -1184,6 +1186,7 @@ int oracle_contnm(const lblx_run *run, const lblx_layer *Ly, int jrad, double *a
   double x_vmr_h2o = wk1 / WTOT, x_vmr_o2 = wk7 / WTOT;
   double x_vmr_n2 = 1. - x_vmr_h2o - x_vmr_o2;
   double wn2 = x_vmr_n2 * WTOT;
+  if (g_icflg > 0) wn2 = 0.; /* analytic-Jacobian species run (contnm.f90:221-227) */
   int ist, last;
 
   /* H2O (contnm.f90:296-327; mt_ckd_h2o_absco) */
@@ -1226,6 +1229,11 @@ int oracle_contnm(const lblx_run *run, const lblx_layer *Ly, int jrad, double *a
       double cself = wk1 * sh[k] * xself;
       double cfor = wk1 * fh[k] * xfrgn;
       AB[k] = AB[k] + cself + cfor;
+      if (g_icflg == 1) { /* /CDERIV/ ICFLG = 1: H2O (contnm.f90:314-322) */
+        double cfrgn_aj = (h2o_fac >= 1.0) ? 0. : wk1 * fh[k] * h2o_fac / (1.0 - h2o_fac);
+        double c = cself - cfrgn_aj;
+        AB[k] = AB[k] + c;
+      }
     }
     free(sc); free(fc); free(sh); free(fh);
   }

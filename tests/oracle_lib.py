@@ -247,3 +247,8 @@ def od_layer_dvout(run, layer, table, dvout, cap=20000000):
     if rc < 0:
         raise RuntimeError("oracle_od_layer_dvout rc=%d" % rc)
     return out[:nw.value], (v1o[:rc], v2o[:rc], nl[:rc])
+
+
+def set_icflg(icflg):
+    """CONTNM's /CDERIV/ ICFLG for the following oracle calls (0: off)."""
+    lib().oracle_set_icflg(int(icflg))
