@@ -525,28 +525,42 @@ __device__ __forceinline__ float rl_t(float v, int j) { return rl_f32(v, j); }
 // T = double: the dbl build.  T = float (FP32 split): the far-wing shape is
 // evaluated and accumulated in FP32; the line centre relative to the R4
 // grid (xnui) and the grid positions stay FP64.
+//
+// The reference's value at distance xm from the centre (oprop.f90:3989-4006)
+//   (ZVSQ <= ZSQBND ? SIV*(A3 + ZVSQ*B3) : SIL/(ALFLI2 + XMSQ)) - SUBTR,
+//   ZVSQ = XMSQ/ALFVI2, SUBTR = F4BND (CO2: (2 - XMSQ/BNDSQ)*F4BND)
+// is kept per line as the coefficients of one polynomial in XMSQ per branch:
+//   near = nA + XMSQ*nB,  far = SIL/(ALFLI2 + XMSQ) + (XMSQ*c1 - c0),
+// with c0 = SUBTR's constant, c1 its XMSQ slope, nA = SIV*A3 - c0,
+// nB = SIV*B3/ALFVI2 + c1, and the branch test XMSQ <= thr = ZSQBND*ALFVI2.
 template <typename T>
 struct F4Par {
   double xnui;
-  T sil, siv_a3, siv_b3, f4bnd, rec_alfvi2, alfli2;
-  int jmin, jmax, co2;
+  T sil, alfli2, nA, nB, c0, c1, thr;
+  int jmin, jmax;
 };
 
+// 1/d for d > 0 in the normal range: the hardware estimate refined by two
+// Newton steps (each doubles the correct bits), within 1 ulp.
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_nr(float d) { return __builtin_amdgcn_rcpf(d); }
+
+// Value at distance xmd = XJJ - XNUI from the line centre.
 template <typename T>
-__device__ __forceinline__ T f4_at(const F4Par<T> &p, double xjj, T r_bndsq, T zsqbnd) {
-  T xm = (T)(xjj - p.xnui);
+__device__ __forceinline__ T f4_at(const F4Par<T> &p, double xmd) {
+  T xm = (T)xmd;
   T xmsq = xm * xm;
-  T zvsq = xmsq * p.rec_alfvi2;
-  T sub = p.co2 ? ((T)2. - (xmsq * r_bndsq)) * p.f4bnd : p.f4bnd;
   // both branches evaluated and selected: the inner region covers a few
   // points per line, so a divergent branch would only add exec-mask work
-  T near = p.siv_a3 + zvsq * p.siv_b3;
-  T far;
-  if constexpr (sizeof(T) == 4)  // FP32 split: hardware reciprocal (1 ulp)
-    far = p.sil * __builtin_amdgcn_rcpf(p.alfli2 + xmsq);
-  else
-    far = p.sil / (p.alfli2 + xmsq);
-  return (zvsq <= zsqbnd ? near : far) - sub;
+  T near = fma(xmsq, p.nB, p.nA);
+  T far = fma(p.sil, rcp_nr(p.alfli2 + xmsq), fma(xmsq, p.c1, -p.c0));
+  return xmsq <= p.thr ? near : far;
 }
 
 template <typename T>
@@ -568,8 +582,6 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
   const double A3 = (1. + 2. * zsqbnd) / ((1. + zsqbnd) * (1. + zsqbnd));
   const double B3 = -1. / ((1. + zsqbnd) * (1. + zsqbnd));
   const double bndsq = bound4 * bound4;
-  const T r_bndsq = (T)(1. / bndsq);
-  const T zsqbnd_t = (T)zsqbnd;
   const double vnulst = W.v2r4 + bound4;
   const double vlo = W.v1r4 - bound4;
   const int nptr4 = W.nptr4;
@@ -605,12 +617,10 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       int i = pos + lane;
       bool valid = i < nl;
       F4Par<T> p;
-      p.co2 = 0;
       p.jmin = 1;
       p.jmax = 0;
       p.xnui = 0.;
-      p.f4bnd = (T)0.;
-      T speak = (T)0., partial = (T)0., pkb = (T)0.;
+      T speak = (T)0., partial = (T)0., pkb = (T)0., f4bnd = (T)0.;
       double vnui = 0.;
       bool nz = false, bnd_ok = false;
       int jj = 1;
@@ -625,7 +635,6 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         double alfavi = (c_avrat[iz - 1] + zetdif * (c_avrat[iz] - c_avrat[iz - 1])) * (alfali + alfadi);
         double ralfvi = 1. / alfavi;
         double sil = sp * recpi * alfali;
-        p.sil = (T)sil;
         double siv = (alfali * ralfvi) * sp * recpi * ralfvi;
         speak = (T)(A3 * fabs(siv));
         vnui = f_v;
@@ -639,11 +648,8 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         if (jmin > nptr4) p.jmax = 0;  // accepted but nothing to add (GO TO 60)
         double alfli2 = alfali * alfali;
         p.alfli2 = (T)alfli2;
-        p.f4bnd = (T)(sil / (alfli2 + bndsq));
-        p.rec_alfvi2 = (T)(1. / (alfavi * alfavi));
-        p.siv_a3 = (T)(siv * A3);
-        p.siv_b3 = (T)(siv * B3);
-        p.co2 = f_mol == 2;
+        double sil4 = sil, f4b = sil / (alfli2 + bndsq), siva3 = siv * A3, sivb3 = siv * B3;
+        bool co2 = f_mol == 2;
         if (Y.voigt) {  // oprop_voigt.f90:3875-3892, 3929-3931: zeta-dependent
                         // a_3, b_3; CO2 only through CHI (= 1)
           const int izx = iz - 1;  // 0..100
@@ -652,19 +658,27 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
           const double f4_64x = (a3x + b3x * zsqbnd);
           const double sivv = sp * ralfvi;
           speak = (T)(a3x * fabs(sivv));
-          const double siv_64 = sivv * f4_64x * (alfli2 + (alfavi * alfavi) * zsqbnd);
-          p.sil = (T)siv_64;
-          p.f4bnd = (T)(siv_64 / (alfli2 + bndsq));
-          p.siv_a3 = (T)(sivv * a3x);
-          p.siv_b3 = (T)(sivv * b3x);
-          p.co2 = 0;
+          sil4 = sivv * f4_64x * (alfli2 + (alfavi * alfavi) * zsqbnd);
+          f4b = sil4 / (alfli2 + bndsq);
+          siva3 = sivv * a3x;
+          sivb3 = sivv * b3x;
+          co2 = false;
         }
-        // peak bound of the shape: near branch <= siv_a3 (siv_b3 < 0), far
-        // branch <= sil / (alfli2 + zsqbnd alfvi2) (it applies beyond
-        // zsqbnd halfwidths squared); margins cover rounding in T
+        const double alfvi2 = alfavi * alfavi;
+        const double c0 = co2 ? 2. * f4b : f4b, c1 = co2 ? f4b / bndsq : 0.;
+        p.sil = (T)sil4;
+        p.c0 = (T)c0;
+        p.c1 = (T)c1;
+        p.nA = (T)(siva3 - c0);
+        p.nB = (T)(sivb3 / alfvi2 + c1);
+        p.thr = (T)(zsqbnd * alfvi2);
+        f4bnd = (T)f4b;
+        // peak bound of the shape: near branch <= SIV*A3 (B3 < 0), far
+        // branch <= SIL / (ALFLI2 + ZSQBND ALFVI2) (it applies beyond
+        // ZSQBND halfwidths squared); margins cover rounding in T
         bnd_ok = sp > 0.;
-        const double pkn = (double)p.siv_a3;
-        const double pkf = (double)p.sil / ((double)p.alfli2 + zsqbnd * (alfavi * alfavi) * (1. - 1e-6));
+        const double pkn = siva3;
+        const double pkf = sil4 / (alfli2 + zsqbnd * alfvi2 * (1. - 1e-6));
         pkb = (T)((pkn > pkf ? pkn : pkf) * (1. + 1e-5));
         partial = r4[jj];
       }
@@ -676,7 +690,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       // against the upper bound accepted, whatever the earlier lines
       // decide.  Only lines up to the last undecided one are taken in order
       // (their exact running R4 is needed); the rest are accepted in bulk.
-      T neg = cand ? (T)2. * p.f4bnd : (T)0.;
+      T neg = cand ? (T)2. * f4bnd : (T)0.;
       T pk = cand ? pkb : (T)0.;
       T pre = neg, preu = pk;
       for (int d = 1; d < 64; d <<= 1) {
@@ -715,15 +729,15 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
           F4Par<T> q;
           q.xnui = rl_f64(p.xnui, j);
           q.sil = rl_t(p.sil, j);
-          q.siv_a3 = rl_t(p.siv_a3, j);
-          q.siv_b3 = rl_t(p.siv_b3, j);
-          q.f4bnd = rl_t(p.f4bnd, j);
-          q.rec_alfvi2 = rl_t(p.rec_alfvi2, j);
           q.alfli2 = rl_t(p.alfli2, j);
+          q.nA = rl_t(p.nA, j);
+          q.nB = rl_t(p.nB, j);
+          q.c0 = rl_t(p.c0, j);
+          q.c1 = rl_t(p.c1, j);
+          q.thr = rl_t(p.thr, j);
           q.jmin = __builtin_amdgcn_readlane(p.jmin, j);
           q.jmax = __builtin_amdgcn_readlane(p.jmax, j);
-          q.co2 = __builtin_amdgcn_readlane(p.co2, j);
-          if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me, r_bndsq, zsqbnd_t);
+          if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me - q.xnui);
           if (lane == j && p.jmax >= p.jmin) par[nacc] = p;
           if (q.jmax >= q.jmin) nacc++;
         }
@@ -757,16 +771,17 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
     }
     __syncthreads();
     const int nacc = s_nacc;
+    const double step = CF4_T * dvr4;
     for (int a = 0; a < nacc; a++) {
       F4Par<T> q = par[a];
       if (tid == 0) nupd += (unsigned long long)(q.jmax - q.jmin + 1);
       int j0 = q.jmin - 1;
       int r = j0 % CF4_T;
       int first = j0 - r + tid + (tid < r ? CF4_T : 0) + 1;
-      for (int j = first; j <= q.jmax; j += CF4_T) {
-        double xjj = (double)(j - 1) * dvr4;
-        r4[j] = r4[j] + f4_at(q, xjj, r_bndsq, zsqbnd_t);
-      }
+      // XJJ - XNUI advanced by a running sum: a line spans a few strides,
+      // so the sum's rounding stays within a few ulp of XJJ
+      double xmd = (double)(first - 1) * dvr4 - q.xnui;
+      for (int j = first; j <= q.jmax; j += CF4_T, xmd += step) r4[j] = r4[j] + f4_at(q, xmd);
     }
     __syncthreads();
     if (s_done) break;
