@@ -121,6 +121,65 @@ struct lblx_ctx {
     if (h < 0) return;
     (void)hipEventRecord(evs[h].second.second, stream);
   }
+  // Pinned staging of the per-call tables copied to the device.  A copy
+  // from pageable memory returns only once the stream has reached it, which
+  // would hold the host until the queued kernels finish; staged copies let
+  // the host return and plan the next call while they run.  Two arenas are
+  // used by alternate generations (one table upload each); an arena is
+  // refilled only after the copies issued from it have completed.
+  struct Stage {
+    char *p = nullptr;
+    size_t cap = 0, used = 0;
+    hipEvent_t ev = nullptr;
+    bool live = false;
+  };
+  Stage stg[2];
+  int scur = 0;
+  void stage_begin() {
+    scur ^= 1;
+    Stage &S = stg[scur];
+    if (S.live) (void)hipEventSynchronize(S.ev);
+    else if (S.used > 0) (void)hipStreamSynchronize(stream);  // a generation that never ended
+    S.live = false;
+    S.used = 0;
+  }
+  void stage_end(hipStream_t s) {
+    Stage &S = stg[scur];
+    if (!S.ev && hipEventCreateWithFlags(&S.ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipStreamSynchronize(s);  // no event: the copies are simply waited for
+      return;
+    }
+    (void)hipEventRecord(S.ev, s);
+    S.live = true;
+  }
+  hipError_t h2d(void *dst, const void *src, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    Stage &S = stg[scur];
+    const size_t rn = (n + 255) & ~(size_t)255;
+    if (S.used + rn > S.cap) {
+      // the copies already issued from this arena finish before it is replaced
+      hipError_t e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return e;
+      const size_t cap = std::max(2 * (S.used + rn), (size_t)16 << 20);
+      char *np = nullptr;
+      if ((e = hipHostMalloc((void **)&np, cap, hipHostMallocDefault)) != hipSuccess) return e;
+      if (S.p) (void)hipHostFree(S.p);
+      S.p = np;
+      S.cap = cap;
+      S.used = 0;
+    }
+    char *q = S.p + S.used;
+    std::memcpy(q, src, n);
+    S.used += rn;
+    return hipMemcpyAsync(dst, q, n, hipMemcpyHostToDevice, s);
+  }
+  void stage_release() {
+    for (Stage &S : stg) {
+      if (S.ev) (void)hipEventDestroy(S.ev);
+      if (S.p) (void)hipHostFree(S.p);
+      S = Stage{};
+    }
+  }
   // host wall time (ST_HOST), summed only while timing is on
   std::chrono::steady_clock::time_point h0;
   void hbegin() {
@@ -298,6 +357,7 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
+  c->stage_release();
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -713,13 +773,14 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   }
   for (int G = 0; G < 3; G++)
     if (need(c->w_tmap[G], tjob[G].size() * 16)) return LBLX_E_HIP;
-  HIPCHK(hipMemcpyAsync(c->w_dlay.p, ys.data(), NJ * sizeof(DevLayer), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(c->w_pan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(c->w_win.p, win.data(), win.size() * sizeof(DevWin), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(c->w_seg.p, seg.data(), seg.size() * sizeof(DevSeg), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(c->w_margin.p, margins.data(), margins.size() * 8, hipMemcpyHostToDevice, s));
+  c->stage_begin();
+  HIPCHK(c->h2d(c->w_dlay.p, ys.data(), NJ * sizeof(DevLayer), s));
+  HIPCHK(c->h2d(c->w_pan.p, pan.data(), pan.size() * sizeof(DevPanel), s));
+  HIPCHK(c->h2d(c->w_win.p, win.data(), win.size() * sizeof(DevWin), s));
+  HIPCHK(c->h2d(c->w_seg.p, seg.data(), seg.size() * sizeof(DevSeg), s));
+  HIPCHK(c->h2d(c->w_margin.p, margins.data(), margins.size() * 8, s));
   if (abs_stride > 0) {  // CONTNM: ABSRB rows of every job
-    HIPCHK(hipMemcpyAsync(c->w_cps.p, cps.data(), NJ * sizeof(ContPlan), hipMemcpyHostToDevice, s));
+    HIPCHK(c->h2d(c->w_cps.p, cps.data(), NJ * sizeof(ContPlan), s));
     int thc = c->tbegin(ST_CONTNM);
     int64_t nth = NJ * abs_stride;
     hipLaunchKernelGGL(k_contnm, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s,
@@ -732,13 +793,14 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   for (int G = 0; G < 3; G++) {
     size_t nt = tjob[G].size();
     int32_t *base = c->w_tmap[G].as<int32_t>();
-    HIPCHK(hipMemcpyAsync(base, tjob[G].data(), nt * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(base + nt, tloc[G].data(), nt * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(c->h2d(base, tjob[G].data(), nt * 4, s));
+    HIPCHK(c->h2d(base + nt, tloc[G].data(), nt * 4, s));
     tm[G].job = base;
     tm[G].loc = base + nt;
     tm[G].lo = base + 2 * nt;
     tm[G].hi = base + 3 * nt;
   }
+  c->stage_end(s);
   const DevLayer *dY = c->w_dlay.as<DevLayer>();
   DevLines L = c->lines();
   ShrinkWork W;
@@ -1185,8 +1247,9 @@ int lblx_xmerge_device(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer
   std::vector<DevPanel> pan;
   merge_panels(K, layer_old ? &base : nullptr, pan);
   if (c->w_mpan.ensure(pan.size() * sizeof(DevPanel))) return LBLX_E_HIP;
-  HIPCHK(hipMemcpyAsync(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice,
-                        c->stream));
+  c->stage_begin();
+  HIPCHK(c->h2d(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), c->stream));
+  c->stage_end(c->stream);
   return merge_launch(c, K, layer_old ? &O : nullptr, itype, layer->tave, tzu,
                       layer_old ? 0. : tbnd, emis, d_od, run->precision == LBLX_FP32_SPLIT, d_rold,
                       d_told, d_rnew, d_tnew, c->w_mpan.as<DevPanel>());
@@ -1236,8 +1299,9 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     return LBLX_E_HIP;
   int rc = run_jobs(c, run, layers, nlay, true, c->w_od.p, off.data(), f32, nullptr, st, plans.data());
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice,
-                        c->stream));
+  c->stage_begin();
+  HIPCHK(c->h2d(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), c->stream));
+  c->stage_end(c->stream);
   double *rt = c->w_rt.as<double>();
   double *R[2] = {rt, rt + nmax}, *T[2] = {rt + 2 * nmax, rt + 3 * nmax};
   for (int l = 0; l < nlay; l++) {
@@ -1759,7 +1823,9 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
                     jplans.data());
   if (rc) return rc;
   hipStream_t s = c->stream;
-  HIPCHK(hipMemcpyAsync(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), hipMemcpyHostToDevice, s));
+  c->stage_begin();
+  HIPCHK(c->h2d(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), s));
+  c->stage_end(s);
   const DevPanel *dpan = c->w_mpan.as<DevPanel>();
   double *rall = c->j_rall.as<double>();
   double *tt[2] = {c->w_rt.as<double>(), c->w_rt.as<double>() + nmax};

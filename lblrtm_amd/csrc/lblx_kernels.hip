@@ -274,8 +274,12 @@ __global__ void k_shr_chain(const DevLayer *Ys, Strides S, ShrinkWork W) {
     W.wsum[o + e] = 0;
     return;
   }
-  k = e + 2;
-  while (k < n && !(x[k] > thr)) k++;
+  // the first line past VLMT from e + 1 on is also the first from e + 2 on
+  // unless it is line e + 1 itself
+  if (k == e + 1) {
+    k = e + 2;
+    while (k < n && !(x[k] > thr)) k++;
+  }
   int f = k - 1;
   W.jmp[o + e] = f;
   W.wsum[o + e] = (int16_t)em_cnt(W.pn + o, W.pc + o, e + 1, f);
