@@ -229,7 +229,7 @@ struct lblx_ctx {
   DevBuf w_dirty;  // blocks whose read panel moved in the last fixpoint step
   DevBuf w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
   DevBuf w_tmap[3];
-  DevBuf w_mpan, w_rt;
+  DevBuf w_mpan, w_rt, w_mrun;
   DevBuf s_z[2], s_tw, s_g, s_ip, s_cof;  // FFTSCN
   DevBuf j_rall, j_a, j_b;                // Jacobians
   int64_t s_twN = 0;
@@ -354,7 +354,7 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
                     &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
-                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt};
+                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->w_mrun, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
   c->stage_release();
@@ -1162,6 +1162,16 @@ static void merge_panels(const LayerPlan &K, const std::vector<int64_t> *base,
 }
 
 // MergeArgs of one layer: grid, temperatures and RADMRG's weights.
+// Whether the merge of layer K onto layer O is the identity map of RADNN:
+// ITYPE 0, the same panels, and new panel p reading old point 2400 p on.
+static bool merge_is_identity(const LayerPlan &O, const LayerPlan &K, int itype, const DevPanel *kpan,
+                              const DevPanel *opan) {
+  if (itype != 0 || K.npts != O.npts || K.dv != O.dv || K.panels.size() != O.panels.size()) return false;
+  for (size_t p = 0; p < K.panels.size(); p++)
+    if (kpan[p].out0 != (int64_t)2400 * (int64_t)p || kpan[p].vft != opan[p].vft) return false;
+  return true;
+}
+
 static MergeArgs merge_args(const LayerPlan &K, const LayerPlan *O, int itype, double tave, double tzu,
                             double tbnd, double emis) {
   MergeArgs A{};
@@ -1299,21 +1309,45 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     return LBLX_E_HIP;
   int rc = run_jobs(c, run, layers, nlay, true, c->w_od.p, off.data(), f32, nullptr, st, plans.data());
   if (rc) return rc;
+  // runs of layers whose merge after the first is the identity map
+  std::vector<int> rstart;
+  std::vector<RunLayer> rl(nlay);
+  for (int l = 0; l < nlay; l++) {
+    rl[l].xkt = layers[l].tave / RADCN2;
+    rl[l].xkta = tzu[l] / RADCN2;
+    rl[l].od = off[l];
+    if (l == 0 || !merge_is_identity(plans[l - 1], plans[l], itype[l], pan.data() + pan0[l],
+                                     pan.data() + pan0[l - 1]))
+      rstart.push_back(l);
+  }
+  rstart.push_back(nlay);
+  if (c->w_mrun.ensure((size_t)nlay * sizeof(RunLayer))) return LBLX_E_HIP;
   c->stage_begin();
   HIPCHK(c->h2d(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), c->stream));
+  HIPCHK(c->h2d(c->w_mrun.p, rl.data(), (size_t)nlay * sizeof(RunLayer), c->stream));
   c->stage_end(c->stream);
   double *rt = c->w_rt.as<double>();
   double *R[2] = {rt, rt + nmax}, *T[2] = {rt + 2 * nmax, rt + 3 * nmax};
-  for (int l = 0; l < nlay; l++) {
-    const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
-    double *rn = (l == nlay - 1) ? d_rad : R[l & 1];
-    double *tn = (l == nlay - 1) ? d_tra : T[l & 1];
-    const double *ro = l > 0 ? R[(l - 1) & 1] : nullptr;
-    const double *to = l > 0 ? T[(l - 1) & 1] : nullptr;
-    rc = merge_launch(c, plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
-                      l == 0 ? tbnd : 0., emis, od, f32, ro, to, rn, tn,
-                      c->w_mpan.as<DevPanel>() + pan0[l]);
-    if (rc) return rc;
+  for (size_t q = 0; q + 1 < rstart.size(); q++) {
+    const int l = rstart[q], nrun = rstart[q + 1] - l, last = l + nrun - 1;
+    double *rn = (last == nlay - 1) ? d_rad : R[q & 1];
+    double *tn = (last == nlay - 1) ? d_tra : T[q & 1];
+    const double *ro = q > 0 ? R[(q - 1) & 1] : nullptr;
+    const double *to = q > 0 ? T[(q - 1) & 1] : nullptr;
+    MergeArgs A = merge_args(plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+                             l == 0 ? tbnd : 0., emis);
+    const RunLayer *drl = c->w_mrun.as<RunLayer>() + l;
+    const DevPanel *dp = c->w_mpan.as<DevPanel>() + pan0[l];
+    int th = c->tbegin(ST_MERGE);
+    dim3 grid((unsigned)((A.n + 255) / 256));
+    if (f32)
+      hipLaunchKernelGGL(k_merge_run<float>, grid, dim3(256), 0, c->stream, A, drl, nrun, dp,
+                         c->w_od.as<float>(), ro, to, rn, tn);
+    else
+      hipLaunchKernelGGL(k_merge_run<double>, grid, dim3(256), 0, c->stream, A, drl, nrun, dp,
+                         c->w_od.as<double>(), ro, to, rn, tn);
+    c->tend(th);
+    HIPCHK(hipGetLastError());
   }
   if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
   return LBLX_OK;

@@ -1550,6 +1550,18 @@ __device__ __forceinline__ MergeTap merge_tap(const MergeArgs &A, int64_t base1,
   return t;
 }
 
+// EMIN's emission of a layer at wavenumber vi (and its transmittance).
+__device__ __forceinline__ double merge_em(double vi, double odvi, double xkt, double xkta, double *tr_out) {
+  const double od_lo = 0.06;
+  double bb = d_planck(vi, xkt);
+  double bb_dif = xkta > 0. ? d_planck(vi, xkta) - bb : 0.;
+  if (odvi < -od_lo) odvi = -od_lo;
+  double tr = exp(-odvi);
+  double f = odvi <= od_lo ? (1. / 6.) * odvi : 1. - 2. * (tr / (tr - 1.) + 1. / odvi);
+  *tr_out = tr;
+  return (1. - tr) * (bb + bb_dif * f);
+}
+
 template <typename IN>
 __global__ void k_merge(MergeArgs A, const DevPanel *pan, const IN *od, const double *rold,
                         const double *told, double *rnew, double *tnew) {
@@ -1558,14 +1570,8 @@ __global__ void k_merge(MergeArgs A, const DevPanel *pan, const IN *od, const do
   const int p = (int)(g / 2400);
   const int k = (int)(g - (int64_t)2400 * p);
   const double vi = (pan[p].vft + 32. * A.dv) + (double)k * A.dv;
-  const double od_lo = 0.06;
-  double bb = d_planck(vi, A.xkt);
-  double bb_dif = A.xkta > 0. ? d_planck(vi, A.xkta) - bb : 0.;
-  double odvi = (double)od[g];
-  if (odvi < -od_lo) odvi = -od_lo;
-  double tr = exp(-odvi);
-  double f = odvi <= od_lo ? (1. / 6.) * odvi : 1. - 2. * (tr / (tr - 1.) + 1. / odvi);
-  double em = (1. - tr) * (bb + bb_dif * f);
+  double tr;
+  double em = merge_em(vi, (double)od[g], A.xkt, A.xkta, &tr);
   if (A.first) {
     if (A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
     rnew[g] = em;
@@ -1577,6 +1583,50 @@ __global__ void k_merge(MergeArgs A, const DevPanel *pan, const IN *od, const do
   double to = t.w0 * told[t.i0] + t.w1 * told[t.i1] + t.w2 * told[t.i2] + t.w3 * told[t.i3];
   rnew[g] = tr * ro + em;
   tnew[g] = tr * to;
+}
+
+// A run of layers on one grid whose merges after the first are the
+// identity map of RADNN (ITYPE = 0, new panel p reading old point 2400 p
+// on): the first layer merges as k_merge does, the rest update R and T in
+// registers point by point (R = T_l R + E_l, T = T_l T; the identity taps
+// add exact zeros, so the values equal the layer-by-layer merge), and only
+// the last layer's R and T are written.
+struct RunLayer {
+  double xkt, xkta;
+  int64_t od;  // element offset of the layer's optical depth
+};
+
+template <typename IN>
+__global__ void k_merge_run(MergeArgs A, const RunLayer *rl, int nrun, const DevPanel *pan,
+                            const IN *odb, const double *rold, const double *told, double *rnew,
+                            double *tnew) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.n) return;
+  const int p = (int)(g / 2400);
+  const int k = (int)(g - (int64_t)2400 * p);
+  const double vi = (pan[p].vft + 32. * A.dv) + (double)k * A.dv;
+  double tr;
+  double em = merge_em(vi, (double)odb[rl[0].od + g], A.xkt, A.xkta, &tr);
+  double r, t;
+  if (A.first) {
+    if (A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
+    r = em;
+    t = tr;
+  } else {
+    MergeTap q = merge_tap(A, pan[p].out0, k);
+    double ro = q.w0 * rold[q.i0] + q.w1 * rold[q.i1] + q.w2 * rold[q.i2] + q.w3 * rold[q.i3];
+    double to = q.w0 * told[q.i0] + q.w1 * told[q.i1] + q.w2 * told[q.i2] + q.w3 * told[q.i3];
+    r = tr * ro + em;
+    t = tr * to;
+  }
+  for (int j = 1; j < nrun; j++) {
+    const RunLayer L = rl[j];
+    em = merge_em(vi, (double)odb[L.od + g], L.xkt, L.xkta, &tr);
+    r = tr * r + em;
+    t = tr * t;
+  }
+  rnew[g] = r;
+  tnew[g] = t;
 }
 
 

@@ -113,3 +113,38 @@ def test_spectrum_column_workload(gpu_ctx):
           flush=True)
     assert d.max() <= BT_TOL
     np.testing.assert_allclose(tra, m_ref.tra, rtol=1e-4, atol=1e-12)
+
+
+def test_spectrum_merge_runs_match_layer_chain(gpu_ctx):
+    """The spectrum merges runs of same-grid layers in one pass; its TOA
+    radiance and transmittance equal the layer-by-layer XMERGE chain
+    (lblx_xmerge_device) over the same layer optical depths, up to the
+    summation order of the optical depths themselves."""
+    tab = synth.config0_lines()
+    gpu_ctx.load_lines(tab)
+    run, lays, atm, it = _layers(600.0, 1100.0)
+    tzu = np.asarray(atm["tlev"][1:], np.float64)
+    tbnd, emis = float(atm["tlev"][0]), 0.98
+    same = sum(1 for l in range(1, len(lays)) if lays[l].dv == lays[l - 1].dv)
+    print("merge runs: layers on the grid of the layer below:", same, flush=True)
+    assert same >= 10
+    rad, tra, v1p, dv = gpu_ctx.spectrum(run, [abi.Layer.from_buffer_copy(x) for x in lays], tzu, tbnd, emis)
+    sizes = [lib.od_size(run, L) for L in lays]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    buf = gpu_ctx.alloc(8 * int(sum(sizes)))
+    gpu_ctx.opdpth_column_device(run, [abi.Layer.from_buffer_copy(x) for x in lays], buf.ptr, offs)
+    nmax = max(sizes)
+    rt = [gpu_ctx.alloc(8 * nmax) for _ in range(4)]
+    for l, L in enumerate(lays):
+        dst_r, dst_t = rt[2 * (l & 1)], rt[2 * (l & 1) + 1]
+        src_r, src_t = rt[2 * ((l - 1) & 1)], rt[2 * ((l - 1) & 1) + 1]
+        gpu_ctx.xmerge_device(run, lays[l - 1] if l else None, L, float(tzu[l]), tbnd if l == 0 else 0.0, emis,
+                              buf.ptr + 8 * int(offs[l]), src_r.ptr if l else 0, src_t.ptr if l else 0,
+                              dst_r.ptr, dst_t.ptr)
+    top = 2 * ((len(lays) - 1) & 1)
+    r_chain = rt[top].to_host(np.float64, sizes[-1])
+    t_chain = rt[top + 1].to_host(np.float64, sizes[-1])
+    for b in rt + [buf]:
+        b.free()
+    np.testing.assert_allclose(rad, r_chain, rtol=1e-12)
+    np.testing.assert_allclose(tra, t_chain, rtol=1e-12, atol=1e-300)
