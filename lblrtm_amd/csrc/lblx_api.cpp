@@ -234,7 +234,7 @@ struct lblx_ctx {
   DevBuf j_rall, j_a, j_b;                // Jacobians
   int64_t s_twN = 0;
   int tiles[3] = {8192, 4096, 2048};      // R1, R2, R3 tile widths (autotuned)
-  DevBuf b_jobs, b_rt;                    // batched merges
+  DevBuf b_jobs, b_rt, b_rl;              // batched merges
   DevBuf w_dvo;                           // PNLINT runs
   struct GraphEntry {
     std::vector<int64_t> key;
@@ -354,7 +354,7 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
                     &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
-                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->w_mrun, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt};
+                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->w_mrun, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt, &c->b_rl};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
   c->stage_release();
@@ -2107,13 +2107,15 @@ int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const
   }
   // the buffers every group's descriptors point into, sized once
   if (c->w_od.ensure((size_t)max_tot * esz) || c->b_rt.ensure((size_t)max_rt * 8) ||
-      c->b_jobs.ensure((size_t)max_jobs * sizeof(MergeJob)))
+      c->b_jobs.ensure((size_t)max_jobs * sizeof(MergeJob)) ||
+      c->b_rl.ensure((size_t)max_jobs * sizeof(RunLayer)))
     return LBLX_E_HIP;
   // Host plan of one group: its OPDPTH jobs and the merge descriptors of
   // every layer index (kstart/kcount: the launch of layer index k).
   struct GroupPlan {
     JobsPrep J;
     std::vector<MergeJob> mj;
+    std::vector<RunLayer> rl;
     std::vector<int> kstart, kcount;
     std::vector<int64_t> key;
     int rc = 0;
@@ -2123,25 +2125,46 @@ int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const
     P.rc = prep_jobs(c, run, layers + first[G.p0], G.njobs, false, G.off.data(), plans.data() + first[G.p0],
                      P.J);
     if (P.rc) return;
+    // runs of same-grid layers of each profile (identity merges after the
+    // first); launch k takes the k-th run of every profile
     P.mj.clear();
-    P.kstart.assign(G.maxl + 1, 0);
-    P.kcount.assign(G.maxl, 0);
-    P.key = {(int64_t)g, (int64_t)f32, (int64_t)G.maxl, (int64_t)(intptr_t)c->b_jobs.p};
-    for (int k = 0; k < G.maxl; k++) {
+    P.rl.assign(G.njobs, RunLayer{});
+    std::vector<std::vector<int>> rs(G.p1 - G.p0);
+    int nstep = 0;
+    for (int p = G.p0; p < G.p1; p++) {
+      std::vector<int> &r = rs[p - G.p0];
+      for (int k = 0; k < nlay[p]; k++) {
+        const int l = first[p] + k;
+        P.rl[l - first[G.p0]] = RunLayer{layers[l].tave / RADCN2, tzu[l] / RADCN2, G.off[l - first[G.p0]]};
+        if (k == 0 || !merge_is_identity(plans[l - 1], plans[l], itype[l], pan.data() + pan0[l],
+                                         pan.data() + pan0[l - 1]))
+          r.push_back(k);
+      }
+      r.push_back(nlay[p]);
+      nstep = std::max(nstep, (int)r.size() - 1);
+    }
+    P.kstart.assign(nstep + 1, 0);
+    P.kcount.assign(nstep, 0);
+    P.key = {(int64_t)g, (int64_t)f32, (int64_t)nstep, (int64_t)(intptr_t)c->b_jobs.p};
+    const RunLayer *drl = c->b_rl.as<RunLayer>();
+    for (int k = 0; k < nstep; k++) {
       P.kstart[k] = (int)P.mj.size();
       int64_t nk = 0;
       for (int p = G.p0; p < G.p1; p++) {
-        if (k >= nlay[p]) continue;
-        const int l = first[p] + k;
+        const std::vector<int> &r = rs[p - G.p0];
+        if (k + 1 >= (int)r.size()) continue;
+        const int l = first[p] + r[k], nrun = r[k + 1] - r[k];
         double *rt = c->b_rt.as<double>() + G.rt_off[p - G.p0];
         const int64_t m = G.nmax[p - G.p0];
         double *R[2] = {rt, rt + m}, *T[2] = {rt + 2 * m, rt + 3 * m};
-        const bool last = k == nlay[p] - 1;
-        MergeJob M;
-        M.A = merge_args(plans[l], k > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
-                         k == 0 ? tbnd[p] : 0., emis[p]);
+        const bool last = r[k + 1] == nlay[p];
+        MergeJob M{};
+        M.A = merge_args(plans[l], r[k] > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+                         r[k] == 0 ? tbnd[p] : 0., emis[p]);
         M.pan = dpan + pan0[l];
-        M.od = (const char *)c->w_od.p + (size_t)G.off[l - first[G.p0]] * esz;
+        M.od = c->w_od.p;
+        M.rl = drl + (l - first[G.p0]);
+        M.nrun = nrun;
         M.rold = k > 0 ? R[(k - 1) & 1] : nullptr;
         M.told = k > 0 ? T[(k - 1) & 1] : nullptr;
         M.rnew = last ? d_rad + out_off[p] : R[k & 1];
@@ -2189,6 +2212,7 @@ int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const
       // (instantiated once per shape)
       HIPCHK(hipMemcpyAsync(c->b_jobs.p, P.mj.data(), P.mj.size() * sizeof(MergeJob), hipMemcpyHostToDevice,
                             s));
+      HIPCHK(hipMemcpyAsync(c->b_rl.p, P.rl.data(), P.rl.size() * sizeof(RunLayer), hipMemcpyHostToDevice, s));
       HIPCHK(hipEventRecord(up_done[g & 1], s));
       hipGraphExec_t exec = nullptr;
       for (auto &e : c->graphs)
@@ -2197,8 +2221,8 @@ int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         const MergeJob *dj = c->b_jobs.as<MergeJob>();
-        const int maxl = groups[g].maxl;
-        for (int k = 0; k < maxl; k++) {
+        const int nstep = (int)P.kcount.size();
+        for (int k = 0; k < nstep; k++) {
           const int64_t nk = P.key[4 + 2 * k + 1];
           dim3 grid((unsigned)((nk + 255) / 256), (unsigned)P.kcount[k]);
           if (f32) hipLaunchKernelGGL(k_merge_batch<float>, grid, dim3(256), 0, s, dj + P.kstart[k]);

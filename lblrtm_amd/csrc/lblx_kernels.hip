@@ -1717,10 +1717,14 @@ __global__ void k_jac_prop(MergeArgs A, const DevPanel *pan, const IN *od, const
 
 
 // One layer merge of many profiles in one launch (blockIdx.y = profile).
+// One run of same-grid layers of one profile (k_merge_run's work), for the
+// batched merges: every launch takes the next run of each profile.
 struct MergeJob {
-  MergeArgs A;
+  MergeArgs A;           // the run's first layer
   const DevPanel *pan;
-  const void *od;        // double, or float for FP32 split
+  const void *od;        // optical depth base (double, or float for FP32 split)
+  const RunLayer *rl;    // the run's layers: temperatures and optical-depth offsets
+  int32_t nrun, pad;
   const double *rold, *told;
   double *rnew, *tnew;
 };
@@ -1734,25 +1738,29 @@ __global__ void k_merge_batch(const MergeJob *jobs) {
   const int p = (int)(g / 2400);
   const int k = (int)(g - (int64_t)2400 * p);
   const double vi = (J.pan[p].vft + 32. * A.dv) + (double)k * A.dv;
-  const double od_lo = 0.06;
-  double bb = d_planck(vi, A.xkt);
-  double bb_dif = A.xkta > 0. ? d_planck(vi, A.xkta) - bb : 0.;
-  double odvi = (double)((const IN *)J.od)[g];
-  if (odvi < -od_lo) odvi = -od_lo;
-  double tr = exp(-odvi);
-  double f = odvi <= od_lo ? (1. / 6.) * odvi : 1. - 2. * (tr / (tr - 1.) + 1. / odvi);
-  double em = (1. - tr) * (bb + bb_dif * f);
+  const IN *odb = (const IN *)J.od;
+  double tr;
+  double em = merge_em(vi, (double)odb[J.rl[0].od + g], A.xkt, A.xkta, &tr);
+  double r, t;
   if (A.first) {
     if (A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
-    J.rnew[g] = em;
-    J.tnew[g] = tr;
-    return;
+    r = em;
+    t = tr;
+  } else {
+    MergeTap q = merge_tap(A, J.pan[p].out0, k);
+    double ro = q.w0 * J.rold[q.i0] + q.w1 * J.rold[q.i1] + q.w2 * J.rold[q.i2] + q.w3 * J.rold[q.i3];
+    double to = q.w0 * J.told[q.i0] + q.w1 * J.told[q.i1] + q.w2 * J.told[q.i2] + q.w3 * J.told[q.i3];
+    r = tr * ro + em;
+    t = tr * to;
   }
-  MergeTap t = merge_tap(A, J.pan[p].out0, k);
-  double ro = t.w0 * J.rold[t.i0] + t.w1 * J.rold[t.i1] + t.w2 * J.rold[t.i2] + t.w3 * J.rold[t.i3];
-  double to = t.w0 * J.told[t.i0] + t.w1 * J.told[t.i1] + t.w2 * J.told[t.i2] + t.w3 * J.told[t.i3];
-  J.rnew[g] = tr * ro + em;
-  J.tnew[g] = tr * to;
+  for (int j = 1; j < J.nrun; j++) {
+    const RunLayer L = J.rl[j];
+    em = merge_em(vi, (double)odb[L.od + g], L.xkt, L.xkta, &tr);
+    r = tr * r + em;
+    t = tr * t;
+  }
+  J.rnew[g] = r;
+  J.tnew[g] = t;
 }
 
 __global__ void k_fill_f64(double *p, int64_t n, double v) {
