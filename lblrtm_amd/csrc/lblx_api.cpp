@@ -539,6 +539,26 @@ static int ensure_voigt(lblx_ctx *c) {
 
 // Host side of an OPDPTH batch: the jobs' descriptors and plan tables, tile
 // maps and the strides of the per-job work arrays.
+// f(i) for i in [0, n) on up to 16 host threads (the calling thread among
+// them); f must not depend on the order.
+template <class F>
+static void parallel_for(int n, F f) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nth = (int)std::min<unsigned>(std::min(16u, hw), (unsigned)std::max(1, n / 8));
+  if (nth <= 1) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int i = next++; i < n; i = next++) f(i);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nth; t++) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+}
+
 struct JobsPrep {
   int njobs = 0;
   std::vector<LayerPlan> own;
@@ -569,9 +589,6 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
   J.own.assign(pre ? 0 : njobs, LayerPlan{});
   J.plans.assign(njobs, nullptr);
   J.ys.assign(njobs, DevLayer{});
-  J.pan.clear();
-  J.win.clear();
-  J.seg.clear();
   J.margins.assign(3 * njobs, 0.);
   J.cps.assign(njobs, ContPlan{});
   auto &own = J.own;
@@ -582,25 +599,49 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
   auto &seg = J.seg;
   auto &margins = J.margins;
   auto &cps = J.cps;
-  int64_t r2tot = 0, r3tot = 0;
-  int maxnn = 0, maxnbk = 0, abs_stride = 0, maxnb4 = 0;
-  int64_t maxline = 0;
-  int prev_jrad = layers[0].jrad;
-  for (int jb = 0; jb < njobs; jb++) {
-    lblx_layer *ly = &layers[jb];
-    if (chain_jrad && jb > 0) ly->jrad = prev_jrad;
-    if (pre) {
+  // 1. layer plans: handed in, or computed here on host threads (JRAD
+  // chained up the batch first when asked)
+  if (pre) {
+    for (int jb = 0; jb < njobs; jb++) {
       plans[jb] = &pre[jb];
-      ly->jrad = pre[jb].jrad;
-    } else {
-      int rc = plan_layer(run, ly, &own[jb]);
-      if (rc) return rc;
+      layers[jb].jrad = pre[jb].jrad;
+    }
+  } else {
+    if (chain_jrad)
+      for (int jb = 1; jb < njobs; jb++) layers[jb].jrad = layer_jrad(&layers[jb - 1]);
+    std::vector<int> prc(njobs, 0);
+    parallel_for(njobs, [&](int jb) { prc[jb] = plan_layer(run, &layers[jb], &own[jb]); });
+    for (int jb = 0; jb < njobs; jb++) {
+      if (prc[jb]) return prc[jb];
       plans[jb] = &own[jb];
     }
+  }
+  // 2. offsets of every job's panels, windows, segments and R2/R3 rows
+  int64_t r2tot = 0, r3tot = 0;
+  size_t npan = 0, nwin_all = 0, nseg = 0;
+  for (int jb = 0; jb < njobs; jb++) {
     const LayerPlan &P = *plans[jb];
-    prev_jrad = ly->jrad;
     DevLayer &Y = ys[jb];
-    Y = DevLayer{};
+    Y.pan0 = (int)npan;
+    Y.win0 = (int)nwin_all;
+    Y.seg0 = (int)nseg;
+    Y.r2o = r2tot;
+    Y.r3o = r3tot;
+    npan += P.panels.size();
+    nwin_all += P.wins.size();
+    nseg += P.segs.size();
+    r2tot += P.n2;
+    r3tot += P.n3;
+  }
+  pan.resize(npan);
+  win.resize(nwin_all);
+  seg.resize(nseg);
+  // 3. each job's device record and table rows, in parallel
+  std::vector<int64_t> jline(njobs, 0);
+  parallel_for(njobs, [&](int jb) {
+    const lblx_layer *ly = &layers[jb];
+    const LayerPlan &P = *plans[jb];
+    DevLayer &Y = ys[jb];
     Y.dv = P.dv;
     Y.dvr2 = P.dvr2;
     Y.dvr3 = P.dvr3;
@@ -645,36 +686,35 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
     Y.v2abs = P.cont.v2abs;
     Y.dvabs = P.cont.dvabs;
     Y.nptabs = P.cont.nptabs;
-    if (P.cont.on) abs_stride = std::max(abs_stride, P.cont.nptabs + 4);
-    Y.pan0 = (int)pan.size();
-    Y.win0 = (int)win.size();
-    Y.seg0 = (int)seg.size();
-    for (const PanelPlan &q : P.panels)
-      pan.push_back(DevPanel{q.vft, q.w, q.nhi, q.istop, q.seg0, q.nseg, 0, q.out0});
-    for (const WinPlan &w : P.wins)
-      win.push_back(DevWin{w.v1r4, w.v2r4, w.nptr4, Y.seg0 + w.seg0, w.nseg, jb});
-    for (const RadSeg &g : P.segs) seg.push_back(DevSeg{g.i1, g.i2, g.radvi, g.rdel});
+    DevPanel *dp = pan.data() + Y.pan0;
+    for (const PanelPlan &q : P.panels) *dp++ = DevPanel{q.vft, q.w, q.nhi, q.istop, q.seg0, q.nseg, 0, q.out0};
+    DevWin *dw = win.data() + Y.win0;
+    for (const WinPlan &w : P.wins) *dw++ = DevWin{w.v1r4, w.v2r4, w.nptr4, Y.seg0 + w.seg0, w.nseg, jb};
+    DevSeg *ds = seg.data() + Y.seg0;
+    for (const RadSeg &g : P.segs) *ds++ = DevSeg{g.i1, g.i2, g.radvi, g.rdel};
     JobLines JL;
-    maxline = std::max(maxline, job_lines(c, P, ly, &JL));
+    jline[jb] = job_lines(c, P, ly, &JL);
     Y.b4 = JL.b4;
     Y.nb4 = JL.nb4;
-    maxnb4 = std::max(maxnb4, JL.nb4);
     const int nlo = JL.nlo, nhi = JL.nlo + JL.nn - 1;
     Y.nlo = JL.nlo;
     Y.nn = JL.nn;
     Y.blk0 = Y.nn > 0 ? c->h_blk[nlo] : 0;
     Y.nbk = Y.nn > 0 ? c->h_blk[nhi] - Y.blk0 + 1 : 0;
     Y.od0 = od_off ? od_off[jb] : 0;
-    Y.r2o = r2tot;
-    Y.r3o = r3tot;
-    r2tot += P.n2;
-    r3tot += P.n3;
-    maxnn = std::max(maxnn, Y.nn);
-    maxnbk = std::max(maxnbk, Y.nbk);
     double shiftm = c->max_pshift * P.rhorat + 2. * P.dv;
     const double dvg[3] = {P.dv, P.dvr2, P.dvr3};
     for (int G = 1; G <= 3; G++)
       margins[3 * jb + G - 1] = (double)(4 * (P.nalfmax + 1) + 8) * dvg[G - 1] + shiftm;
+  });
+  int maxnn = 0, maxnbk = 0, abs_stride = 0, maxnb4 = 0;
+  int64_t maxline = 0;
+  for (int jb = 0; jb < njobs; jb++) {
+    maxline = std::max(maxline, jline[jb]);
+    maxnn = std::max(maxnn, ys[jb].nn);
+    maxnbk = std::max(maxnbk, ys[jb].nbk);
+    maxnb4 = std::max(maxnb4, ys[jb].nb4);
+    if (plans[jb]->cont.on) abs_stride = std::max(abs_stride, plans[jb]->cont.nptabs + 4);
   }
   if (seg.empty()) seg.push_back(DevSeg{0, 0, 0., 0.});
   const int nwin = (int)win.size();
@@ -1276,13 +1316,15 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
   std::vector<LayerPlan> plans(nlay);
   std::vector<int64_t> off(nlay);
   int64_t tot = 0, nmax = 0;
-  int jr = layers[0].jrad;
+  {  // JRAD chained up the column, then the layer plans on host threads
+    std::vector<lblx_layer> tmp(layers, layers + nlay);
+    for (int l = 1; l < nlay; l++) tmp[l].jrad = layer_jrad(&tmp[l - 1]);
+    std::vector<int> prc(nlay, 0);
+    parallel_for(nlay, [&](int l) { prc[l] = plan_layer(run, &tmp[l], &plans[l]); });
+    for (int l = 0; l < nlay; l++)
+      if (prc[l]) return prc[l];
+  }
   for (int l = 0; l < nlay; l++) {
-    lblx_layer tmp = layers[l];
-    tmp.jrad = jr;
-    int rc = plan_layer(run, &tmp, &plans[l]);
-    if (rc) return rc;
-    jr = tmp.jrad;
     off[l] = tot;
     tot += plans[l].npts;
     nmax = std::max(nmax, plans[l].npts);
@@ -1472,13 +1514,15 @@ int lblx_column_od_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, 
   std::vector<LayerPlan> plans(nlay);
   std::vector<int64_t> off(nlay);
   int64_t tot = 0, nmax = 0;
-  int jr = layers[0].jrad;
+  {  // JRAD chained up the column, then the layer plans on host threads
+    std::vector<lblx_layer> tmp(layers, layers + nlay);
+    for (int l = 1; l < nlay; l++) tmp[l].jrad = layer_jrad(&tmp[l - 1]);
+    std::vector<int> prc(nlay, 0);
+    parallel_for(nlay, [&](int l) { prc[l] = plan_layer(run, &tmp[l], &plans[l]); });
+    for (int l = 0; l < nlay; l++)
+      if (prc[l]) return prc[l];
+  }
   for (int l = 0; l < nlay; l++) {
-    lblx_layer tmp = layers[l];
-    tmp.jrad = jr;
-    int rc = plan_layer(run, &tmp, &plans[l]);
-    if (rc) return rc;
-    jr = tmp.jrad;
     off[l] = tot;
     tot += plans[l].npts;
     nmax = std::max(nmax, plans[l].npts);
@@ -1778,17 +1822,17 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
   std::vector<lblx_layer> jobs;
   std::vector<int> icflg;
   jobs.reserve((size_t)nlay * (1 + P));
-  int jr = layers[0].jrad;
-  for (int l = 0; l < nlay; l++) {
-    lblx_layer tmp = layers[l];
-    tmp.jrad = jr;
-    int rc = plan_layer(run, &tmp, &plans[l]);
-    if (rc) return rc;
-    layers[l].jrad = tmp.jrad;
-    tmp.jrad = jr;  // the job plans again from its input flag
-    jr = layers[l].jrad;
-    jobs.push_back(tmp);
-    icflg.push_back(0);
+  {  // JRAD chained up the column (each job plans again from its input flag)
+    std::vector<lblx_layer> tmp(layers, layers + nlay);
+    for (int l = 1; l < nlay; l++) tmp[l].jrad = layer_jrad(&tmp[l - 1]);
+    jobs.assign(tmp.begin(), tmp.end());
+    icflg.assign(nlay, 0);
+    std::vector<int> prc(nlay, 0);
+    parallel_for(nlay, [&](int l) { prc[l] = plan_layer(run, &tmp[l], &plans[l]); });
+    for (int l = 0; l < nlay; l++) {
+      if (prc[l]) return prc[l];
+      layers[l].jrad = tmp[l].jrad;
+    }
   }
   for (int l = 0; l < nlay; l++)
     for (int q = 0; q < P; q++) {
@@ -1815,10 +1859,14 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
   const int nj = (int)jobs.size();
   // every job's plan (the species runs with their ICFLG)
   std::vector<LayerPlan> jplans(nj);
-  for (int j = 0; j < nj; j++) {
-    lblx_layer t = jobs[j];
-    int rc = plan_layer(run, &t, &jplans[j], icflg[j]);
-    if (rc) return rc;
+  {
+    std::vector<int> prc(nj, 0);
+    parallel_for(nj, [&](int j) {
+      lblx_layer t = jobs[j];
+      prc[j] = plan_layer(run, &t, &jplans[j], icflg[j]);
+    });
+    for (int j = 0; j < nj; j++)
+      if (prc[j]) return prc[j];
   }
   std::vector<int64_t> off(nj);
   int64_t tot = 0, nmax = 0, tot_base = 0;
@@ -1874,14 +1922,12 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
   HIPCHK(hipMemcpyAsync(d_rad, rall + off[nlay - 1], (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
   // 3. sources and tangent propagation.  The layers at the top whose merge
   // is the identity map (ITYPE 0, same panels: J <- T_k J pointwise) are
-  // handled from the top down with a running transmittance product; below
-  // them every tangent is carried layer by layer.
+  // handled from the top down with a running transmittance product (K0:
+  // the lowest of them; merge K0 - 1 is a real one).
   int K0 = nlay;
   for (int l = nlay - 1; l >= 1; l--) {
-    bool ident = itype[l] == 0 && plans[l].npts == plans[l - 1].npts;
-    for (size_t p = 0; ident && p < plans[l].panels.size(); p++)
-      ident = pan[pan0[l] + p].out0 == (int64_t)2400 * (int64_t)p;
-    if (!ident) break;
+    if (!merge_is_identity(plans[l - 1], plans[l], itype[l], pan.data() + pan0[l], pan.data() + pan0[l - 1]))
+      break;
     K0 = l;
   }
   int th = c->tbegin(ST_MERGE);
@@ -1909,41 +1955,76 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
       hipLaunchKernelGGL(k_jac_src<double>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const double *)od,
                          (const double *)od, rold, dst, scale);
   };
-  for (int l = 0; l < K0; l++) {
+  const dim3 gtop((unsigned)((ntop + 255) / 256));
+  const int64_t qstr = (int64_t)nlay * n_out;  // between parameter planes of d_jac
+  double *ttop = c->w_rt.as<double>() + 3 * nmax;  // transmittance product of the top region
+  double *st = c->w_rt.as<double>() + 2 * nmax;  // pending product of a run of identity merges
+  auto trans_launch = [&](int l, double *prod, int64_t n) {
+    const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (f32) hipLaunchKernelGGL(k_jac_trans<float>, g, dim3(256), 0, s, (const float *)od, n, prod);
+    else hipLaunchKernelGGL(k_jac_trans<double>, g, dim3(256), 0, s, (const double *)od, n, prod);
+  };
+  // the top region first (top down, its sources scaled by the product of
+  // the transmittances above them), so that its product is known below
+  if (K0 < nlay) {
+    hipLaunchKernelGGL(k_fill_f64, gtop, dim3(256), 0, s, ttop, ntop, 1.0);
+    for (int l = nlay - 1; l >= K0; l--) {
+      src_launch(l, d_jac + (int64_t)l * n_out, qstr, ttop);
+      trans_launch(l, ttop, ntop);
+    }
+  }
+  // Below it, layer by layer up to K0 - 1.  A run of layers a..b whose
+  // merges a+1..b are identity maps is taken at once: the sources of its
+  // layers j > a are scaled by the product of the run's transmittances
+  // above j, and the tangents from layers <= a carry the whole run's
+  // product into the next real merge (st, applied at the taps).  The last
+  // layer (K0 - 1) writes into d_jac scaled by the top region's product.
+  int nsc = 0;
+  for (int l = 0; l < K0;) {
+    const bool to_out = K0 < nlay && l == K0 - 1;
     if (l > 0) {
       MergeArgs A = merge_args(plans[l], &plans[l - 1], itype[l], layers[l].tave, tzu[l], 0., emis);
       const dim3 grid((unsigned)((A.n + 255) / 256));
       const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+      JacOut O{};
+      if (to_out) O = JacOut{d_jac, n_out, qstr, P, ttop};
+      else O = JacOut{jb, (int64_t)P * plane, plane, P, nullptr};
+      const double *sp = nsc > 0 ? st : nullptr;
       if (f32)
         hipLaunchKernelGGL(k_jac_prop<float>, grid, dim3(256), 0, s, A, dpan + pan0[l], (const float *)od,
-                           (const double *)ja, plane, jb, plane, l * P);
+                           (const double *)ja, plane, l * P, sp, nsc, O);
       else
         hipLaunchKernelGGL(k_jac_prop<double>, grid, dim3(256), 0, s, A, dpan + pan0[l], (const double *)od,
-                           (const double *)ja, plane, jb, plane, l * P);
+                           (const double *)ja, plane, l * P, sp, nsc, O);
+      nsc = 0;
       std::swap(ja, jb);
     }
-    src_launch(l, ja + (int64_t)l * P * plane, plane, nullptr);
-  }
-  const dim3 gtop((unsigned)((ntop + 255) / 256));
-  const int64_t qstr = (int64_t)nlay * n_out;  // between parameter planes of d_jac
-  if (K0 < nlay) {
-    double *tt = jb;  // running product of the transmittances above
-    hipLaunchKernelGGL(k_fill_f64, gtop, dim3(256), 0, s, tt, ntop, 1.0);
-    for (int l = nlay - 1; l >= K0; l--) {
-      src_launch(l, d_jac + (int64_t)l * n_out, qstr, tt);
-      const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
-      if (f32) hipLaunchKernelGGL(k_jac_trans<float>, gtop, dim3(256), 0, s, (const float *)od, ntop, tt);
-      else hipLaunchKernelGGL(k_jac_trans<double>, gtop, dim3(256), 0, s, (const double *)od, ntop, tt);
+    if (to_out) {
+      src_launch(l, d_jac + (int64_t)l * n_out, qstr, ttop);
+      break;
     }
-    // layers below the identity region: their tangents on grid K0-1 times
-    // the product of every transmittance above
-    for (int l = 0; l < K0; l++)
-      for (int q = 0; q < P; q++)
-        hipLaunchKernelGGL(k_jac_scale, gtop, dim3(256), 0, s, (const double *)(ja + ((int64_t)l * P + q) * plane),
-                           plane, (const double *)tt, ntop, d_jac + (int64_t)q * qstr + (int64_t)l * n_out,
-                           n_out, 1);
-  } else {
-    // 4. planes (layer l, parameter q) -> d_jac[(q * nlay + l) * n_out + g]
+    src_launch(l, ja + (int64_t)l * P * plane, plane, nullptr);
+    int b = l;
+    while (b + 1 < K0 && merge_is_identity(plans[b], plans[b + 1], itype[b + 1], pan.data() + pan0[b + 1],
+                                           pan.data() + pan0[b]))
+      b++;
+    if (b > l) {
+      const int64_t ng = plans[l].npts;
+      hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, st, ng, 1.0);
+      for (int j = b; j > l; j--) {
+        src_launch(j, ja + (int64_t)j * P * plane, plane, st);
+        trans_launch(j, st, ng);
+      }
+      nsc = (l + 1) * P;
+    }
+    l = b + 1;
+  }
+  if (K0 == nlay) {
+    // no identity region at the top: a pending product still applies to
+    // the tangents from below its run; the planes go to d_jac as they are
+    if (nsc > 0)
+      hipLaunchKernelGGL(k_jac_mul, gtop, dim3(256), 0, s, ja, plane, nsc, (const double *)st, ntop);
     for (int l = 0; l < nlay; l++)
       for (int q = 0; q < P; q++)
         HIPCHK(hipMemcpyAsync(d_jac + (int64_t)q * qstr + (int64_t)l * n_out, ja + ((int64_t)l * P + q) * plane,

@@ -1697,22 +1697,54 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
 }
 
 // One layer of tangent propagation for nv vectors (plane stride istr on
-// the old grid, ostr on the new one): J_new = T_k L4(J_old).
+// the old grid): J_new = T_k L4(S J_old), where S (on the old grid) is the
+// pending transmittance product of the identity merges just below, for the
+// first nsc vectors only (S null: none).  Vector v = l P + q is written at
+// jout + l lstr + q qstr, times oscale (on the new grid) when given.
+struct JacOut {
+  double *jout;
+  int64_t lstr, qstr;
+  int32_t P;
+  const double *oscale;
+};
+
 template <typename IN>
 __global__ void k_jac_prop(MergeArgs A, const DevPanel *pan, const IN *od, const double *jin,
-                           int64_t istr, double *jout, int64_t ostr, int nv) {
+                           int64_t istr, int nv, const double *st, int nsc, JacOut O) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.n) return;
   const int p = (int)(g / 2400);
   const int k = (int)(g - (int64_t)2400 * p);
   double odvi = (double)od[g];
   if (odvi < -0.06) odvi = -0.06;
-  const double tr = exp(-odvi);
+  double tr = exp(-odvi);
+  if (O.oscale) tr = tr * O.oscale[g];
   MergeTap t = merge_tap(A, pan[p].out0, k);
+  double s0 = 1., s1 = 1., s2 = 1., s3 = 1.;
+  if (st) {
+    s0 = st[t.i0];
+    s1 = st[t.i1];
+    s2 = st[t.i2];
+    s3 = st[t.i3];
+  }
   for (int v = 0; v < nv; v++) {
     const double *a = jin + (int64_t)v * istr;
-    jout[(int64_t)v * ostr + g] = tr * (t.w0 * a[t.i0] + t.w1 * a[t.i1] + t.w2 * a[t.i2] + t.w3 * a[t.i3]);
+    double x;
+    if (v < nsc)
+      x = t.w0 * (s0 * a[t.i0]) + t.w1 * (s1 * a[t.i1]) + t.w2 * (s2 * a[t.i2]) + t.w3 * (s3 * a[t.i3]);
+    else
+      x = t.w0 * a[t.i0] + t.w1 * a[t.i1] + t.w2 * a[t.i2] + t.w3 * a[t.i3];
+    const int l = v / O.P, q = v - l * O.P;
+    O.jout[(int64_t)l * O.lstr + (int64_t)q * O.qstr + g] = tr * x;
   }
+}
+
+// planes[v][g] *= s[g] for the first nv planes (stride istr)
+__global__ void k_jac_mul(double *planes, int64_t istr, int nv, const double *s, int64_t n) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const double f = s[g];
+  for (int v = 0; v < nv; v++) planes[(int64_t)v * istr + g] *= f;
 }
 
 

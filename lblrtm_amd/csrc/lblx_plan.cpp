@@ -330,6 +330,14 @@ static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPl
   return LBLX_OK;
 }
 
+// JRAD after OPDPTH's test (lblrtm.f90:6202-6204): the radiation term is
+// folded into the optical depth (JRAD = 0) when V1 / XKT < 5, unless the
+// caller asked for none (-1).  It becomes the next layer's JRAD.
+int layer_jrad(const lblx_layer *Ly) {
+  const double xkt_l = Ly->tave / RADCN2;
+  return ((Ly->v1 / xkt_l) < 5. && Ly->jrad != -1) ? 0 : Ly->jrad;
+}
+
 int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   if (!run || !Ly || !P) return LBLX_E_ARG;
   if (Ly->nmol < 1 || Ly->nmol > 7) return LBLX_E_UNSUPPORTED;
@@ -343,7 +351,7 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   P->dv = DV;
   // OPDPTH (lblrtm.f90:6202-6256)
   double xkt_l = Ly->tave / RADCN2;
-  if ((V1 / xkt_l) < 5. && Ly->jrad != -1) Ly->jrad = 0;
+  Ly->jrad = layer_jrad(Ly);
   P->jrad = Ly->jrad;
   P->xkt = xkt_l;
   P->tave = Ly->tave;

@@ -129,6 +129,8 @@ int plan_scan(const lblx_scan *scan, double v1c, double dv, int64_t n_in, ScanPl
 // /CDERIV/ ICFLG, contnm.f90:221-239, 314-322): no N2 continuum, and the
 // H2O continuum carries its ln q derivative term.
 int plan_layer(const lblx_run *run, lblx_layer *layer, LayerPlan *P, int icflg = 0);
+// the layer's JRAD as plan_layer leaves it (the next layer's JRAD in a column)
+int layer_jrad(const lblx_layer *layer);
 
 // PATH DV selection for a column (lblrtm.f90:5336-5500).
 int plan_dv(const lblx_run *run, int nlay, int nmol, const double *pave,
