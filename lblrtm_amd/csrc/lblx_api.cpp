@@ -793,8 +793,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_cal, NJ * S.line * 8) || need(c->w_cad, NJ * S.line * 8) ||
       need(c->w_cco2, NJ * S.line) || need(c->w_pn, NJ * S.line * 4) ||
       need(c->w_pc, NJ * S.line * 4) || need(c->w_g, NJ * S.line * 4) ||
-      need(c->w_jmp, NJ * S.line * 4 * NLEV) || need(c->w_wsum, NJ * S.line * 2 * NLEV) ||
-      need(c->w_bcnt, NJ * S.blk * 4 * 7) || need(c->w_etab, NJ * S.blk * 4 * EBLK) ||
+      need(c->w_jmp, NJ * S.line * 4 * 3) || need(c->w_wsum, NJ * S.line * 2) ||
+      need(c->w_bcnt, NJ * S.blk * 4 * 7) || need(c->w_etab, NJ * S.blk * 4 * (EBLK + 1)) ||
       need(c->w_jobi, NJ * 4 * 4) || need(c->w_bat, NJ * S.line * 4 * 3) ||
       need(c->w_t9vnu, NJ * S.line * 8) || need(c->w_t9sp, NJ * S.line * 8) ||
       need(c->w_t9alfa, NJ * S.line * 8) || need(c->w_t9epp, NJ * S.line * 8) ||
@@ -853,6 +853,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   W.pc = c->w_pc.as<int32_t>();
   W.g = c->w_g.as<int32_t>();
   W.jmp = c->w_jmp.as<int32_t>();
+  W.grp = W.jmp + NJ * S.line;
+  W.gout = W.jmp + 2 * NJ * S.line;
   W.wsum = c->w_wsum.as<int16_t>();
   int32_t *bb = c->w_bcnt.as<int32_t>();
   W.bcnt = bb;
@@ -867,6 +869,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   W.stop = ji + NJ;
   W.nbat = ji + 2 * NJ;
   W.etab = c->w_etab.as<int32_t>();
+  W.estop = W.etab + NJ * S.blk * EBLK;
   int32_t *bat = c->w_bat.as<int32_t>();
   W.bat_s = bat;
   W.bat_t = bat + NJ * S.line;
@@ -893,11 +896,12 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   hipLaunchKernelGGL(k_shr_scan, dim3(njobs), dim3(256), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_compact, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
   hipLaunchKernelGGL(k_shr_chain, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W);
-  for (int lev = 1; lev < NLEV; lev++)
-    hipLaunchKernelGGL(k_shr_lift, dim3(lgrid, njobs), dim3(256), 0, s, lev, S, W);
-  hipLaunchKernelGGL(k_shr_etab, dim3((bgrid * EBLK + 255) / 256, njobs), dim3(256), 0, s, dY, S, W, njobs);
-  hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), 0, s, dY, S, W, T9, njobs);
-  hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9, njobs);
+  hipLaunchKernelGGL(k_shr_ewalk, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W);
+  hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), 0, s, dY, S, W, T9);
+  hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.line + 255) / 256)), dim3(256), 0, s, W.grp, NJ * S.line,
+                     -1);
+  hipLaunchKernelGGL(k_shr_bwalk, dim3(bgrid, njobs), dim3(64), 0, s, S, W);
+  hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9);
   hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);
   c->tend(th);
   // 3. far-wing windows
@@ -2056,8 +2060,8 @@ int lblx_jacobian_aj_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers
 // about nline records.
 static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
   const size_t line = ((size_t)nline + 127) / 64 * 64;
-  return line * (32 + 1 + 12 + 4 * NLEV + 2 * NLEV + 12 + 36 + sizeof(NearLine) + 8 + sizeof(ScatterRec)) +
-         (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 3) + (size_t)(c->nlines / 1000 + 16) * 24;
+  return line * (32 + 1 + 12 + 4 * 3 + 2 + 12 + 36 + sizeof(NearLine) + 8 + sizeof(ScatterRec)) +
+         (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 1 + 3) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
 
 // Many profiles in one call: OPDPTH for the layers of a group of profiles

@@ -6,8 +6,7 @@
 
 namespace lblx {
 
-constexpr int NLEV = 11;   // pointer-doubling levels: SHRINK batches < 2^11 lines
-constexpr int EBLK = 8;    // batch lengths (in TAPE3 blocks) precomputed per block
+constexpr int EBLK = 64;   // batch lengths (in TAPE3 blocks) tabled per block (a batch holds < LIMIN4 + one block of passing lines)
 constexpr int R4ROW = 2504;
 
 // Resident line table (TAPE3 records in file order, struct-of-arrays).
@@ -77,12 +76,14 @@ struct ShrinkWork {
   int8_t *cco2;
   int32_t *pn, *pc;       // [n+1] exclusive prefix of positive non-CO2 / CO2 lines
   int32_t *g;             // first-group end from each start
-  int32_t *jmp;           // [NLEV][line] chain doubling
-  int16_t *wsum;          // [NLEV][line] emitted lines over the doubled chain
+  int32_t *jmp;           // [line] f: next group end after a group end
+  int16_t *wsum;          // [line] lines the group after a group end emits
+  int32_t *grp, *gout;    // [line] at a group's last line: its first line, first output slot (else -1)
   int32_t *bcnt, *bcntn, *bcntc, *bstop;  // per TAPE3 block
   int32_t *bp, *bpn, *bpc;                // block prefixes
   int32_t *npass, *stop;                  // per job
   int32_t *etab;          // [blk][EBLK] emitted lines of a batch of k+1 blocks
+  int32_t *estop;         // [blk] emitted lines of the batch from the block to the stop line (-1: not tabled)
   int32_t *bat_s, *bat_t, *bat_off, *nbat;  // batches
 };
 

@@ -124,10 +124,13 @@ __device__ __forceinline__ bool linf4_line(const DevLines &L, const DevLayer &Y,
 //     f(e) = (first k >= e+2 with x_k > x_e + DVR4/2) - 1,
 //     g(s) = (first k >= s+1 with x_k > x_s + DVR4/2) - 1,
 // cut at the batch end.  A group emits one non-CO2 and one CO2 effective
-// line if it holds positive strength of that class.  The chain is doubled
-// (jmp, wsum) so that the number of lines a batch emits, and the group of
-// any line, are O(log) queries; the batch boundaries, which depend on
-// those counts, are walked by one thread per job over per-block tables.
+// line if it holds positive strength of that class.  f and the count each
+// step emits (jmp, wsum) are tabled per line; a thread per TAPE3 block walks
+// the chain from the block's first line through the next EBLK blocks and
+// tables the count a batch of those blocks would emit; one thread per job
+// then walks LINF4's batch boundaries over those tables, and one thread per
+// batch walks its chain once more to mark each group's last line with the
+// group's first line and output slot.
 constexpr int SB_T = 256;
 
 __device__ __forceinline__ int em_cnt(const int32_t *pn, const int32_t *pc, int a, int b) {
@@ -285,73 +288,101 @@ __global__ void k_shr_chain(const DevLayer *Ys, Strides S, ShrinkWork W) {
   W.wsum[o + e] = (int16_t)em_cnt(W.pn + o, W.pc + o, e + 1, f);
 }
 
-// doubling level lev from lev-1
-__global__ void k_shr_lift(int lev, Strides S, ShrinkWork W) {
-  const int job = blockIdx.y;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = W.npass[job];
-  if (e > n) return;
-  const int64_t o = job * S.line;
-  const int32_t *j0 = W.jmp + (int64_t)(lev - 1) * S.line * gridDim.y + o;
-  const int16_t *w0 = W.wsum + (int64_t)(lev - 1) * S.line * gridDim.y + o;
-  int32_t *j1 = W.jmp + (int64_t)lev * S.line * gridDim.y + o;
-  int16_t *w1 = W.wsum + (int64_t)lev * S.line * gridDim.y + o;
-  int a = j0[e];
-  j1[e] = j0[a];
-  int w = (int)w0[e] + (int)w0[a];
-  w1[e] = (int16_t)(w > 32767 ? 32767 : w);
-}
-
-// Lines emitted by SHRINK for the batch [s, t] (passing indices); with
-// *last_node the last chain node below t (or s-1 when the first group is
-// cut) and *acc_before its emitted-line count.
-__device__ int shr_query(const ShrinkWork &W, int64_t o, int64_t lstride, int s, int t, int stop_at,
-                         int *cur_out, int *acc_out) {
-  const int32_t *pn = W.pn + o, *pc = W.pc + o;
-  int e1 = W.g[o + s];
-  if (e1 >= t || e1 >= stop_at) {
-    if (cur_out) *cur_out = s - 1;
-    if (acc_out) *acc_out = 0;
-    return em_cnt(pn, pc, s, t);
+// Lines SHRINK emits for the batch [s, t] of passing indices: the first
+// group ends at g(s) (or is cut at t), then the chain e -> f(e) while it
+// stays below t, and the last group is cut at t.
+// One wave walks the chain: the lanes hold f and the emitted count of 64
+// consecutive lines from the window base, so a step is a lane read and the
+// line table is fetched once per 64 lines.  Every value but the window is
+// wave-uniform; all lanes of the wave call it together.
+struct ChainWalk {
+  const int32_t *pn, *pc, *jmp;
+  const int16_t *wsum;
+  int lim, s, e1, cur, acc, base, wj, ww;
+  __device__ void load(int b) {
+    base = b;
+    const int i = b + (int)(threadIdx.x & 63);
+    wj = i < lim ? jmp[i] : INT_MAX;
+    ww = i < lim ? (int)wsum[i] : 0;
   }
-  int acc = em_cnt(pn, pc, s, e1);
-  int cur = e1;
-  for (int lev = NLEV - 1; lev >= 0; lev--) {
-    int nx = W.jmp[(int64_t)lev * lstride + o + cur];
-    if (nx < t && nx <= stop_at) {
-      acc += W.wsum[(int64_t)lev * lstride + o + cur];
+  __device__ ChainWalk(const ShrinkWork &W, int64_t o, int n, int s_) {
+    pn = W.pn + o;
+    pc = W.pc + o;
+    jmp = W.jmp + o;
+    wsum = W.wsum + o;
+    lim = n + 1;
+    s = s_;
+    e1 = __builtin_amdgcn_readfirstlane(W.g[o + s]);
+    cur = e1;
+    acc = em_cnt(pn, pc, s, e1);
+    load(cur);
+  }
+  // f(cur) and the count the group after cur emits
+  __device__ int next(int *w) {
+    if (cur >= base + 64) load(cur);
+    const int k = __builtin_amdgcn_readfirstlane(cur - base);
+    *w = __builtin_amdgcn_readlane(ww, k);
+    return __builtin_amdgcn_readlane(wj, k);
+  }
+  // t never decreases from one call to the next
+  __device__ int count_to(int t) {
+    if (e1 >= t) return em_cnt(pn, pc, s, t);
+    for (;;) {
+      int w;
+      const int nx = next(&w);
+      if (nx >= t) break;
+      acc += w;
       cur = nx;
     }
+    return acc + em_cnt(pn, pc, cur + 1, t);
   }
-  if (cur_out) *cur_out = cur;
-  if (acc_out) *acc_out = acc;
-  return acc + em_cnt(pn, pc, cur + 1, t);
-}
+};
 
-// per (job, block b, k): lines emitted by a batch of blocks b..b+k
-// (block indices relative to the job's first far-wing block)
-__global__ void k_shr_etab(const DevLayer *Ys, Strides S, ShrinkWork W, int njobs) {
+// per (job, block b): lines emitted by a batch of blocks b..b+k for k <
+// EBLK (block indices relative to the job's first far-wing block), and by
+// the batch from b to the stop line when that lies within those blocks
+// (-1 otherwise).  A batch ends at the first block that brings its passing
+// lines to LIMIN4 or more, so the table stops there (-1 beyond).
+__global__ __launch_bounds__(64) void k_shr_ewalk(const DevLayer *Ys, Strides S, ShrinkWork W) {
   const int job = blockIdx.y;
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.x;  // one wave per block
+  const int lane = threadIdx.x;
   const int nblk = Ys[job].nb4;
-  if (id >= nblk * EBLK) return;
-  const int b = id / EBLK, k = id % EBLK;
+  if (b >= nblk) return;
   const int64_t ob = job * S.blk;
-  int v = 0;
-  if (b + k + 1 <= nblk) {
-    int s = W.bp[ob + b], t = W.bp[ob + b + k + 1] - 1;
-    if (t >= s) v = shr_query(W, job * S.line, S.line * njobs, s, t, INT_MAX, nullptr, nullptr);
+  const int s = W.bp[ob + b];
+  const int stop_p = W.stop[job];
+  int32_t *et = W.etab + (ob + b) * EBLK;
+  if (W.bp[ob + nblk] <= s) {  // no passing line from here on
+    for (int k = lane; k < EBLK; k += 64) et[k] = 0;
+    if (lane == 0) W.estop[ob + b] = -1;
+    return;
   }
-  W.etab[ob * EBLK + id] = v;
+  ChainWalk cw(W, job * S.line, W.npass[job], s);
+  int es = -1, mine = -1;  // lane k keeps entry k
+  int k = 0;
+  for (; k < EBLK; k++) {
+    if (b + k + 1 > nblk) {
+      if (lane == k) mine = 0;
+      continue;
+    }
+    const int t = W.bp[ob + b + k + 1] - 1;
+    if (es < 0 && stop_p >= s && stop_p <= t) es = cw.count_to(stop_p);
+    const int v = t >= s ? cw.count_to(t) : 0;
+    if (lane == k) mine = v;
+    if (t + 1 - s >= LIMIN4) break;
+  }
+  if (lane < EBLK) et[lane] = mine;
+  if (lane == 0) W.estop[ob + b] = es;
 }
 
 // one thread per job: LINF4's read loop over blocks (oprop.f90:3130-3290)
 // deciding batch boundaries and TAPE9 blocks from the emitted counts.
 // Blocks run from the job's first far-wing block; its last one holds the
 // stop line unless the file ends first.
-__global__ void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9, int njobs) {
+__global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9) {
   const int job = blockIdx.x;
-  if (threadIdx.x != 0) return;
+  const bool lead = threadIdx.x == 0;  // the wave walks together; lane 0 writes
   const int64_t ob = job * S.blk;
   const int64_t o = job * S.line;
   const int64_t ot = job * S.t9blk;
@@ -381,19 +412,25 @@ __global__ void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9
     int em = 0;
     if (t >= s) {
       int nb = b - bs;
-      if (!stop && nb >= 1 && nb <= EBLK) em = W.etab[(ob + bs) * EBLK + nb - 1];
-      else em = shr_query(W, o, S.line * njobs, s, t, INT_MAX, nullptr, nullptr);
-      W.bat_s[o + nbat] = s;
-      W.bat_t[o + nbat] = t;
-      W.bat_off[o + nbat] = out;
+      if (stop && nb <= EBLK && W.estop[ob + bs] >= 0) em = W.estop[ob + bs];
+      else if (!stop && nb >= 1 && nb <= EBLK && W.etab[(ob + bs) * EBLK + nb - 1] >= 0)
+        em = W.etab[(ob + bs) * EBLK + nb - 1];
+      else em = ChainWalk(W, o, W.npass[job], s).count_to(t);  // a batch longer than the tables
+      if (lead) {
+        W.bat_s[o + nbat] = s;
+        W.bat_t[o + nbat] = t;
+        W.bat_off[o + nbat] = out;
+      }
       nbat++;
     }
     out += em;
     J += em;
     if (J >= LIMIN4 || stop || eof) {
       if (J > 0) {
-        T9.bstart[ot + nt9] = t9s;
-        T9.bcount[ot + nt9] = J;
+        if (lead) {
+          T9.bstart[ot + nt9] = t9s;
+          T9.bcount[ot + nt9] = J;
+        }
         nt9++;
       }
       t9s = out;
@@ -401,55 +438,65 @@ __global__ void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9
     }
     if (stop || eof) break;
   }
-  W.nbat[job] = nbat;
-  T9.n[job] = out;
-  T9.nblk[job] = nt9;
+  if (lead) {
+    W.nbat[job] = nbat;
+    T9.n[job] = out;
+    T9.nblk[job] = nt9;
+  }
 }
 
-// per passing line: its group within its batch; the group's last line
-// writes the effective lines (strength-weighted centre and widths, sums in
-// line order as SHRINK does, oprop.f90:3560-3631)
-__global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9, int njobs) {
+// per (job, batch), one wave each: the batch's chain, marking the last line
+// of each group with the group's first line (grp) and its first output
+// slot (gout)
+__global__ __launch_bounds__(64) void k_shr_bwalk(Strides S, ShrinkWork W) {
+  const int job = blockIdx.y;
+  const int q = blockIdx.x;
+  if (q >= W.nbat[job]) return;
+  const bool lead = threadIdx.x == 0;
+  const int64_t o = job * S.line;
+  const int s = W.bat_s[o + q], t = W.bat_t[o + q], off = W.bat_off[o + q];
+  ChainWalk cw(W, o, W.npass[job], s);
+  if (cw.e1 >= t) {
+    if (lead) {
+      W.grp[o + t] = s;
+      W.gout[o + t] = off;
+    }
+    return;
+  }
+  if (lead) {
+    W.grp[o + cw.e1] = s;
+    W.gout[o + cw.e1] = off;
+  }
+  for (;;) {
+    int w;
+    const int nx = cw.next(&w);
+    if (nx >= t) break;
+    if (lead) {
+      W.grp[o + nx] = cw.cur + 1;
+      W.gout[o + nx] = off + cw.acc;
+    }
+    cw.acc += w;
+    cw.cur = nx;
+  }
+  if (lead) {
+    W.grp[o + t] = cw.cur + 1;
+    W.gout[o + t] = off + cw.acc;
+  }
+}
+
+// per passing line: a group's last line writes the group's effective lines
+// (strength-weighted centre and widths, sums in line order as SHRINK does,
+// oprop.f90:3560-3631)
+__global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9) {
   const int job = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nbat = W.nbat[job];
   const int64_t o = job * S.line;
   if (nbat == 0) return;
   if (i < W.bat_s[o] || i > W.bat_t[o + nbat - 1]) return;
-  int lo = 0, hi = nbat - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (W.bat_s[o + mid] <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  const int s = W.bat_s[o + lo], t = W.bat_t[o + lo], off = W.bat_off[o + lo];
-  if (i > t) return;
-  const int64_t ls = S.line * njobs;
-  int gs, ge, acc;
-  int e1 = W.g[o + s];
-  if (e1 >= t) {
-    gs = s;
-    ge = t;
-    acc = 0;
-  } else if (i <= e1) {
-    gs = s;
-    ge = e1;
-    acc = 0;
-  } else {
-    int cur = e1;
-    acc = em_cnt(W.pn + o, W.pc + o, s, e1);
-    for (int lev = NLEV - 1; lev >= 0; lev--) {
-      int nx = W.jmp[(int64_t)lev * ls + o + cur];
-      if (nx <= i - 1) {
-        acc += W.wsum[(int64_t)lev * ls + o + cur];
-        cur = nx;
-      }
-    }
-    gs = cur + 1;
-    ge = W.jmp[o + cur];
-    if (ge > t) ge = t;
-  }
-  if (ge != i) return;
+  const int gs = W.grp[o + i];
+  if (gs < 0) return;
+  const int ge = i;
   double sumv = 0., sums = 0., sumal = 0., sumad = 0.;
   double sumv2 = 0., sums2 = 0., sumal2 = 0., sumad2 = 0.;
   for (int k = gs; k <= ge; k++) {
@@ -466,7 +513,7 @@ __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9
       sumad2 = sumad2 + sp * W.cad[o + k];
     }
   }
-  int64_t q = o + off + acc;
+  int64_t q = o + W.gout[o + i];
   if (sums > 0.) {
     T9.vnu[q] = sumv / sums;
     T9.sp[q] = sums;
