@@ -55,6 +55,24 @@ def stage_bytes(stage, stats, sizes, nwin):
     return LINE_BYTES * 500000 * len(sizes)
 
 
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations of one far-wing point update in the reference's CONVF4
+# loop (oprop.f90:3989-4006): XJJ step, XM, XMSQ, ZVSQ, ALFLI2 + XMSQ and
+# the division, R4 + F4FN - SUBTR
+CF4_FLOPS_PER_UPDATE = 8
+
+
+def compute_view(stage, stats, per):
+    """The far-wing stage against the FP64 peak: it does no HBM-bound work
+    (its inputs are the shrunk lines; its output one R4 row per window)."""
+    if stage != "convf4":
+        return None
+    upd = sum(s["k3_updates"] for s in stats)
+    tf = CF4_FLOPS_PER_UPDATE * upd / (per[stage] * 1e-3) / 1e12
+    return {"bound": "fp64", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf / FP64_PEAK_TFLOPS, "updates": upd, "flops_per_update": CF4_FLOPS_PER_UPDATE}
+
+
 # stage -> kernel of the FP64 column (names as rocprofv3 lists them)
 STAGE_KERNEL = {
     "convf4": "k_convf4<double>",
@@ -303,7 +321,8 @@ def main():
                    "formulation": "reference analytic Jacobian: single-species OPDPTH runs, +1 K"}},
         "batch": None if batch is None else dict(batch, spectra_ratio_vs_fp64=batch["value"] * ms / 1000.0),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(dom)},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(dom),
+                     "compute": compute_view(dom, stats, per)},
         "cpu_baseline": None,
     }
     if not a.no_cpu:
