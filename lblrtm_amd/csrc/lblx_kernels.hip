@@ -783,16 +783,24 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       const int lastund = undm ? 63 - __builtin_clzll(undm) : -1;
       const uint64_t head = lastund < 0 ? 0ull : (lastund == 63 ? ~0ull : ((1ull << (lastund + 1)) - 1));
       uint64_t todo = __ballot(cand && !surely_rej) & head;
+      // a surely accepted line's decision is known without the running R4,
+      // so its step does not wait on the previous line's contribution
+      const uint64_t sacc_m = __ballot(surely_acc);
+      const uint64_t past_m = __ballot(vnui >= vnulst);
       const double xjj_me = (double)(jj - 1) * dvr4;
       int nacc = 0;
       int done = 0;
       while (todo) {
         const int j = __builtin_ctzll(todo);
         todo &= todo - 1;
-        bool acc = cand && !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial));
-        bool stp = acc && (vnui >= vnulst);
-        const int accj = __builtin_amdgcn_readlane(acc ? 1 : 0, j);
-        const int stpj = __builtin_amdgcn_readlane(stp ? 1 : 0, j);
+        int accj;
+        if ((sacc_m >> j) & 1ull) {
+          accj = 1;
+        } else {
+          bool acc = cand && !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial));
+          accj = __builtin_amdgcn_readlane(acc ? 1 : 0, j);
+        }
+        const int stpj = accj && ((past_m >> j) & 1ull);
         if (stpj) {
           done = 1;
           break;
