@@ -17,6 +17,7 @@
 #include <string>
 #include <thread>
 #include <vector>
+#include <type_traits>
 
 #include "../../include/lblx.h"
 #include "lblx_kernels.h"
@@ -1899,7 +1900,7 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
   const bool f32 = run->precision == LBLX_FP32_SPLIT;
   const size_t esz = f32 ? 4 : 8;
   const int64_t plane = nmax;
-  const size_t jbytes = (size_t)P * nlay * plane * 8;
+  const size_t jbytes = (size_t)P * nlay * plane * 4;  // FP32 tangent planes
   if (c->w_od.ensure((size_t)tot * esz) || c->w_rt.ensure((size_t)nmax * 8 * 4) ||
       c->w_mpan.ensure(pan.size() * sizeof(DevPanel)) || c->j_rall.ensure((size_t)tot_base * 8) ||
       c->j_a.ensure(jbytes) || c->j_b.ensure(jbytes))
@@ -1935,8 +1936,9 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
     K0 = l;
   }
   int th = c->tbegin(ST_MERGE);
-  double *ja = c->j_a.as<double>(), *jb = c->j_b.as<double>();
-  auto src_launch = [&](int l, double *dst, int64_t stride, const double *scale) {
+  float *ja = c->j_a.as<float>(), *jb = c->j_b.as<float>();
+  auto src_launch = [&](int l, auto *dst, int64_t stride, const double *scale) {
+    using JP = std::remove_pointer_t<decltype(dst)>;
     MergeArgs A = merge_args(plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
                              l == 0 ? tbnd : 0., emis);
     const dim3 grid((unsigned)((A.n + 255) / 256));
@@ -1953,10 +1955,10 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
     }
     const double *rold = l > 0 ? rall + off[l - 1] : nullptr;
     if (f32)
-      hipLaunchKernelGGL(k_jac_src<float>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const float *)od,
+      hipLaunchKernelGGL((k_jac_src<float, JP>), grid, dim3(256), 0, s, S, dpan + pan0[l], (const float *)od,
                          (const float *)od, rold, dst, scale);
     else
-      hipLaunchKernelGGL(k_jac_src<double>, grid, dim3(256), 0, s, S, dpan + pan0[l], (const double *)od,
+      hipLaunchKernelGGL((k_jac_src<double, JP>), grid, dim3(256), 0, s, S, dpan + pan0[l], (const double *)od,
                          (const double *)od, rold, dst, scale);
   };
   const dim3 gtop((unsigned)((ntop + 255) / 256));
@@ -1991,16 +1993,24 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
       MergeArgs A = merge_args(plans[l], &plans[l - 1], itype[l], layers[l].tave, tzu[l], 0., emis);
       const dim3 grid((unsigned)((A.n + 255) / 256));
       const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
-      JacOut O{};
-      if (to_out) O = JacOut{d_jac, n_out, qstr, P, ttop};
-      else O = JacOut{jb, (int64_t)P * plane, plane, P, nullptr};
       const double *sp = nsc > 0 ? st : nullptr;
-      if (f32)
-        hipLaunchKernelGGL(k_jac_prop<float>, grid, dim3(256), 0, s, A, dpan + pan0[l], (const float *)od,
-                           (const double *)ja, plane, l * P, sp, nsc, O);
-      else
-        hipLaunchKernelGGL(k_jac_prop<double>, grid, dim3(256), 0, s, A, dpan + pan0[l], (const double *)od,
-                           (const double *)ja, plane, l * P, sp, nsc, O);
+      if (to_out) {
+        JacOut<double> O{d_jac, n_out, qstr, P, ttop};
+        if (f32)
+          hipLaunchKernelGGL((k_jac_prop<float, double>), grid, dim3(256), 0, s, A, dpan + pan0[l],
+                             (const float *)od, (const float *)ja, plane, l * P, sp, nsc, O);
+        else
+          hipLaunchKernelGGL((k_jac_prop<double, double>), grid, dim3(256), 0, s, A, dpan + pan0[l],
+                             (const double *)od, (const float *)ja, plane, l * P, sp, nsc, O);
+      } else {
+        JacOut<float> O{jb, (int64_t)P * plane, plane, P, nullptr};
+        if (f32)
+          hipLaunchKernelGGL((k_jac_prop<float, float>), grid, dim3(256), 0, s, A, dpan + pan0[l],
+                             (const float *)od, (const float *)ja, plane, l * P, sp, nsc, O);
+        else
+          hipLaunchKernelGGL((k_jac_prop<double, float>), grid, dim3(256), 0, s, A, dpan + pan0[l],
+                             (const double *)od, (const float *)ja, plane, l * P, sp, nsc, O);
+      }
       nsc = 0;
       std::swap(ja, jb);
     }
@@ -2027,12 +2037,8 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
   if (K0 == nlay) {
     // no identity region at the top: a pending product still applies to
     // the tangents from below its run; the planes go to d_jac as they are
-    if (nsc > 0)
-      hipLaunchKernelGGL(k_jac_mul, gtop, dim3(256), 0, s, ja, plane, nsc, (const double *)st, ntop);
-    for (int l = 0; l < nlay; l++)
-      for (int q = 0; q < P; q++)
-        HIPCHK(hipMemcpyAsync(d_jac + (int64_t)q * qstr + (int64_t)l * n_out, ja + ((int64_t)l * P + q) * plane,
-                              (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_jac_out, gtop, dim3(256), 0, s, (const float *)ja, plane, nlay * P, P,
+                       nsc > 0 ? (const double *)st : nullptr, nsc, ntop, d_jac, n_out, qstr);
   }
   c->tend(th);
   HIPCHK(hipGetLastError());

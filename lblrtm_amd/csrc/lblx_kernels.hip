@@ -1726,9 +1726,12 @@ struct JacSrc {
   int32_t direct[8];     // the perturbed run's optical depth is the derivative itself
 };
 
-template <typename IN>
+// Tangent planes are stored in FP32 (JP = float) between merges and in the
+// caller's FP64 (JP = double) where they are written out; the arithmetic is
+// FP64 throughout.
+template <typename IN, typename JP>
 __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN *odall,
-                          const double *rold, double *jout, const double *scale) {
+                          const double *rold, JP *jout, const double *scale) {
   const MergeArgs &A = S.A;
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.n) return;
@@ -1769,7 +1772,7 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
     double v = D * dtau;
     if (q == 0) v = v + (1. - tr) * (1. - f) * dbdt;
     if (scale) v = v * scale[g];
-    jout[(int64_t)q * S.stride + g] = v;
+    jout[(int64_t)q * S.stride + g] = (JP)v;
   }
 }
 
@@ -1778,16 +1781,17 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
 // pending transmittance product of the identity merges just below, for the
 // first nsc vectors only (S null: none).  Vector v = l P + q is written at
 // jout + l lstr + q qstr, times oscale (on the new grid) when given.
+template <typename JP>
 struct JacOut {
-  double *jout;
+  JP *jout;
   int64_t lstr, qstr;
   int32_t P;
   const double *oscale;
 };
 
-template <typename IN>
-__global__ void k_jac_prop(MergeArgs A, const DevPanel *pan, const IN *od, const double *jin,
-                           int64_t istr, int nv, const double *st, int nsc, JacOut O) {
+template <typename IN, typename JP>
+__global__ void k_jac_prop(MergeArgs A, const DevPanel *pan, const IN *od, const float *jin,
+                           int64_t istr, int nv, const double *st, int nsc, JacOut<JP> O) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.n) return;
   const int p = (int)(g / 2400);
@@ -1805,24 +1809,19 @@ __global__ void k_jac_prop(MergeArgs A, const DevPanel *pan, const IN *od, const
     s3 = st[t.i3];
   }
   for (int v = 0; v < nv; v++) {
-    const double *a = jin + (int64_t)v * istr;
+    const float *a = jin + (int64_t)v * istr;
+    const double a0 = a[t.i0], a1 = a[t.i1], a2 = a[t.i2], a3 = a[t.i3];
     double x;
     if (v < nsc)
-      x = t.w0 * (s0 * a[t.i0]) + t.w1 * (s1 * a[t.i1]) + t.w2 * (s2 * a[t.i2]) + t.w3 * (s3 * a[t.i3]);
+      x = t.w0 * (s0 * a0) + t.w1 * (s1 * a1) + t.w2 * (s2 * a2) + t.w3 * (s3 * a3);
     else
-      x = t.w0 * a[t.i0] + t.w1 * a[t.i1] + t.w2 * a[t.i2] + t.w3 * a[t.i3];
+      x = t.w0 * a0 + t.w1 * a1 + t.w2 * a2 + t.w3 * a3;
     const int l = v / O.P, q = v - l * O.P;
-    O.jout[(int64_t)l * O.lstr + (int64_t)q * O.qstr + g] = tr * x;
+    O.jout[(int64_t)l * O.lstr + (int64_t)q * O.qstr + g] = (JP)(tr * x);
   }
 }
 
-// planes[v][g] *= s[g] for the first nv planes (stride istr)
-__global__ void k_jac_mul(double *planes, int64_t istr, int nv, const double *s, int64_t n) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  const double f = s[g];
-  for (int v = 0; v < nv; v++) planes[(int64_t)v * istr + g] *= f;
-}
+
 
 
 // One layer merge of many profiles in one launch (blockIdx.y = profile).
@@ -1888,12 +1887,19 @@ __global__ void k_jac_trans(const IN *od, int64_t n, double *tt) {
 }
 
 // dst plane v = tt * src plane v (nv planes, strides istr / ostr).
-__global__ void k_jac_scale(const double *src, int64_t istr, const double *tt, int64_t n, double *dst,
-                            int64_t ostr, int nv) {
+// FP32 planes (stride istr) out to the caller's FP64 planes: vector v =
+// l P + q at dst + l lstr + q qstr, times s[g] for v < nsc (s null: none)
+__global__ void k_jac_out(const float *src, int64_t istr, int nv, int P, const double *s, int nsc,
+                          int64_t n, double *dst, int64_t lstr, int64_t qstr) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
-  const double t = tt[g];
-  for (int v = 0; v < nv; v++) dst[(int64_t)v * ostr + g] = t * src[(int64_t)v * istr + g];
+  const double f = s ? s[g] : 1.;
+  for (int v = 0; v < nv; v++) {
+    const int l = v / P, q = v - l * P;
+    double x = (double)src[(int64_t)v * istr + g];
+    if (v < nsc) x = x * f;
+    dst[(int64_t)l * lstr + (int64_t)q * qstr + g] = x;
+  }
 }
 
 
