@@ -17,6 +17,9 @@
 #include <string>
 #include <thread>
 #include <vector>
+#include <memory>
+#include <condition_variable>
+#include <functional>
 #include <type_traits>
 
 #include "../../include/lblx.h"
@@ -231,6 +234,8 @@ struct lblx_ctx {
   DevBuf w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
   DevBuf w_tmap[3];
   DevBuf w_mpan, w_rt, w_mrun;
+  std::shared_ptr<void> jprep;            // host job tables of run_jobs (JobsPrep)
+  std::vector<LayerPlan> jplans;          // the Jacobian's job plans, storage reused
   DevBuf s_z[2], s_tw, s_g, s_ip, s_cof;  // FFTSCN
   DevBuf j_rall, j_a, j_b;                // Jacobians
   int64_t s_twN = 0;
@@ -540,10 +545,94 @@ static int ensure_voigt(lblx_ctx *c) {
 
 // Host side of an OPDPTH batch: the jobs' descriptors and plan tables, tile
 // maps and the strides of the per-job work arrays.
-// f(i) for i in [0, n) on up to 16 host threads (the calling thread among
+// Host worker threads kept for the process (up to 15 beside the caller).
+// One task runs at a time; a task is a counter of indices that workers and
+// the caller claim until none is left, and it is finished when every index
+// has run.  A worker that wakes late holds the task it saw and finds no
+// index left, so it never runs a finished task's function.
+class HostPool {
+ public:
+  static HostPool &get() {
+    static HostPool pool;
+    return pool;
+  }
+  // false when another task is running (the caller then does it otherwise)
+  bool run(int n, const std::function<void(int)> &f) {
+    std::unique_lock<std::mutex> busy(run_m_, std::try_to_lock);
+    if (!busy.owns_lock() || th_.empty()) return false;
+    auto t = std::make_shared<Task>();
+    t->f = f;
+    t->n = n;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      cur_ = t;
+      gen_++;
+    }
+    cv_.notify_all();
+    work(*t);
+    while (t->done.load() < n) std::this_thread::yield();
+    std::lock_guard<std::mutex> lk(m_);
+    if (cur_ == t) cur_.reset();
+    return true;
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      quit_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+
+ private:
+  struct Task {
+    std::function<void(int)> f;
+    int n = 0;
+    std::atomic<int> next{0}, done{0};
+  };
+  HostPool() {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned k = std::min(15u, hw > 1 ? hw - 1 : 0u);
+    for (unsigned i = 0; i < k; i++) th_.emplace_back([this]() { loop(); });
+  }
+  static void work(Task &t) {
+    for (int i = t.next++; i < t.n; i = t.next++) {
+      t.f(i);
+      t.done++;
+    }
+  }
+  void loop() {
+    long seen = 0;
+    for (;;) {
+      std::shared_ptr<Task> t;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&]() { return quit_ || (cur_ && gen_ != seen); });
+        if (quit_) return;
+        seen = gen_;
+        t = cur_;
+      }
+      work(*t);
+    }
+  }
+  std::mutex run_m_, m_;
+  std::condition_variable cv_;
+  std::vector<std::thread> th_;
+  std::shared_ptr<Task> cur_;
+  long gen_ = 0;
+  bool quit_ = false;
+};
+
+// f(i) for i in [0, n) on the host worker threads (the calling thread among
 // them); f must not depend on the order.
 template <class F>
 static void parallel_for(int n, F f) {
+  if (n <= 0) return;
+  if (n < 8) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  if (HostPool::get().run(n, std::function<void(int)>(f))) return;
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   const int nth = (int)std::min<unsigned>(std::min(16u, hw), (unsigned)std::max(1, n / 8));
   if (nth <= 1) {
@@ -587,7 +676,7 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
   if (njobs <= 0) return LBLX_E_ARG;
   const int n = c->nlines, nblk = c->nblk;
   J.njobs = njobs;
-  J.own.assign(pre ? 0 : njobs, LayerPlan{});
+  if (!pre && (int)J.own.size() < njobs) J.own.resize(njobs);  // plans keep their storage
   J.plans.assign(njobs, nullptr);
   J.ys.assign(njobs, DevLayer{});
   J.margins.assign(3 * njobs, 0.);
@@ -1051,7 +1140,9 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
                     void *od_base, const int64_t *od_off, bool f32out, double *v1p, lblx_stats *st,
                     const LayerPlan *pre = nullptr) {
   c->hbegin();
-  JobsPrep J;
+  // the job tables live with the context, so their storage is reused
+  if (!c->jprep) c->jprep = std::shared_ptr<void>(new JobsPrep(), [](void *p) { delete (JobsPrep *)p; });
+  JobsPrep &J = *(JobsPrep *)c->jprep.get();
   int rc = prep_jobs(c, run, layers, njobs, chain_jrad, od_off, pre, J);
   c->hend();
   if (rc) return rc;
@@ -1863,7 +1954,8 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
     }
   const int nj = (int)jobs.size();
   // every job's plan (the species runs with their ICFLG)
-  std::vector<LayerPlan> jplans(nj);
+  if ((int)c->jplans.size() < nj) c->jplans.resize(nj);
+  std::vector<LayerPlan> &jplans = c->jplans;
   {
     std::vector<int> prc(nj, 0);
     parallel_for(nj, [&](int j) {

@@ -342,7 +342,18 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   if (!run || !Ly || !P) return LBLX_E_ARG;
   if (Ly->nmol < 1 || Ly->nmol > 7) return LBLX_E_UNSUPPORTED;
   if (run->ilblf4 < 1) return LBLX_E_UNSUPPORTED;
-  *P = LayerPlan{};
+  {  // reset, keeping the tables' storage for reuse
+    std::vector<PanelPlan> pv = std::move(P->panels);
+    std::vector<WinPlan> wv = std::move(P->wins);
+    std::vector<RadSeg> sv = std::move(P->segs);
+    *P = LayerPlan{};
+    pv.clear();
+    wv.clear();
+    sv.clear();
+    P->panels = std::move(pv);
+    P->wins = std::move(wv);
+    P->segs = std::move(sv);
+  }
   const double V1 = Ly->v1, V2 = Ly->v2, DV = Ly->dv;
   const double SAMPLE = run->sample, ALFAL0 = run->alfal0;
   if (!(DV > 0.) || !(V2 > V1)) return LBLX_E_ARG;
