@@ -258,29 +258,33 @@ def main():
     dbt = np.abs(bt(r32, v) - bt(r64, v))
     # Jacobians of the FP64 spectrum: T and H2O/CO2/O3 at every layer
     jac_ms = jac_aj_ms = None
+    jac32 = None
     if not a.no_jacobian:
         nj = 4 * nlay * ntop
         jbuf = ctx.alloc(8 * nj)
-        ctx.jacobian_device(run, [abi.Layer.from_buffer_copy(x) for x in layers], tzu, tbnd, emis, [1, 2, 3],
-                            1e-3, 1e-4, rad.ptr, jbuf.ptr, ntop)
-        ctx.sync()
-        t0 = time.perf_counter()
         nrep = max(1, a.steps // 10)
-        for _ in range(nrep):
-            ctx.jacobian_device(run, [abi.Layer.from_buffer_copy(x) for x in layers], tzu, tbnd, emis, [1, 2, 3],
-                                1e-3, 1e-4, rad.ptr, jbuf.ptr, ntop)
-        ctx.sync()
-        jac_ms = (time.perf_counter() - t0) * 1000.0 / nrep
-        # the reference's analytic-Jacobian formulation (single-species runs, +1 K)
-        ctx.jacobian_aj_device(run, [abi.Layer.from_buffer_copy(x) for x in layers], tzu, tbnd, emis, [1, 2, 3],
-                               rad.ptr, jbuf.ptr, ntop)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(nrep):
-            ctx.jacobian_aj_device(run, [abi.Layer.from_buffer_copy(x) for x in layers], tzu, tbnd, emis,
-                                   [1, 2, 3], rad.ptr, jbuf.ptr, ntop)
-        ctx.sync()
-        jac_aj_ms = (time.perf_counter() - t0) * 1000.0 / nrep
+
+        def time_jac(run_, aj):
+            def once():
+                L = [abi.Layer.from_buffer_copy(x) for x in layers]
+                if aj:  # the reference's analytic-Jacobian formulation (single-species runs, +1 K)
+                    ctx.jacobian_aj_device(run_, L, tzu, tbnd, emis, [1, 2, 3], rad.ptr, jbuf.ptr, ntop)
+                else:
+                    ctx.jacobian_device(run_, L, tzu, tbnd, emis, [1, 2, 3], 1e-3, 1e-4, rad.ptr, jbuf.ptr, ntop)
+            once()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(nrep):
+                once()
+            ctx.sync()
+            return (time.perf_counter() - t0) * 1000.0 / nrep
+
+        jac_ms = time_jac(run, False)
+        jac_aj_ms = time_jac(run, True)
+        # the adjoint's own precision form: FP32 line shapes and tangent
+        # planes, FP64 positions, against the FP32-split forward
+        j32, j32aj = time_jac(run32, False), time_jac(run32, True)
+        jac32 = {"ms": j32, "ratio_vs_forward": j32 / ms32, "aj_ms": j32aj, "aj_ratio_vs_forward": j32aj / ms32}
         jbuf.free()
     # batched FP32-split forward: ragged profiles with their own surface
     # pressures and band windows over the same line table
@@ -318,7 +322,8 @@ def main():
             "ms": jac_ms, "ratio_vs_forward": jac_ms / ms, "columns": 4 * nlay,
             "parameters": "T, ln q of H2O, CO2, O3 at every layer; OD forward differences, exact merge tangents",
             "aj": {"ms": jac_aj_ms, "ratio_vs_forward": jac_aj_ms / ms,
-                   "formulation": "reference analytic Jacobian: single-species OPDPTH runs, +1 K"}},
+                   "formulation": "reference analytic Jacobian: single-species OPDPTH runs, +1 K"},
+            "fp32_split": jac32},
         "batch": None if batch is None else dict(batch, spectra_ratio_vs_fp64=batch["value"] * ms / 1000.0),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(dom),

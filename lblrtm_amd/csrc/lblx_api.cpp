@@ -1454,6 +1454,7 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     rl[l].xkt = layers[l].tave / RADCN2;
     rl[l].xkta = tzu[l] / RADCN2;
     rl[l].od = off[l];
+    rl[l].rout = -1;
     if (l == 0 || !merge_is_identity(plans[l - 1], plans[l], itype[l], pan.data() + pan0[l],
                                      pan.data() + pan0[l - 1]))
       rstart.push_back(l);
@@ -1480,10 +1481,10 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     dim3 grid((unsigned)((A.n + 255) / 256));
     if (f32)
       hipLaunchKernelGGL(k_merge_run<float>, grid, dim3(256), 0, c->stream, A, drl, nrun, dp,
-                         c->w_od.as<float>(), ro, to, rn, tn);
+                         c->w_od.as<float>(), ro, to, rn, tn, (double *)nullptr);
     else
       hipLaunchKernelGGL(k_merge_run<double>, grid, dim3(256), 0, c->stream, A, drl, nrun, dp,
-                         c->w_od.as<double>(), ro, to, rn, tn);
+                         c->w_od.as<double>(), ro, to, rn, tn, (double *)nullptr);
     c->tend(th);
     HIPCHK(hipGetLastError());
   }
@@ -2008,13 +2009,40 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
   const DevPanel *dpan = c->w_mpan.as<DevPanel>();
   double *rall = c->j_rall.as<double>();
   double *tt[2] = {c->w_rt.as<double>(), c->w_rt.as<double>() + nmax};
-  // 2. XMERGE up the column, keeping every layer's radiance
-  for (int l = 0; l < nlay; l++) {
-    const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
-    rc = merge_launch(c, plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
-                      l == 0 ? tbnd : 0., emis, od, f32, l > 0 ? rall + off[l - 1] : nullptr,
-                      l > 0 ? tt[(l - 1) & 1] : nullptr, rall + off[l], tt[l & 1], dpan + pan0[l]);
-    if (rc) return rc;
+  // 2. XMERGE up the column, keeping every layer's radiance: runs of
+  // same-grid layers in one pass each, as the spectrum does
+  {
+    std::vector<int> rstart;
+    std::vector<RunLayer> rl(nlay);
+    for (int l = 0; l < nlay; l++) {
+      rl[l] = RunLayer{layers[l].tave / RADCN2, tzu[l] / RADCN2, off[l], off[l]};
+      if (l == 0 || !merge_is_identity(plans[l - 1], plans[l], itype[l], pan.data() + pan0[l],
+                                       pan.data() + pan0[l - 1]))
+        rstart.push_back(l);
+    }
+    rstart.push_back(nlay);
+    if (c->w_mrun.ensure((size_t)nlay * sizeof(RunLayer))) return LBLX_E_HIP;
+    c->stage_begin();
+    HIPCHK(c->h2d(c->w_mrun.p, rl.data(), (size_t)nlay * sizeof(RunLayer), s));
+    c->stage_end(s);
+    int thm = c->tbegin(ST_MERGE);
+    for (size_t q = 0; q + 1 < rstart.size(); q++) {
+      const int l = rstart[q], nrun = rstart[q + 1] - l, last = l + nrun - 1;
+      MergeArgs A = merge_args(plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+                               l == 0 ? tbnd : 0., emis);
+      const double *ro = l > 0 ? rall + off[l - 1] : nullptr;
+      const double *to = l > 0 ? tt[(q - 1) & 1] : nullptr;
+      const RunLayer *drl = c->w_mrun.as<RunLayer>() + l;
+      dim3 grid((unsigned)((A.n + 255) / 256));
+      if (f32)
+        hipLaunchKernelGGL(k_merge_run<float>, grid, dim3(256), 0, s, A, drl, nrun, dpan + pan0[l],
+                           c->w_od.as<float>(), ro, to, rall + off[last], tt[q & 1], rall);
+      else
+        hipLaunchKernelGGL(k_merge_run<double>, grid, dim3(256), 0, s, A, drl, nrun, dpan + pan0[l],
+                           c->w_od.as<double>(), ro, to, rall + off[last], tt[q & 1], rall);
+    }
+    c->tend(thm);
+    HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(d_rad, rall + off[nlay - 1], (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
   // 3. sources and tangent propagation.  The layers at the top whose merge
@@ -2318,7 +2346,7 @@ int lblx_spectra_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const
       std::vector<int> &r = rs[p - G.p0];
       for (int k = 0; k < nlay[p]; k++) {
         const int l = first[p] + k;
-        P.rl[l - first[G.p0]] = RunLayer{layers[l].tave / RADCN2, tzu[l] / RADCN2, G.off[l - first[G.p0]]};
+        P.rl[l - first[G.p0]] = RunLayer{layers[l].tave / RADCN2, tzu[l] / RADCN2, G.off[l - first[G.p0]], -1};
         if (k == 0 || !merge_is_identity(plans[l - 1], plans[l], itype[l], pan.data() + pan0[l],
                                          pan.data() + pan0[l - 1]))
           r.push_back(k);

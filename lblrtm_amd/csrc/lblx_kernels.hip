@@ -1670,13 +1670,14 @@ __global__ void k_merge(MergeArgs A, const DevPanel *pan, const IN *od, const do
 // the last layer's R and T are written.
 struct RunLayer {
   double xkt, xkta;
-  int64_t od;  // element offset of the layer's optical depth
+  int64_t od;    // element offset of the layer's optical depth
+  int64_t rout;  // element offset of the layer's radiance in rall (-1: not kept)
 };
 
 template <typename IN>
 __global__ void k_merge_run(MergeArgs A, const RunLayer *rl, int nrun, const DevPanel *pan,
                             const IN *odb, const double *rold, const double *told, double *rnew,
-                            double *tnew) {
+                            double *tnew, double *rall) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.n) return;
   const int p = (int)(g / 2400);
@@ -1696,11 +1697,13 @@ __global__ void k_merge_run(MergeArgs A, const RunLayer *rl, int nrun, const Dev
     r = tr * ro + em;
     t = tr * to;
   }
+  if (rl[0].rout >= 0) rall[rl[0].rout + g] = r;
   for (int j = 1; j < nrun; j++) {
     const RunLayer L = rl[j];
     em = merge_em(vi, (double)odb[L.od + g], L.xkt, L.xkta, &tr);
     r = tr * r + em;
     t = tr * t;
+    if (L.rout >= 0) rall[L.rout + g] = r;
   }
   rnew[g] = r;
   tnew[g] = t;
