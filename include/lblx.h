@@ -14,9 +14,12 @@
  * LBLX_OK (0) or a negative LBLX_E* code; the library never exits, the
  * Fortran shim maps a code to the reference's STOP text.
  *
- * Threading: one context per host thread; calls are synchronous to the
- * caller.  A context owns device memory, the resident line table and one
- * HIP stream.
+ * Threading: one context per host thread.  Calls that fill host memory
+ * return with the results in place; the *_device calls queue their work on
+ * the context's HIP stream and return (lblx_sync waits for it), so the
+ * next call's host planning overlaps the queued kernels.  A context owns
+ * device memory, the resident line table and one HIP stream; the library
+ * plans on a process-wide pool of up to 15 host worker threads.
  */
 #ifndef LBLX_H
 #define LBLX_H
