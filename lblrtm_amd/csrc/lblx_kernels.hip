@@ -571,6 +571,7 @@ __global__ void k_t9_vhi(Strides S, Tape9 T9) {
 // residue class of points, so additions to a point happen in line order
 // without atomics.
 constexpr int CF4_T = 256;
+constexpr int CF4_S = CF4_T - 64;  // threads that scatter (waves 1-3)
 
 // Broadcast lane j's value to the wave (j wave-uniform).
 __device__ __forceinline__ double rl_f64(double v, int j) {
@@ -637,8 +638,8 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
                                                   const DevWin *wins, const DevSeg *segs,
                                                   double *r4out, unsigned long long *k3upd) {
   __shared__ T r4[R4ROW];
-  __shared__ F4Par<T> par[64];
-  __shared__ int s_nacc, s_done, s_b;
+  __shared__ F4Par<T> par[2][64];
+  __shared__ int s_nacc2[2], s_done2[2], s_b;
   const int tid = threadIdx.x;
   const DevWin W = wins[blockIdx.x];
   const int job = W.job;
@@ -659,7 +660,8 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
   for (int j = tid; j < R4ROW; j += CF4_T) r4[j] = (T)0.;
   if (tid == 0) {
     s_b = nb9;
-    s_done = 0;
+    s_nacc2[0] = 0;  // an empty chunk before the first
+    s_done2[0] = 0;
   }
   __syncthreads();
   for (int b = tid; b < nb9; b += CF4_T)
@@ -680,22 +682,31 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
   // margin that sends lines within rounding of the rejection bound to the
   // exact sequential test
   const T lb_margin = sizeof(T) == 4 ? (T)1e-5 : (T)1e-12;
-  while (pos < nl) {
-    if (tid < 64) {
+  // Decisions and scatter overlap: while waves 1-3 add the chunk decided
+  // last (par[cur]) into R4, wave 0 decides the next chunk against R4 as it
+  // stood before that scatter plus the last chunk's accepted lines
+  // evaluated at its own line centres (in the order the scatter adds them).
+  // Wave 0 reads R4 before the barrier that starts the scatter.
+  int cur = 0;
+  int npos = pos;
+  for (;;) {
+    const bool more = !s_done2[cur] && npos < nl;
+    const int lane = tid & 63;
+    const int i = npos + lane;
+    const bool valid = more && i < nl;
+    // wave 0's lines of the chunk it decides (unused by waves 1-3)
+    F4Par<T> p;
+    p.jmin = 1;
+    p.jmax = 0;
+    p.xnui = 0.;
+    T speak = (T)0., partial = (T)0., pkb = (T)0., f4bnd = (T)0.;
+    double vnui = 0.;
+    bool nz = false, bnd_ok = false;
+    int jj = 1;
+    if (tid < 64 && more) {
       // the in-order decisions are the window's critical path: wave 0 is
       // issued ahead of the scattering waves on its SIMD while it decides
       __builtin_amdgcn_s_setprio(3);
-      const int lane = tid;
-      int i = pos + lane;
-      bool valid = i < nl;
-      F4Par<T> p;
-      p.jmin = 1;
-      p.jmax = 0;
-      p.xnui = 0.;
-      T speak = (T)0., partial = (T)0., pkb = (T)0., f4bnd = (T)0.;
-      double vnui = 0.;
-      bool nz = false, bnd_ok = false;
-      int jj = 1;
       if (valid) {
         double sp = f_sp;
         nz = sp != 0.;
@@ -754,6 +765,32 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         pkb = (T)((pkn > pkf ? pkn : pkf) * (1. + 1e-5));
         partial = r4[jj];
       }
+    }
+    __syncthreads();
+    if (tid >= 64) {  // waves 1-3: the last decided chunk into R4
+      const int ts = tid - 64;
+      const int nacc = s_nacc2[cur];
+      const double step = CF4_S * dvr4;
+      for (int a = 0; a < nacc; a++) {
+        F4Par<T> q = par[cur][a];
+        if (ts == 0) nupd += (unsigned long long)(q.jmax - q.jmin + 1);
+        int j0 = q.jmin - 1;
+        int r = j0 % CF4_S;
+        int first = j0 - r + ts + (ts < r ? CF4_S : 0) + 1;
+        // XJJ - XNUI advanced by a running sum: a line spans a few strides,
+        // so the sum's rounding stays within a few ulp of XJJ
+        double xmd = (double)(first - 1) * dvr4 - q.xnui;
+        for (int j = first; j <= q.jmax; j += CF4_S, xmd += step) r4[j] = r4[j] + f4_at(q, xmd);
+      }
+    } else if (more) {
+      const int nb = cur ^ 1;
+      const double xjj_me = (double)(jj - 1) * dvr4;
+      // the last chunk's accepted lines at this chunk's line centres
+      const int nprev = s_nacc2[cur];
+      for (int a = 0; a < nprev; a++) {
+        const F4Par<T> q = par[cur][a];
+        if (valid && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me - q.xnui);
+      }
       const bool cand = valid && nz;
       // Bounds of the running R4 at this line over the undecided earlier
       // lines of the chunk: each adds at least -2 F4BND (shape >= 0,
@@ -787,7 +824,6 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       // so its step does not wait on the previous line's contribution
       const uint64_t sacc_m = __ballot(surely_acc);
       const uint64_t past_m = __ballot(vnui >= vnulst);
-      const double xjj_me = (double)(jj - 1) * dvr4;
       int nacc = 0;
       int done = 0;
       while (todo) {
@@ -818,7 +854,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
           q.jmin = __builtin_amdgcn_readlane(p.jmin, j);
           q.jmax = __builtin_amdgcn_readlane(p.jmax, j);
           if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me - q.xnui);
-          if (lane == j && p.jmax >= p.jmin) par[nacc] = p;
+          if (lane == j && p.jmax >= p.jmin) par[nb][nacc] = p;
           if (q.jmax >= q.jmin) nacc++;
         }
       }
@@ -832,15 +868,15 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         }
         const uint64_t addm = rest & __ballot(p.jmax >= p.jmin);
         const uint64_t below_me = lane == 0 ? 0ull : (addm & ((1ull << lane) - 1));
-        if ((addm >> lane) & 1) par[nacc + __builtin_popcountll(below_me)] = p;
+        if ((addm >> lane) & 1) par[nb][nacc + __builtin_popcountll(below_me)] = p;
         nacc += __builtin_popcountll(addm);
       }
       if (lane == 0) {
-        s_nacc = nacc;
-        s_done = done;
+        s_nacc2[nb] = nacc;
+        s_done2[nb] = done;
       }
       // prefetch the next chunk while the group scatters this one
-      const int in = pos + 64 + lane;
+      const int in = npos + 64 + lane;
       if (!done && in < nl) {
         f_sp = T9.sp[o + in];
         f_al = T9.alfa[o + in];
@@ -851,24 +887,11 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
-    const int nacc = s_nacc;
-    const double step = CF4_T * dvr4;
-    for (int a = 0; a < nacc; a++) {
-      F4Par<T> q = par[a];
-      if (tid == 0) nupd += (unsigned long long)(q.jmax - q.jmin + 1);
-      int j0 = q.jmin - 1;
-      int r = j0 % CF4_T;
-      int first = j0 - r + tid + (tid < r ? CF4_T : 0) + 1;
-      // XJJ - XNUI advanced by a running sum: a line spans a few strides,
-      // so the sum's rounding stays within a few ulp of XJJ
-      double xmd = (double)(first - 1) * dvr4 - q.xnui;
-      for (int j = first; j <= q.jmax; j += CF4_T, xmd += step) r4[j] = r4[j] + f4_at(q, xmd);
-    }
-    __syncthreads();
-    if (s_done) break;
-    pos += 64;
+    if (!more) break;
+    cur ^= 1;
+    npos += 64;
   }
-  if (tid == 0 && nupd) atomicAdd(&k3upd[job], nupd);
+  if (tid == 64 && nupd) atomicAdd(&k3upd[job], nupd);
   __syncthreads();
   // JRAD = 1: radiation term as a piecewise-linear factor (oprop.f90:3733-3758)
   if (W.nseg > 0) {
