@@ -131,7 +131,7 @@ def cpu_baseline(run, layers, tab, tzu, tbnd, emis):
                       "C oracle on one host core, %.1f s" % dt}
 
 
-def batch_block(ctx, lib, abi, synth, nprof, steps):
+def batch_block(ctx, lib, abi, synth, nprof, steps, nadj):
     profs = synth.batch_profiles(nprof)
     run = abi.make_run(500.0, 2500.0, icntnm=1, precision=abi.FP32_SPLIT)
     items, sizes, npts = [], [], 0
@@ -160,7 +160,29 @@ def batch_block(ctx, lib, abi, synth, nprof, steps):
     ms = (time.perf_counter() - t0) * 1000.0 / steps
     rad.free()
     tra.free()
+    adj = None
+    if nadj > 0:  # forward and adjoint (AJ formulation) of the first nadj profiles
+        sub = items[:nadj]
+        P = 4
+        jsz = [P * len(it[0]) * n for it, n in zip(sub, sizes[:nadj])]
+        roff = np.concatenate([[0], np.cumsum(sizes[:nadj])[:-1]]).astype(np.int64)
+        joff = np.concatenate([[0], np.cumsum(jsz)[:-1]]).astype(np.int64)
+        rad = ctx.alloc(8 * int(sum(sizes[:nadj])))
+        jac = ctx.alloc(8 * int(sum(jsz)))
+        ctx.jacobians_batch_device(run, sub, [1, 2, 3], rad.ptr, roff, jac.ptr, joff, aj=True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.jacobians_batch_device(run, sub, [1, 2, 3], rad.ptr, roff, jac.ptr, joff, aj=True)
+        ctx.sync()
+        ms_adj = (time.perf_counter() - t0) * 1000.0
+        rad.free()
+        jac.free()
+        adj = {"value": nadj * 1000.0 / ms_adj, "unit": "spectra/s", "ms_per_step": ms_adj, "profiles": nadj,
+               "columns_per_layer": P,
+               "workload": "the first %d profiles above: TOA radiance and its Jacobians (T, ln q of H2O, "
+                           "CO2, O3 at every layer; AJ OD runs), FP32 split" % nadj}
     return {"value": nprof * 1000.0 / ms, "unit": "spectra/s", "ms_per_step": ms, "steps": steps,
+            "forward_and_adjoint": adj,
             "profiles": nprof, "layers": int(sum(len(i[0]) for i in items)), "points": int(npts),
             "tiles": list(tiles), "autotune_s": t_tune,
             "workload": "ragged profiles: 20-80 layers, psfc 600-1050 hPa, 200-800 cm-1 windows "
@@ -177,6 +199,8 @@ def main():
     ap.add_argument("--batch-profiles", type=int, default=256,
                     help="profiles in the batched FP32-split form (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-adjoint", type=int, default=8,
+                    help="profiles of the batch also run forward and adjoint (0 skips it)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run the plain copy kernel over this many MB (the reference "
                          "dispatch for scaling HBM traffic counters)")
@@ -290,7 +314,8 @@ def main():
     # pressures and band windows over the same line table
     batch = None
     if a.batch_profiles > 0:
-        batch = batch_block(ctx, lib, abi, synth, a.batch_profiles, a.batch_steps)
+        batch = batch_block(ctx, lib, abi, synth, a.batch_profiles, a.batch_steps,
+                            min(a.batch_adjoint, a.batch_profiles))
     dom = max((k for k in per if k != "host"), key=per.get)
     nwin = sum(s["nwin4"] for s in stats)
     nbytes = stage_bytes(dom, stats, sizes, nwin)

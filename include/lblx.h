@@ -239,14 +239,31 @@ int lblx_jacobian_aj_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *laye
  * first, concatenated in layers/tzu; each profile's own band in its
  * layers' v1/v2).  Layer optical depths run in batches of whole profiles
  * sized to the device memory, and the merges of a batch run one launch per
- * layer index over all its profiles, the launch chain captured as a HIP
- * graph and replayed on later calls of the same shape.  Profile p's TOA
+ * run of same-grid layers over all its profiles, the launch chain captured
+ * as a HIP graph and replayed on later calls of the same shape.  Profile p's TOA
  * radiance/transmittance at d_rad/d_tra + out_off[p] (its top-grid size,
  * lblx_od_size of its last layer); v1p_top[p] its first wavenumber.      */
 int lblx_spectra_batch_device(lblx_ctx *ctx, const lblx_run *run, int nprof, const int *nlay,
                               lblx_layer *layers, const double *tzu, const double *tbnd,
                               const double *emis, double *d_rad, double *d_tra,
                               const int64_t *out_off, double *v1p_top);
+
+/* Forward and adjoint for a batch of independent profiles (the layout of
+ * lblx_spectra_batch_device): profile p's TOA radiance at d_rad +
+ * rad_off[p] and its Jacobian planes (lblx_jacobian_device's layout with
+ * its own layer count and top-grid size) at d_jac + jac_off[p].  aj = 0
+ * takes OD derivatives by forward differences (dT, dlnq), aj = 1 the
+ * reference's analytic-Jacobian runs (lblx_jacobian_aj_device, dT and dlnq
+ * unused).  Each profile's perturbed layers run as one OPDPTH batch; the
+ * profiles are queued one after another on the context's stream, the next
+ * one planned on the host while the last one runs.  No counterpart call in
+ * the reference, whose AJ runs are one TAPE5 per profile
+ * (src/lblrtm.f90:2757-3110).                                            */
+int lblx_jacobians_batch_device(lblx_ctx *ctx, const lblx_run *run, int nprof, const int *nlay,
+                                lblx_layer *layers, const double *tzu, const double *tbnd,
+                                const double *emis, const int *species, int nspec, int aj, double dT,
+                                double dlnq, double *d_rad, const int64_t *rad_off, double *d_jac,
+                                const int64_t *jac_off, double *v1p_top);
 
 /* Tile widths (points) of the R1, R2, R3 near-wing scatter: chosen by
  * timing candidates on the given layers (lblx_autotune), or set directly

@@ -2181,6 +2181,30 @@ int lblx_jacobian_aj_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers
                       n_out, v1p_top);
 }
 
+int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const int *nlay,
+                                lblx_layer *layers, const double *tzu, const double *tbnd,
+                                const double *emis, const int *species, int nspec, int aj, double dT,
+                                double dlnq, double *d_rad, const int64_t *rad_off, double *d_jac,
+                                const int64_t *jac_off, double *v1p_top) {
+  if (!c || !run || nprof <= 0 || !nlay || !layers || !tzu || !tbnd || !emis || !d_rad || !rad_off || !d_jac ||
+      !jac_off)
+    return LBLX_E_ARG;
+  if (aj) dT = dlnq = 1.0;
+  int64_t l0 = 0;
+  for (int p = 0; p < nprof; p++) {
+    if (nlay[p] <= 0) return LBLX_E_ARG;
+    int64_t n_out = 0;
+    int rc = lblx_od_size(run, &layers[l0 + nlay[p] - 1], &n_out);
+    if (rc) return rc;
+    rc = jacobian_run(c, run, layers + l0, nlay[p], tzu + l0, tbnd[p], emis[p], species, nspec, dT, dlnq,
+                      aj ? 1 : 0, d_rad + rad_off[p], d_jac + jac_off[p], n_out,
+                      v1p_top ? v1p_top + p : nullptr);
+    if (rc) return rc;
+    l0 += nlay[p];
+  }
+  return LBLX_OK;
+}
+
 // Per-job device bytes of run_jobs (line-table-sized work arrays).
 // Device bytes one job takes in run_jobs at a per-job line stride of
 // about nline records.

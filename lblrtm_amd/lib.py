@@ -78,6 +78,10 @@ def load():
         "lblx_spectra_batch_device": (C.c_int, [C.c_void_p, P(abi.Run), C.c_int, P(C.c_int), P(abi.Layer),
                                                 P(C.c_double), P(C.c_double), P(C.c_double), C.c_void_p,
                                                 C.c_void_p, P(C.c_int64), P(C.c_double)]),
+        "lblx_jacobians_batch_device": (C.c_int, [C.c_void_p, P(abi.Run), C.c_int, P(C.c_int), P(abi.Layer),
+                                                  P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_int), C.c_int,
+                                                  C.c_int, C.c_double, C.c_double, C.c_void_p, P(C.c_int64),
+                                                  C.c_void_p, P(C.c_int64), P(C.c_double)]),
         "lblx_autotune": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int)]),
         "lblx_set_tiles": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lblx_voigt_tables": (C.c_int, [P(C.c_double), P(C.c_double)]),
@@ -391,6 +395,31 @@ class Context:
                                                  abi.ptr(em, C.c_double), C.c_void_p(d_rad), C.c_void_p(d_tra),
                                                  abi.ptr(oo, C.c_int64), abi.ptr(v1p, C.c_double)),
              "lblx_spectra_batch_device")
+        return v1p
+
+    def jacobians_batch_device(self, run, profiles, species, d_rad, rad_off, d_jac, jac_off, aj=False,
+                               dT=1e-3, dlnq=1e-4):
+        """Forward and adjoint of a batch of profiles (lblx_jacobians_batch_device):
+        profiles is a list of (layers, tzu, tbnd, emis); returns v1p of each
+        profile's top grid."""
+        nl = [len(p[0]) for p in profiles]
+        lays = [abi.Layer.from_buffer_copy(x) for p in profiles for x in p[0]]
+        arr = (abi.Layer * len(lays))(*lays)
+        tz = np.ascontiguousarray(np.concatenate([np.asarray(p[1], np.float64) for p in profiles]))
+        tb = np.ascontiguousarray([p[2] for p in profiles], np.float64)
+        em = np.ascontiguousarray([p[3] for p in profiles], np.float64)
+        ro = np.ascontiguousarray(rad_off, np.int64)
+        jo = np.ascontiguousarray(jac_off, np.int64)
+        sp = (C.c_int * max(1, len(species)))(*species)
+        v1p = np.zeros(len(profiles))
+        nla = (C.c_int * len(nl))(*nl)
+        _chk(self._lib.lblx_jacobians_batch_device(self._h, C.byref(run), len(profiles), nla, arr,
+                                                   abi.ptr(tz, C.c_double), abi.ptr(tb, C.c_double),
+                                                   abi.ptr(em, C.c_double), sp, len(species), 1 if aj else 0,
+                                                   dT, dlnq, C.c_void_p(d_rad), abi.ptr(ro, C.c_int64),
+                                                   C.c_void_p(d_jac), abi.ptr(jo, C.c_int64),
+                                                   abi.ptr(v1p, C.c_double)),
+             "lblx_jacobians_batch_device")
         return v1p
 
     def autotune(self, run, layers):

@@ -68,3 +68,42 @@ def test_autotune_keeps_results(gpu_ctx):
     gpu_ctx.set_tiles((8192, 4096, 2048))
     with pytest.raises(lib.LblxError):
         gpu_ctx.set_tiles((100, 1024, 512))
+
+
+@pytest.mark.parametrize("aj", [False, True])
+def test_jacobians_batch_equal_single(gpu_ctx, aj):
+    """Forward and adjoint of a ragged batch (lblx_jacobians_batch_device):
+    each profile's radiance and Jacobian planes equal the same profile's
+    lblx_jacobian_device / lblx_jacobian_aj_device alone."""
+    gpu_ctx.load_lines(synth.config1_lines())
+    run = abi.make_run(500.0, 2500.0, icntnm=1)
+    specs = [(22, 1000.0, 1020.0, 1013.0), (26, 700.0, 715.0, 850.0), (20, 1500.0, 1530.0, 600.0)]
+    items = []
+    for k, (nl, v1, v2, ps) in enumerate(specs):
+        atm = synth.atmosphere(nlev=nl + 1, psfc=ps, seed=900 + k)
+        atm["v1"], atm["v2"] = v1, v2
+        items.append((_profile_layers(run, atm), atm["tlev"][1:], atm["tlev"][0], 0.98))
+    species = [1, 2, 3]
+    P = 1 + len(species)
+    sizes = [lib.od_size(run, it[0][-1]) for it in items]
+    jsz = [P * len(it[0]) * n for it, n in zip(items, sizes)]
+    roff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    joff = np.concatenate([[0], np.cumsum(jsz)[:-1]]).astype(np.int64)
+    rad = gpu_ctx.alloc(8 * int(sum(sizes)))
+    jac = gpu_ctx.alloc(8 * int(sum(jsz)))
+    v1p = gpu_ctx.jacobians_batch_device(run, items, species, rad.ptr, roff, jac.ptr, joff, aj=aj)
+    rb = rad.to_host(np.float64)
+    jb = jac.to_host(np.float64)
+    rad.free()
+    jac.free()
+    for k, it in enumerate(items):
+        rs, js, v1s, _ = gpu_ctx.jacobian(run, [abi.Layer.from_buffer_copy(x) for x in it[0]], it[1], it[2], it[3],
+                                          species, aj=aj)
+        print("aj %d profile %d: %d layers, %d points" % (aj, k, len(it[0]), sizes[k]), flush=True)
+        assert v1p[k] == v1s
+        np.testing.assert_allclose(rb[roff[k]:roff[k] + sizes[k]], rs, rtol=1e-12)
+        # same kernels; the tangent planes are FP32, so an OD difference in the
+        # last bits (order of LDS atomic adds) can move a plane value by one
+        # FP32 ulp
+        jk = jb[joff[k]:joff[k] + jsz[k]].reshape(js.shape)
+        np.testing.assert_allclose(jk, js, rtol=1e-5, atol=1e-6 * np.abs(js).max())
