@@ -988,8 +988,6 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   hipLaunchKernelGGL(k_shr_chain, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_ewalk, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), 0, s, dY, S, W, T9);
-  hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.line + 255) / 256)), dim3(256), 0, s, W.grp, NJ * S.line,
-                     -1);
   hipLaunchKernelGGL(k_shr_bwalk, dim3(bgrid, njobs), dim3(64), 0, s, S, W);
   hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9);
   hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);

@@ -78,7 +78,7 @@ struct ShrinkWork {
   int32_t *g;             // first-group end from each start
   int32_t *jmp;           // [line] f: next group end after a group end
   int16_t *wsum;          // [line] lines the group after a group end emits
-  int32_t *grp, *gout;    // [line] at a group's last line: its first line, first output slot (else -1)
+  int32_t *grp, *gout;    // [output slot] first and last line of the group emitted there (grp -1: a group's second slot)
   int32_t *bcnt, *bcntn, *bcntc, *bstop;  // per TAPE3 block
   int32_t *bp, *bpn, *bpc;                // block prefixes
   int32_t *npass, *stop;                  // per job

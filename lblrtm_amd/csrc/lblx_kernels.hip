@@ -461,9 +461,10 @@ __global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides 
   }
 }
 
-// per (job, batch), one wave each: the batch's chain, marking the last line
-// of each group with the group's first line (grp) and its first output
-// slot (gout)
+// per (job, batch), one wave each: the batch's chain.  Each group that
+// emits lines is recorded at its first output slot: its first line (grp)
+// and last line (gout); the second slot of a group that emits both a
+// non-CO2 and a CO2 line is marked -1.  Every output slot is written.
 __global__ __launch_bounds__(64) void k_shr_bwalk(Strides S, ShrinkWork W) {
   const int job = blockIdx.y;
   const int q = blockIdx.x;
@@ -472,51 +473,41 @@ __global__ __launch_bounds__(64) void k_shr_bwalk(Strides S, ShrinkWork W) {
   const int64_t o = job * S.line;
   const int s = W.bat_s[o + q], t = W.bat_t[o + q], off = W.bat_off[o + q];
   ChainWalk cw(W, o, W.npass[job], s);
+  auto mark = [&](int gs, int ge, int slot, int em) {
+    if (!lead || em <= 0) return;
+    W.grp[o + slot] = gs;
+    W.gout[o + slot] = ge;
+    if (em == 2) W.grp[o + slot + 1] = -1;
+  };
   if (cw.e1 >= t) {
-    if (lead) {
-      W.grp[o + t] = s;
-      W.gout[o + t] = off;
-    }
+    mark(s, t, off, em_cnt(cw.pn, cw.pc, s, t));
     return;
   }
-  if (lead) {
-    W.grp[o + cw.e1] = s;
-    W.gout[o + cw.e1] = off;
-  }
+  mark(s, cw.e1, off, cw.acc);
   for (;;) {
     int w;
     const int nx = cw.next(&w);
     if (nx >= t) break;
-    if (lead) {
-      W.grp[o + nx] = cw.cur + 1;
-      W.gout[o + nx] = off + cw.acc;
-    }
+    mark(cw.cur + 1, nx, off + cw.acc, w);
     cw.acc += w;
     cw.cur = nx;
   }
-  if (lead) {
-    W.grp[o + t] = cw.cur + 1;
-    W.gout[o + t] = off + cw.acc;
-  }
+  mark(cw.cur + 1, t, off + cw.acc, em_cnt(cw.pn, cw.pc, cw.cur + 1, t));
 }
 
-// per passing line: a group's last line writes the group's effective lines
+// per output slot: the group recorded there writes its effective lines
 // (strength-weighted centre and widths, sums in line order as SHRINK does,
 // oprop.f90:3560-3631)
 __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9) {
   const int job = blockIdx.y;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nbat = W.nbat[job];
+  const int qs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qs >= T9.n[job]) return;
   const int64_t o = job * S.line;
-  if (nbat == 0) return;
-  if (i < W.bat_s[o] || i > W.bat_t[o + nbat - 1]) return;
-  const int gs = W.grp[o + i];
+  const int gs = W.grp[o + qs];
   if (gs < 0) return;
-  const int ge = i;
+  const int ge = W.gout[o + qs];
   double sumv = 0., sums = 0., sumal = 0., sumad = 0.;
   double sumv2 = 0., sums2 = 0., sumal2 = 0., sumad2 = 0.;
-  // loads of several lines in flight; the sums stay in line order
-#pragma unroll 4
   for (int k = gs; k <= ge; k++) {
     double sp = W.csp[o + k];
     if (!W.cco2[o + k]) {
@@ -531,7 +522,7 @@ __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9
       sumad2 = sumad2 + sp * W.cad[o + k];
     }
   }
-  int64_t q = o + W.gout[o + i];
+  int64_t q = o + qs;
   if (sums > 0.) {
     T9.vnu[q] = sumv / sums;
     T9.sp[q] = sums;
