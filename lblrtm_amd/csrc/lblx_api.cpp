@@ -987,7 +987,14 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   hipLaunchKernelGGL(k_shr_compact, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
   hipLaunchKernelGGL(k_shr_chain, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_ewalk, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W);
-  hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), 0, s, dY, S, W, T9);
+  {  // block tables of a job in LDS when they fit (ints: counts, prefixes, ECACHE entries per block)
+    const int per = ECACHE + 2;
+    int lcap = std::min(maxnb4, (int)((150u << 10) / (4u * per)) - 1);
+    if (lcap < 1) lcap = 1;
+    const size_t shm = (size_t)(lcap * per + 1) * 4;
+    HIPCHK(hipFuncSetAttribute((const void *)k_shr_batches, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), shm, s, dY, S, W, T9, lcap);
+  }
   hipLaunchKernelGGL(k_shr_bwalk, dim3(bgrid, njobs), dim3(64), 0, s, S, W);
   hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9);
   hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);

@@ -380,7 +380,12 @@ __global__ __launch_bounds__(64) void k_shr_ewalk(const DevLayer *Ys, Strides S,
 // deciding batch boundaries and TAPE9 blocks from the emitted counts.
 // Blocks run from the job's first far-wing block; its last one holds the
 // stop line unless the file ends first.
-__global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9) {
+// Jobs of up to lcap blocks keep their block counts, prefixes and the
+// first ECACHE batch-table entries of each block in LDS for the walk.
+constexpr int ECACHE = 12;
+__global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9,
+                                                     int lcap) {
+  extern __shared__ int32_t sm_b[];
   const int job = blockIdx.x;
   const bool lead = threadIdx.x == 0;  // the wave walks together; lane 0 writes
   const int64_t ob = job * S.blk;
@@ -388,10 +393,26 @@ __global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides 
   const int64_t ot = job * S.t9blk;
   const int stop_p = W.stop[job];
   const int nblk = Ys[job].nb4;
-  // the blocks' passing counts and prefixes, 64 blocks at a time in lanes
   const int lane = threadIdx.x & 63;
+  const bool inl = nblk <= lcap;
+  int32_t *l_cnt = sm_b, *l_bp = sm_b + lcap, *l_et = sm_b + 2 * lcap + 1;
+  if (inl) {
+    for (int i = lane; i < nblk; i += 64) {
+      l_cnt[i] = W.bcnt[ob + i];
+      l_bp[i] = W.bp[ob + i];
+      for (int k = 0; k < ECACHE; k++) l_et[i * ECACHE + k] = W.etab[(ob + i) * EBLK + k];
+    }
+    if (lane == 0) l_bp[nblk] = W.bp[ob + nblk];
+  }
+  __syncthreads();
+  // otherwise the blocks' passing counts and prefixes, 64 blocks at a time in lanes
   int wb = -64, wc = 0, wp = 0;
   auto blk = [&](int bb, int *cnt, int *pnext) {  // bcnt[bb], bp[bb + 1]
+    if (inl) {
+      *cnt = l_cnt[bb];
+      *pnext = l_bp[bb + 1];
+      return;
+    }
     if (bb >= wb + 64) {
       wb = bb;
       const int i = bb + lane;
@@ -423,14 +444,16 @@ __global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides 
       }
       if (J + raw >= LIMIN4) break;
     }
-    int s = W.bp[ob + bs];
-    int t = stop ? stop_p : W.bp[ob + b] - 1;
+    int s = inl ? l_bp[bs] : W.bp[ob + bs];
+    int t = stop ? stop_p : (inl ? l_bp[b] : W.bp[ob + b]) - 1;
     int em = 0;
     if (t >= s) {
       int nb = b - bs;
+      const int et = (!stop && nb >= 1 && nb <= EBLK)
+                         ? ((inl && nb <= ECACHE) ? l_et[bs * ECACHE + nb - 1] : W.etab[(ob + bs) * EBLK + nb - 1])
+                         : -1;
       if (stop && nb <= EBLK && W.estop[ob + bs] >= 0) em = W.estop[ob + bs];
-      else if (!stop && nb >= 1 && nb <= EBLK && W.etab[(ob + bs) * EBLK + nb - 1] >= 0)
-        em = W.etab[(ob + bs) * EBLK + nb - 1];
+      else if (et >= 0) em = et;
       else em = ChainWalk(W, o, W.npass[job], s).count_to(t);  // a batch longer than the tables
       if (lead) {
         W.bat_s[o + nbat] = s;
