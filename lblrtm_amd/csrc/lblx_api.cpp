@@ -890,7 +890,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_t9alfa, NJ * S.line * 8) || need(c->w_t9epp, NJ * S.line * 8) ||
       need(c->w_t9mol, NJ * S.line * 4) || need(c->w_t9n, NJ * 8) ||
       need(c->w_t9blk, NJ * S.t9blk * 16) || need(c->w_t9bvhi, NJ * S.t9blk * 8) ||
-      need(c->w_near, NJ * S.line * sizeof(NearLine)) || need(c->w_keep, NJ * S.line * 4) ||
+      need(c->w_near, NJ * S.line * (sizeof(NearHot) + sizeof(NearCold))) || need(c->w_keep, NJ * S.line * 4) ||
       need(c->w_blkmax, NJ * S.blk * 4) || need(c->w_pread, NJ * S.blk * 4) ||
       need(c->w_dirty, NJ * S.blk * 4) ||
       need(c->w_changed, 16) || need(c->w_rec, NJ * S.line * sizeof(ScatterRec)) ||
@@ -1008,12 +1008,15 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
                        c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
   c->tend(th);
+  NearArr nearr;
+  nearr.hot = c->w_near.as<NearHot>();
+  nearr.cold = (NearCold *)(c->w_near.as<char>() + (size_t)NJ * S.line * sizeof(NearHot));
   if (maxnn > 0) {
     const int ngrid = (maxnn + 255) / 256;
     // 4. LNCOR1 + first fit
     th = c->tbegin(ST_LNCOR1);
     hipLaunchKernelGGL(k_lncor1, dim3(ngrid, njobs), dim3(256), 0, s, L, dY, S,
-                       c->w_pan.as<DevPanel>(), c->w_near.as<NearLine>());
+                       c->w_pan.as<DevPanel>(), nearr);
     c->tend(th);
     // 5. block-read panel fixpoint
     th = c->tbegin(ST_REJECT);
@@ -1026,7 +1029,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     bool converged = false;
     for (int it = 0; it < 64; it++) {
       hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, c->w_changed.as<int>(), 1, 0);
-      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY,
+      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, (const NearHot *)nearr.hot, L, dY,
                          S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
                          c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>(),
                          c->w_dirty.as<int>());
@@ -1044,14 +1047,14 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     }
     // keep flags for the converged read panels
     if (!converged)
-      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(), L, dY, S,
+      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, (const NearHot *)nearr.hot, L, dY, S,
                          c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
                          c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>(),
                          c->w_dirty.as<int>());
     c->tend(th);
     // 6. CNVFNV scatter records
     th = c->tbegin(ST_SCATTER);
-    hipLaunchKernelGGL(k_scatter_rec, dim3(maxnbk, njobs), dim3(256), 0, s, c->w_near.as<NearLine>(),
+    hipLaunchKernelGGL(k_scatter_rec, dim3(maxnbk, njobs), dim3(256), 0, s, nearr,
                        c->w_keep.as<int>(), L, dY, S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(),
                        c->w_rec.as<ScatterRec>());
     if (st)  // TAPE6 counters only
@@ -2215,7 +2218,7 @@ int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, con
 // about nline records.
 static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
   const size_t line = ((size_t)nline + 127) / 64 * 64;
-  return line * (32 + 1 + 12 + 4 * 3 + 2 + 12 + 36 + sizeof(NearLine) + 8 + sizeof(ScatterRec)) +
+  return line * (32 + 1 + 12 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(ScatterRec)) +
          (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 1 + 3) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
 

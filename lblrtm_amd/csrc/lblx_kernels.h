@@ -109,6 +109,40 @@ struct NearLine {
   int32_t proc;    // panel in which CNVFNV processes it
 };
 
+// NearLine stored as two arrays: the fields every rejection pass reads,
+// and the ones only the scatter records need.
+struct NearHot {
+  double vnu, recalf, speak;
+  int32_t ff, keep;
+};
+struct NearCold {
+  double sp, zetai;
+  int32_t izeta, proc;
+};
+struct NearArr {
+  NearHot *hot;
+  NearCold *cold;
+  __device__ void put(int64_t i, const NearLine &o) const {
+    hot[i] = NearHot{o.vnu, o.recalf, o.speak, o.ff, o.keep};
+    cold[i] = NearCold{o.sp, o.zetai, o.izeta, o.proc};
+  }
+  __device__ NearLine get(int64_t i) const {
+    const NearHot h = hot[i];
+    const NearCold c = cold[i];
+    NearLine o;
+    o.vnu = h.vnu;
+    o.recalf = h.recalf;
+    o.speak = h.speak;
+    o.ff = h.ff;
+    o.keep = h.keep;
+    o.sp = c.sp;
+    o.zetai = c.zetai;
+    o.izeta = c.izeta;
+    o.proc = c.proc;
+    return o;
+  }
+};
+
 // Scatter record of one kept line: indices and weights for R1/R2/R3.
 struct ScatterRec {
   double zfl1, zfl2, zfl3, zslope;

@@ -931,7 +931,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
 // the R4 rejection); those are applied in k_reject.  Also computes the
 // first panel whose R1 buffer can hold the line (CNVFNV JMAX1 <= MAX1).
 __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPanel *pans,
-                         NearLine *out) {
+                         NearArr out) {
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -997,7 +997,7 @@ __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPan
       o.ff = p;  // == npanel: never processed before the last panel
     }
   }
-  out[job * S.line + t] = o;
+  out.put(job * S.line + t, o);
 }
 
 // One step of the block-read fixpoint: with the current guess of the panel
@@ -1006,7 +1006,7 @@ __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPan
 // lines per block.  Only blocks whose read panel changed in the last step
 // (dirty) are redone; the others keep their flags and block maxima.  The
 // per-block maximum is reduced across the wave before the atomic.
-__global__ __launch_bounds__(256) void k_reject(const NearLine *nl, DevLines L, const DevLayer *Ys, Strides S,
+__global__ __launch_bounds__(256) void k_reject(const NearHot *nl, DevLines L, const DevLayer *Ys, Strides S,
                                                 const int *pread, const DevPanel *pans, const DevWin *wins,
                                                 const double *r4, int *keep_out, int *blkmax, const int *dirty) {
   const int job = blockIdx.y;
@@ -1017,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_reject(const NearLine *nl, DevLines L, 
   const bool act = in && dirty[job * S.blk + k] != 0;
   int val = -1;
   if (act) {
-    const NearLine &o = nl[job * S.line + t];
+    const NearHot o = nl[job * S.line + t];
     int keep = o.keep;
     if (keep) {
       int pr = pread[job * S.blk + k];
@@ -1095,7 +1095,7 @@ __global__ __launch_bounds__(256) void k_pread_scan(const DevLayer *Ys, Strides 
 // raised by the first-fit panels of kept lines before it in the block
 // (CNVFNV stops at the first line that does not fit, oprop.f90:1287-1292).
 // Then the scatter record of CNVFNV (oprop.f90:1262-1305).
-__global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const int *keep, DevLines L,
+__global__ __launch_bounds__(256) void k_scatter_rec(NearArr nl, const int *keep, DevLines L,
                                                      const DevLayer *Ys, Strides S, const int *pread,
                                                      const DevPanel *pans, ScatterRec *recs) {
   __shared__ int buf[256];
@@ -1112,8 +1112,8 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearLine *nl, const i
   const bool valid = t < e;
   const int64_t o = job * S.line;
   int kp = valid ? keep[o + t] : 0;
-  NearLine n;
-  if (valid) n = nl[o + t];
+  NearLine n{};
+  if (valid) n = nl.get(o + t);
   buf[tid] = (valid && kp) ? n.ff : 0;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
