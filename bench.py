@@ -28,7 +28,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "spectra/s FP32-mixed vs plain FP64 at ≤1e-4 rel; Jacobian/forward time"
 HBM_PEAK_GBS = 8000.0
-REC_BYTES = 104      # ScatterRec
+REC1_BYTES = 48      # R1 scatter record (SRec1)
+REC_BYTES = 32       # R2 / R3 scatter record (SRecA)
 T9_BYTES = 36        # shrunk line: centre, strength, two widths, molecule
 LINE_BYTES = 36      # TAPE3 line read per layer (centre f64, six f32, mol)
 
@@ -43,7 +44,7 @@ def stage_bytes(stage, stats, sizes, nwin):
         return 8 * 2007 * len(sizes)
     if stage == "tile_r1":
         # write OD, read R2 (npts/4) and the scatter record of every kept line
-        return 8 * npts + 8 * npts / 4 + REC_BYTES * sum(s["lincnt"] for s in stats)
+        return 8 * npts + 8 * npts / 4 + REC1_BYTES * sum(s["lincnt"] for s in stats)
     if stage == "tile_r2":
         return 8 * npts / 4 + 8 * npts / 16 + REC_BYTES * sum(s["lincnt"] for s in stats)
     if stage == "tile_r3":

@@ -893,7 +893,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_near, NJ * S.line * (sizeof(NearHot) + sizeof(NearCold))) || need(c->w_keep, NJ * S.line * 4) ||
       need(c->w_blkmax, NJ * S.blk * 4) || need(c->w_pread, NJ * S.blk * 4) ||
       need(c->w_dirty, NJ * S.blk * 4) ||
-      need(c->w_changed, 16) || need(c->w_rec, NJ * S.line * sizeof(ScatterRec)) ||
+      need(c->w_changed, 16) || need(c->w_rec, NJ * S.line * (sizeof(SRec1) + 2 * sizeof(SRecA) + (voigt ? sizeof(SRecV) : 0))) ||
       need(c->w_r3, (size_t)r3tot * 8) || need(c->w_r2, (size_t)r2tot * 8) ||
       need(c->w_cnt, NJ * 24) || need(c->w_margin, NJ * 3 * 8) ||
       (abs_stride > 0 && (need(c->w_absrb, NJ * abs_stride * 8) ||
@@ -1008,6 +1008,15 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
                        c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
   c->tend(th);
+  Recs recs;
+  {
+    char *rb = c->w_rec.as<char>();
+    const size_t nl = (size_t)NJ * S.line;
+    recs.r1 = (SRec1 *)rb;
+    recs.r2 = (SRecA *)(rb + nl * sizeof(SRec1));
+    recs.r3 = recs.r2 + nl;
+    recs.v = voigt ? (SRecV *)(rb + nl * (sizeof(SRec1) + 2 * sizeof(SRecA))) : nullptr;
+  }
   NearArr nearr;
   nearr.hot = c->w_near.as<NearHot>();
   nearr.cold = (NearCold *)(c->w_near.as<char>() + (size_t)NJ * S.line * sizeof(NearHot));
@@ -1056,9 +1065,9 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     th = c->tbegin(ST_SCATTER);
     hipLaunchKernelGGL(k_scatter_rec, dim3(maxnbk, njobs), dim3(256), 0, s, nearr,
                        c->w_keep.as<int>(), L, dY, S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(),
-                       c->w_rec.as<ScatterRec>());
+                       recs);
     if (st)  // TAPE6 counters only
-      hipLaunchKernelGGL(k_sum_npts, dim3(ngrid, njobs), dim3(256), 0, s, dY, S, c->w_rec.as<ScatterRec>(),
+      hipLaunchKernelGGL(k_sum_npts, dim3(ngrid, njobs), dim3(256), 0, s, dY, S, (const SRec1 *)recs.r1,
                          cnt, cnt + 2 * NJ);
     c->tend(th);
   }
@@ -1075,7 +1084,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     // FP32 shapes: accumulator in FP64 LDS for the grids in LBLX_FP32_LDS64
     const bool l64 = !split || ((LBLX_FP32_LDS64 >> G) & 1);
     size_t shm = (size_t)T * (l64 ? 8 : esz) + (size_t)(voigt ? 0 : (G == 1 ? 3 : 1) * 2601) * esz;
-    const ScatterRec *rec = c->w_rec.as<ScatterRec>();
+    const Recs rec = recs;
     const DevPanel *dp = c->w_pan.as<DevPanel>();
     const DevWin *dw = c->w_win.as<DevWin>();
     const DevSeg *dg = c->w_seg.as<DevSeg>();
@@ -2218,7 +2227,7 @@ int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, con
 // about nline records.
 static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
   const size_t line = ((size_t)nline + 127) / 64 * 64;
-  return line * (32 + 1 + 12 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(ScatterRec)) +
+  return line * (32 + 1 + 12 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(SRec1) + 2 * sizeof(SRecA) + sizeof(SRecV)) +
          (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 1 + 3) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
 

@@ -143,14 +143,28 @@ struct NearArr {
   }
 };
 
-// Scatter record of one kept line: indices and weights for R1/R2/R3.
-struct ScatterRec {
-  double zfl1, zfl2, zfl3, zslope;
-  double strf1, strf2, strf3, strd, strver;
-  int64_t g1lo;    // global fine index of JMIN1
-  int64_t s2, s3;  // global shifts: G2 = G1 - s2, G3 = G1 - s3
-  int32_t npts;    // JMAX1 - JMIN1 + 1 (0 = no contribution)
-  int32_t izm;     // exact-Voigt build: zeta column IZM (strd holds ZETDIF)
+// Scatter records of the near lines, one array per grid so that each tile
+// kernel reads only its own fields: the R1 record (with the FG and XVER
+// weights), the R2 and R3 records, and the exact-Voigt build's zeta
+// column and fraction (all grids).  glo is the line's first point on the
+// grid (G2 = G1 - j2shift - 1800 p, G3 = G1 - j3shift - 2250 p); npts = 0
+// means no contribution.
+struct SRec1 {
+  double zfl, zslope, str, strd, strver;
+  int32_t glo, npts;
+};
+struct SRecA {
+  double zfl, zslope, str;
+  int32_t glo, npts;
+};
+struct SRecV {
+  double zdif;  // ZETDIF
+  int32_t izm, pad;
+};
+struct Recs {
+  SRec1 *r1;
+  SRecA *r2, *r3;
+  SRecV *v;  // written and read only in the exact-Voigt build
 };
 
 }  // namespace lblx

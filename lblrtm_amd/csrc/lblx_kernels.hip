@@ -1097,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_pread_scan(const DevLayer *Ys, Strides 
 // Then the scatter record of CNVFNV (oprop.f90:1262-1305).
 __global__ __launch_bounds__(256) void k_scatter_rec(NearArr nl, const int *keep, DevLines L,
                                                      const DevLayer *Ys, Strides S, const int *pread,
-                                                     const DevPanel *pans, ScatterRec *recs) {
+                                                     const DevPanel *pans, Recs recs) {
   __shared__ int buf[256];
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
@@ -1124,28 +1124,25 @@ __global__ __launch_bounds__(256) void k_scatter_rec(NearArr nl, const int *keep
   }
   if (!valid) return;
   int P = max(pread[job * S.blk + k], buf[tid]);
-  ScatterRec r;
-  r.npts = 0;
-  r.izm = 1;
-  r.g1lo = 0;
-  r.s2 = r.s3 = 0;
-  r.zfl1 = r.zfl2 = r.zfl3 = r.zslope = 0.;
-  r.strf1 = r.strf2 = r.strf3 = r.strd = r.strver = 0.;
+  SRec1 r1{};
+  SRecA r2{}, r3{};
+  SRecV rv{};
+  rv.izm = 1;
   if (kp && P < Y.npanel) {
     double vft = pans[Y.pan0 + P].vft;
     double depthi = n.sp * n.recalf;
     int izm = n.izeta;
     double zetdif = 100. * n.zetai - (double)(izm - 1);
     if (Y.voigt) {  // oprop_voigt.f90:1273-1276: DEPTHI and the zeta column
-      r.strf1 = r.strf2 = r.strf3 = depthi;
-      r.strd = zetdif;
-      r.izm = izm < 1 ? 1 : (izm > NZETA ? NZETA : izm);
+      r1.str = r2.str = r3.str = depthi;
+      rv.zdif = zetdif;
+      rv.izm = izm < 1 ? 1 : (izm > NZETA ? NZETA : izm);
     } else {
-      r.strf1 = depthi * (c_cf1[izm - 1] + zetdif * (c_cf1[izm] - c_cf1[izm - 1]));
-      r.strf2 = depthi * (c_cf2[izm - 1] + zetdif * (c_cf2[izm] - c_cf2[izm - 1]));
-      r.strf3 = depthi * (c_cf3[izm - 1] + zetdif * (c_cf3[izm] - c_cf3[izm - 1]));
-      r.strd = depthi * (c_cgauss[izm - 1] + zetdif * (c_cgauss[izm] - c_cgauss[izm - 1]));
-      r.strver = depthi * (c_cer[izm - 1] + zetdif * (c_cer[izm] - c_cer[izm - 1]));
+      r1.str = depthi * (c_cf1[izm - 1] + zetdif * (c_cf1[izm] - c_cf1[izm - 1]));
+      r2.str = depthi * (c_cf2[izm - 1] + zetdif * (c_cf2[izm] - c_cf2[izm - 1]));
+      r3.str = depthi * (c_cf3[izm - 1] + zetdif * (c_cf3[izm] - c_cf3[izm - 1]));
+      r1.strd = depthi * (c_cgauss[izm - 1] + zetdif * (c_cgauss[izm] - c_cgauss[izm - 1]));
+      r1.strver = depthi * (c_cer[izm - 1] + zetdif * (c_cer[izm] - c_cer[izm - 1]));
     }
     double zslope = n.recalf * (Y.dv / 0.002);
     double zint = (n.vnu - vft) / Y.dv;
@@ -1157,21 +1154,27 @@ __global__ __launch_bounds__(256) void k_scatter_rec(NearArr nl, const int *keep
     int j2shft = fint(zint * (1. - conf2) + rshft);
     int j3shft = fint(zint * (1. - conf3) + rshft);
     int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
-    r.zfl1 = ((double)(jmin1 - 2) - zint) * zslope;
-    r.zfl2 = ((double)(jmin2 - 2) - zint * conf2) * zslope;
-    r.zfl3 = ((double)(jmin3 - 2) - zint * conf3) * zslope;
-    r.zslope = zslope;
-    r.g1lo = (int64_t)jmin1 - 1 + (int64_t)2400 * P;
-    r.s2 = (int64_t)j2shft + (int64_t)1800 * P;
-    r.s3 = (int64_t)j3shft + (int64_t)2250 * P;
-    r.npts = jmax1 - jmin1 + 1;
+    r1.zfl = ((double)(jmin1 - 2) - zint) * zslope;
+    r2.zfl = ((double)(jmin2 - 2) - zint * conf2) * zslope;
+    r3.zfl = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+    r1.zslope = r2.zslope = r3.zslope = zslope;
+    // global fine index of JMIN1 and its R2/R3 counterparts (G1 < 2^31 for
+    // any layer the planner accepts)
+    const int32_t g1lo = jmin1 - 1 + 2400 * P;
+    r1.glo = g1lo;
+    r2.glo = g1lo - (j2shft + 1800 * P);
+    r3.glo = g1lo - (j3shft + 2250 * P);
+    r1.npts = r2.npts = r3.npts = jmax1 - jmin1 + 1;
   }
-  recs[o + t] = r;
+  recs.r1[o + t] = r1;
+  recs.r2[o + t] = r2;
+  recs.r3[o + t] = r3;
+  if (Y.voigt) recs.v[o + t] = rv;
 }
 
 // Near-wing point updates and convolved lines per job (sum of
 // JMAX1-JMIN1+1 and count over kept lines; LINCNT of the TAPE6 table).
-__global__ void k_sum_npts(const DevLayer *Ys, Strides S, const ScatterRec *rec,
+__global__ void k_sum_npts(const DevLayer *Ys, Strides S, const SRec1 *rec,
                            unsigned long long *acc, unsigned long long *lines) {
   const int job = blockIdx.y;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1259,22 +1262,34 @@ struct TileLine {
   int64_t glo;
   double zfl, zslope, str, strd, strver;
   int npts, izm;
-  __device__ __forceinline__ void load(const ScatterRec *rec, int li, int hi) {
+  __device__ __forceinline__ void load(const Recs &R, int64_t base, int li, int hi) {
     strd = strver = 0.;
     izm = 1;
     if (li < hi) {
-      const ScatterRec &r = rec[li];
-      npts = r.npts;
-      glo = G == 1 ? r.g1lo : (G == 2 ? r.g1lo - r.s2 : r.g1lo - r.s3);
-      zfl = G == 1 ? r.zfl1 : (G == 2 ? r.zfl2 : r.zfl3);
-      zslope = r.zslope;
-      str = G == 1 ? r.strf1 : (G == 2 ? r.strf2 : r.strf3);
+      const int64_t i = base + li;
+      if (G == 1) {
+        const SRec1 r = R.r1[i];
+        npts = r.npts;
+        glo = r.glo;
+        zfl = r.zfl;
+        zslope = r.zslope;
+        str = r.str;
+        if (!V) {
+          strd = r.strd;
+          strver = r.strver;
+        }
+      } else {
+        const SRecA r = (G == 2 ? R.r2 : R.r3)[i];
+        npts = r.npts;
+        glo = r.glo;
+        zfl = r.zfl;
+        zslope = r.zslope;
+        str = r.str;
+      }
       if (V) {  // ZETDIF and IZM of the exact-Voigt build
-        strd = r.strd;
-        izm = r.izm;
-      } else if (G == 1) {
-        strd = r.strd;
-        strver = r.strver;
+        const SRecV v = R.v[i];
+        strd = v.zdif;
+        izm = v.izm;
       }
     } else {
       npts = 0;
@@ -1303,7 +1318,7 @@ constexpr int TILE_T = LBLX_TILE_T;
 //   LACC: type of the LDS accumulator (ACC, or double under FP32 shapes
 //   where LDS float atomics stall).
 template <int G, typename OUT, typename ACC, bool V, typename LACC>
-__global__ __launch_bounds__(TILE_T) void k_tile(const ScatterRec *recs, TileMap M, int tile,
+__global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
                                               const double *tabs, const ACC *vtab, const DevLayer *Ys,
                                               Strides S, const DevPanel *pans, const DevWin *wins,
                                               const double *r4, const double *absrb,
@@ -1317,7 +1332,7 @@ __global__ __launch_bounds__(TILE_T) void k_tile(const ScatterRec *recs, TileMap
   const int tid = threadIdx.x;
   const int job = M.job[blockIdx.x];
   const DevLayer &Y = Ys[job];
-  const ScatterRec *rec = recs + job * S.line;
+  const int64_t rbase = job * S.line;
   for (int k = tid; k < tile; k += TILE_T) acc[k] = (LACC)0.;
   if (!V) {
     const double *src = tabs + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NFMX;
@@ -1341,10 +1356,10 @@ __global__ __launch_bounds__(TILE_T) void k_tile(const ScatterRec *recs, TileMap
   // chunk's records are loaded before the current chunk is scattered.
   TileLine<G, V> cur, nxt;
   int c0 = lo + wave * 64;
-  if (c0 < hi) cur.load(rec, c0 + lane, hi);
+  if (c0 < hi) cur.load(recs, rbase, c0 + lane, hi);
   for (; c0 < hi; c0 += TILE_T) {
     const int c1 = c0 + TILE_T;
-    if (c1 < hi) nxt.load(rec, c1 + lane, hi);
+    if (c1 < hi) nxt.load(recs, rbase, c1 + lane, hi);
     const int nj = hi - c0 < 64 ? hi - c0 : 64;
     for (int j = 0; j < nj; j++) {
       const int npts = __builtin_amdgcn_readlane(cur.npts, j);

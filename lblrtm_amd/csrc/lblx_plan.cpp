@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstring>
 
 #include "lblx_contdata.h"
@@ -494,6 +495,7 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   P->n1 = (int64_t)2400 * pl + P->max1 + 16;
   P->n2 = (int64_t)600 * pl + P->max2 + 16;
   P->n3 = (int64_t)150 * pl + P->max3 + 16;
+  if (P->n1 >= (int64_t)INT32_MAX) return LBLX_E_SIZE;  // grid indices are kept in 32 bits
   return LBLX_OK;
 }
 
