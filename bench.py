@@ -287,7 +287,7 @@ def main():
     if not a.no_jacobian:
         nj = 4 * nlay * ntop
         jbuf = ctx.alloc(8 * nj)
-        nrep = max(1, a.steps // 10)
+        nrep = max(2, a.steps // 5)
 
         def time_jac(run_, aj):
             def once():
