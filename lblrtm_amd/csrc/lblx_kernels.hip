@@ -1795,7 +1795,14 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
   const int k = (int)(g - (int64_t)2400 * p);
   const double vi = (pan[p].vft + 32. * A.dv) + (double)k * A.dv;
   const double od_lo = 0.06;
-  const double bb = d_planck(vi, A.xkt);
+  // PLANCK at Tave with its exponential kept for dB/dT
+  const double x = vi * rcp_nr(A.xkt);
+  double bb = 0., ex = 0.;
+  if (x < 1e-2) bb = (vi * vi) * A.xkt * rcp_nr(1. + 0.5 * x) * 1.191042722E-12;
+  else if (x <= 80.) {
+    ex = exp(-x);
+    bb = (vi * vi * vi) * ex * rcp_nr(1. - ex) * 1.191042722E-12;
+  }
   const double bb_dif = A.xkta > 0. ? d_planck(vi, A.xkta) - bb : 0.;
   double odvi = (double)od[g];
   const bool clamped = odvi < -od_lo;
@@ -1805,10 +1812,10 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
   if (odvi <= od_lo) {
     f = (1. / 6.) * odvi;
     fp = 1. / 6.;
-  } else {
-    const double q = tr - 1.;
-    f = 1. - 2. * (tr / q + 1. / odvi);
-    fp = -2. * (tr / (q * q) - 1. / (odvi * odvi));
+  } else {  // the forward merge's F (merge_em) and its derivative
+    const double rq = rcp_nr(tr - 1.), ro = rcp_nr(odvi);
+    f = 1. - 2. * (tr * rq + ro);
+    fp = -2. * (tr * rq * rq - ro * ro);
   }
   double below;
   if (A.first) {
@@ -1820,8 +1827,7 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
   double D = -tr * below + tr * (bb + bb_dif * f) + (1. - tr) * bb_dif * fp;
   if (clamped) D = 0.;
   // dB/dT of PLANCK at Tave (zero where PLANCK is cut off)
-  const double x = vi / A.xkt;
-  const double dbdt = (x >= 1e-2 && x <= 80.) ? bb * (x / S.tave) / (1. - exp(-x)) : 0.;
+  const double dbdt = (x >= 1e-2 && x <= 80.) ? bb * (x * rcp_nr(S.tave)) * rcp_nr(1. - ex) : 0.;
   for (int q = 0; q < S.nparam; q++) {
     double dtau = S.direct[q] ? (double)odall[S.odp[q] + g] * S.rh[q]
                               : ((double)odall[S.odp[q] + g] - (double)od[g]) * S.rh[q];
