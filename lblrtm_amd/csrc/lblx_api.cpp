@@ -114,16 +114,20 @@ struct lblx_ctx {
     (void)hipEventCreate(&e);
     return e;
   }
-  int tbegin(int st) {
+  // second stream (default priority) for work that overlaps the main chain;
+  // every fork is joined back into `stream` within the same call
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int tbegin(int st, hipStream_t on = nullptr) {
     if (!timing) return -1;
     hipEvent_t a = ev_get(), b = ev_get();
-    (void)hipEventRecord(a, stream);
+    (void)hipEventRecord(a, on ? on : stream);
     evs.push_back({st, {a, b}});
     return (int)evs.size() - 1;
   }
-  void tend(int h) {
+  void tend(int h, hipStream_t on = nullptr) {
     if (h < 0) return;
-    (void)hipEventRecord(evs[h].second.second, stream);
+    (void)hipEventRecord(evs[h].second.second, on ? on : stream);
   }
   // Pinned staging of the per-call tables copied to the device.  A copy
   // from pageable memory returns only once the stream has reached it, which
@@ -298,7 +302,12 @@ int lblx_create(lblx_ctx **out, int device) {
   HIPCHK(hipSetDevice(device));
   lblx_ctx *c = new lblx_ctx();
   c->device = device;
-  if (hipStreamCreate(&c->stream) != hipSuccess) {
+  if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->side) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LBLX_E_HIP;
   }
@@ -365,6 +374,9 @@ void lblx_destroy(lblx_ctx *c) {
   c->tcollect();
   c->stage_release();
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -987,8 +999,31 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   hipLaunchKernelGGL(k_shr_compact, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
   hipLaunchKernelGGL(k_shr_chain, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_ewalk, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W);
-  {  // block tables of a job in LDS when they fit (ints: counts, prefixes, ECACHE entries per block)
-    const int per = ECACHE + 2;
+  NearArr nearr;
+  nearr.hot = c->w_near.as<NearHot>();
+  nearr.cold = (NearCold *)(c->w_near.as<char>() + (size_t)NJ * S.line * sizeof(NearHot));
+  if (maxnn > 0) {
+    // LNCOR1 reads only the line table and the layer/panel tables, so it runs
+    // on the side stream beside SHRINK's batch walk (one wave per job, most
+    // of the chip idle) and CONVF4; the rejection fixpoint waits for it
+    hipStream_t s2 = c->side;
+    HIPCHK(hipEventRecord(c->ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(s2, c->ev_fork, 0));
+    const int ngrid = (maxnn + 255) / 256;
+    int thl = c->tbegin(ST_LNCOR1, s2);
+    hipLaunchKernelGGL(k_lncor1, dim3(ngrid, njobs), dim3(256), 0, s2, L, dY, S,
+                       c->w_pan.as<DevPanel>(), nearr);
+    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s2,
+                       c->w_pread.as<int>(), NJ * S.blk, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s2,
+                       c->w_blkmax.as<int>(), NJ * S.blk, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s2,
+                       c->w_dirty.as<int>(), NJ * S.blk, 1);
+    c->tend(thl, s2);
+    HIPCHK(hipEventRecord(c->ev_join, s2));
+  }
+  {  // block tables of a job in LDS when they fit (ints: prefix, ECACHE entries per block)
+    const int per = ECACHE + 1;
     int lcap = std::min(maxnb4, (int)((150u << 10) / (4u * per)) - 1);
     if (lcap < 1) lcap = 1;
     const size_t shm = (size_t)(lcap * per + 1) * 4;
@@ -1017,24 +1052,11 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     recs.r3 = recs.r2 + nl;
     recs.v = voigt ? (SRecV *)(rb + nl * (sizeof(SRec1) + 2 * sizeof(SRecA))) : nullptr;
   }
-  NearArr nearr;
-  nearr.hot = c->w_near.as<NearHot>();
-  nearr.cold = (NearCold *)(c->w_near.as<char>() + (size_t)NJ * S.line * sizeof(NearHot));
   if (maxnn > 0) {
     const int ngrid = (maxnn + 255) / 256;
-    // 4. LNCOR1 + first fit
-    th = c->tbegin(ST_LNCOR1);
-    hipLaunchKernelGGL(k_lncor1, dim3(ngrid, njobs), dim3(256), 0, s, L, dY, S,
-                       c->w_pan.as<DevPanel>(), nearr);
-    c->tend(th);
-    // 5. block-read panel fixpoint
+    // 4-5. block-read panel fixpoint, after LNCOR1 on the side stream
+    HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
     th = c->tbegin(ST_REJECT);
-    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s,
-                       c->w_pread.as<int>(), NJ * S.blk, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s,
-                       c->w_blkmax.as<int>(), NJ * S.blk, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s,
-                       c->w_dirty.as<int>(), NJ * S.blk, 1);
     bool converged = false;
     for (int it = 0; it < 64; it++) {
       hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, c->w_changed.as<int>(), 1, 0);

@@ -4,7 +4,7 @@
 // src/oprop.f90:25-668).  Here every layer ("job") of a batch is resident
 // in HBM over its whole band and each stage is one launch for all jobs:
 //   LINF4 + SHRINK   k_shr_count, k_shr_scan, k_shr_compact, k_shr_chain,
-//                    k_shr_lift, k_shr_etab, k_shr_batches, k_shr_emit
+//                    k_shr_ewalk, k_shr_batches, k_shr_bwalk, k_shr_emit
 //                    (oprop.f90:3093-3290, 3508-3640)
 //   LBLF4/CONVF4     k_convf4, one far-wing window per workgroup
 //                    (oprop.f90:3641-4039)
@@ -376,12 +376,15 @@ __global__ __launch_bounds__(64) void k_shr_ewalk(const DevLayer *Ys, Strides S,
   if (lane == 0) W.estop[ob + b] = es;
 }
 
-// one thread per job: LINF4's read loop over blocks (oprop.f90:3130-3290)
+// one wave per job: LINF4's read loop over blocks (oprop.f90:3130-3290)
 // deciding batch boundaries and TAPE9 blocks from the emitted counts.
 // Blocks run from the job's first far-wing block; its last one holds the
-// stop line unless the file ends first.
-// Jobs of up to lcap blocks keep their block counts, prefixes and the
-// first ECACHE batch-table entries of each block in LDS for the walk.
+// stop line unless the file ends first.  A batch reads blocks until it
+// holds the stop line or J + (passing lines read) >= LIMIN4; both tests are
+// monotone in the block prefix bp, so the wave tests 64 blocks at once and
+// takes the first that ends the batch.
+// Jobs of up to lcap blocks keep their block prefixes and the first ECACHE
+// batch-table entries of each block in LDS for the walk.
 constexpr int ECACHE = 12;
 __global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9,
                                                      int lcap) {
@@ -395,57 +398,47 @@ __global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides 
   const int nblk = Ys[job].nb4;
   const int lane = threadIdx.x & 63;
   const bool inl = nblk <= lcap;
-  int32_t *l_cnt = sm_b, *l_bp = sm_b + lcap, *l_et = sm_b + 2 * lcap + 1;
+  int32_t *l_bp = sm_b, *l_et = sm_b + lcap + 1;
   if (inl) {
-    for (int i = lane; i < nblk; i += 64) {
-      l_cnt[i] = W.bcnt[ob + i];
-      l_bp[i] = W.bp[ob + i];
+    for (int i = lane; i <= nblk; i += 64) l_bp[i] = W.bp[ob + i];
+    for (int i = lane; i < nblk; i += 64)
       for (int k = 0; k < ECACHE; k++) l_et[i * ECACHE + k] = W.etab[(ob + i) * EBLK + k];
-    }
-    if (lane == 0) l_bp[nblk] = W.bp[ob + nblk];
   }
   __syncthreads();
-  // otherwise the blocks' passing counts and prefixes, 64 blocks at a time in lanes
-  int wb = -64, wc = 0, wp = 0;
-  auto blk = [&](int bb, int *cnt, int *pnext) {  // bcnt[bb], bp[bb + 1]
-    if (inl) {
-      *cnt = l_cnt[bb];
-      *pnext = l_bp[bb + 1];
-      return;
-    }
-    if (bb >= wb + 64) {
-      wb = bb;
-      const int i = bb + lane;
-      wc = i < nblk ? W.bcnt[ob + i] : 0;
-      wp = i < nblk ? W.bp[ob + i + 1] : 0;
-    }
-    const int k = __builtin_amdgcn_readfirstlane(bb - wb);
-    *cnt = __builtin_amdgcn_readlane(wc, k);
-    *pnext = __builtin_amdgcn_readlane(wp, k);
-  };
+  auto bp = [&](int k) { return inl ? l_bp[k] : W.bp[ob + k]; };
   int b = 0;
   int J = 0, out = 0, nbat = 0, nt9 = 0, t9s = 0;
   for (;;) {
-    int bs = b;
-    int raw = 0;
+    const int bs = b;
+    const int s = bp(bs);
     bool stop = false, eof = false;
-    for (;;) {
-      if (b >= nblk) {
-        eof = true;
-        break;
+    int pe = 0;  // bp[b] after the batch's last block
+    if (bs >= nblk) {
+      eof = true;
+    } else {
+      const int lim = LIMIN4 - J + s;
+      for (int base = bs;; base += 64) {
+        const int k = base + lane;
+        const int pn = k < nblk ? bp(k + 1) : 0;
+        const bool hs = k < nblk && stop_p >= 0 && pn > stop_p;
+        const bool hit = hs || (k < nblk && pn >= lim);
+        const uint64_t m = __ballot(hit);
+        if (m) {
+          const int f = __ffsll((unsigned long long)m) - 1;
+          b = base + f + 1;
+          pe = __builtin_amdgcn_readlane(pn, f);
+          stop = (__ballot(hs) >> f) & 1;
+          break;
+        }
+        if (base + 64 >= nblk) {
+          b = nblk;
+          eof = true;
+          pe = bp(nblk);
+          break;
+        }
       }
-      int cnt, pnext;
-      blk(b, &cnt, &pnext);
-      raw += cnt;
-      b++;
-      if (stop_p >= 0 && stop_p < pnext) {
-        stop = true;
-        break;
-      }
-      if (J + raw >= LIMIN4) break;
     }
-    int s = inl ? l_bp[bs] : W.bp[ob + bs];
-    int t = stop ? stop_p : (inl ? l_bp[b] : W.bp[ob + b]) - 1;
+    const int t = stop ? stop_p : pe - 1;
     int em = 0;
     if (t >= s) {
       int nb = b - bs;
