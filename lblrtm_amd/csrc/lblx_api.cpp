@@ -98,6 +98,7 @@ static const char *kStageNames[ST_N] = {"linf4", "shrink", "convf4", "lncor1", "
 
 struct lblx_ctx {
   int device = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   bool timing = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
@@ -230,7 +231,7 @@ struct lblx_ctx {
   ContTabs cont_tabs{};
   DevBuf w_cps;   // ContPlan per job
   // work buffers
-  DevBuf w_dlay, w_pan, w_win, w_seg, w_r4, w_margin;
+  DevBuf w_dlay, w_pan, w_win, w_word, w_seg, w_r4, w_margin;
   DevBuf w_cx, w_csp, w_cal, w_cad, w_cco2, w_pn, w_pc, w_g, w_jmp, w_wsum, w_bcnt, w_etab, w_jobi,
       w_bat;
   DevBuf w_t9vnu, w_t9sp, w_t9alfa, w_t9epp, w_t9mol, w_t9n, w_t9blk, w_t9bvhi;
@@ -302,6 +303,11 @@ int lblx_create(lblx_ctx **out, int device) {
   HIPCHK(hipSetDevice(device));
   lblx_ctx *c = new lblx_ctx();
   c->device = device;
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+      c->ncu = ncu;
+  }
   if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->side) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
@@ -363,7 +369,7 @@ void lblx_destroy(lblx_ctx *c) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   DevBuf *bufs[] = {&c->d_vnu, &c->d_sp, &c->d_alfa, &c->d_epp, &c->d_hwhm, &c->d_tmpalf,
                     &c->d_pshift, &c->d_mol, &c->d_blk, &c->d_bstart, &c->d_bvmax, &c->d_tabs, &c->d_cont, &c->w_cps,
-                    &c->w_dlay, &c->w_pan, &c->w_win, &c->w_seg, &c->w_r4, &c->w_margin,
+                    &c->w_dlay, &c->w_pan, &c->w_win, &c->w_word, &c->w_seg, &c->w_r4, &c->w_margin,
                     &c->w_cx, &c->w_csp, &c->w_cal, &c->w_cad, &c->w_cco2, &c->w_pn, &c->w_pc,
                     &c->w_g, &c->w_jmp, &c->w_wsum, &c->w_bcnt, &c->w_etab, &c->w_jobi, &c->w_bat,
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
@@ -668,6 +674,7 @@ struct JobsPrep {
   std::vector<DevLayer> ys;
   std::vector<DevPanel> pan;
   std::vector<DevWin> win;
+  std::vector<int32_t> worder;  // far-wing workgroup -> window
   std::vector<DevSeg> seg;
   std::vector<double> margins;
   std::vector<ContPlan> cps;
@@ -677,6 +684,34 @@ struct JobsPrep {
   int maxnn = 0, maxnbk = 0, abs_stride = 0, maxnb4 = 0;
   Strides S{};
 };
+
+// Far-wing windows are all resident at once (one workgroup each), and a
+// window's time follows its line-centre density: windows of the same band
+// in neighbouring layers are equally heavy, and in index order they land on
+// the same compute units.  The windows are dealt out heaviest first, one per
+// compute unit per tier, in alternating direction, with the raw lines within
+// BOUND4 of the window times its points as the weight.
+static void far_wing_order(const lblx_ctx *c, JobsPrep &J) {
+  const int nwin = (int)J.win.size();
+  std::vector<double> cost(nwin);
+  for (int i = 0; i < nwin; i++) {
+    const DevWin &w = J.win[i];
+    const double b4 = J.ys[w.job].bound4;
+    const auto lo = std::lower_bound(c->h_vnu.begin(), c->h_vnu.end(), w.v1r4 - b4);
+    const auto hi = std::upper_bound(lo, c->h_vnu.end(), w.v2r4 + b4);
+    cost[i] = (double)(hi - lo) * w.nptr4;
+  }
+  std::vector<int32_t> by(nwin);
+  for (int i = 0; i < nwin; i++) by[i] = i;
+  std::stable_sort(by.begin(), by.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  auto &ord = J.worder;
+  ord.resize(nwin);
+  const int T = std::max(1, c->ncu);
+  for (int i = 0; i < nwin; i++) {
+    const int tier = i / T, slot = i % T, m = std::min(T, nwin - tier * T);
+    ord[(size_t)tier * T + ((tier & 1) ? m - 1 - slot : slot)] = by[i];
+  }
+}
 
 // Plan a batch of jobs (layers) on the host.  Reads only the context's line
 // table and settings, so it may run beside launches of another batch.
@@ -820,6 +855,7 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
   }
   if (seg.empty()) seg.push_back(DevSeg{0, 0, 0., 0.});
   const int nwin = (int)win.size();
+  far_wing_order(c, J);
   abs_stride = (abs_stride + 63) / 64 * 64;
   for (int jb = 0; jb < njobs; jb++) ys[jb].abs0 = (int64_t)jb * abs_stride;
   // tile maps
@@ -889,7 +925,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   const int64_t NJ = njobs;
   auto need = [&](DevBuf &b, size_t bytes) { return b.ensure(bytes) != 0; };
   if (need(c->w_dlay, NJ * sizeof(DevLayer)) || need(c->w_pan, pan.size() * sizeof(DevPanel)) ||
-      need(c->w_win, win.size() * sizeof(DevWin)) || need(c->w_seg, seg.size() * sizeof(DevSeg)) ||
+      need(c->w_win, win.size() * sizeof(DevWin)) || need(c->w_word, win.size() * 4) || need(c->w_seg, seg.size() * sizeof(DevSeg)) ||
       need(c->w_r4, (size_t)nwin * R4ROW * 8) ||
       need(c->w_cx, NJ * S.line * 8) || need(c->w_csp, NJ * S.line * 8) ||
       need(c->w_cal, NJ * S.line * 8) || need(c->w_cad, NJ * S.line * 8) ||
@@ -919,6 +955,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   HIPCHK(c->h2d(c->w_dlay.p, ys.data(), NJ * sizeof(DevLayer), s));
   HIPCHK(c->h2d(c->w_pan.p, pan.data(), pan.size() * sizeof(DevPanel), s));
   HIPCHK(c->h2d(c->w_win.p, win.data(), win.size() * sizeof(DevWin), s));
+  HIPCHK(c->h2d(c->w_word.p, J.worder.data(), win.size() * 4, s));
   HIPCHK(c->h2d(c->w_seg.p, seg.data(), seg.size() * sizeof(DevSeg), s));
   HIPCHK(c->h2d(c->w_margin.p, margins.data(), margins.size() * 8, s));
   if (abs_stride > 0) {  // CONTNM: ABSRB rows of every job
@@ -1038,10 +1075,10 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   th = c->tbegin(ST_CONVF4);
   if (split)
     hipLaunchKernelGGL(k_convf4<float>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
-                       c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
+                       c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ);
   else
     hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
-                       c->w_seg.as<DevSeg>(), c->w_r4.as<double>(), cnt + NJ);
+                       c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ);
   c->tend(th);
   Recs recs;
   {

@@ -642,13 +642,14 @@ __device__ __forceinline__ T f4_at(const F4Par<T> &p, double xmd) {
 
 template <typename T>
 __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S,
-                                                  const DevWin *wins, const DevSeg *segs,
+                                                  const DevWin *wins, const DevSeg *segs, const int32_t *worder,
                                                   double *r4out, unsigned long long *k3upd) {
   __shared__ T r4[R4ROW];
   __shared__ F4Par<T> par[2][64];
   __shared__ int s_nacc2[2], s_done2[2], s_b;
   const int tid = threadIdx.x;
-  const DevWin W = wins[blockIdx.x];
+  const int wi = worder[blockIdx.x];
+  const DevWin W = wins[wi];
   const int job = W.job;
   const DevLayer &Y = Ys[job];
   const int64_t o = job * S.line;
@@ -914,7 +915,7 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
     }
     __syncthreads();
   }
-  double *row = r4out + (size_t)blockIdx.x * R4ROW;
+  double *row = r4out + (size_t)wi * R4ROW;
   for (int j = tid; j < R4ROW; j += CF4_T) row[j] = (j >= 1 && j <= nptr4) ? (double)r4[j] : 0.;
 }
 
