@@ -640,10 +640,13 @@ __device__ __forceinline__ T f4_at(const F4Par<T> &p, double xmd) {
   return xmsq <= p.thr ? near : far;
 }
 
+// Five waves per SIMD (registers held to 96): with the 29 KB of LDS of the
+// FP64 build that keeps five windows per compute unit resident, so the
+// 1230 windows of a 50-layer column run in one round.
 template <typename T>
-__global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S,
-                                                  const DevWin *wins, const DevSeg *segs, const int32_t *worder,
-                                                  double *r4out, unsigned long long *k3upd) {
+__global__ __launch_bounds__(CF4_T) __attribute__((amdgpu_waves_per_eu(5)))
+void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const DevSeg *segs,
+              const int32_t *worder, double *r4out, unsigned long long *k3upd) {
   __shared__ T r4[R4ROW];
   __shared__ F4Par<T> par[2][64];
   __shared__ int s_nacc2[2], s_done2[2], s_b;
@@ -779,8 +782,12 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
       const int ts = tid - 64;
       const int nacc = s_nacc2[cur];
       const double step = CF4_S * dvr4;
+      // a line's parameters are read from LDS one line ahead, so the read
+      // overlaps the previous line's points
+      F4Par<T> qn = par[cur][0];
       for (int a = 0; a < nacc; a++) {
-        F4Par<T> q = par[cur][a];
+        const F4Par<T> q = qn;
+        qn = par[cur][a + 1 < nacc ? a + 1 : a];
         if (ts == 0) nupd += (unsigned long long)(q.jmax - q.jmin + 1);
         int j0 = q.jmin - 1;
         int r = j0 % CF4_S;
@@ -788,15 +795,31 @@ __global__ __launch_bounds__(CF4_T) void k_convf4(Tape9 T9, const DevLayer *Ys, 
         // XJJ - XNUI advanced by a running sum: a line spans a few strides,
         // so the sum's rounding stays within a few ulp of XJJ
         double xmd = (double)(first - 1) * dvr4 - q.xnui;
-        for (int j = first; j <= q.jmax; j += CF4_S, xmd += step) r4[j] = r4[j] + f4_at(q, xmd);
+        int j = first;
+        if constexpr (sizeof(T) == 4) {
+          // FP32 shapes: two points per step, two independent evaluations in
+          // flight per thread (a thread's points are distinct, so the order
+          // of additions to each point is unchanged).  The FP64 build is
+          // bound by FP64 issue, where this only adds instructions.
+          for (; j + CF4_S <= q.jmax; j += 2 * CF4_S) {
+            const double x1 = xmd + step;
+            const T v0 = f4_at(q, xmd), v1 = f4_at(q, x1);
+            xmd = x1 + step;
+            r4[j] = r4[j] + v0;
+            r4[j + CF4_S] = r4[j + CF4_S] + v1;
+          }
+        }
+        for (; j <= q.jmax; j += CF4_S, xmd += step) r4[j] = r4[j] + f4_at(q, xmd);
       }
     } else if (more) {
       const int nb = cur ^ 1;
       const double xjj_me = (double)(jj - 1) * dvr4;
       // the last chunk's accepted lines at this chunk's line centres
       const int nprev = s_nacc2[cur];
+      F4Par<T> qn = par[cur][0];
       for (int a = 0; a < nprev; a++) {
-        const F4Par<T> q = par[cur][a];
+        const F4Par<T> q = qn;
+        qn = par[cur][a + 1 < nprev ? a + 1 : a];
         if (valid && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me - q.xnui);
       }
       const bool cand = valid && nz;
