@@ -18,8 +18,11 @@
  * return with the results in place; the *_device calls queue their work on
  * the context's HIP stream and return (lblx_sync waits for it), so the
  * next call's host planning overlaps the queued kernels.  A context owns
- * device memory, the resident line table and one HIP stream; the library
- * plans on a process-wide pool of up to 15 host worker threads.
+ * device memory, the resident line table and its HIP stream, plus a second
+ * internal stream for work that overlaps (LNCOR1 beside SHRINK); that work
+ * is always joined back into the context stream within the same call, so
+ * waiting for the context stream waits for everything.  The library plans
+ * on a process-wide pool of up to 15 host worker threads.
  */
 #ifndef LBLX_H
 #define LBLX_H
@@ -328,7 +331,7 @@ int lblx_fftscn_device(lblx_ctx *ctx, const lblx_scan *scan, const double *d_in,
 int lblx_sync(lblx_ctx *ctx);
 void *lblx_stream(lblx_ctx *ctx);
 
-/* Stage timers: HIP events on the context stream around each stage
+/* Stage timers: HIP events around each stage, on the stream that runs it
  * (linf4, shrink, convf4, lncor1, reject, scatter_rec, tile_r3, tile_r2,
  * tile_r1, merge, contnm, scan), the device analogue of the TAPE6 timing
  * table (oprop.f90:547-620), and "host": wall milliseconds the host spends
