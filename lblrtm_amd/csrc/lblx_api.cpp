@@ -236,6 +236,7 @@ struct lblx_ctx {
       w_bat;
   DevBuf w_t9vnu, w_t9sp, w_t9alfa, w_t9epp, w_t9mol, w_t9n, w_t9blk, w_t9bvhi;
   DevBuf w_dirty;  // blocks whose read panel moved in the last fixpoint step
+  DevBuf w_t9s;    // TAPE9-block start tables of the batch walk (six ints per block)
   DevBuf w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
   DevBuf w_tmap[3];
   DevBuf w_mpan, w_rt, w_mrun;
@@ -371,7 +372,7 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->d_pshift, &c->d_mol, &c->d_blk, &c->d_bstart, &c->d_bvmax, &c->d_tabs, &c->d_cont, &c->w_cps,
                     &c->w_dlay, &c->w_pan, &c->w_win, &c->w_word, &c->w_seg, &c->w_r4, &c->w_margin,
                     &c->w_cx, &c->w_csp, &c->w_cal, &c->w_cad, &c->w_cco2, &c->w_pn, &c->w_pc,
-                    &c->w_g, &c->w_jmp, &c->w_wsum, &c->w_bcnt, &c->w_etab, &c->w_jobi, &c->w_bat,
+                    &c->w_g, &c->w_jmp, &c->w_wsum, &c->w_bcnt, &c->w_etab, &c->w_t9s, &c->w_jobi, &c->w_bat,
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
                     &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
@@ -932,7 +933,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_cco2, NJ * S.line) || need(c->w_pn, NJ * S.line * 4) ||
       need(c->w_pc, NJ * S.line * 4) || need(c->w_g, NJ * S.line * 4) ||
       need(c->w_jmp, NJ * S.line * 4 * 3) || need(c->w_wsum, NJ * S.line * 2) ||
-      need(c->w_bcnt, NJ * S.blk * 4 * 7) || need(c->w_etab, NJ * S.blk * 4 * (EBLK + 1)) ||
+      need(c->w_bcnt, NJ * S.blk * 4 * 7) || need(c->w_etab, NJ * S.blk * 4 * (EBLK + 1)) || need(c->w_t9s, NJ * S.blk * 4 * 6) ||
       need(c->w_jobi, NJ * 4 * 4) || need(c->w_bat, NJ * S.line * 4 * 3) ||
       need(c->w_t9vnu, NJ * S.line * 8) || need(c->w_t9sp, NJ * S.line * 8) ||
       need(c->w_t9alfa, NJ * S.line * 8) || need(c->w_t9epp, NJ * S.line * 8) ||
@@ -1013,6 +1014,13 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   W.bat_s = bat;
   W.bat_t = bat + NJ * S.line;
   W.bat_off = bat + 2 * NJ * S.line;
+  int32_t *t9s = c->w_t9s.as<int32_t>();
+  W.t9nxt = t9s;
+  W.t9nem = t9s + NJ * S.blk;
+  W.t9nbt = t9s + 2 * NJ * S.blk;
+  W.t9out = t9s + 3 * NJ * S.blk;
+  W.t9bat = t9s + 4 * NJ * S.blk;
+  W.t9k = t9s + 5 * NJ * S.blk;
   Tape9 T9;
   T9.vnu = c->w_t9vnu.as<double>();
   T9.sp = c->w_t9sp.as<double>();
@@ -1059,14 +1067,16 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     c->tend(thl, s2);
     HIPCHK(hipEventRecord(c->ev_join, s2));
   }
-  {  // block tables of a job in LDS when they fit (ints: prefix, ECACHE entries per block)
-    const int per = ECACHE + 1;
-    int lcap = std::min(maxnb4, (int)((150u << 10) / (4u * per)) - 1);
+  hipLaunchKernelGGL(k_shr_t9sim, dim3((bgrid + 63) / 64, njobs), dim3(64), 0, s, dY, S, W);
+  hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s, W.t9k, NJ * S.blk, -1);
+  {  // a job's start tables in LDS when they fit (three ints per block)
+    int lcap = std::min(maxnb4, (int)((150u << 10) / 12u));
     if (lcap < 1) lcap = 1;
-    const size_t shm = (size_t)(lcap * per + 1) * 4;
-    HIPCHK(hipFuncSetAttribute((const void *)k_shr_batches, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(k_shr_batches, dim3(njobs), dim3(64), shm, s, dY, S, W, T9, lcap);
+    const size_t shm = (size_t)lcap * 12;
+    HIPCHK(hipFuncSetAttribute((const void *)k_shr_t9chain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(k_shr_t9chain, dim3(njobs), dim3(64), shm, s, dY, S, W, T9, lcap);
   }
+  hipLaunchKernelGGL(k_shr_t9replay, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W, T9);
   hipLaunchKernelGGL(k_shr_bwalk, dim3(bgrid, njobs), dim3(64), 0, s, S, W);
   hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9);
   hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);
@@ -2287,7 +2297,7 @@ int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, con
 static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
   const size_t line = ((size_t)nline + 127) / 64 * 64;
   return line * (32 + 1 + 12 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(SRec1) + 2 * sizeof(SRecA) + sizeof(SRecV)) +
-         (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 1 + 3) + (size_t)(c->nlines / 1000 + 16) * 24;
+         (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 1 + 3 + 6) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
 
 // Many profiles in one call: OPDPTH for the layers of a group of profiles

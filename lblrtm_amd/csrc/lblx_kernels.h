@@ -85,6 +85,10 @@ struct ShrinkWork {
   int32_t *etab;          // [blk][EBLK] emitted lines of a batch of k+1 blocks
   int32_t *estop;         // [blk] emitted lines of the batch from the block to the stop line (-1: not tabled)
   int32_t *bat_s, *bat_t, *bat_off, *nbat;  // batches
+  // [blk] a TAPE9 block started at the block: where the next one starts
+  // (-1: none, -2: not tabled), the lines it emits, its batches; and for
+  // the starts on the job's chain: output offset, batch offset, index (-1: not a start)
+  int32_t *t9nxt, *t9nem, *t9nbt, *t9out, *t9bat, *t9k;
 };
 
 // Shrunk far-wing line list (TAPE9) and its block table, per job.

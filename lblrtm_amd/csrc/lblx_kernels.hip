@@ -4,7 +4,8 @@
 // src/oprop.f90:25-668).  Here every layer ("job") of a batch is resident
 // in HBM over its whole band and each stage is one launch for all jobs:
 //   LINF4 + SHRINK   k_shr_count, k_shr_scan, k_shr_compact, k_shr_chain,
-//                    k_shr_ewalk, k_shr_batches, k_shr_bwalk, k_shr_emit
+//                    k_shr_ewalk, k_shr_t9sim/t9chain/t9replay, k_shr_bwalk,
+//                    k_shr_emit
 //                    (oprop.f90:3093-3290, 3508-3640)
 //   LBLF4/CONVF4     k_convf4, one far-wing window per workgroup
 //                    (oprop.f90:3641-4039)
@@ -376,105 +377,233 @@ __global__ __launch_bounds__(64) void k_shr_ewalk(const DevLayer *Ys, Strides S,
   if (lane == 0) W.estop[ob + b] = es;
 }
 
-// one wave per job: LINF4's read loop over blocks (oprop.f90:3130-3290)
-// deciding batch boundaries and TAPE9 blocks from the emitted counts.
-// Blocks run from the job's first far-wing block; its last one holds the
-// stop line unless the file ends first.  A batch reads blocks until it
-// holds the stop line or J + (passing lines read) >= LIMIN4; both tests are
-// monotone in the block prefix bp, so the wave tests 64 blocks at once and
-// takes the first that ends the batch.
-// Jobs of up to lcap blocks keep their block prefixes and the first ECACHE
-// batch-table entries of each block in LDS for the walk.
-constexpr int ECACHE = 12;
-__global__ __launch_bounds__(64) void k_shr_batches(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9,
-                                                     int lcap) {
-  extern __shared__ int32_t sm_b[];
-  const int job = blockIdx.x;
-  const bool lead = threadIdx.x == 0;  // the wave walks together; lane 0 writes
-  const int64_t ob = job * S.blk;
-  const int64_t o = job * S.line;
-  const int64_t ot = job * S.t9blk;
-  const int stop_p = W.stop[job];
-  const int nblk = Ys[job].nb4;
-  const int lane = threadIdx.x & 63;
-  const bool inl = nblk <= lcap;
-  int32_t *l_bp = sm_b, *l_et = sm_b + lcap + 1;
-  if (inl) {
-    for (int i = lane; i <= nblk; i += 64) l_bp[i] = W.bp[ob + i];
-    for (int i = lane; i < nblk; i += 64)
-      for (int k = 0; k < ECACHE; k++) l_et[i * ECACHE + k] = W.etab[(ob + i) * EBLK + k];
-  }
-  __syncthreads();
-  auto bp = [&](int k) { return inl ? l_bp[k] : W.bp[ob + k]; };
-  int b = 0;
-  int J = 0, out = 0, nbat = 0, nt9 = 0, t9s = 0;
-  for (;;) {
-    const int bs = b;
-    const int s = bp(bs);
-    bool stop = false, eof = false;
-    int pe = 0;  // bp[b] after the batch's last block
-    if (bs >= nblk) {
-      eof = true;
-    } else {
-      const int lim = LIMIN4 - J + s;
-      for (int base = bs;; base += 64) {
-        const int k = base + lane;
-        const int pn = k < nblk ? bp(k + 1) : 0;
-        const bool hs = k < nblk && stop_p >= 0 && pn > stop_p;
-        const bool hit = hs || (k < nblk && pn >= lim);
-        const uint64_t m = __ballot(hit);
-        if (m) {
-          const int f = __ffsll((unsigned long long)m) - 1;
-          b = base + f + 1;
-          pe = __builtin_amdgcn_readlane(pn, f);
-          stop = (__ballot(hs) >> f) & 1;
-          break;
-        }
-        if (base + 64 >= nblk) {
-          b = nblk;
-          eof = true;
-          pe = bp(nblk);
-          break;
-        }
+// LINF4's read loop over blocks (oprop.f90:3130-3290) decides the batch
+// boundaries and TAPE9 blocks in order.  A batch reads blocks until it
+// holds the stop line or J + (passing lines read) >= LIMIN4, J being the
+// shrunk lines kept from the TAPE9 block's earlier batches; both tests are
+// monotone in the block prefix bp.  Blocks run from the job's first
+// far-wing block; its last one holds the stop line unless the file ends
+// first.  A TAPE9 block starts at a block boundary with J = 0, so the walk
+// runs in three launches:
+//   k_shr_t9sim    every block as a possible TAPE9-block start, one lane
+//                  each: where the next start is, the lines the block
+//                  emits and its batch count;
+//   k_shr_t9chain  one wave per job follows the starts from block 0;
+//   k_shr_t9replay one wave per start on the chain writes its batches and
+//                  its TAPE9 block entry.
+struct Batch {
+  int b, s, t;  // next block, passing-line range [s, t]
+  bool stop, eof;
+};
+
+// one batch from block bs, one lane alone
+__device__ __forceinline__ Batch batch_lane(const int32_t *bp, int nblk, int stop_p, int J, int bs) {
+  Batch B;
+  B.s = bp[bs];
+  B.stop = B.eof = false;
+  int pe = 0;  // bp after the batch's last block
+  if (bs >= nblk) {
+    B.eof = true;
+    B.b = bs;
+  } else {
+    const int lim = LIMIN4 - J + B.s;
+    int k = bs;
+    for (;;) {
+      const int pn = bp[k + 1];
+      if (stop_p >= 0 && pn > stop_p) {
+        B.stop = true;
+        pe = pn;
+        break;
+      }
+      if (pn >= lim) {
+        pe = pn;
+        break;
+      }
+      if (++k >= nblk) {
+        B.eof = true;
+        pe = pn;
+        break;
       }
     }
-    const int t = stop ? stop_p : pe - 1;
+    B.b = B.eof ? nblk : k + 1;
+  }
+  B.t = B.stop ? stop_p : pe - 1;
+  return B;
+}
+
+// the same batch, the wave testing 64 blocks at once
+__device__ __forceinline__ Batch batch_wave(const int32_t *bp, int nblk, int stop_p, int J, int bs) {
+  const int lane = threadIdx.x & 63;
+  Batch B;
+  B.s = bp[bs];
+  B.stop = B.eof = false;
+  int pe = 0;
+  B.b = bs;
+  if (bs >= nblk) {
+    B.eof = true;
+  } else {
+    const int lim = LIMIN4 - J + B.s;
+    for (int base = bs;; base += 64) {
+      const int k = base + lane;
+      const int pn = k < nblk ? bp[k + 1] : 0;
+      const bool hs = k < nblk && stop_p >= 0 && pn > stop_p;
+      const bool hit = hs || (k < nblk && pn >= lim);
+      const uint64_t m = __ballot(hit);
+      if (m) {
+        const int f = __ffsll((unsigned long long)m) - 1;
+        B.b = base + f + 1;
+        pe = __builtin_amdgcn_readlane(pn, f);
+        B.stop = (__ballot(hs) >> f) & 1;
+        break;
+      }
+      if (base + 64 >= nblk) {
+        B.b = nblk;
+        B.eof = true;
+        pe = bp[nblk];
+        break;
+      }
+    }
+  }
+  B.t = B.stop ? stop_p : pe - 1;
+  return B;
+}
+
+// Lines a batch emits from the block tables, or -1 when they do not cover it
+__device__ __forceinline__ int batch_em(const ShrinkWork &W, int64_t ob, const Batch &B, int bs) {
+  if (B.t < B.s) return 0;
+  const int nb = B.b - bs;
+  if (B.stop && nb <= EBLK && W.estop[ob + bs] >= 0) return W.estop[ob + bs];
+  return (!B.stop && nb >= 1 && nb <= EBLK) ? W.etab[(ob + bs) * EBLK + nb - 1] : -1;
+}
+
+// One TAPE9 block from block b0, the whole wave: its batches (a batch longer
+// than the tables takes the chain walk), written when WRITE with the block's
+// output offset out0, batch offset bat0 and TAPE9 block index k9.
+template <bool WRITE>
+__device__ void t9_block_wave(const ShrinkWork &W, const Tape9 &T9, int64_t ob, int64_t o, int64_t ot, int stop_p,
+                              int nblk, int npass, int b0, int out0, int bat0, int k9, int *nxt, int *nem,
+                              int *nbt) {
+  const bool lead = (threadIdx.x & 63) == 0;
+  int b = b0, J = 0, nbat = 0;
+  for (;;) {
+    const int bs = b;
+    const Batch B = batch_wave(W.bp + ob, nblk, stop_p, J, bs);
+    b = B.b;
     int em = 0;
-    if (t >= s) {
-      int nb = b - bs;
-      const int et = (!stop && nb >= 1 && nb <= EBLK)
-                         ? ((inl && nb <= ECACHE) ? l_et[bs * ECACHE + nb - 1] : W.etab[(ob + bs) * EBLK + nb - 1])
-                         : -1;
-      if (stop && nb <= EBLK && W.estop[ob + bs] >= 0) em = W.estop[ob + bs];
-      else if (et >= 0) em = et;
-      else em = ChainWalk(W, o, W.npass[job], s).count_to(t);  // a batch longer than the tables
-      if (lead) {
-        W.bat_s[o + nbat] = s;
-        W.bat_t[o + nbat] = t;
-        W.bat_off[o + nbat] = out;
+    if (B.t >= B.s) {
+      em = batch_em(W, ob, B, bs);
+      if (em < 0) em = ChainWalk(W, o, npass, B.s).count_to(B.t);
+      if (WRITE && lead) {
+        W.bat_s[o + bat0 + nbat] = B.s;
+        W.bat_t[o + bat0 + nbat] = B.t;
+        W.bat_off[o + bat0 + nbat] = out0 + J;
       }
       nbat++;
     }
-    out += em;
     J += em;
-    if (J >= LIMIN4 || stop || eof) {
-      if (J > 0) {
-        if (lead) {
-          T9.bstart[ot + nt9] = t9s;
-          T9.bcount[ot + nt9] = J;
-        }
-        nt9++;
+    if (J >= LIMIN4 || B.stop || B.eof) {
+      if (WRITE && lead && J > 0) {
+        T9.bstart[ot + k9] = out0;
+        T9.bcount[ot + k9] = J;
       }
-      t9s = out;
-      J = 0;
+      *nxt = (B.stop || B.eof) ? -1 : b;
+      *nem = J;
+      *nbt = nbat;
+      return;
     }
-    if (stop || eof) break;
   }
-  if (lead) {
+}
+
+__global__ __launch_bounds__(64) void k_shr_t9sim(const DevLayer *Ys, Strides S, ShrinkWork W) {
+  const int job = blockIdx.y;
+  const int nblk = Ys[job].nb4;
+  const int b0 = blockIdx.x * 64 + (int)threadIdx.x;
+  if (b0 >= nblk) return;
+  const int64_t ob = job * S.blk;
+  const int stop_p = W.stop[job];
+  const int32_t *bp = W.bp + ob;
+  int b = b0, J = 0, nbat = 0, nxt = -1;
+  for (;;) {
+    const int bs = b;
+    const Batch B = batch_lane(bp, nblk, stop_p, J, bs);
+    b = B.b;
+    if (B.t >= B.s) {
+      const int em = batch_em(W, ob, B, bs);
+      if (em < 0) {  // left to the chain wave
+        nxt = -2;
+        break;
+      }
+      J += em;
+      nbat++;
+    }
+    if (J >= LIMIN4 || B.stop || B.eof) {
+      nxt = (B.stop || B.eof) ? -1 : b;
+      break;
+    }
+  }
+  W.t9nxt[ob + b0] = nxt;
+  W.t9nem[ob + b0] = J;
+  W.t9nbt[ob + b0] = nbat;
+}
+
+// One wave per job; the start tables in LDS when the job has at most lcap
+// blocks.  t9k must hold -1 for every block on entry.
+__global__ __launch_bounds__(64) void k_shr_t9chain(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9,
+                                                    int lcap) {
+  extern __shared__ int32_t sm_c[];
+  const int job = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int nblk = Ys[job].nb4;
+  const int64_t ob = job * S.blk, o = job * S.line, ot = job * S.t9blk;
+  const int stop_p = W.stop[job], npass = W.npass[job];
+  const bool inl = nblk <= lcap;
+  int32_t *l_nxt = sm_c, *l_nem = sm_c + lcap, *l_nbt = sm_c + 2 * lcap;
+  if (inl)
+    for (int i = lane; i < nblk; i += 64) {
+      l_nxt[i] = W.t9nxt[ob + i];
+      l_nem[i] = W.t9nem[ob + i];
+      l_nbt[i] = W.t9nbt[ob + i];
+    }
+  __syncthreads();
+  int b = 0, out = 0, nbat = 0, k = 0;
+  while (b < nblk) {
+    int nx = inl ? l_nxt[b] : W.t9nxt[ob + b];
+    int J, nb;
+    if (nx == -2) {
+      t9_block_wave<false>(W, T9, ob, o, ot, stop_p, nblk, npass, b, 0, 0, 0, &nx, &J, &nb);
+    } else {
+      J = inl ? l_nem[b] : W.t9nem[ob + b];
+      nb = inl ? l_nbt[b] : W.t9nbt[ob + b];
+    }
+    if (lane == 0) {
+      W.t9k[ob + b] = k;
+      W.t9out[ob + b] = out;
+      W.t9bat[ob + b] = nbat;
+    }
+    out += J;
+    nbat += nb;
+    if (J > 0) k++;
+    if (nx < 0) break;
+    b = nx;
+  }
+  if (lane == 0) {
     W.nbat[job] = nbat;
     T9.n[job] = out;
-    T9.nblk[job] = nt9;
+    T9.nblk[job] = k;
   }
+}
+
+__global__ __launch_bounds__(64) void k_shr_t9replay(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9) {
+  const int job = blockIdx.y;
+  const int b0 = blockIdx.x;
+  const int nblk = Ys[job].nb4;
+  if (b0 >= nblk) return;
+  const int64_t ob = job * S.blk;
+  const int k = W.t9k[ob + b0];
+  if (k < 0) return;
+  int nx, J, nb;
+  t9_block_wave<true>(W, T9, ob, job * S.line, job * S.t9blk, W.stop[job], nblk, W.npass[job], b0,
+                      W.t9out[ob + b0], W.t9bat[ob + b0], k, &nx, &J, &nb);
 }
 
 // per (job, batch), one wave each: the batch's chain.  Each group that
