@@ -1521,16 +1521,19 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
       int64_t a = glo > T0 ? glo : T0;
       int64_t b = glo + npts - 1;
       if (b > T0 + tile - 1) b = T0 + tile - 1;
-      for (int64_t k = a + lane; k <= b; k += 64) {
-        int off = (int)(k - glo);
+      // tile-local point index kl; the line's point offset is kl + d0
+      // (|d0| is at most the tile width or the line's point count)
+      const int kb = (int)(b - T0), d0 = (int)(T0 - glo);
+      for (int kl = (int)(a - T0) + lane; kl <= kb; kl += 64) {
+        const int off = kl + d0;
         double zf = zfl + (double)(off + 1) * zslope;
         int iz = fint(fabs(zf) + 1.5) - 1;
         iz = iz > NFMX - 1 ? NFMX - 1 : iz;  // |zf| <= 2500 by construction
         ACC v;
         if (V) v = str * ((((ACC)1. - za) * ca[iz]) + (za * ca[iz + NFMX]));
-        else if (G == 1) v = str * t1[iz] + strd * tg[iz] + strver * tv[iz];
+        else if (G == 1) v = fma(str, t1[iz], fma(strd, tg[iz], strver * tv[iz]));
         else v = str * t1[iz];
-        atomicAdd(&acc[k - T0], (LACC)v);
+        atomicAdd(&acc[kl], (LACC)v);
       }
     }
     cur = nxt;
