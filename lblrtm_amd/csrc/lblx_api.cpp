@@ -13,6 +13,7 @@
 #include <mutex>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -99,6 +100,9 @@ static const char *kStageNames[ST_N] = {"linf4", "shrink", "convf4", "lncor1", "
 struct lblx_ctx {
   int device = 0;
   int ncu = 256;  // compute units of the device
+  // LBLX_SHRINK_WAVE_WALK=1 at creation: every TAPE9 block of the batch
+  // walk is simulated by the chain wave (tests compare it with the tables)
+  int t9_all_wave = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
@@ -304,6 +308,10 @@ int lblx_create(lblx_ctx **out, int device) {
   HIPCHK(hipSetDevice(device));
   lblx_ctx *c = new lblx_ctx();
   c->device = device;
+  {
+    const char *e = std::getenv("LBLX_SHRINK_WAVE_WALK");
+    c->t9_all_wave = (e && std::atoi(e) != 0) ? 1 : 0;
+  }
   {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
@@ -1067,7 +1075,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     c->tend(thl, s2);
     HIPCHK(hipEventRecord(c->ev_join, s2));
   }
-  hipLaunchKernelGGL(k_shr_t9sim, dim3((bgrid + 63) / 64, njobs), dim3(64), 0, s, dY, S, W);
+  hipLaunchKernelGGL(k_shr_t9sim, dim3((bgrid + 63) / 64, njobs), dim3(64), 0, s, dY, S, W, c->t9_all_wave);
   hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s, W.t9k, NJ * S.blk, -1);
   {  // a job's start tables in LDS when they fit (three ints per block)
     int lcap = std::min(maxnb4, (int)((150u << 10) / 12u));

@@ -514,12 +514,17 @@ __device__ void t9_block_wave(const ShrinkWork &W, const Tape9 &T9, int64_t ob, 
   }
 }
 
-__global__ __launch_bounds__(64) void k_shr_t9sim(const DevLayer *Ys, Strides S, ShrinkWork W) {
+// all_wave: leave every start to the chain wave (a test of that path)
+__global__ __launch_bounds__(64) void k_shr_t9sim(const DevLayer *Ys, Strides S, ShrinkWork W, int all_wave) {
   const int job = blockIdx.y;
   const int nblk = Ys[job].nb4;
   const int b0 = blockIdx.x * 64 + (int)threadIdx.x;
   if (b0 >= nblk) return;
   const int64_t ob = job * S.blk;
+  if (all_wave) {
+    W.t9nxt[ob + b0] = -2;
+    return;
+  }
   const int stop_p = W.stop[job];
   const int32_t *bp = W.bp + ob;
   int b = b0, J = 0, nbat = 0, nxt = -1;

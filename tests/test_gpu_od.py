@@ -148,3 +148,52 @@ def test_tape3_file_and_errors(gpu_ctx, tmp_path):
     with pytest.raises(lib.LblxError) as e:
         gpu_ctx.load_lines(bad2)
     assert e.value.code == abi.E_NREC
+
+
+def _tape9_oracle(run, lay, tab):
+    cap = 100000
+    arrs = [np.zeros(cap) for _ in range(4)] + [np.zeros(cap, np.int32)]
+    nb = C.c_int(0)
+    blk = np.zeros(1000, np.int32)
+    n_o = O.lib().oracle_linf4(C.byref(run), C.byref(abi.Layer.from_buffer_copy(lay)), C.byref(tab.struct()),
+                               *[abi.ptr(a, C.c_double) for a in arrs[:4]], abi.ptr(arrs[4], C.c_int32),
+                               abi.ptr(blk, C.c_int32), cap, 1000, C.byref(nb))
+    return [a[:n_o] for a in arrs]
+
+
+@pytest.mark.parametrize("case", ["eof", "stop"])
+def test_batch_walk_wave_path(monkeypatch, case):
+    """LINF4's batch walk with every TAPE9 block simulated by the per-job
+    chain wave instead of the per-block tables (the path the tables leave
+    to it, forced here by LBLX_SHRINK_WAVE_WALK at context creation): the
+    TAPE9 list and the OD equal the oracle's, with the band ending at the
+    end of the line file (eof) and before it (the stop line)."""
+    monkeypatch.setenv("LBLX_SHRINK_WAVE_WALK", "1")
+    ctx = lib.Context(0)
+    try:
+        if case == "eof":
+            tab = synth.config0_lines()
+            p, t, wk, wb = _layer0()
+            v1, v2 = 600.0, 1100.0
+        else:
+            tab = synth.synth_lines(1234, 600.0, 700.0, [1, 2, 7], 11)
+            p, t, wk, wb = 800.0, 280.0, [1e22, 3e21, 2e24], 1.0e25
+            v1, v2 = 610.0, 660.0  # lines continue past V2 + BOUND4
+        ctx.load_lines(tab)
+        run = abi.make_run(v1, v2)
+        dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+        lay = abi.make_layer(p, t, wk, wb, dv[0], v1, v2)
+        print("batch walk, wave path:", case, flush=True)
+        od, ref, st, st_r = _run_both(ctx, run, lay, tab)
+        assert_od_close(od, ref, "wave walk " + case)
+        assert st["nlins4"] == st_r["nlins4"]
+        ov, osp, oal, oad, om = _tape9_oracle(run, lay, tab)
+        gv, gs, ga, ge, gm = ctx.debug_tape9(0)
+        assert len(gv) == len(ov) > 0
+        np.testing.assert_array_equal(gm, om)
+        np.testing.assert_allclose(gv, ov, rtol=1e-14)
+        np.testing.assert_allclose(gs, osp, rtol=1e-13)
+        np.testing.assert_allclose(ga, oal, rtol=1e-13)
+        np.testing.assert_allclose(ge, oad, rtol=1e-13)
+    finally:
+        ctx.close()
