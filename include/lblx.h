@@ -19,7 +19,7 @@
  * the context's HIP stream and return (lblx_sync waits for it), so the
  * next call's host planning overlaps the queued kernels.  A context owns
  * device memory, the resident line table and its HIP stream, plus a second
- * internal stream for work that overlaps (LNCOR1 beside SHRINK); that work
+ * internal stream for work that overlaps (LNCOR1 beside CONVF4); that work
  * is always joined back into the context stream within the same call, so
  * waiting for the context stream waits for everything.  The library plans
  * on a process-wide pool of up to 15 host worker threads.

@@ -1043,6 +1043,9 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   unsigned long long *cnt = c->w_cnt.as<unsigned long long>();  // [3][njobs]: k1, k3, lines
   hipLaunchKernelGGL(k_fill_i32, dim3((6 * NJ + 255) / 256), dim3(256), 0, s, (int *)cnt, 6 * NJ, 0);
 
+  NearArr nearr;
+  nearr.hot = c->w_near.as<NearHot>();
+  nearr.cold = (NearCold *)(c->w_near.as<char>() + (size_t)NJ * S.line * sizeof(NearHot));
   // 1-2. LINF4 + SHRINK
   int th = c->tbegin(ST_SHRINK);
   const int lgrid = (int)((maxline + 1 + 255) / 256);
@@ -1052,13 +1055,25 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   hipLaunchKernelGGL(k_shr_compact, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
   hipLaunchKernelGGL(k_shr_chain, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_ewalk, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W);
-  NearArr nearr;
-  nearr.hot = c->w_near.as<NearHot>();
-  nearr.cold = (NearCold *)(c->w_near.as<char>() + (size_t)NJ * S.line * sizeof(NearHot));
+  hipLaunchKernelGGL(k_shr_t9sim, dim3((bgrid + 63) / 64, njobs), dim3(64), 0, s, dY, S, W, c->t9_all_wave);
+  hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s, W.t9k, NJ * S.blk, -1);
+  {  // a job's start tables in LDS when they fit (three ints per block)
+    int lcap = std::min(maxnb4, (int)((150u << 10) / 12u));
+    if (lcap < 1) lcap = 1;
+    const size_t shm = (size_t)lcap * 12;
+    HIPCHK(hipFuncSetAttribute((const void *)k_shr_t9chain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(k_shr_t9chain, dim3(njobs), dim3(64), shm, s, dY, S, W, T9, lcap);
+  }
+  hipLaunchKernelGGL(k_shr_t9replay, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W, T9);
+  hipLaunchKernelGGL(k_shr_bwalk, dim3(bgrid, njobs), dim3(64), 0, s, S, W);
+  hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9);
+  hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);
+  c->tend(th);
   if (maxnn > 0) {
     // LNCOR1 reads only the line table and the layer/panel tables, so it runs
-    // on the side stream beside SHRINK's batch walk (one wave per job, most
-    // of the chip idle) and CONVF4; the rejection fixpoint waits for it
+    // on the side stream beside CONVF4, which is bound by FP64 issue and
+    // moves little HBM traffic (beside SHRINK it would slow SHRINK's
+    // streaming passes); the rejection fixpoint waits for it
     hipStream_t s2 = c->side;
     HIPCHK(hipEventRecord(c->ev_fork, s));
     HIPCHK(hipStreamWaitEvent(s2, c->ev_fork, 0));
@@ -1075,20 +1090,6 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     c->tend(thl, s2);
     HIPCHK(hipEventRecord(c->ev_join, s2));
   }
-  hipLaunchKernelGGL(k_shr_t9sim, dim3((bgrid + 63) / 64, njobs), dim3(64), 0, s, dY, S, W, c->t9_all_wave);
-  hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s, W.t9k, NJ * S.blk, -1);
-  {  // a job's start tables in LDS when they fit (three ints per block)
-    int lcap = std::min(maxnb4, (int)((150u << 10) / 12u));
-    if (lcap < 1) lcap = 1;
-    const size_t shm = (size_t)lcap * 12;
-    HIPCHK(hipFuncSetAttribute((const void *)k_shr_t9chain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(k_shr_t9chain, dim3(njobs), dim3(64), shm, s, dY, S, W, T9, lcap);
-  }
-  hipLaunchKernelGGL(k_shr_t9replay, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W, T9);
-  hipLaunchKernelGGL(k_shr_bwalk, dim3(bgrid, njobs), dim3(64), 0, s, S, W);
-  hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9);
-  hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);
-  c->tend(th);
   // 3. far-wing windows
   th = c->tbegin(ST_CONVF4);
   if (split)
