@@ -950,7 +950,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_near, NJ * S.line * (sizeof(NearHot) + sizeof(NearCold))) || need(c->w_keep, NJ * S.line * 4) ||
       need(c->w_blkmax, NJ * S.blk * 4) || need(c->w_pread, NJ * S.blk * 4) ||
       need(c->w_dirty, NJ * S.blk * 4) ||
-      need(c->w_changed, 16) || need(c->w_rec, NJ * S.line * (sizeof(SRec1) + 2 * sizeof(SRecA) + (voigt ? sizeof(SRecV) : 0))) ||
+      need(c->w_changed, 64 * 4) || need(c->w_rec, NJ * S.line * (sizeof(SRec1) + 2 * sizeof(SRecA) + (voigt ? sizeof(SRecV) : 0))) ||
       need(c->w_r3, (size_t)r3tot * 8) || need(c->w_r2, (size_t)r2tot * 8) ||
       need(c->w_cnt, NJ * 24) || need(c->w_margin, NJ * 3 * 8) ||
       (abs_stride > 0 && (need(c->w_absrb, NJ * abs_stride * 8) ||
@@ -1113,18 +1113,27 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     // 4-5. block-read panel fixpoint, after LNCOR1 on the side stream
     HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
     th = c->tbegin(ST_REJECT);
+    // Pass it raises ch[it] when a read panel moved; the fixpoint holds at
+    // the first pass after the first that moves none.  The first passes
+    // are queued without a host check: from the third on, a pass whose
+    // predecessor moved nothing is a no-op (nothing is dirty) and returns
+    // at once, so ch[it] of the last queued pass is 0 exactly when the
+    // fixpoint has been reached.
+    constexpr int QUEUED = 4;
+    int *ch = c->w_changed.as<int>();
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ch, 64, 0);
     bool converged = false;
     for (int it = 0; it < 64; it++) {
-      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, c->w_changed.as<int>(), 1, 0);
+      const int *skip = it >= 2 ? ch + it - 1 : nullptr;
       hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, (const NearHot *)nearr.hot, L, dY,
                          S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
                          c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>(),
-                         c->w_dirty.as<int>());
+                         c->w_dirty.as<int>(), skip);
       hipLaunchKernelGGL(k_pread_scan, dim3(njobs), dim3(256), 0, s, dY, S, c->w_blkmax.as<int>(),
-                         c->w_pread.as<int>(), c->w_changed.as<int>(), c->w_dirty.as<int>());
-      if (it >= 1) {
+                         c->w_pread.as<int>(), ch + it, c->w_dirty.as<int>(), skip);
+      if (it >= QUEUED - 1) {
         int changed = 1;
-        HIPCHK(hipMemcpyAsync(&changed, c->w_changed.p, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&changed, ch + it, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (!changed) {
           converged = true;  // this pass already used the final read panels
@@ -1137,7 +1146,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, (const NearHot *)nearr.hot, L, dY, S,
                          c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
                          c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>(),
-                         c->w_dirty.as<int>());
+                         c->w_dirty.as<int>(), nullptr);
     c->tend(th);
     // 6. CNVFNV scatter records
     th = c->tbegin(ST_SCATTER);

@@ -1159,7 +1159,9 @@ __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPan
 // per-block maximum is reduced across the wave before the atomic.
 __global__ __launch_bounds__(256) void k_reject(const NearHot *nl, DevLines L, const DevLayer *Ys, Strides S,
                                                 const int *pread, const DevPanel *pans, const DevWin *wins,
-                                                const double *r4, int *keep_out, int *blkmax, const int *dirty) {
+                                                const double *r4, int *keep_out, int *blkmax, const int *dirty,
+                                                const int *skip) {
+  if (skip && *skip == 0) return;  // the previous pass moved no read panel: nothing is dirty
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1205,7 +1207,8 @@ __global__ __launch_bounds__(256) void k_reject(const NearHot *nl, DevLines L, c
 // Raises *changed when the guess moved and marks the blocks whose read
 // panel moved dirty (their block maximum is reset for the next pass).
 __global__ __launch_bounds__(256) void k_pread_scan(const DevLayer *Ys, Strides S, int *blkmax,
-                                                    int *pread, int *changed, int *dirty) {
+                                                    int *pread, int *changed, int *dirty, const int *skip) {
+  if (skip && *skip == 0) return;  // as in k_reject: the pass is a no-op
   __shared__ int buf[256];
   __shared__ int carry;
   const int job = blockIdx.x;
