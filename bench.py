@@ -80,7 +80,7 @@ STAGE_KERNEL = {
     "tile_r1": "k_tile<1, double, double, false, double>",
     "tile_r2": "k_tile<2, double, double, false, double>",
     "tile_r3": "k_tile<3, double, double, false, double>",
-    "merge": "k_merge<double>",
+    "merge": "k_merge_run<double>",
     "scatter_rec": "k_scatter_rec",
     "lncor1": "k_lncor1",
 }
