@@ -98,7 +98,13 @@ typedef struct lblx_run {
                                weights); 1: the tabulated exact-Voigt build
                                src/oprop_voigt.f90 (voigt_init tables over
                                101 zeta, CNVFNV:1207-1393, CONVF4:3807-3999) */
-  int32_t pad0;
+  int32_t ibrd;             /* IBRD of record 1.2 (lblrtm.f90:676): > 0 uses
+                               the TAPE3 broadener widths and shifts
+                               (LNCOR1 oprop.f90:1069-1096, LINF4
+                               :3164-3221).  Not modelled: an OD call with
+                               ibrd > 0 on a line table that carries
+                               broadener flags returns LBLX_E_UNSUPPORTED;
+                               without flags IBRD changes nothing           */
 } lblx_run;
 
 /* One layer, as /FILHDR/ holds it when OPDPTH is called.                */
@@ -127,6 +133,13 @@ typedef struct lblx_stats {
   int64_t k1_updates;       /* near-wing (R1,R2,R3) point updates          */
   int64_t k3_updates;       /* far-wing R4 point updates                   */
   int64_t nwin4;            /* far-wing (LBLF4) windows computed           */
+  int64_t nfix;             /* passes of the block-read fixpoint that
+                               decides LNCOR1's rejections (the whole
+                               batch's count; 0 without near lines)        */
+  int64_t fixseq;           /* 1: the fixpoint had not settled within its
+                               pass budget (LBLX_FIX_MAXPASS, default 16)
+                               and the decisions were taken block by block
+                               in RDLIN's order instead; same results      */
 } lblx_stats;
 
 /* ------------------------------------------------------------------ */

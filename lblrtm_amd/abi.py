@@ -64,7 +64,7 @@ class Run(C.Structure):
         ("ifrgnx", C.c_int32),
         ("iaersl", C.c_int32),
         ("voigt", C.c_int32),
-        ("pad0", C.c_int32),
+        ("ibrd", C.c_int32),
     ]
 
 
@@ -93,6 +93,8 @@ class Stats(C.Structure):
         ("k1_updates", C.c_int64),
         ("k3_updates", C.c_int64),
         ("nwin4", C.c_int64),
+        ("nfix", C.c_int64),
+        ("fixseq", C.c_int64),
     ]
 
     def as_dict(self):
@@ -195,7 +197,7 @@ class LineTable:
 
 def make_run(v1, v2, sample=4.0, alfal0=0.04, avmass=36.0, dptmin=2e-4,
              dptfac=1e-3, ilblf4=1, icntnm=0, irad=0, precision=FP64, cntscl=None,
-             ifrgnx=0, iaersl=0, voigt=0):
+             ifrgnx=0, iaersl=0, voigt=0, ibrd=0):
     r = Run()
     r.v1, r.v2 = v1, v2
     r.sample, r.alfal0, r.avmass = sample, alfal0, avmass
@@ -205,6 +207,7 @@ def make_run(v1, v2, sample=4.0, alfal0=0.04, avmass=36.0, dptmin=2e-4,
         r.cntscl[i] = x
     r.ifrgnx, r.iaersl = ifrgnx, iaersl
     r.voigt = voigt
+    r.ibrd = ibrd
     return r
 
 

@@ -103,6 +103,10 @@ struct lblx_ctx {
   // LBLX_SHRINK_WAVE_WALK=1 at creation: every TAPE9 block of the batch
   // walk is simulated by the chain wave (tests compare it with the tables)
   int t9_all_wave = 0;
+  // block-read fixpoint: pass budget before the sequential walk takes over
+  // (LBLX_FIX_MAXPASS at creation, 1..64), and what the last OD batch used
+  int fix_maxpass = 16;
+  int fix_passes = 0, fix_seq = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
@@ -225,6 +229,7 @@ struct lblx_ctx {
   int64_t last_line_stride = 0;      // per-job line stride of the last OD run
   double max_pshift = 0.;
   int nlines = 0, nblk = 0;
+  int64_t nbrd = 0;  // TAPE3 lines carrying broadener flags (IBRD data)
   bool sorted = true;
   // device line table
   DevBuf d_vnu, d_sp, d_alfa, d_epp, d_hwhm, d_tmpalf, d_pshift, d_mol, d_blk, d_bstart, d_bvmax;
@@ -311,6 +316,8 @@ int lblx_create(lblx_ctx **out, int device) {
   {
     const char *e = std::getenv("LBLX_SHRINK_WAVE_WALK");
     c->t9_all_wave = (e && std::atoi(e) != 0) ? 1 : 0;
+    const char *f = std::getenv("LBLX_FIX_MAXPASS");
+    if (f) c->fix_maxpass = std::min(64, std::max(1, std::atoi(f)));
   }
   {
     int ncu = 0;
@@ -462,6 +469,7 @@ int lblx_load_lines(lblx_ctx *c, const lblx_lines *L) {
   c->nblk = L->nblk;
   c->max_pshift = mp;
   c->sorted = sorted;
+  c->nbrd = 0;
   return LBLX_OK;
 }
 
@@ -730,6 +738,7 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
                      const int64_t *od_off, const LayerPlan *pre, JobsPrep &J) {
   if (c->nlines <= 0) return LBLX_E_NOLINES;
   if (njobs <= 0) return LBLX_E_ARG;
+  if (run->ibrd > 0 && c->nbrd > 0) return LBLX_E_UNSUPPORTED;  // broadener widths and shifts
   const int n = c->nlines, nblk = c->nblk;
   J.njobs = njobs;
   if (!pre && (int)J.own.size() < njobs) J.own.resize(njobs);  // plans keep their storage
@@ -1108,6 +1117,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     recs.r3 = recs.r2 + nl;
     recs.v = voigt ? (SRecV *)(rb + nl * (sizeof(SRec1) + 2 * sizeof(SRecA))) : nullptr;
   }
+  c->fix_passes = 0;
+  c->fix_seq = 0;
   if (maxnn > 0) {
     const int ngrid = (maxnn + 255) / 256;
     // 4-5. block-read panel fixpoint, after LNCOR1 on the side stream
@@ -1123,7 +1134,9 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     int *ch = c->w_changed.as<int>();
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ch, 64, 0);
     bool converged = false;
-    for (int it = 0; it < 64; it++) {
+    int passes = 0;
+    const int maxpass = c->fix_maxpass;
+    for (int it = 0; it < maxpass; it++) {
       const int *skip = it >= 2 ? ch + it - 1 : nullptr;
       hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, (const NearHot *)nearr.hot, L, dY,
                          S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
@@ -1131,22 +1144,28 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
                          c->w_dirty.as<int>(), skip);
       hipLaunchKernelGGL(k_pread_scan, dim3(njobs), dim3(256), 0, s, dY, S, c->w_blkmax.as<int>(),
                          c->w_pread.as<int>(), ch + it, c->w_dirty.as<int>(), skip);
-      if (it >= QUEUED - 1) {
-        int changed = 1;
-        HIPCHK(hipMemcpyAsync(&changed, ch + it, 4, hipMemcpyDeviceToHost, s));
+      if (it >= QUEUED - 1 || it == maxpass - 1) {
+        int flags[64];
+        HIPCHK(hipMemcpyAsync(flags, ch, (it + 1) * 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        if (!changed) {
+        if (!flags[it]) {
           converged = true;  // this pass already used the final read panels
+          passes = it + 1;
+          while (passes > 1 && !flags[passes - 2]) passes--;  // queued no-op passes
           break;
         }
       }
     }
-    // keep flags for the converged read panels
-    if (!converged)
-      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, (const NearHot *)nearr.hot, L, dY, S,
+    // Not settled within the budget (a read panel that moves back and forth
+    // between passes): solve the recurrence block by block instead.
+    if (!converged) {
+      passes = maxpass;
+      hipLaunchKernelGGL(k_reject_seq, dim3(njobs), dim3(64), 0, s, (const NearHot *)nearr.hot, L, dY, S,
                          c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
-                         c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>(),
-                         c->w_dirty.as<int>(), nullptr);
+                         c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
+    }
+    c->fix_passes = passes;
+    c->fix_seq = converged ? 0 : 1;
     c->tend(th);
     // 6. CNVFNV scatter records
     th = c->tbegin(ST_SCATTER);
@@ -1234,6 +1253,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       st[jb].npanel = (int64_t)plans[jb]->panels.size();
       st[jb].npts = plans[jb]->npts;
       st[jb].nwin4 = (int64_t)plans[jb]->wins.size();
+      st[jb].nfix = c->fix_passes;
+      st[jb].fixseq = c->fix_seq;
     }
   }
   return LBLX_OK;
@@ -2781,6 +2802,7 @@ extern "C" int lblx_load_tape3(lblx_ctx *c, const char *path) {
   std::vector<double> vmin, vmax, vnu;
   std::vector<float> sp, alfa, epp, hwhm, tmpalf, pshift;
   const int R = LBLX_LINREC;
+  int64_t nbrd = 0;  // lines with a broadener flag
   while (rec(&b, &len)) {
     if (len < 24) return LBLX_E_ARG;
     double v0, v1;
@@ -2810,6 +2832,29 @@ extern "C" int lblx_load_tape3(lblx_ctx *c, const char *path) {
     take(tmpalf, 4);
     take(pshift, 4);
     take(iflg, 4);
+    // BRD_MOL_FLG_IN(7,250) then BRD_MOL_DAT(21,250): per molecule j the
+    // width, its temperature exponent and the shift (RDLIN,
+    // oprop.f90:801-810).  RDLIN and RDLNFL convert an O2 line's shift to
+    // the foreign shift whenever its own O2 flag is set, whatever IBRD
+    // (oprop.f90:818-827, 3473-3477); the other broadener data enter only
+    // with IBRD > 0 (LNCOR1 :1069-1096, LINF4 :3164-3221).
+    const char *fl = b + o;
+    const char *dat = fl + (size_t)R * 7 * 4;
+    const size_t base = pshift.size() - R;
+    for (int i = 0; i < n; i++) {
+      int32_t f[7];
+      std::memcpy(f, fl + (size_t)i * 28, 28);
+      bool any = false;
+      for (int j = 0; j < 7; j++) any = any || f[j] != 0;
+      if (any) nbrd++;
+      const int m = mol[base + i] % 100;
+      if (m == 7 && iflg[base + i] >= 0 && f[6] > 0) {
+        float shft7;
+        std::memcpy(&shft7, dat + ((size_t)i * 21 + 6 * 3 + 2) * 4, 4);
+        const double rvmr = 0.21;
+        pshift[base + i] = (float)(((double)pshift[base + i] - rvmr * (double)shft7) / (1.0 - rvmr));
+      }
+    }
   }
   lblx_lines L;
   L.nblk = (int32_t)nrec.size();
@@ -2825,5 +2870,7 @@ extern "C" int lblx_load_tape3(lblx_ctx *c, const char *path) {
   L.tmpalf = tmpalf.data();
   L.pshift = pshift.data();
   L.iflg = iflg.data();
-  return lblx_load_lines(c, &L);
+  int rc = lblx_load_lines(c, &L);
+  if (rc == LBLX_OK) c->nbrd = nbrd;
+  return rc;
 }

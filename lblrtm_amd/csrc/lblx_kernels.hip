@@ -1151,12 +1151,31 @@ __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPan
   out.put(job * S.line + t, o);
 }
 
+// LNCOR1's VFT test and R4 rejection (oprop.f90:1120, 1153-1177) of one
+// near line when its block is read at panel pr.
+__device__ __forceinline__ int reject_one(const NearHot &o, int pr, const DevLayer &Y, const DevPanel *pans,
+                                          const DevWin *wins, const double *r4) {
+  int keep = o.keep;
+  if (!keep) return 0;
+  int pq = pr < Y.npanel ? pr : Y.npanel - 1;
+  const DevPanel &pp = pans[Y.pan0 + pq];
+  double alfv = 1. / o.recalf;
+  if (64. * alfv + o.vnu < pp.vft) return 0;
+  int w = Y.win0 + pp.w;
+  DevWin W = wins[w];
+  int jj = fint((o.vnu - W.v1r4) / Y.dvr4 + 1.);
+  jj = jj < 1 ? 1 : (jj > W.nptr4 ? W.nptr4 : jj);
+  double r = r4[(size_t)w * R4ROW + jj];
+  if (o.speak <= (Y.dptmn_hirac + Y.dptfc * r)) return 0;
+  return 1;
+}
+
 // One step of the block-read fixpoint: with the current guess of the panel
 // at which each block is read, apply LNCOR1's VFT test and R4 rejection
-// (oprop.f90:1120, 1153-1177) and reduce the first-fit panel of the kept
-// lines per block.  Only blocks whose read panel changed in the last step
-// (dirty) are redone; the others keep their flags and block maxima.  The
-// per-block maximum is reduced across the wave before the atomic.
+// and reduce the first-fit panel of the kept lines per block.  Only blocks
+// whose read panel changed in the last step (dirty) are redone; the others
+// keep their flags and block maxima.  The per-block maximum is reduced
+// across the wave before the atomic.
 __global__ __launch_bounds__(256) void k_reject(const NearHot *nl, DevLines L, const DevLayer *Ys, Strides S,
                                                 const int *pread, const DevPanel *pans, const DevWin *wins,
                                                 const double *r4, int *keep_out, int *blkmax, const int *dirty,
@@ -1171,22 +1190,7 @@ __global__ __launch_bounds__(256) void k_reject(const NearHot *nl, DevLines L, c
   int val = -1;
   if (act) {
     const NearHot o = nl[job * S.line + t];
-    int keep = o.keep;
-    if (keep) {
-      int pr = pread[job * S.blk + k];
-      int pq = pr < Y.npanel ? pr : Y.npanel - 1;
-      const DevPanel &pp = pans[Y.pan0 + pq];
-      double alfv = 1. / o.recalf;
-      if (64. * alfv + o.vnu < pp.vft) keep = 0;
-      if (keep) {
-        int w = Y.win0 + pp.w;
-        DevWin W = wins[w];
-        int jj = fint((o.vnu - W.v1r4) / Y.dvr4 + 1.);
-        jj = jj < 1 ? 1 : (jj > W.nptr4 ? W.nptr4 : jj);
-        double r = r4[(size_t)w * R4ROW + jj];
-        if (o.speak <= (Y.dptmn_hirac + Y.dptfc * r)) keep = 0;
-      }
-    }
+    const int keep = reject_one(o, pread[job * S.blk + k], Y, pans, wins, r4);
     keep_out[job * S.line + t] = keep;
     if (keep) val = o.ff;
   }
@@ -1200,6 +1204,40 @@ __global__ __launch_bounds__(256) void k_reject(const NearHot *nl, DevLines L, c
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
     if (lane == leader) atomicMax(&blkmax[job * S.blk + kl], m);
     pend &= ~__ballot(mine);
+  }
+}
+
+// The same decisions in RDLIN's order: block k is read at the exclusive
+// prefix maximum of the kept first-fit panels of blocks 0..k-1, and its
+// lines are kept or rejected at that panel before block k+1 is considered.
+// One wave per job walks its blocks in file order (at most 250 lines a
+// block, four per lane).  This is the recurrence the fixpoint solves; it
+// runs when the fixpoint has not settled within its pass budget, and
+// leaves pread, blkmax and keep exactly as a settled fixpoint would.
+__global__ __launch_bounds__(64) void k_reject_seq(const NearHot *nl, DevLines L, const DevLayer *Ys, Strides S,
+                                                   int *pread, const DevPanel *pans, const DevWin *wins,
+                                                   const double *r4, int *keep_out, int *blkmax) {
+  const int job = blockIdx.x;
+  const DevLayer &Y = Ys[job];
+  const int lane = threadIdx.x;
+  int run = 0;
+  for (int k = 0; k < Y.nbk; k++) {
+    const int b = Y.blk0 + k;
+    const int t0 = max(L.bstart[b] - Y.nlo, 0);
+    const int t1 = min(L.bstart[b + 1] - Y.nlo, Y.nn);
+    int m = 0;
+    for (int t = t0 + lane; t < t1; t += 64) {
+      const NearHot o = nl[job * S.line + t];
+      const int keep = reject_one(o, run, Y, pans, wins, r4);
+      keep_out[job * S.line + t] = keep;
+      if (keep) m = max(m, o.ff);
+    }
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    if (lane == 0) {
+      pread[job * S.blk + k] = run;
+      blkmax[job * S.blk + k] = m;
+    }
+    run = max(run, m);
   }
 }
 

@@ -76,7 +76,10 @@ def _header(table):
     return p
 
 
-def write_tape3(path, table):
+def write_tape3(path, table, brd_flg=None, brd_dat=None):
+    """Write `table` as a TAPE3.  brd_flg [nblk*250, 7] int and brd_dat
+    [nblk*250, 21] float (per molecule: width, its temperature exponent,
+    shift) fill the broadener words; zero when omitted."""
     with open(path, "wb") as f:
         f.write(_rec(_header(table)))
         for b in range(table.nblk):
@@ -87,6 +90,10 @@ def write_tape3(path, table):
             for name in ("vnu", "sp", "alfa", "epp", "mol", "hwhm", "tmpalf",
                          "pshift", "iflg"):
                 blk[name][0] = getattr(table, name)[sl]
+            if brd_flg is not None:
+                blk["brd_flg"][0] = np.asarray(brd_flg, np.int32)[sl]
+            if brd_dat is not None:
+                blk["brd_dat"][0] = np.asarray(brd_dat, np.float32)[sl]
             f.write(_rec(blk.tobytes()))
 
 
@@ -128,7 +135,25 @@ def read_tape3(path):
         vmin.append(v0)
         vmax.append(v1)
     arrays = {}
-    for name in ("vnu", "sp", "alfa", "epp", "mol", "hwhm", "tmpalf", "pshift", "iflg"):
+    for name in ("vnu", "sp", "alfa", "epp", "mol", "hwhm", "tmpalf", "pshift", "iflg",
+                 "brd_flg", "brd_dat"):
         arrays[name] = (np.concatenate([blk[name] for blk in blocks])
                         if blocks else np.zeros(0))
-    return LineTable(np.array(nrec), np.array(vmin), np.array(vmax), **arrays)
+    # RDLIN/RDLNFL: an O2 line whose own O2 broadener flag is set carries
+    # its shift converted to the foreign shift, whatever IBRD
+    # (src/oprop.f90:818-827, 3473-3477); stored back in single precision
+    flg = arrays.pop("brd_flg")
+    dat = arrays.pop("brd_dat")
+    nbrd = 0
+    if len(flg):
+        nbrd = int(np.any(flg != 0, axis=1).sum())
+        o2 = (arrays["mol"] % 100 == 7) & (arrays["iflg"] >= 0) & (flg[:, 6] > 0)
+        if o2.any():
+            ps = arrays["pshift"].astype(np.float64)
+            sh = dat[:, 6 * 3 + 2].astype(np.float64)
+            rvmr = 0.21
+            arrays["pshift"] = np.where(o2, ((ps - rvmr * sh) / (1.0 - rvmr)).astype(np.float32),
+                                        arrays["pshift"])
+    t = LineTable(np.array(nrec), np.array(vmin), np.array(vmax), **arrays)
+    t.nbrd = nbrd  # lines carrying broadener flags (IBRD data)
+    return t

@@ -80,10 +80,14 @@ def test_column_workload_layers(gpu_ctx):
             continue
         ref, _, st_r, _ = O.od_layer(run, inp, tab)
         od = allod[offs[l]:offs[l] + sizes[l]]
-        print("layer", l, "npts", len(od), "max rel", np.max(np.abs(od - ref) / np.abs(ref)), flush=True)
+        print("layer", l, "npts", len(od), "max rel", np.max(np.abs(od - ref) / np.abs(ref)),
+              "fixpoint passes", st[l]["nfix"], flush=True)
         assert_od_close(od, ref, "layer %d" % l)
         for k in ("npanel", "npts", "k1_updates", "k3_updates", "nlins4"):
             assert st[l][k] == st_r[k], (l, k)
+    # the block-read fixpoint settles well inside its 16-pass budget
+    assert all(x["fixseq"] == 0 for x in st)
+    assert max(x["nfix"] for x in st) <= 8, [x["nfix"] for x in st]
 
 
 def test_radiation_in_line_strengths(gpu_ctx):
@@ -197,3 +201,97 @@ def test_batch_walk_wave_path(monkeypatch, case):
         np.testing.assert_allclose(ge, oad, rtol=1e-13)
     finally:
         ctx.close()
+
+
+def test_fixpoint_sequential_walk(monkeypatch):
+    """LNCOR1's block-read decisions taken block by block (k_reject_seq),
+    forced by a one-pass fixpoint budget (LBLX_FIX_MAXPASS=1 at context
+    creation): the same lines are kept as by the settled fixpoint (equal
+    line and update counts; optical depths equal to the order of the LDS
+    accumulation, 1e-12) and the oracle's, on the single-layer workload and
+    on five layers of the column workload."""
+    monkeypatch.setenv("LBLX_FIX_MAXPASS", "1")
+    seq = lib.Context(0)
+    monkeypatch.delenv("LBLX_FIX_MAXPASS")
+    fix = lib.Context(0)
+    try:
+        tab = synth.config0_lines()
+        p, t, wk, wb = _layer0()
+        run = abi.make_run(600.0, 1100.0)
+        dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+        lay = abi.make_layer(p, t, wk, wb, dv[0], 600.0, 1100.0)
+        for c in (seq, fix):
+            c.load_lines(tab)
+        print("sequential walk: single layer", flush=True)
+        od_s, _, st_s = seq.opdpth(run, abi.Layer.from_buffer_copy(lay))
+        od_f, _, st_f = fix.opdpth(run, abi.Layer.from_buffer_copy(lay))
+        assert st_s["fixseq"] == 1 and st_f["fixseq"] == 0 and st_f["nfix"] >= 1
+        assert (st_s["lincnt"], st_s["k1_updates"]) == (st_f["lincnt"], st_f["k1_updates"])
+        np.testing.assert_allclose(od_s, od_f, rtol=1e-12, atol=0)
+        ref, _, st_r, _ = O.od_layer(run, abi.Layer.from_buffer_copy(lay), tab)
+        assert_od_close(od_s, ref, "sequential walk")
+        assert st_s["k1_updates"] == st_r["k1_updates"]
+
+        tab = synth.config1_lines()
+        atm = synth.atmosphere()
+        run = abi.make_run(500.0, 2500.0)
+        dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+        pick = [0, 12, 25, 37, 49]
+        layers = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l],
+                                 dv[l], 500.0, 2500.0) for l in pick]
+        sizes = [lib.od_size(run, L) for L in layers]
+        offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        outs = []
+        for c in (seq, fix):
+            c.load_lines(tab)
+            print("sequential walk: column layers", pick, flush=True)
+            buf = c.alloc(8 * int(sum(sizes)))
+            lays = [abi.Layer.from_buffer_copy(x) for x in layers]
+            _, st = c.opdpth_column_device(run, lays, buf.ptr, offs, stats=True)
+            outs.append((buf.to_host(np.float64), st))
+            buf.free()
+        assert outs[0][1][0]["fixseq"] == 1 and outs[1][1][0]["fixseq"] == 0
+        print("column fixpoint passes", outs[1][1][0]["nfix"], flush=True)
+        np.testing.assert_allclose(outs[0][0], outs[1][0], rtol=1e-12, atol=1e-300)
+        for l in range(len(pick)):
+            for k in ("lincnt", "k1_updates"):
+                assert outs[0][1][l][k] == outs[1][1][l][k], (l, k)
+    finally:
+        seq.close()
+        fix.close()
+
+
+def test_tape3_broadener_words(gpu_ctx, tmp_path):
+    """TAPE3 broadener words: an O2 line whose own O2 flag is set has its
+    shift converted to the foreign shift on reading, whatever IBRD
+    (src/oprop.f90:818-827, 3473-3477); IBRD > 0 on a table carrying
+    flags is refused rather than answered with IBRD = 0 widths."""
+    tab = synth.synth_lines(3000, 700.0, 760.0, [1, 7], 23)
+    n = len(tab.vnu)
+    flg = np.zeros((n, 7), np.int32)
+    dat = np.zeros((n, 21), np.float32)
+    o2 = np.nonzero(tab.mol % 100 == 7)[0][::3]
+    flg[o2, 6] = 1
+    flg[o2, 0] = 1
+    dat[o2, 6 * 3 + 2] = np.linspace(-0.004, 0.004, len(o2)).astype(np.float32)
+    path = tmp_path / "TAPE3"
+    tape.write_tape3(path, tab, brd_flg=flg, brd_dat=dat)
+    back = tape.read_tape3(path)
+    assert back.nbrd == len(o2) > 0
+    assert np.any(back.pshift != tab.pshift)
+    gpu_ctx.load_tape3(path)
+    p, t, wb = 700.0, 270.0, 1.0e25
+    wk = [1e22, 0.0, 0.0, 0.0, 0.0, 0.0, 2e24]
+    run = abi.make_run(705.0, 755.0)
+    dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+    lay = abi.make_layer(p, t, wk, wb, dv[0], 705.0, 755.0)
+    od, ref, st, st_r = _run_both(gpu_ctx, run, lay, back)
+    assert_od_close(od, ref, "broadener shift conversion")
+    with pytest.raises(lib.LblxError) as e:
+        gpu_ctx.opdpth(abi.make_run(705.0, 755.0, ibrd=1), abi.Layer.from_buffer_copy(lay))
+    assert e.value.code == abi.E_UNSUPPORTED
+    # without broadener flags IBRD changes nothing
+    gpu_ctx.load_lines(tab)
+    od1, _, _ = gpu_ctx.opdpth(abi.make_run(705.0, 755.0, ibrd=1), abi.Layer.from_buffer_copy(lay))
+    od0, _, _ = gpu_ctx.opdpth(run, abi.Layer.from_buffer_copy(lay))
+    np.testing.assert_allclose(od1, od0, rtol=1e-12, atol=0)  # LDS accumulation order only
