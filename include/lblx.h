@@ -45,7 +45,9 @@ enum {
   LBLX_E_SIZE = -6,         /* output buffer too small                     */
   LBLX_E_NOLINES = -7,      /* no line table loaded                        */
   LBLX_E_DVRATIO = -8,      /* layer DV ratio above 2.5 (lblrtm.f90:5479)  */
-  LBLX_E_COARSER = -9       /* 'RADMRG; ITYPE LT 0' (xmerge.f90:2741)      */
+  LBLX_E_COARSER = -9,      /* 'RADMRG; ITYPE LT 0' (xmerge.f90:2741)      */
+  LBLX_E_SCRATCH = -10      /* request larger than the fixed scratch of an
+                               allocation-free entry point                 */
 };
 
 /* Precision of the line-shape evaluation and panel accumulation.        */
@@ -338,6 +340,21 @@ int lblx_scan_size(const lblx_scan *scan, double v1_in, double dv_in, int64_t n_
 int lblx_fftscn_device(lblx_ctx *ctx, const lblx_scan *scan, const double *d_in, double v1_in,
                        double dv_in, int64_t n_in, double *d_out, int64_t cap, int64_t *n_out,
                        double *v1_out, double *dv_out);
+
+/* lblx_fftscn_device without allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and supported inputs as lblx_fftscn_device; the
+ * scan's work buffers live in a fixed device array of 23 MiB that each
+ * device holds until the process ends, so the FFT length is limited to
+ * 2^19 points and the output (dvout > 0) to 2^18 points: larger requests
+ * return LBLX_E_SCRATCH.  The call allocates nothing (no hipMalloc,
+ * hipFree or pinned host memory).  Every call on a device shares that
+ * array: calls that overlap on one device (from two contexts or threads)
+ * race.  With dvout > 0 it returns after the scan has completed.         */
+int lblx_fftscn_noalloc_device(lblx_ctx *ctx, const lblx_scan *scan, const double *d_in, double v1_in,
+                               double dv_in, int64_t n_in, double *d_out, int64_t cap, int64_t *n_out,
+                               double *v1_out, double *dv_out);
 
 /* Wait for the context stream; the stream handle (hipStream_t) for callers
  * that order their own device work after the library's.                  */

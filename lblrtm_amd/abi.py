@@ -20,6 +20,7 @@ E_SIZE = -6
 E_NOLINES = -7
 E_DVRATIO = -8
 E_COARSER = -9
+E_SCRATCH = -10
 
 FP64 = 0
 FP32_SPLIT = 1

@@ -56,6 +56,42 @@ using namespace lblx;
 static thread_local std::string g_err;
 static void set_err(const std::string &s) { g_err = s; }
 
+// Work memory of the allocation-free FFTSCN entry (lblx_fftscn_noalloc_device):
+// the two FFT planes, the twiddles and INTPDR's index and coefficient
+// tables, sized for FFTs of up to 2^19 points and 2^18 output points
+// (23 MiB).  Each device holds its own copy until the process ends.  In
+// host code `scratch` is only a handle: its device address is looked up
+// once per device (scratch_lookup, from lblx_create) and that address is
+// what the entry passes to kernels and copies.
+constexpr int64_t NA_NMAX = (int64_t)1 << 19, NA_OUTMAX = (int64_t)1 << 18;
+constexpr size_t NA_OFF_Z1 = (size_t)NA_NMAX * sizeof(double2);
+constexpr size_t NA_OFF_TW = 2 * NA_OFF_Z1;
+constexpr size_t NA_OFF_G = NA_OFF_TW + (size_t)(NA_NMAX / 2) * sizeof(double2);
+constexpr size_t NA_OFF_IP = NA_OFF_G + (size_t)NA_OUTMAX * sizeof(int64_t);
+constexpr size_t NA_OFF_COF = NA_OFF_IP + (size_t)NA_OUTMAX * sizeof(int32_t);
+constexpr size_t NA_BYTES = NA_OFF_COF + 8192;
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];  // device address of `scratch` per device
+
+// With `device` current: the device address of `scratch` on it.
+static int scratch_lookup(int device, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= 64) return LBLX_E_ARG;
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_err(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return LBLX_E_HIP;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return LBLX_OK;
+}
+
 namespace {
 
 struct DevBuf {
@@ -107,6 +143,7 @@ struct lblx_ctx {
   // (LBLX_FIX_MAXPASS at creation, 1..64), and what the last OD batch used
   int fix_maxpass = 16;
   int fix_passes = 0, fix_seq = 0;
+  void *scratch = nullptr;  // device address of `scratch` on this device
   hipStream_t stream = nullptr;
   bool timing = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
@@ -295,6 +332,7 @@ const char *lblx_strerror(int code) {
     case LBLX_E_NOLINES: return "no line table loaded";
     case LBLX_E_DVRATIO: return "layer DV ratio above 2.5";
     case LBLX_E_COARSER: return "RADMRG; ITYPE LT 0";
+    case LBLX_E_SCRATCH: return g_err.empty() ? "request larger than the fixed scratch" : g_err.c_str();
     default: return "unknown error";
   }
 }
@@ -332,6 +370,17 @@ int lblx_create(lblx_ctx **out, int device) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LBLX_E_HIP;
+  }
+  {
+    int rs = scratch_lookup(device, &c->scratch);
+    if (rs) {
+      (void)hipEventDestroy(c->ev_fork);
+      (void)hipEventDestroy(c->ev_join);
+      (void)hipStreamDestroy(c->side);
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      return rs;
+    }
   }
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(lblx::c_avrat), LBLX_AVRAT, sizeof(LBLX_AVRAT)));
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(lblx::c_cf1), LBLX_CF1, sizeof(LBLX_CF1)));
@@ -2014,6 +2063,78 @@ int lblx_fftscn_device(lblx_ctx *c, const lblx_scan *scan, const double *d_in, d
                        d_out);
   }
   c->tend(th);
+  HIPCHK(hipGetLastError());
+  if (n_out) *n_out = P.nout;
+  if (v1_out) *v1_out = P.v1out;
+  if (dv_out) *dv_out = P.dvout;
+  return LBLX_OK;
+}
+
+// INTPDR tables of the allocation-free scan, planned on the host into
+// fixed storage and copied into `scratch`; the lock serialises host threads
+// over them (a call returns after its copies have completed).
+static std::mutex g_na_mu;
+static int64_t g_na_g[NA_OUTMAX];
+static int32_t g_na_ip[NA_OUTMAX];
+
+int lblx_fftscn_noalloc_device(lblx_ctx *c, const lblx_scan *scan, const double *d_in, double v1_in,
+                               double dv_in, int64_t n_in, double *d_out, int64_t cap, int64_t *n_out,
+                               double *v1_out, double *dv_out) {
+  if (!c || !scan || !d_in || !d_out) return LBLX_E_ARG;
+  if (!c->scratch) return LBLX_E_HIP;
+  HIPCHK(hipSetDevice(c->device));
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  ScanPlan P;  // its table vectors stay empty: the tables go to g_na_g / g_na_ip
+  const ScanTab tab{g_na_g, g_na_ip, NA_OUTMAX};
+  int rc = plan_scan(scan, v1_in, dv_in, n_in, &P, &tab);
+  if (rc == LBLX_E_SCRATCH) set_err("lblx_fftscn_noalloc_device: more than 262144 output points");
+  if (rc) return rc;
+  if (P.N > NA_NMAX) {
+    set_err("lblx_fftscn_noalloc_device: FFT of " + std::to_string(P.N) + " points, above 524288");
+    return LBLX_E_SCRATCH;
+  }
+  if (cap < P.nout) return LBLX_E_SIZE;
+  const int64_t N = P.N;
+  hipStream_t s = c->stream;
+  char *base = (char *)c->scratch;
+  double2 *z0 = (double2 *)base, *z1 = (double2 *)(base + NA_OFF_Z1);
+  double2 *tw = (double2 *)(base + NA_OFF_TW);
+  int64_t *g = (int64_t *)(base + NA_OFF_G);
+  int32_t *ip = (int32_t *)(base + NA_OFF_IP);
+  double *cof = (double *)(base + NA_OFF_COF);
+  // every element the kernels read is written first in this call: the
+  // planes over all N points by k_scan_load, the twiddles for m < N/2, the
+  // tables for the nout output points
+  const unsigned gN = (unsigned)((N + 255) / 256), gH = (unsigned)((N / 2 + 255) / 256);
+  hipLaunchKernelGGL(k_scan_load, dim3(gN), dim3(256), 0, s, d_in, P.n0, P.M, P.ktot, N, P.fill, z0);
+  double2 *cur = z0, *oth = z1;
+  if (P.jfn != 0) {
+    hipLaunchKernelGGL(k_scan_twiddle, dim3(gH), dim3(256), 0, s, tw, N);
+    for (int64_t ns = 1; ns < N; ns <<= 1) {
+      hipLaunchKernelGGL(k_fft_pass, dim3(gH), dim3(256), 0, s, cur, oth, (const double2 *)tw, N, ns, -1);
+      std::swap(cur, oth);
+    }
+    ScanFilter F{N, P.k4, P.DX, P.A, P.dvb, P.r4, P.jfn, P.fix};
+    hipLaunchKernelGGL(k_scan_filter, dim3((unsigned)((N / 2 + 1 + 255) / 256)), dim3(256), 0, s, F, cur);
+    for (int64_t ns = 1; ns < N; ns <<= 1) {
+      hipLaunchKernelGGL(k_fft_pass, dim3(gH), dim3(256), 0, s, cur, oth, (const double2 *)tw, N, ns, +1);
+      std::swap(cur, oth);
+    }
+  }
+  const double rn = P.jfn != 0 ? 1.0 / (double)N : 1.0;
+  const int64_t off = P.n1 - 1;
+  if (scan->dvout > 0) {
+    HIPCHK(hipMemcpyAsync(g, g_na_g, (size_t)P.nout * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ip, g_na_ip, (size_t)P.nout * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_scan_cof, dim3(1), dim3(256), 0, s, cof);
+    hipLaunchKernelGGL(k_scan_out, dim3((unsigned)((P.nout + 255) / 256)), dim3(256), 0, s, cur, off, P.ntot,
+                       rn, P.nout, (const int64_t *)g, (const int32_t *)ip, (const double *)cof, d_out);
+    HIPCHK(hipStreamSynchronize(s));  // the host tables are reused by the next call
+  } else {
+    hipLaunchKernelGGL(k_scan_out, dim3((unsigned)((P.nout + 255) / 256)), dim3(256), 0, s, cur, off, P.ntot,
+                       rn, P.nout, (const int64_t *)nullptr, (const int32_t *)nullptr, (const double *)nullptr,
+                       d_out);
+  }
   HIPCHK(hipGetLastError());
   if (n_out) *n_out = P.nout;
   if (v1_out) *v1_out = P.v1out;

@@ -583,7 +583,7 @@ int scan_records(int64_t npts) {
 }
 }  // namespace
 
-int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan *P) {
+int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan *P, const ScanTab *fixed) {
   if (!S || !P || !(dv > 0.) || n_in < 4) return LBLX_E_ARG;
   *P = ScanPlan{};
   double HWHM = S->hwhm;
@@ -679,43 +679,59 @@ int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan 
     P->nout = P->ntot;
     return LBLX_OK;
   }
-  // INTPDR / INTERP four-point walk over Wrtspc's 2400-point panels
+  // INTPDR / INTERP four-point walk over Wrtspc's 2400-point panels.
+  // Panel k holds min(2400, nz - 2400 k) points from v1p(k); v1p(k+1) =
+  // v2p(k) + DV.  The walk only moves forward, so each panel's bounds are
+  // formed when it is reached, in the order Wrtspc writes them.
   const int64_t nz = P->ntot;
   const double dvi = dv, dvo = S->dvout;
   const int npan = (int)((nz + 2399) / 2400);
-  std::vector<double> pv1(npan), pv2(npan);
-  std::vector<int> pn(npan);
-  double v1p = V1S, v2p = 0.;
-  for (int k = 0; k < npan; k++) {
-    int np = (int)std::min<int64_t>(2400, nz - 2400LL * k);
-    if (k > 0) v1p = v2p + dvi;
-    v2p = v1p + dvi * (double)(np - 1);
-    pv1[k] = v1p;
-    pv2[k] = v2p;
-    pn[k] = np;
-  }
-  const double xnumcf = 201.;
   int pk = 0;
-  while (pk < npan && pv2[pk] < V1 - 4.0 * dvo) pk++;
+  double pv1 = V1S, pv2 = 0.;
+  int pn = 0;
+  auto panel_at = [&](int k) {  // bounds of panel k, the one after pk
+    pn = (int)std::min<int64_t>(2400, nz - 2400LL * k);
+    if (k > 0) pv1 = pv2 + dvi;
+    pv2 = pv1 + dvi * (double)(pn - 1);
+  };
+  panel_at(0);
+  const double xnumcf = 201.;
+  while (pk < npan && pv2 < V1 - 4.0 * dvo) {
+    pk++;
+    if (pk < npan) panel_at(pk);
+  }
   if (pk >= npan) return LBLX_E_ARG;
-  double V1I = pv1[pk] - 4.0 * dvi, V2I = pv2[pk];
-  int NNI = pn[pk] + 4;
+  double V1I = pv1 - 4.0 * dvi, V2I = pv2;
+  int NNI = pn + 4;
   int64_t base = 2400LL * pk - 5;  // global index of T(0)
   bool hit_eof = false;
   double V1J = V1, DVJ = dvo, VJ = V1J;
   const double vratio = DVJ / dvi;
   if (V1J < V1I) return LBLX_E_UNSUPPORTED;
   int J2 = 0;
-  std::vector<int64_t> gpan;
-  std::vector<int32_t> ippan;
+  // one output panel of INTPDR (2400 points) before it is flushed
+  int64_t gpan[2400];
+  int32_t ippan[2400];
+  int64_t nout = 0;
+  bool over = false;
   auto flush = [&](int n) {
-    for (int j = 0; j < n; j++) {
-      P->g.push_back(gpan[j]);
-      P->ip.push_back(ippan[j]);
+    if (fixed) {
+      if (nout + n > fixed->cap) {
+        over = true;
+        return;
+      }
+      std::memcpy(fixed->g + nout, gpan, (size_t)n * sizeof(int64_t));
+      std::memcpy(fixed->ip + nout, ippan, (size_t)n * sizeof(int32_t));
+    } else {
+      P->g.insert(P->g.end(), gpan, gpan + n);
+      P->ip.insert(P->ip.end(), ippan, ippan + n);
     }
+    nout += n;
   };
-  gpan.assign(2401, 0);
-  ippan.assign(2401, -1);
+  for (int j = 0; j < 2400; j++) {
+    gpan[j] = 0;
+    ippan[j] = -1;
+  }
   for (;;) {
     int I = fint((VJ - V1I) / dvi + 1.00001);
     double VI = V1I + dvi * (double)(I - 1);
@@ -738,6 +754,7 @@ int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan 
       V1J = V1J + DVJ * (J2 - 1) + DVJ;
       J2 = 0;
     }
+    if (over) return LBLX_E_SCRATCH;
     if (VJ >= V2) break;
     if (hit_eof) {  // label 80: zeros to the end of the band
       for (;;) {
@@ -751,6 +768,7 @@ int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan 
           V1J = V1J + DVJ * (J2 - 1) + DVJ;
           J2 = 0;
         }
+        if (over) return LBLX_E_SCRATCH;
         if (VJ >= V2) break;
       }
       break;
@@ -758,9 +776,10 @@ int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan 
     if (I >= NNI - 2) {
       if (pk + 1 < npan) {
         pk++;
-        V1I = pv1[pk] - 4.0 * dvi;
-        V2I = pv2[pk];
-        NNI = pn[pk] + 4;
+        panel_at(pk);
+        V1I = pv1 - 4.0 * dvi;
+        V2I = pv2;
+        NNI = pn + 4;
         base = 2400LL * pk - 5;
       } else {
         hit_eof = true;  // T(NNI+1) extrapolated, T(NNI+2) = 0
@@ -770,7 +789,7 @@ int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan 
   }
   P->v1out = V1;
   P->dvout = dvo;
-  P->nout = (int64_t)P->g.size();
+  P->nout = nout;
   return LBLX_OK;
 }
 

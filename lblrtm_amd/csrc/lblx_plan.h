@@ -121,7 +121,17 @@ struct ScanPlan {
   std::vector<int64_t> g; // INTPDR: input index of T(I) per output point
   std::vector<int32_t> ip;  // coefficient index (0..202), -1: zero fill
 };
-int plan_scan(const lblx_scan *scan, double v1c, double dv, int64_t n_in, ScanPlan *P);
+
+// Fixed storage for the INTPDR tables (g, ip of ScanPlan) for callers that
+// allocate nothing: plan_scan writes up to cap entries there instead of
+// growing the vectors, and returns LBLX_E_SCRATCH when the output has more.
+struct ScanTab {
+  int64_t *g;
+  int32_t *ip;
+  int64_t cap;
+};
+int plan_scan(const lblx_scan *scan, double v1c, double dv, int64_t n_in, ScanPlan *P,
+              const ScanTab *fixed = nullptr);
 
 // Build the plan; layer->jrad is updated as OPDPTH does.  Returns LBLX_OK
 // or an LBLX_E_* code.

@@ -45,6 +45,29 @@ __global__ void k_fft_pass(const double2 *src, double2 *dst, const double2 *tw, 
   dst[o + ns] = make_double2(a.x - b.x, a.y - b.y);
 }
 
+// Twiddles exp(-2 pi i m / N), m < N/2, formed on the device (the
+// allocation-free scan has no host table to copy).
+__global__ void k_scan_twiddle(double2 *tw, int64_t N) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N / 2) return;
+  double sn, cs;
+  sincospi(2.0 * (double)m / (double)N, &sn, &cs);
+  tw[m] = make_double2(cs, -sn);
+}
+
+// INTERP's four-point coefficients over 203 fractional positions
+// (postsub.f90:2966-2974): cof[c], cof[203 + c], cof[406 + c], cof[609 + c].
+__global__ void k_scan_cof(double *cof) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip > 202) return;
+  const double xnumcf = 201.;
+  const double p = ((double)ip - 1.0) / (xnumcf - 1.0), pp = p * p;
+  cof[ip] = -p / 2.0 * (1 - p) * (1 - p);
+  cof[203 + ip] = 1.0 - pp * (3.0 - 2.0 * p) + pp / 2.0 * (1.0 - p);
+  cof[406 + ip] = pp * (3.0 - 2.0 * p) + p / 2.0 * (1.0 - p) * (1.0 - p);
+  cof[609 + ip] = -pp / 2 * (1.0 - p);
+}
+
 // Scnfnt interferogram-domain functions (IAPSC = 1) at OPD x.
 __device__ __forceinline__ double scan_apod(int jfn, double a, double x) {
   const double pi = 3.141592653589793;

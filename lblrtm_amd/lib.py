@@ -71,6 +71,9 @@ def load():
         "lblx_fftscn_device": (C.c_int, [C.c_void_p, P(abi.Scan), C.c_void_p, C.c_double, C.c_double,
                                          C.c_int64, C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_double),
                                          P(C.c_double)]),
+        "lblx_fftscn_noalloc_device": (C.c_int, [C.c_void_p, P(abi.Scan), C.c_void_p, C.c_double, C.c_double,
+                                                 C.c_int64, C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_double),
+                                                 P(C.c_double)]),
         "lblx_jacobian_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_double),
                                            C.c_double, C.c_double, P(C.c_int), C.c_int, C.c_double,
                                            C.c_double, C.c_void_p, C.c_void_p, C.c_int64,
@@ -442,6 +445,17 @@ class Context:
         _chk(self._lib.lblx_fftscn_device(self._h, C.byref(scan), C.c_void_p(d_in), v1_in, dv_in, n_in,
                                           C.c_void_p(d_out), cap, C.byref(n), C.byref(v1), C.byref(dv)),
              "lblx_fftscn_device")
+        return n.value, v1.value, dv.value
+
+    def fftscn_noalloc_device(self, scan, d_in, v1_in, dv_in, n_in, d_out, cap):
+        """FFTSCN through the allocation-free entry (fixed device scratch);
+        returns (points, v1, dv) of the output."""
+        n = C.c_int64(0)
+        v1 = C.c_double(0.0)
+        dv = C.c_double(0.0)
+        _chk(self._lib.lblx_fftscn_noalloc_device(self._h, C.byref(scan), C.c_void_p(d_in), v1_in, dv_in, n_in,
+                                                  C.c_void_p(d_out), cap, C.byref(n), C.byref(v1), C.byref(dv)),
+             "lblx_fftscn_noalloc_device")
         return n.value, v1.value, dv.value
 
     def fftscn(self, scan, spectrum, v1_in, dv_in):
