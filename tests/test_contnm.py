@@ -42,6 +42,15 @@ def radterm(v, t):
     return v * np.tanh(v * RADCN2 / (2.0 * t))
 
 
+def radfn(v, t):
+    """RADFN with its branches (src/oprop.f90:2235-2296): x/2 v below
+    x = 0.01, v tanh(x/2) up to x = 10, v beyond (x = v/kT)."""
+    v = np.asarray(v, np.float64)
+    x = v * RADCN2 / t
+    e = np.exp(-np.minimum(x, 10.0))
+    return np.where(x <= 0.01, 0.5 * x * v, np.where(x <= 10.0, v * (1.0 - e) / (1.0 + e), v))
+
+
 def layer(v1, v2, t, p=1013.0, wk=(1e22, 4e21, 0.0, 0.0, 0.0, 0.0, 5e23), wb=2e24):
     return abi.make_layer(p, t, list(wk), wb, 0.001, v1, v2)
 
@@ -152,6 +161,106 @@ def test_n2_fundamental_at_272K():
     a, v = absrb(run_scaled(1900.0, 3000.0, one(5)), lay, jrad=1)
     _check(a, v, interp(vk, ck * radterm(vk, t), v), 2006, 2893, 2e-4)
     assert np.all(a[(v < 1998) | (v > 2901)] == 0.0)
+
+
+@pytest.mark.parametrize("t,p", [(220.0, 300.0), (250.0, 700.0), (310.0, 1013.0)])
+def test_h2o_self_temperature_law(t, p):
+    """MT_CKD self continuum away from 296 K: ref x (296/T)^texp x vmr x
+    (P/1013)(296/T), foreign ref x (1 - vmr) x the same density ratio
+    (src/mt_ckd_h2o_module.f90:141-171)."""
+    lay = layer(500.0, 2500.0, t, p=p)
+    wk1, wtot = 1e22, 1e22 + 4e21 + 5e23 + 2e24
+    vmr = wk1 / wtot
+    rho = (p / 1013.0) * (296.0 / t)
+    vk = T["LBLX_MTCKD_WVN"]
+    self_t = T["LBLX_MTCKD_SELF"] * (296.0 / t) ** T["LBLX_MTCKD_TEXP"]
+    for k, ck in [(0, self_t * vmr * rho), (1, T["LBLX_MTCKD_FOR"] * (1.0 - vmr) * rho)]:
+        a0, v = absrb(run_scaled(500.0, 2500.0, one(k)), lay, jrad=0)
+        _check(a0, v, wk1 * interp(vk, ck, v), 510, 2490, 1e-12)
+        a1, _ = absrb(run_scaled(500.0, 2500.0, one(k)), lay, jrad=1)
+        _check(a1, v, wk1 * interp(vk, ck * radfn(vk, t), v), 510, 2490, 1e-12)
+    # the law is not the identity here: the texp factor moves the result
+    if t != 296.0:
+        a0, v = absrb(run_scaled(500.0, 2500.0, one(0)), lay, jrad=0)
+        m = (v >= 510) & (v <= 2490)
+        flat = wk1 * interp(vk, T["LBLX_MTCKD_SELF"] * vmr * rho, v)
+        assert np.max(np.abs(a0[m] / flat[m] - 1.0)) > 0.05
+
+
+@pytest.mark.parametrize("t", [200.0, 280.0])
+def test_co2_band_head_temperature_law(t):
+    """FRNCO2 away from T_eff = 246 K: the 25 band-head coefficients
+    (table entries 1196-1220) scale as (T/246)^tdep, the rest is unchanged
+    (src/contnm.f90:1295-1351); the v3 scale factor XFACCO2 applies on
+    2000-2998 cm-1 (:338-375)."""
+    p = 700.0
+    lay = layer(500.0, 2500.0, t, p=p)
+    rho = (p / 1013.0) * (296.0 / t)
+    hdr = T["LBLX_FCO2_HDR"]
+    vk = hdr[0] + hdr[2] * np.arange(int(hdr[3]))
+    tcor = np.ones_like(vk)
+    tcor[1195:1220] = (t / 246.0) ** T["LBLX_CO2_TDEP"]
+    cfac = np.ones_like(vk)
+    m = (vk >= 2000) & (vk <= 2998)
+    cfac[m] = T["LBLX_XFACCO2"][((vk[m] - 1998.0) / 2.0 + 1e-5).astype(int) - 1]
+    ck = T["LBLX_FCO2"] * tcor * cfac * 4e21 * rho * 1e-20
+    a, v = absrb(run_scaled(500.0, 2500.0, one(2)), lay, jrad=0)
+    _check(a, v, interp(vk, ck, v), 505, 2495, 1e-12)
+    a, v = absrb(run_scaled(500.0, 2500.0, one(2)), lay, jrad=1)
+    _check(a, v, interp(vk, ck * radfn(vk, t), v), 505, 2495, 1e-12)
+    head = (v > 2390) & (v < 2430)
+    flat = interp(vk, T["LBLX_FCO2"] * cfac * 4e21 * rho * 1e-20 * radfn(vk, t), v)
+    assert np.max(np.abs(a[head] / flat[head] - 1.0)) > 0.02
+
+
+@pytest.mark.parametrize("t", [220.0, 260.0])
+def test_o2_fundamental_temperature_law(t):
+    """O2 fundamental away from 296 K: xo2 x exp(xo2t (1/296 - 1/T)) / v
+    (o2_ver_1, src/contnm.f90:7254-7318)."""
+    p = 500.0
+    lay = layer(1200.0, 2000.0, t, p=p)
+    amagat = (p / 1013.0) * (273.0 / t)
+    hdr = T["LBLX_O2F_HDR"]
+    vk = hdr[0] + hdr[2] * np.arange(int(hdr[3]))
+    law = np.exp(T["LBLX_O2F_XO2T"] * (1.0 / 296.0 - 1.0 / t))
+    ck = 5e23 * 1e-20 * amagat * (1e20 / XLOSMT) * T["LBLX_O2F_XO2"] * law / vk
+    a, v = absrb(run_scaled(1200.0, 2000.0, one(4)), lay, jrad=0)
+    _check(a, v, interp(vk, ck, v), 1345, 1845, 1e-12)
+    a1, _ = absrb(run_scaled(1200.0, 2000.0, one(4)), lay, jrad=1)
+    _check(a1, v, interp(vk, ck * radfn(vk, t), v), 1345, 1845, 1e-12)
+    assert np.max(np.abs(law - 1.0)) > 0.1
+
+
+@pytest.mark.parametrize("t", [228.0, 250.0, 300.0])
+def test_n2_fundamental_temperature_law(t):
+    """N2 fundamental between and beyond its 272 K and 228 K tables:
+    logarithmic interpolation in 1/T where both are positive, linear in T
+    elsewhere; the O2 efficiency 1.294 - 0.4545 T/296 and the H2O
+    efficiency (9/7) a_h2o (n2_ver_1, src/contnm.f90:2668-2754)."""
+    wk = (1e22, 4e21, 0.0, 0.0, 0.0, 0.0, 5e23)
+    wtot = sum(wk) + 2e24
+    p = 800.0
+    lay = layer(1900.0, 3000.0, t, p=p, wk=wk)
+    amagat = (p / 1013.0) * (273.0 / t)
+    xh, xo = wk[0] / wtot, wk[6] / wtot
+    xn = 1.0 - xh - xo
+    hdr = T["LBLX_N2F_HDR"]
+    vk = hdr[0] + hdr[2] * np.arange(int(hdr[3]))
+    a272, a228 = T["LBLX_N2F_272"], T["LBLX_N2F_228"]
+    xtfac = (1.0 / t - 1.0 / 272.0) / (1.0 / 228.0 - 1.0 / 272.0)
+    xtlin = (t - 272.0) / (228.0 - 272.0)
+    pos = (a272 > 0) & (a228 > 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c0 = np.where(pos, a272 * (a228 / np.where(pos, a272, 1.0)) ** xtfac, a272 + (a228 - a272) * xtlin) / vk
+    a_o2 = 1.294 - 0.4545 * t / 296.0
+    ck = (xn * wtot / XLOSMT) * amagat * (xn * c0 + xo * a_o2 * c0 + xh * (9.0 / 7.0) * T["LBLX_N2F_AH2O"] * c0)
+    a, v = absrb(run_scaled(1900.0, 3000.0, one(5)), lay, jrad=0)
+    want = interp(vk, ck, v)
+    m = (v >= 2006) & (v <= 2893)
+    # pow of the table ratio: 1e-12 of the band's peak absolute
+    np.testing.assert_allclose(a[m], want[m], rtol=1e-12, atol=1e-12 * np.abs(want[m]).max())
+    if t == 228.0:  # the 228 K table itself
+        np.testing.assert_allclose(c0[pos], (a228 / vk)[pos], rtol=1e-12)
 
 
 def test_rayleigh_closed_form():
