@@ -46,9 +46,12 @@
  * convolution of the tables at their identity temperatures, Rayleigh,
  * scale-factor linearity, continuum-only optical depth), tests/test_merge.py
  * (isothermal, transparent and opaque columns, cubic reproduction of RADNN,
- * the RADMRG panel walk).  Partly pinned (invariants only): the exact
- * line-rejection choices of CONVF4 (src/oprop.f90:3900-3917) and LNCOR1
- * (src/oprop.f90:1153-1177); no executable reference exists to check them.
+ * the RADMRG panel walk), tests/test_oracle_pins.py (TIPS against the
+ * reference's QofT tables, one line away from 296 K / 1013.25 mb against
+ * the Faddeeva profile), tests/test_oracle_inorder.py (CONVF4's and
+ * LNCOR1's in-order rejections, src/oprop.f90:3900-3917, 1120-1177, and
+ * the panel at which RDLIN reads each block, against a sequential Python
+ * restatement, decision by decision through oracle_log).
  */
 #include <math.h>
 #include <stdint.h>
@@ -233,6 +236,33 @@ void shape_tables(double *f1, double *f2, double *f3, double *fg, double *xv)
  * reference's F_k(IZ, IZM + 1) does at IZM = 101. */
 #define NZETA 101
 static int g_voigt = 0; /* set per call by oracle_od_layer from lblx_run.voigt */
+
+/* Decision log (tests only): CONVF4's and LNCOR1's accept/reject decisions
+ * in the order the reference takes them (the FREJ record of
+ * src/oprop.f90:3895-3905 and LNCOR1's rejections, :1120, :1153-1177).
+ * Entries of four ints: kind 0 (CONVF4): window number within the layer,
+ * TAPE9 line, accepted; kind 1 (LNCOR1): TAPE3 slot (block*250 + record),
+ * panels emitted when RDLIN read the block, kept. */
+static int g_log_on = 0, g_log_win = 0;
+static int32_t *g_log = NULL;
+static int64_t g_log_n = 0, g_log_cap = 0;
+static void log_dec(int kind, int a, int b, int c)
+{
+  if (!g_log_on) return;
+  if (g_log_n == g_log_cap) {
+    g_log_cap = g_log_cap ? 2 * g_log_cap : 4096;
+    g_log = realloc(g_log, (size_t)g_log_cap * 4 * sizeof(int32_t));
+  }
+  int32_t *e = g_log + 4 * g_log_n++;
+  e[0] = kind; e[1] = a; e[2] = b; e[3] = c;
+}
+void oracle_log(int on) { g_log_on = on; g_log_n = 0; g_log_win = 0; }
+int64_t oracle_log_get(int32_t *out, int64_t cap)
+{
+  int64_t n = g_log_n < cap ? g_log_n : cap;
+  if (out && n > 0) memcpy(out, g_log, (size_t)n * 4 * sizeof(int32_t));
+  return g_log_n;
+}
 static int g_icflg = 0; /* CONTNM's /CDERIV/ ICFLG: > 0 for an analytic-Jacobian species run */
 void oracle_set_icflg(int icflg) { g_icflg = icflg; }
 static double VF[3 * NZETA + 1][NFMX + 1];        /* [column][IZ], IZ 1-based */
@@ -979,7 +1009,8 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
       int jj = fint((T9->vnu[i] - W->v1r4) / W->dvr4 + 1.);
       if (jj < 1) jj = 1;
       if (jj > W->nptr4) jj = W->nptr4;
-      if (speak <= (dptmn + dptfc * r4[jj]) && spp == 0.) continue;
+      if (speak <= (dptmn + dptfc * r4[jj]) && spp == 0.) { log_dec(0, g_log_win, i, 0); continue; }
+      log_dec(0, g_log_win, i, 1);
       *ilin4 += 1;
       double vnui = T9->vnu[i];
       double xnui = vnui - W->v1r4;
@@ -1039,6 +1070,7 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
       }
     }
   }
+  g_log_win++;
   if (jrad == 1) {
     double xkt = 1. / beta;
     double vitst = W->v1r4 - W->dvr4;
@@ -1517,6 +1549,7 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
 {
   lblx_stats stats;
   memset(&stats, 0, sizeof(stats));
+  g_log_win = 0;
   const double V1 = Ly->v1, V2 = Ly->v2;
   double DV = Ly->dv;
   const double SAMPLE = run->sample, ALFAL0 = run->alfal0;
@@ -1624,6 +1657,7 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
     absrb_own = NULL;
   }
   t3reader_t rd = {L, 0};
+  int log_blk = -1, log_pan = 0;  /* block RDLIN read last, and at which panel */
   int IEOF = 0, ILO = 0, IHI = -999, IDATA = 0, IPANEL = 0, ISTOP = 0;
   int IFPAN = 1;
   int N1R2, N1R3, N2R2, N2R3;
@@ -1644,6 +1678,8 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
       IDATA = 0;
       int b = t3_next_block(&rd, VBOT);
       if (b < 0) { IEOF = 1; goto label80; }
+      log_blk = b;
+      log_pan = (int)stats.npanel;
       int nrec = L->nrec[b];
       if (nrec > NLIMIN) { rc = LBLX_E_NREC; goto done_hirac; }
       for (int i = 1; i <= nrec; i++) {
@@ -1719,9 +1755,11 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
           if (SPEAK <= (DPTMN + DPTFC * W->r4[JJ]) && B->sppsp[i] == 0.) goto rej25;
         }
         stats.lincnt++;
+        log_dec(1, log_blk * NLIMIN + i - 1, log_pan, 1);
         continue;
       }
     rej25:
+      log_dec(1, log_blk * NLIMIN + i - 1, log_pan, 0);
       B->sp[i] = 0.;
       B->sppsp[i] = 0.;
     }
