@@ -251,3 +251,64 @@ def test_single_line_pins_detect_plausible_mistakes():
     assert err(good) < 3e-3
     for bad in (dict(width_exp=-1.0), dict(shift_sign=-1.0), dict(self_on=0.0), dict(epp_on=0.0)):
         assert err(model(**bad)[0]) > 5e-2, bad
+
+
+@ref_src
+def test_linf4_line_correction_away_from_reference_state():
+    """LINF4's correction of the far-wing lines (src/oprop.f90:3050-3230) at
+    230 K / 400 mb: lines 1.3 cm-1 apart, so every SHRINK group holds one
+    line and TAPE9 carries each corrected line itself: centre
+    nu + (P/P0)(T0/T) shift, Lorentz width with the (T/T0)^n law and self
+    broadening, Doppler width nu' ALFD1 (no radiation term in the
+    strength, jrad4 = 0), strength S W SCOR exp(-E'' (c2/T - c2/T0))
+    (1 + exp(-c2 nu'/T))."""
+    t, p = 230.0, 400.0
+    n = 60
+    rng = np.random.Generator(np.random.PCG64(3))
+    vnu = 700.0 + 1.3 * np.arange(n)
+    mol = np.where(np.arange(n) % 3 == 0, 2, 1)
+    f32 = lambda a: np.asarray(a, np.float32)
+    sp = f32(10.0 ** rng.uniform(-23, -20, n))
+    alfa = f32(rng.uniform(0.04, 0.1, n))
+    hwhm = f32(alfa * rng.uniform(2, 5, n))
+    tmp = f32(rng.uniform(0.5, 0.8, n))
+    epp = f32(rng.uniform(0, 2500, n))
+    sh = f32(rng.uniform(-0.005, 0.005, n))
+    tab = abi.LineTable.from_lines(vnu, sp, alfa, epp, mol + 100, hwhm, tmp, sh)
+    wk = [2e22, 6e21, 0, 0, 0, 0, 0]
+    wb = 1e24
+    v1, v2 = 730.0, 740.0
+    run = abi.make_run(v1, v2)
+    dv, _ = O.plan_dv(run, [p], [t], [wk], [wb])
+    lay = abi.make_layer(p, t, wk, wb, dv[0], v1, v2)
+    cap = 1000
+    arrs = [np.zeros(cap) for _ in range(4)] + [np.zeros(cap, np.int32)]
+    nb = C.c_int(0)
+    bc = np.zeros(100, np.int32)
+    nt9 = O.lib().oracle_linf4(C.byref(run), C.byref(abi.Layer.from_buffer_copy(lay)), C.byref(tab.struct()),
+                               *[abi.ptr(a, C.c_double) for a in arrs[:4]], abi.ptr(arrs[4], C.c_int32),
+                               abi.ptr(bc, C.c_int32), cap, 100, C.byref(nb))
+    g_vnu, g_sp, g_al, g_ad = (a[:nt9] for a in arrs[:4])
+    assert nt9 > 30
+    tdat, q = reference_tips()
+    masses = reference_masses()
+    rhorat = (p / P0) * (T0 / t)
+    wtot = sum(wk) + wb
+    c2 = RADCN2
+    kb, c, amu = 1.380649e-23, 2.99792458e8, 1.66053906660e-27
+    # TAPE9 lines in order: match each to its source line by centre
+    for k in range(nt9):
+        i = int(np.argmin(np.abs(vnu - g_vnu[k])))
+        m = int(mol[i])
+        nup = vnu[i] + rhorat * float(sh[i])
+        rhoslf = rhorat * wk[m - 1] / wtot
+        tc = (t / T0) ** float(tmp[i])
+        al = float(alfa[i]) * tc * (rhorat - rhoslf) + float(hwhm[i]) * tc * rhoslf
+        ad = nup / c * math.sqrt(2 * math.log(2) * kb * t / (masses[m][0] * amu))
+        scor = lagrange4(tdat, q[m][0], 296.0) / lagrange4(tdat, q[m][0], t)
+        s = float(sp[i]) * wk[m - 1] * scor * math.exp(-float(epp[i]) * (c2 / t - c2 / T0)) * \
+            (1 + math.exp(-nup * c2 / t))
+        assert abs(g_vnu[k] - nup) < 1e-9, k
+        assert abs(g_al[k] / al - 1) < 1e-10, k
+        assert abs(g_ad[k] / ad - 1) < 1e-5, k
+        assert abs(g_sp[k] / s - 1) < 1e-10, k
