@@ -108,28 +108,93 @@ def bt(rad, v):
     return 1.4387752 * v / np.log1p(1.191042722e-12 * v ** 3 / rad)
 
 
-def cpu_baseline(run, layers, tab, tzu, tbnd, emis):
-    """The oracle (single-threaded C restatement of OPDPTH and XMERGE) on
-    the whole spectrum: every layer of the column, one core."""
+def _oracle_spectrum(job):
+    """One process of the CPU baseline: the oracle (single-threaded C
+    restatement of OPDPTH and XMERGE) on the whole column spectrum.  The
+    inputs are rebuilt here from the seeded generator and the layer grids
+    the parent planned; the timed part is the spectrum alone.  Process 0
+    also returns the radiance and the optical depths of the picked layers
+    for the accuracy figures."""
+    idx, dvs, picks = job
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    from lblrtm_amd import abi
-    t0 = time.perf_counter()
+    from lblrtm_amd import abi, synth
+    tab = synth.config1_lines()
+    atm = synth.atmosphere()
+    run = abi.make_run(500.0, 2500.0, icntnm=1)
+    tzu = np.asarray(atm["tlev"][1:], np.float64)
+    t0 = time.time()
     jrad = 1
     m = None
-    for l, L in enumerate(layers):
-        lay = abi.Layer.from_buffer_copy(L)
+    ods = {}
+    for l, dv in enumerate(dvs):
+        lay = abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l], dv,
+                             500.0, 2500.0)
         lay.jrad = jrad
         od, _, _, jrad = O.od_layer(run, lay, tab)
         pan = O.panels()
+        if idx == 0 and l in picks:
+            ods[l] = od
         if m is None:
-            m = O.merge_first(od, pan, L.dv, L.tave, tzu[l], tbnd, emis)
+            m = O.merge_first(od, pan, dv, lay.tave, tzu[l], 295.0, 0.98)
         else:
-            m, _ = O.merge_next(m, od, pan, L.dv, L.tave, tzu[l], run.v2)
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "spectra/s", "cores": 1, "kind": "port",
-            "sample": "one full spectrum (all 50 layers, 500k lines, continuum, merge) with the "
-                      "C oracle on one host core, %.1f s" % dt}
+            m, _ = O.merge_next(m, od, pan, dv, lay.tave, tzu[l], run.v2)
+    t1 = time.time()
+    return t0, t1, (m.rad if idx == 0 else None), ods
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(dvs, picks):
+    """The oracle on the column spectrum in one process per host core (at
+    most 16, the CPU share of one GPU on the box), each computing the whole
+    spectrum; aggregate spectra/s = processes / wall time from the first
+    start to the last end.  Returns the baseline record and the first
+    process's radiance and picked-layer optical depths."""
+    import multiprocessing as mp
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    nproc = max(1, min(16, ncpu))
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        res = pool.map(_oracle_spectrum, [(i, list(dvs), list(picks)) for i in range(nproc)])
+    start = min(r[0] for r in res)
+    end = max(r[1] for r in res)
+    single = float(np.median([r[1] - r[0] for r in res]))
+    rec = {"value": nproc / (end - start), "unit": "spectra/s", "cores": nproc, "kind": "port",
+           "cpu_model": cpu_model(), "single_core_s_per_spectrum": single,
+           "sample": "the whole column spectrum (50 layers, 500k lines, continuum, merge) with the C oracle "
+                     "in each of %d processes, one per host core, %.1f s wall" % (nproc, end - start)}
+    return rec, res[0][2], res[0][3]
+
+
+def accuracy(ref_rad, ref_ods, rads, ods, v):
+    """Errors of the GPU spectra and layer optical depths against the
+    oracle's: max and RMS brightness temperature, and the OD tolerance's
+    two measures (relative where OD > 1e-6, absolute below)."""
+    out = {}
+    for name in rads:
+        d = np.abs(bt(rads[name], v) - bt(ref_rad, v))
+        rel, absd = 0.0, 0.0
+        for l, ref in ref_ods.items():
+            got = ods[name][l]
+            big = np.abs(ref) > 1e-6
+            if big.any():
+                rel = max(rel, float(np.max(np.abs(got[big] - ref[big]) / np.abs(ref[big]))))
+            if (~big).any():
+                absd = max(absd, float(np.max(np.abs(got[~big] - ref[~big]))))
+        out[name] = {"max_dbt_K": float(d.max()), "rms_dbt_K": float(np.sqrt(np.mean(d ** 2))),
+                     "max_rel_dod": rel, "max_abs_dod_below_1e-6": absd}
+    return out
 
 
 def batch_block(ctx, lib, abi, synth, nprof, steps, nadj):
@@ -317,6 +382,19 @@ def main():
     if a.batch_profiles > 0:
         batch = batch_block(ctx, lib, abi, synth, a.batch_profiles, a.batch_steps,
                             min(a.batch_adjoint, a.batch_profiles))
+    # layer optical depths of a few layers across the column, both forms,
+    # for the accuracy against the oracle
+    picks = (0, 12, 25, 37, 49)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    gods = {}
+    for name, r_ in (("fp64", run), ("fp32_split", run32)):
+        esz = 4 if name == "fp32_split" else 8
+        obuf = ctx.alloc(esz * int(sum(sizes)))
+        ctx.opdpth_column_device(r_, [abi.Layer.from_buffer_copy(x) for x in layers], obuf.ptr, offs)
+        ctx.sync()
+        gods[name] = {l: obuf.to_host(np.float32 if esz == 4 else np.float64, sizes[l], int(offs[l])).astype(np.float64)
+                      for l in picks}
+        obuf.free()
     dom = max((k for k in per if k != "host"), key=per.get)
     nwin = sum(s["nwin4"] for s in stats)
     nbytes = stage_bytes(dom, stats, sizes, nwin)
@@ -357,7 +435,11 @@ def main():
         "cpu_baseline": None,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(run, layers, tab, tzu, tbnd, emis)
+        rec, ref_rad, ref_ods = cpu_baseline(dv, picks)
+        out["cpu_baseline"] = rec
+        out["accuracy"] = dict(accuracy(ref_rad, ref_ods, {"fp64": r64, "fp32_split": r32}, gods, v),
+                               against="the oracle's FP64 spectrum from the cpu_baseline leg; optical depths "
+                                       "of layers %s" % (list(picks),))
     print(json.dumps(out), flush=True)
 
 

@@ -526,12 +526,15 @@ class DeviceBuffer:
             raise RuntimeError("lblx_dev_alloc(%d) failed" % self.nbytes)
         self.ptr = p
 
-    def to_host(self, dtype, count=None):
+    def to_host(self, dtype, count=None, offset=0):
+        """count elements of dtype from element offset (all by default)."""
         dtype = np.dtype(dtype)
-        n = self.nbytes // dtype.itemsize if count is None else int(count)
+        n = self.nbytes // dtype.itemsize - offset if count is None else int(count)
+        assert 0 <= offset and (offset + n) * dtype.itemsize <= self.nbytes
         out = np.empty(n, dtype)
         _chk(self.ctx._lib.lblx_copy(self.ctx._h, out.ctypes.data_as(C.c_void_p),
-                                     C.c_void_p(self.ptr), n * dtype.itemsize, 1), "lblx_copy")
+                                     C.c_void_p(self.ptr + offset * dtype.itemsize), n * dtype.itemsize, 1),
+             "lblx_copy")
         return out
 
     def from_host(self, arr):
