@@ -170,6 +170,29 @@ int lblx_plan_dv(const lblx_run *run, int nlay, int nmol, const double *pave,
 /* Number of output points OPDPTH writes for a layer (sum of panel NLIM). */
 int lblx_od_size(const lblx_run *run, const lblx_layer *layer, int64_t *npts);
 
+/* Panel headers OPDPTH writes for the layer (PANEL, src/oprop.f90:1418-
+ * 1451): V1P, V2P and NLIM of each panel of <= 2400 points; the panels'
+ * points are the lblx_od_layer output in order.  npanel receives the
+ * count; with cap below it nothing else is written and LBLX_E_SIZE is
+ * returned (a count query).  Needs no device.                            */
+int lblx_od_panels(const lblx_run *run, const lblx_layer *layer, int cap, double *v1p, double *v2p,
+                   int32_t *nlim, int *npanel);
+
+/* The reference's panel files (TAPE10 layer optical depth, TAPE11/TAPE12
+ * merged results) written byte for byte as its BUFOUT records would hold
+ * them: sequential unformatted records with 4-byte markers; an optional
+ * file header record (the /FILHDR/ words, filhdr_bytes long, written as
+ * given); per panel a header record V1P, V2P (f64), DVP, NLIM (f64 and
+ * i64 when word = 8, the dbl build; f32 and i32 when word = 4), then the
+ * panel's values of a (the optical depth, or the radiance) and, when b is
+ * not NULL, of b (the transmittance, EMOUT src/xmerge.f90:3737-3790), as
+ * REALs of `word` bytes; then, with endfil, ENDFIL's record of six -99
+ * (src/lblrtm.f90:1368-1377).  a and b hold sum(nlim) values in panel
+ * order.  append != 0 adds to an existing file.                          */
+int lblx_write_panels(const char *path, int append, int word, const void *filhdr, int64_t filhdr_bytes,
+                      int npanel, const double *v1p, const double *v2p, double dvp, const int32_t *nlim,
+                      const double *a, const double *b, int endfil);
+
 /* OPDPTH for one layer (IHIRAC=1, IOD=0): continuum (ICNTNM) + LINF4 +
  * HIRAC1/LBLF4 panels.  od receives npts values on the grid
  * v1p + k*dv (v1p returned); double for LBLX_FP64, float for

@@ -41,5 +41,27 @@ def build(verbose=False):
     return LIB
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+
+def build_fortran(verbose=False):
+    """The Fortran interface (lblx_c) and the OPDPTH/XMERGE shims, compiled
+    with the reference's double-precision flags into lblrtm_amd/fortran/
+    _build (objects and module files a maintainer links with the
+    reference).  Returns the build directory, or None without flang."""
+    if not os.path.exists(FLANG):
+        return None
+    src = os.path.join(HERE, "fortran")
+    out = os.path.join(src, "_build")
+    os.makedirs(out, exist_ok=True)
+    flags = ["-O1", "-fdefault-integer-8", "-fdefault-real-8", "-fdefault-double-8", "-module-dir", out,
+             "-I", out]
+    for name in ("lblx_c", "lblx_shims"):
+        _run([FLANG, "-c"] + flags + [os.path.join(src, name + ".f90"), "-o", os.path.join(out, name + ".o")],
+             verbose)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

@@ -537,6 +537,102 @@ int lblx_od_size(const lblx_run *run, const lblx_layer *layer, int64_t *npts) {
   return LBLX_OK;
 }
 
+int lblx_od_panels(const lblx_run *run, const lblx_layer *layer, int cap, double *v1p, double *v2p,
+                   int32_t *nlim, int *npanel) {
+  if (!run || !layer || !npanel) return LBLX_E_ARG;
+  lblx_layer tmp = *layer;
+  LayerPlan P;
+  int rc = plan_layer(run, &tmp, &P);
+  if (rc) return rc;
+  const int np = (int)P.panels.size();
+  *npanel = np;
+  if (cap < np) return (v1p || v2p || nlim) ? LBLX_E_SIZE : LBLX_OK;
+  for (int k = 0; k < np; k++) {
+    const PanelPlan &q = P.panels[k];
+    // PANEL: V1P = VFT + (NLO-1) DV with NLO = NSHIFT+1; NLIM = NHI-NLO+1
+    if (v1p) v1p[k] = q.vft + (double)NSHIFT * P.dv;
+    if (v2p) v2p[k] = q.v2p;
+    if (nlim) nlim[k] = q.nhi - NSHIFT;
+  }
+  return LBLX_OK;
+}
+
+int lblx_write_panels(const char *path, int append, int word, const void *filhdr, int64_t filhdr_bytes,
+                      int npanel, const double *v1p, const double *v2p, double dvp, const int32_t *nlim,
+                      const double *a, const double *b, int endfil) {
+  if (!path || (word != 4 && word != 8) || npanel < 0 || (npanel > 0 && (!v1p || !v2p || !nlim || !a)) ||
+      (filhdr && filhdr_bytes <= 0) || filhdr_bytes > INT32_MAX)
+    return LBLX_E_ARG;
+  FILE *f = std::fopen(path, append ? "ab" : "wb");
+  if (!f) return LBLX_E_ARG;
+  bool ok = true;
+  // one sequential unformatted record: 4-byte length, body, 4-byte length
+  auto record = [&](const void *body, int64_t n) {
+    const int32_t m = (int32_t)n;
+    ok = ok && std::fwrite(&m, 4, 1, f) == 1 && (n == 0 || std::fwrite(body, 1, (size_t)n, f) == (size_t)n) &&
+         std::fwrite(&m, 4, 1, f) == 1;
+  };
+  if (filhdr) record(filhdr, filhdr_bytes);
+  std::vector<unsigned char> buf;
+  int64_t o = 0;
+  for (int k = 0; k < npanel && ok; k++) {
+    const int32_t n = nlim[k];
+    if (n < 0 || (int64_t)n * word > INT32_MAX) {
+      ok = false;
+      break;
+    }
+    // /BUFPNL/ V1P, V2P (REAL*8 under IMPLICIT REAL*8 (V)), DVP (REAL),
+    // NLIM (INTEGER): default REAL and INTEGER are `word` bytes
+    unsigned char h[32];
+    size_t hn = 0;
+    std::memcpy(h + hn, &v1p[k], 8);
+    hn += 8;
+    std::memcpy(h + hn, &v2p[k], 8);
+    hn += 8;
+    if (word == 8) {
+      const int64_t nl = n;
+      std::memcpy(h + hn, &dvp, 8);
+      std::memcpy(h + hn + 8, &nl, 8);
+      hn += 16;
+    } else {
+      const float d4 = (float)dvp;
+      std::memcpy(h + hn, &d4, 4);
+      std::memcpy(h + hn + 4, &n, 4);
+      hn += 8;
+    }
+    record(h, (int64_t)hn);
+    for (const double *src : {a, b}) {
+      if (!src) continue;
+      buf.resize((size_t)n * word);
+      if (word == 8) {
+        std::memcpy(buf.data(), src + o, (size_t)n * 8);
+      } else {
+        for (int32_t i = 0; i < n; i++) {
+          const float x = (float)src[o + i];
+          std::memcpy(buf.data() + 4 * (size_t)i, &x, 4);
+        }
+      }
+      record(buf.data(), (int64_t)n * word);
+    }
+    o += n;
+  }
+  if (endfil && ok) {  // ENDFIL: six default integers of -99 (lblrtm.f90:1368-1377)
+    unsigned char e[48];
+    for (int i = 0; i < 6; i++) {
+      if (word == 8) {
+        const int64_t v = -99;
+        std::memcpy(e + 8 * i, &v, 8);
+      } else {
+        const int32_t v = -99;
+        std::memcpy(e + 4 * i, &v, 4);
+      }
+    }
+    record(e, 6 * word);
+  }
+  if (std::fclose(f) != 0) ok = false;
+  return ok ? LBLX_OK : LBLX_E_ARG;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------

@@ -71,6 +71,11 @@ def load():
         "lblx_fftscn_device": (C.c_int, [C.c_void_p, P(abi.Scan), C.c_void_p, C.c_double, C.c_double,
                                          C.c_int64, C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_double),
                                          P(C.c_double)]),
+        "lblx_od_panels": (C.c_int, [P(abi.Run), P(abi.Layer), C.c_int, P(C.c_double), P(C.c_double),
+                                     P(C.c_int32), P(C.c_int)]),
+        "lblx_write_panels": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_int,
+                                        P(C.c_double), P(C.c_double), C.c_double, P(C.c_int32), P(C.c_double),
+                                        P(C.c_double), C.c_int]),
         "lblx_fftscn_noalloc_device": (C.c_int, [C.c_void_p, P(abi.Scan), C.c_void_p, C.c_double, C.c_double,
                                                  C.c_int64, C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_double),
                                                  P(C.c_double)]),
@@ -141,6 +146,35 @@ def plan_dv(run, pave, tave, wk, wbroad, iemit=0):
                         abi.ptr(wbroad, C.c_double), iemit, abi.ptr(dv, C.c_double),
                         abi.ptr(it, C.c_int32)), "lblx_plan_dv")
     return dv, it
+
+
+def od_panels(run, layer):
+    """Panel headers OPDPTH writes for the layer: (v1p, v2p, nlim) arrays."""
+    L = load()
+    n = C.c_int(0)
+    L.lblx_od_panels(C.byref(run), C.byref(layer), 0, None, None, None, C.byref(n))
+    v1 = np.zeros(n.value)
+    v2 = np.zeros(n.value)
+    nl = np.zeros(n.value, np.int32)
+    _chk(L.lblx_od_panels(C.byref(run), C.byref(layer), n.value, abi.ptr(v1, C.c_double), abi.ptr(v2, C.c_double),
+                          abi.ptr(nl, C.c_int32), C.byref(n)), "lblx_od_panels")
+    return v1, v2, nl
+
+
+def write_panels(path, v1p, v2p, dvp, nlim, a, b=None, filhdr=None, word=8, endfil=False, append=False):
+    """The reference's panel file (TAPE10 with a only, TAPE11/12 with a and
+    b) through lblx_write_panels; filhdr is the header record's bytes."""
+    v1p = np.ascontiguousarray(v1p, np.float64)
+    v2p = np.ascontiguousarray(v2p, np.float64)
+    nlim = np.ascontiguousarray(nlim, np.int32)
+    a = np.ascontiguousarray(a, np.float64)
+    bb = None if b is None else np.ascontiguousarray(b, np.float64)
+    hb = None if filhdr is None else bytes(filhdr)
+    _chk(load().lblx_write_panels(str(path).encode(), int(append), word, hb, len(hb) if hb else 0, len(nlim),
+                                  abi.ptr(v1p, C.c_double), abi.ptr(v2p, C.c_double), dvp,
+                                  abi.ptr(nlim, C.c_int32), abi.ptr(a, C.c_double),
+                                  abi.ptr(bb, C.c_double) if bb is not None else None, int(endfil)),
+         "lblx_write_panels")
 
 
 def od_size(run, layer):

@@ -157,3 +157,66 @@ def read_tape3(path):
     t = LineTable(np.array(nrec), np.array(vmin), np.array(vmax), **arrays)
     t.nbrd = nbrd  # lines carrying broadener flags (IBRD data)
     return t
+
+
+def read_panels(path, word=8, nvals=1, filhdr_words=None):
+    """Read a panel file of the reference (TAPE10: nvals = 1, the optical
+    depth; TAPE11/TAPE12: nvals = 2, radiance and transmittance), laid out
+    as BUFOUT writes it: an optional file header record of filhdr_words
+    words, then per panel the /BUFPNL/ header V1P, V2P (REAL*8), DVP
+    (REAL), NLIM (INTEGER) and nvals records of NLIM REALs, default REAL and
+    INTEGER being `word` bytes; an ENDFIL record of six -99 may end it
+    (src/oprop.f90:1524-1526, src/xmerge.f90:3737-3790,
+    src/lblrtm.f90:1368-1377).  Returns a dict: filhdr (bytes or None),
+    v1p, v2p, dvp, nlim (arrays per panel), values (list of nvals arrays,
+    panels concatenated) and endfil."""
+    data = open(path, "rb").read()
+    pos = 0
+    recs = []
+    while pos < len(data):
+        (n,) = struct.unpack_from("<i", data, pos)
+        body = data[pos + 4:pos + 4 + n]
+        (n2,) = struct.unpack_from("<i", data, pos + 4 + n)
+        if n2 != n:
+            raise ValueError("panel file: record marker mismatch at %d" % pos)
+        recs.append(body)
+        pos += 8 + n
+    fdt = "<f8" if word == 8 else "<f4"
+    idt = "<i8" if word == 8 else "<i4"
+    out = {"filhdr": None, "v1p": [], "v2p": [], "dvp": [], "nlim": [], "endfil": False}
+    k = 0
+    if filhdr_words is not None:
+        if len(recs[0]) != filhdr_words * word:
+            raise ValueError("panel file: header record of %d bytes" % len(recs[0]))
+        out["filhdr"] = recs[0]
+        k = 1
+    vals = [[] for _ in range(nvals)]
+    hsize = 16 + 2 * word
+    while k < len(recs):
+        r = recs[k]
+        if len(r) == 6 * word and np.all(np.frombuffer(r, idt) == -99):
+            out["endfil"] = True
+            k += 1
+            break
+        if len(r) != hsize:
+            raise ValueError("panel file: panel header of %d bytes" % len(r))
+        v1, v2 = struct.unpack_from("<dd", r, 0)
+        dvp = float(np.frombuffer(r[16:16 + word], fdt)[0])
+        nlim = int(np.frombuffer(r[16 + word:16 + 2 * word], idt)[0])
+        for j in range(nvals):
+            body = recs[k + 1 + j]
+            if len(body) != nlim * word:
+                raise ValueError("panel file: %d values for NLIM %d" % (len(body) // word, nlim))
+            vals[j].append(np.frombuffer(body, fdt).astype(np.float64))
+        out["v1p"].append(v1)
+        out["v2p"].append(v2)
+        out["dvp"].append(dvp)
+        out["nlim"].append(nlim)
+        k += 1 + nvals
+    if k != len(recs):
+        raise ValueError("panel file: records after ENDFIL")
+    for key in ("v1p", "v2p", "dvp"):
+        out[key] = np.array(out[key])
+    out["nlim"] = np.array(out["nlim"], np.int64)
+    out["values"] = [np.concatenate(v) if v else np.zeros(0) for v in vals]
+    return out

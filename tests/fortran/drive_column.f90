@@ -1,0 +1,135 @@
+! Test driver in the role of XLAYER's IMRG = 0 layer loop (src/lblrtm.f90:
+! 3634-3655): for each layer of the column in layers.txt it fills the
+! reference's COMMON blocks, calls OPDPTH and XMERGE (the liblblx shims of
+! lblrtm_amd/fortran/lblx_shims.f90) with KFILE = TAPE10 and LFILE/MFILE
+! alternating between TAPE11 and TAPE12, and writes the unit of the file
+! that holds the final merged result to final.txt.  TAPE3 is read from the
+! working directory.
+!
+! layers.txt: nlay nmol icntnm / v1 v2 tbnd emis / then per layer
+! pave tave dv tzu wbroad wk(1:nmol)
+program drive_column
+  IMPLICIT REAL*8 (V)
+  COMMON /MANE/ P0,TEMP0,NLAYER,DVXM,H2OSLF,WTOT,ALBAR,ADBAR,AVBAR,    &
+       &        AVFIX,LAYER,SECNT0,SAMPLE,DVSET,ALFAL0,AVMASS,           &
+       &        DPTMIN,DPTFAC,ALTAV,AVTRAT,TDIFF1,TDIFF2,ALTD1,          &
+       &        ALTD2,ANGLE,IANT,LTGNT,LH1,LH2,IPFLAG,PLAY,TLAY,         &
+       &        EXTID(10)
+  CHARACTER*8 EXTID
+  character*8 XID, HMOLID, YID
+  real*8 SECANT, XALTZ
+  COMMON /FILHDR/ XID(10),SECANT,PAVE,TAVE,HMOLID(60),XALTZ(4),        &
+       &          WK(60),PZL,PZU,TZL,TZU,WBROAD,DV ,V1 ,V2 ,TBOUND,      &
+       &          EMISIV,FSCDID(17),NMOL,LAYRS ,YI1,YID(10),LSTWDF
+  COMMON /IFIL/ IRD,IPR,IPU,NOPR,NFHDRF,NPHDRF,NFHDRL,NPHDRL,          &
+       &        NLNGTH,KFILE,KPANEL,LINFIL,NFILE,IAFIL,IEXFIL,           &
+       &        NLTEFL,LNFIL4,LNGTH4,IBRD
+  COMMON /IODFLG/ DVOUT
+  COMMON /RCNTRL/ ILNFLG
+  COMMON /CNTSCL/ XSELF,XFRGN,XCO2C,XO3CN,XO2CN,XN2CN,XRAYL
+  COMMON /CLOSURE/ FRGNX
+  character*1 FRGNX
+  COMMON /BNDPRP/ TMPBND,BNDEMI(3),BNDRFL(3),IBPROP,surf_refl,         &
+       &          pad_3,angle_path, secant_diffuse, secant_path, diffuse_fac
+  character*1 surf_refl
+  character*3 pad_3
+  EQUIVALENCE (FSCDID(1),IHIRAC) , (FSCDID(2),ILBLF4),                 &
+       &      (FSCDID(3),IXSCNT) , (FSCDID(4),IAERSL),                   &
+       &      (FSCDID(5),IEMIT) , (FSCDID(8),IPATHL),                    &
+       &      (FSCDID(9),JRAD) , (FSCDID(11),IMRG)
+  INTEGER NL, NM, ICN, L, M, LF, MF, MSTOR, K
+  REAL EM
+
+  open (20, file='layers.txt', status='old')
+  read (20, *) NL, NM, ICN
+  read (20, *) V1, V2, TMPBND, EM
+  ! /FILHDR/ in the double-precision build: 177 eight-byte words before
+  ! LSTWDF; a panel header V1P, V2P, DVP, NLIM is 4 words
+  NFHDRF = 177
+  NPHDRF = 4
+  IRD = 5
+  IPR = 6
+  NOPR = 1
+  KFILE = 10
+  IBRD = 0
+  P0 = 1013.25
+  TEMP0 = 296.
+  SAMPLE = 4.
+  ALFAL0 = 0.04
+  AVMASS = 36.
+  DPTMIN = 2.E-4
+  DPTFAC = 1.E-3
+  XSELF = 1.
+  XFRGN = 1.
+  XCO2C = 1.
+  XO3CN = 1.
+  XO2CN = 1.
+  XN2CN = 1.
+  XRAYL = 1.
+  FRGNX = '0'
+  DVOUT = 0.
+  ILNFLG = 0
+  BNDEMI(1) = EM
+  BNDEMI(2) = 0.
+  BNDEMI(3) = 0.
+  BNDRFL = 0.
+  IBPROP = 0
+  do K = 1, 17
+     FSCDID(K) = 0.
+  end do
+  IHIRAC = 1
+  ILBLF4 = 1
+  IXSCNT = ICN
+  IAERSL = 0
+  IEMIT = 1
+  IPATHL = 1
+  JRAD = 1
+  IMRG = 0
+  LH1 = 1
+  LH2 = NL
+  NLAYER = NL
+  do K = 1, 10
+     XID(K) = 'LBLX    '
+     YID(K) = '        '
+  end do
+  do K = 1, 60
+     HMOLID(K) = '        '
+     WK(K) = 0.
+  end do
+  SECANT = 1.
+  XALTZ = 0.
+  PZL = 0.
+  PZU = 0.
+  TZL = 0.
+  TBOUND = TMPBND
+  EMISIV = EM
+  YI1 = 0.
+  NMOL = NM
+  open (10, file='TAPE10', form='unformatted', status='replace')
+  open (11, file='TAPE11', form='unformatted', status='replace')
+  open (12, file='TAPE12', form='unformatted', status='replace')
+  LF = 11
+  MF = 12
+  do L = 1, NL
+     read (20, *) PAVE, TAVE, DV, TZU, WBROAD, (WK(M), M = 1, NM)
+     LAYER = L
+     LAYRS = L
+     rewind 10
+     call OPDPTH (0)
+     rewind 10
+     rewind MF
+     rewind LF
+     call XMERGE (0, LF, MF, 0)
+     if (L == NL) exit
+     MSTOR = MF
+     MF = LF
+     LF = MSTOR
+  end do
+  close (10)
+  close (11)
+  close (12)
+  close (20)
+  open (21, file='final.txt')
+  write (21, *) MF
+  close (21)
+end program drive_column
