@@ -276,6 +276,34 @@ int lblx_jacobian_aj_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *laye
                             const double *tzu, double tbnd, double emis, const int *species, int nspec,
                             double *d_rad, double *d_jac, int64_t n_out, double *v1p_top);
 
+/* The same outputs with the layer optical-depth derivatives taken
+ * analytically, in one OPDPTH run of the column: its tangent-linear passes
+ * differentiate LNCOR1/CNVFNV (strengths, Lorentz, Doppler and Voigt
+ * widths, zeta interpolation, pressure-shifted centres; oprop.f90:
+ * 995-1320), LINF4/SHRINK/CONVF4 (each TAPE9 line from its group's
+ * members, oprop.f90:3093-3640, 3878-4039) and the radiation term on the
+ * forward run's own decisions (kept lines, groups, accepted far-wing lines,
+ * panels), and take the continuum (CONTNM) by central differences.  The
+ * molecules' other-line width change through the total amount is left out
+ * (below W_m / W_tot of their self-broadened share).  species[] must be
+ * distinct; the exact-Voigt build is not supported (LBLX_E_UNSUPPORTED).
+ * Replaces the per-parameter OPDPTH runs of the reference's analytic
+ * Jacobians (src/lblrtm.f90:2930-2975) for the same layout as
+ * lblx_jacobian_device.                                                  */
+int lblx_jacobian_tl_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay,
+                            const double *tzu, double tbnd, double emis, const int *species, int nspec,
+                            double *d_rad, double *d_jac, int64_t n_out, double *v1p_top);
+
+/* The tangent-linear layer optical depths alone: OPDPTH for nlay layers
+ * (JRAD chained up the list, as in a column) with layer l's optical depth
+ * at d_od + l * n_out and its derivatives at d_dod + (l * (1 + nspec) + q)
+ * * n_out, q = 0 temperature (per K), q >= 1 species q (per unit ln
+ * amount); elements double, or float under LBLX_FP32_SPLIT.  The layer-
+ * level counterpart of lblx_jacobian_tl_device (oprop.f90:25-668 with its
+ * derivatives, as there).                                                */
+int lblx_od_tangent_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay,
+                           const int *species, int nspec, void *d_od, void *d_dod, int64_t n_out);
+
 /* A batch of independent profiles (ragged: nlay[p] layers each, surface
  * first, concatenated in layers/tzu; each profile's own band in its
  * layers' v1/v2).  Layer optical depths run in batches of whole profiles
@@ -294,8 +322,9 @@ int lblx_spectra_batch_device(lblx_ctx *ctx, const lblx_run *run, int nprof, con
  * rad_off[p] and its Jacobian planes (lblx_jacobian_device's layout with
  * its own layer count and top-grid size) at d_jac + jac_off[p].  aj = 0
  * takes OD derivatives by forward differences (dT, dlnq), aj = 1 the
- * reference's analytic-Jacobian runs (lblx_jacobian_aj_device, dT and dlnq
- * unused).  Each profile's perturbed layers run as one OPDPTH batch; the
+ * reference's analytic-Jacobian runs (lblx_jacobian_aj_device), aj = 2 the
+ * tangent-linear optical depths (lblx_jacobian_tl_device; dT and dlnq
+ * unused for aj >= 1).  Each profile's perturbed layers run as one OPDPTH batch; the
  * profiles are queued one after another on the context's stream, the next
  * one planned on the host while the last one runs.  No counterpart call in
  * the reference, whose AJ runs are one TAPE5 per profile
@@ -387,11 +416,12 @@ void *lblx_stream(lblx_ctx *ctx);
 /* Stage timers: HIP events around each stage, on the stream that runs it
  * (linf4, shrink, convf4, lncor1, reject, scatter_rec, tile_r3, tile_r2,
  * tile_r1, merge, contnm, scan), the device analogue of the TAPE6 timing
- * table (oprop.f90:547-620), and "host": wall milliseconds the host spends
- * planning layers and staging descriptors.  lblx_timing(ctx, 1) clears and enables;
+ * table (oprop.f90:547-620), "host": wall milliseconds the host spends
+ * planning layers and staging descriptors, and "tangent": the
+ * tangent-linear optical-depth passes of lblx_jacobian_tl_device.  lblx_timing(ctx, 1) clears and enables;
  * lblx_timing_get returns the summed milliseconds and launch count of a
  * stage index (0 .. LBLX_NSTAGE-1).                                       */
-#define LBLX_NSTAGE 13
+#define LBLX_NSTAGE 14
 
 /* Tables of the exact-Voigt build (voigt_init, src/oprop_voigt.f90:
  * 11739-12008) as this library computes them on the host: f = F1, F2, F3,

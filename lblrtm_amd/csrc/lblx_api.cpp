@@ -29,6 +29,9 @@
 #ifndef LBLX_FP32_LDS64
 #define LBLX_FP32_LDS64 0xC  // bits G = 2, 3
 #endif
+#ifndef LBLX_TAN_LDS64
+#define LBLX_TAN_LDS64 0xC  // bits G = 2, 3
+#endif
 #include "lblx_plan.h"
 #include "lblx_contdata.h"
 #include "lblx_refdata.h"
@@ -36,6 +39,7 @@
 // The kernels are compiled in this translation unit (one HIP module).
 #include "lblx_kernels.hip"
 #include "lblx_scan.hip"
+#include "lblx_tangent.hip"
 
 namespace lblx {
 void shape_tables_host(double *tab /* 5 x 2601: F1 F2 F3 FG XVER */);
@@ -128,10 +132,10 @@ struct DevBuf {
 // oprop.f90:547-620): HIP events recorded on the context stream around each
 // stage, summed when collected.
 enum Stage { ST_LINF4, ST_SHRINK, ST_CONVF4, ST_LNCOR1, ST_REJECT, ST_SCATTER, ST_TILE3, ST_TILE2,
-             ST_TILE1, ST_MERGE, ST_CONTNM, ST_SCAN, ST_HOST, ST_N };
+             ST_TILE1, ST_MERGE, ST_CONTNM, ST_SCAN, ST_HOST, ST_TANGENT, ST_N };
 static const char *kStageNames[ST_N] = {"linf4", "shrink", "convf4", "lncor1", "reject",
                                         "scatter_rec", "tile_r3", "tile_r2", "tile_r1",
-                                        "merge", "contnm", "scan", "host"};
+                                        "merge", "contnm", "scan", "host", "tangent"};
 
 struct lblx_ctx {
   int device = 0;
@@ -143,6 +147,9 @@ struct lblx_ctx {
   // (LBLX_FIX_MAXPASS at creation, 1..64), and what the last OD batch used
   int fix_maxpass = 16;
   int fix_passes = 0, fix_seq = 0;
+  // tangent tiles of grid G accumulate in FP64 LDS when bit G is set
+  // (LBLX_TAN_LDS64 at creation)
+  int tan_lds64 = LBLX_TAN_LDS64;
   void *scratch = nullptr;  // device address of `scratch` on this device
   hipStream_t stream = nullptr;
   bool timing = false;
@@ -290,6 +297,10 @@ struct lblx_ctx {
   std::vector<LayerPlan> jplans;          // the Jacobian's job plans, storage reused
   DevBuf s_z[2], s_tw, s_g, s_ip, s_cof;  // FFTSCN
   DevBuf j_rall, j_a, j_b;                // Jacobians
+  // tangent-linear optical depth: compacted-line records, CONVF4's accepted
+  // lines, per-line derivative weights, channel rows, continuum channels
+  DevBuf t_craw, t_amask, t_wspan, t_tl, t_t9, t_r4c, t_tlay, t_cps, t_absab, t_absc, t_rh, t_r3c, t_r2c,
+      t_odcoff, t_tmap[3], t_tabs, t_mem, t_memq, t_f4c, t_f4d;
   int64_t s_twN = 0;
   int tiles[3] = {8192, 4096, 2048};      // R1, R2, R3 tile widths (autotuned)
   DevBuf b_jobs, b_rt, b_rl;              // batched merges
@@ -356,6 +367,8 @@ int lblx_create(lblx_ctx **out, int device) {
     c->t9_all_wave = (e && std::atoi(e) != 0) ? 1 : 0;
     const char *f = std::getenv("LBLX_FIX_MAXPASS");
     if (f) c->fix_maxpass = std::min(64, std::max(1, std::atoi(f)));
+    const char *tl = std::getenv("LBLX_TAN_LDS64");
+    if (tl) c->tan_lds64 = (int)std::strtol(tl, nullptr, 0) & 0xE;
   }
   {
     int ncu = 0;
@@ -440,7 +453,11 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
                     &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
-                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->w_mrun, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt, &c->b_rl};
+                    &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->w_mrun, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt, &c->b_rl,
+                    &c->w_dirty, &c->d_vtab, &c->d_vtabf, &c->w_dvo,
+                    &c->t_craw, &c->t_amask, &c->t_wspan, &c->t_tl, &c->t_t9, &c->t_r4c, &c->t_tlay, &c->t_cps,
+                    &c->t_absab, &c->t_absc, &c->t_rh, &c->t_r3c, &c->t_r2c, &c->t_odcoff, &c->t_tmap[0],
+                    &c->t_tmap[1], &c->t_tmap[2], &c->t_tabs, &c->t_mem, &c->t_memq, &c->t_f4c, &c->t_f4d};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
   c->stage_release();
@@ -1055,10 +1072,217 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
   return LBLX_OK;
 }
 
+// A tangent-linear request riding on an OD batch: the channels (the
+// temperature, then the molecules), per job the tangent constants, per
+// (job, channel) the continuum plans stepped up and down, and where each
+// (job, channel) optical-depth derivative goes (element offsets from odc,
+// in the batch's output type).
+struct TanReq {
+  int P = 1;
+  std::vector<TanLayer> tl;
+  std::vector<ContPlan> cps;  // [(job P + q) 2 + {0: +h, 1: -h}] (empty without ICNTNM)
+  double rh[TAN_NQ] = {0};    // 1 / (2 h) per channel
+  void *odc = nullptr;
+  std::vector<int64_t> odc_off;  // [job P + q]
+};
+
+// Largest power of two tile (at most cap points) whose P channel
+// accumulators of `esz` bytes fit the LDS beside `fixed` bytes of tables.
+static int tan_tile_width(int cap, int P, size_t esz, size_t fixed) {
+  const size_t lds = 160u << 10;
+  int t = 256;
+  while (2 * t <= cap && fixed + (size_t)P * (2 * t) * esz <= lds) t *= 2;
+  return t;
+}
+
+template <int P>
+static void launch_convf4_tan(dim3 g, hipStream_t s, const Tape9 &T9, const F4TConst *fc, const float4 *fd,
+                              const DevLayer *dY, const TanLayer *dT, Strides S, const DevWin *dw, const DevSeg *dg,
+                              const uint64_t *am, int64_t mwords, const int32_t *ws, const double *r4, double *r4c) {
+  hipLaunchKernelGGL(k_convf4_tan<P>, g, dim3(CFT_T), 0, s, T9, fc, fd, dY, dT, S, dw, dg, am, mwords, ws, r4,
+                     r4c);
+}
+
+// The tangent passes of a batch, after its forward launches (which left
+// the scatter records, TAPE9, the compacted LINF4 lines, the accepted
+// far-wing lines and the forward R4 rows in the context's work buffers).
+static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, const TanReq &R, const Recs &recs,
+                          const Tape9 &T9, const ShrinkWork &W, int64_t mwords, bool f32out, void *od_base) {
+  hipStream_t s = c->stream;
+  const int njobs = J.njobs;
+  const int64_t NJ = njobs;
+  const Strides S = J.S;
+  const int PO = R.P;                  // output channels
+  const int P = PO > 1 ? PO + 1 : PO;  // internal channels (U when species are asked for)
+  const int nwin = (int)J.win.size();
+  const int abs_stride = J.abs_stride;
+  const bool cont = abs_stride > 0;
+  if (PO < 1 || P > TAN_NQ || (int64_t)R.tl.size() != NJ || (int64_t)R.odc_off.size() != NJ * PO || !R.odc ||
+      (cont && (int64_t)R.cps.size() != 2 * NJ * PO))
+    return LBLX_E_ARG;
+  for (int jb = 0; jb < njobs; jb++)
+    if (R.tl[jb].nchan != PO || R.tl[jb].nint != P) return LBLX_E_ARG;
+  if (!c->t_tabs.p) {  // F1 FG XVER F2 F3 and their slopes per table step, FP32
+    std::vector<double> t5(5 * 2601);
+    shape_tables_host(t5.data());
+    const int ord[5] = {0, 3, 4, 1, 2};  // from F1 F2 F3 FG XVER
+    std::vector<float> tt(10 * 2601);
+    for (int a = 0; a < 5; a++) {
+      const double *f = t5.data() + (size_t)ord[a] * 2601;
+      for (int i = 0; i < 2601; i++) {
+        // the shapes are even in z: zero slope at the centre
+        const double sl = i == 0 ? 0. : (i == 2600 ? f[i] - f[i - 1] : 0.5 * (f[i + 1] - f[i - 1]));
+        tt[(size_t)a * 2601 + i] = (float)f[i];
+        tt[(size_t)(5 + a) * 2601 + i] = (float)sl;
+      }
+    }
+    if (c->t_tabs.ensure(tt.size() * 4)) return LBLX_E_HIP;
+    HIPCHK(hipMemcpy(c->t_tabs.p, tt.data(), tt.size() * 4, hipMemcpyHostToDevice));
+  }
+  // tile maps of the tangent tiles (narrower than the forward's: P
+  // accumulators and the slope tables share the LDS)
+  const size_t tabb[3] = {6u * 2601 * 4, 2u * 2601 * 4, 2u * 2601 * 4};
+  int tw[3];
+  std::vector<int32_t> tmv[3];
+  bool l64[3];
+  for (int G = 0; G < 3; G++) {
+    l64[G] = (c->tan_lds64 >> (G + 1)) & 1;
+    tw[G] = tan_tile_width(J.tiles[G], P, l64[G] ? 8 : 4, tabb[G]);
+    std::vector<int32_t> tj, tlc;
+    for (int jb = 0; jb < njobs; jb++) {
+      const LayerPlan &Pl = *J.plans[jb];
+      const int64_t ng[3] = {Pl.g_last + 1, Pl.n2, Pl.n3};
+      const int nt = (int)((ng[G] + tw[G] - 1) / tw[G]);
+      for (int t = 0; t < nt; t++) {
+        tj.push_back(jb);
+        tlc.push_back(t);
+      }
+    }
+    tmv[G] = tj;
+    tmv[G].insert(tmv[G].end(), tlc.begin(), tlc.end());
+  }
+  auto need = [&](DevBuf &b, size_t bytes) { return b.ensure(bytes) != 0; };
+  if (need(c->t_tlay, NJ * sizeof(TanLayer)) || need(c->t_tl, NJ * S.line * sizeof(TanLine)) ||
+      need(c->t_t9, NJ * S.line * 4 * P * 4) || need(c->t_r4c, (size_t)nwin * P * R4ROW * 8) ||
+      need(c->t_mem, NJ * S.line * sizeof(MemTan)) || need(c->t_memq, NJ * S.line) ||
+      need(c->t_f4c, NJ * S.line * sizeof(F4TConst)) || need(c->t_f4d, NJ * S.line * P * sizeof(float4)) ||
+      need(c->t_r3c, (size_t)P * J.r3tot * 8) || need(c->t_r2c, (size_t)P * J.r2tot * 8) ||
+      need(c->t_odcoff, NJ * PO * 8) || need(c->t_rh, TAN_NQ * 8) ||
+      (cont && (need(c->t_cps, 2 * NJ * PO * sizeof(ContPlan)) || need(c->t_absab, 2 * NJ * PO * abs_stride * 8) ||
+                need(c->t_absc, NJ * PO * abs_stride * 8)))) {
+    set_err("tangent work buffer allocation failed");
+    return LBLX_E_HIP;
+  }
+  for (int G = 0; G < 3; G++)
+    if (need(c->t_tmap[G], tmv[G].size() * 8)) return LBLX_E_HIP;
+  c->stage_begin();
+  HIPCHK(c->h2d(c->t_tlay.p, R.tl.data(), NJ * sizeof(TanLayer), s));
+  HIPCHK(c->h2d(c->t_odcoff.p, R.odc_off.data(), NJ * PO * 8, s));
+  HIPCHK(c->h2d(c->t_rh.p, R.rh, TAN_NQ * 8, s));
+  if (cont) HIPCHK(c->h2d(c->t_cps.p, R.cps.data(), 2 * NJ * PO * sizeof(ContPlan), s));
+  TileMap tm[3];
+  for (int G = 0; G < 3; G++) {
+    const size_t nt = tmv[G].size() / 2;
+    int32_t *base = c->t_tmap[G].as<int32_t>();
+    HIPCHK(c->h2d(base, tmv[G].data(), nt * 8, s));
+    tm[G].job = base;
+    tm[G].loc = base + nt;
+    tm[G].lo = base + 2 * nt;
+    tm[G].hi = base + 3 * nt;
+  }
+  c->stage_end(s);
+  const DevLayer *dY = c->w_dlay.as<DevLayer>();
+  const TanLayer *dT = c->t_tlay.as<TanLayer>();
+  const DevPanel *dp = c->w_pan.as<DevPanel>();
+  const DevWin *dw = c->w_win.as<DevWin>();
+  const DevSeg *dg = c->w_seg.as<DevSeg>();
+  DevLines L = c->lines();
+  TanLine *tl = c->t_tl.as<TanLine>();
+  float *t9 = c->t_t9.as<float>();
+  double *r4c = c->t_r4c.as<double>();
+  int th = c->tbegin(ST_TANGENT);
+  if (J.maxnn > 0)
+    hipLaunchKernelGGL(k_near_tan, dim3((J.maxnn + 255) / 256, njobs), dim3(256), 0, s, L, dY, dT, S, tl);
+  const int lgrid = (int)((J.maxline + 1 + 255) / 256);
+  MemTan *mt = c->t_mem.as<MemTan>();
+  int8_t *mq = c->t_memq.as<int8_t>();
+  hipLaunchKernelGGL(k_shr_mem_tan, dim3(lgrid, njobs), dim3(256), 0, s, L, dY, dT, S, W, mt, mq);
+  hipLaunchKernelGGL(k_shr_emit_tan, dim3(lgrid, njobs), dim3(256), 0, s, dT, S, W, T9, (const MemTan *)mt,
+                     (const int8_t *)mq, t9);
+  F4TConst *fc = c->t_f4c.as<F4TConst>();
+  float4 *fd = c->t_f4d.as<float4>();
+  hipLaunchKernelGGL(k_f4_tan_lines, dim3(lgrid, njobs), dim3(256), 0, s, T9, (const float *)t9, dY, dT, S, fc, fd);
+  {
+    const dim3 g(nwin);
+    const uint64_t *am = c->t_amask.as<uint64_t>();
+    const int32_t *ws = c->t_wspan.as<int32_t>();
+    const double *r4 = c->w_r4.as<double>();
+    const F4TConst *cfc = fc;
+    const float4 *cfd = fd;
+    switch (P) {
+      case 1: launch_convf4_tan<1>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      case 2: launch_convf4_tan<2>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      case 3: launch_convf4_tan<3>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      case 4: launch_convf4_tan<4>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      case 5: launch_convf4_tan<5>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      case 6: launch_convf4_tan<6>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      case 7: launch_convf4_tan<7>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      case 8: launch_convf4_tan<8>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      default: launch_convf4_tan<9>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+    }
+  }
+  if (cont) {  // the output channels' continuum (U carries none)
+    const int64_t nplan = 2 * NJ * PO;
+    const int64_t nth = nplan * abs_stride;
+    hipLaunchKernelGGL(k_contnm, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s, c->t_cps.as<ContPlan>(),
+                       c->cont_tabs, abs_stride, (int)nplan, c->t_absab.as<double>());
+    const int64_t nd = NJ * PO * abs_stride;
+    hipLaunchKernelGGL(k_cont_diff, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s,
+                       (const double *)c->t_absab.as<double>(), abs_stride, njobs, PO,
+                       (const double *)c->t_rh.as<double>(), c->t_absc.as<double>());
+  }
+  const float *tabs = c->t_tabs.as<float>();
+  const double *absc = cont ? c->t_absc.as<double>() : nullptr;
+  double *r3c = c->t_r3c.as<double>(), *r2c = c->t_r2c.as<double>();
+  const int64_t *odcoff = c->t_odcoff.as<int64_t>();
+  for (int G = 3; G >= 1; G--) {
+    const int T = tw[G - 1];
+    const int nt = (int)(tmv[G - 1].size() / 2);
+    hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY, tm[G - 1],
+                       nt, T, G, c->w_margin.as<double>());
+    const bool a64 = l64[G - 1];
+    const size_t shm = (size_t)P * T * (a64 ? 8 : 4) + tabb[G - 1];
+#define LBLX_TAN_A(GG, OUTT, LACC, RIN, SIN, ROUT, SOUT)                                                           \
+  do {                                                                                                            \
+    HIPCHK(hipFuncSetAttribute((const void *)k_tan_tile<GG, OUTT, LACC>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                               (int)shm));                                                                        \
+    hipLaunchKernelGGL((k_tan_tile<GG, OUTT, LACC>), dim3(nt), dim3(TAN_TILE_T), shm, s, recs, (const TanLine *)tl,   \
+                       tm[GG - 1], T, tabs, dY, dT, S, dp, dw, (const double *)r4c, absc, abs_stride, dg,         \
+                       (const double *)(RIN), (int64_t)(SIN), (double *)(ROUT), (int64_t)(SOUT),                  \
+                       (const OUTT *)od_base, (OUTT *)R.odc, odcoff);                                             \
+  } while (0)
+#define LBLX_TAN_L(GG, OUTT, RIN, SIN, ROUT, SOUT)                   \
+  do {                                                              \
+    if (a64) LBLX_TAN_A(GG, OUTT, double, RIN, SIN, ROUT, SOUT);    \
+    else LBLX_TAN_A(GG, OUTT, float, RIN, SIN, ROUT, SOUT);         \
+  } while (0)
+    if (G == 3) LBLX_TAN_L(3, double, nullptr, 0, r3c, J.r3tot);
+    else if (G == 2) LBLX_TAN_L(2, double, r3c, J.r3tot, r2c, J.r2tot);
+    else if (f32out) LBLX_TAN_L(1, float, r2c, J.r2tot, nullptr, 0);
+    else LBLX_TAN_L(1, double, r2c, J.r2tot, nullptr, 0);
+#undef LBLX_TAN_L
+#undef LBLX_TAN_A
+  }
+  c->tend(th);
+  HIPCHK(hipGetLastError());
+  (void)run;
+  return LBLX_OK;
+}
+
 // Allocate, upload and launch a planned batch.  od_base + od_off[j]
 // receives job j's npts values.
 static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void *od_base, bool f32out,
-                       double *v1p, lblx_stats *st) {
+                       double *v1p, lblx_stats *st, const TanReq *tan = nullptr) {
   // FP32 split: line shapes evaluated and accumulated in FP32, positions,
   // grid offsets and the panel combine in FP64; OD written as float.
   const bool split = run->precision == LBLX_FP32_SPLIT;
@@ -1081,6 +1305,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   const int nwin = (int)win.size();
   c->last_line_stride = S.line;
   const bool voigt = run->voigt != 0;
+  if (tan && voigt) return LBLX_E_UNSUPPORTED;  // the tangent follows the tabulated shapes
   if (voigt) {
     int rv = ensure_voigt(c);
     if (rv) return rv;
@@ -1114,6 +1339,14 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   }
   for (int G = 0; G < 3; G++)
     if (need(c->w_tmap[G], tjob[G].size() * 16)) return LBLX_E_HIP;
+  // the tangent passes read back the compacted lines' records and the
+  // far-wing lines each window accepted (one word per 64-line chunk)
+  const int64_t mwords = S.line / 64 + 1;
+  if (tan && (need(c->t_craw, NJ * S.line * 4) || need(c->t_amask, (size_t)nwin * mwords * 8) ||
+              need(c->t_wspan, (size_t)nwin * 8))) {
+    set_err("tangent work buffer allocation failed");
+    return LBLX_E_HIP;
+  }
   c->stage_begin();
   HIPCHK(c->h2d(c->w_dlay.p, ys.data(), NJ * sizeof(DevLayer), s));
   HIPCHK(c->h2d(c->w_pan.p, pan.data(), pan.size() * sizeof(DevPanel), s));
@@ -1183,6 +1416,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   W.t9out = t9s + 3 * NJ * S.blk;
   W.t9bat = t9s + 4 * NJ * S.blk;
   W.t9k = t9s + 5 * NJ * S.blk;
+  W.craw = tan ? c->t_craw.as<int32_t>() : nullptr;
   Tape9 T9;
   T9.vnu = c->w_t9vnu.as<double>();
   T9.sp = c->w_t9sp.as<double>();
@@ -1246,12 +1480,18 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   }
   // 3. far-wing windows
   th = c->tbegin(ST_CONVF4);
-  if (split)
-    hipLaunchKernelGGL(k_convf4<float>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
-                       c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ);
-  else
-    hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
-                       c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ);
+  {
+    uint64_t *am = tan ? c->t_amask.as<uint64_t>() : nullptr;
+    int32_t *ws = tan ? c->t_wspan.as<int32_t>() : nullptr;
+    if (split)
+      hipLaunchKernelGGL(k_convf4<float>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
+                         c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ, am, mwords,
+                         ws);
+    else
+      hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
+                         c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ, am, mwords,
+                         ws);
+  }
   c->tend(th);
   Recs recs;
   {
@@ -1381,6 +1621,10 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     c->tend(tht);
   }
   HIPCHK(hipGetLastError());
+  if (tan) {
+    int rt = tangent_launch(c, run, J, *tan, recs, T9, W, mwords, f32out, od_base);
+    if (rt) return rt;
+  }
   if (v1p)
     for (int jb = 0; jb < njobs; jb++) v1p[jb] = plans[jb]->panels[0].vft + (double)NSHIFT * plans[jb]->dv;
   if (st) {
@@ -1408,7 +1652,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
 // OPDPTH for a batch of jobs (layers): every stage one launch for all jobs.
 static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int njobs, bool chain_jrad,
                     void *od_base, const int64_t *od_off, bool f32out, double *v1p, lblx_stats *st,
-                    const LayerPlan *pre = nullptr) {
+                    const LayerPlan *pre = nullptr, const TanReq *tan = nullptr) {
   c->hbegin();
   // the job tables live with the context, so their storage is reused
   if (!c->jprep) c->jprep = std::shared_ptr<void>(new JobsPrep(), [](void *p) { delete (JobsPrep *)p; });
@@ -1416,7 +1660,7 @@ static int run_jobs(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nj
   int rc = prep_jobs(c, run, layers, njobs, chain_jrad, od_off, pre, J);
   c->hend();
   if (rc) return rc;
-  return launch_jobs(c, run, J, od_base, f32out, v1p, st);
+  return launch_jobs(c, run, J, od_base, f32out, v1p, st, tan);
 }
 
 extern "C" {
@@ -2238,25 +2482,86 @@ int lblx_fftscn_noalloc_device(lblx_ctx *c, const lblx_scan *scan, const double 
   return LBLX_OK;
 }
 
+// The tangent request of a batch of layers whose input JRAD is set
+// (chained up the column): channel 0 the temperature, channel q the
+// molecule species[q - 1]; derivative (l, q) at odc + odc_off[l P + q].
+static int tan_request(const lblx_run *run, const lblx_layer *lay, int nlay, const int *species, int nspec,
+                       void *odc, const int64_t *odc_off, TanReq &R) {
+  const int P = 1 + nspec;
+  if (nspec < 0 || nspec > 7 || (nspec > 0 && !species)) return LBLX_E_ARG;
+  for (int q = 1; q < P; q++) {
+    if (species[q - 1] < 1 || species[q - 1] > 7) return LBLX_E_ARG;
+    for (int k = 0; k < q - 1; k++)
+      if (species[k] == species[q - 1]) return LBLX_E_ARG;
+  }
+  R.P = P;
+  R.odc = odc;
+  R.tl.assign(nlay, TanLayer{});
+  R.odc_off.assign(odc_off, odc_off + (size_t)nlay * P);
+  const double hT = 0.1, hq = 1e-3;  // steps of the continuum's central differences
+  R.rh[0] = 1. / (2. * hT);
+  for (int q = 1; q < P; q++) R.rh[q] = 1. / (2. * hq);
+  R.cps.clear();
+  if (run->icntnm != 0) R.cps.resize((size_t)2 * nlay * P);
+  for (int l = 0; l < nlay; l++) {
+    TanLayer &T = R.tl[l];
+    const lblx_layer &y = lay[l];
+    for (int q = 1; q < P; q++)
+      if (species[q - 1] > y.nmol) return LBLX_E_ARG;
+    T.tave = y.tave;
+    tips_dlnscor(T.tave, 0.01, T.dlnscor);
+    double wtot = y.wbroad;
+    for (int m = 0; m < y.nmol; m++) wtot += y.wk[m];
+    for (int m = 0; m < 7; m++) T.chan[m] = 0;
+    for (int q = 0; q < TAN_NQ; q++) T.wq[q] = 0.;
+    for (int q = 1; q < P; q++) {
+      const int m = species[q - 1] - 1;
+      T.chan[m] = q;
+      T.wq[q] = wtot > 0. ? y.wk[m] / wtot : 0.;
+    }
+    T.nchan = P;
+    T.nint = P > 1 ? P + 1 : P;
+    if (run->icntnm == 0) continue;
+    const int jrad = layer_jrad(&y);  // the JRAD the layer's own plan takes
+    ContPlan base;
+    int rc = plan_cont_layer(run, &y, jrad, &base);
+    if (rc) return rc;
+    for (int q = 0; q < P; q++)
+      for (int sg = 0; sg < 2; sg++) {
+        lblx_layer t = y;
+        const double h = sg == 0 ? 1. : -1.;
+        if (q == 0) t.tave = t.tave + h * hT;
+        else t.wk[species[q - 1] - 1] *= std::exp(h * hq);
+        ContPlan &cp = R.cps[((size_t)l * P + q) * 2 + sg];
+        rc = plan_cont_layer(run, &t, jrad, &cp);
+        if (rc) return rc;
+        if (cp.nptabs != base.nptabs || cp.v1abs != base.v1abs) return LBLX_E_UNSUPPORTED;
+      }
+  }
+  return LBLX_OK;
+}
+
 // Jacobians of the TOA radiance with respect to each layer's temperature
-// and the log amounts of chosen molecules.  Each layer's optical-depth
-// derivative is a forward difference of OPDPTH itself (all perturbed
-// layers in one batch with the base column); the radiative part is exact:
-// dR_l/dx on the layer grid, carried to the top through every layer above
-// as J <- T_k L4(J).
-// Jacobians of the TOA radiance.  aj = 0: forward differences of OPDPTH
+// and the log amounts of chosen molecules: each layer's optical-depth
+// derivative, then the exact radiative part, dR_l/dx on the layer grid
+// carried to the top through every layer above as J <- T_k L4(J).
+// The optical-depth derivatives: aj = 0: forward differences of OPDPTH
 // (T + dT, q e^dlnq).  aj = 1: the reference's analytic-Jacobian runs
 // (lblrtm.f90:2930-2975): T + 1 K, and for molecule m a single-species run
 // (every other amount zero, O2's amount moved to the broadener, CONTNM with
-// ICFLG = m) whose optical depth is dtau/dln q itself.  The radiative part
-// (sources and merge tangents) is the same exact tangent in both.
+// ICFLG = m) whose optical depth is dtau/dln q itself.  aj = 2: the
+// tangent-linear optical depths of lblx_tangent.hip, computed beside the
+// column's own OPDPTH run.  The radiative part (sources and merge tangents)
+// is the same exact tangent in all three.
 static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay, const double *tzu,
                         double tbnd, double emis, const int *species, int nspec, double dT, double dlnq,
                         int aj, double *d_rad, double *d_jac, int64_t n_out, double *v1p_top) {
   if (!c || !run || !layers || nlay <= 0 || !tzu || !d_rad || !d_jac) return LBLX_E_ARG;
   if (nspec < 0 || nspec > 7 || (nspec > 0 && !species) || !(dT > 0.) || !(dlnq > 0.)) return LBLX_E_ARG;
+  if (aj < 0 || aj > 2) return LBLX_E_ARG;
   HIPCHK(hipSetDevice(c->device));
   const int P = 1 + nspec;
+  const bool tl = aj == 2;
   std::vector<LayerPlan> plans(nlay);
   std::vector<lblx_layer> jobs;
   std::vector<int> icflg;
@@ -2277,6 +2582,10 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
     for (int q = 0; q < P; q++) {
       lblx_layer t = jobs[l];
       int flg = 0;
+      if (q > 0 && (species[q - 1] < 1 || species[q - 1] > t.nmol)) return LBLX_E_ARG;
+      for (int k = 0; tl && k < q - 1; k++)  // one channel per molecule
+        if (species[k] == species[q - 1]) return LBLX_E_ARG;
+      if (tl) continue;  // the column's own run carries the tangents
       if (q == 0) {
         t.tave = t.tave + dT;
       } else {
@@ -2308,9 +2617,10 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
     for (int j = 0; j < nj; j++)
       if (prc[j]) return prc[j];
   }
-  std::vector<int64_t> off(nj);
+  const int nplane = nlay * (1 + P);  // the column's layers, then (layer, parameter)
+  std::vector<int64_t> off(nplane);
   int64_t tot = 0, nmax = 0, tot_base = 0;
-  for (int j = 0; j < nj; j++) {
+  for (int j = 0; j < nplane; j++) {
     const LayerPlan &K = plans[j < nlay ? j : (j - nlay) / P];
     off[j] = tot;
     tot += K.npts;
@@ -2340,9 +2650,15 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
       c->w_mpan.ensure(pan.size() * sizeof(DevPanel)) || c->j_rall.ensure((size_t)tot_base * 8) ||
       c->j_a.ensure(jbytes) || c->j_b.ensure(jbytes))
     return LBLX_E_HIP;
-  // 1. OPDPTH for the column and every perturbed layer, one batch
+  // 1. OPDPTH for the column and every perturbed layer, one batch (aj = 2:
+  // the column alone, with its tangent-linear optical depths)
+  TanReq req;
+  if (tl) {
+    int rq = tan_request(run, jobs.data(), nlay, species, nspec, c->w_od.p, off.data() + nlay, req);
+    if (rq) return rq;
+  }
   int rc = run_jobs(c, run, jobs.data(), nj, false, c->w_od.p, off.data(), f32, nullptr, nullptr,
-                    jplans.data());
+                    jplans.data(), tl ? &req : nullptr);
   if (rc) return rc;
   hipStream_t s = c->stream;
   c->stage_begin();
@@ -2411,8 +2727,8 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
     S.stride = stride;
     S.nparam = P;
     for (int q = 0; q < P; q++) {
-      S.direct[q] = (aj && q > 0) ? 1 : 0;
-      S.rh[q] = q == 0 ? 1.0 / dT : (aj ? 1.0 : 1.0 / dlnq);
+      S.direct[q] = (tl || (aj && q > 0)) ? 1 : 0;
+      S.rh[q] = tl ? 1.0 : (q == 0 ? 1.0 / dT : (aj ? 1.0 : 1.0 / dlnq));
       S.odp[q] = off[nlay + l * P + q] - off[l];  // relative to the layer's own optical depth
     }
     const double *rold = l > 0 ? rall + off[l - 1] : nullptr;
@@ -2523,6 +2839,38 @@ int lblx_jacobian_aj_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers
                       n_out, v1p_top);
 }
 
+int lblx_od_tangent_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay, const int *species,
+                           int nspec, void *d_od, void *d_dod, int64_t n_out) {
+  if (!c || !run || !layers || nlay <= 0 || !d_od || !d_dod || n_out <= 0) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  const int P = 1 + nspec;
+  for (int l = 1; l < nlay; l++) layers[l].jrad = layer_jrad(&layers[l - 1]);
+  for (int l = 0; l < nlay; l++) {
+    int64_t n = 0;
+    int rc = lblx_od_size(run, &layers[l], &n);
+    if (rc) return rc;
+    if (n > n_out) return LBLX_E_SIZE;
+  }
+  std::vector<int64_t> off(nlay), doff((size_t)nlay * P);
+  for (int l = 0; l < nlay; l++) {
+    off[l] = (int64_t)l * n_out;
+    for (int q = 0; q < P; q++) doff[(size_t)l * P + q] = ((int64_t)l * P + q) * n_out;
+  }
+  TanReq req;
+  int rc = tan_request(run, layers, nlay, species, nspec, d_dod, doff.data(), req);
+  if (rc) return rc;
+  const bool f32 = run->precision == LBLX_FP32_SPLIT;
+  // JRAD is already chained above: each layer plans from its own input flag
+  return run_jobs(c, run, layers, nlay, false, d_od, off.data(), f32, nullptr, nullptr, nullptr, &req);
+}
+
+int lblx_jacobian_tl_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
+                            const double *tzu, double tbnd, double emis, const int *species, int nspec,
+                            double *d_rad, double *d_jac, int64_t n_out, double *v1p_top) {
+  return jacobian_run(c, run, layers, nlay, tzu, tbnd, emis, species, nspec, 1.0, 1.0, 2, d_rad, d_jac,
+                      n_out, v1p_top);
+}
+
 int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const int *nlay,
                                 lblx_layer *layers, const double *tzu, const double *tbnd,
                                 const double *emis, const int *species, int nspec, int aj, double dT,
@@ -2531,6 +2879,7 @@ int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, con
   if (!c || !run || nprof <= 0 || !nlay || !layers || !tzu || !tbnd || !emis || !d_rad || !rad_off || !d_jac ||
       !jac_off)
     return LBLX_E_ARG;
+  if (aj < 0 || aj > 2) return LBLX_E_ARG;
   if (aj) dT = dlnq = 1.0;
   int64_t l0 = 0;
   for (int p = 0; p < nprof; p++) {
@@ -2539,7 +2888,7 @@ int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, con
     int rc = lblx_od_size(run, &layers[l0 + nlay[p] - 1], &n_out);
     if (rc) return rc;
     rc = jacobian_run(c, run, layers + l0, nlay[p], tzu + l0, tbnd[p], emis[p], species, nspec, dT, dlnq,
-                      aj ? 1 : 0, d_rad + rad_off[p], d_jac + jac_off[p], n_out,
+                      aj, d_rad + rad_off[p], d_jac + jac_off[p], n_out,
                       v1p_top ? v1p_top + p : nullptr);
     if (rc) return rc;
     l0 += nlay[p];

@@ -89,6 +89,7 @@ struct ShrinkWork {
   // (-1: none, -2: not tabled), the lines it emits, its batches; and for
   // the starts on the job's chain: output offset, batch offset, index (-1: not a start)
   int32_t *t9nxt, *t9nem, *t9nbt, *t9out, *t9bat, *t9k;
+  int32_t *craw;          // [line] TAPE3 record of each compacted line (null: not kept)
 };
 
 // Shrunk far-wing line list (TAPE9) and its block table, per job.

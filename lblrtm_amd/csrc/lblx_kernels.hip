@@ -252,6 +252,7 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
     W.cco2[k] = (int8_t)o.co2;
     W.pn[k] = W.bpn[ob] + posn;
     W.pc[k] = W.bpc[ob] + posc;
+    if (W.craw) W.craw[k] = i;
   }
 }
 
@@ -780,7 +781,8 @@ __device__ __forceinline__ T f4_at(const F4Par<T> &p, double xmd) {
 template <typename T>
 __global__ __launch_bounds__(CF4_T) __attribute__((amdgpu_waves_per_eu(5)))
 void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const DevSeg *segs,
-              const int32_t *worder, double *r4out, unsigned long long *k3upd) {
+              const int32_t *worder, double *r4out, unsigned long long *k3upd, uint64_t *amask,
+              int64_t mwords, int32_t *wspan) {
   __shared__ T r4[R4ROW];
   __shared__ F4Par<T> par[2][64];
   __shared__ int s_nacc2[2], s_done2[2], s_b;
@@ -991,6 +993,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       const uint64_t past_m = __ballot(vnui >= vnulst);
       int nacc = 0;
       int done = 0;
+      uint64_t am = 0ull;  // accepted lines with points in the window
       while (todo) {
         const int j = __builtin_ctzll(todo);
         todo &= todo - 1;
@@ -1020,7 +1023,10 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
           q.jmax = __builtin_amdgcn_readlane(p.jmax, j);
           if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me - q.xnui);
           if (lane == j && p.jmax >= p.jmin) par[nb][nacc] = p;
-          if (q.jmax >= q.jmin) nacc++;
+          if (q.jmax >= q.jmin) {
+            nacc++;
+            am |= 1ull << j;
+          }
         }
       }
       if (!done) {  // lanes after the last undecided one: accepted in bulk
@@ -1035,10 +1041,13 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         const uint64_t below_me = lane == 0 ? 0ull : (addm & ((1ull << lane) - 1));
         if ((addm >> lane) & 1) par[nb][nacc + __builtin_popcountll(below_me)] = p;
         nacc += __builtin_popcountll(addm);
+        am |= addm;
       }
       if (lane == 0) {
         s_nacc2[nb] = nacc;
         s_done2[nb] = done;
+        // the tangent pass reads the accepted lines back (chunk index < nl / 64)
+        if (amask) amask[(size_t)wi * mwords + (npos - pos) / 64] = am;
       }
       // prefetch the next chunk while the group scatters this one
       const int in = npos + 64 + lane;
@@ -1057,6 +1066,10 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
     npos += 64;
   }
   if (tid == 64 && nupd) atomicAdd(&k3upd[job], nupd);
+  if (wspan && tid == 0) {
+    wspan[2 * wi] = pos;
+    wspan[2 * wi + 1] = (npos - pos) / 64;
+  }
   __syncthreads();
   // JRAD = 1: radiation term as a piecewise-linear factor (oprop.f90:3733-3758)
   if (W.nseg > 0) {

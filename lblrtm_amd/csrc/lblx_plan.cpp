@@ -115,6 +115,15 @@ void tips_scor(double temp, double scor[7][10]) {
     }
 }
 
+void tips_dlnscor(double temp, double h, double out[7][10]) {
+  double sp[7][10], sm[7][10];
+  tips_scor(temp + h, sp);
+  tips_scor(temp - h, sm);
+  for (int m = 0; m < 7; m++)
+    for (int iso = 0; iso < 10; iso++)
+      out[m][iso] = (sp[m][iso] > 0. && sm[m][iso] > 0.) ? std::log(sp[m][iso] / sm[m][iso]) / (2. * h) : 0.;
+}
+
 int plan_dv(const lblx_run *run, int nlay, int nmol, const double *pave,
             const double *tave, const double *wk, const double *wbroad,
             int iemit, double *dv_out, int32_t *itype_out) {
@@ -329,6 +338,10 @@ static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPl
     C->ray_f = WTOT * conv;
   }
   return LBLX_OK;
+}
+
+int plan_cont_layer(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPlan *C) {
+  return plan_cont(run, Ly, jrad, C, 0);
 }
 
 // JRAD after OPDPTH's test (lblrtm.f90:6202-6204): the radiation term is

@@ -151,5 +151,9 @@ double radfn(double vi, double xkt);
 double radfni(double vi, double dvi, double xkt, double *vinew, double *rdel,
               double *rdlast);
 void tips_scor(double temp, double scor[7][10]);
+// d ln SCOR/dT: central difference of the TIPS interpolation over +-h
+void tips_dlnscor(double temp, double h, double out[7][10]);
+// CONTNM's plan alone for a layer whose JRAD is already set
+int plan_cont_layer(const lblx_run *run, const lblx_layer *layer, int jrad, ContPlan *C);
 
 }  // namespace lblx

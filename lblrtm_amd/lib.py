@@ -16,7 +16,7 @@ from . import abi
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "liblblx.so")
 _LIB = None
-NSTAGE = 13
+NSTAGE = 14
 
 
 class LblxError(RuntimeError):
@@ -96,6 +96,11 @@ def load():
         "lblx_jacobian_aj_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_double),
                                               C.c_double, C.c_double, P(C.c_int), C.c_int, C.c_void_p,
                                               C.c_void_p, C.c_int64, P(C.c_double)]),
+        "lblx_jacobian_tl_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_double),
+                                              C.c_double, C.c_double, P(C.c_int), C.c_int, C.c_void_p,
+                                              C.c_void_p, C.c_int64, P(C.c_double)]),
+        "lblx_od_tangent_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int), C.c_int,
+                                             C.c_void_p, C.c_void_p, C.c_int64]),
         "lblx_od_dvout_size": (C.c_int, [P(abi.Run), P(abi.Layer), C.c_double, P(C.c_int64)]),
         "lblx_od_layer_dvout_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_double, C.c_void_p,
                                                  C.c_int64, P(C.c_int64), P(C.c_double)]),
@@ -364,15 +369,57 @@ class Context:
             layers[k].jrad = arr[k].jrad
         return v1p.value
 
-    def jacobian(self, run, layers, tzu, tbnd, emis, species, dT=1e-3, dlnq=1e-4, aj=False):
+    def jacobian_tl_device(self, run, layers, tzu, tbnd, emis, species, d_rad, d_jac, n_out):
+        """Jacobians from the tangent-linear optical depths (lblx_jacobian_tl_device)."""
+        n = len(layers)
+        arr = (abi.Layer * n)(*layers)
+        tz = np.ascontiguousarray(tzu, np.float64)
+        sp = (C.c_int * max(1, len(species)))(*species)
+        v1p = C.c_double(0.0)
+        _chk(self._lib.lblx_jacobian_tl_device(self._h, C.byref(run), arr, n, abi.ptr(tz, C.c_double), tbnd,
+                                               emis, sp, len(species), C.c_void_p(d_rad), C.c_void_p(d_jac),
+                                               n_out, C.byref(v1p)), "lblx_jacobian_tl_device")
+        for k in range(n):
+            layers[k].jrad = arr[k].jrad
+        return v1p.value
+
+    def od_tangent(self, run, layers, species):
+        """Layer optical depths and their tangent-linear derivatives
+        (lblx_od_tangent_device): (od[layer, point], dod[layer, q, point],
+        sizes); each layer's values fill its first sizes[l] points."""
+        n = max(od_size(run, x) for x in layers)
+        P = 1 + len(species)
+        arr = (abi.Layer * len(layers))(*layers)
+        sp = (C.c_int * max(1, len(species)))(*species)
+        esz = 4 if run.precision == abi.FP32_SPLIT else 8
+        dt = np.float32 if esz == 4 else np.float64
+        od = self.alloc(esz * n * len(layers))
+        dod = self.alloc(esz * n * len(layers) * P)
+        try:
+            _chk(self._lib.lblx_od_tangent_device(self._h, C.byref(run), arr, len(layers), sp, len(species),
+                                                  C.c_void_p(od.ptr), C.c_void_p(dod.ptr), n),
+                 "lblx_od_tangent_device")
+            for k in range(len(layers)):
+                layers[k].jrad = arr[k].jrad
+            sizes = [od_size(run, x) for x in layers]
+            return (od.to_host(dt, n * len(layers)).reshape(len(layers), n),
+                    dod.to_host(dt, n * len(layers) * P).reshape(len(layers), P, n), sizes)
+        finally:
+            od.free()
+            dod.free()
+
+    def jacobian(self, run, layers, tzu, tbnd, emis, species, dT=1e-3, dlnq=1e-4, aj=False, tl=False):
         """Host-side convenience: (radiance, jac[q, layer, point], v1p, dv);
-        aj selects the reference's analytic-Jacobian formulation."""
+        aj selects the reference's analytic-Jacobian formulation, tl the
+        tangent-linear optical depths."""
         n = od_size(run, layers[-1])
         P = 1 + len(species)
         rad = self.alloc(8 * n)
         jac = self.alloc(8 * n * P * len(layers))
         try:
-            if aj:
+            if tl:
+                v1p = self.jacobian_tl_device(run, layers, tzu, tbnd, emis, species, rad.ptr, jac.ptr, n)
+            elif aj:
                 v1p = self.jacobian_aj_device(run, layers, tzu, tbnd, emis, species, rad.ptr, jac.ptr, n)
             else:
                 v1p = self.jacobian_device(run, layers, tzu, tbnd, emis, species, dT, dlnq, rad.ptr, jac.ptr, n)
@@ -435,10 +482,11 @@ class Context:
         return v1p
 
     def jacobians_batch_device(self, run, profiles, species, d_rad, rad_off, d_jac, jac_off, aj=False,
-                               dT=1e-3, dlnq=1e-4):
+                               dT=1e-3, dlnq=1e-4, tl=False):
         """Forward and adjoint of a batch of profiles (lblx_jacobians_batch_device):
         profiles is a list of (layers, tzu, tbnd, emis); returns v1p of each
-        profile's top grid."""
+        profile's top grid.  aj: the reference's analytic-Jacobian runs, tl:
+        the tangent-linear optical depths."""
         nl = [len(p[0]) for p in profiles]
         lays = [abi.Layer.from_buffer_copy(x) for p in profiles for x in p[0]]
         arr = (abi.Layer * len(lays))(*lays)
@@ -452,7 +500,8 @@ class Context:
         nla = (C.c_int * len(nl))(*nl)
         _chk(self._lib.lblx_jacobians_batch_device(self._h, C.byref(run), len(profiles), nla, arr,
                                                    abi.ptr(tz, C.c_double), abi.ptr(tb, C.c_double),
-                                                   abi.ptr(em, C.c_double), sp, len(species), 1 if aj else 0,
+                                                   abi.ptr(em, C.c_double), sp, len(species),
+                                                   2 if tl else (1 if aj else 0),
                                                    dT, dlnq, C.c_void_p(d_rad), abi.ptr(ro, C.c_int64),
                                                    C.c_void_p(d_jac), abi.ptr(jo, C.c_int64),
                                                    abi.ptr(v1p, C.c_double)),
