@@ -304,6 +304,27 @@ int lblx_jacobian_tl_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *laye
 int lblx_od_tangent_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay,
                            const int *species, int nspec, void *d_od, void *d_dod, int64_t n_out);
 
+/* Reverse mode for a batch of profiles (the layout of
+ * lblx_jacobians_batch_device): given dL/dR on each profile's top grid
+ * (d_dLdR + rad_off[p]), the gradient of L with respect to every layer's
+ * temperature and the log amounts of the nspec molecules, profile p's at
+ * d_grad + grad_off[p] as (1 + nspec) x nlay[p] values, [q * nlay[p] + l]
+ * (q = 0 temperature, per K; q >= 1 species q, per unit ln amount).  The
+ * layer optical depths and their tangent-linear derivatives are those of
+ * lblx_jacobian_tl_device; the radiative part runs backwards: lam = dL/dR
+ * from the top, each layer's parameters take sum(lam dR_l/dx), and lam
+ * moves to the layer below through the transpose of the merge (T_l and
+ * the carry-over interpolation of XMERGE, xmerge.f90:2788-2796).  Memory
+ * and work grow with layers x points, not with layers^2 x parameters as
+ * the Jacobian planes do; equal to dL/dR contracted with those planes.
+ * d_rad (may be null): the TOA radiance.  No counterpart in the reference,
+ * whose retrievals contract the AJ Jacobian files on the host.           */
+int lblx_backward_batch(lblx_ctx *ctx, const lblx_run *run, int nprof, const int *nlay,
+                        lblx_layer *layers, const double *tzu, const double *tbnd,
+                        const double *emis, const int *species, int nspec,
+                        const double *d_dLdR, const int64_t *rad_off, double *d_rad,
+                        double *d_grad, const int64_t *grad_off, double *v1p_top);
+
 /* A batch of independent profiles (ragged: nlay[p] layers each, surface
  * first, concatenated in layers/tzu; each profile's own band in its
  * layers' v1/v2).  Layer optical depths run in batches of whole profiles

@@ -2553,10 +2553,15 @@ static int tan_request(const lblx_run *run, const lblx_layer *lay, int nlay, con
 // tangent-linear optical depths of lblx_tangent.hip, computed beside the
 // column's own OPDPTH run.  The radiative part (sources and merge tangents)
 // is the same exact tangent in all three.
+// d_dLdR given (with aj = 2): reverse mode instead of the Jacobian planes:
+// d_grad[q nlay + l] = sum over the top grid of dL/dR dR/dx_(q,l).
 static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay, const double *tzu,
                         double tbnd, double emis, const int *species, int nspec, double dT, double dlnq,
-                        int aj, double *d_rad, double *d_jac, int64_t n_out, double *v1p_top) {
-  if (!c || !run || !layers || nlay <= 0 || !tzu || !d_rad || !d_jac) return LBLX_E_ARG;
+                        int aj, double *d_rad, double *d_jac, int64_t n_out, double *v1p_top,
+                        const double *d_dLdR = nullptr, double *d_grad = nullptr) {
+  const bool adj = d_dLdR != nullptr;
+  if (!c || !run || !layers || nlay <= 0 || !tzu || (!adj && (!d_rad || !d_jac)) || (adj && (!d_grad || aj != 2)))
+    return LBLX_E_ARG;
   if (nspec < 0 || nspec > 7 || (nspec > 0 && !species) || !(dT > 0.) || !(dlnq > 0.)) return LBLX_E_ARG;
   if (aj < 0 || aj > 2) return LBLX_E_ARG;
   HIPCHK(hipSetDevice(c->device));
@@ -2645,10 +2650,14 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
   const bool f32 = run->precision == LBLX_FP32_SPLIT;
   const size_t esz = f32 ? 4 : 8;
   const int64_t plane = nmax;
-  const size_t jbytes = (size_t)P * nlay * plane * 4;  // FP32 tangent planes
+  // FP32 tangent planes (forward mode); reverse mode: two adjoint rows and
+  // the blocks' partial sums
+  const int64_t nblk_max = (nmax + ADJ_T - 1) / ADJ_T;
+  const size_t jbytes = adj ? (size_t)2 * nmax * 8 : (size_t)P * nlay * plane * 4;
+  const size_t jbytes_b = adj ? (size_t)nblk_max * P * 8 : jbytes;
   if (c->w_od.ensure((size_t)tot * esz) || c->w_rt.ensure((size_t)nmax * 8 * 4) ||
       c->w_mpan.ensure(pan.size() * sizeof(DevPanel)) || c->j_rall.ensure((size_t)tot_base * 8) ||
-      c->j_a.ensure(jbytes) || c->j_b.ensure(jbytes))
+      c->j_a.ensure(jbytes) || c->j_b.ensure(jbytes_b))
     return LBLX_E_HIP;
   // 1. OPDPTH for the column and every perturbed layer, one batch (aj = 2:
   // the column alone, with its tangent-linear optical depths)
@@ -2702,7 +2711,56 @@ static int jacobian_run(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, in
     c->tend(thm);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipMemcpyAsync(d_rad, rall + off[nlay - 1], (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
+  if (d_rad) HIPCHK(hipMemcpyAsync(d_rad, rall + off[nlay - 1], (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
+  if (adj) {
+    // 3'. reverse mode: lam from the top down; each layer's parameters take
+    // the sum of lam times their source terms, and lam moves to the layer
+    // below through the transposed merge
+    int tha = c->tbegin(ST_MERGE);
+    double *lam[2] = {c->j_a.as<double>(), c->j_a.as<double>() + nmax};
+    double *part = c->j_b.as<double>();
+    HIPCHK(hipMemcpyAsync(lam[0], d_dLdR, (size_t)ntop * 8, hipMemcpyDeviceToDevice, s));
+    int cur = 0;
+    for (int l = nlay - 1; l >= 0; l--) {
+      MergeArgs A = merge_args(plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
+                               l == 0 ? tbnd : 0., emis);
+      JacSrc S{};
+      S.A = A;
+      S.tave = layers[l].tave;
+      S.nparam = P;
+      for (int q = 0; q < P; q++) {
+        S.direct[q] = 1;
+        S.rh[q] = 1.0;
+        S.odp[q] = off[nlay + l * P + q] - off[l];
+      }
+      const void *od = (const char *)c->w_od.p + (size_t)off[l] * esz;
+      const double *rold = l > 0 ? rall + off[l - 1] : nullptr;
+      const int nb = (int)((A.n + ADJ_T - 1) / ADJ_T);
+      if (f32)
+        hipLaunchKernelGGL(k_adj_src<float>, dim3(nb), dim3(ADJ_T), 0, s, S, dpan + pan0[l], (const float *)od,
+                           (const float *)od, rold, (const double *)lam[cur], part);
+      else
+        hipLaunchKernelGGL(k_adj_src<double>, dim3(nb), dim3(ADJ_T), 0, s, S, dpan + pan0[l], (const double *)od,
+                           (const double *)od, rold, (const double *)lam[cur], part);
+      hipLaunchKernelGGL(k_adj_sum, dim3(P), dim3(ADJ_T), 0, s, (const double *)part, nb, P, d_grad + l,
+                         (int64_t)nlay);
+      if (l == 0) break;
+      const int64_t nold = plans[l - 1].npts;
+      hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, s, lam[cur ^ 1], nold, 0.0);
+      const dim3 g((unsigned)((A.n + 255) / 256));
+      if (f32)
+        hipLaunchKernelGGL(k_adj_prop<float>, g, dim3(256), 0, s, A, dpan + pan0[l], (const float *)od,
+                           (const double *)lam[cur], lam[cur ^ 1]);
+      else
+        hipLaunchKernelGGL(k_adj_prop<double>, g, dim3(256), 0, s, A, dpan + pan0[l], (const double *)od,
+                           (const double *)lam[cur], lam[cur ^ 1]);
+      cur ^= 1;
+    }
+    c->tend(tha);
+    HIPCHK(hipGetLastError());
+    if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
+    return LBLX_OK;
+  }
   // 3. sources and tangent propagation.  The layers at the top whose merge
   // is the identity map (ITYPE 0, same panels: J <- T_k J pointwise) are
   // handled from the top down with a running transmittance product (K0:
@@ -2869,6 +2927,28 @@ int lblx_jacobian_tl_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers
                             double *d_rad, double *d_jac, int64_t n_out, double *v1p_top) {
   return jacobian_run(c, run, layers, nlay, tzu, tbnd, emis, species, nspec, 1.0, 1.0, 2, d_rad, d_jac,
                       n_out, v1p_top);
+}
+
+int lblx_backward_batch(lblx_ctx *c, const lblx_run *run, int nprof, const int *nlay, lblx_layer *layers,
+                        const double *tzu, const double *tbnd, const double *emis, const int *species, int nspec,
+                        const double *d_dLdR, const int64_t *rad_off, double *d_rad, double *d_grad,
+                        const int64_t *grad_off, double *v1p_top) {
+  if (!c || !run || nprof <= 0 || !nlay || !layers || !tzu || !tbnd || !emis || !d_dLdR || !rad_off || !d_grad ||
+      !grad_off)
+    return LBLX_E_ARG;
+  int64_t l0 = 0;
+  for (int p = 0; p < nprof; p++) {
+    if (nlay[p] <= 0) return LBLX_E_ARG;
+    int64_t n_out = 0;
+    int rc = lblx_od_size(run, &layers[l0 + nlay[p] - 1], &n_out);
+    if (rc) return rc;
+    rc = jacobian_run(c, run, layers + l0, nlay[p], tzu + l0, tbnd[p], emis[p], species, nspec, 1.0, 1.0, 2,
+                      d_rad ? d_rad + rad_off[p] : nullptr, nullptr, n_out, v1p_top ? v1p_top + p : nullptr,
+                      d_dLdR + rad_off[p], d_grad + grad_off[p]);
+    if (rc) return rc;
+    l0 += nlay[p];
+  }
+  return LBLX_OK;
 }
 
 int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, const int *nlay,

@@ -1990,15 +1990,12 @@ struct JacSrc {
   int32_t direct[8];     // the perturbed run's optical depth is the derivative itself
 };
 
-// Tangent planes are stored in FP32 (JP = float) between merges and in the
-// caller's FP64 (JP = double) where they are written out; the arithmetic is
-// FP64 throughout.
-template <typename IN, typename JP>
-__global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN *odall,
-                          const double *rold, JP *jout, const double *scale) {
+// The source terms of layer point g: D = dR_l/dtau_l and the
+// temperature's Planck term (1 - T_l)(1 - F) dB/dT, with T_l.
+template <typename IN>
+__device__ __forceinline__ void jac_src_terms(const JacSrc &S, const DevPanel *pan, const IN *od, const double *rold,
+                                              int64_t g, double *D_out, double *bt_out, double *tr_out) {
   const MergeArgs &A = S.A;
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= A.n) return;
   const int p = (int)(g / 2400);
   const int k = (int)(g - (int64_t)2400 * p);
   const double vi = (pan[p].vft + 32. * A.dv) + (double)k * A.dv;
@@ -2036,14 +2033,99 @@ __global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN 
   if (clamped) D = 0.;
   // dB/dT of PLANCK at Tave (zero where PLANCK is cut off)
   const double dbdt = (x >= 1e-2 && x <= 80.) ? bb * (x * rcp_nr(S.tave)) * rcp_nr(1. - ex) : 0.;
+  *D_out = D;
+  *bt_out = (1. - tr) * (1. - f) * dbdt;
+  *tr_out = tr;
+}
+
+// Tangent planes are stored in FP32 (JP = float) between merges and in the
+// caller's FP64 (JP = double) where they are written out; the arithmetic is
+// FP64 throughout.
+template <typename IN, typename JP>
+__global__ void k_jac_src(JacSrc S, const DevPanel *pan, const IN *od, const IN *odall,
+                          const double *rold, JP *jout, const double *scale) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= S.A.n) return;
+  double D, bt, tr;
+  jac_src_terms(S, pan, od, rold, g, &D, &bt, &tr);
   for (int q = 0; q < S.nparam; q++) {
     double dtau = S.direct[q] ? (double)odall[S.odp[q] + g] * S.rh[q]
                               : ((double)odall[S.odp[q] + g] - (double)od[g]) * S.rh[q];
     double v = D * dtau;
-    if (q == 0) v = v + (1. - tr) * (1. - f) * dbdt;
+    if (q == 0) v = v + bt;
     if (scale) v = v * scale[g];
     jout[(int64_t)q * S.stride + g] = (JP)v;
   }
+}
+
+// Reverse mode (vector-Jacobian products).  With R_l = T_l L4(R_{l-1}) +
+// E_l and lam_l = dL/dR_l, layer l's parameter x gets sum_g lam_l(g)
+// dR_l(g)/dx (the source terms above), and lam_{l-1} = L4^T (T_l lam_l),
+// the transpose of the carry-over interpolation.
+//
+// k_adj_src: each block's partial sums of lam (D dtau_q + [q = 0] bt) over
+// its 256 points, in a fixed order: part[block P + q].
+constexpr int ADJ_T = 256;
+template <typename IN>
+__global__ __launch_bounds__(ADJ_T) void k_adj_src(JacSrc S, const DevPanel *pan, const IN *od, const IN *odall,
+                                                   const double *rold, const double *lam, double *part) {
+  __shared__ double red[ADJ_T];
+  const int64_t g = (int64_t)blockIdx.x * ADJ_T + threadIdx.x;
+  double D = 0., bt = 0., tr = 0., l = 0.;
+  if (g < S.A.n) {
+    jac_src_terms(S, pan, od, rold, g, &D, &bt, &tr);
+    l = lam[g];
+  }
+  for (int q = 0; q < S.nparam; q++) {
+    double v = 0.;
+    if (g < S.A.n) {
+      const double dtau = (double)odall[S.odp[q] + g] * S.rh[q];
+      v = l * (D * dtau + (q == 0 ? bt : 0.));
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = ADJ_T / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * S.nparam + q] = red[0];
+    __syncthreads();
+  }
+}
+
+// grad[q gstride] (+)= the sum of the blocks' partials of parameter q
+// (blockIdx.x = q), in a fixed order.
+__global__ __launch_bounds__(ADJ_T) void k_adj_sum(const double *part, int nblk, int P, double *grad, int64_t gstride) {
+  __shared__ double red[ADJ_T];
+  const int q = blockIdx.x;
+  double v = 0.;
+  for (int b = threadIdx.x; b < nblk; b += ADJ_T) v += part[(int64_t)b * P + q];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = ADJ_T / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) grad[(int64_t)q * gstride] = red[0];
+}
+
+// lam_old += L4^T (T_l lam): every point of the new grid adds its four
+// weighted taps (lam_old cleared first).
+template <typename IN>
+__global__ void k_adj_prop(MergeArgs A, const DevPanel *pan, const IN *od, const double *lam, double *lam_old) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.n) return;
+  const int p = (int)(g / 2400);
+  const int k = (int)(g - (int64_t)2400 * p);
+  double odvi = (double)od[g];
+  if (odvi < -0.06) odvi = -0.06;
+  const double x = exp(-odvi) * lam[g];
+  if (x == 0.) return;
+  const MergeTap t = merge_tap(A, pan[p].out0, k);
+  atomicAdd(&lam_old[t.i0], t.w0 * x);
+  if (t.w1 != 0.) atomicAdd(&lam_old[t.i1], t.w1 * x);
+  if (t.w2 != 0.) atomicAdd(&lam_old[t.i2], t.w2 * x);
+  if (t.w3 != 0.) atomicAdd(&lam_old[t.i3], t.w3 * x);
 }
 
 // One layer of tangent propagation for nv vectors (plane stride istr on

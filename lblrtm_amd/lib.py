@@ -101,6 +101,10 @@ def load():
                                               C.c_void_p, C.c_int64, P(C.c_double)]),
         "lblx_od_tangent_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int), C.c_int,
                                              C.c_void_p, C.c_void_p, C.c_int64]),
+        "lblx_backward_batch": (C.c_int, [C.c_void_p, P(abi.Run), C.c_int, P(C.c_int), P(abi.Layer),
+                                          P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_int), C.c_int,
+                                          C.c_void_p, P(C.c_int64), C.c_void_p, C.c_void_p, P(C.c_int64),
+                                          P(C.c_double)]),
         "lblx_od_dvout_size": (C.c_int, [P(abi.Run), P(abi.Layer), C.c_double, P(C.c_int64)]),
         "lblx_od_layer_dvout_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_double, C.c_void_p,
                                                  C.c_int64, P(C.c_int64), P(C.c_double)]),
@@ -507,6 +511,49 @@ class Context:
                                                    abi.ptr(v1p, C.c_double)),
              "lblx_jacobians_batch_device")
         return v1p
+
+    def backward_batch_device(self, run, profiles, species, d_dLdR, rad_off, d_grad, grad_off, d_rad=None):
+        """Reverse mode for a batch of profiles (lblx_backward_batch):
+        profiles is a list of (layers, tzu, tbnd, emis); dL/dR of profile p
+        at d_dLdR + rad_off[p], its gradient [(1 + nspec) x nlay] at d_grad +
+        grad_off[p].  Returns v1p of each profile's top grid."""
+        nl = [len(p[0]) for p in profiles]
+        lays = [abi.Layer.from_buffer_copy(x) for p in profiles for x in p[0]]
+        arr = (abi.Layer * len(lays))(*lays)
+        tz = np.ascontiguousarray(np.concatenate([np.asarray(p[1], np.float64) for p in profiles]))
+        tb = np.ascontiguousarray([p[2] for p in profiles], np.float64)
+        em = np.ascontiguousarray([p[3] for p in profiles], np.float64)
+        ro = np.ascontiguousarray(rad_off, np.int64)
+        go = np.ascontiguousarray(grad_off, np.int64)
+        sp = (C.c_int * max(1, len(species)))(*species)
+        v1p = np.zeros(len(profiles))
+        nla = (C.c_int * len(nl))(*nl)
+        _chk(self._lib.lblx_backward_batch(self._h, C.byref(run), len(profiles), nla, arr, abi.ptr(tz, C.c_double),
+                                           abi.ptr(tb, C.c_double), abi.ptr(em, C.c_double), sp, len(species),
+                                           C.c_void_p(d_dLdR), abi.ptr(ro, C.c_int64),
+                                           C.c_void_p(d_rad) if d_rad else None, C.c_void_p(d_grad),
+                                           abi.ptr(go, C.c_int64), abi.ptr(v1p, C.c_double)),
+             "lblx_backward_batch")
+        return v1p
+
+    def backward(self, run, layers, tzu, tbnd, emis, species, dLdR):
+        """Host-side convenience for one profile: (radiance, grad[q, layer])."""
+        n = od_size(run, layers[-1])
+        P = 1 + len(species)
+        g = np.ascontiguousarray(dLdR, np.float64)
+        assert g.shape == (n,)
+        dl = self.alloc(8 * n)
+        rad = self.alloc(8 * n)
+        grad = self.alloc(8 * P * len(layers))
+        try:
+            dl.from_host(g)
+            self.backward_batch_device(run, [(layers, tzu, tbnd, emis)], species, dl.ptr, [0], grad.ptr, [0],
+                                       d_rad=rad.ptr)
+            return rad.to_host(np.float64, n), grad.to_host(np.float64, P * len(layers)).reshape(P, len(layers))
+        finally:
+            dl.free()
+            rad.free()
+            grad.free()
 
     def autotune(self, run, layers):
         """Time the near-wing tile widths on these layers; returns (R1, R2, R3)."""

@@ -206,3 +206,67 @@ def test_tl_rejects_bad_requests(gpu_ctx, lines):
     with pytest.raises(lib.LblxError) as e:
         gpu_ctx.od_tangent(vrun, _cp(layers[:2]), SPECIES)
     assert e.value.code == abi.E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("scale", [1.0, 1e-4], ids=["column", "thin-column"])
+def test_backward_equals_jacobian_contraction(gpu_ctx, lines, scale):
+    """lblx_backward_batch (reverse mode) against the tangent-linear Jacobian
+    planes contracted with the same seeded dL/dR: the same derivative taken
+    the other way round, equal to the FP32 rounding of the forward-mode
+    planes (1e-5 of the largest gradient).  The thin column (amounts x 1e-4,
+    molecules 1-6) is transparent much deeper, so the adjoint crosses more
+    of the merge chain before it meets the layers it differentiates."""
+    atm = synth.atmosphere()
+    run = abi.make_run(880.0, 960.0, icntnm=1)
+    wk = atm["wk"] if scale == 1.0 else atm["wk"][:, :6] * scale
+    dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], wk, atm["wbroad"], iemit=1)
+    layers = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(wk[l]), atm["wbroad"][l], dv[l], 880.0, 960.0)
+              for l in range(len(dv))]
+    tzu = np.asarray(atm["tlev"][1:], np.float64)
+    rad, jac, _, _ = gpu_ctx.jacobian(run, _cp(layers), tzu, 295.0, 0.98, SPECIES, tl=True)
+    g = np.random.default_rng(7).normal(size=len(rad))
+    r2, grad = gpu_ctx.backward(run, _cp(layers), tzu, 295.0, 0.98, SPECIES, g)
+    np.testing.assert_allclose(r2, rad, rtol=1e-12)
+    ref = np.einsum("qln,n->ql", jac, g)
+    err = np.abs(grad - ref).max() / np.abs(ref).max()
+    print("%s: max|grad| %.3e, max deviation from J^T g %.2e (relative)" % (
+        "column" if scale == 1.0 else "thin column", np.abs(ref).max(), err), flush=True)
+    assert err < 1e-5
+    seen = int(np.sum(np.abs(ref).max(axis=0) > 1e-6 * np.abs(ref).max()))
+    print("layers with a gradient above 1e-6 of the largest: %d" % seen, flush=True)
+    assert seen >= (20 if scale != 1.0 else 5)
+
+
+def test_backward_batch_equals_single(gpu_ctx, lines):
+    """Two profiles in one lblx_backward_batch call equal their single runs
+    (the adjoint's scatter uses atomic adds: equal to rounding)."""
+    profs = synth.batch_profiles(256)[3:5]
+    items = []
+    for atm in profs:
+        r = abi.make_run(atm["v1"], atm["v2"], icntnm=1)
+        dv, _ = lib.plan_dv(r, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+        lays = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l], dv[l],
+                               atm["v1"], atm["v2"]) for l in range(len(dv))]
+        items.append((lays, atm["tlev"][1:], atm["tlev"][0], 0.98))
+    run = abi.make_run(500.0, 2500.0, icntnm=1)
+    P = 1 + len(SPECIES)
+    sizes = [lib.od_size(run, it[0][-1]) for it in items]
+    roff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    gsz = [P * len(it[0]) for it in items]
+    goff = np.concatenate([[0], np.cumsum(gsz)[:-1]]).astype(np.int64)
+    rng = np.random.default_rng(11)
+    dl = rng.normal(size=int(sum(sizes)))
+    d_dl = gpu_ctx.alloc(8 * len(dl))
+    d_g = gpu_ctx.alloc(8 * int(sum(gsz)))
+    try:
+        d_dl.from_host(dl)
+        gpu_ctx.backward_batch_device(run, items, SPECIES, d_dl.ptr, roff, d_g.ptr, goff)
+        gb = d_g.to_host(np.float64, int(sum(gsz)))
+    finally:
+        d_dl.free()
+        d_g.free()
+    for k, it in enumerate(items):
+        _, gs = gpu_ctx.backward(run, _cp(it[0]), it[1], it[2], it[3], SPECIES, dl[roff[k]:roff[k] + sizes[k]])
+        gk = gb[goff[k]:goff[k] + gsz[k]].reshape(gs.shape)
+        print("profile %d: %d layers, max|grad| %.3e" % (k, len(it[0]), np.abs(gs).max()), flush=True)
+        np.testing.assert_allclose(gk, gs, rtol=1e-10, atol=1e-12 * np.abs(gs).max())
