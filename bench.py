@@ -226,19 +226,19 @@ def batch_block(ctx, lib, abi, synth, nprof, steps, nadj):
     ms = (time.perf_counter() - t0) * 1000.0 / steps
     rad.free()
     tra.free()
-    adj = None
-    if nadj > 0:  # forward and adjoint (AJ formulation) of the first nadj profiles
+    adj = bwd = None
+    P = 4
+    if nadj > 0:  # forward and Jacobians (tangent-linear OD) of the first nadj profiles
         sub = items[:nadj]
-        P = 4
         jsz = [P * len(it[0]) * n for it, n in zip(sub, sizes[:nadj])]
         roff = np.concatenate([[0], np.cumsum(sizes[:nadj])[:-1]]).astype(np.int64)
         joff = np.concatenate([[0], np.cumsum(jsz)[:-1]]).astype(np.int64)
         rad = ctx.alloc(8 * int(sum(sizes[:nadj])))
         jac = ctx.alloc(8 * int(sum(jsz)))
-        ctx.jacobians_batch_device(run, sub, [1, 2, 3], rad.ptr, roff, jac.ptr, joff, aj=True)
+        ctx.jacobians_batch_device(run, sub, [1, 2, 3], rad.ptr, roff, jac.ptr, joff, tl=True)
         ctx.sync()
         t0 = time.perf_counter()
-        ctx.jacobians_batch_device(run, sub, [1, 2, 3], rad.ptr, roff, jac.ptr, joff, aj=True)
+        ctx.jacobians_batch_device(run, sub, [1, 2, 3], rad.ptr, roff, jac.ptr, joff, tl=True)
         ctx.sync()
         ms_adj = (time.perf_counter() - t0) * 1000.0
         rad.free()
@@ -246,9 +246,30 @@ def batch_block(ctx, lib, abi, synth, nprof, steps, nadj):
         adj = {"value": nadj * 1000.0 / ms_adj, "unit": "spectra/s", "ms_per_step": ms_adj, "profiles": nadj,
                "columns_per_layer": P,
                "workload": "the first %d profiles above: TOA radiance and its Jacobians (T, ln q of H2O, "
-                           "CO2, O3 at every layer; AJ OD runs), FP32 split" % nadj}
+                           "CO2, O3 at every layer; tangent-linear OD), FP32 split" % nadj}
+        # reverse mode over every profile (dL/dR = 1: the band-integrated
+        # radiance's gradient); memory grows with layers x points only
+        roff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        gsz = [P * len(it[0]) for it in items]
+        goff = np.concatenate([[0], np.cumsum(gsz)[:-1]]).astype(np.int64)
+        dl = ctx.alloc(8 * int(sum(sizes)))
+        dl.from_host(np.ones(int(sum(sizes))))
+        grad = ctx.alloc(8 * int(sum(gsz)))
+        ctx.backward_batch_device(run, items[:2], [1, 2, 3], dl.ptr, roff[:2], grad.ptr, goff[:2])
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.backward_batch_device(run, items, [1, 2, 3], dl.ptr, roff, grad.ptr, goff)
+        ctx.sync()
+        ms_bwd = (time.perf_counter() - t0) * 1000.0
+        dl.free()
+        grad.free()
+        bwd = {"value": nprof * 1000.0 / ms_bwd, "unit": "spectra/s", "ms_per_step": ms_bwd, "profiles": nprof,
+               "columns_per_layer": P,
+               "workload": "all %d profiles: TOA radiance and the gradient of a scalar of it (dL/dR given) "
+                           "with respect to T, ln q of H2O, CO2, O3 at every layer, reverse mode, FP32 split"
+                           % nprof}
     return {"value": nprof * 1000.0 / ms, "unit": "spectra/s", "ms_per_step": ms, "steps": steps,
-            "forward_and_adjoint": adj,
+            "forward_and_adjoint": adj, "backward": bwd,
             "profiles": nprof, "layers": int(sum(len(i[0]) for i in items)), "points": int(npts),
             "tiles": list(tiles), "autotune_s": t_tune,
             "workload": "ragged profiles: 20-80 layers, psfc 600-1050 hPa, 200-800 cm-1 windows "
@@ -347,18 +368,26 @@ def main():
     v = v1top + np.arange(ntop) * layers[-1].dv
     dbt = np.abs(bt(r32, v) - bt(r64, v))
     # Jacobians of the FP64 spectrum: T and H2O/CO2/O3 at every layer
-    jac_ms = jac_aj_ms = None
+    jac_ms = jac_aj_ms = jac_tl_ms = bwd_ms = None
     jac32 = None
     if not a.no_jacobian:
         nj = 4 * nlay * ntop
         jbuf = ctx.alloc(8 * nj)
+        gbuf = ctx.alloc(8 * 4 * nlay)
+        dlbuf = ctx.alloc(8 * ntop)
+        dlbuf.from_host(np.random.default_rng(5).normal(size=ntop))
         nrep = max(2, a.steps // 5)
 
-        def time_jac(run_, aj):
+        def time_jac(run_, mode):
             def once():
                 L = [abi.Layer.from_buffer_copy(x) for x in layers]
-                if aj:  # the reference's analytic-Jacobian formulation (single-species runs, +1 K)
+                if mode == "aj":  # the reference's analytic-Jacobian formulation (single-species runs, +1 K)
                     ctx.jacobian_aj_device(run_, L, tzu, tbnd, emis, [1, 2, 3], rad.ptr, jbuf.ptr, ntop)
+                elif mode == "tl":  # tangent-linear optical depths
+                    ctx.jacobian_tl_device(run_, L, tzu, tbnd, emis, [1, 2, 3], rad.ptr, jbuf.ptr, ntop)
+                elif mode == "bwd":  # reverse mode: one gradient
+                    ctx.backward_batch_device(run_, [(L, tzu, tbnd, emis)], [1, 2, 3], dlbuf.ptr, [0], gbuf.ptr,
+                                              [0], d_rad=rad.ptr)
                 else:
                     ctx.jacobian_device(run_, L, tzu, tbnd, emis, [1, 2, 3], 1e-3, 1e-4, rad.ptr, jbuf.ptr, ntop)
             once()
@@ -369,13 +398,19 @@ def main():
             ctx.sync()
             return (time.perf_counter() - t0) * 1000.0 / nrep
 
-        jac_ms = time_jac(run, False)
-        jac_aj_ms = time_jac(run, True)
+        jac_ms = time_jac(run, "fd")
+        jac_aj_ms = time_jac(run, "aj")
+        jac_tl_ms = time_jac(run, "tl")
+        bwd_ms = time_jac(run, "bwd")
         # the adjoint's own precision form: FP32 line shapes and tangent
         # planes, FP64 positions, against the FP32-split forward
-        j32, j32aj = time_jac(run32, False), time_jac(run32, True)
-        jac32 = {"ms": j32, "ratio_vs_forward": j32 / ms32, "aj_ms": j32aj, "aj_ratio_vs_forward": j32aj / ms32}
+        j32, j32aj, j32tl, b32 = (time_jac(run32, m) for m in ("fd", "aj", "tl", "bwd"))
+        jac32 = {"ms": j32, "ratio_vs_forward": j32 / ms32, "aj_ms": j32aj, "aj_ratio_vs_forward": j32aj / ms32,
+                 "tl_ms": j32tl, "tl_ratio_vs_forward": j32tl / ms32, "backward_ms": b32,
+                 "backward_ratio_vs_forward": b32 / ms32}
         jbuf.free()
+        gbuf.free()
+        dlbuf.free()
     # batched FP32-split forward: ragged profiles with their own surface
     # pressures and band windows over the same line table
     batch = None
@@ -427,6 +462,12 @@ def main():
             "parameters": "T, ln q of H2O, CO2, O3 at every layer; OD forward differences, exact merge tangents",
             "aj": {"ms": jac_aj_ms, "ratio_vs_forward": jac_aj_ms / ms,
                    "formulation": "reference analytic Jacobian: single-species OPDPTH runs, +1 K"},
+            "tl": {"ms": jac_tl_ms, "ratio_vs_forward": jac_tl_ms / ms,
+                   "formulation": "tangent-linear optical depths beside one OPDPTH run of the column, "
+                                  "exact merge tangents"},
+            "backward": {"ms": bwd_ms, "ratio_vs_forward": bwd_ms / ms,
+                         "formulation": "reverse mode: gradient of a scalar of the TOA spectrum (dL/dR given) "
+                                        "for the same 200 parameters"},
             "fp32_split": jac32},
         "batch": None if batch is None else dict(batch, spectra_ratio_vs_fp64=batch["value"] * ms / 1000.0),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -30,7 +30,7 @@
 #define LBLX_FP32_LDS64 0xC  // bits G = 2, 3
 #endif
 #ifndef LBLX_TAN_LDS64
-#define LBLX_TAN_LDS64 0xC  // bits G = 2, 3
+#define LBLX_TAN_LDS64 0xE  // bits G = 1, 2, 3: FP32 LDS atomics are a slow path on gfx950
 #endif
 #include "lblx_plan.h"
 #include "lblx_contdata.h"
@@ -300,7 +300,7 @@ struct lblx_ctx {
   // tangent-linear optical depth: compacted-line records, CONVF4's accepted
   // lines, per-line derivative weights, channel rows, continuum channels
   DevBuf t_craw, t_amask, t_wspan, t_tl, t_t9, t_r4c, t_tlay, t_cps, t_absab, t_absc, t_rh, t_r3c, t_r2c,
-      t_odcoff, t_tmap[3], t_tabs, t_mem, t_memq, t_f4c, t_f4d;
+      t_odcoff, t_tmap[3], t_tabs, t_mem, t_memq, t_f4c, t_f4d, t_parts, t_r4p;
   int64_t s_twN = 0;
   int tiles[3] = {8192, 4096, 2048};      // R1, R2, R3 tile widths (autotuned)
   DevBuf b_jobs, b_rt, b_rl;              // batched merges
@@ -457,7 +457,8 @@ void lblx_destroy(lblx_ctx *c) {
                     &c->w_dirty, &c->d_vtab, &c->d_vtabf, &c->w_dvo,
                     &c->t_craw, &c->t_amask, &c->t_wspan, &c->t_tl, &c->t_t9, &c->t_r4c, &c->t_tlay, &c->t_cps,
                     &c->t_absab, &c->t_absc, &c->t_rh, &c->t_r3c, &c->t_r2c, &c->t_odcoff, &c->t_tmap[0],
-                    &c->t_tmap[1], &c->t_tmap[2], &c->t_tabs, &c->t_mem, &c->t_memq, &c->t_f4c, &c->t_f4d};
+                    &c->t_tmap[1], &c->t_tmap[2], &c->t_tabs, &c->t_mem, &c->t_memq, &c->t_f4c, &c->t_f4d,
+                    &c->t_parts, &c->t_r4p};
   for (DevBuf *b : bufs) b->release();
   c->tcollect();
   c->stage_release();
@@ -870,9 +871,9 @@ struct JobsPrep {
 // the same compute units.  The windows are dealt out heaviest first, one per
 // compute unit per tier, in alternating direction, with the raw lines within
 // BOUND4 of the window times its points as the weight.
-static void far_wing_order(const lblx_ctx *c, JobsPrep &J) {
+static void window_costs(const lblx_ctx *c, const JobsPrep &J, std::vector<double> &cost) {
   const int nwin = (int)J.win.size();
-  std::vector<double> cost(nwin);
+  cost.resize(nwin);
   for (int i = 0; i < nwin; i++) {
     const DevWin &w = J.win[i];
     const double b4 = J.ys[w.job].bound4;
@@ -880,6 +881,12 @@ static void far_wing_order(const lblx_ctx *c, JobsPrep &J) {
     const auto hi = std::upper_bound(lo, c->h_vnu.end(), w.v2r4 + b4);
     cost[i] = (double)(hi - lo) * w.nptr4;
   }
+}
+
+static void far_wing_order(const lblx_ctx *c, JobsPrep &J) {
+  const int nwin = (int)J.win.size();
+  std::vector<double> cost;
+  window_costs(c, J, cost);
   std::vector<int32_t> by(nwin);
   for (int i = 0; i < nwin; i++) by[i] = i;
   std::stable_sort(by.begin(), by.end(), [&](int a, int b) { return cost[a] > cost[b]; });
@@ -1086,21 +1093,20 @@ struct TanReq {
   std::vector<int64_t> odc_off;  // [job P + q]
 };
 
-// Largest power of two tile (at most cap points) whose P channel
+// Widest tile (a multiple of 512 points, at most cap) whose P channel
 // accumulators of `esz` bytes fit the LDS beside `fixed` bytes of tables.
 static int tan_tile_width(int cap, int P, size_t esz, size_t fixed) {
   const size_t lds = 160u << 10;
-  int t = 256;
-  while (2 * t <= cap && fixed + (size_t)P * (2 * t) * esz <= lds) t *= 2;
+  int t = 512;
+  while (t + 512 <= cap && fixed + (size_t)P * (t + 512) * esz <= lds) t += 512;
   return t;
 }
 
 template <int P>
 static void launch_convf4_tan(dim3 g, hipStream_t s, const Tape9 &T9, const F4TConst *fc, const float4 *fd,
-                              const DevLayer *dY, const TanLayer *dT, Strides S, const DevWin *dw, const DevSeg *dg,
-                              const uint64_t *am, int64_t mwords, const int32_t *ws, const double *r4, double *r4c) {
-  hipLaunchKernelGGL(k_convf4_tan<P>, g, dim3(CFT_T), 0, s, T9, fc, fd, dY, dT, S, dw, dg, am, mwords, ws, r4,
-                     r4c);
+                              const DevLayer *dY, Strides S, const DevWin *dw, const F4TPart *parts,
+                              const uint64_t *am, int64_t mwords, const int32_t *ws, float *r4p) {
+  hipLaunchKernelGGL(k_convf4_tan<P>, g, dim3(CFT_T), 0, s, T9, fc, fd, dY, S, dw, parts, am, mwords, ws, r4p);
 }
 
 // The tangent passes of a batch, after its forward launches (which left
@@ -1161,8 +1167,35 @@ static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, c
     tmv[G] = tj;
     tmv[G].insert(tmv[G].end(), tlc.begin(), tlc.end());
   }
+  // far-wing parts: a window heavier than the total over the resident
+  // workgroups is shared out over several (heaviest parts first)
+  std::vector<F4TPart> parts;
+  std::vector<int32_t> wfirst(nwin), wnp(nwin);
+  {
+    std::vector<double> cost;
+    window_costs(c, J, cost);
+    double tot = 0.;
+    for (double x : cost) tot += x;
+    const double per = tot / (double)std::max(1, 4 * c->ncu);
+    std::vector<std::pair<double, F4TPart>> order;
+    int dst = 0;
+    for (int w = 0; w < nwin; w++) {
+      int np = per > 0. ? (int)std::ceil(cost[w] / per) : 1;
+      np = std::min(64, std::max(1, np));
+      wfirst[w] = dst;
+      wnp[w] = np;
+      for (int k = 0; k < np; k++) order.push_back({cost[w] / np, F4TPart{w, k, np, dst + k}});
+      dst += np;
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    parts.reserve(order.size());
+    for (auto &o : order) parts.push_back(o.second);
+  }
+  const int nparts = (int)parts.size();
   auto need = [&](DevBuf &b, size_t bytes) { return b.ensure(bytes) != 0; };
-  if (need(c->t_tlay, NJ * sizeof(TanLayer)) || need(c->t_tl, NJ * S.line * sizeof(TanLine)) ||
+  if (need(c->t_parts, (size_t)nparts * sizeof(F4TPart) + (size_t)nwin * 8) ||
+      need(c->t_r4p, (size_t)nparts * P * R4ROW * 4) ||
+      need(c->t_tlay, NJ * sizeof(TanLayer)) || need(c->t_tl, NJ * S.line * sizeof(TanLine)) ||
       need(c->t_t9, NJ * S.line * 4 * P * 4) || need(c->t_r4c, (size_t)nwin * P * R4ROW * 8) ||
       need(c->t_mem, NJ * S.line * sizeof(MemTan)) || need(c->t_memq, NJ * S.line) ||
       need(c->t_f4c, NJ * S.line * sizeof(F4TConst)) || need(c->t_f4d, NJ * S.line * P * sizeof(float4)) ||
@@ -1177,6 +1210,12 @@ static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, c
     if (need(c->t_tmap[G], tmv[G].size() * 8)) return LBLX_E_HIP;
   c->stage_begin();
   HIPCHK(c->h2d(c->t_tlay.p, R.tl.data(), NJ * sizeof(TanLayer), s));
+  HIPCHK(c->h2d(c->t_parts.p, parts.data(), (size_t)nparts * sizeof(F4TPart), s));
+  {
+    int32_t *wf = (int32_t *)(c->t_parts.as<char>() + (size_t)nparts * sizeof(F4TPart));
+    HIPCHK(c->h2d(wf, wfirst.data(), (size_t)nwin * 4, s));
+    HIPCHK(c->h2d(wf + nwin, wnp.data(), (size_t)nwin * 4, s));
+  }
   HIPCHK(c->h2d(c->t_odcoff.p, R.odc_off.data(), NJ * PO * 8, s));
   HIPCHK(c->h2d(c->t_rh.p, R.rh, TAN_NQ * 8, s));
   if (cont) HIPCHK(c->h2d(c->t_cps.p, R.cps.data(), 2 * NJ * PO * sizeof(ContPlan), s));
@@ -1213,23 +1252,27 @@ static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, c
   float4 *fd = c->t_f4d.as<float4>();
   hipLaunchKernelGGL(k_f4_tan_lines, dim3(lgrid, njobs), dim3(256), 0, s, T9, (const float *)t9, dY, dT, S, fc, fd);
   {
-    const dim3 g(nwin);
+    const dim3 g(nparts);
     const uint64_t *am = c->t_amask.as<uint64_t>();
     const int32_t *ws = c->t_wspan.as<int32_t>();
-    const double *r4 = c->w_r4.as<double>();
     const F4TConst *cfc = fc;
     const float4 *cfd = fd;
+    const F4TPart *pp = c->t_parts.as<F4TPart>();
+    float *r4p = c->t_r4p.as<float>();
     switch (P) {
-      case 1: launch_convf4_tan<1>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
-      case 2: launch_convf4_tan<2>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
-      case 3: launch_convf4_tan<3>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
-      case 4: launch_convf4_tan<4>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
-      case 5: launch_convf4_tan<5>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
-      case 6: launch_convf4_tan<6>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
-      case 7: launch_convf4_tan<7>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
-      case 8: launch_convf4_tan<8>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
-      default: launch_convf4_tan<9>(g, s, T9, cfc, cfd, dY, dT, S, dw, dg, am, mwords, ws, r4, r4c); break;
+      case 1: launch_convf4_tan<1>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
+      case 2: launch_convf4_tan<2>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
+      case 3: launch_convf4_tan<3>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
+      case 4: launch_convf4_tan<4>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
+      case 5: launch_convf4_tan<5>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
+      case 6: launch_convf4_tan<6>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
+      case 7: launch_convf4_tan<7>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
+      case 8: launch_convf4_tan<8>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
+      default: launch_convf4_tan<9>(g, s, T9, cfc, cfd, dY, S, dw, pp, am, mwords, ws, r4p); break;
     }
+    const int32_t *wf = (const int32_t *)(c->t_parts.as<char>() + (size_t)nparts * sizeof(F4TPart));
+    hipLaunchKernelGGL(k_r4_tan_sum, dim3((unsigned)((P * R4ROW + 255) / 256), nwin), dim3(256), 0, s, dY, dT, dw,
+                       dg, wf, wf + nwin, (const float *)r4p, P, (const double *)c->w_r4.as<double>(), r4c);
   }
   if (cont) {  // the output channels' continuum (U carries none)
     const int64_t nplan = 2 * NJ * PO;

@@ -202,16 +202,15 @@ __global__ __launch_bounds__(TAN_TILE_T) void k_tan_tile(Recs recs, const TanLin
   const int PU = Ts[job].nint > Ts[job].nchan ? Ts[job].nchan : 0;  // row of U (0: none)
   constexpr int NT = G == 1 ? 3 : 1;  // tables of the grid
   LACC *acc = reinterpret_cast<LACC *>(sm_t);                     // [P][tile]
-  float *tf = reinterpret_cast<float *>(acc + (size_t)P * tile);  // NT tables, then NT slopes
+  // NT tables, each entry with its slope (one 64-bit LDS read per table)
+  float2 *tf = reinterpret_cast<float2 *>(acc + (size_t)P * tile);
   for (int k = tid; k < P * tile; k += TAN_TILE_T) acc[k] = (LACC)0.;
   const int t0 = G == 1 ? 0 : (G == 2 ? 3 : 4);
   for (int k = tid; k < NT * NFMX; k += TAN_TILE_T) {
     const int a = k / NFMX, r = k - a * NFMX;
-    tf[k] = tabs[(size_t)(t0 + a) * NFMX + r];
-    tf[NT * NFMX + k] = tabs[(size_t)(5 + t0 + a) * NFMX + r];
+    tf[k] = make_float2(tabs[(size_t)(t0 + a) * NFMX + r], tabs[(size_t)(5 + t0 + a) * NFMX + r]);
   }
   __syncthreads();
-  const float *tfp = tf + NT * NFMX;
   const int64_t T0 = (int64_t)M.loc[blockIdx.x] * tile;
   const int lo = M.lo[blockIdx.x], hi = M.hi[blockIdx.x];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -285,12 +284,14 @@ __global__ __launch_bounds__(TAN_TILE_T) void k_tan_tile(Recs recs, const TanLin
         const float u = (float)fabs(zf);
         int iz = fint(fabs(zf) + 1.5) - 1;
         iz = iz > NFMX - 1 ? NFMX - 1 : iz;
-        const float f = tf[iz];
-        float sl = w0 * tfp[iz];  // the weights times the slopes
+        const float2 fe = tf[iz];
+        const float f = fe.x;
+        float sl = w0 * fe.y;  // the weights times the slopes
         float tv = td0 * f, sv = sd0 * f, uv = ud0 * f;
         if (G == 1) {
-          const float g = tf[NFMX + iz], x = tf[2 * NFMX + iz];
-          sl = fmaf(w1, tfp[NFMX + iz], fmaf(w2, tfp[2 * NFMX + iz], sl));
+          const float2 ge = tf[NFMX + iz], xe = tf[2 * NFMX + iz];
+          const float g = ge.x, x = xe.x;
+          sl = fmaf(w1, ge.y, fmaf(w2, xe.y, sl));
           tv = fmaf(td1, g, fmaf(td2, x, tv));
           sv = fmaf(sd1, g, fmaf(sd2, x, sv));
           uv = fmaf(ud1, g, fmaf(ud2, x, uv));
@@ -586,32 +587,38 @@ __global__ void k_f4_tan_lines(Tape9 T9, const float *t9tan, const DevLayer *Ys,
   }
 }
 
-// k_convf4_tan, one workgroup per window: every line the forward run
-// accepted with points in the window (amask: one 64-bit word per 64-line
-// chunk from the window's first line; wspan: that line and the chunk
-// count) adds its derivative to each channel's R4 row.  The accepted lines
-// of CFT_T lines at a time are compacted into LDS with their constants;
-// every thread holds the channels of its points j = tid + 256 s in
-// registers, so a point's additions happen in line order without atomics.
+// k_convf4_tan: every line the forward run accepted with points in a
+// window (amask: one 64-bit word per 64-line chunk from the window's first
+// line; wspan: that line and the chunk count) adds its derivative to each
+// channel's R4 row.  The decisions are already taken, so a heavy window's
+// chunks are shared out over several workgroups (F4TPart: window, part,
+// parts, and the row of partial sums the part writes); k_r4_tan_sum adds a
+// window's parts in a fixed order.  The accepted lines of CFT_T lines at a
+// time are compacted into LDS with their constants; every thread holds the
+// channels of its points j = tid + 256 s in registers, so a point's
+// additions happen in line order without atomics.
 constexpr int CFT_T = 256;
 constexpr int CFT_S = (R4ROW + CFT_T - 1) / CFT_T;  // points per thread
 
+struct F4TPart {
+  int32_t wi, part, nparts, dst;
+};
+
 template <int P>
 __global__ __launch_bounds__(CFT_T) void k_convf4_tan(Tape9 T9, const F4TConst *fc, const float4 *fd,
-                                                      const DevLayer *Ys, const TanLayer *Ts, Strides S,
-                                                      const DevWin *wins, const DevSeg *segs, const uint64_t *amask,
-                                                      int64_t mwords, const int32_t *wspan, const double *r4fwd,
-                                                      double *r4c) {
+                                                      const DevLayer *Ys, Strides S, const DevWin *wins,
+                                                      const F4TPart *parts, const uint64_t *amask, int64_t mwords,
+                                                      const int32_t *wspan, float *r4part) {
   __shared__ F4TConst par[CFT_T];
   __shared__ double pxn[CFT_T];
   __shared__ int2 pj[CFT_T];
   __shared__ float4 cof[CFT_T * P];
   __shared__ int s_wcnt[CFT_T / 64];
-  const int wi = blockIdx.x;
+  const F4TPart pt = parts[blockIdx.x];
+  const int wi = pt.wi;
   const DevWin W = wins[wi];
   const int job = W.job;
   const DevLayer &Y = Ys[job];
-  const double T = Ts[job].tave;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int64_t o = job * S.line;
@@ -625,9 +632,12 @@ __global__ __launch_bounds__(CFT_T) void k_convf4_tan(Tape9 T9, const F4TConst *
 #pragma unroll
     for (int c = 0; c < P; c++) acc[sl][c] = 0.f;
   const int pos = wspan[2 * wi], nch = wspan[2 * wi + 1];
+  // this part's chunks, rounded to CFT_T-line steps
+  const int nstep = (nch + CFT_T / 64 - 1) / (CFT_T / 64);
+  const int s0 = (int)((int64_t)nstep * pt.part / pt.nparts), s1 = (int)((int64_t)nstep * (pt.part + 1) / pt.nparts);
   const int end = pos + 64 * nch;
   const uint64_t *mk = amask + (size_t)wi * mwords;
-  for (int c0 = pos; c0 < end; c0 += CFT_T) {
+  for (int c0 = pos + CFT_T * s0; c0 < pos + CFT_T * s1 && c0 < end; c0 += CFT_T) {
     // lines c0 .. c0 + 255: chunk words (c0 - pos)/64 + wave
     const int i = c0 + tid;
     const int w = (c0 - pos) / 64 + wave;
@@ -635,7 +645,7 @@ __global__ __launch_bounds__(CFT_T) void k_convf4_tan(Tape9 T9, const F4TConst *
     const bool acc_me = (word >> lane) & 1ull;
     const int below = lane == 0 ? 0 : __builtin_popcountll(word & ((1ull << lane) - 1));
     if (lane == 0) s_wcnt[wave] = __builtin_popcountll(word);
-    __syncthreads();  // counts ready; the previous chunk's constants no longer read
+    __syncthreads();  // counts ready; the previous step's constants no longer read
     int base = 0, nacc = 0;
 #pragma unroll
     for (int v = 0; v < CFT_T / 64; v++) {
@@ -659,8 +669,10 @@ __global__ __launch_bounds__(CFT_T) void k_convf4_tan(Tape9 T9, const F4TConst *
     }
     __syncthreads();
     for (int a = 0; a < nacc; a++) {
-      const int2 jr = pj[a];
-      if (jr.y < jr.x) continue;
+      // the line's point range is the same for every lane
+      const int2 jl = pj[a];
+      const int jx = __builtin_amdgcn_readfirstlane(jl.x), jy = __builtin_amdgcn_readfirstlane(jl.y);
+      if (jy < jx) continue;
       const F4TConst q = par[a];
       const double xnui = pxn[a];
       float4 k[P];
@@ -670,27 +682,24 @@ __global__ __launch_bounds__(CFT_T) void k_convf4_tan(Tape9 T9, const F4TConst *
       for (int sl = 0; sl < CFT_S; sl++) {
         // this wave's points of slot sl: j = 256 sl + 64 wave + lane
         const int wlo = CFT_T * sl + 64 * wave;
-        if (wlo + 63 < jr.x || wlo > jr.y) continue;  // wave-uniform
+        if (wlo + 63 < jx || wlo > jy) continue;  // wave-uniform
         const int j = wlo + lane;
-        if (j < jr.x || j > jr.y) continue;
+        if (j < jx || j > jy) continue;
         const float xm = (float)((double)(j - 1) * dvr4 - xnui);
         const float xmsq = xm * xm;
         const float sub = fmaf(-xmsq, q.c1, q.c0);      // SUB
         const float dsubl = fmaf(-xmsq, q.fl1, q.fl0);  // dSUB/daL
-        float f, fl, fv, fx;                            // f, df/daL, df/daV, df/dXMSQ
-        if (xmsq <= q.thr) {
-          const float sh = fmaf(xmsq, q.b3v, a3);       // A3 + B3 XMSQ/aV^2
-          f = q.siv * sh - sub;
-          fl = q.sivl * sh - dsubl;
-          fv = q.sivv * fmaf(2.f * xmsq, q.b3v, a3);
-          fx = q.siv * q.b3v + q.c1;
-        } else {
-          const float rr = 1.f / (q.alfli2 + xmsq);
-          f = q.sil * rr - sub;
-          fl = fmaf(q.silv, rr, -q.sil2 * rr * rr) - dsubl;
-          fv = 0.f;
-          fx = -q.sil * rr * rr + q.c1;
-        }
+        // both branches, then the one the forward took (a branch would
+        // diverge at the ZSQBND boundary inside most waves)
+        const float sh = fmaf(xmsq, q.b3v, a3);         // A3 + B3 XMSQ/aV^2
+        const float rr = __builtin_amdgcn_rcpf(q.alfli2 + xmsq);
+        const float rr2 = rr * rr;
+        const bool nearb = xmsq <= q.thr;
+        // f, df/daL, df/daV, df/dXMSQ
+        const float f = (nearb ? q.siv * sh : q.sil * rr) - sub;
+        const float fl = (nearb ? q.sivl * sh : fmaf(q.silv, rr, -q.sil2 * rr2)) - dsubl;
+        const float fv = nearb ? q.sivv * fmaf(2.f * xmsq, q.b3v, a3) : 0.f;
+        const float fx = (nearb ? q.siv * q.b3v : -q.sil * rr2) + q.c1;
         const float fs = f * q.rs;
         const float fn = -2.f * xm * fx;  // df/dv: dXMSQ/dv = -2 xm
 #pragma unroll
@@ -699,35 +708,53 @@ __global__ __launch_bounds__(CFT_T) void k_convf4_tan(Tape9 T9, const F4TConst *
       }
     }
   }
-  // JRAD = 1: the window's piecewise-linear radiation factor on every
-  // channel, and for the temperature its derivative on the forward R4
-  double *rows = r4c + (size_t)wi * P * R4ROW;
+  float *row = r4part + (size_t)pt.dst * P * R4ROW;
 #pragma unroll
   for (int sl = 0; sl < CFT_S; sl++) {
     const int j = CFT_T * sl + tid;
     if (j >= R4ROW) continue;
-    const bool in = j >= 1 && j <= nptr4;
-    double rad = 1., dlr = 0.;
-    if (in && W.nseg > 0) {
-      int lo2 = 0, hi2 = W.nseg - 1;
-      while (lo2 < hi2) {
-        const int mid = (lo2 + hi2 + 1) >> 1;
-        if (segs[W.seg0 + mid].i1 <= j) lo2 = mid;
-        else hi2 = mid - 1;
-      }
-      const DevSeg sg = segs[W.seg0 + lo2];
-      if (j >= sg.i1 && j <= sg.i2) {
-        rad = sg.radvi + (double)(j - sg.i1) * sg.rdel;
-        dlr = d_dlnradfn_dT(W.v1r4 + (double)(j - 1) * dvr4, Y.xkt, T);
-      }
-    }
 #pragma unroll
-    for (int c = 0; c < P; c++) {
-      double v = in ? (double)acc[sl][c] * rad : 0.;
-      if (c == 0 && in) v += r4fwd[(size_t)wi * R4ROW + j] * dlr;
-      rows[(size_t)c * R4ROW + j] = v;
+    for (int c = 0; c < P; c++) row[(size_t)c * R4ROW + j] = acc[sl][c];
+  }
+}
+
+// A window's R4 channel rows: the sum of its parts (wfirst[w] .. +
+// nparts - 1, in order), the window's piecewise-linear radiation factor
+// (JRAD = 1) on every channel, and for the temperature that factor's
+// derivative on the forward R4.  One thread per (channel, point).
+__global__ void k_r4_tan_sum(const DevLayer *Ys, const TanLayer *Ts, const DevWin *wins, const DevSeg *segs,
+                             const int32_t *wfirst, const int32_t *wnparts, const float *r4part, int P,
+                             const double *r4fwd, double *r4c) {
+  const int wi = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P * R4ROW) return;
+  const int c = t / R4ROW, j = t - c * R4ROW;
+  const DevWin W = wins[wi];
+  const DevLayer &Y = Ys[W.job];
+  double *rows = r4c + (size_t)wi * P * R4ROW;
+  const bool in = j >= 1 && j <= W.nptr4;
+  if (!in) {
+    rows[t] = 0.;
+    return;
+  }
+  double v = 0.;
+  const int f0 = wfirst[wi], np = wnparts[wi];
+  for (int s = 0; s < np; s++) v += (double)r4part[(size_t)(f0 + s) * P * R4ROW + t];
+  if (W.nseg > 0) {
+    int lo2 = 0, hi2 = W.nseg - 1;
+    while (lo2 < hi2) {
+      const int mid = (lo2 + hi2 + 1) >> 1;
+      if (segs[W.seg0 + mid].i1 <= j) lo2 = mid;
+      else hi2 = mid - 1;
+    }
+    const DevSeg sg = segs[W.seg0 + lo2];
+    if (j >= sg.i1 && j <= sg.i2) {
+      v = v * (sg.radvi + (double)(j - sg.i1) * sg.rdel);
+      if (c == 0)
+        v += r4fwd[(size_t)wi * R4ROW + j] * d_dlnradfn_dT(W.v1r4 + (double)(j - 1) * Y.dvr4, Y.xkt, Ts[W.job].tave);
     }
   }
+  rows[t] = v;
 }
 
 // Central differences of the continuum rows: absc[(job P + q) stride + k]

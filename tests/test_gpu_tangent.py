@@ -270,3 +270,71 @@ def test_backward_batch_equals_single(gpu_ctx, lines):
         gk = gb[goff[k]:goff[k] + gsz[k]].reshape(gs.shape)
         print("profile %d: %d layers, max|grad| %.3e" % (k, len(it[0]), np.abs(gs).max()), flush=True)
         np.testing.assert_allclose(gk, gs, rtol=1e-10, atol=1e-12 * np.abs(gs).max())
+
+
+def test_tl_jacobian_against_oracle_differences(gpu_ctx, lines):
+    """The tangent-linear Jacobian of the 880-960 cm-1 column against central
+    differences of the oracle's own OPDPTH + XMERGE chain (CPU restatement of
+    the reference): perturbing one layer reruns that layer's OPDPTH in the
+    oracle and the merge chain above it.  Points whose h and 2h differences
+    disagree (rejection flips) are left out; median below 1e-3 and share off
+    by more than 1 % below 2e-2 on the rest."""
+    import oracle_lib as O
+    tab = synth.config1_lines()
+    run, layers, tzu = _column(880.0, 960.0)
+    rad, jac, _, _ = gpu_ctx.jacobian(run, _cp(layers), tzu, 295.0, 0.98, SPECIES, tl=True)
+    tbnd, emis, v2 = 295.0, 0.98, 960.0
+
+    def layer_od(L, jrad_in):
+        inp = abi.Layer.from_buffer_copy(L)
+        inp.jrad = jrad_in
+        od, _, _, jout = O.od_layer(run, inp, tab)
+        return od, O.panels(), jout
+
+    base, jin = [], 1
+    for L in layers:
+        od, pan, jout = layer_od(L, jin)
+        base.append((od, pan, jin))
+        jin = jout
+
+    def chain(items):
+        m = None
+        for k, (od, pan, _) in enumerate(items):
+            if k == 0:
+                m = O.merge_first(od, pan, layers[k].dv, layers[k].tave, tzu[k], tbnd, emis)
+            else:
+                m, _ = O.merge_next(m, od, pan, layers[k].dv, layers[k].tave, tzu[k], v2)
+        return m.rad
+
+    r0 = chain(base)
+    np.testing.assert_allclose(rad, r0, rtol=1e-9)  # GPU spectrum = oracle spectrum
+
+    def spec(l, q, h):
+        L = _perturb(layers, l, q, h)[l]
+        od, pan, jout = layer_od(L, base[l][2])
+        items = list(base)
+        items[l] = (od, pan, base[l][2])
+        if l + 1 < len(layers) and jout != base[l + 1][2]:  # the next layer's JRAD moved
+            od1, pan1, _ = layer_od(layers[l + 1], jout)
+            items[l + 1] = (od1, pan1, jout)
+        # the merge uses the perturbed layer's temperature too
+        saved = layers[l].tave
+        layers[l].tave = L.tave
+        try:
+            return chain(items)
+        finally:
+            layers[l].tave = saved
+
+    for l, q in ((49, 0), (47, 0), (45, 0), (49, 1), (45, 2), (47, 2), (49, 3), (45, 3)):
+        h = 1e-3 * layers[l].tave if q == 0 else 1e-4
+        d = {k: (spec(l, q, k * h) - spec(l, q, -k * h)) / (2 * k * h) for k in (1, 2)}
+        j = jac[q, l]
+        scale = np.abs(d[1]).max()
+        ok = np.abs(d[1] - d[2]) <= 1e-3 * np.abs(d[1])
+        big = ok & (np.abs(d[1]) > 1e-2 * scale)
+        r = np.abs(j[big] - d[1][big]) / np.abs(d[1][big])
+        print("oracle differences, layer %2d param %d: %d points, median %.2e, off>1%% %.2e" %
+              (l, q, int(big.sum()), np.median(r), np.mean(r > 1e-2)), flush=True)
+        assert big.sum() > 100
+        assert np.median(r) < 1e-3
+        assert np.mean(r > 1e-2) < 2e-2
