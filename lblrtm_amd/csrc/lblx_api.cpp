@@ -27,10 +27,11 @@
 #include "lblx_kernels.h"
 
 #ifndef LBLX_FP32_LDS64
+// FP32 shapes accumulate R2/R3 in FP64 LDS: ds_add_f32 alone runs about 23x
+// slower than ds_add_f64 on gfx950 (203 vs 4650 G atomics/s); in the R1
+// tile, where the adds interleave with three table reads and the FP64
+// index arithmetic, FP32 accumulation is still the faster (2.39 vs 2.69 ms)
 #define LBLX_FP32_LDS64 0xC  // bits G = 2, 3
-#endif
-#ifndef LBLX_TAN_LDS64
-#define LBLX_TAN_LDS64 0xE  // bits G = 1, 2, 3: FP32 LDS atomics are a slow path on gfx950
 #endif
 #include "lblx_plan.h"
 #include "lblx_contdata.h"
@@ -147,9 +148,6 @@ struct lblx_ctx {
   // (LBLX_FIX_MAXPASS at creation, 1..64), and what the last OD batch used
   int fix_maxpass = 16;
   int fix_passes = 0, fix_seq = 0;
-  // tangent tiles of grid G accumulate in FP64 LDS when bit G is set
-  // (LBLX_TAN_LDS64 at creation)
-  int tan_lds64 = LBLX_TAN_LDS64;
   void *scratch = nullptr;  // device address of `scratch` on this device
   hipStream_t stream = nullptr;
   bool timing = false;
@@ -367,8 +365,6 @@ int lblx_create(lblx_ctx **out, int device) {
     c->t9_all_wave = (e && std::atoi(e) != 0) ? 1 : 0;
     const char *f = std::getenv("LBLX_FIX_MAXPASS");
     if (f) c->fix_maxpass = std::min(64, std::max(1, std::atoi(f)));
-    const char *tl = std::getenv("LBLX_TAN_LDS64");
-    if (tl) c->tan_lds64 = (int)std::strtol(tl, nullptr, 0) & 0xE;
   }
   {
     int ncu = 0;
@@ -1093,15 +1089,6 @@ struct TanReq {
   std::vector<int64_t> odc_off;  // [job P + q]
 };
 
-// Widest tile (a multiple of 512 points, at most cap) whose P channel
-// accumulators of `esz` bytes fit the LDS beside `fixed` bytes of tables.
-static int tan_tile_width(int cap, int P, size_t esz, size_t fixed) {
-  const size_t lds = 160u << 10;
-  int t = 512;
-  while (t + 512 <= cap && fixed + (size_t)P * (t + 512) * esz <= lds) t += 512;
-  return t;
-}
-
 template <int P>
 static void launch_convf4_tan(dim3 g, hipStream_t s, const Tape9 &T9, const F4TConst *fc, const float4 *fd,
                               const DevLayer *dY, Strides S, const DevWin *dw, const F4TPart *parts,
@@ -1145,15 +1132,14 @@ static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, c
     if (c->t_tabs.ensure(tt.size() * 4)) return LBLX_E_HIP;
     HIPCHK(hipMemcpy(c->t_tabs.p, tt.data(), tt.size() * 4, hipMemcpyHostToDevice));
   }
-  // tile maps of the tangent tiles (narrower than the forward's: P
-  // accumulators and the slope tables share the LDS)
+  // tile maps of the tangent tiles (k_tan_seg's widths; LDS holds the
+  // grid's tables with their slopes)
   const size_t tabb[3] = {6u * 2601 * 4, 2u * 2601 * 4, 2u * 2601 * 4};
   int tw[3];
   std::vector<int32_t> tmv[3];
-  bool l64[3];
   for (int G = 0; G < 3; G++) {
-    l64[G] = (c->tan_lds64 >> (G + 1)) & 1;
-    tw[G] = tan_tile_width(J.tiles[G], P, l64[G] ? 8 : 4, tabb[G]);
+    if (G == 0) tw[G] = PO - 1 <= 3 ? TSeg<1, 3>::TILE : TSeg<1, 7>::TILE;
+    else tw[G] = G == 1 ? TSeg<2, 3>::TILE : TSeg<3, 3>::TILE;
     std::vector<int32_t> tj, tlc;
     for (int jb = 0; jb < njobs; jb++) {
       const LayerPlan &Pl = *J.plans[jb];
@@ -1293,28 +1279,32 @@ static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, c
     const int nt = (int)(tmv[G - 1].size() / 2);
     hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY, tm[G - 1],
                        nt, T, G, c->w_margin.as<double>());
-    const bool a64 = l64[G - 1];
-    const size_t shm = (size_t)P * T * (a64 ? 8 : 4) + tabb[G - 1];
-#define LBLX_TAN_A(GG, OUTT, LACC, RIN, SIN, ROUT, SOUT)                                                           \
-  do {                                                                                                            \
-    HIPCHK(hipFuncSetAttribute((const void *)k_tan_tile<GG, OUTT, LACC>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
-                               (int)shm));                                                                        \
-    hipLaunchKernelGGL((k_tan_tile<GG, OUTT, LACC>), dim3(nt), dim3(TAN_TILE_T), shm, s, recs, (const TanLine *)tl,   \
-                       tm[GG - 1], T, tabs, dY, dT, S, dp, dw, (const double *)r4c, absc, abs_stride, dg,         \
-                       (const double *)(RIN), (int64_t)(SIN), (double *)(ROUT), (int64_t)(SOUT),                  \
-                       (const OUTT *)od_base, (OUTT *)R.odc, odcoff);                                             \
+    const int nsp = PO - 1;
+    const int NSPsel = nsp == 0 ? 0 : (nsp <= 3 ? 3 : 7);
+    const size_t shm = tabb[G - 1];
+    const double *vr = c->d_vnu.as<double>();
+    const double *mgn = c->w_margin.as<double>();
+#define LBLX_TSEG(GG, OUTT, NSPV, RIN, SIN, ROUT, SOUT)                                                             \
+  do {                                                                                                             \
+    HIPCHK(hipFuncSetAttribute((const void *)k_tan_seg<GG, OUTT, NSPV>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                               (int)shm));                                                                         \
+    hipLaunchKernelGGL((k_tan_seg<GG, OUTT, NSPV>), dim3(nt), dim3(TSEG_T), shm, s, recs, (const TanLine *)tl,       \
+                       tm[GG - 1], vr, mgn, tabs, dY, dT, S, dp, dw, (const double *)r4c, absc, abs_stride, dg,    \
+                       (const double *)(RIN), (int64_t)(SIN), (double *)(ROUT), (int64_t)(SOUT),                   \
+                       (const OUTT *)od_base, (OUTT *)R.odc, odcoff);                                              \
   } while (0)
-#define LBLX_TAN_L(GG, OUTT, RIN, SIN, ROUT, SOUT)                   \
-  do {                                                              \
-    if (a64) LBLX_TAN_A(GG, OUTT, double, RIN, SIN, ROUT, SOUT);    \
-    else LBLX_TAN_A(GG, OUTT, float, RIN, SIN, ROUT, SOUT);         \
+#define LBLX_TSEG_N(GG, OUTT, RIN, SIN, ROUT, SOUT)                      \
+  do {                                                                  \
+    if (NSPsel == 0) LBLX_TSEG(GG, OUTT, 0, RIN, SIN, ROUT, SOUT);      \
+    else if (NSPsel == 3) LBLX_TSEG(GG, OUTT, 3, RIN, SIN, ROUT, SOUT); \
+    else LBLX_TSEG(GG, OUTT, 7, RIN, SIN, ROUT, SOUT);                  \
   } while (0)
-    if (G == 3) LBLX_TAN_L(3, double, nullptr, 0, r3c, J.r3tot);
-    else if (G == 2) LBLX_TAN_L(2, double, r3c, J.r3tot, r2c, J.r2tot);
-    else if (f32out) LBLX_TAN_L(1, float, r2c, J.r2tot, nullptr, 0);
-    else LBLX_TAN_L(1, double, r2c, J.r2tot, nullptr, 0);
-#undef LBLX_TAN_L
-#undef LBLX_TAN_A
+    if (G == 3) LBLX_TSEG_N(3, double, nullptr, 0, r3c, J.r3tot);
+    else if (G == 2) LBLX_TSEG_N(2, double, r3c, J.r3tot, r2c, J.r2tot);
+    else if (f32out) LBLX_TSEG_N(1, float, r2c, J.r2tot, nullptr, 0);
+    else LBLX_TSEG_N(1, double, r2c, J.r2tot, nullptr, 0);
+#undef LBLX_TSEG_N
+#undef LBLX_TSEG
   }
   c->tend(th);
   HIPCHK(hipGetLastError());

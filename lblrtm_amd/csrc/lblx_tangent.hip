@@ -175,63 +175,114 @@ __global__ __launch_bounds__(256) void k_near_tan(DevLines L, const DevLayer *Ys
 }
 
 // ---------------------------------------------------------------------
-// Tangent tiles of grid G: the channels' near-line scatter into LDS, then
-// PANEL's combine per channel as k_tile does it:
+// Tangent tiles of grid G: the channels' near-line scatter, then PANEL's
+// combine per channel as k_tile does it:
 //   G = 3: R3' = lines + XINT(R4') + XINT(ABSRB')
 //   G = 2: R2' = lines + Lagrange4(R3')
 //   G = 1: OD' = (lines + Lagrange4(R2')) R  [+ OD d ln R/dT for the
 //          temperature when JRAD = 0]
 // tabs: F1 FG XVER F2 F3, then their slopes per table step (10 x NFMX
 // floats).  Channel q of the R2/R3 rows is at rin/rout + q * sin/sout;
-// its optical-depth derivative goes to odc + odcoff[job * P + q].
-constexpr int TAN_TILE_T = 1024;
+// its optical-depth derivative goes to odc + odcoff[job * nchan + q].
+//
+// No LDS accumulators (k_tan_seg): each of the 16 waves
+// owns a contiguous segment of the tile, 64 NS points, lane i the points
+// i + 64 s, and keeps their channels in registers; it walks the lines whose
+// extent may reach its segment (the tile's line range narrowed by a binary
+// search on the line centres) and adds each line's contributions to its own
+// points.  Lines reaching two segments are walked by both waves; nothing is
+// shared between waves, so no atomics and no accumulator in LDS (on gfx950
+// ds_add_f32 runs near lane-serial and ds_add_f64 needs twice the LDS).
+// NSP: species channels held
+// (>= nspec); NS points per lane, sized to keep the registers of a
+// 1024-thread workgroup (128 per lane) free of spills: 6 (4 with seven
+// species) for R1, 4 for R2, 2 for R3 (tiles of 64 NS 16 points).
+constexpr int TSEG_T = 1024;
+template <int G, int NSP>
+struct TSeg {
+  static constexpr int NS = G == 1 ? (NSP <= 3 ? 6 : 4) : (G == 2 ? 4 : 2);
+  static constexpr int TILE = 64 * NS * (TSEG_T / 64);
+};
 
-template <int G, typename OUT, typename LACC>
-__global__ __launch_bounds__(TAN_TILE_T) void k_tan_tile(Recs recs, const TanLine *tl, TileMap M, int tile,
-                                                          const float *tabs, const DevLayer *Ys, const TanLayer *Ts,
-                                                          Strides S, const DevPanel *pans, const DevWin *wins,
-                                                          const double *r4c, const double *absc, int abs_stride,
-                                                          const DevSeg *segs, const double *rin, int64_t sin,
-                                                          double *rout, int64_t sout, const OUT *od, OUT *odc,
-                                                          const int64_t *odcoff) {
+template <int G, typename OUT, int NSP>
+__global__ __launch_bounds__(TSEG_T) void k_tan_seg(Recs recs, const TanLine *__restrict__ tl, TileMap M,
+                                                     const double *vraw,
+                                                     const double *margin, const float *tabs, const DevLayer *Ys,
+                                                     const TanLayer *Ts, Strides S, const DevPanel *pans,
+                                                     const DevWin *wins, const double *r4c, const double *absc,
+                                                     int abs_stride, const DevSeg *segs, const double *rin, int64_t sin,
+                                                     double *rout, int64_t sout, const OUT *od, OUT *odc,
+                                                     const int64_t *odcoff) {
+  constexpr int NS = TSeg<G, NSP>::NS;
+  constexpr int TILE = TSeg<G, NSP>::TILE;
+  constexpr int NT = G == 1 ? 3 : 1;  // tables of the grid
   extern __shared__ double sm_t[];
+  float2 *tf = reinterpret_cast<float2 *>(sm_t);  // each table entry with its slope
   const int tid = threadIdx.x;
   const int job = M.job[blockIdx.x];
   const DevLayer &Y = Ys[job];
-  const int P = Ts[job].nint;            // rows: every internal channel
-  const int PU = Ts[job].nint > Ts[job].nchan ? Ts[job].nchan : 0;  // row of U (0: none)
-  constexpr int NT = G == 1 ? 3 : 1;  // tables of the grid
-  LACC *acc = reinterpret_cast<LACC *>(sm_t);                     // [P][tile]
-  // NT tables, each entry with its slope (one 64-bit LDS read per table)
-  float2 *tf = reinterpret_cast<float2 *>(acc + (size_t)P * tile);
-  for (int k = tid; k < P * tile; k += TAN_TILE_T) acc[k] = (LACC)0.;
+  const TanLayer &TL = Ts[job];
+  const int NO = TL.nchan, nsp = NO - 1;
+  const bool hasu = TL.nint > TL.nchan;
   const int t0 = G == 1 ? 0 : (G == 2 ? 3 : 4);
-  for (int k = tid; k < NT * NFMX; k += TAN_TILE_T) {
+  for (int k = tid; k < NT * NFMX; k += TSEG_T) {
     const int a = k / NFMX, r = k - a * NFMX;
     tf[k] = make_float2(tabs[(size_t)(t0 + a) * NFMX + r], tabs[(size_t)(5 + t0 + a) * NFMX + r]);
   }
   __syncthreads();
-  const int64_t T0 = (int64_t)M.loc[blockIdx.x] * tile;
-  const int lo = M.lo[blockIdx.x], hi = M.hi[blockIdx.x];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
+  const int64_t T0 = (int64_t)M.loc[blockIdx.x] * TILE;
+  const int64_t s_lo = T0 + (int64_t)wave * 64 * NS;  // this wave's segment [s_lo, s_lo + 64 NS)
+  const int64_t s_hi = s_lo + 64 * NS - 1;
+  // lines whose extent may reach the segment: centres within the margin
+  const double dvg = G == 1 ? Y.dv : (G == 2 ? Y.dvr2 : Y.dvr3);
+  const double mg = margin[job * 3 + G - 1];
+  const double va = Y.vft0 + (double)s_lo * dvg - mg, vb = Y.vft0 + (double)(s_hi + 1) * dvg + mg;
+  const double *v = vraw + Y.nlo;
+  int l0 = M.lo[blockIdx.x], h0 = M.hi[blockIdx.x];
+  {
+    int l = l0, h = h0;
+    while (l < h) {
+      const int m = (l + h) >> 1;
+      if (v[m] < va) l = m + 1;
+      else h = m;
+    }
+    int l2 = l, h2 = h0;
+    while (l2 < h2) {
+      const int m = (l2 + h2) >> 1;
+      if (v[m] <= vb) l2 = m + 1;
+      else h2 = m;
+    }
+    l0 = l;
+    h0 = l2;
+  }
+  float accT[NS], accU[NS], accS[NS][NSP > 0 ? NSP : 1];
+#pragma unroll
+  for (int sl = 0; sl < NS; sl++) {
+    accT[sl] = accU[sl] = 0.f;
+#pragma unroll
+    for (int c = 0; c < (NSP > 0 ? NSP : 1); c++) accS[sl][c] = 0.f;
+  }
   const int64_t rbase = job * S.line;
   const float ascale = G == 1 ? 1.f : (G == 2 ? 0.25f : 0.0625f);  // DXF1 / DXF_G
-  for (int c0 = lo + wave * 64; c0 < hi; c0 += TAN_TILE_T) {
+  for (int c0 = l0; c0 < h0; c0 += 64) {
+    // lane i holds line c0 + i; the wave walks the lines that reach its
+    // segment, each line's fields broadcast with readlane
     const int li = c0 + lane;
     int npts = 0;
     int64_t glo = 0;
-    double zfl = 0., zslope = 0.;
+    float zfl = 0.f, zslope = 0.f;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     TanLine tn{};
-    if (li < hi) {
+    if (li < h0) {
       const int64_t r = rbase + li;
       if (G == 1) {
         const SRec1 q = recs.r1[r];
         npts = q.npts;
         glo = q.glo;
-        zfl = q.zfl;
-        zslope = q.zslope;
+        zfl = (float)q.zfl;
+        zslope = (float)q.zslope;
         s0 = (float)q.str;
         s1 = (float)q.strd;
         s2 = (float)q.strver;
@@ -239,18 +290,21 @@ __global__ __launch_bounds__(TAN_TILE_T) void k_tan_tile(Recs recs, const TanLin
         const SRecA q = (G == 2 ? recs.r2 : recs.r3)[r];
         npts = q.npts;
         glo = q.glo;
-        zfl = q.zfl;
-        zslope = q.zslope;
+        zfl = (float)q.zfl;
+        zslope = (float)q.zslope;
         s0 = (float)q.str;
       }
+      // lines that miss the segment are not read further
+      if (npts > 0 && (glo + npts - 1 < s_lo || glo > s_hi)) npts = 0;
       if (npts > 0) tn = tl[r];
     }
-    const int nj = hi - c0 < 64 ? hi - c0 : 64;
-    for (int j = 0; j < nj; j++) {
+    uint64_t todo = __ballot(npts > 0);
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
       const int np = __builtin_amdgcn_readlane(npts, j);
-      if (np <= 0) continue;
       const int64_t g0 = rl_i64(glo, j);
-      const double zf0 = rl_f64(zfl, j), zs = rl_f64(zslope, j);
+      const float zf0 = rl_f32(zfl, j), zs = rl_f32(zslope, j);
       const float w0 = rl_f32(s0, j);
       const float w1 = G == 1 ? rl_f32(s1, j) : 0.f, w2 = G == 1 ? rl_f32(s2, j) : 0.f;
       const int q = __builtin_amdgcn_readlane(tn.q, j);
@@ -272,128 +326,146 @@ __global__ __launch_bounds__(TAN_TILE_T) void k_tan_tile(Recs recs, const TanLin
         sd0 = rl_f32(G == 2 ? tn.s.d2 : tn.s.d3, j);
         ud0 = rl_f32(G == 2 ? tn.u.d2 : tn.u.d3, j);
       }
-      LACC *accq = acc + (size_t)q * tile;
-      LACC *accu = acc + (size_t)PU * tile;
-      int64_t a = g0 > T0 ? g0 : T0;
-      int64_t b = g0 + np - 1;
-      if (b > T0 + tile - 1) b = T0 + tile - 1;
-      const int kb = (int)(b - T0), d0 = (int)(T0 - g0);
-      for (int kl = (int)(a - T0) + lane; kl <= kb; kl += 64) {
-        const int off = kl + d0;
-        const double zf = zf0 + (double)(off + 1) * zs;
-        const float u = (float)fabs(zf);
-        int iz = fint(fabs(zf) + 1.5) - 1;
+      // the line's points on the segment, segment-local: [pa, pb]; point
+      // p of the segment is the line's point p + d0
+      const int64_t ga = g0 > s_lo ? g0 : s_lo;
+      const int64_t gb = g0 + np - 1 < s_hi ? g0 + np - 1 : s_hi;
+      const int pa = (int)(ga - s_lo), pb = (int)(gb - s_lo), d0 = (int)(s_lo - g0) + 1;
+#pragma unroll
+      for (int sl = 0; sl < NS; sl++) {
+        if (64 * sl + 63 < pa || 64 * sl > pb) continue;  // wave-uniform
+        const int p = 64 * sl + lane;
+        if (p < pa || p > pb) continue;
+        // table position in FP32: the tangent needs the shape's slope, not
+        // the forward's exact choice of entry at half-way points
+        const float zf = fmaf((float)(p + d0), zs, zf0);
+        const float u = fabsf(zf);
+        int iz = (int)(u + 1.5f) - 1;
         iz = iz > NFMX - 1 ? NFMX - 1 : iz;
         const float2 fe = tf[iz];
         const float f = fe.x;
-        float sl = w0 * fe.y;  // the weights times the slopes
+        float sl2 = w0 * fe.y;  // the weights times the slopes
         float tv = td0 * f, sv = sd0 * f, uv = ud0 * f;
         if (G == 1) {
           const float2 ge = tf[NFMX + iz], xe = tf[2 * NFMX + iz];
-          const float g = ge.x, x = xe.x;
-          sl = fmaf(w1, ge.y, fmaf(w2, xe.y, sl));
-          tv = fmaf(td1, g, fmaf(td2, x, tv));
-          sv = fmaf(sd1, g, fmaf(sd2, x, sv));
-          uv = fmaf(ud1, g, fmaf(ud2, x, uv));
+          sl2 = fmaf(w1, ge.y, fmaf(w2, xe.y, sl2));
+          tv = fmaf(td1, ge.x, fmaf(td2, xe.x, tv));
+          sv = fmaf(sd1, ge.x, fmaf(sd2, xe.x, sv));
+          uv = fmaf(ud1, ge.x, fmaf(ud2, xe.x, uv));
         }
-        const float sgn = zf < 0. ? -1.f : 1.f;
-        tv = fmaf(-sl, fmaf(ta, sgn, tb * u), tv);
-        atomicAdd(&acc[kl], (LACC)tv);
-        if (q > 0) atomicAdd(&accq[kl], (LACC)fmaf(-sl, sb * u, sv));
-        if (PU > 0) atomicAdd(&accu[kl], (LACC)fmaf(-sl, ub * u, uv));
+        const float sgn = zf < 0.f ? -1.f : 1.f;
+        accT[sl] += fmaf(-sl2, fmaf(ta, sgn, tb * u), tv);
+        if (hasu) accU[sl] += fmaf(-sl2, ub * u, uv);
+#pragma unroll
+        for (int c = 0; c < NSP; c++)
+          if (c + 1 == q) accS[sl][c] += fmaf(-sl2, sb * u, sv);
       }
     }
   }
-  __syncthreads();
+  // the combine, point by point
   const double X00 = -7. / 128., X01 = 105. / 128., X02 = 35. / 128., X03 = -5. / 128.;
   const double X10 = -1. / 16., X11 = 9. / 16.;
   const DevPanel *pan = pans + Y.pan0;
-  const double tave = Ts[job].tave;
-  if (G != 1) {
-    for (int k = tid; k < P * tile; k += TAN_TILE_T) {
-      const int qc = k / tile, kk = k - qc * tile;
-      const int64_t g = T0 + kk;
-      double v = (double)acc[k];
-      if (G == 3) {
-        if (g >= Y.n3) continue;
-        if (g <= Y.g3max) {
-          int p = g < 155 ? 0 : (int)((g - 5) / 150);
-          if (p > Y.npanel - 1) p = Y.npanel - 1;
-          const int I = (int)(g - (int64_t)150 * p) + 1;
-          const int n1r3 = p == 0 ? 1 : 6;
-          const DevPanel pp = pan[p];
-          const int w = Y.win0 + pp.w;
+  const int P = TL.nint;
+  const int PU = hasu ? NO : -1;
+#pragma unroll
+  for (int sl = 0; sl < NS; sl++) {
+    const int64_t g = s_lo + 64 * sl + lane;
+    double vq[TAN_NQ];
+    vq[0] = (double)accT[sl];
+#pragma unroll
+    for (int c = 1; c < TAN_NQ; c++) {
+      double x = 0.;
+      if (c - 1 < NSP && c <= nsp) x = (double)accS[sl][(c - 1 < NSP ? c - 1 : 0)];
+      if (c == PU) x = (double)accU[sl];
+      vq[c] = x;
+    }
+    if (G == 3) {
+      if (g >= Y.n3) continue;
+      int p = 0, I = 0, n1r3 = 1, w = 0;
+      double vft = 0.;
+      const bool inx = g <= Y.g3max;
+      if (inx) {
+        p = g < 155 ? 0 : (int)((g - 5) / 150);
+        if (p > Y.npanel - 1) p = Y.npanel - 1;
+        I = (int)(g - (int64_t)150 * p) + 1;
+        n1r3 = p == 0 ? 1 : 6;
+        w = Y.win0 + pan[p].w;
+        vft = pan[p].vft;
+      }
+#pragma unroll
+      for (int qc = 0; qc < TAN_NQ; qc++) {
+        if (qc >= P) break;
+        double x = vq[qc];
+        if (inx) {
           const DevWin W = wins[w];
-          v = v + xint_at(W.v1r4, W.v2r4, Y.dvr4, r4c + ((size_t)w * P + qc) * R4ROW, pp.vft, Y.dvr3, I, n1r3, 155);
-          // the continuum's channels are its own central differences (U: none)
-          if (Y.icntnm >= 1 && qc < Ts[job].nchan)
-            v = v + xint_at(Y.v1abs, Y.v2abs, Y.dvabs, absc + ((size_t)job * Ts[job].nchan + qc) * abs_stride,
-                            pp.vft, Y.dvr3, I, n1r3, 155);
+          x = x + xint_at(W.v1r4, W.v2r4, Y.dvr4, r4c + ((size_t)w * P + qc) * R4ROW, vft, Y.dvr3, I, n1r3, 155);
+          if (Y.icntnm >= 1 && qc < NO)
+            x = x + xint_at(Y.v1abs, Y.v2abs, Y.dvabs, absc + ((size_t)job * NO + qc) * abs_stride, vft, Y.dvr3, I,
+                            n1r3, 155);
         }
-        rout[qc * sout + Y.r3o + g] = v;
-      } else {
-        if (g >= Y.n2) continue;
-        if (g >= 4 && g <= Y.g2max) {
+        rout[qc * sout + Y.r3o + g] = x;
+      }
+    } else if (G == 2) {
+      if (g >= Y.n2) continue;
+      const bool inl = g >= 4 && g <= Y.g2max;
+      const int64_t q4 = g >> 2;
+      const int ph = (int)(g & 3);
+#pragma unroll
+      for (int qc = 0; qc < TAN_NQ; qc++) {
+        if (qc >= P) break;
+        double x = vq[qc];
+        if (inl) {
           const double *r3 = rin + qc * sin + Y.r3o;
-          const int64_t q4 = g >> 2;
-          const int ph = (int)(g & 3);
           const double a0 = r3[q4 - 1], a1 = r3[q4], a2 = r3[q4 + 1], a3 = r3[q4 + 2];
-          if (ph == 0) v = v + a1;
-          else if (ph == 1) v = v + X00 * a0 + X01 * a1 + X02 * a2 + X03 * a3;
-          else if (ph == 2) v = v + X10 * (a0 + a3) + X11 * (a1 + a2);
-          else v = v + X03 * a0 + X02 * a1 + X01 * a2 + X00 * a3;
+          if (ph == 0) x = x + a1;
+          else if (ph == 1) x = x + X00 * a0 + X01 * a1 + X02 * a2 + X03 * a3;
+          else if (ph == 2) x = x + X10 * (a0 + a3) + X11 * (a1 + a2);
+          else x = x + X03 * a0 + X02 * a1 + X01 * a2 + X00 * a3;
         }
-        rout[qc * sout + Y.r2o + g] = v;
+        rout[qc * sout + Y.r2o + g] = x;
       }
-    }
-    return;
-  }
-  // G = 1: every internal channel of a point, then the output channels
-  // (species q: own - W_q/W_tot U)
-  const TanLayer &TL = Ts[job];
-  const int NO = TL.nchan;
-  for (int kk = tid; kk < tile; kk += TAN_TILE_T) {
-    const int64_t g = T0 + kk;
-    if (g < 32 || g > Y.g_last) continue;
-    const int64_t q4 = g >> 2;
-    const int ph = (int)(g & 3);
-    double v[TAN_NQ];
+    } else {
+      if (g < 32 || g > Y.g_last) continue;
+      const int64_t q4 = g >> 2;
+      const int ph = (int)(g & 3);
 #pragma unroll
-    for (int qc = 0; qc < TAN_NQ; qc++) {
-      if (qc >= P) break;
-      const double *r2 = rin + qc * sin + Y.r2o;
-      double x = (double)acc[(size_t)qc * tile + kk];
-      const double a0 = r2[q4 - 1], a1 = r2[q4], a2 = r2[q4 + 1], a3 = r2[q4 + 2];
-      if (ph == 0) x = x + a1;
-      else if (ph == 1) x = x + X00 * a0 + X01 * a1 + X02 * a2 + X03 * a3;
-      else if (ph == 2) x = x + X10 * (a0 + a3) + X11 * (a1 + a2);
-      else x = x + X03 * a0 + X02 * a1 + X01 * a2 + X00 * a3;
-      v[qc] = x;
-    }
-    double rad = 1., dlr = 0.;
-    if (Y.jrad == 0) {
-      int p = (int)((g - 32) / 2400);
-      if (p > Y.npanel - 1) p = Y.npanel - 1;
-      const DevPanel pp = pan[p];
-      const int I = (int)(g - (int64_t)2400 * p) + 1;
-      int lo2 = 0, hi2 = pp.nseg - 1;
-      while (lo2 < hi2) {
-        const int mid = (lo2 + hi2 + 1) >> 1;
-        if (segs[Y.seg0 + pp.seg0 + mid].i1 <= I) lo2 = mid;
-        else hi2 = mid - 1;
+      for (int qc = 0; qc < TAN_NQ; qc++) {
+        if (qc >= P) break;
+        const double *r2 = rin + qc * sin + Y.r2o;
+        const double a0 = r2[q4 - 1], a1 = r2[q4], a2 = r2[q4 + 1], a3 = r2[q4 + 2];
+        double x = vq[qc];
+        if (ph == 0) x = x + a1;
+        else if (ph == 1) x = x + X00 * a0 + X01 * a1 + X02 * a2 + X03 * a3;
+        else if (ph == 2) x = x + X10 * (a0 + a3) + X11 * (a1 + a2);
+        else x = x + X03 * a0 + X02 * a1 + X01 * a2 + X00 * a3;
+        vq[qc] = x;
       }
-      const DevSeg sg = segs[Y.seg0 + pp.seg0 + lo2];
-      rad = sg.radvi + (double)(I - sg.i1) * sg.rdel;
-      dlr = d_dlnradfn_dT(pp.vft + (double)(I - 1) * Y.dv, Y.xkt, tave);
-    }
+      double rad = 1., dlr = 0.;
+      if (Y.jrad == 0) {
+        int p = (int)((g - 32) / 2400);
+        if (p > Y.npanel - 1) p = Y.npanel - 1;
+        const DevPanel pp = pan[p];
+        const int I = (int)(g - (int64_t)2400 * p) + 1;
+        int lo2 = 0, hi2 = pp.nseg - 1;
+        while (lo2 < hi2) {
+          const int mid = (lo2 + hi2 + 1) >> 1;
+          if (segs[Y.seg0 + pp.seg0 + mid].i1 <= I) lo2 = mid;
+          else hi2 = mid - 1;
+        }
+        const DevSeg sg = segs[Y.seg0 + pp.seg0 + lo2];
+        rad = sg.radvi + (double)(I - sg.i1) * sg.rdel;
+        dlr = d_dlnradfn_dT(pp.vft + (double)(I - 1) * Y.dv, Y.xkt, TL.tave);
+      }
 #pragma unroll
-    for (int qc = 0; qc < TAN_NQ; qc++) {
-      if (qc >= NO) break;
-      double x = v[qc];
-      if (qc > 0 && PU > 0) x = x - TL.wq[qc] * v[PU];
-      x = x * rad;
-      if (qc == 0 && Y.jrad == 0) x = x + (double)od[Y.od0 + g - 32] * dlr;
-      odc[odcoff[job * NO + qc] + g - 32] = (OUT)x;
+      for (int qc = 0; qc < TAN_NQ; qc++) {
+        if (qc >= NO) break;
+        double x = vq[qc];
+        if (qc > 0 && PU > 0) x = x - TL.wq[qc] * vq[PU];
+        x = x * rad;
+        if (qc == 0 && Y.jrad == 0) x = x + (double)od[Y.od0 + g - 32] * dlr;
+        odc[odcoff[job * NO + qc] + g - 32] = (OUT)x;
+      }
     }
   }
 }
