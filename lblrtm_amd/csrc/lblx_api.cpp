@@ -283,12 +283,12 @@ struct lblx_ctx {
   DevBuf w_cps;   // ContPlan per job
   // work buffers
   DevBuf w_dlay, w_pan, w_win, w_word, w_seg, w_r4, w_margin;
-  DevBuf w_cx, w_csp, w_cal, w_cad, w_cco2, w_pn, w_pc, w_g, w_jmp, w_wsum, w_bcnt, w_etab, w_jobi,
+  DevBuf w_cx, w_csp, w_cal, w_cad, w_cfl, w_g, w_jmp, w_wsum, w_bcnt, w_etab, w_jobi,
       w_bat;
   DevBuf w_t9vnu, w_t9sp, w_t9alfa, w_t9epp, w_t9mol, w_t9n, w_t9blk, w_t9bvhi;
   DevBuf w_dirty;  // blocks whose read panel moved in the last fixpoint step
   DevBuf w_t9s;    // TAPE9-block start tables of the batch walk (six ints per block)
-  DevBuf w_near, w_keep, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
+  DevBuf w_near, w_keep, w_kidx, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
   DevBuf w_tmap[3];
   DevBuf w_mpan, w_rt, w_mrun;
   std::shared_ptr<void> jprep;            // host job tables of run_jobs (JobsPrep)
@@ -444,10 +444,10 @@ void lblx_destroy(lblx_ctx *c) {
   DevBuf *bufs[] = {&c->d_vnu, &c->d_sp, &c->d_alfa, &c->d_epp, &c->d_hwhm, &c->d_tmpalf,
                     &c->d_pshift, &c->d_mol, &c->d_blk, &c->d_bstart, &c->d_bvmax, &c->d_tabs, &c->d_cont, &c->w_cps,
                     &c->w_dlay, &c->w_pan, &c->w_win, &c->w_word, &c->w_seg, &c->w_r4, &c->w_margin,
-                    &c->w_cx, &c->w_csp, &c->w_cal, &c->w_cad, &c->w_cco2, &c->w_pn, &c->w_pc,
+                    &c->w_cx, &c->w_csp, &c->w_cal, &c->w_cad, &c->w_cfl,
                     &c->w_g, &c->w_jmp, &c->w_wsum, &c->w_bcnt, &c->w_etab, &c->w_t9s, &c->w_jobi, &c->w_bat,
                     &c->w_t9vnu, &c->w_t9sp, &c->w_t9alfa, &c->w_t9epp, &c->w_t9mol, &c->w_t9n,
-                    &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_blkmax, &c->w_pread,
+                    &c->w_t9blk, &c->w_t9bvhi, &c->w_near, &c->w_keep, &c->w_kidx, &c->w_blkmax, &c->w_pread,
                     &c->w_changed, &c->w_rec, &c->w_r3, &c->w_r2, &c->w_cnt, &c->w_od,
                     &c->w_absrb, &c->w_tmap[0], &c->w_tmap[1], &c->w_tmap[2], &c->w_mpan, &c->w_rt, &c->w_mrun, &c->s_z[0], &c->s_z[1], &c->s_tw, &c->s_g, &c->s_ip, &c->s_cof, &c->j_rall, &c->j_a, &c->j_b, &c->b_jobs, &c->b_rt, &c->b_rl,
                     &c->w_dirty, &c->d_vtab, &c->d_vtabf, &c->w_dvo,
@@ -1227,7 +1227,8 @@ static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, c
   double *r4c = c->t_r4c.as<double>();
   int th = c->tbegin(ST_TANGENT);
   if (J.maxnn > 0)
-    hipLaunchKernelGGL(k_near_tan, dim3((J.maxnn + 255) / 256, njobs), dim3(256), 0, s, L, dY, dT, S, tl);
+    hipLaunchKernelGGL(k_near_tan, dim3((J.maxnn + 255) / 256, njobs), dim3(256), 0, s, L, dY, dT, S,
+                       (const int *)c->w_keep.p, (const int32_t *)c->w_kidx.p, tl);
   const int lgrid = (int)((J.maxline + 1 + 255) / 256);
   MemTan *mt = c->t_mem.as<MemTan>();
   int8_t *mq = c->t_memq.as<int8_t>();
@@ -1278,7 +1279,7 @@ static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, c
     const int T = tw[G - 1];
     const int nt = (int)(tmv[G - 1].size() / 2);
     hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY, tm[G - 1],
-                       nt, T, G, c->w_margin.as<double>());
+                       nt, T, G, c->w_margin.as<double>(), (const int32_t *)nullptr, S);
     const int nsp = PO - 1;
     const int NSPsel = nsp == 0 ? 0 : (nsp <= 3 ? 3 : 7);
     const size_t shm = tabb[G - 1];
@@ -1289,7 +1290,8 @@ static int tangent_launch(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, c
     HIPCHK(hipFuncSetAttribute((const void *)k_tan_seg<GG, OUTT, NSPV>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
                                (int)shm));                                                                         \
     hipLaunchKernelGGL((k_tan_seg<GG, OUTT, NSPV>), dim3(nt), dim3(TSEG_T), shm, s, recs, (const TanLine *)tl,       \
-                       tm[GG - 1], vr, mgn, tabs, dY, dT, S, dp, dw, (const double *)r4c, absc, abs_stride, dg,    \
+                       tm[GG - 1], vr, (const int32_t *)c->w_kidx.p, mgn, tabs, dY, dT, S, dp, dw,                 \
+                       (const double *)r4c, absc, abs_stride, dg,                                                  \
                        (const double *)(RIN), (int64_t)(SIN), (double *)(ROUT), (int64_t)(SOUT),                   \
                        (const OUTT *)od_base, (OUTT *)R.odc, odcoff);                                              \
   } while (0)
@@ -1350,16 +1352,15 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_r4, (size_t)nwin * R4ROW * 8) ||
       need(c->w_cx, NJ * S.line * 8) || need(c->w_csp, NJ * S.line * 8) ||
       need(c->w_cal, NJ * S.line * 8) || need(c->w_cad, NJ * S.line * 8) ||
-      need(c->w_cco2, NJ * S.line) || need(c->w_pn, NJ * S.line * 4) ||
-      need(c->w_pc, NJ * S.line * 4) || need(c->w_g, NJ * S.line * 4) ||
+      need(c->w_cfl, NJ * S.line) || need(c->w_g, NJ * S.line * 4) ||
       need(c->w_jmp, NJ * S.line * 4 * 3) || need(c->w_wsum, NJ * S.line * 2) ||
-      need(c->w_bcnt, NJ * S.blk * 4 * 7) || need(c->w_etab, NJ * S.blk * 4 * (EBLK + 1)) || need(c->w_t9s, NJ * S.blk * 4 * 6) ||
+      need(c->w_bcnt, NJ * S.blk * 4 * 3) || need(c->w_etab, NJ * S.blk * 4 * (EBLK + 1)) || need(c->w_t9s, NJ * S.blk * 4 * 6) ||
       need(c->w_jobi, NJ * 4 * 4) || need(c->w_bat, NJ * S.line * 4 * 3) ||
       need(c->w_t9vnu, NJ * S.line * 8) || need(c->w_t9sp, NJ * S.line * 8) ||
       need(c->w_t9alfa, NJ * S.line * 8) || need(c->w_t9epp, NJ * S.line * 8) ||
       need(c->w_t9mol, NJ * S.line * 4) || need(c->w_t9n, NJ * 8) ||
       need(c->w_t9blk, NJ * S.t9blk * 16) || need(c->w_t9bvhi, NJ * S.t9blk * 8) ||
-      need(c->w_near, NJ * S.line * (sizeof(NearHot) + sizeof(NearCold))) || need(c->w_keep, NJ * S.line * 4) ||
+      need(c->w_near, NJ * S.line * (sizeof(NearHot) + sizeof(NearCold))) || need(c->w_keep, NJ * S.line * 4) || need(c->w_kidx, (NJ * S.line + 2 * NJ * S.blk) * 4) ||
       need(c->w_blkmax, NJ * S.blk * 4) || need(c->w_pread, NJ * S.blk * 4) ||
       need(c->w_dirty, NJ * S.blk * 4) ||
       need(c->w_changed, 64 * 4) || need(c->w_rec, NJ * S.line * (sizeof(SRec1) + 2 * sizeof(SRecA) + (voigt ? sizeof(SRecV) : 0))) ||
@@ -1416,9 +1417,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   W.csp = c->w_csp.as<double>();
   W.cal = c->w_cal.as<double>();
   W.cad = c->w_cad.as<double>();
-  W.cco2 = c->w_cco2.as<int8_t>();
-  W.pn = c->w_pn.as<int32_t>();
-  W.pc = c->w_pc.as<int32_t>();
+  W.cfl = c->w_cfl.as<int8_t>();
   W.g = c->w_g.as<int32_t>();
   W.jmp = c->w_jmp.as<int32_t>();
   W.grp = W.jmp + NJ * S.line;
@@ -1426,12 +1425,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   W.wsum = c->w_wsum.as<int16_t>();
   int32_t *bb = c->w_bcnt.as<int32_t>();
   W.bcnt = bb;
-  W.bcntn = bb + NJ * S.blk;
-  W.bcntc = bb + 2 * NJ * S.blk;
-  W.bstop = bb + 3 * NJ * S.blk;
-  W.bp = bb + 4 * NJ * S.blk;
-  W.bpn = bb + 5 * NJ * S.blk;
-  W.bpc = bb + 6 * NJ * S.blk;
+  W.bstop = bb + NJ * S.blk;
+  W.bp = bb + 2 * NJ * S.blk;
   int32_t *ji = c->w_jobi.as<int32_t>();
   W.npass = ji;
   W.stop = ji + NJ;
@@ -1464,9 +1459,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   unsigned long long *cnt = c->w_cnt.as<unsigned long long>();  // [3][njobs]: k1, k3, lines
   hipLaunchKernelGGL(k_fill_i32, dim3((6 * NJ + 255) / 256), dim3(256), 0, s, (int *)cnt, 6 * NJ, 0);
 
-  NearArr nearr;
-  nearr.hot = c->w_near.as<NearHot>();
-  nearr.cold = (NearCold *)(c->w_near.as<char>() + (size_t)NJ * S.line * sizeof(NearHot));
+  NearHot *nearh = c->w_near.as<NearHot>();
+  NearCold *nearc = (NearCold *)(c->w_near.as<char>() + (size_t)NJ * S.line * sizeof(NearHot));
   // 1-2. LINF4 + SHRINK
   int th = c->tbegin(ST_SHRINK);
   const int lgrid = (int)((maxline + 1 + 255) / 256);
@@ -1474,7 +1468,6 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   hipLaunchKernelGGL(k_shr_count, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
   hipLaunchKernelGGL(k_shr_scan, dim3(njobs), dim3(256), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_compact, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
-  hipLaunchKernelGGL(k_shr_chain, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_ewalk, dim3(bgrid, njobs), dim3(64), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_t9sim, dim3((bgrid + 63) / 64, njobs), dim3(64), 0, s, dY, S, W, c->t9_all_wave);
   hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s, W.t9k, NJ * S.blk, -1);
@@ -1501,7 +1494,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     const int ngrid = (maxnn + 255) / 256;
     int thl = c->tbegin(ST_LNCOR1, s2);
     hipLaunchKernelGGL(k_lncor1, dim3(ngrid, njobs), dim3(256), 0, s2, L, dY, S,
-                       c->w_pan.as<DevPanel>(), nearr);
+                       c->w_pan.as<DevPanel>(), nearh, nearc);
     hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s2,
                        c->w_pread.as<int>(), NJ * S.blk, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3((int)((NJ * S.blk + 255) / 256)), dim3(256), 0, s2,
@@ -1556,7 +1549,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     const int maxpass = c->fix_maxpass;
     for (int it = 0; it < maxpass; it++) {
       const int *skip = it >= 2 ? ch + it - 1 : nullptr;
-      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, (const NearHot *)nearr.hot, L, dY,
+      hipLaunchKernelGGL(k_reject, dim3(ngrid, njobs), dim3(256), 0, s, (const NearHot *)nearh, L, dY,
                          S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
                          c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>(),
                          c->w_dirty.as<int>(), skip);
@@ -1578,7 +1571,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     // between passes): solve the recurrence block by block instead.
     if (!converged) {
       passes = maxpass;
-      hipLaunchKernelGGL(k_reject_seq, dim3(njobs), dim3(64), 0, s, (const NearHot *)nearr.hot, L, dY, S,
+      hipLaunchKernelGGL(k_reject_seq, dim3(njobs), dim3(64), 0, s, (const NearHot *)nearh, L, dY, S,
                          c->w_pread.as<int>(), c->w_pan.as<DevPanel>(), c->w_win.as<DevWin>(),
                          c->w_r4.as<double>(), c->w_keep.as<int>(), c->w_blkmax.as<int>());
     }
@@ -1587,12 +1580,13 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     c->tend(th);
     // 6. CNVFNV scatter records
     th = c->tbegin(ST_SCATTER);
-    hipLaunchKernelGGL(k_scatter_rec, dim3(maxnbk, njobs), dim3(256), 0, s, nearr,
-                       c->w_keep.as<int>(), L, dY, S, c->w_pread.as<int>(), c->w_pan.as<DevPanel>(),
-                       recs);
-    if (st)  // TAPE6 counters only
-      hipLaunchKernelGGL(k_sum_npts, dim3(ngrid, njobs), dim3(256), 0, s, dY, S, (const SRec1 *)recs.r1,
-                         cnt, cnt + 2 * NJ);
+    int32_t *kidx = c->w_kidx.as<int32_t>(), *bkc = kidx + NJ * S.line, *bkp = bkc + NJ * S.blk;
+    hipLaunchKernelGGL(k_keep_blk, dim3(maxnbk, njobs), dim3(256), 0, s, dY, L, S, c->w_keep.as<int>(), bkc);
+    hipLaunchKernelGGL(k_keep_scan, dim3(njobs), dim3(256), 0, s, dY, S, (const int32_t *)bkc, bkp, kidx);
+    hipLaunchKernelGGL(k_scatter_rec, dim3(maxnbk, njobs), dim3(256), 0, s, (const NearHot *)nearh,
+                       (const NearCold *)nearc, c->w_keep.as<int>(), L, dY, S, c->w_pread.as<int>(),
+                       c->w_pan.as<DevPanel>(), (const int32_t *)bkp, kidx, recs, st ? cnt : nullptr,
+                       cnt + 2 * NJ);  // TAPE6 counters with st
     c->tend(th);
   }
   // 7. tiles: R3, R2, R1 (+ PANEL combine)
@@ -1603,7 +1597,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     const int nt = (int)tjob[G - 1].size();
     int tht = c->tbegin(G == 3 ? ST_TILE3 : (G == 2 ? ST_TILE2 : ST_TILE1));
     hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY,
-                       tm[G - 1], nt, T, G, c->w_margin.as<double>());
+                       tm[G - 1], nt, T, G, c->w_margin.as<double>(), (const int32_t *)c->w_kidx.p, S);
     const size_t esz = split ? sizeof(float) : sizeof(double);
     // FP32 shapes: accumulator in FP64 LDS for the grids in LBLX_FP32_LDS64
     const bool l64 = !split || ((LBLX_FP32_LDS64 >> G) & 1);
@@ -3014,8 +3008,8 @@ int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, con
 // about nline records.
 static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
   const size_t line = ((size_t)nline + 127) / 64 * 64;
-  return line * (32 + 1 + 12 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(SRec1) + 2 * sizeof(SRecA) + sizeof(SRecV)) +
-         (size_t)(c->nblk + 2) * 4 * (7 + EBLK + 1 + 3 + 6) + (size_t)(c->nlines / 1000 + 16) * 24;
+  return line * (32 + 1 + 4 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(SRec1) + 2 * sizeof(SRecA) + sizeof(SRecV)) +
+         (size_t)(c->nblk + 2) * 4 * (3 + EBLK + 1 + 3 + 6) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
 
 // Many profiles in one call: OPDPTH for the layers of a group of profiles

@@ -73,14 +73,13 @@ struct Strides {
 // SHRINK work arrays (compacted passing lines of LINF4 and the chain tables).
 struct ShrinkWork {
   double *cx, *csp, *cal, *cad;  // centre, strength (0 below 1e-35), widths
-  int8_t *cco2;
-  int32_t *pn, *pc;       // [n+1] exclusive prefix of positive non-CO2 / CO2 lines
+  int8_t *cfl;            // bit 0: CO2, bit 1: positive strength
   int32_t *g;             // first-group end from each start
   int32_t *jmp;           // [line] f: next group end after a group end
   int16_t *wsum;          // [line] lines the group after a group end emits
   int32_t *grp, *gout;    // [output slot] first and last line of the group emitted there (grp -1: a group's second slot)
-  int32_t *bcnt, *bcntn, *bcntc, *bstop;  // per TAPE3 block
-  int32_t *bp, *bpn, *bpc;                // block prefixes
+  int32_t *bcnt, *bstop;  // per TAPE3 block: passing lines, stop line
+  int32_t *bp;            // block prefix of passing lines
   int32_t *npass, *stop;                  // per job
   int32_t *etab;          // [blk][EBLK] emitted lines of a batch of k+1 blocks
   int32_t *estop;         // [blk] emitted lines of the batch from the block to the stop line (-1: not tabled)
@@ -101,51 +100,19 @@ struct Tape9 {
   double *bvhi;
 };
 
-// Near-line record after LNCOR1 (one per TAPE3 record in the near range).
-struct NearLine {
-  double vnu;      // shifted centre
-  double recalf;   // 1/ALFV
-  double sp;       // corrected strength (0 = skipped)
-  double speak;    // SUI * RECALF, rejection peak
-  double zetai;
-  int32_t izeta;
-  int32_t ff;      // first panel whose buffer holds the line (firstfit)
-  int32_t keep;    // survives every rejection test
-  int32_t proc;    // panel in which CNVFNV processes it
-};
-
-// NearLine stored as two arrays: the fields every rejection pass reads,
-// and the ones only the scatter records need.
+// Near-line record after LNCOR1 (one per TAPE3 record in the near range)
+// as two arrays: the fields every rejection pass reads, and the ones only
+// the scatter records of the surviving lines need.
 struct NearHot {
-  double vnu, recalf, speak;
-  int32_t ff, keep;
+  double vnu;     // shifted centre
+  double recalf;  // 1/ALFV
+  double speak;   // SUI * RECALF, rejection peak
+  int32_t ff;     // first panel whose buffer holds the line (firstfit)
+  int32_t keep;   // LNCOR1 keeps the record (not skipped)
 };
 struct NearCold {
-  double sp, zetai;
-  int32_t izeta, proc;
-};
-struct NearArr {
-  NearHot *hot;
-  NearCold *cold;
-  __device__ void put(int64_t i, const NearLine &o) const {
-    hot[i] = NearHot{o.vnu, o.recalf, o.speak, o.ff, o.keep};
-    cold[i] = NearCold{o.sp, o.zetai, o.izeta, o.proc};
-  }
-  __device__ NearLine get(int64_t i) const {
-    const NearHot h = hot[i];
-    const NearCold c = cold[i];
-    NearLine o;
-    o.vnu = h.vnu;
-    o.recalf = h.recalf;
-    o.speak = h.speak;
-    o.ff = h.ff;
-    o.keep = h.keep;
-    o.sp = c.sp;
-    o.zetai = c.zetai;
-    o.izeta = c.izeta;
-    o.proc = c.proc;
-    return o;
-  }
+  double sp, zetai;  // corrected strength, zeta
+  int32_t izeta, pad;
 };
 
 // Scatter records of the near lines, one array per grid so that each tile
@@ -153,7 +120,8 @@ struct NearArr {
 // weights), the R2 and R3 records, and the exact-Voigt build's zeta
 // column and fraction (all grids).  glo is the line's first point on the
 // grid (G2 = G1 - j2shift - 1800 p, G3 = G1 - j3shift - 2250 p); npts = 0
-// means no contribution.
+// means no contribution.  Only the lines that survive the rejection tests
+// have records, stored contiguously in file order.
 struct SRec1 {
   double zfl, zslope, str, strd, strver;
   int32_t glo, npts;

@@ -3,7 +3,7 @@
 // The reference computes one layer panel by panel on one core (HIRAC1,
 // src/oprop.f90:25-668).  Here every layer ("job") of a batch is resident
 // in HBM over its whole band and each stage is one launch for all jobs:
-//   LINF4 + SHRINK   k_shr_count, k_shr_scan, k_shr_compact, k_shr_chain,
+//   LINF4 + SHRINK   k_shr_count, k_shr_scan, k_shr_compact,
 //                    k_shr_ewalk, k_shr_t9sim/t9chain/t9replay, k_shr_bwalk,
 //                    k_shr_emit
 //                    (oprop.f90:3093-3290, 3508-3640)
@@ -87,17 +87,28 @@ struct F4Line {
   int co2;
 };
 
-__device__ __forceinline__ bool linf4_line(const DevLines &L, const DevLayer &Y, int i, F4Line &o) {
-  int b = L.blk[i];
-  int mc = L.mol[i];
-  int m = mc % 100, iso = (mc % 1000) / 100;
-  double vraw = L.vnu[i];
+// LINF4's tests alone: whether record i enters SHRINK, and its shifted
+// centre.  linf4_line applies the same tests.
+__device__ __forceinline__ bool linf4_pass(const DevLines &L, const DevLayer &Y, int i, double &x) {
+  const int b = L.blk[i];
+  const int mc = L.mol[i];
+  const int m = mc % 100, iso = (mc % 1000) / 100;
+  const double vraw = L.vnu[i];
   if (L.bvmax[b] < Y.v1l4 || m < 1 || m > Y.nmol || iso < 1 || iso > c_isomax[m - 1]) return false;
-  double sui = (double)L.sp[i] * Y.wk[m - 1];
+  const double sui = (double)L.sp[i] * Y.wk[m - 1];
   if (sui == 0. || vraw < Y.v1l4) return false;
+  x = vraw + Y.rhorat * (double)L.pshift[i];
+  return true;
+}
+
+__device__ __forceinline__ bool linf4_line(const DevLines &L, const DevLayer &Y, int i, F4Line &o) {
+  double x;
+  if (!linf4_pass(L, Y, i, x)) return false;
+  const int mc = L.mol[i];
+  const int m = mc % 100, iso = (mc % 1000) / 100;
+  double sui = (double)L.sp[i] * Y.wk[m - 1];
   double alfa = L.alfa[i], hwhm = L.hwhm[i];
   if (m == 7) alfa = (alfa - 0.21 * hwhm) / (1.0 - 0.21);
-  double x = vraw + Y.rhorat * (double)L.pshift[i];
   double epp = L.epp[i];
   if (epp <= -1.001) epp = fabs(epp);
   if (epp > -0.999) sui = sui * Y.scor[m - 1][iso - 1] * exp(-epp * Y.betacr4) * (1. + exp(-x * Y.beta4));
@@ -125,22 +136,29 @@ __device__ __forceinline__ bool linf4_line(const DevLines &L, const DevLayer &Y,
 //     f(e) = (first k >= e+2 with x_k > x_e + DVR4/2) - 1,
 //     g(s) = (first k >= s+1 with x_k > x_s + DVR4/2) - 1,
 // cut at the batch end.  A group emits one non-CO2 and one CO2 effective
-// line if it holds positive strength of that class.  f and the count each
-// step emits (jmp, wsum) are tabled per line; a thread per TAPE3 block walks
-// the chain from the block's first line through the next EBLK blocks and
-// tables the count a batch of those blocks would emit; one thread per job
-// then walks LINF4's batch boundaries over those tables, and one thread per
-// batch walks its chain once more to mark each group's last line with the
-// group's first line and output slot.
+// line if it holds positive strength of that class.  g, f and the count
+// each step emits (jmp, wsum) are tabled per line as the passing lines are
+// compacted; a wave per TAPE3 block walks the chain from the block's first
+// line through the next EBLK blocks and tables the count a batch of those
+// blocks would emit; one wave per job then walks LINF4's batch boundaries
+// over those tables, and one wave per batch walks its chain once more to
+// mark each group's last line with the group's first line and output slot.
 constexpr int SB_T = 256;
 
-__device__ __forceinline__ int em_cnt(const int32_t *pn, const int32_t *pc, int a, int b) {
-  return (pn[b + 1] - pn[a] > 0 ? 1 : 0) + (pc[b + 1] - pc[a] > 0 ? 1 : 0);
+// Per passing line, cfl: bit 0 CO2, bit 1 positive strength.  The lines a
+// group [a, b] emits: one per class with a positive line.
+__device__ __forceinline__ int em_bits(int f) { return (f & 2) ? 1 << (f & 1) : 0; }
+__device__ __forceinline__ int em_of(int m) { return (m & 1) + (m >> 1); }
+// the same over [a, b] by a whole wave (a, b wave-uniform)
+__device__ __forceinline__ int em_cnt(const int8_t *cfl, int a, int b) {
+  int m = 0;
+  for (int k = a + (int)(threadIdx.x & 63); k <= b; k += 64) m |= em_bits(cfl[k]);
+  return (__ballot(m & 1) ? 1 : 0) + (__ballot(m & 2) ? 1 : 0);
 }
 
-// per TAPE3 block: passing lines, positive non-CO2 / CO2 lines, and the
-// block-local position of the first passing line above V2L4 (LINF4 stops
-// there with IEOF = 1, oprop.f90:3228-3231)
+// per TAPE3 block: passing lines, and the block-local position of the
+// first passing line above V2L4 (LINF4 stops there with IEOF = 1,
+// oprop.f90:3228-3231)
 __global__ __launch_bounds__(SB_T) void k_shr_count(DevLines L, const DevLayer *Ys, Strides S,
                                                     ShrinkWork W) {
   __shared__ int buf[SB_T];
@@ -151,143 +169,181 @@ __global__ __launch_bounds__(SB_T) void k_shr_count(DevLines L, const DevLayer *
   if (bl >= Y.nb4) return;
   const int b = Y.b4 + bl;
   int i = L.bstart[b] + threadIdx.x;
-  int pass = 0, pn = 0, pc = 0, stp = 0;
+  int pass = 0, stp = 0;
   if (i < L.bstart[b + 1]) {
-    F4Line o;
-    if (linf4_line(L, Y, i, o)) {
+    double x;
+    if (linf4_pass(L, Y, i, x)) {
       pass = 1;
-      pn = (o.sp > 0. && !o.co2);
-      pc = (o.sp > 0. && o.co2);
-      stp = o.x > Y.v2l4;
+      stp = x > Y.v2l4;
     }
   }
   if (threadIdx.x == 0) s_stop = INT_MAX;
-  int tot, tn, tc;
+  int tot;
   int pos = block_scan_excl<SB_T>(pass, buf, &tot);
-  block_scan_excl<SB_T>(pn, buf, &tn);
-  block_scan_excl<SB_T>(pc, buf, &tc);
   if (stp) atomicMin(&s_stop, pos);
   __syncthreads();
   if (threadIdx.x == 0) {
     int64_t o = job * S.blk + bl;
     W.bcnt[o] = tot;
-    W.bcntn[o] = tn;
-    W.bcntc[o] = tc;
     W.bstop[o] = s_stop == INT_MAX ? -1 : s_stop;
   }
 }
 
-// per job: exclusive prefix over its blocks of the three counts; passing
-// index of the stop line (or -1); the closing entries of the positive-line
-// prefixes
+// per job: exclusive prefix of the passing lines over its blocks; passing
+// index of the stop line (or -1); the chain's closing entry at n
 __global__ __launch_bounds__(256) void k_shr_scan(const DevLayer *Ys, Strides S, ShrinkWork W) {
   __shared__ int buf[256];
-  __shared__ int carry[3];
+  __shared__ int carry;
   __shared__ int s_stop;
   const int job = blockIdx.x;
   const int nblk = Ys[job].nb4;
   const int64_t o = job * S.blk;
-  if (threadIdx.x < 3) carry[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_stop = INT_MAX;
+  if (threadIdx.x == 0) {
+    carry = 0;
+    s_stop = INT_MAX;
+  }
   __syncthreads();
   for (int base = 0; base < nblk; base += 256) {
-    int b = base + threadIdx.x;
-    int v[3] = {0, 0, 0};
+    const int b = base + threadIdx.x;
+    const int v = b < nblk ? W.bcnt[o + b] : 0;
+    int tot;
+    const int ex = block_scan_excl<256>(v, buf, &tot);
     if (b < nblk) {
-      v[0] = W.bcnt[o + b];
-      v[1] = W.bcntn[o + b];
-      v[2] = W.bcntc[o + b];
+      W.bp[o + b] = carry + ex;
+      if (W.bstop[o + b] >= 0) atomicMin(&s_stop, carry + ex + W.bstop[o + b]);
     }
-    int32_t *dst[3] = {W.bp, W.bpn, W.bpc};
-    for (int q = 0; q < 3; q++) {
-      int tot;
-      int ex = block_scan_excl<256>(v[q], buf, &tot);
-      if (b < nblk) dst[q][o + b] = carry[q] + ex;
-      if (q == 0 && b < nblk && W.bstop[o + b] >= 0) atomicMin(&s_stop, carry[0] + ex + W.bstop[o + b]);
-      __syncthreads();
-      if (threadIdx.x == 0) carry[q] += tot;
-      __syncthreads();
-    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
   }
   if (threadIdx.x == 0) {
-    W.bp[o + nblk] = carry[0];
-    W.bpn[o + nblk] = carry[1];
-    W.bpc[o + nblk] = carry[2];
-    W.npass[job] = carry[0];
+    const int n = carry;
+    W.bp[o + nblk] = n;
+    W.npass[job] = n;
     W.stop[job] = s_stop == INT_MAX ? -1 : s_stop;
-    W.pn[job * S.line + carry[0]] = carry[1];
-    W.pc[job * S.line + carry[0]] = carry[2];
+    W.jmp[job * S.line + n] = n;
+    W.wsum[job * S.line + n] = 0;
   }
 }
 
-// per TAPE3 block: write the passing lines compacted in file order, with
-// the exclusive prefixes of positive non-CO2 / CO2 lines
+// Passing lines after the current TAPE3 block, read from the raw records
+// (for the chain entries of a block's last lines): x and cfl of passing
+// index k = k0, k0 + 1, ... up to n (the job's passing lines).
+struct RawCursor {
+  const DevLines *L;
+  const DevLayer *Y;
+  int i, iend, k, n;
+  // the next passing line: false past the job's last one
+  __device__ bool next(double &x, int &f) {
+    while (k < n && i < iend) {
+      F4Line o;
+      const int r = i++;
+      if (linf4_line(*L, *Y, r, o)) {
+        k++;
+        x = o.x;
+        f = (o.co2 ? 1 : 0) | (o.sp > 0. ? 2 : 0);
+        return true;
+      }
+    }
+    return false;
+  }
+};
+
+// per TAPE3 block: write the passing lines compacted in file order, then
+// each one's chain entries g(e), f(e) and the count the group after e
+// emits.  The block's lines are in LDS; a line whose search runs past the
+// block continues over the raw records that follow.
+constexpr int SB_HALO = 64;  // raw records after the block read into LDS
 __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer *Ys, Strides S,
                                                       ShrinkWork W) {
   __shared__ int buf[SB_T];
+  __shared__ double sx[SB_T + SB_HALO];
+  __shared__ int8_t sf[SB_T + SB_HALO];
+  __shared__ int s_hn;
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
   const int bl = blockIdx.x;
   if (bl >= Y.nb4) return;
   const int b = Y.b4 + bl;
-  int i = L.bstart[b] + threadIdx.x;
+  const int i = L.bstart[b] + threadIdx.x;
   F4Line o{};
-  int pass = 0, pn = 0, pc = 0;
-  if (i < L.bstart[b + 1] && linf4_line(L, Y, i, o)) {
-    pass = 1;
-    pn = (o.sp > 0. && !o.co2);
-    pc = (o.sp > 0. && o.co2);
-  }
-  int t0, t1, t2;
-  int pos = block_scan_excl<SB_T>(pass, buf, &t0);
-  int posn = block_scan_excl<SB_T>(pn, buf, &t1);
-  int posc = block_scan_excl<SB_T>(pc, buf, &t2);
+  int pass = 0;
+  if (i < L.bstart[b + 1] && linf4_line(L, Y, i, o)) pass = 1;
+  int tot;
+  const int pos = block_scan_excl<SB_T>(pass, buf, &tot);
   const int64_t ob = job * S.blk + bl;
+  const int64_t oj = job * S.line;
+  const int k0 = W.bp[ob];  // passing index of the block's first passing line
+  const int8_t fl = (int8_t)((o.co2 ? 1 : 0) | (o.sp > 0. ? 2 : 0));
   if (pass) {
-    int64_t k = job * S.line + W.bp[ob] + pos;
+    const int64_t k = oj + k0 + pos;
     W.cx[k] = o.x;
     W.csp[k] = o.sp;
     W.cal[k] = o.al;
     W.cad[k] = o.ad;
-    W.cco2[k] = (int8_t)o.co2;
-    W.pn[k] = W.bpn[ob] + posn;
-    W.pc[k] = W.bpc[ob] + posc;
+    W.cfl[k] = fl;
     if (W.craw) W.craw[k] = i;
+    sx[pos] = o.x;
+    sf[pos] = fl;
   }
-}
-
-// per passing line e: g(e), f(e) and the first doubling level
-__global__ void k_shr_chain(const DevLayer *Ys, Strides S, ShrinkWork W) {
-  const int job = blockIdx.y;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  // the passing lines among the SB_HALO raw records after the block (the
+  // searches of the block's last lines run into them), after its own
+  const int iend = L.bstart[Y.b4 + Y.nb4];
+  const int ih = L.bstart[b + 1] + (int)threadIdx.x;
+  if (threadIdx.x < SB_HALO) {
+    F4Line oh{};
+    const bool ph = ih < iend && linf4_line(L, Y, ih, oh);
+    const uint64_t mh = __ballot(ph);
+    const int ph_pos = __builtin_popcountll(mh & ((1ull << threadIdx.x) - 1ull));
+    if (ph) {
+      sx[tot + ph_pos] = oh.x;
+      sf[tot + ph_pos] = (int8_t)((oh.co2 ? 1 : 0) | (oh.sp > 0. ? 2 : 0));
+    }
+    if (threadIdx.x == 0) s_hn = __builtin_popcountll(mh);
+  }
+  __syncthreads();
+  if (threadIdx.x >= tot) return;
   const int n = W.npass[job];
-  if (e > n) return;
-  const int64_t o = job * S.line;
-  if (e == n) {
-    W.jmp[o + e] = n;
-    W.wsum[o + e] = 0;
-    return;
+  const int nl = tot + s_hn;  // lines in LDS
+  const int el = threadIdx.x;  // the block's el-th passing line
+  const int e = k0 + el;
+  const double thr = sx[el] + Y.dvr4 / 2.;
+  RawCursor rc{&L, &Y, L.bstart[b + 1] + SB_HALO, iend, k0 + nl, n};
+  // passing line k > e: from LDS within the block and its halo, else from
+  // the cursor (k increases by one per call)
+  auto line_at = [&](int k, double &x, int &f) -> bool {
+    if (k >= n) return false;
+    if (k - k0 < nl) {
+      x = sx[k - k0];
+      f = sf[k - k0];
+      return true;
+    }
+    return rc.next(x, f);
+  };
+  // g: first k >= e+1 with x_k > thr; f from e+2 on; the emitted classes
+  // of [e+1, f]
+  int k = e + 1, m = 0, g = -1;
+  double x = 0.;
+  int f = 0;
+  for (;; k++) {
+    if (!line_at(k, x, f)) break;  // k = n
+    if (x > thr) {
+      if (g < 0) g = k - 1;
+      if (k >= e + 2) break;
+      m |= em_bits(f);  // line e+1 is past VLMT but opens the group after e
+      continue;
+    }
+    m |= em_bits(f);
   }
-  const double dv2 = Ys[job].dvr4 / 2.;
-  const double *x = W.cx + o;
-  double thr = x[e] + dv2;
-  int k = e + 1;
-  while (k < n && !(x[k] > thr)) k++;
-  W.g[o + e] = k - 1;
+  if (g < 0) g = k - 1;
+  W.g[oj + e] = g;
   if (e == n - 1) {
-    W.jmp[o + e] = n;
-    W.wsum[o + e] = 0;
+    W.jmp[oj + e] = n;
+    W.wsum[oj + e] = 0;
     return;
   }
-  // the first line past VLMT from e + 1 on is also the first from e + 2 on
-  // unless it is line e + 1 itself
-  if (k == e + 1) {
-    k = e + 2;
-    while (k < n && !(x[k] > thr)) k++;
-  }
-  int f = k - 1;
-  W.jmp[o + e] = f;
-  W.wsum[o + e] = (int16_t)em_cnt(W.pn + o, W.pc + o, e + 1, f);
+  W.jmp[oj + e] = k - 1;
+  W.wsum[oj + e] = (int16_t)em_of(m);
 }
 
 // Lines SHRINK emits for the batch [s, t] of passing indices: the first
@@ -298,7 +354,8 @@ __global__ void k_shr_chain(const DevLayer *Ys, Strides S, ShrinkWork W) {
 // line table is fetched once per 64 lines.  Every value but the window is
 // wave-uniform; all lanes of the wave call it together.
 struct ChainWalk {
-  const int32_t *pn, *pc, *jmp;
+  const int8_t *cfl;
+  const int32_t *jmp;
   const int16_t *wsum;
   int lim, s, e1, cur, acc, base, wj, ww;
   __device__ void load(int b) {
@@ -308,15 +365,14 @@ struct ChainWalk {
     ww = i < lim ? (int)wsum[i] : 0;
   }
   __device__ ChainWalk(const ShrinkWork &W, int64_t o, int n, int s_) {
-    pn = W.pn + o;
-    pc = W.pc + o;
+    cfl = W.cfl + o;
     jmp = W.jmp + o;
     wsum = W.wsum + o;
     lim = n + 1;
     s = s_;
     e1 = __builtin_amdgcn_readfirstlane(W.g[o + s]);
     cur = e1;
-    acc = em_cnt(pn, pc, s, e1);
+    acc = em_cnt(cfl, s, e1);
     load(cur);
   }
   // f(cur) and the count the group after cur emits
@@ -328,7 +384,7 @@ struct ChainWalk {
   }
   // t never decreases from one call to the next
   __device__ int count_to(int t) {
-    if (e1 >= t) return em_cnt(pn, pc, s, t);
+    if (e1 >= t) return em_cnt(cfl, s, t);
     for (;;) {
       int w;
       const int nx = next(&w);
@@ -336,7 +392,7 @@ struct ChainWalk {
       acc += w;
       cur = nx;
     }
-    return acc + em_cnt(pn, pc, cur + 1, t);
+    return acc + em_cnt(cfl, cur + 1, t);
   }
 };
 
@@ -631,7 +687,7 @@ __global__ __launch_bounds__(64) void k_shr_bwalk(Strides S, ShrinkWork W) {
     if (em == 2) W.grp[o + slot + 1] = -1;
   };
   if (cw.e1 >= t) {
-    mark(s, t, off, em_cnt(cw.pn, cw.pc, s, t));
+    mark(s, t, off, em_cnt(cw.cfl, s, t));
     return;
   }
   mark(s, cw.e1, off, cw.acc);
@@ -643,7 +699,7 @@ __global__ __launch_bounds__(64) void k_shr_bwalk(Strides S, ShrinkWork W) {
     cw.acc += w;
     cw.cur = nx;
   }
-  mark(cw.cur + 1, t, off + cw.acc, em_cnt(cw.pn, cw.pc, cw.cur + 1, t));
+  mark(cw.cur + 1, t, off + cw.acc, em_cnt(cw.cfl, cw.cur + 1, t));
 }
 
 // per output slot: the group recorded there writes its effective lines
@@ -661,7 +717,7 @@ __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9
   double sumv2 = 0., sums2 = 0., sumal2 = 0., sumad2 = 0.;
   for (int k = gs; k <= ge; k++) {
     double sp = W.csp[o + k];
-    if (!W.cco2[o + k]) {
+    if (!(W.cfl[o + k] & 1)) {
       sumv = sumv + W.cx[o + k] * sp;
       sums = sums + sp;
       sumal = sumal + sp * W.cal[o + k];
@@ -1090,78 +1146,84 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
 }
 
 // ---------------------------------------------------------------------
-// LNCOR1 for the near lines (oprop.f90:995-1193) except the two tests
-// that depend on the panel at which the block is read (the VFT test and
-// the R4 rejection); those are applied in k_reject.  Also computes the
-// first panel whose R1 buffer can hold the line (CNVFNV JMAX1 <= MAX1).
+// LNCOR1's corrections of TAPE3 record i (oprop.f90:995-1193) except the
+// two tests that depend on the panel at which its block is read: shifted
+// centre, 1/ALFV, corrected strength and the zeta column.  False for a
+// record LNCOR1 skips.
+struct NearCore {
+  double vnu, recalf, sp, zetai;
+  int izeta;
+};
+__device__ __forceinline__ bool lncor1_core(const DevLines &L, const DevLayer &Y, int i, NearCore &c) {
+  const int mc = L.mol[i];
+  const int m = mc % 100, iso = (mc % 1000) / 100;
+  if (m < 1 || m > 7 || iso < 1 || iso > c_isomax[m - 1]) return false;
+  double sui = (m <= Y.nmol) ? (double)L.sp[i] * Y.wk[m - 1] : 0.;
+  if (Y.jrad == 1) sui = sui * d_radfn(L.vnu[i], Y.xkt);
+  if (sui == 0.) return false;
+  double alfa0 = L.alfa[i], hwhm = L.hwhm[i];
+  if (m == 7) alfa0 = (alfa0 - 0.21 * hwhm) / (1.0 - 0.21);  // RDLIN (oprop.f90:818-827)
+  const double vnu = L.vnu[i] + Y.rhorat * (double)L.pshift[i];
+  const double tmpcor = pow(Y.tratio, (double)L.tmpalf[i]);
+  const double alfa0i = alfa0 * tmpcor;
+  const double hwhmsi = hwhm * tmpcor;
+  const double alfl = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
+  const double alfad = vnu * Y.alfd1[m - 1][iso - 1];
+  const double zeta = alfl / (alfl + alfad);
+  const double fzeta = 100. * zeta;
+  int iz = fint(fzeta + ONEPL_D);
+  iz = iz < 1 ? 1 : (iz > 101 ? 101 : iz);  // zeta in [0,1] keeps iz in range
+  const double zetdif = fzeta - (double)(iz - 1);
+  double alfv = (c_avrat[iz - 1] + zetdif * (c_avrat[iz] - c_avrat[iz - 1])) * (alfl + alfad);
+  if (alfv < Y.dv) alfv = Y.dv;
+  if (alfv > Y.alfmax) alfv = Y.alfmax;
+  double epp = L.epp[i];
+  if (epp <= -1.001) epp = fabs(epp);
+  if (epp > -0.999) sui = sui * Y.scor[m - 1][iso - 1] * exp(-epp * Y.betacr_l) * (1. + exp(-vnu / Y.xkt));
+  c.vnu = vnu;
+  c.recalf = 1. / alfv;
+  c.sp = sui;
+  c.zetai = zeta;
+  c.izeta = iz;
+  return true;
+}
+
+// LNCOR1 for the near lines: the fields every rejection pass reads, the
+// first panel whose R1 buffer can hold the line (CNVFNV JMAX1 <= MAX1), and
+// the strength and zeta column the scatter records of the surviving lines
+// read.
 __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPanel *pans,
-                         NearArr out) {
+                         NearHot *out, NearCold *cold) {
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Y.nn) return;
   const DevPanel *pan = pans + Y.pan0;
-  int i = Y.nlo + t;
-  NearLine o;
-  o.vnu = L.vnu[i];
-  o.recalf = 0.;
-  o.sp = 0.;
-  o.speak = 0.;
-  o.zetai = 0.;
-  o.izeta = 1;
-  o.ff = 0;
-  o.keep = 0;
-  o.proc = -1;
-  int mc = L.mol[i];
-  int m = mc % 100, iso = (mc % 1000) / 100;
-  if (m >= 1 && m <= 7 && iso >= 1 && iso <= c_isomax[m - 1]) {
-    double sui = (m <= Y.nmol) ? (double)L.sp[i] * Y.wk[m - 1] : 0.;
-    if (Y.jrad == 1) sui = sui * d_radfn(L.vnu[i], Y.xkt);
-    if (sui != 0.) {
-      double alfa0 = L.alfa[i], hwhm = L.hwhm[i];
-      if (m == 7) alfa0 = (alfa0 - 0.21 * hwhm) / (1.0 - 0.21);  // RDLIN (oprop.f90:818-827)
-      double vnu = L.vnu[i] + Y.rhorat * (double)L.pshift[i];
-      double tmpcor = pow(Y.tratio, (double)L.tmpalf[i]);
-      double alfa0i = alfa0 * tmpcor;
-      double hwhmsi = hwhm * tmpcor;
-      double alfl = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
-      double alfad = vnu * Y.alfd1[m - 1][iso - 1];
-      double zeta = alfl / (alfl + alfad);
-      double fzeta = 100. * zeta;
-      int iz = fint(fzeta + ONEPL_D);
-      iz = iz < 1 ? 1 : (iz > 101 ? 101 : iz);  // zeta in [0,1] keeps iz in range
-      double zetdif = fzeta - (double)(iz - 1);
-      double alfv = (c_avrat[iz - 1] + zetdif * (c_avrat[iz] - c_avrat[iz - 1])) * (alfl + alfad);
-      if (alfv < Y.dv) alfv = Y.dv;
-      if (alfv > Y.alfmax) alfv = Y.alfmax;
-      double recalf = 1. / alfv;
-      double epp = L.epp[i];
-      if (epp <= -1.001) epp = fabs(epp);
-      if (epp > -0.999) sui = sui * Y.scor[m - 1][iso - 1] * exp(-epp * Y.betacr_l) * (1. + exp(-vnu / Y.xkt));
-      o.vnu = vnu;
-      o.recalf = recalf;
-      o.sp = sui;
-      o.speak = sui * recalf;
-      o.zetai = zeta;
-      o.izeta = iz;
-      o.keep = 1;
-      // first fit: INT(ZINT_p + BHWDXF + 1.5) <= MAX1
-      double zslope = recalf * (Y.dv / 0.002);
-      double bhwdxf = (4. / 0.002) / zslope;
-      double z0 = (vnu - Y.vft0) / Y.dv;
-      int p = (int)floor((z0 + bhwdxf + 0.5 - (double)Y.max1) / 2400.) + 1;
-      if (p < 0) p = 0;
-      if (p > Y.npanel) p = Y.npanel;
-      auto fits = [&](int q) {
-        double zint = (vnu - pan[q].vft) / Y.dv;
-        return fint(zint + bhwdxf + 1.5) <= Y.max1;
-      };
-      while (p > 0 && fits(p - 1)) p--;
-      while (p < Y.npanel && !fits(p)) p++;
-      o.ff = p;  // == npanel: never processed before the last panel
-    }
+  const int i = Y.nlo + t;
+  NearHot o{L.vnu[i], 0., 0., 0, 0};
+  NearCore c;
+  if (lncor1_core(L, Y, i, c)) {
+    o.vnu = c.vnu;
+    o.recalf = c.recalf;
+    o.speak = c.sp * c.recalf;
+    o.keep = 1;
+    // first fit: INT(ZINT_p + BHWDXF + 1.5) <= MAX1
+    const double zslope = c.recalf * (Y.dv / 0.002);
+    const double bhwdxf = (4. / 0.002) / zslope;
+    const double z0 = (c.vnu - Y.vft0) / Y.dv;
+    int p = (int)floor((z0 + bhwdxf + 0.5 - (double)Y.max1) / 2400.) + 1;
+    if (p < 0) p = 0;
+    if (p > Y.npanel) p = Y.npanel;
+    auto fits = [&](int q) {
+      double zint = (c.vnu - pan[q].vft) / Y.dv;
+      return fint(zint + bhwdxf + 1.5) <= Y.max1;
+    };
+    while (p > 0 && fits(p - 1)) p--;
+    while (p < Y.npanel && !fits(p)) p++;
+    o.ff = p;  // == npanel: never processed before the last panel
+    cold[job * S.line + t] = NearCold{c.sp, c.zetai, c.izeta, 0};
   }
-  out.put(job * S.line + t, o);
+  out[job * S.line + t] = o;
 }
 
 // LNCOR1's VFT test and R4 rejection (oprop.f90:1120, 1153-1177) of one
@@ -1296,13 +1358,58 @@ __global__ __launch_bounds__(256) void k_pread_scan(const DevLayer *Ys, Strides 
   }
 }
 
+// Lines that survive the rejection tests per near block, and their
+// exclusive prefix per job over the blocks (bkp) with the total at kidx[nn].
+__global__ __launch_bounds__(256) void k_keep_blk(const DevLayer *Ys, DevLines L, Strides S, const int *keep,
+                                                  int32_t *bkc) {
+  __shared__ int buf[256];
+  const int job = blockIdx.y;
+  const DevLayer &Y = Ys[job];
+  const int k = blockIdx.x;
+  if (k >= Y.nbk) return;
+  const int b = Y.blk0 + k;
+  int s = L.bstart[b] - Y.nlo, e = L.bstart[b + 1] - Y.nlo;
+  if (s < 0) s = 0;
+  if (e > Y.nn) e = Y.nn;
+  const int t = s + (int)threadIdx.x;
+  int tot;
+  block_scan_excl<256>(t < e ? keep[job * S.line + t] : 0, buf, &tot);
+  if (threadIdx.x == 0) bkc[job * S.blk + k] = tot;
+}
+__global__ __launch_bounds__(256) void k_keep_scan(const DevLayer *Ys, Strides S, const int32_t *bkc, int32_t *bkp,
+                                                   int32_t *kidx) {
+  __shared__ int buf[256];
+  __shared__ int carry;
+  const int job = blockIdx.x;
+  const int nbk = Ys[job].nbk;
+  const int64_t o = job * S.blk;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nbk; base += 256) {
+    const int k = base + (int)threadIdx.x;
+    int tot;
+    const int ex = block_scan_excl<256>(k < nbk ? bkc[o + k] : 0, buf, &tot);
+    if (k < nbk) bkp[o + k] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) kidx[job * S.line + Ys[job].nn] = carry;
+}
+
 // Processing panel of each kept line: the panel at which its block is read,
 // raised by the first-fit panels of kept lines before it in the block
 // (CNVFNV stops at the first line that does not fit, oprop.f90:1287-1292).
-// Then the scatter record of CNVFNV (oprop.f90:1262-1305).
-__global__ __launch_bounds__(256) void k_scatter_rec(NearArr nl, const int *keep, DevLines L,
+// Then the scatter record of CNVFNV (oprop.f90:1262-1305).  The records of
+// the kept lines are stored contiguously in file order, at their kept-line
+// index (kidx, written for every near record).  With k1acc, the
+// near-wing point updates and convolved lines per job (sum of JMAX1-JMIN1+1
+// and count over kept lines; LINCNT of the TAPE6 table) are added up too.
+__global__ __launch_bounds__(256) void k_scatter_rec(const NearHot *nh, const NearCold *nc, const int *keep, DevLines L,
                                                      const DevLayer *Ys, Strides S, const int *pread,
-                                                     const DevPanel *pans, Recs recs) {
+                                                     const DevPanel *pans, const int32_t *bkp, int32_t *kidx,
+                                                     Recs recs, unsigned long long *k1acc,
+                                                     unsigned long long *lines) {
   __shared__ int buf[256];
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
@@ -1316,10 +1423,13 @@ __global__ __launch_bounds__(256) void k_scatter_rec(NearArr nl, const int *keep
   const int t = s + tid;
   const bool valid = t < e;
   const int64_t o = job * S.line;
-  int kp = valid ? keep[o + t] : 0;
-  NearLine n{};
-  if (valid) n = nl.get(o + t);
-  buf[tid] = (valid && kp) ? n.ff : 0;
+  const int kp = valid ? keep[o + t] : 0;
+  NearHot h{};
+  if (kp) h = nh[o + t];
+  int nkb;
+  const int64_t ci = o + bkp[job * S.blk + k] + block_scan_excl<256>(kp, buf, &nkb);  // record slot
+  if (valid) kidx[o + t] = (int32_t)(ci - o);
+  buf[tid] = kp ? h.ff : 0;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
     int q = tid >= off ? buf[tid - off] : 0;
@@ -1327,74 +1437,67 @@ __global__ __launch_bounds__(256) void k_scatter_rec(NearArr nl, const int *keep
     buf[tid] = max(buf[tid], q);
     __syncthreads();
   }
-  if (!valid) return;
-  int P = max(pread[job * S.blk + k], buf[tid]);
-  SRec1 r1{};
-  SRecA r2{}, r3{};
-  SRecV rv{};
-  rv.izm = 1;
-  if (kp && P < Y.npanel) {
-    double vft = pans[Y.pan0 + P].vft;
-    double depthi = n.sp * n.recalf;
-    int izm = n.izeta;
-    double zetdif = 100. * n.zetai - (double)(izm - 1);
-    if (Y.voigt) {  // oprop_voigt.f90:1273-1276: DEPTHI and the zeta column
-      r1.str = r2.str = r3.str = depthi;
-      rv.zdif = zetdif;
-      rv.izm = izm < 1 ? 1 : (izm > NZETA ? NZETA : izm);
-    } else {
-      r1.str = depthi * (c_cf1[izm - 1] + zetdif * (c_cf1[izm] - c_cf1[izm - 1]));
-      r2.str = depthi * (c_cf2[izm - 1] + zetdif * (c_cf2[izm] - c_cf2[izm - 1]));
-      r3.str = depthi * (c_cf3[izm - 1] + zetdif * (c_cf3[izm] - c_cf3[izm - 1]));
-      r1.strd = depthi * (c_cgauss[izm - 1] + zetdif * (c_cgauss[izm] - c_cgauss[izm - 1]));
-      r1.strver = depthi * (c_cer[izm - 1] + zetdif * (c_cer[izm] - c_cer[izm - 1]));
+  int np = 0;
+  if (kp) {
+    const int P = max(pread[job * S.blk + k], buf[tid]);
+    SRec1 r1{};
+    SRecA r2{}, r3{};
+    SRecV rv{};
+    rv.izm = 1;
+    if (P < Y.npanel) {
+      const NearCold n = nc[o + t];
+      double vft = pans[Y.pan0 + P].vft;
+      double depthi = n.sp * h.recalf;
+      int izm = n.izeta;
+      double zetdif = 100. * n.zetai - (double)(izm - 1);
+      if (Y.voigt) {  // oprop_voigt.f90:1273-1276: DEPTHI and the zeta column
+        r1.str = r2.str = r3.str = depthi;
+        rv.zdif = zetdif;
+        rv.izm = izm < 1 ? 1 : (izm > NZETA ? NZETA : izm);
+      } else {
+        r1.str = depthi * (c_cf1[izm - 1] + zetdif * (c_cf1[izm] - c_cf1[izm - 1]));
+        r2.str = depthi * (c_cf2[izm - 1] + zetdif * (c_cf2[izm] - c_cf2[izm - 1]));
+        r3.str = depthi * (c_cf3[izm - 1] + zetdif * (c_cf3[izm] - c_cf3[izm - 1]));
+        r1.strd = depthi * (c_cgauss[izm - 1] + zetdif * (c_cgauss[izm] - c_cgauss[izm - 1]));
+        r1.strver = depthi * (c_cer[izm - 1] + zetdif * (c_cer[izm] - c_cer[izm - 1]));
+      }
+      double zslope = h.recalf * (Y.dv / 0.002);
+      double zint = (h.vnu - vft) / Y.dv;
+      double bhwdxf = (4. / 0.002) / zslope;
+      int jmax1 = fint(zint + bhwdxf + 1.5);
+      int jmin1 = fint(zint - bhwdxf + 1.5);
+      double conf2 = Y.dv / Y.dvr2, conf3 = Y.dv / Y.dvr3;
+      double rshft = zint < 0.0 ? -0.5 : 0.5;
+      int j2shft = fint(zint * (1. - conf2) + rshft);
+      int j3shft = fint(zint * (1. - conf3) + rshft);
+      int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
+      r1.zfl = ((double)(jmin1 - 2) - zint) * zslope;
+      r2.zfl = ((double)(jmin2 - 2) - zint * conf2) * zslope;
+      r3.zfl = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+      r1.zslope = r2.zslope = r3.zslope = zslope;
+      // global fine index of JMIN1 and its R2/R3 counterparts (G1 < 2^31 for
+      // any layer the planner accepts)
+      const int32_t g1lo = jmin1 - 1 + 2400 * P;
+      r1.glo = g1lo;
+      r2.glo = g1lo - (j2shft + 1800 * P);
+      r3.glo = g1lo - (j3shft + 2250 * P);
+      r1.npts = r2.npts = r3.npts = np = jmax1 - jmin1 + 1;
     }
-    double zslope = n.recalf * (Y.dv / 0.002);
-    double zint = (n.vnu - vft) / Y.dv;
-    double bhwdxf = (4. / 0.002) / zslope;
-    int jmax1 = fint(zint + bhwdxf + 1.5);
-    int jmin1 = fint(zint - bhwdxf + 1.5);
-    double conf2 = Y.dv / Y.dvr2, conf3 = Y.dv / Y.dvr3;
-    double rshft = zint < 0.0 ? -0.5 : 0.5;
-    int j2shft = fint(zint * (1. - conf2) + rshft);
-    int j3shft = fint(zint * (1. - conf3) + rshft);
-    int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
-    r1.zfl = ((double)(jmin1 - 2) - zint) * zslope;
-    r2.zfl = ((double)(jmin2 - 2) - zint * conf2) * zslope;
-    r3.zfl = ((double)(jmin3 - 2) - zint * conf3) * zslope;
-    r1.zslope = r2.zslope = r3.zslope = zslope;
-    // global fine index of JMIN1 and its R2/R3 counterparts (G1 < 2^31 for
-    // any layer the planner accepts)
-    const int32_t g1lo = jmin1 - 1 + 2400 * P;
-    r1.glo = g1lo;
-    r2.glo = g1lo - (j2shft + 1800 * P);
-    r3.glo = g1lo - (j3shft + 2250 * P);
-    r1.npts = r2.npts = r3.npts = jmax1 - jmin1 + 1;
+    recs.r1[ci] = r1;
+    recs.r2[ci] = r2;
+    recs.r3[ci] = r3;
+    if (Y.voigt) recs.v[ci] = rv;
   }
-  recs.r1[o + t] = r1;
-  recs.r2[o + t] = r2;
-  recs.r3[o + t] = r3;
-  if (Y.voigt) recs.v[o + t] = rv;
-}
-
-// Near-wing point updates and convolved lines per job (sum of
-// JMAX1-JMIN1+1 and count over kept lines; LINCNT of the TAPE6 table).
-__global__ void k_sum_npts(const DevLayer *Ys, Strides S, const SRec1 *rec,
-                           unsigned long long *acc, unsigned long long *lines) {
-  const int job = blockIdx.y;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long v = 0ull, c = 0ull;
-  if (i < Ys[job].nn && rec[job * S.line + i].npts > 0) {
-    v = (unsigned long long)rec[job * S.line + i].npts;
-    c = 1ull;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    v += __shfl_down(v, off);
-    c += __shfl_down(c, off);
-  }
-  if ((threadIdx.x & 63) == 0 && v) {
-    atomicAdd(&acc[job], v);
-    atomicAdd(&lines[job], c);
+  if (k1acc) {  // TAPE6 counters (statistics passes only)
+    unsigned long long v = np > 0 ? (unsigned long long)np : 0ull, c = np > 0 ? 1ull : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+      v += __shfl_down(v, off);
+      c += __shfl_down(c, off);
+    }
+    if ((tid & 63) == 0 && v) {
+      atomicAdd(&k1acc[job], v);
+      atomicAdd(&lines[job], c);
+    }
   }
 }
 
@@ -1412,8 +1515,12 @@ struct TileMap {
 };
 
 // Line range [lo, hi) whose scatter extent may touch each tile of grid G.
+// The range is found over the near records (raw centres, in file order)
+// and handed out in kept-line indices (kidx: kept lines before each
+// record), which is how the scatter records are stored; without kidx, in
+// record indices.
 __global__ void k_tile_ranges(const double *vraw, const DevLayer *Ys, TileMap M, int ntiles,
-                              int tile, int G, const double *margin) {
+                              int tile, int G, const double *margin, const int32_t *kidx, Strides S) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   const int job = M.job[t];
@@ -1435,8 +1542,14 @@ __global__ void k_tile_ranges(const double *vraw, const DevLayer *Ys, TileMap M,
     if (v[m] <= b) l2 = m + 1;
     else h2 = m;
   }
-  M.lo[t] = l;
-  M.hi[t] = l2;
+  if (!kidx) {
+    M.lo[t] = l;
+    M.hi[t] = l2;
+    return;
+  }
+  // (no near records: kidx is not written)
+  M.lo[t] = nn > 0 ? kidx[job * S.line + l] : 0;
+  M.hi[t] = nn > 0 ? kidx[job * S.line + l2] : 0;
 }
 
 // XINT of a coarse array into R3 point I of the panel that owns it
@@ -1555,17 +1668,20 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
   const int lo = M.lo[blockIdx.x], hi = M.hi[blockIdx.x];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
-  // Each wave takes chunks of 64 consecutive lines: lane i loads the
+  // Each wave takes chunks of CH consecutive lines: lane i < CH loads the
   // record of line c0 + i with vector loads, then the wave walks the chunk
   // line by line, broadcasting each line's fields with readlane.  The next
   // chunk's records are loaded before the current chunk is scattered.
+  // Chunks of 16-32 spread a tile's lines over its waves more evenly (16
+  // measured best for the FP64 R1 tile, 32 for the others).
+  constexpr int CH = (G == 1 && sizeof(ACC) == 8) ? 16 : 32, CSTEP = TILE_T / 64 * CH;
   TileLine<G, V> cur, nxt;
-  int c0 = lo + wave * 64;
-  if (c0 < hi) cur.load(recs, rbase, c0 + lane, hi);
-  for (; c0 < hi; c0 += TILE_T) {
-    const int c1 = c0 + TILE_T;
-    if (c1 < hi) nxt.load(recs, rbase, c1 + lane, hi);
-    const int nj = hi - c0 < 64 ? hi - c0 : 64;
+  int c0 = lo + wave * CH;
+  if (c0 < hi) cur.load(recs, rbase, c0 + lane, lane < CH ? hi : c0);
+  for (; c0 < hi; c0 += CSTEP) {
+    const int c1 = c0 + CSTEP;
+    if (c1 < hi) nxt.load(recs, rbase, c1 + lane, lane < CH ? hi : c1);
+    const int nj = hi - c0 < CH ? hi - c0 : CH;
     for (int j = 0; j < nj; j++) {
       const int npts = __builtin_amdgcn_readlane(cur.npts, j);
       if (npts <= 0) continue;

@@ -90,21 +90,23 @@ __device__ __forceinline__ double d_dlns_dT(double dlnscor, double epp, double v
 
 // ---------------------------------------------------------------------
 // Near lines: LNCOR1's correction of each line again, with its
-// derivatives, as the weights of the tangent tiles.  Lines without a
-// scatter record are never read.
+// derivatives, as the weights of the tangent tiles.  For the kept lines
+// only, stored like their scatter records (at the kept-line index kidx).
 __global__ __launch_bounds__(256) void k_near_tan(DevLines L, const DevLayer *Ys, const TanLayer *Ts,
-                                                  Strides S, TanLine *out) {
+                                                  Strides S, const int *keep, const int32_t *kidx,
+                                                  TanLine *out) {
   const int job = blockIdx.y;
   const DevLayer &Y = Ys[job];
   const TanLayer &TL = Ts[job];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= Y.nn) return;
+  if (t >= Y.nn || !keep[job * S.line + t]) return;
+  const int64_t slot = job * S.line + kidx[job * S.line + t];
   TanLine o{};
   const int i = Y.nlo + t;
   const int mc = L.mol[i];
   const int m = mc % 100, iso = (mc % 1000) / 100;
   if (m < 1 || m > 7 || m > Y.nmol || iso < 1 || iso > c_isomax[m - 1]) {
-    out[job * S.line + t] = o;
+    out[slot] = o;
     return;
   }
   const double T = TL.tave;
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(256) void k_near_tan(DevLines L, const DevLayer *Ys
   const double dals = (hwhmsi - alfa0i) * rs;  // daL for d rho_s = rho_s
   if (q > 0) fill(o.s, 1., dals, 0., 0.);
   if (TL.nint > TL.nchan) fill(o.u, 0., dals, 0., 0.);
-  out[job * S.line + t] = o;
+  out[slot] = o;
 }
 
 // ---------------------------------------------------------------------
@@ -206,7 +208,7 @@ struct TSeg {
 
 template <int G, typename OUT, int NSP>
 __global__ __launch_bounds__(TSEG_T) void k_tan_seg(Recs recs, const TanLine *__restrict__ tl, TileMap M,
-                                                     const double *vraw,
+                                                     const double *vraw, const int32_t *kidx,
                                                      const double *margin, const float *tabs, const DevLayer *Ys,
                                                      const TanLayer *Ts, Strides S, const DevPanel *pans,
                                                      const DevWin *wins, const double *r4c, const double *absc,
@@ -254,8 +256,10 @@ __global__ __launch_bounds__(TSEG_T) void k_tan_seg(Recs recs, const TanLine *__
       if (v[m] <= vb) l2 = m + 1;
       else h2 = m;
     }
-    l0 = l;
-    h0 = l2;
+    // record range -> kept-line range (the records and weights are stored
+    // for the kept lines only)
+    l0 = Y.nn > 0 ? kidx[job * S.line + l] : 0;
+    h0 = Y.nn > 0 ? kidx[job * S.line + l2] : 0;
   }
   float accT[NS], accU[NS], accS[NS][NSP > 0 ? NSP : 1];
 #pragma unroll
@@ -544,7 +548,7 @@ __global__ void k_shr_emit_tan(const TanLayer *Ts, Strides S, ShrinkWork W, Tape
       for (int e = 0; e < 4; e++) acc[q][e] = 0.;
     bool any = false;
     for (int k = gs; k <= ge; k++) {
-      if ((W.cco2[o + k] ? 1 : 0) != c) continue;
+      if ((W.cfl[o + k] & 1) != c) continue;
       const double sp = W.csp[o + k];
       if (sp == 0.) continue;
       any = true;
