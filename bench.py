@@ -450,6 +450,8 @@ def main():
         "config": {"workload": "50-layer column upwelling TOA spectrum: 500k lines 500-2500 cm-1, "
                                "7 molecules, LBLF4 panels, continuum on, XMERGE radiance, FP64",
                    "layers": nlay, "lines": 500000, "points": int(sum(sizes)), "top_points": ntop,
+                   "counts": {k: int(sum(s[k] for s in stats))
+                              for k in ("nlin", "lincnt", "nlins4", "ilin4", "nwin4", "k1_updates", "k3_updates")},
                    "stage_ms": {k: round(v, 4) for k, v in per.items()}},
         "fp32_split": {"value": 1000.0 / ms32, "ms_per_step": ms32, "ratio_vs_fp64": ms / ms32,
                        "max_dbt_K": float(dbt.max()), "rms_dbt_K": float(np.sqrt(np.mean(dbt ** 2))),
