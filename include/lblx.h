@@ -429,6 +429,40 @@ int lblx_fftscn_noalloc_device(lblx_ctx *ctx, const lblx_scan *scan, const doubl
                                double dv_in, int64_t n_in, double *d_out, int64_t cap, int64_t *n_out,
                                double *v1_out, double *dv_out);
 
+/* SCANFN (src/postsub.f90:25-560): record 8.1 of TAPE5, the scanning
+ * function convolved in the spectral domain.  jfn 1 triangle, 2 Gauss,
+ * 3 sinc^2, 4 sinc (-3, -4: hwhm holds the first zero crossing);
+ * jemit -1 absorption (1 - T), 0 transmission, 1 emission; sampl > 0
+ * points per half width, < 0 minus the output spacing, 0 the function's
+ * default; npanel is the point count of the input file's panels (0: 2400),
+ * which fixes where SHRKSC's running average starts.                    */
+typedef struct lblx_scanfn {
+  double hwhm;              /* HWHM (cm-1)                                 */
+  double v1, v2;            /* band of the scanned output                  */
+  double sampl;             /* SAMPL field                                 */
+  int32_t jemit;            /* -1, 0, 1                                    */
+  int32_t jfn;              /* +-3, +-4, 1, 2                              */
+  int32_t jvar;             /* 0 (a variable slit, 1, is refused)          */
+  int32_t npanel;           /* input panel length (0: 2400)                */
+} lblx_scanfn;
+
+/* Output grid of SCANFN for an input spectrum on v1_in + k*dv_in, k < n_in:
+ * PANLSC's points from v1 on dv_out up to v2.  iemit is the input file's
+ * IEMIT: 0 an optical depth (TAPE10-like; jemit 0 or -1 convolve
+ * exp(-OD)), 1 a radiance file whose radiance (jemit 1) or transmittance
+ * (jemit 0, -1) record d_in holds (JTREM, postsub.f90:325-335).
+ * Refused (LBLX_E_UNSUPPORTED): jfn 0, 5, 6 (CNVRCT, CNVVRC, CNVVRL),
+ * jvar 1, and input spacing above hwhm/12 (SCNINT interpolates first).   */
+int lblx_scanfn_size(const lblx_scanfn *scan, int iemit, double v1_in, double dv_in, int64_t n_in,
+                     int64_t *n_out, double *v1_out, double *dv_out);
+
+/* SCANFN on device buffers: RDSCAN's transform, SHRKSC's running average
+ * (postsub.f90:1276-1373), then CONVSC's convolution with the tabulated
+ * function (:1516-1598) gathered per output point.  d_out gets n_out
+ * values (lblx_scanfn_size).                                             */
+int lblx_scanfn_device(lblx_ctx *ctx, const lblx_scanfn *scan, int iemit, const double *d_in,
+                       double v1_in, double dv_in, int64_t n_in, double *d_out, int64_t n_out);
+
 /* Wait for the context stream; the stream handle (hipStream_t) for callers
  * that order their own device work after the library's.                  */
 int lblx_sync(lblx_ctx *ctx);

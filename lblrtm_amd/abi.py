@@ -118,6 +118,27 @@ class Scan(C.Structure):
     ]
 
 
+class ScanFn(C.Structure):
+    _fields_ = [
+        ("hwhm", C.c_double),
+        ("v1", C.c_double),
+        ("v2", C.c_double),
+        ("sampl", C.c_double),
+        ("jemit", C.c_int32),
+        ("jfn", C.c_int32),
+        ("jvar", C.c_int32),
+        ("npanel", C.c_int32),
+    ]
+
+
+def make_scanfn(hwhm, v1, v2, jemit=1, jfn=1, sampl=0.0, jvar=0, npanel=0):
+    """TAPE5 record 8.1 of SCANFN (src/postsub.f90:189-190)."""
+    s = ScanFn()
+    s.hwhm, s.v1, s.v2, s.sampl = hwhm, v1, v2, sampl
+    s.jemit, s.jfn, s.jvar, s.npanel = jemit, jfn, jvar, npanel
+    return s
+
+
 def make_scan(hwhm, v1, v2, jemit=1, jfnin=-6, mratin=0, dvout=0.0, ivx=0, nofix=0, param=0.0):
     """TAPE5 record 10.1 of FFTSCN (src/fftscn_dbl.f90:313-315)."""
     s = Scan()

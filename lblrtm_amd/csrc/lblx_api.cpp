@@ -294,6 +294,7 @@ struct lblx_ctx {
   std::shared_ptr<void> jprep;            // host job tables of run_jobs (JobsPrep)
   std::vector<LayerPlan> jplans;          // the Jacobian's job plans, storage reused
   DevBuf s_z[2], s_tw, s_g, s_ip, s_cof;  // FFTSCN
+  DevBuf f_s, f_xf, f_zp, f_q;            // SCANFN: shrunk input, function table, ZPEAK and R1 panel per group
   DevBuf j_rall, j_a, j_b;                // Jacobians
   // tangent-linear optical depth: compacted-line records, CONVF4's accepted
   // lines, per-line derivative weights, channel rows, continuum channels
@@ -2361,6 +2362,57 @@ int lblx_scan_size(const lblx_scan *scan, double v1_in, double dv_in, int64_t n_
   *n_out = P.nout;
   if (v1_out) *v1_out = P.v1out;
   if (dv_out) *dv_out = P.dvout;
+  return LBLX_OK;
+}
+
+int lblx_scanfn_size(const lblx_scanfn *scan, int iemit, double v1_in, double dv_in, int64_t n_in,
+                     int64_t *n_out, double *v1_out, double *dv_out) {
+  if (!scan || !n_out) return LBLX_E_ARG;
+  ScanFnPlan P;
+  int rc = plan_scanfn(scan, iemit, v1_in, dv_in, n_in, &P);
+  if (rc) return rc;
+  *n_out = P.nout;
+  if (v1_out) *v1_out = P.v1out;
+  if (dv_out) *dv_out = P.dvo;
+  return LBLX_OK;
+}
+
+int lblx_scanfn_device(lblx_ctx *c, const lblx_scanfn *scan, int iemit, const double *d_in, double v1_in,
+                       double dv_in, int64_t n_in, double *d_out, int64_t n_out) {
+  if (!c || !scan || !d_in || !d_out) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  ScanFnPlan P;
+  int rc = plan_scanfn(scan, iemit, v1_in, dv_in, n_in, &P);
+  if (rc) return rc;
+  if (n_out < P.nout) return LBLX_E_SIZE;
+  hipStream_t s = c->stream;
+  const size_t ng = P.zpk.size();
+  if (c->f_s.ensure(P.K * sizeof(double)) || c->f_xf.ensure(P.xf.size() * sizeof(double)) ||
+      c->f_zp.ensure(ng * sizeof(double)) || c->f_q.ensure(ng * sizeof(int32_t)))
+    return LBLX_E_HIP;
+  if (P.g_hi >= P.K) return LBLX_E_ARG;  // the walk's groups lie inside the shrunk input
+  int th = c->tbegin(ST_SCAN);
+  HIPCHK(hipMemcpyAsync(c->f_xf.p, P.xf.data(), P.xf.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->f_zp.p, P.zpk.data(), ng * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->f_q.p, P.q.data(), ng * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_sfn_shrink, dim3((unsigned)((P.K + 255) / 256)), dim3(256), 0, s, d_in, P.o, P.R, P.K,
+                     P.jtrem, P.jabs, c->f_s.as<double>());
+  SfnArgs A;
+  A.z0 = (P.vg0 - P.vft) / P.dvo;
+  A.ratio = P.dvsc / P.dvo;
+  A.zbound = P.zbound;
+  A.zslope = P.zslope;
+  A.xnorm = P.xnorm;
+  A.g_lo = P.g_lo;
+  A.g_hi = P.g_hi;
+  A.j0 = 32 + 1;  // PANLSC's first point NLO = NSHIFT + 1
+  A.nfmax = (int)P.xf.size();
+  hipLaunchKernelGGL(k_sfn_conv, dim3((unsigned)((P.nout + 255) / 256)), dim3(256), 0, s, A,
+                     c->f_s.as<double>(), c->f_zp.as<double>(), c->f_q.as<int32_t>(), c->f_xf.as<double>(),
+                     P.nout, d_out);
+  HIPCHK(hipGetLastError());
+  c->tend(th);
+  HIPCHK(hipStreamSynchronize(s));  // the host table is freed on return
   return LBLX_OK;
 }
 

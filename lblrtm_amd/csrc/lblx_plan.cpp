@@ -596,6 +596,206 @@ int scan_records(int64_t npts) {
 }
 }  // namespace
 
+// SCANFN's function table (postsub.f90:74-107 DATA; SHAPET :1377-1408,
+// SHAPEG oprop.f90:1912-1941, SINCSQ postsub.f90:2445-2484, SINC :2486-2525).
+static void scanfn_table(int jfn, double dxf, int nf, int nfmax, double xscale, std::vector<double> &xf) {
+  xf.assign(nfmax, 0.0);
+  const double pi = 2. * std::asin(1.);
+  if (jfn == 1) {
+    xf[0] = 0.5;
+    for (int i = 2; i <= nf; i++) {
+      const double x = (double)(i - 1) * dxf;
+      xf[i - 1] = 0.5 * (1. - 0.5 * x);
+    }
+  } else if (jfn == 2) {
+    const double fln2 = std::log(2.), recpi = 1. / pi;
+    const double fgnorm = std::sqrt(fln2 * recpi);
+    xf[0] = fgnorm * std::exp(-fln2 * 0.);
+    for (int jj = 2; jj <= nf; jj++) {
+      const double x = (double)(jj - 1) * dxf;
+      xf[jj - 1] = fgnorm * std::exp(-fln2 * (x * x));
+    }
+    xf[nf - 1] = 0.;
+  } else {
+    const double xnorm = xscale / pi;
+    xf[0] = xnorm;
+    for (int i = 2; i <= nf; i++) {
+      const double x = (double)(i - 1) * dxf * xscale;
+      const double sc = std::sin(x) / x;
+      xf[i - 1] = jfn == 3 ? xnorm * (sc * sc) : xnorm * sc;
+    }
+  }
+}
+
+int plan_scanfn(const lblx_scanfn *S, int iemit, double v1c, double dv, int64_t n_in, ScanFnPlan *P) {
+  if (!S || !P || !(dv > 0.) || n_in < 1) return LBLX_E_ARG;
+  // postsub.f90:74-107: HWJ, DXJ, NJ, NJMX, SMPLJ, XSCAL for JFN 0..4
+  static const double HWJ[5] = {1., 2., 4., 54.1826, 119.332818};
+  static const double DXJ[5] = {0., 0.02, 0.02, 0.02, 0.02};
+  static const int NJ[5] = {0, 101, 201, 2710, 5968}, NJMX[5] = {0, 251, 251, 2760, 6018};
+  static const double SMPLJ[5] = {.5, 2., 4., 4., 4.};
+  static const double XSCAL[5] = {0., 0., 0., 1.391557377, 1.89549425};
+  static const int JRATIO[24] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 25, 30, 32, 40, 48, 50,
+                                 60, 75, 80, 100, 120};
+  const int NSHIFT = 32;
+  double HWHM = S->hwhm;
+  if (!(HWHM > 0.)) return LBLX_E_ARG;
+  int jemit = S->jemit, jabs = 0;
+  if (jemit < -1 || jemit > 1 || (iemit != 0 && iemit != 1)) return LBLX_E_ARG;
+  if (jemit < 0) {  // :190-198
+    jabs = 1;
+    jemit = 0;
+  }
+  int jfn = S->jfn;
+  const int ifn = jfn < 0 ? -jfn : jfn;
+  if (ifn > 6) return LBLX_E_ARG;  // 'SCANF; JFN GT LIMIT'
+  if (ifn == 0 || ifn == 5 || ifn == 6 || S->jvar != 0) return LBLX_E_UNSUPPORTED;
+  const double pi = 2. * std::asin(1.);
+  if (jfn < 0) {  // :222-233: HWHM holds the first zero
+    jfn = -jfn;
+    if (jfn != 3 && jfn != 4) return LBLX_E_ARG;
+    HWHM = HWHM / (pi / XSCAL[jfn]);
+  }
+  const double HWF = HWJ[jfn], DXF = DXJ[jfn];
+  double SAMPLE = SMPLJ[jfn];
+  const double DVINT = HWHM / 12.;
+  if (S->sampl < 0.) SAMPLE = HWHM / (-S->sampl);
+  else if (S->sampl > 0.) SAMPLE = S->sampl;
+  if (dv > DVINT) return LBLX_E_UNSUPPORTED;  // SCNINT interpolates onto DVINT first
+  // JTREM (:325-335): 0 exp(-OD), 1 radiance, 2 transmittance
+  int jtrem = -1;
+  if (iemit == 0 && jemit == 0) jtrem = 0;
+  if (iemit == 1 && jemit == 0) jtrem = 2;
+  if (iemit == 1 && jemit == 1) jtrem = 1;
+  if (jtrem < 0) return LBLX_E_ARG;  // ' SCANF; JTREM LT 0'
+  const double V1 = S->v1, V2 = S->v2;
+  if (!(V2 > V1)) return LBLX_E_ARG;
+  const double DVO = HWHM / SAMPLE;
+  const int NBOUND = (int)((2. * HWF) * SAMPLE + 0.01);  // :384-394
+  const double BOUND = (double)NBOUND * DVO / 2.;
+  const double VFT = V1 - (double)NSHIFT * DVO;
+  const double VBOT = V1 - BOUND, VTOP = V2 + BOUND;
+  // SHRKSC's ratio (postsub.f90:1305-1322)
+  int R = 1;
+  double DVSC = dv;
+  {
+    const double dvsc = HWHM / 12.;
+    int irat = fint(dvsc / dv + 0.5);
+    for (int i = 2; i <= 24; i++)
+      if (JRATIO[i - 1] > irat) {
+        irat = JRATIO[i - 2];
+        break;
+      }
+    if (irat > JRATIO[23]) irat = JRATIO[23];
+    if (irat > 1) {
+      R = irat;
+      DVSC = (double)irat * dv;
+    }
+  }
+  // The input streams in panels of L points (RDSCAN :1439-1496); SHRKSC's
+  // average runs on from the first panel read, carrying NREN points
+  // across panels (:1298-1372); CONVSC takes each panel's ILO..IHI with
+  // ZPEAK = (I - 1) RATIO + (V1I - VFT) / DVO, and a group whose JMAX
+  // passes MAXF first has PANLSC write the R1 panel and move VFT on by
+  // NLIMF - 1 points (:1562-1566, 1633-1643).  The walk records each
+  // group's ZPEAK and R1 panel; it needs no data (a group of zeros adds
+  // nothing and moves no later group's panel).
+  const int64_t L = S->npanel > 0 ? S->npanel : 2400;
+  const int NLIMF = 2401;
+  const double ZBOUND = (HWF / DVO) * HWHM;
+  const int MAXF = NLIMF + 2 * NBOUND + NSHIFT;
+  const double V1SHFT = R > 1 ? (double)(R - 1) * dv / 2. : 0.;
+  int64_t o = -1, gbase = 0;
+  int nren = 0, q = 0;
+  double vft = VFT;
+  std::vector<double> zpk;
+  std::vector<int32_t> qg;
+  int64_t glo = -1, ghi = -1;
+  for (int64_t p0 = 0; p0 < n_in;) {
+    const int64_t np = std::min(L, n_in - p0);
+    double vmin = v1c + (double)p0 * dv;
+    const double vmax = v1c + (double)(p0 + np - 1) * dv;
+    const int64_t pstart = p0;
+    p0 += np;
+    if (vmax < VBOT) continue;  // skipped before any data: nren is 0
+    if (o < 0) o = pstart;
+    double dvi = dv;
+    const int nlow = nren > 0 ? nren + 1 : 1;
+    vmin = vmin - (double)(nlow - 1) * dvi;
+    const int nni = (int)np + nlow - 1;
+    int ilo = 1, ihi = nni, idata = 0;
+    const double dif = (vmin - VBOT) / dvi;
+    if (dif < 0.) ilo = fint(-dif + 1.5);
+    if (vmax > VTOP) {
+      ihi = fint((VTOP - vmin) / dvi + 1.5);
+      idata = 1;
+    }
+    double v1i = vmin;
+    int nlim = nni;
+    if (R > 1) {
+      nren = nlim - (nlim / R) * R;
+      v1i = v1i + V1SHFT;
+      nlim = (nlim - nren) / R;
+      dvi = DVSC;
+      ilo = fint((VBOT - v1i) / dvi + 1.5);
+      if (ilo < 1) ilo = 1;
+      ihi = fint((VTOP - v1i) / dvi + 1.5);
+      if (ihi > nlim) ihi = nlim;
+    }
+    const double ratio = dvi / DVO;
+    double zint = ((v1i - vft) / DVO);
+    for (int i = ilo; i <= ihi; i++) {
+      double zpeak = (double)(i - 1) * ratio + zint;
+      if (fint(zpeak + ZBOUND + 1.5) > MAXF) {  // PANLSC, then CONVSC again from I
+        vft = vft + (double)(NLIMF - 1) * DVO;
+        q++;
+        zint = ((v1i - vft) / DVO);
+        zpeak = (double)(i - 1) * ratio + zint;
+        if (fint(zpeak + ZBOUND + 1.5) > MAXF) return LBLX_E_ARG;  // a group spans a panel
+      }
+      const int64_t g = gbase + i - 1;
+      if (glo < 0) glo = g;
+      for (int64_t h = ghi + 1; ghi >= 0 && h < g; h++) {  // not convolved
+        zpk.push_back(0.);
+        qg.push_back(-1);
+      }
+      zpk.push_back(zpeak);
+      qg.push_back(q);
+      ghi = g;
+    }
+    gbase += nlim;
+    if (idata) break;  // CONVSC's IPANEL = IDATA: no further panel is read
+  }
+  if (o < 0 || glo < 0) return LBLX_E_ARG;  // 'END OF FILE ENCOUNTERED'
+  const int64_t K = (n_in - o) / R;
+  const double vg0 = v1c + (double)o * dv + V1SHFT;
+  P->jfn = jfn;
+  P->jtrem = jtrem;
+  P->jabs = jabs;
+  P->hwhm = HWHM;
+  P->dvo = DVO;
+  P->vft = VFT;
+  P->v1out = VFT + (double)NSHIFT * DVO;
+  P->nout = (int64_t)((V2 - VFT) / DVO + 1.5) - NSHIFT;  // PANLSC NNHI (:1627-1630)
+  if (P->nout < 1) return LBLX_E_ARG;
+  P->R = R;
+  P->o = o;
+  P->K = K;
+  P->vg0 = vg0;
+  P->dvsc = DVSC;
+  P->g_lo = glo;
+  P->g_hi = ghi;
+  P->maxf = MAXF;
+  P->zpk.swap(zpk);
+  P->q.swap(qg);
+  // CONVSC's constants (:1541-1560), JVAR = 0
+  P->zbound = ZBOUND;
+  P->zslope = (DVO / DXF) / HWHM;
+  P->xnorm = DVSC / HWHM;
+  scanfn_table(jfn, DXF, NJ[jfn], NJMX[jfn], XSCAL[jfn], P->xf);
+  return LBLX_OK;
+}
+
 int plan_scan(const lblx_scan *S, double v1c, double dv, int64_t n_in, ScanPlan *P, const ScanTab *fixed) {
   if (!S || !P || !(dv > 0.) || n_in < 4) return LBLX_E_ARG;
   *P = ScanPlan{};

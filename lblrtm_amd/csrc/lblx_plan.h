@@ -133,6 +133,31 @@ struct ScanTab {
 int plan_scan(const lblx_scan *scan, double v1c, double dv, int64_t n_in, ScanPlan *P,
               const ScanTab *fixed = nullptr);
 
+// SCANFN (postsub.f90:25-560) for one input spectrum: the function table,
+// SHRKSC's average (ratio R from JRATIO, running from the first point of
+// the first input panel reaching VBOT), the shrunk points CONVSC takes
+// (g_lo..g_hi), and CONVSC's grid constants in one global output index J
+// (1-based, v = vft + (J - 1) dvo), whose points NSHIFT + 1 .. NSHIFT +
+// nout PANLSC writes.
+struct ScanFnPlan {
+  int jfn, jtrem, jabs;
+  double hwhm, dvo, vft, v1out;
+  int64_t nout;
+  int R;                 // SHRKSC ratio (1: no average)
+  int64_t o;             // first input point of the first group
+  int64_t K;             // groups (the last partial group is dropped)
+  double vg0, dvsc;      // wavenumber of group 0, group spacing
+  int64_t g_lo, g_hi;    // groups CONVSC convolves
+  double zbound, zslope, xnorm;
+  int maxf;              // R1 buffer length NLIMF + 2 NBOUND + NSHIFT
+  std::vector<double> xf;  // XF(1..NFMAX) at xf[0..NFMAX-1]
+  // per group g_lo..g_hi: ZPEAK as CONVSC computes it, in the R1 panel q
+  // current when the group is convolved (q = -1: outside ILO..IHI)
+  std::vector<double> zpk;
+  std::vector<int32_t> q;
+};
+int plan_scanfn(const lblx_scanfn *scan, int iemit, double v1c, double dv, int64_t n_in, ScanFnPlan *P);
+
 // Build the plan; layer->jrad is updated as OPDPTH does.  Returns LBLX_OK
 // or an LBLX_E_* code.
 // icflg > 0: the layer is an analytic-Jacobian single-species run (CONTNM's

@@ -154,4 +154,90 @@ __global__ void k_scan_out(const double2 *z, int64_t off, int64_t nz, double rn,
            cof[406 + c] * scan_pt(z, off, nz, rn, q + 1) + cof[609 + c] * scan_pt(z, off, nz, rn, q + 2);
 }
 
+// ---------------------------------------------------------------------
+// SCANFN (src/postsub.f90:25-560) in the spectral domain.
+
+// RDSCAN's transform of one input point (postsub.f90:1465-1485): an
+// optical depth becomes its transmittance (linear below 1e-4, EXPMIN past
+// ARGMIN = 34); JABS takes 1 - T.
+__device__ __forceinline__ double sfn_in(double si, int jtrem, int jabs) {
+  double v = si;
+  if (jtrem == 0) {
+    v = 1.;
+    if (si > 1.0e-04) v = si < 34. ? exp(-si) : exp(-34.);
+    else v = 1. - si;
+  }
+  return jabs ? 1. - v : v;
+}
+
+// SHRKSC (postsub.f90:1326-1338): group g is the mean of input points
+// o + g R .. o + g R + R - 1, added in order.
+__global__ void k_sfn_shrink(const double *in, int64_t o, int R, int64_t K, int jtrem, int jabs,
+                             double *s) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= K) return;
+  const double *p = in + o + g * (int64_t)R;
+  if (R == 1) {
+    s[g] = sfn_in(p[0], jtrem, jabs);
+    return;
+  }
+  double sum = 0.;
+  for (int m = 0; m < R; m++) sum = sum + sfn_in(p[m], jtrem, jabs);
+  s[g] = sum / (double)R;
+}
+
+// CONVSC (postsub.f90:1516-1598) gathered per output point.  Group g was
+// convolved into R1 panel q_g (R1 moves by NLIMF - 1 = 2400 points per
+// panel), with its peak at ZPEAK_g in that panel's index: it adds
+// XNORM S_g XF(IT) to the points JMIN_g .. JMAX_g, IT = INT(|ZF| + 1.5),
+// ZF advanced by ZSLOPE from (JMIN_g - 1 - ZPEAK_g) ZSLOPE point by point.
+// ZF is formed directly, and where |ZF| + 1.5 lies within 1e-6 of an
+// integer (the running sum could round across it) by CONVSC's running sum
+// itself.  The groups that reach a point are added in order of g, as
+// CONVSC's loop over the input adds them; output n is R1 point
+// NSHIFT + 1 + n of the global numbering.
+struct SfnArgs {
+  double z0, ratio, zbound, zslope, xnorm;
+  int64_t g_lo, g_hi;
+  int64_t j0;  // global index J of output 0
+  int nfmax;
+};
+__global__ void k_sfn_conv(SfnArgs A, const double *s, const double *zpk, const int32_t *qg, const double *xf,
+                           int64_t nout, double *out) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nout) return;
+  const int64_t J = A.j0 + n;
+  // global peak z0 + g RATIO within [J - 1.5 - ZBOUND, J - 0.5 + ZBOUND],
+  // widened by three groups for the rounding of the per-panel ZPEAK
+  int64_t ga = (int64_t)floor(((double)(J - 1) - 0.5 - A.zbound - A.z0) / A.ratio) - 3;
+  int64_t gb = (int64_t)floor(((double)(J - 1) + 0.5 + A.zbound - A.z0) / A.ratio) + 3;
+  if (ga < A.g_lo) ga = A.g_lo;
+  if (gb > A.g_hi) gb = A.g_hi;
+  double r = 0.;
+  for (int64_t g = ga; g <= gb; g++) {
+    const int64_t k = g - A.g_lo;
+    const int q = qg[k];
+    if (q < 0) continue;
+    const double zp = zpk[k];
+    const int64_t jl = J - 2400 * (int64_t)q;  // index in the group's R1 panel
+    const int64_t jmax = (int64_t)(zp + A.zbound + 1.5);
+    int64_t jmin = (int64_t)(zp - A.zbound + 1.5);
+    if (jmin < 1) jmin = 1;
+    if (jl < jmin || jl > jmax) continue;
+    const double sg = s[g];
+    if (sg == 0.) continue;
+    double zf = ((double)(jl - 1) - zp) * A.zslope;
+    const double t = fabs(zf) + 1.5;
+    if (fabs(t - rint(t)) < 1e-6) {  // near a step of the table index
+      zf = ((double)(jmin - 1) - zp) * A.zslope;
+      for (int64_t jf = jmin; jf < jl; jf++) zf = zf + A.zslope;
+    }
+    int it = (int)(fabs(zf) + 1.5);
+    if (it > A.nfmax) it = A.nfmax;  // beyond the table: its zero tail
+    const double si = A.xnorm * sg;
+    r = r + si * xf[it - 1];
+  }
+  out[n] = r;
+}
+
 }  // namespace lblx

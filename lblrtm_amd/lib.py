@@ -71,6 +71,10 @@ def load():
         "lblx_fftscn_device": (C.c_int, [C.c_void_p, P(abi.Scan), C.c_void_p, C.c_double, C.c_double,
                                          C.c_int64, C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_double),
                                          P(C.c_double)]),
+        "lblx_scanfn_size": (C.c_int, [P(abi.ScanFn), C.c_int, C.c_double, C.c_double, C.c_int64,
+                                       P(C.c_int64), P(C.c_double), P(C.c_double)]),
+        "lblx_scanfn_device": (C.c_int, [C.c_void_p, P(abi.ScanFn), C.c_int, C.c_void_p, C.c_double,
+                                         C.c_double, C.c_int64, C.c_void_p, C.c_int64]),
         "lblx_od_panels": (C.c_int, [P(abi.Run), P(abi.Layer), C.c_int, P(C.c_double), P(C.c_double),
                                      P(C.c_int32), P(C.c_int)]),
         "lblx_write_panels": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_int,
@@ -223,6 +227,16 @@ def scan_size(scan, v1_in, dv_in, n_in):
     dv = C.c_double(0.0)
     _chk(load().lblx_scan_size(C.byref(scan), v1_in, dv_in, n_in, C.byref(n), C.byref(v1), C.byref(dv)),
          "lblx_scan_size")
+    return n.value, v1.value, dv.value
+
+
+def scanfn_size(scan, iemit, v1_in, dv_in, n_in):
+    """(points, v1, dv) of SCANFN's output for an input on v1_in + k dv_in."""
+    n = C.c_int64(0)
+    v1 = C.c_double(0.0)
+    dv = C.c_double(0.0)
+    _chk(load().lblx_scanfn_size(C.byref(scan), iemit, v1_in, dv_in, n_in, C.byref(n), C.byref(v1),
+                                 C.byref(dv)), "lblx_scanfn_size")
     return n.value, v1.value, dv.value
 
 
@@ -587,6 +601,25 @@ class Context:
                                                   C.c_void_p(d_out), cap, C.byref(n), C.byref(v1), C.byref(dv)),
              "lblx_fftscn_noalloc_device")
         return n.value, v1.value, dv.value
+
+    def scanfn_device(self, scan, iemit, d_in, v1_in, dv_in, n_in, d_out, n_out):
+        """SCANFN on device pointers (spectral-domain scanning function)."""
+        _chk(self._lib.lblx_scanfn_device(self._h, C.byref(scan), iemit, C.c_void_p(d_in), v1_in, dv_in, n_in,
+                                          C.c_void_p(d_out), n_out), "lblx_scanfn_device")
+
+    def scanfn(self, scan, spectrum, v1_in, dv_in, iemit=1):
+        """Host-side convenience: SCANFN of a spectrum held in host memory;
+        returns (output, v1, dv)."""
+        x = np.ascontiguousarray(spectrum, np.float64)
+        n, v1, dv = scanfn_size(scan, iemit, v1_in, dv_in, len(x))
+        bi, bo = self.alloc(x.nbytes), self.alloc(8 * max(n, 1))
+        try:
+            bi.from_host(x)
+            self.scanfn_device(scan, iemit, bi.ptr, v1_in, dv_in, len(x), bo.ptr, n)
+            return bo.to_host(np.float64, n), v1, dv
+        finally:
+            bi.free()
+            bo.free()
 
     def fftscn(self, scan, spectrum, v1_in, dv_in):
         """Host-side convenience: scan a spectrum held in host memory."""

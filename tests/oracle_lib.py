@@ -64,6 +64,9 @@ def lib():
         L.oracle_fftscn.restype = C.c_int
         L.oracle_fftscn.argtypes = [C.POINTER(abi.Scan), dp, C.c_double, C.c_double, C.c_int64, dp,
                                     C.c_int64, C.POINTER(C.c_int64), dp, dp]
+        L.oracle_scanfn.restype = C.c_int
+        L.oracle_scanfn.argtypes = [C.POINTER(abi.ScanFn), C.c_int, dp, C.c_double, C.c_double, C.c_int64, dp,
+                                    C.c_int64, C.POINTER(C.c_int64), dp, dp]
         L.oracle_merge_quirks.restype = C.c_int64
         L.oracle_merge_quirks.argtypes = []
         _LIB = L
@@ -196,6 +199,19 @@ def fftscn(scan, spectrum, v1_in, dv_in, cap=4000000):
                              abi.ptr(out, C.c_double), cap, C.byref(n), C.byref(v1), C.byref(dv))
     if rc:
         raise RuntimeError("oracle_fftscn rc=%d" % rc)
+    return out[:n.value].copy(), v1.value, dv.value
+
+
+def scanfn(scan, spectrum, v1_in, dv_in, iemit=1, cap=4000000):
+    """The oracle's SCANFN, panel by panel: (out, v1, dv)."""
+    x = np.ascontiguousarray(spectrum, np.float64)
+    out = np.zeros(cap)
+    n = C.c_int64(0)
+    v1, dv = C.c_double(0.0), C.c_double(0.0)
+    rc = lib().oracle_scanfn(C.byref(scan), iemit, abi.ptr(x, C.c_double), v1_in, dv_in, len(x),
+                             abi.ptr(out, C.c_double), cap, C.byref(n), C.byref(v1), C.byref(dv))
+    if rc:
+        raise RuntimeError("oracle_scanfn rc=%d" % rc)
     return out[:n.value].copy(), v1.value, dv.value
 
 
