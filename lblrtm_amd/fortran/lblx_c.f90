@@ -36,6 +36,11 @@ module lblx_c
      integer(c_int32_t) :: jemit, jfnin, mratin, ivx, nofix, pad
   end type lblx_scan
 
+  type, bind(c) :: lblx_scanfn
+     real(c_double) :: hwhm, v1, v2, sampl
+     integer(c_int32_t) :: jemit, jfn, jvar, npanel
+  end type lblx_scanfn
+
   interface
      integer(c_int) function lblx_create(ctx, device) bind(c, name="lblx_create")
        import :: c_int, c_ptr
@@ -119,6 +124,26 @@ module lblx_c
        integer(c_int64_t) :: n_out
        real(c_double) :: v1_out, dv_out
      end function lblx_scan_size
+     integer(c_int) function lblx_scanfn_size(scan, iemit, v1_in, dv_in, n_in, n_out, v1_out, dv_out) &
+          bind(c, name="lblx_scanfn_size")
+       import :: c_int, c_double, c_int64_t, lblx_scanfn
+       type(lblx_scanfn), intent(in) :: scan
+       integer(c_int), value :: iemit
+       real(c_double), value :: v1_in, dv_in
+       integer(c_int64_t), value :: n_in
+       integer(c_int64_t) :: n_out
+       real(c_double) :: v1_out, dv_out
+     end function lblx_scanfn_size
+     integer(c_int) function lblx_scanfn_device(ctx, scan, iemit, d_in, v1_in, dv_in, n_in, d_out, n_out) &
+          bind(c, name="lblx_scanfn_device")
+       import :: c_int, c_ptr, c_double, c_int64_t, lblx_scanfn
+       type(c_ptr), value :: ctx
+       type(lblx_scanfn), intent(in) :: scan
+       integer(c_int), value :: iemit
+       type(c_ptr), value :: d_in, d_out
+       real(c_double), value :: v1_in, dv_in
+       integer(c_int64_t), value :: n_in, n_out
+     end function lblx_scanfn_device
      integer(c_int) function lblx_fftscn_device(ctx, scan, d_in, v1_in, dv_in, n_in, d_out, cap, n_out, &
           v1_out, dv_out) bind(c, name="lblx_fftscn_device")
        import :: c_int, c_ptr, c_double, c_int64_t, lblx_scan

@@ -21,5 +21,7 @@ int main(void)
   printf("size lblx_stats %zu\n", sizeof(lblx_stats));
   F(lblx_scan, hwhm); F(lblx_scan, param); F(lblx_scan, jemit); F(lblx_scan, nofix);
   printf("size lblx_scan %zu\n", sizeof(lblx_scan));
+  F(lblx_scanfn, hwhm); F(lblx_scanfn, sampl); F(lblx_scanfn, jemit); F(lblx_scanfn, npanel);
+  printf("size lblx_scanfn %zu\n", sizeof(lblx_scanfn));
   return 0;
 }

@@ -10,6 +10,7 @@ program abi_layout
   type(lblx_layer), target :: l
   type(lblx_stats), target :: s
   type(lblx_scan), target :: c
+  type(lblx_scanfn), target :: sf
   real(c_double) :: pave(1), tave(1), wk(1), wb(1), dv(1), q1(1)
   integer(c_int32_t) :: it(1), qn(1)
   integer(c_int64_t) :: n
@@ -53,6 +54,11 @@ program abi_layout
   call f('lblx_scan', 'jemit', c_loc(c), c_loc(c%jemit), c_sizeof(c%jemit))
   call f('lblx_scan', 'nofix', c_loc(c), c_loc(c%nofix), c_sizeof(c%nofix))
   write (*, '(a,1x,a,1x,i0)') 'size', 'lblx_scan', c_sizeof(c)
+  call f('lblx_scanfn', 'hwhm', c_loc(sf), c_loc(sf%hwhm), c_sizeof(sf%hwhm))
+  call f('lblx_scanfn', 'sampl', c_loc(sf), c_loc(sf%sampl), c_sizeof(sf%sampl))
+  call f('lblx_scanfn', 'jemit', c_loc(sf), c_loc(sf%jemit), c_sizeof(sf%jemit))
+  call f('lblx_scanfn', 'npanel', c_loc(sf), c_loc(sf%npanel), c_sizeof(sf%npanel))
+  write (*, '(a,1x,a,1x,i0)') 'size', 'lblx_scanfn', c_sizeof(sf)
 
   ! the single-layer workload's run and layer through the host-only calls
   r = lblx_run(600.d0, 1100.d0, 4.d0, 0.04d0, 36.d0, 2.d-4, 1.d-3, 1, 0, 0, LBLX_FP64, &
