@@ -249,6 +249,32 @@ int lblx_spectrum_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers,
                          const double *tzu, double tbnd, double emis, double *d_rad,
                          double *d_tra, int64_t n_out, double *v1p_top, lblx_stats *st);
 
+/* XMERGE for either looking direction (xmerge.f90:25-167, RADMRG
+ * :2587-2990, RADNN :2991-3240).  ipathl = 1 is lblx_xmerge_device.
+ * ipathl = 3, downwelling radiance at an observer looking up (layers taken
+ * outward from the observer, each new layer on a grid no coarser than the
+ * last): EMIN with tz = the layer's boundary temperature on the observer's
+ * side (TZL); layer_old == NULL is EMINIT's layer emission, R = E, T = T;
+ * otherwise RADNN's IPATH_FLG = 3, R = L4(R_old) + E*L4(T_old),
+ * T = T*L4(T_old), and tbnd > 0 adds EMBND (xmerge.f90:3652-3733),
+ * R += T*emis*B(tbnd): XMERGE passes TBND to the path's last layer only
+ * (LAYER = LH2, :140).  Other ipathl: LBLX_E_UNSUPPORTED.                 */
+int lblx_xmerge_path_device(lblx_ctx *ctx, const lblx_run *run, const lblx_layer *layer_old,
+                            const lblx_layer *layer, int ipathl, double tz, double tbnd, double emis,
+                            const void *d_od, const double *d_rold, const double *d_told,
+                            double *d_rnew, double *d_tnew);
+
+/* A path's spectrum in either direction: OPDPTH for all nlay layers in one
+ * batch, then XMERGE in path order (layers[0] next to the observer for
+ * ipathl = 3, at the surface for ipathl = 1); radiance and transmittance on
+ * the last layer's grid.  tz[l]: TZU (ipathl 1) or TZL (ipathl 3); tbnd,
+ * emis: the surface (ipathl 1, EMINIT) or a boundary beyond the last layer
+ * (ipathl 3, EMBND after the last merge of a path of two or more layers;
+ * tbnd = 0: space).                                                      */
+int lblx_spectrum_path_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay,
+                              int ipathl, const double *tz, double tbnd, double emis, double *d_rad,
+                              double *d_tra, int64_t n_out, double *v1p_end, lblx_stats *st);
+
 /* Jacobians of the TOA upwelling radiance (the forward/backward pair of
  * lblx_spectrum_device) with respect to every layer's average temperature
  * and the log column amounts of nspec molecules (species[] 1-based ids).

@@ -1875,11 +1875,15 @@ static MergeArgs merge_args(const LayerPlan &K, const LayerPlan *O, int itype, d
 }
 
 // XMERGE (IEMIT=1, IPATHL=1) for one layer on device buffers.
+// Upwelling (down = false): tbnd is the surface of the first layer
+// (EMINIT).  Downwelling: tbnd > 0 is EMBND after a later layer's merge.
 static int merge_launch(lblx_ctx *c, const LayerPlan &K, const LayerPlan *O, int itype, double tave,
-                        double tzu, double tbnd, double emis, const void *d_od, bool od32,
+                        double tzu, double tbnd, double emis, bool down, const void *d_od, bool od32,
                         const double *d_rold, const double *d_told, double *d_rnew, double *d_tnew,
                         const DevPanel *d_pan) {
-  MergeArgs A = merge_args(K, O, itype, tave, tzu, tbnd, emis);
+  MergeArgs A = merge_args(K, O, itype, tave, tzu, (O == nullptr && !down) ? tbnd : 0., emis);
+  A.down = down ? 1 : 0;
+  if (down && O != nullptr && tbnd > 0.) A.xktend = tbnd / RADCN2;
   int th = c->tbegin(ST_MERGE);
   dim3 grid((unsigned)((A.n + 255) / 256));
   if (od32)
@@ -1913,11 +1917,12 @@ int lblx_merge_size(const lblx_run *run, const lblx_layer *layer_old, const lblx
   return LBLX_OK;
 }
 
-int lblx_xmerge_device(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer_old,
-                       const lblx_layer *layer, double tzu, double tbnd, double emis,
+static int xmerge_path(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer_old,
+                       const lblx_layer *layer, int ipathl, double tzu, double tbnd, double emis,
                        const void *d_od, const double *d_rold, const double *d_told, double *d_rnew,
                        double *d_tnew) {
   if (!c || !run || !layer || !d_od || !d_rnew || !d_tnew) return LBLX_E_ARG;
+  if (ipathl != 1 && ipathl != 3) return LBLX_E_UNSUPPORTED;
   if (layer_old && (!d_rold || !d_told)) return LBLX_E_ARG;
   HIPCHK(hipSetDevice(c->device));
   lblx_layer tmp = *layer;
@@ -1938,18 +1943,38 @@ int lblx_xmerge_device(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer
   c->stage_begin();
   HIPCHK(c->h2d(c->w_mpan.p, pan.data(), pan.size() * sizeof(DevPanel), c->stream));
   c->stage_end(c->stream);
-  return merge_launch(c, K, layer_old ? &O : nullptr, itype, layer->tave, tzu,
-                      layer_old ? 0. : tbnd, emis, d_od, run->precision == LBLX_FP32_SPLIT, d_rold,
-                      d_told, d_rnew, d_tnew, c->w_mpan.as<DevPanel>());
+  return merge_launch(c, K, layer_old ? &O : nullptr, itype, layer->tave, tzu, tbnd, emis, ipathl == 3, d_od,
+                      run->precision == LBLX_FP32_SPLIT, d_rold, d_told, d_rnew, d_tnew,
+                      c->w_mpan.as<DevPanel>());
 }
 
-// OPDPTH for every layer of a column in one batch, then XMERGE layer by
-// layer from the surface up: upwelling radiance and transmittance at the
-// top of the atmosphere on the top layer's grid.
-int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
+int lblx_xmerge_device(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer_old,
+                       const lblx_layer *layer, double tzu, double tbnd, double emis,
+                       const void *d_od, const double *d_rold, const double *d_told, double *d_rnew,
+                       double *d_tnew) {
+  return xmerge_path(c, run, layer_old, layer, 1, tzu, tbnd, emis, d_od, d_rold, d_told, d_rnew, d_tnew);
+}
+
+int lblx_xmerge_path_device(lblx_ctx *c, const lblx_run *run, const lblx_layer *layer_old,
+                            const lblx_layer *layer, int ipathl, double tz, double tbnd, double emis,
+                            const void *d_od, const double *d_rold, const double *d_told, double *d_rnew,
+                            double *d_tnew) {
+  return xmerge_path(c, run, layer_old, layer, ipathl, tz, tbnd, emis, d_od, d_rold, d_told, d_rnew, d_tnew);
+}
+
+// OPDPTH for every layer of a path in one batch, then XMERGE layer by
+// layer in path order, on the last layer's grid: IPATHL = 1, upwelling
+// radiance and transmittance at the top from the surface up (tzu: each
+// layer's upper boundary temperature, tbnd/emis the surface, EMINIT);
+// IPATHL = 3, downwelling radiance at the observer from the layer next to
+// it outward (tz: each layer's boundary temperature on the observer's
+// side, TZL; tbnd/emis a boundary beyond the last layer, EMBND).
+static int spectrum_path(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay, int ipathl,
                          const double *tzu, double tbnd, double emis, double *d_rad, double *d_tra,
                          int64_t n_out, double *v1p_top, lblx_stats *st) {
   if (!c || !run || !layers || nlay <= 0 || !tzu || !d_rad || !d_tra) return LBLX_E_ARG;
+  if (ipathl != 1 && ipathl != 3) return LBLX_E_UNSUPPORTED;
+  const bool down = ipathl == 3;
   HIPCHK(hipSetDevice(c->device));
   std::vector<LayerPlan> plans(nlay);
   std::vector<int64_t> off(nlay);
@@ -2016,7 +2041,11 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
     const double *ro = q > 0 ? R[(q - 1) & 1] : nullptr;
     const double *to = q > 0 ? T[(q - 1) & 1] : nullptr;
     MergeArgs A = merge_args(plans[l], l > 0 ? &plans[l - 1] : nullptr, itype[l], layers[l].tave, tzu[l],
-                             l == 0 ? tbnd : 0., emis);
+                             (l == 0 && !down) ? tbnd : 0., emis);
+    A.down = down ? 1 : 0;
+    // EMBND after the last merge of a downwelling path (RADMRG with TBND,
+    // xmerge.f90:137-141, 2980); a one-layer path is EMINIT's alone
+    if (down && last == nlay - 1 && nlay > 1 && tbnd > 0.) A.xktend = tbnd / RADCN2;
     const RunLayer *drl = c->w_mrun.as<RunLayer>() + l;
     const DevPanel *dp = c->w_mpan.as<DevPanel>() + pan0[l];
     int th = c->tbegin(ST_MERGE);
@@ -2032,6 +2061,18 @@ int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, i
   }
   if (v1p_top) *v1p_top = plans[nlay - 1].panels[0].vft + (double)NSHIFT * plans[nlay - 1].dv;
   return LBLX_OK;
+}
+
+int lblx_spectrum_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay,
+                         const double *tzu, double tbnd, double emis, double *d_rad, double *d_tra,
+                         int64_t n_out, double *v1p_top, lblx_stats *st) {
+  return spectrum_path(c, run, layers, nlay, 1, tzu, tbnd, emis, d_rad, d_tra, n_out, v1p_top, st);
+}
+
+int lblx_spectrum_path_device(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay, int ipathl,
+                              const double *tz, double tbnd, double emis, double *d_rad, double *d_tra,
+                              int64_t n_out, double *v1p_end, lblx_stats *st) {
+  return spectrum_path(c, run, layers, nlay, ipathl, tz, tbnd, emis, d_rad, d_tra, n_out, v1p_end, st);
 }
 
 // ABSMRG's walk for a merge of layer K's OD onto the accumulated OD on layer

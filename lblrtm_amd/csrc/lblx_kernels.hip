@@ -1948,11 +1948,20 @@ __global__ __launch_bounds__(256) void k_contnm(const ContPlan *cps, ContTabs TB
 // point base_p + j - 1 (base_p from the host's replay of RADMRG's panel
 // reads, in DevPanel::out0); RADMRG duplicates the first old point below
 // slot 1 and the last one past the end, which the index clamp reproduces.
+// down (IPATHL = 3, downwelling radiance at the observer, layers taken
+// outward from it): EMIN's boundary temperature is the layer's side toward
+// the observer (TZL), EMINIT adds no boundary, and RADNN's IPATH_FLG = 3
+// branch (xmerge.f90:3022-3026, 3151-3159) adds the new layer behind the
+// old ones: R = L4(R_old) + E L4(T_old), T = T L4(T_old).  xktend > 0:
+// EMBND (xmerge.f90:3652-3733) after the merge, R += T eps B(Tbnd), for
+// the last layer of a downwelling path.
 struct MergeArgs {
   double dv, xkt, xkta, xktbnd, emis;
   int64_t n, nold;
   int32_t first, itype;
   double a1[12], a2[12], a3[12], a4[12];
+  int32_t down, pad;
+  double xktend;
 };
 
 // The quotients use the reciprocal of rcp_nr (within 2.2e-15 relative).
@@ -2026,17 +2035,21 @@ __global__ void k_merge(MergeArgs A, const DevPanel *pan, const IN *od, const do
   const double vi = (pan[p].vft + 32. * A.dv) + (double)k * A.dv;
   double tr;
   double em = merge_em(vi, (double)od[g], A.xkt, A.xkta, &tr);
+  double r, t;
   if (A.first) {
-    if (A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
-    rnew[g] = em;
-    tnew[g] = tr;
-    return;
+    if (!A.down && A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
+    r = em;
+    t = tr;
+  } else {
+    MergeTap q = merge_tap(A, pan[p].out0, k);
+    double ro = q.w0 * rold[q.i0] + q.w1 * rold[q.i1] + q.w2 * rold[q.i2] + q.w3 * rold[q.i3];
+    double to = q.w0 * told[q.i0] + q.w1 * told[q.i1] + q.w2 * told[q.i2] + q.w3 * told[q.i3];
+    r = A.down ? ro + em * to : tr * ro + em;
+    t = tr * to;
   }
-  MergeTap t = merge_tap(A, pan[p].out0, k);
-  double ro = t.w0 * rold[t.i0] + t.w1 * rold[t.i1] + t.w2 * rold[t.i2] + t.w3 * rold[t.i3];
-  double to = t.w0 * told[t.i0] + t.w1 * told[t.i1] + t.w2 * told[t.i2] + t.w3 * told[t.i3];
-  rnew[g] = tr * ro + em;
-  tnew[g] = tr * to;
+  if (A.xktend > 0.) r = r + t * A.emis * d_planck(vi, A.xktend);
+  rnew[g] = r;
+  tnew[g] = t;
 }
 
 // A run of layers on one grid whose merges after the first are the
@@ -2064,24 +2077,25 @@ __global__ void k_merge_run(MergeArgs A, const RunLayer *rl, int nrun, const Dev
   double em = merge_em(vi, (double)odb[rl[0].od + g], A.xkt, A.xkta, &tr);
   double r, t;
   if (A.first) {
-    if (A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
+    if (!A.down && A.xktbnd > 0.) em = em + tr * d_planck(vi, A.xktbnd) * A.emis;
     r = em;
     t = tr;
   } else {
     MergeTap q = merge_tap(A, pan[p].out0, k);
     double ro = q.w0 * rold[q.i0] + q.w1 * rold[q.i1] + q.w2 * rold[q.i2] + q.w3 * rold[q.i3];
     double to = q.w0 * told[q.i0] + q.w1 * told[q.i1] + q.w2 * told[q.i2] + q.w3 * told[q.i3];
-    r = tr * ro + em;
+    r = A.down ? ro + em * to : tr * ro + em;
     t = tr * to;
   }
   if (rl[0].rout >= 0) rall[rl[0].rout + g] = r;
   for (int j = 1; j < nrun; j++) {
     const RunLayer L = rl[j];
     em = merge_em(vi, (double)odb[L.od + g], L.xkt, L.xkta, &tr);
-    r = tr * r + em;
+    r = A.down ? r + em * t : tr * r + em;
     t = tr * t;
     if (L.rout >= 0) rall[L.rout + g] = r;
   }
+  if (A.xktend > 0.) r = r + t * A.emis * d_planck(vi, A.xktend);
   rnew[g] = r;
   tnew[g] = t;
 }

@@ -123,6 +123,12 @@ def load():
         "lblx_spectrum_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int,
                                            P(C.c_double), C.c_double, C.c_double, C.c_void_p,
                                            C.c_void_p, C.c_int64, P(C.c_double), P(abi.Stats)]),
+        "lblx_spectrum_path_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, C.c_int,
+                                                P(C.c_double), C.c_double, C.c_double, C.c_void_p,
+                                                C.c_void_p, C.c_int64, P(C.c_double), P(abi.Stats)]),
+        "lblx_xmerge_path_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), P(abi.Layer), C.c_int,
+                                              C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -344,6 +350,35 @@ class Context:
         for k in range(n):
             layers[k].jrad = arr[k].jrad
         return v1p.value, ([x.as_dict() for x in st] if stats else None)
+
+    def xmerge_path_device(self, run, layer_old, layer, ipathl, tz, tbnd, emis, d_od, d_rold, d_told,
+                           d_rnew, d_tnew):
+        """XMERGE of one layer for IPATHL 1 or 3 (downwelling: tz = TZL,
+        tbnd > 0 adds EMBND after a later layer's merge)."""
+        _chk(self._lib.lblx_xmerge_path_device(
+            self._h, C.byref(run), C.byref(layer_old) if layer_old is not None else None,
+            C.byref(layer), ipathl, tz, tbnd, emis, C.c_void_p(d_od), C.c_void_p(d_rold or 0),
+            C.c_void_p(d_told or 0), C.c_void_p(d_rnew), C.c_void_p(d_tnew)), "lblx_xmerge_path_device")
+
+    def spectrum_path(self, run, layers, ipathl, tz, tbnd, emis):
+        """A path's radiance and transmittance in either direction
+        (lblx_spectrum_path_device): (rad, tra, v1p, dv) on the last
+        layer's grid."""
+        n = od_size(run, layers[-1])
+        nl = len(layers)
+        arr = (abi.Layer * nl)(*layers)
+        t = np.ascontiguousarray(tz, np.float64)
+        rad, tra = self.alloc(8 * n), self.alloc(8 * n)
+        v1p = C.c_double(0.0)
+        try:
+            _chk(self._lib.lblx_spectrum_path_device(self._h, C.byref(run), arr, nl, ipathl,
+                                                     abi.ptr(t, C.c_double), tbnd, emis, C.c_void_p(rad.ptr),
+                                                     C.c_void_p(tra.ptr), n, C.byref(v1p), None),
+                 "lblx_spectrum_path_device")
+            return rad.to_host(np.float64), tra.to_host(np.float64), v1p.value, layers[-1].dv
+        finally:
+            rad.free()
+            tra.free()
 
     def spectrum(self, run, layers, tzu, tbnd, emis):
         """Host-side convenience over spectrum_device: returns (radiance,

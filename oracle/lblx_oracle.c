@@ -2178,11 +2178,32 @@ static int64_t g_merge_quirks = 0;
  * Zero for grids whose merge equals the point-wise interpolation.         */
 int64_t oracle_merge_quirks(void) { return g_merge_quirks; }
 
+/* ipathl = 3 (downwelling at an observer looking up): RADNN's
+ * IPATH_FLG = 3 branches (src/xmerge.f90:3022-3026, 3151-3159) with EMIN
+ * at the layer's TZL (passed as tzu), and EMBND (:3652-3733) after the
+ * merge when tbnd > 0.                                                    */
+int oracle_merge_next_path(int npo, const double *v1po, const double *v2po, const int *nlimo, double dvo,
+                           const double *rado_in, const double *trao_in, int npk, const double *v1pk,
+                           const double *v2pk, const int *nlimk, double dvk, const double *od, double tave,
+                           double tzu, double v2, int ipathl, double tbnd, double emis, double *rad,
+                           double *tra);
+
 int oracle_merge_next(int npo, const double *v1po, const double *v2po, const int *nlimo, double dvo,
                       const double *rado_in, const double *trao_in, int npk, const double *v1pk,
                       const double *v2pk, const int *nlimk, double dvk, const double *od, double tave,
                       double tzu, double v2, double *rad, double *tra)
 {
+  return oracle_merge_next_path(npo, v1po, v2po, nlimo, dvo, rado_in, trao_in, npk, v1pk, v2pk, nlimk, dvk, od,
+                                tave, tzu, v2, 1, 0., 0., rad, tra);
+}
+
+int oracle_merge_next_path(int npo, const double *v1po, const double *v2po, const int *nlimo, double dvo,
+                           const double *rado_in, const double *trao_in, int npk, const double *v1pk,
+                           const double *v2pk, const int *nlimk, double dvk, const double *od, double tave,
+                           double tzu, double v2, int ipathl, double tbnd, double emis, double *rad,
+                           double *tra)
+{
+  if (ipathl != 1 && ipathl != 3) return LBLX_E_UNSUPPORTED;
   g_merge_quirks = 0;
   int itype;
   if (dvo == dvk) itype = 0;
@@ -2248,7 +2269,8 @@ int oracle_merge_next(int npo, const double *v1po, const double *v2po, const int
         for (int I = NL; I <= NLIM; I += LL) {
           IPL = IPL + LLM1;
           if (IPL > lim_ok) g_merge_quirks++;
-          radn[I - 1] = tran[I - 1] * RADO[IPL] + radn[I - 1];
+          if (ipathl == 3) radn[I - 1] = RADO[IPL] + radn[I - 1] * TRAO[IPL];
+          else radn[I - 1] = tran[I - 1] * RADO[IPL] + radn[I - 1];
           tran[I - 1] = tran[I - 1] * TRAO[IPL];
         }
       } else {
@@ -2256,12 +2278,27 @@ int oracle_merge_next(int npo, const double *v1po, const double *v2po, const int
         for (int I = NL; I <= NLIM; I += LL) {
           IPL = IPL + LLM1;
           if (IPL + 2 > lim_ok) g_merge_quirks++;
-          radn[I - 1] = tran[I - 1] * (a1 * RADO[IPL - 1] + a2 * RADO[IPL] + a3 * RADO[IPL + 1] + a4 * RADO[IPL + 2]) + radn[I - 1];
-          tran[I - 1] = tran[I - 1] * (a1 * TRAO[IPL - 1] + a2 * TRAO[IPL] + a3 * TRAO[IPL + 1] + a4 * TRAO[IPL + 2]);
+          if (ipathl == 3) {
+            double traoi = a1 * TRAO[IPL - 1] + a2 * TRAO[IPL] + a3 * TRAO[IPL + 1] + a4 * TRAO[IPL + 2];
+            radn[I - 1] = a1 * RADO[IPL - 1] + a2 * RADO[IPL] + a3 * RADO[IPL + 1] + a4 * RADO[IPL + 2] +
+                          radn[I - 1] * traoi;
+            tran[I - 1] = tran[I - 1] * traoi;
+          } else {
+            radn[I - 1] = tran[I - 1] * (a1 * RADO[IPL - 1] + a2 * RADO[IPL] + a3 * RADO[IPL + 1] + a4 * RADO[IPL + 2]) + radn[I - 1];
+            tran[I - 1] = tran[I - 1] * (a1 * TRAO[IPL - 1] + a2 * TRAO[IPL] + a3 * TRAO[IPL + 1] + a4 * TRAO[IPL + 2]);
+          }
         }
       }
     }
     npl = IPL;
+    if (ipathl == 3 && tbnd > 0.) { /* EMBND, constant emissivity */
+      double xktbnd = tbnd / RADCN2, vi = V1P;
+      for (int j = 0; j < NLIM; j++) {
+        double bb = oracle_planck(vi, xktbnd);
+        radn[j] = radn[j] + tran[j] * emis * bb;
+        vi = vi + dvk;
+      }
+    }
     for (int j = 0; j < NLIM; j++) { rad[ok + j] = radn[j]; tra[ok + j] = tran[j]; }
     ok += NLIM;
     int ip = -1;

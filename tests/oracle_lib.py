@@ -57,6 +57,10 @@ def lib():
         L.oracle_merge_next.argtypes = [C.c_int, dp, dp, ip, C.c_double, dp, dp, C.c_int, dp, dp,
                                         ip, C.c_double, dp, C.c_double, C.c_double, C.c_double,
                                         dp, dp]
+        L.oracle_merge_next_path.restype = C.c_int
+        L.oracle_merge_next_path.argtypes = [C.c_int, dp, dp, ip, C.c_double, dp, dp, C.c_int, dp, dp,
+                                             ip, C.c_double, dp, C.c_double, C.c_double, C.c_double,
+                                             C.c_int, C.c_double, C.c_double, dp, dp]
         L.oracle_cont_grid.restype = C.c_int
         L.oracle_cont_grid.argtypes = [C.POINTER(abi.Layer), dp, dp, dp, ip]
         L.oracle_contnm.restype = C.c_int
@@ -152,9 +156,10 @@ def merge_first(layer_od, pan, dv, tave, tzu, tbnd, emis):
     return Merged(pan, dv, rad, tra)
 
 
-def merge_next(old, layer_od, pan, dv, tave, tzu, v2run):
+def merge_next(old, layer_od, pan, dv, tave, tzu, v2run, ipathl=1, tbnd=0.0, emis=0.0):
     """Returns (Merged, quirks) -- quirks counts RADNN reads past the old
-    points read so far."""
+    points read so far.  ipathl = 3: downwelling (tzu carries TZL; tbnd > 0
+    adds EMBND)."""
     v1o, v2o, nlo = old.pan
     v1, v2, nl = pan
     n = int(nl.sum())
@@ -165,10 +170,11 @@ def merge_next(old, layer_od, pan, dv, tave, tzu, v2run):
     nlc = np.ascontiguousarray(nl, np.int32)
     ro = np.ascontiguousarray(old.rad)
     to = np.ascontiguousarray(old.tra)
-    rc = lib().oracle_merge_next(len(v1o), _d(v1o), _d(v2o), abi.ptr(nloc, C.c_int), old.dv,
-                                 abi.ptr(ro, C.c_double), abi.ptr(to, C.c_double), len(v1), _d(v1),
-                                 _d(v2), abi.ptr(nlc, C.c_int), dv, abi.ptr(od, C.c_double), tave, tzu,
-                                 v2run, abi.ptr(rad, C.c_double), abi.ptr(tra, C.c_double))
+    rc = lib().oracle_merge_next_path(len(v1o), _d(v1o), _d(v2o), abi.ptr(nloc, C.c_int), old.dv,
+                                      abi.ptr(ro, C.c_double), abi.ptr(to, C.c_double), len(v1), _d(v1),
+                                      _d(v2), abi.ptr(nlc, C.c_int), dv, abi.ptr(od, C.c_double), tave, tzu,
+                                      v2run, ipathl, tbnd, emis, abi.ptr(rad, C.c_double),
+                                      abi.ptr(tra, C.c_double))
     if rc:
         raise RuntimeError("oracle_merge_next rc=%d" % rc)
     return Merged(pan, dv, rad, tra), int(lib().oracle_merge_quirks())

@@ -175,3 +175,72 @@ def test_coarser_layer_is_refused():
     with pytest.raises(lib.LblxError) as e:
         lib.merge_size(run, lays[1], lays[0])
     assert e.value.code == abi.E_COARSER
+
+
+# Downwelling radiance at an observer looking up (IPATHL = 3): layers are
+# taken outward from the observer, R = L4(R_old) + E L4(T_old).  Closed
+# forms: an isothermal column at T radiates B(T)(1 - T_path) (B(T) with a
+# black boundary beyond it, EMBND); a transparent layer next to the
+# observer shows the layer behind it unchanged, an opaque one hides it
+# (the upwelling order R = T L4(R_old) + E would show the other layer);
+# RADNN's interpolation of the layers so far reproduces cubics.
+
+@pytest.mark.parametrize("dvs,rtol", [([0.006, 0.006, 0.006], 1e-12), ([0.009, 0.006, 0.0048], 2e-5)])
+def test_downwelling_isothermal_column(dvs, rtol):
+    """Across grids the carried radiance B (1 - T) and transmittance are
+    interpolated separately, and B varies by 1e-5 over the four taps."""
+    t = 255.0
+    rng = np.random.default_rng(9)
+    v1, n0 = 800.0, 3 * 2400 + 77
+    span = (n0 - 1) * dvs[0]
+    pans = [grid(v1, dv, int(round(span / dv)) + 1) for dv in dvs]
+    od = rng.exponential(0.5, int(pans[0][2].sum()))
+    m = O.merge_first(od, pans[0], dvs[0], t, t, 0.0, 0.0)
+    for k in (1, 2):
+        od = rng.exponential(0.5, int(pans[k][2].sum()))
+        last = k == 2
+        m, q = O.merge_next(m, od, pans[k], dvs[k], t, t, v1 + span, ipathl=3,
+                            tbnd=t if last else 0.0, emis=1.0 if last else 0.0)
+        assert q == 0
+    v = v1 + np.arange(len(m.rad)) * dvs[2]
+    # B(T)(1 - T_path) + T_path B(T): the black boundary closes the sum
+    np.testing.assert_allclose(m.rad[8:-8], planck(v, t)[8:-8], rtol=rtol)
+
+
+def test_downwelling_order_of_layers():
+    v1, dv, n = 1000.0, 0.004, 5000
+    pan = grid(v1, dv, n)
+    v = v1 + np.arange(n) * dv
+    tau = 800.0
+    ba, bl = planck(v, 230.0), planck(v, 240.0)
+    opaque = ba + (bl - ba) * (1.0 - 2.0 / tau)  # EMIN's source at TZL = 240
+    # transparent layer at the observer, opaque layer behind it
+    m = O.merge_first(np.zeros(n), pan, dv, 260.0, 250.0, 0.0, 0.0)
+    np.testing.assert_array_equal(m.rad, 0.0)
+    m2, _ = O.merge_next(m, np.full(n, tau), pan, dv, 230.0, 240.0, v[-1], ipathl=3)
+    np.testing.assert_allclose(m2.rad, opaque, rtol=1e-12)
+    assert np.all(m2.tra == 0.0)
+    # opaque layer at the observer hides everything behind it
+    m = O.merge_first(np.full(n, tau), pan, dv, 230.0, 240.0, 0.0, 0.0)
+    m3, _ = O.merge_next(m, np.full(n, 0.5), pan, dv, 300.0, 310.0, v[-1], ipathl=3, tbnd=320.0, emis=1.0)
+    np.testing.assert_allclose(m3.rad, opaque, rtol=1e-12)
+
+
+@pytest.mark.parametrize("itype", [1, 2, 4, 5])
+def test_downwelling_radnn_interpolates_cubics(itype):
+    LL = itype + 1
+    dvo = 0.012
+    dvk = dvo * itype / LL
+    v1 = 900.0
+    no = 2400 * 3 + 500
+    span = (no - 1) * dvo
+    nk = int(round(span / dvk)) + 1
+    po, pk = grid(v1, dvo, no), grid(v1, dvk, nk)
+    c = lambda j: 3.0 + 0.01 * j - 2e-6 * j ** 2 + 1e-10 * j ** 3
+    ct = lambda j: 0.5 + 1e-5 * j
+    old = O.Merged(po, dvo, c(np.arange(no, dtype=float)), ct(np.arange(no, dtype=float)))
+    m, q = O.merge_next(old, np.zeros(nk), pk, dvk, 250.0, 250.0, v1 + span, ipathl=3)
+    assert q == 0
+    i = np.arange(nk, dtype=float)
+    np.testing.assert_allclose(m.rad[LL:-3 * LL], c(i * itype / LL)[LL:-3 * LL], rtol=1e-12)
+    np.testing.assert_allclose(m.tra[LL:-3 * LL], ct(i * itype / LL)[LL:-3 * LL], rtol=1e-12)
