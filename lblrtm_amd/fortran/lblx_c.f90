@@ -115,6 +115,17 @@ module lblx_c
        real(c_double), value :: tzu, tbnd, emis
        type(c_ptr), value :: d_od, d_rold, d_told, d_rnew, d_tnew
      end function lblx_xmerge_device
+     integer(c_int) function lblx_xmerge_path_device(ctx, run, layer_old, layer, ipathl, tz, tbnd, emis, &
+          d_od, d_rold, d_told, d_rnew, d_tnew) bind(c, name="lblx_xmerge_path_device")
+       import :: c_int, c_ptr, c_double, lblx_run, lblx_layer
+       type(c_ptr), value :: ctx
+       type(lblx_run), intent(in) :: run
+       type(c_ptr), value :: layer_old
+       type(lblx_layer), intent(in) :: layer
+       integer(c_int), value :: ipathl
+       real(c_double), value :: tz, tbnd, emis
+       type(c_ptr), value :: d_od, d_rold, d_told, d_rnew, d_tnew
+     end function lblx_xmerge_path_device
      integer(c_int) function lblx_scan_size(scan, v1_in, dv_in, n_in, n_out, v1_out, dv_out) &
           bind(c, name="lblx_scan_size")
        import :: c_int, c_double, c_int64_t, lblx_scan

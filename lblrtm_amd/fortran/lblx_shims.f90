@@ -15,7 +15,7 @@
 ! latter on the device), so XMERGE does not read KFILE or LFILE back.
 !
 ! Supported: IHIRAC = 1 with no line-file reuse (ILNFLG = 0), IOD = 0,
-! no cross sections; XMERGE with IEMIT = 1, IPATHL = 1 (upwelling), no
+! no cross sections; XMERGE with IEMIT = 1, IPATHL = 1 or 3 (upwelling or downwelling), no
 ! aerosols, a constant surface emissivity and no surface-property file.
 ! Anything else stops with the reason, as the reference stops on errors.
 
@@ -209,7 +209,7 @@ SUBROUTINE XMERGE (NPTS,LFILE,MFILE,JPATHL)
   INTEGER NLIMK
 
   IF (JPATHL.GE.1) IPATHL = JPATHL
-  if (IEMIT /= 1 .or. IPATHL /= 1 .or. IMRG == 36 .or. IMRG == 46 .or. IMRG == 10 .or. &
+  if (IEMIT /= 1 .or. (IPATHL /= 1 .and. IPATHL /= 3) .or. IMRG == 36 .or. IMRG == 46 .or. IMRG == 10 .or. &
        (IAERSL >= 1 .and. IAERSL /= 5) .or. IBPROP == 1 .or. BNDEMI(2) /= 0. .or. BNDEMI(3) /= 0.) &
        stop 'XMERGE: option outside the liblblx path (IEMIT, IPATHL, IMRG, aerosols, surface)'
   if (.not. od_valid .or. od_layer%dv /= DV .or. od_layer%v1 /= V1 .or. od_layer%v2 /= V2 .or. &
@@ -219,10 +219,11 @@ SUBROUTINE XMERGE (NPTS,LFILE,MFILE,JPATHL)
   tbnd = 0.
   pold = c_null_ptr
   if (LAYER == LH1) then                     ! EMINIT: the layer at the surface
-     tbnd = TMPBND
+     if (IPATHL == 1) tbnd = TMPBND          ! (observer's layer for IPATHL = 3)
      old_valid = .false.
-  else                                       ! RADMRG onto the merged layers below
+  else                                       ! RADMRG onto the merged layers so far
      if (.not. old_valid) stop 'XMERGE: no merged result of the layers below'
+     if (IPATHL == 3 .and. LAYER == LH2) tbnd = TMPBND   ! EMBND (src/xmerge.f90:140)
      pold = c_loc(old_layer)
   end if
   rc = lblx_merge_size(od_run, pold, lay, n)
@@ -235,7 +236,12 @@ SUBROUTINE XMERGE (NPTS,LFILE,MFILE,JPATHL)
        stop 'XMERGE: device memory'
   rc = lblx_copy(ctx, d_od, c_loc(od), nb, 0_c_int)
   if (rc /= LBLX_OK) call lblx_stop('XMERGE', rc)
-  rc = lblx_xmerge_device(ctx, od_run, pold, lay, TZU, tbnd, BNDEMI(1), d_od, d_rold, d_told, d_rnew, d_tnew)
+  if (IPATHL == 3) then                      ! EMIN at the observer's side, TZL
+     rc = lblx_xmerge_path_device(ctx, od_run, pold, lay, 3_c_int, TZL, tbnd, BNDEMI(1), d_od, d_rold, &
+          d_told, d_rnew, d_tnew)
+  else
+     rc = lblx_xmerge_device(ctx, od_run, pold, lay, TZU, tbnd, BNDEMI(1), d_od, d_rold, d_told, d_rnew, d_tnew)
+  end if
   if (rc /= LBLX_OK) call lblx_stop('XMERGE', rc)
   allocate (rad(n), tra(n))
   rc = lblx_copy(ctx, c_loc(rad), d_rnew, 8_c_int64_t * n, 1_c_int)

@@ -6,8 +6,9 @@
 ! that holds the final merged result to final.txt.  TAPE3 is read from the
 ! working directory.
 !
-! layers.txt: nlay nmol icntnm / v1 v2 tbnd emis / then per layer
-! pave tave dv tzu wbroad wk(1:nmol)
+! layers.txt: nlay nmol icntnm ipathl / v1 v2 tbnd emis / then per layer
+! pave tave dv tz wbroad wk(1:nmol), tz = TZU (ipathl 1) or TZL (ipathl 3:
+! downwelling at an observer at layer 1 looking up)
 program drive_column
   IMPLICIT REAL*8 (V)
   COMMON /MANE/ P0,TEMP0,NLAYER,DVXM,H2OSLF,WTOT,ALBAR,ADBAR,AVBAR,    &
@@ -37,11 +38,11 @@ program drive_column
        &      (FSCDID(3),IXSCNT) , (FSCDID(4),IAERSL),                   &
        &      (FSCDID(5),IEMIT) , (FSCDID(8),IPATHL),                    &
        &      (FSCDID(9),JRAD) , (FSCDID(11),IMRG)
-  INTEGER NL, NM, ICN, L, M, LF, MF, MSTOR, K
-  REAL EM
+  INTEGER NL, NM, ICN, L, M, LF, MF, MSTOR, K, IPL
+  REAL EM, TZX
 
   open (20, file='layers.txt', status='old')
-  read (20, *) NL, NM, ICN
+  read (20, *) NL, NM, ICN, IPL
   read (20, *) V1, V2, TMPBND, EM
   ! /FILHDR/ in the double-precision build: 177 eight-byte words before
   ! LSTWDF; a panel header V1P, V2P, DVP, NLIM is 4 words
@@ -82,7 +83,7 @@ program drive_column
   IXSCNT = ICN
   IAERSL = 0
   IEMIT = 1
-  IPATHL = 1
+  IPATHL = IPL
   JRAD = 1
   IMRG = 0
   LH1 = 1
@@ -111,7 +112,12 @@ program drive_column
   LF = 11
   MF = 12
   do L = 1, NL
-     read (20, *) PAVE, TAVE, DV, TZU, WBROAD, (WK(M), M = 1, NM)
+     read (20, *) PAVE, TAVE, DV, TZX, WBROAD, (WK(M), M = 1, NM)
+     if (IPL == 3) then      ! the boundary on the observer's side
+        TZL = TZX
+     else
+        TZU = TZX
+     end if
      LAYER = L
      LAYRS = L
      rewind 10

@@ -28,7 +28,11 @@ def bt(rad, v):
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="flang drivers not built")
-def test_fortran_column_through_shims(gpu_ctx, tmp_path):
+@pytest.mark.parametrize("ipathl", [1, 3])
+def test_fortran_column_through_shims(gpu_ctx, tmp_path, ipathl):
+    """ipathl 1: upwelling at the top over a surface; 3: downwelling at the
+    surface looking up, with a 240 K black boundary beyond the top layer
+    (EMBND)."""
     tab = synth.synth_lines(20000, 575.0, 725.0, [1, 2, 3], 41)
     tape.write_tape3(tmp_path / "TAPE3", tab)
     atm = synth.atmosphere(nlev=11, ptop=100.0, seed=5)
@@ -36,10 +40,14 @@ def test_fortran_column_through_shims(gpu_ctx, tmp_path):
     run = abi.make_run(v1, v2, icntnm=1)
     dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
     nl, nm = len(dv), atm["wk"].shape[1]
-    tzu = np.asarray(atm["tlev"][1:], np.float64)
-    tbnd, emis = float(atm["tlev"][0]), 0.98
+    if ipathl == 1:
+        tzu = np.asarray(atm["tlev"][1:], np.float64)
+        tbnd, emis = float(atm["tlev"][0]), 0.98
+    else:
+        tzu = np.asarray(atm["tlev"][:-1], np.float64)  # TZL
+        tbnd, emis = 240.0, 1.0
     with open(tmp_path / "layers.txt", "w") as f:
-        f.write("%d %d %d\n" % (nl, nm, 1))
+        f.write("%d %d %d %d\n" % (nl, nm, 1, ipathl))
         f.write("%r %r %r %r\n" % (v1, v2, tbnd, emis))
         for l in range(nl):
             vals = [atm["pave"][l], atm["tave"][l], dv[l], tzu[l], atm["wbroad"][l]] + list(atm["wk"][l])
@@ -58,8 +66,14 @@ def test_fortran_column_through_shims(gpu_ctx, tmp_path):
         inp.jrad = jrad
         od_ref, _, _, jrad_next = O.od_layer(run, inp, tab)
         pan = O.panels()
-        m = (O.merge_first(od_ref, pan, L.dv, L.tave, tzu[l], tbnd, emis) if l == 0
-             else O.merge_next(m, od_ref, pan, L.dv, L.tave, tzu[l], v2)[0])
+        if ipathl == 1:
+            m = (O.merge_first(od_ref, pan, L.dv, L.tave, tzu[l], tbnd, emis) if l == 0
+                 else O.merge_next(m, od_ref, pan, L.dv, L.tave, tzu[l], v2)[0])
+        else:
+            last_l = l == nl - 1
+            m = (O.merge_first(od_ref, pan, L.dv, L.tave, tzu[l], 0.0, 0.0) if l == 0
+                 else O.merge_next(m, od_ref, pan, L.dv, L.tave, tzu[l], v2, ipathl=3,
+                                   tbnd=tbnd if last_l else 0.0, emis=emis if last_l else 0.0)[0])
         if l < nl - 1:
             jrad = jrad_next
     # TAPE10: the last layer's optical depth in PANEL's records
@@ -77,10 +91,12 @@ def test_fortran_column_through_shims(gpu_ctx, tmp_path):
     mf = tape.read_panels(tmp_path / ("TAPE%d" % final), word=8, nvals=2, filhdr_words=177)
     rad, tra = mf["values"]
     gpu_ctx.load_lines(tab)
-    rs, ts, v1p, dvt = gpu_ctx.spectrum(run, [abi.Layer.from_buffer_copy(x) for x in layers], tzu, tbnd, emis)
+    rs, ts, v1p, dvt = gpu_ctx.spectrum_path(run, [abi.Layer.from_buffer_copy(x) for x in layers], ipathl, tzu,
+                                             tbnd, emis)
     np.testing.assert_allclose(rad, rs, rtol=1e-10)
     np.testing.assert_allclose(tra, ts, rtol=1e-10, atol=1e-300)
     v = v1p + dvt * np.arange(len(rad))
     d = np.abs(bt(rad, v) - bt(m.rad, v))
-    print("fortran column: %d TOA points, max |dBT| vs oracle %.2e K" % (len(rad), d.max()), flush=True)
+    print("fortran column (IPATHL %d): %d points, max |dBT| vs oracle %.2e K" % (ipathl, len(rad), d.max()),
+          flush=True)
     assert d.max() <= 0.01
