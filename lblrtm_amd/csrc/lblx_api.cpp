@@ -1341,7 +1341,11 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   const int nwin = (int)win.size();
   c->last_line_stride = S.line;
   const bool voigt = run->voigt != 0;
+  // ILBLF4 = 0: no far wing; R4 stays zero, so LNCOR1 rejects against
+  // DPTMIN alone (oprop.f90:1156-1160) and XINT(R4) adds nothing
+  const bool nof4 = run->ilblf4 == 0;
   if (tan && voigt) return LBLX_E_UNSUPPORTED;  // the tangent follows the tabulated shapes
+  if (tan && nof4) return LBLX_E_UNSUPPORTED;   // the tangent differentiates LBLF4's decisions
   if (voigt) {
     int rv = ensure_voigt(c);
     if (rv) return rv;
@@ -1466,6 +1470,9 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   int th = c->tbegin(ST_SHRINK);
   const int lgrid = (int)((maxline + 1 + 255) / 256);
   const int bgrid = std::max(maxnb4, 1);
+  if (nof4) {  // no TAPE9: its line and block counts are zero
+    hipLaunchKernelGGL(k_fill_i32, dim3((int)((2 * NJ + 255) / 256)), dim3(256), 0, s, T9.n, 2 * NJ, 0);
+  } else {
   hipLaunchKernelGGL(k_shr_count, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
   hipLaunchKernelGGL(k_shr_scan, dim3(njobs), dim3(256), 0, s, dY, S, W);
   hipLaunchKernelGGL(k_shr_compact, dim3(bgrid, njobs), dim3(256), 0, s, L, dY, S, W);
@@ -1483,6 +1490,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   hipLaunchKernelGGL(k_shr_bwalk, dim3(bgrid, njobs), dim3(64), 0, s, S, W);
   hipLaunchKernelGGL(k_shr_emit, dim3(lgrid, njobs), dim3(256), 0, s, dY, S, W, T9);
   hipLaunchKernelGGL(k_t9_vhi, dim3((int)((S.t9blk + 255) / 256), njobs), dim3(256), 0, s, S, T9);
+  }
   c->tend(th);
   if (maxnn > 0) {
     // LNCOR1 reads only the line table and the layer/panel tables, so it runs
@@ -1510,7 +1518,12 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   {
     uint64_t *am = tan ? c->t_amask.as<uint64_t>() : nullptr;
     int32_t *ws = tan ? c->t_wspan.as<int32_t>() : nullptr;
-    if (split)
+    if (nof4 && nwin > 0)
+      hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)(((int64_t)nwin * R4ROW + 255) / 256)), dim3(256), 0, s,
+                         c->w_r4.as<double>(), (int64_t)nwin * R4ROW, 0.0);
+    else if (nof4)
+      ;
+    else if (split)
       hipLaunchKernelGGL(k_convf4<float>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
                          c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ, am, mwords,
                          ws);
@@ -1669,7 +1682,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       st[jb].nlins4 = t9n[jb];
       st[jb].npanel = (int64_t)plans[jb]->panels.size();
       st[jb].npts = plans[jb]->npts;
-      st[jb].nwin4 = (int64_t)plans[jb]->wins.size();
+      st[jb].nwin4 = nof4 ? 0 : (int64_t)plans[jb]->wins.size();
       st[jb].nfix = c->fix_passes;
       st[jb].fixseq = c->fix_seq;
     }

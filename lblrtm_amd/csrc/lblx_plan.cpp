@@ -355,7 +355,7 @@ int layer_jrad(const lblx_layer *Ly) {
 int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   if (!run || !Ly || !P) return LBLX_E_ARG;
   if (Ly->nmol < 1 || Ly->nmol > 7) return LBLX_E_UNSUPPORTED;
-  if (run->ilblf4 < 1) return LBLX_E_UNSUPPORTED;
+  if (run->ilblf4 < 0 || run->ilblf4 > 2) return LBLX_E_ARG;
   {  // reset, keeping the tables' storage for reuse
     std::vector<PanelPlan> pv = std::move(P->panels);
     std::vector<WinPlan> wv = std::move(P->wins);
@@ -395,7 +395,7 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   P->v2r4st = V2 + 2. * P->dvr4;
   P->v1l4 = P->v1r4st - P->bound4 - P->dvr4;
   P->v2l4 = P->v2r4st + P->bound4 + 2 * P->dvr4;
-  if (P->v1l4 <= 25.) return LBLX_E_UNSUPPORTED;  // no VVW mirror / RSYM on device
+  if (run->ilblf4 >= 1 && P->v1l4 <= 25.) return LBLX_E_UNSUPPORTED;  // no VVW mirror / RSYM on device
   // HIRAC1 (oprop.f90:223-315)
   P->dptmn_hirac = run->dptmin;
   if (P->jrad != 1) P->dptmn_hirac = run->dptmin / radfn(V2, Ly->tave / RADCN2);

@@ -295,3 +295,35 @@ def test_tape3_broadener_words(gpu_ctx, tmp_path):
     od1, _, _ = gpu_ctx.opdpth(abi.make_run(705.0, 755.0, ibrd=1), abi.Layer.from_buffer_copy(lay))
     od0, _, _ = gpu_ctx.opdpth(run, abi.Layer.from_buffer_copy(lay))
     np.testing.assert_allclose(od1, od0, rtol=1e-12, atol=0)  # LDS accumulation order only
+
+
+@pytest.mark.parametrize("icntnm", [0, 1])
+def test_no_far_wing(gpu_ctx, icntnm):
+    """ILBLF4 = 0: no LBLF4 far wing (no TAPE9, R4 zero), LNCOR1 rejects
+    against DPTMIN alone (oprop.f90:1156-1160); the single-layer workload
+    and three layers of the column against the oracle, point by point."""
+    tab = synth.config0_lines()
+    gpu_ctx.load_lines(tab)
+    p, t, wk, wb = _layer0()
+    run = abi.make_run(600.0, 1100.0, ilblf4=0, icntnm=icntnm)
+    dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+    lay = abi.make_layer(p, t, wk, wb, dv[0], 600.0, 1100.0)
+    od, ref, st, st_r = _run_both(gpu_ctx, run, lay, tab)
+    print("no far wing (icntnm %d): npts %d, lincnt %d, k3 %d" % (icntnm, len(od), st["lincnt"],
+                                                                  st["k3_updates"]), flush=True)
+    assert_od_close(od, ref, "ILBLF4 = 0")
+    for k in ("npanel", "npts", "k1_updates", "k3_updates", "nlins4", "lincnt", "nwin4"):
+        assert st[k] == st_r[k], k
+    assert st["nlins4"] == 0 and st["k3_updates"] == 0
+    run1 = abi.make_run(600.0, 1100.0, ilblf4=1, icntnm=icntnm)
+    od1, _, _ = gpu_ctx.opdpth(run1, abi.Layer.from_buffer_copy(lay))
+    assert np.max(od1 - od) > 0.0  # the far wing's share is gone
+    atm = synth.atmosphere()
+    runc = abi.make_run(600.0, 1100.0, ilblf4=0, icntnm=icntnm)
+    dvc, _ = lib.plan_dv(runc, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    for l in (0, 20, 45):
+        L = abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l], dvc[l],
+                           600.0, 1100.0)
+        od, ref, st, st_r = _run_both(gpu_ctx, runc, L, tab)
+        assert_od_close(od, ref, "ILBLF4 = 0 layer %d" % l)
+        assert st["lincnt"] == st_r["lincnt"]

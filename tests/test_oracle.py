@@ -189,3 +189,53 @@ def test_rejection_only_removes_weak_lines():
     assert sa["lincnt"] < sb["lincnt"]
     assert np.all(a <= b * (1 + 1e-12))
     assert np.median((b - a) / b) < 1e-2
+
+
+def test_far_wing_off_and_on():
+    """ILBLF4 = 0 drops LBLF4's far wing: one line at P0, T0 (corrections
+    the identity) has no optical depth beyond the near functions' 64
+    halfwidths, and with ILBLF4 = 1 the optical depth there is the far-wing
+    Lorentz line less its value at BOUND4 = 25 cm-1 (oprop.f90:3989-4006),
+    times RADFN, interpolated from the coarse grid.  Without lines in
+    reach the two settings give the same continuum-only OD."""
+    p, t = 1013.25, 296.0
+    w, wb = 1e20, 2.5e25
+    nu0 = 850.0
+    tab = _single_line(nu0)
+    ods = {}
+    for f4 in (0, 1):
+        run = abi.make_run(800.0, 900.0, dptmin=0.0, dptfac=0.0, ilblf4=f4)
+        dv, _ = O.plan_dv(run, [p], [t], [[w]], [wb])
+        lay = abi.make_layer(p, t, [w], wb, dv[0], 800.0, 900.0)
+        od, v1p, st, _ = O.od_layer(run, lay, tab)
+        ods[f4] = od
+        if f4 == 0:
+            assert st["nlins4"] == 0 and st["k3_updates"] == 0
+    nu = v1p + dv[0] * np.arange(len(od))
+    rho_s = w / (w + wb)
+    al = 0.07 * (1 - rho_s) + 0.2 * rho_s
+    x = np.abs(nu - nu0)
+    # beyond the near functions' 64 halfwidths and the PANEL interpolation
+    # of the coarsest grid (two points of 16 DV either side)
+    far = (x > 64 * al * 1.05 + 2 * 16 * dv[0] * 2) & (x < 20.0)
+    assert far.sum() > 1000
+    assert np.all(ods[0][far] == 0.0)
+    xkt = t / 1.4387752
+    rad = nu * np.tanh(nu / (2 * xkt))
+    sil = 1e-22 * w * (1 + math.exp(-nu0 / xkt)) * al / math.pi
+    want = rad * sil * (1.0 / (al * al + x * x) - 1.0 / (al * al + 25.0 ** 2))
+    # XINT's four-point interpolation from the 64 DV grid leaves about
+    # 0.8 (64 DV)^4 / x^4 of a 1/x^2 wing: 1e-3 at 5 cm-1, 2e-4 beyond 8
+    mid = far & (x > 8.0)
+    np.testing.assert_allclose(ods[1][mid], want[mid], rtol=3e-4)
+    np.testing.assert_allclose(ods[1][far], want[far], rtol=2e-3)
+    # no line within reach: continuum only, equal bit for bit
+    p2, t2, wk2, wb2 = synth.config0_layer()
+    outs = []
+    for f4 in (0, 1):
+        run = abi.make_run(1000.0, 1040.0, icntnm=1, ilblf4=f4)
+        dv2, _ = O.plan_dv(run, [p2], [t2], [wk2], [wb2])
+        lay = abi.make_layer(p2, t2, wk2, wb2, dv2[0], 1000.0, 1040.0)
+        outs.append(O.od_layer(run, lay, tab)[0])
+    assert np.max(outs[0]) > 0.0
+    np.testing.assert_array_equal(outs[0], outs[1])
