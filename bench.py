@@ -271,7 +271,7 @@ def batch_block(ctx, lib, abi, synth, nprof, steps, nadj):
     return {"value": nprof * 1000.0 / ms, "unit": "spectra/s", "ms_per_step": ms, "steps": steps,
             "forward_and_adjoint": adj, "backward": bwd,
             "profiles": nprof, "layers": int(sum(len(i[0]) for i in items)), "points": int(npts),
-            "tiles": list(tiles), "autotune_s": t_tune,
+            "tiles": list(tiles), "line_batch": list(ctx.tuning()[1]), "autotune_s": t_tune,
             "workload": "ragged profiles: 20-80 layers, psfc 600-1050 hPa, 200-800 cm-1 windows "
                         "in 500-2500 cm-1, FP32 split, continuum on, upwelling TOA radiance"}
 

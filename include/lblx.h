@@ -388,6 +388,26 @@ int lblx_jacobians_batch_device(lblx_ctx *ctx, const lblx_run *run, int nprof, c
 int lblx_autotune(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers, int nlay, int *tiles_out);
 int lblx_set_tiles(lblx_ctx *ctx, const int *tiles);
 
+/* The tuning of the near-wing scatter: tile widths (as lblx_set_tiles) and
+ * each grid's line batch, the lines a wave takes per chunk (1..64, 0: the
+ * built-in 16 for the FP64 R1 tile, 32 otherwise; a 1024-thread tile
+ * workgroup takes 16 chunks per step, 128..1024 lines for 8..64).
+ * lblx_autotune times both on the given layers (widths first, then 8, 16,
+ * 32, 64 lines at the chosen width).  lblx_tuning_save writes the current
+ * tuning with the device's architecture, name and compute-unit count to a
+ * text file; lblx_tuning_load sets it from such a file, refusing one made
+ * on another device (LBLX_E_UNSUPPORTED) or malformed (LBLX_E_ARG).  No
+ * counterpart in the reference; none of this changes results beyond the
+ * order of floating-point additions in the LDS accumulators.              */
+typedef struct lblx_tuning {
+  int32_t tiles[3];         /* R1, R2, R3 tile widths                       */
+  int32_t line_batch[3];    /* lines per wave chunk, R1, R2, R3             */
+} lblx_tuning;
+int lblx_get_tuning(lblx_ctx *ctx, lblx_tuning *t);
+int lblx_set_tuning(lblx_ctx *ctx, const lblx_tuning *t);
+int lblx_tuning_save(lblx_ctx *ctx, const char *path);
+int lblx_tuning_load(lblx_ctx *ctx, const char *path);
+
 /* OPDPTH with DVOUT (IOD = 1 or 4, src/lblrtm.f90:1062-1076): the layer's
  * optical depth regridded by PNLINT (src/oprop.f90:1564-1806) onto a DVOUT
  * grid starting at the layer's first point, in output panels of 2400

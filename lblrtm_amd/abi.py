@@ -118,6 +118,10 @@ class Scan(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("tiles", C.c_int32 * 3), ("line_batch", C.c_int32 * 3)]
+
+
 class ScanFn(C.Structure):
     _fields_ = [
         ("hwhm", C.c_double),

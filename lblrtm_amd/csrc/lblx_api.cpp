@@ -302,6 +302,7 @@ struct lblx_ctx {
       t_odcoff, t_tmap[3], t_tabs, t_mem, t_memq, t_f4c, t_f4d, t_parts, t_r4p;
   int64_t s_twN = 0;
   int tiles[3] = {8192, 4096, 2048};      // R1, R2, R3 tile widths (autotuned)
+  int lbatch[3] = {0, 0, 0};              // lines per wave chunk in the tiles (0: 16 FP64 R1, else 32)
   DevBuf b_jobs, b_rt, b_rl;              // batched merges
   DevBuf w_dvo;                           // PNLINT runs
   struct GraphEntry {
@@ -1608,6 +1609,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   const double *d_abs = abs_stride > 0 ? c->w_absrb.as<double>() : nullptr;
   for (int G = 3; G >= 1; G--) {
     const int T = tiles[G - 1];
+    const int CH = c->lbatch[G - 1] > 0 ? c->lbatch[G - 1] : ((G == 1 && !split) ? 16 : 32);
     const int nt = (int)tjob[G - 1].size();
     int tht = c->tbegin(G == 3 ? ST_TILE3 : (G == 2 ? ST_TILE2 : ST_TILE1));
     hipLaunchKernelGGL(k_tile_ranges, dim3((nt + 255) / 256), dim3(256), 0, s, c->d_vnu.as<double>(), dY,
@@ -1627,7 +1629,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT, VV, LACCT>,                    \
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));             \
     hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT, VV, LACCT>), dim3(nt), dim3(TILE_T), shm, s, rec,   \
-                       tm[GG - 1], T, tabs, (const ACCT *)(VTAB), dY, S, dp, dw, r4,               \
+                       tm[GG - 1], T, CH, tabs, (const ACCT *)(VTAB), dY, S, dp, dw, r4,           \
                        (const double *)(ABS), dg, (const double *)(RIN), (double *)(ROUT),         \
                        (OUTT *)(ODP));                                                            \
   } while (0)
@@ -3446,6 +3448,19 @@ int lblx_autotune(lblx_ctx *c, const lblx_run *run, lblx_layer *layers, int nlay
       }
     }
     c->tiles[G] = bestw;
+    // then the line batch of the tile kernel at that width
+    const int chs[4] = {8, 16, 32, 64};
+    int bestc = c->lbatch[G];
+    double bc = 1e30;
+    for (int ch : chs) {
+      c->lbatch[G] = ch;
+      double t = trial(stage[G]);
+      if (t < bc) {
+        bc = t;
+        bestc = ch;
+      }
+    }
+    c->lbatch[G] = bestc;
   }
   c->timing = was;
   for (int i = 0; i < ST_N; i++) {
@@ -3464,6 +3479,71 @@ int lblx_voigt_tables(double *f, double *coef) {
   std::memcpy(f, t.data(), (size_t)3 * 101 * 2601 * 8);
   std::memcpy(coef, cf->data(), 6 * 101 * 8);
   return LBLX_OK;
+}
+
+static bool tuning_ok(const lblx_tuning *t) {
+  const int lo[3] = {256, 128, 64};
+  for (int G = 0; G < 3; G++) {
+    if (t->tiles[G] < lo[G] || t->tiles[G] > 8192 || (t->tiles[G] & 63)) return false;
+    if (t->line_batch[G] < 0 || t->line_batch[G] > 64) return false;
+  }
+  return true;
+}
+
+int lblx_get_tuning(lblx_ctx *c, lblx_tuning *t) {
+  if (!c || !t) return LBLX_E_ARG;
+  for (int G = 0; G < 3; G++) {
+    t->tiles[G] = c->tiles[G];
+    t->line_batch[G] = c->lbatch[G];
+  }
+  return LBLX_OK;
+}
+
+int lblx_set_tuning(lblx_ctx *c, const lblx_tuning *t) {
+  if (!c || !t || !tuning_ok(t)) return LBLX_E_ARG;
+  for (int G = 0; G < 3; G++) {
+    c->tiles[G] = t->tiles[G];
+    c->lbatch[G] = t->line_batch[G];
+  }
+  return LBLX_OK;
+}
+
+// The device a tuning belongs to: its architecture, name and compute units.
+static std::string tuning_device_key(lblx_ctx *c) {
+  hipDeviceProp_t pr{};
+  if (hipGetDeviceProperties(&pr, c->device) != hipSuccess) return "unknown";
+  std::string name = pr.gcnArchName;
+  name += "|";
+  name += pr.name;
+  name += "|" + std::to_string(pr.multiProcessorCount);
+  for (char &ch : name)
+    if (ch == ' ' || ch == '\n') ch = '_';
+  return name;
+}
+
+int lblx_tuning_save(lblx_ctx *c, const char *path) {
+  if (!c || !path) return LBLX_E_ARG;
+  FILE *f = std::fopen(path, "w");
+  if (!f) return LBLX_E_ARG;
+  std::fprintf(f, "lblx-tuning 1 %s\n", tuning_device_key(c).c_str());
+  std::fprintf(f, "tiles %d %d %d\nline_batch %d %d %d\n", c->tiles[0], c->tiles[1], c->tiles[2], c->lbatch[0],
+               c->lbatch[1], c->lbatch[2]);
+  return std::fclose(f) == 0 ? LBLX_OK : LBLX_E_ARG;
+}
+
+int lblx_tuning_load(lblx_ctx *c, const char *path) {
+  if (!c || !path) return LBLX_E_ARG;
+  FILE *f = std::fopen(path, "r");
+  if (!f) return LBLX_E_ARG;
+  char key[512] = {0};
+  int ver = 0;
+  lblx_tuning t{};
+  const int ok = std::fscanf(f, "lblx-tuning %d %511s tiles %d %d %d line_batch %d %d %d", &ver, key, &t.tiles[0],
+                             &t.tiles[1], &t.tiles[2], &t.line_batch[0], &t.line_batch[1], &t.line_batch[2]);
+  std::fclose(f);
+  if (ok != 8 || ver != 1) return LBLX_E_ARG;
+  if (tuning_device_key(c) != key) return LBLX_E_UNSUPPORTED;  // tuned on another device
+  return lblx_set_tuning(c, &t);
 }
 
 int lblx_set_tiles(lblx_ctx *c, const int *tiles) {

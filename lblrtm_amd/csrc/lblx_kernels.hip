@@ -1636,7 +1636,7 @@ constexpr int TILE_T = LBLX_TILE_T;
 //   LACC: type of the LDS accumulator (ACC, or double under FP32 shapes
 //   where LDS float atomics stall).
 template <int G, typename OUT, typename ACC, bool V, typename LACC>
-__global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
+__global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile, int ch,
                                               const double *tabs, const ACC *vtab, const DevLayer *Ys,
                                               Strides S, const DevPanel *pans, const DevWin *wins,
                                               const double *r4, const double *absrb,
@@ -1672,9 +1672,11 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
   // record of line c0 + i with vector loads, then the wave walks the chunk
   // line by line, broadcasting each line's fields with readlane.  The next
   // chunk's records are loaded before the current chunk is scattered.
-  // Chunks of 16-32 spread a tile's lines over its waves more evenly (16
-  // measured best for the FP64 R1 tile, 32 for the others).
-  constexpr int CH = (G == 1 && sizeof(ACC) == 8) ? 16 : 32, CSTEP = TILE_T / 64 * CH;
+  // The chunk length ch (1..64 lines; a workgroup step takes TILE_T / 64
+  // chunks) is a tuning parameter: chunks of 16-32 spread a tile's lines
+  // over its waves more evenly (16 measured best for the FP64 R1 tile, 32
+  // for the others).
+  const int CH = ch, CSTEP = TILE_T / 64 * CH;
   TileLine<G, V> cur, nxt;
   int c0 = lo + wave * CH;
   if (c0 < hi) cur.load(recs, rbase, c0 + lane, lane < CH ? hi : c0);

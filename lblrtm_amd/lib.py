@@ -96,6 +96,10 @@ def load():
                                                   C.c_void_p, P(C.c_int64), P(C.c_double)]),
         "lblx_autotune": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_int)]),
         "lblx_set_tiles": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lblx_get_tuning": (C.c_int, [C.c_void_p, P(abi.Tuning)]),
+        "lblx_set_tuning": (C.c_int, [C.c_void_p, P(abi.Tuning)]),
+        "lblx_tuning_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "lblx_tuning_load": (C.c_int, [C.c_void_p, C.c_char_p]),
         "lblx_voigt_tables": (C.c_int, [P(C.c_double), P(C.c_double)]),
         "lblx_jacobian_aj_device": (C.c_int, [C.c_void_p, P(abi.Run), P(abi.Layer), C.c_int, P(C.c_double),
                                               C.c_double, C.c_double, P(C.c_int), C.c_int, C.c_void_p,
@@ -615,6 +619,24 @@ class Context:
     def set_tiles(self, tiles):
         t = (C.c_int * 3)(*tiles)
         _chk(self._lib.lblx_set_tiles(self._h, t), "lblx_set_tiles")
+
+    def tuning(self):
+        """(tiles, line_batch) of the near-wing scatter."""
+        t = abi.Tuning()
+        _chk(self._lib.lblx_get_tuning(self._h, C.byref(t)), "lblx_get_tuning")
+        return tuple(t.tiles), tuple(t.line_batch)
+
+    def set_tuning(self, tiles, line_batch):
+        t = abi.Tuning()
+        t.tiles[:] = list(tiles)
+        t.line_batch[:] = list(line_batch)
+        _chk(self._lib.lblx_set_tuning(self._h, C.byref(t)), "lblx_set_tuning")
+
+    def tuning_save(self, path):
+        _chk(self._lib.lblx_tuning_save(self._h, str(path).encode()), "lblx_tuning_save")
+
+    def tuning_load(self, path):
+        _chk(self._lib.lblx_tuning_load(self._h, str(path).encode()), "lblx_tuning_load")
 
     def fftscn_device(self, scan, d_in, v1_in, dv_in, n_in, d_out, cap):
         """FFTSCN on device pointers; returns (points, v1, dv) of the output."""

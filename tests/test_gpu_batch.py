@@ -70,6 +70,49 @@ def test_autotune_keeps_results(gpu_ctx):
         gpu_ctx.set_tiles((100, 1024, 512))
 
 
+def test_tuning_line_batch_and_persistence(gpu_ctx, tmp_path):
+    """The tiles' line batch (lines per wave chunk, 1..64) changes only the
+    order of the LDS additions; the autotuner picks widths and batches, the
+    tuning file round-trips into a new context of the same device, and a
+    file from another device or a bad value is refused."""
+    tab = synth.config1_lines()
+    gpu_ctx.load_lines(tab)
+    atm = synth.atmosphere()
+    run = abi.make_run(900.0, 1000.0, icntnm=1)
+    dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    layers = [abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l], dv[l],
+                             900.0, 1000.0) for l in range(len(dv))]
+    tz = atm["tlev"][1:]
+    base = gpu_ctx.tuning()
+    r0 = gpu_ctx.spectrum(run, [abi.Layer.from_buffer_copy(x) for x in layers], tz, 295.0, 0.98)[0]
+    for lb in ((8, 8, 8), (64, 64, 64), (1, 13, 47)):
+        gpu_ctx.set_tuning(base[0], lb)
+        r = gpu_ctx.spectrum(run, [abi.Layer.from_buffer_copy(x) for x in layers], tz, 295.0, 0.98)[0]
+        np.testing.assert_allclose(r, r0, rtol=1e-12)
+    gpu_ctx.set_tuning(*base)
+    gpu_ctx.autotune(run, layers[::5])
+    tuned = gpu_ctx.tuning()
+    print("autotuned tiles %s, line batch %s" % tuned, flush=True)
+    assert all(b in (8, 16, 32, 64) for b in tuned[1])
+    path = tmp_path / "tuning.txt"
+    gpu_ctx.tuning_save(path)
+    other = lib.Context(0)
+    try:
+        other.tuning_load(path)
+        assert other.tuning() == tuned
+        txt = open(path).read().split("\n")
+        bad = tmp_path / "other.txt"
+        open(bad, "w").write("\n".join(["lblx-tuning 1 gfx000|elsewhere|1"] + txt[1:]))
+        with pytest.raises(lib.LblxError) as e:
+            other.tuning_load(bad)
+        assert e.value.code == abi.E_UNSUPPORTED
+        with pytest.raises(lib.LblxError):
+            other.set_tuning((8192, 4096, 2048), (16, 65, 32))
+    finally:
+        other.close()
+    gpu_ctx.set_tuning(*base)
+
+
 @pytest.mark.parametrize("aj", [False, True])
 def test_jacobians_batch_equal_single(gpu_ctx, aj):
     """Forward and adjoint of a ragged batch (lblx_jacobians_batch_device):
