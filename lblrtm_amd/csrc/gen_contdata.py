@@ -12,7 +12,10 @@ Tables (physical data, not algorithm):
       (src/contnm.f90:1298-1305);
   O2 fundamental 1340-1850 cm-1 (BLOCK DATA bo2f, src/contnm.f90:7320-7383);
   N2 fundamental 1997.8-2901.6 cm-1 (BLOCK DATA bn2f,
-      src/contnm.f90:2756-2912).
+      src/contnm.f90:2756-2912);
+  N2 rotational (collision-induced) -10-350 cm-1 at 296 K and 220 K with
+      the O2 scale factors (BLOCK DATA BN2T296, BN2T220,
+      src/contnm.f90:2569-2666).
 
 Usage:  python gen_contdata.py /path/to/lblrtm > lblx_contdata.h
 The generated header is committed; nothing at run time reads the sources.
@@ -84,6 +87,10 @@ def extract(root):
     co2_head, co2 = block_data(text, "BFCO2")
     o2_head, o2 = block_data(text, "bo2f")
     n2_head, n2 = block_data(text, "bn2f")
+    n2r296_head, n2r296 = block_data(text, "BN2T296")
+    n2r220_head, n2r220 = block_data(text, "BN2T220")
+    assert n2r296_head == n2r220_head and int(n2r296_head[3]) == 73
+    assert len(n2r296) == len(n2r220) == 146
     assert int(co2_head[3]) == len(co2) == 5003
     assert int(o2_head[3]) == 103 and len(o2) == 206
     assert int(n2_head[3]) == 228 and len(n2) == 684
@@ -95,7 +102,7 @@ def extract(root):
     nc["ref_press"] = float(f.variables["ref_press"].data)
     f.close()
     return dict(xfac=xfac, tdep=tdep, co2_head=co2_head, co2=co2, o2_head=o2_head, o2=o2,
-                n2_head=n2_head, n2=n2, nc=nc)
+                n2_head=n2_head, n2=n2, n2r_head=n2r296_head, n2r296=n2r296, n2r220=n2r220, nc=nc)
 
 
 def _arr(w, name, v, per=6):
@@ -108,7 +115,8 @@ def _arr(w, name, v, per=6):
 def emit(d, out):
     w = out.write
     w("/* Generated by gen_contdata.py from LBLRTM src/contnm.f90 DATA statements\n"
-      "   (BFCO2, XFACCO2, FRNCO2 band-head exponents, bo2f, bn2f) and\n"
+      "   (BFCO2, XFACCO2, FRNCO2 band-head exponents, bo2f, bn2f, BN2T296,\n"
+      "   BN2T220) and\n"
       "   data/absco-ref_wv-mt-ckd.nc (MT_CKD H2O).  Do not edit. */\n")
     w("#ifndef LBLX_CONTDATA_H\n#define LBLX_CONTDATA_H\n\n")
     nc = d["nc"]
@@ -135,6 +143,12 @@ def emit(d, out):
     _arr(w, "LBLX_N2F_272", d["n2"][:228])
     _arr(w, "LBLX_N2F_228", d["n2"][228:456])
     _arr(w, "LBLX_N2F_AH2O", d["n2"][456:])
+    h = d["n2r_head"]
+    w("static const double LBLX_N2R_HDR[4] = {%r, %r, %r, %r};\n" % tuple(h))
+    _arr(w, "LBLX_N2R_C296", d["n2r296"][:73])
+    _arr(w, "LBLX_N2R_SF296", d["n2r296"][73:])
+    _arr(w, "LBLX_N2R_C220", d["n2r220"][:73])
+    _arr(w, "LBLX_N2R_SF220", d["n2r220"][73:])
     w("\n#endif\n")
 
 

@@ -414,7 +414,8 @@ int lblx_create(lblx_ctx **out, int device) {
         {LBLX_MTCKD_FOR, LBLX_MTCKD_N}, {LBLX_MTCKD_FORCL, LBLX_MTCKD_N},
         {LBLX_MTCKD_TEXP, LBLX_MTCKD_N}, {LBLX_FCO2, 5003}, {LBLX_XFACCO2, 500},
         {LBLX_CO2_TDEP, 25}, {LBLX_O2F_XO2, 103}, {LBLX_O2F_XO2T, 103},
-        {LBLX_N2F_272, 228}, {LBLX_N2F_228, 228}, {LBLX_N2F_AH2O, 228}};
+        {LBLX_N2F_272, 228}, {LBLX_N2F_228, 228}, {LBLX_N2F_AH2O, 228},
+        {LBLX_N2R_C296, 73}, {LBLX_N2R_SF296, 73}, {LBLX_N2R_C220, 73}, {LBLX_N2R_SF220, 73}};
     std::vector<double> all;
     std::vector<size_t> off;
     for (auto &q : parts) {
@@ -431,7 +432,8 @@ int lblx_create(lblx_ctx **out, int device) {
                             &c->cont_tabs.forcl, &c->cont_tabs.texp, &c->cont_tabs.fco2,
                             &c->cont_tabs.xfac, &c->cont_tabs.tdep, &c->cont_tabs.o2x,
                             &c->cont_tabs.o2t, &c->cont_tabs.n2a, &c->cont_tabs.n2b,
-                            &c->cont_tabs.n2h};
+                            &c->cont_tabs.n2h, &c->cont_tabs.n2r296, &c->cont_tabs.n2rsf296,
+                            &c->cont_tabs.n2r220, &c->cont_tabs.n2rsf220};
     for (size_t k = 0; k < off.size(); k++) *dst[k] = b + off[k];
   }
   *out = c;

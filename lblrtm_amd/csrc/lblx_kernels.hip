@@ -1794,6 +1794,7 @@ struct ContTabs {
   const double *fco2, *xfac, *tdep;                // 5003, 500, 25
   const double *o2x, *o2t;                         // 103
   const double *n2a, *n2b, *n2h;                   // 228
+  const double *n2r296, *n2rsf296, *n2r220, *n2rsf220;  // 73
 };
 
 __device__ __forceinline__ double d_mt_radfn(double vi, double xkt) {  // myradfn
@@ -1904,6 +1905,25 @@ __global__ __launch_bounds__(256) void k_contnm(const ContPlan *cps, ContTabs TB
       return c;
     };
     ab = ab + xint_fn(K.v1c, K.v2c, K.dvc, o2, C.v1abs, C.dvabs, i, K.ist, K.last) * 1.0;
+  }
+  if (C.n2r.on) {  // N2 rotational (xn2_r): 296 K / 220 K tables, geometric in T
+    const ContTerm &K = C.n2r;
+    auto n2r = [&](int J) -> double {
+      if (J < 1 || J > K.nptc) return 0.;
+      int I = K.i1 + (J - 1);
+      double VJ = K.v1c + K.dvc * (double)(J - 1);
+      double c0 = 0., c1 = 0.;
+      if (I >= 1 && I <= 73) {
+        const double a = TB.n2r296[I - 1], sa = TB.n2rsf296[I - 1];
+        c0 = a * pow(TB.n2r220[I - 1] / a, K.f1);
+        const double sf_T = sa * pow(TB.n2rsf220[I - 1] / sa, K.f1);
+        c1 = (sf_T - 1.) * (0.79) / (0.21);
+      }
+      double c = K.f0 * c0 * (K.f4 + c1 * K.f5 + 1. * K.f6);
+      if (jrad == 1) c = c * d_radfn(VJ, xkt);
+      return c;
+    };
+    ab = ab + xint_fn(K.v1c, K.v2c, K.dvc, n2r, C.v1abs, C.dvabs, i, K.ist, K.last) * 1.0;
   }
   if (C.n2.on) {
     const ContTerm &K = C.n2;

@@ -252,9 +252,7 @@ static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPl
   const double xself = X[0], xfrgn = X[1], xco2c = X[2], xo3cn = X[3], xo2cn = X[4],
                xn2cn = X[5], xrayl = X[6];
   const double V1 = Ly->v1, V2 = Ly->v2, PAVE = Ly->pave, TAVE = Ly->tave;
-  if ((V2 > -10.0 && V1 < 350. && xn2cn > 0.) ||
-      (V2 > 4340.0 && (xn2cn > 0. || xo2cn > 0. || xo3cn > 0.)))
-    return LBLX_E_UNSUPPORTED;
+  if (V2 > 4340.0 && (xn2cn > 0. || xo2cn > 0. || xo3cn > 0.)) return LBLX_E_UNSUPPORTED;
   C->on = 1;
   // OPDPTH ABSRB grid (lblrtm.f90:6216-6221)
   C->dvabs = 1.;
@@ -318,6 +316,15 @@ static int plan_cont(const lblx_run *run, const lblx_layer *Ly, int jrad, ContPl
     C->o2.f0 = xo2cn * wk7 * 1.e-20 * amagat;    // tau_fac
     C->o2.f1 = (1. / 296.) - (1. / TAVE);        // xktfac
     C->o2.f2 = (1.e+20 / XLOSMT);                // factor
+  }
+  // N2 rotational (contnm.f90:754-790, xn2_r :2497-2567)
+  if (V2 > -10.0 && V1 < 350. && xn2cn > 0.) {
+    cont_walk(*C, LBLX_N2R_HDR, &C->n2r);
+    C->n2r.f0 = xn2cn * (wn2 / XLOSMT) * amagat;   // tau_fac
+    C->n2r.f1 = (TAVE - 296.) / (220. - 296.);     // tfac
+    C->n2r.f4 = x_vmr_n2;
+    C->n2r.f5 = x_vmr_o2;
+    C->n2r.f6 = x_vmr_h2o;
   }
   // N2 fundamental (contnm.f90:734-753)
   if (V2 > 2001.77 && V1 < 2897.59 && xn2cn > 0.) {

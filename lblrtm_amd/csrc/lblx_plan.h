@@ -62,7 +62,7 @@ struct ContPlan {
   double v1abs, v2abs, dvabs;
   int32_t nptabs, jrad;
   double xkt, tave;
-  ContTerm h2o, co2, o2, n2;
+  ContTerm h2o, co2, o2, n2, n2r;
   int32_t ray_on;
   double ray_f;      // WTOT * conv_cm2mol
   double xself, xfrgn, wk1;

@@ -1203,7 +1203,7 @@ int oracle_contnm(const lblx_run *run, const lblx_layer *Ly, int jrad, double *a
   const double xself = X[0], xfrgn = X[1], xco2c = X[2], xo3cn = X[3], xo2cn = X[4], xn2cn = X[5],
                xrayl = X[6];
   const double V1 = Ly->v1, V2 = Ly->v2, PAVE = Ly->pave, TAVE = Ly->tave;
-  if ((V2 > -10.0 && V1 < 350. && xn2cn > 0.) || (V2 > 4340.0 && (xn2cn > 0. || xo2cn > 0. || xo3cn > 0.)))
+  if (V2 > 4340.0 && (xn2cn > 0. || xo2cn > 0. || xo3cn > 0.))
     return LBLX_E_UNSUPPORTED;
   double *AB = calloc(N_ABSRB + 8, sizeof(double)); /* 1-based ABSRB */
   double *C = calloc(N_ABSRB + 8, sizeof(double));  /* 1-based coarse array */
@@ -1317,6 +1317,34 @@ int oracle_contnm(const lblx_run *run, const lblx_layer *Ly, int jrad, double *a
       double c0 = 0.;
       if (I >= 1 && I <= (int)H[3]) c0 = factor * LBLX_O2F_XO2[I - 1] * exp(LBLX_O2F_XO2T[I - 1] * xktfac) / VJ;
       C[J] = tau_fac * c0;
+      if (jrad == 1) C[J] = C[J] * radfn(VJ, XKT);
+    }
+    C[NPTC + 1] = C[NPTC + 2] = 0.;
+    pre_xint(H[0], H[1], v1abs, dvabs, nptabs, &ist, &last);
+    xint(V1C, V2C, H[2], C, 1.0, v1abs, dvabs, AB, ist, last);
+  }
+
+  /* N2 rotational (contnm.f90:754-790, xn2_r :2497-2567): the 296 K and
+   * 220 K tables and O2 scale factors interpolated geometrically in T */
+  if (V2 > -10.0 && V1 < 350. && xn2cn > 0.) {
+    const double *H = LBLX_N2R_HDR;
+    double tau_fac = xn2cn * (wn2 / XLOSMT) * amagat;
+    double V1C, V2C;
+    int NPTC, I1;
+    cont_grid(v1abs, v2abs, H[0], H[2], (int)H[3], &V1C, &V2C, &NPTC, &I1);
+    const double xo2 = 0.21, xn2 = 0.79, T_296 = 296., T_220 = 220.;
+    double tfac = (TAVE - T_296) / (T_220 - T_296);
+    double a_h2o = 1.;
+    for (int J = 1; J <= NPTC; J++) {
+      int I = I1 + (J - 1);
+      double c0 = 0., c1 = 0.;
+      if (I >= 1 && I <= (int)H[3]) {
+        c0 = LBLX_N2R_C296[I - 1] * pow(LBLX_N2R_C220[I - 1] / LBLX_N2R_C296[I - 1], tfac);
+        double sf_T = LBLX_N2R_SF296[I - 1] * pow(LBLX_N2R_SF220[I - 1] / LBLX_N2R_SF296[I - 1], tfac);
+        c1 = (sf_T - 1.) * (xn2) / (xo2);
+      }
+      double VJ = V1C + H[2] * (double)(J - 1);
+      C[J] = tau_fac * c0 * (x_vmr_n2 + c1 * x_vmr_o2 + a_h2o * x_vmr_h2o);
       if (jrad == 1) C[J] = C[J] * radfn(VJ, XKT);
     }
     C[NPTC + 1] = C[NPTC + 2] = 0.;

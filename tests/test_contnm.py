@@ -96,6 +96,9 @@ def test_tables_match_reference_files():
                                                                               3.981461525, 228.0]
     assert T["LBLX_O2F_XO2"][1] == 9.744e-09 and T["LBLX_O2F_XO2T"][5] == 467.0
     assert list(T["LBLX_CO2_TDEP"][:2]) == [0.144, 0.361]
+    assert list(T["LBLX_N2R_HDR"]) == [-10.0, 350.0, 5.0, 73.0]
+    assert T["LBLX_N2R_C296"][2] == 0.4979e-06 and T["LBLX_N2R_SF296"][-1] == 1.6157
+    assert T["LBLX_N2R_SF220"][1] == 1.3515
 
 
 def _check(got, v, want, lo, hi, rtol):
@@ -263,6 +266,37 @@ def test_n2_fundamental_temperature_law(t):
         np.testing.assert_allclose(c0[pos], (a228 / vk)[pos], rtol=1e-12)
 
 
+@pytest.mark.parametrize("t", [296.0, 220.0, 250.0, 310.0])
+def test_n2_rotational_temperature_law(t):
+    """N2 rotational (collision-induced) continuum below 350 cm-1 (xn2_r,
+    src/contnm.f90:2497-2567): the 296 K and 220 K tables interpolated
+    geometrically in T with tfac = (T - 296)/(220 - 296), the O2 efficiency
+    from the interpolated scale factor, (sf_T - 1) 0.79/0.21, and H2O at
+    efficiency 1, in units of amagat (contnm.f90:754-790)."""
+    wk = (1e22, 4e21, 0.0, 0.0, 0.0, 0.0, 5e23)
+    wtot = sum(wk) + 2e24
+    p = 900.0
+    lay = layer(60.0, 340.0, t, p=p, wk=wk)
+    amagat = (p / 1013.0) * (273.0 / t)
+    xh, xo = wk[0] / wtot, wk[6] / wtot
+    xn = 1.0 - xh - xo
+    hdr = T["LBLX_N2R_HDR"]
+    vk = hdr[0] + hdr[2] * np.arange(int(hdr[3]))
+    tfac = (t - 296.0) / (220.0 - 296.0)
+    c0 = T["LBLX_N2R_C296"] * (T["LBLX_N2R_C220"] / T["LBLX_N2R_C296"]) ** tfac
+    sf = T["LBLX_N2R_SF296"] * (T["LBLX_N2R_SF220"] / T["LBLX_N2R_SF296"]) ** tfac
+    ck = (xn * wtot / XLOSMT) * amagat * c0 * (xn + (sf - 1.0) * 0.79 / 0.21 * xo + xh)
+    for jrad in (0, 1):
+        a, v = absrb(run_scaled(60.0, 340.0, one(5)), lay, jrad=jrad)
+        want = interp(vk, ck * (radfn(vk, t) if jrad else 1.0), v)
+        m = (v >= 60) & (v <= 340)
+        np.testing.assert_allclose(a[m], want[m], rtol=1e-12)
+    if t == 220.0:
+        np.testing.assert_allclose(c0, T["LBLX_N2R_C220"], rtol=1e-14)
+    if t == 296.0:
+        np.testing.assert_array_equal(c0, T["LBLX_N2R_C296"])
+
+
 def test_rayleigh_closed_form():
     lay = layer(800.0, 2500.0, 250.0)
     wtot = 1e22 + 4e21 + 5e23 + 2e24
@@ -298,7 +332,7 @@ def test_scale_factor_linearity_and_codes():
 
 
 def test_unsupported_bands():
-    for v1, v2 in [(300.0, 600.0), (2000.0, 5000.0)]:
+    for v1, v2 in [(4000.0, 4500.0), (2000.0, 5000.0)]:
         with pytest.raises(RuntimeError):
             O.contnm(abi.make_run(v1, v2, icntnm=1), layer(v1, v2, 250.0), 1)
 

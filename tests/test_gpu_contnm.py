@@ -20,7 +20,7 @@ def _atm_layer(l, v1, v2, dv=0.001):
 def test_absrb_column_layers(gpu_ctx, icntnm):
     """ABSRB of layers across the column (JRAD 0 at the warm surface
     layers' 500 cm-1 edge, 1 above) on 500-2500 and 1200-2000 cm-1."""
-    for v1, v2 in [(500.0, 2500.0), (1200.0, 2000.0)]:
+    for v1, v2 in [(500.0, 2500.0), (1200.0, 2000.0), (60.0, 700.0)]:
         run = abi.make_run(v1, v2, icntnm=icntnm)
         for l in (0, 7, 20, 35, 49):
             lay = _atm_layer(l, v1, v2)
@@ -44,10 +44,29 @@ def test_absrb_scale_factors_and_closure_table(gpu_ctx):
 
 
 def test_unsupported_continuum_band(gpu_ctx):
-    run = abi.make_run(300.0, 600.0, icntnm=1)
+    run = abi.make_run(4000.0, 4600.0, icntnm=1)
     with pytest.raises(lib.LblxError) as e:
-        gpu_ctx.contnm(run, _atm_layer(3, 300.0, 600.0))
+        gpu_ctx.contnm(run, _atm_layer(3, 4000.0, 4600.0))
     assert e.value.code == abi.E_UNSUPPORTED
+
+
+def test_far_infrared_layer_od(gpu_ctx):
+    """60-340 cm-1 with the N2 rotational continuum (below 350 cm-1) and
+    lines: the layer optical depth against the oracle's, surface and upper
+    layers."""
+    tab = synth.synth_lines(8000, 30.0, 370.0, [1, 2, 3, 7], 20261211)
+    gpu_ctx.load_lines(tab)
+    run = abi.make_run(60.0, 340.0, icntnm=1)
+    atm = synth.atmosphere()
+    dv, _ = lib.plan_dv(run, atm["pave"], atm["tave"], atm["wk"], atm["wbroad"], iemit=1)
+    for l in (0, 25, 45):
+        lay = abi.make_layer(atm["pave"][l], atm["tave"][l], list(atm["wk"][l]), atm["wbroad"][l], dv[l],
+                             60.0, 340.0)
+        od, v1p, st = gpu_ctx.opdpth(run, abi.Layer.from_buffer_copy(lay))
+        ref, v1r, st_r, _ = O.od_layer(run, abi.Layer.from_buffer_copy(lay), tab)
+        print("far-IR layer", l, "points", len(od), flush=True)
+        assert v1p == v1r
+        assert_od_close(od, ref, "far IR layer %d" % l)
 
 
 def test_layer_od_with_continuum(gpu_ctx):
