@@ -219,9 +219,9 @@ int lblx_od_column_device(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layers
  * 1 cm-1 grid v1abs + (k-1)*dvabs of OPDPTH (src/lblrtm.f90:6214-6226),
  * with the radiation flag OPDPTH sets (layer->jrad is updated).  The same
  * ABSRB enters every lblx_od_* call with run->icntnm != 0.  Continua of
- * 0..4340 cm-1 (MT_CKD H2O, CO2, O2 and N2 fundamentals, Rayleigh); bands
- * reaching the N2 rotational (< 350 cm-1) or any continuum above 4340
- * cm-1 return LBLX_E_UNSUPPORTED.                                        */
+ * -10..4340 cm-1 (MT_CKD H2O, CO2, the N2 rotational band below 350 cm-1,
+ * the O2 and N2 fundamentals, Rayleigh); a band reaching above 4340 cm-1
+ * with the N2, O2 or O3 continuum on returns LBLX_E_UNSUPPORTED.        */
 int lblx_contnm(lblx_ctx *ctx, const lblx_run *run, lblx_layer *layer, double *absrb,
                 int cap, int *nptabs, double *v1abs, double *dvabs);
 
