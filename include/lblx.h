@@ -61,7 +61,16 @@ enum {
  * nblk blocks of LBLX_LINREC slots; block b holds nrec[b] records in
  * slots [b*LBLX_LINREC, b*LBLX_LINREC + nrec[b]).  vmin/vmax are the
  * block-header wavenumbers RDLIN uses to skip blocks (oprop.f90:768).
- * mol = molecule + 100*isotope, iflg < 0 marks a line-coupling record. */
+ * mol = molecule + 100*isotope, iflg < 0 marks a line-coupling record.
+ * Line coupling (LNCOR1/LINF4, oprop.f90:789-795, 1039-1061, 3118-3135):
+ * a line with iflg 1 (first-order Y and G) or 3 (reduced widths GAMMA1,
+ * GAMMA2) takes its coefficients at 200, 250, 296 and 340 K from the
+ * record after it in the same block (iflg < 0): Y or GAMMA1 in vnu, alfa,
+ * mol (the bits of a float) and tmpalf; G or GAMMA2 in sp, epp, hwhm and
+ * pshift.  A line with iflg != 0 is never rejected by LNCOR1.  Lines with
+ * iflg != 0 are refused (LBLX_E_UNSUPPORTED) by the exact-Voigt build
+ * (voigt = 1) and the tangent-linear entry points; a coupled line without
+ * its coefficient record is refused by lblx_load_lines.                  */
 #define LBLX_LINREC 250
 typedef struct lblx_lines {
   int32_t nblk;
@@ -534,6 +543,8 @@ int lblx_voigt_tables(double *f, double *coef);
  * SHRINK, oprop.f90:2934-3640) of job `job` of the last OPDPTH call.      */
 int lblx_debug_tape9(lblx_ctx *ctx, int job, double *vnu, double *sp, double *alfa,
                      double *epp, int32_t *mol, int cap, int32_t *n_out);
+/* The same list's line-coupling strengths SPP (0 for the lines SHRINK sums). */
+int lblx_debug_tape9_spp(lblx_ctx *ctx, int job, double *spp, int cap, int32_t *n_out);
 
 /* Device memory of the context's device for callers without their own
  * allocator; lblx_copy kind: 0 host->device, 1 device->host, 2 device->

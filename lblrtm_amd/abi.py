@@ -199,6 +199,59 @@ class LineTable:
                 vmax[b] = seg.max()
         return cls(nrec, vmin, vmax, **out)
 
+    @classmethod
+    def from_lines_coupled(cls, vnu, sp, alfa, epp, mol, hwhm, tmpalf, pshift, iflg, ycoef, gcoef):
+        """Pack sorted lines into 250-slot blocks with line coupling: a line
+        with iflg 1 or 3 is followed by its coefficient record (iflg -1),
+        ycoef[i] (Y or GAMMA1) and gcoef[i] (G or GAMMA2) at 200, 250, 296
+        and 340 K stored in vnu, alfa, mol (float bits), tmpalf and sp, epp,
+        hwhm, pshift (src/oprop.f90:789-795, 1039-1048).  A line and its
+        record share a block; block VMIN/VMAX are over the lines."""
+        n = len(vnu)
+        iflg = np.asarray(iflg, np.int32)
+        ycoef = np.asarray(ycoef, np.float64).reshape(n, 4)
+        gcoef = np.asarray(gcoef, np.float64).reshape(n, 4)
+        blocks = [[]]
+        for i in range(n):
+            cp = iflg[i] in (1, 3)
+            need = 2 if cp else 1
+            if len(blocks[-1]) + need > LINREC:
+                blocks.append([])
+            blocks[-1].append(("line", i))
+            if cp:
+                blocks[-1].append(("coef", i))
+        nblk = len(blocks)
+        tot = nblk * LINREC
+        out = {name: np.zeros(tot, dtype=dt) for name, dt in cls.FIELDS}
+        nrec = np.zeros(nblk, np.int32)
+        vmin = np.zeros(nblk)
+        vmax = np.zeros(nblk)
+        src = dict(vnu=vnu, sp=sp, alfa=alfa, epp=epp, mol=mol, hwhm=hwhm, tmpalf=tmpalf, pshift=pshift)
+        for b, blk in enumerate(blocks):
+            nrec[b] = len(blk)
+            vs = []
+            for k, (kind, i) in enumerate(blk):
+                j = b * LINREC + k
+                if kind == "line":
+                    for name in src:
+                        out[name][j] = src[name][i]
+                    out["iflg"][j] = iflg[i]
+                    vs.append(float(vnu[i]))
+                else:
+                    y, g = ycoef[i], gcoef[i]
+                    out["vnu"][j] = y[0]
+                    out["alfa"][j] = y[1]
+                    out["mol"][j] = np.float32(y[2]).view(np.int32)
+                    out["tmpalf"][j] = y[3]
+                    out["sp"][j] = g[0]
+                    out["epp"][j] = g[1]
+                    out["hwhm"][j] = g[2]
+                    out["pshift"][j] = g[3]
+                    out["iflg"][j] = -1
+            if vs:
+                vmin[b], vmax[b] = min(vs), max(vs)
+        return cls(nrec, vmin, vmax, **out)
+
     def nlines(self):
         return int(self.nrec.sum())
 

@@ -275,6 +275,8 @@ struct lblx_ctx {
   bool sorted = true;
   // device line table
   DevBuf d_vnu, d_sp, d_alfa, d_epp, d_hwhm, d_tmpalf, d_pshift, d_mol, d_blk, d_bstart, d_bvmax;
+  DevBuf d_cidx, d_cco;  // line coupling (DevLines::cidx, cco)
+  int64_t ncpl = 0;      // lines with IFLG > 0
   DevBuf d_tabs;  // F1 F2 F3 FG XVER
   DevBuf d_vtab, d_vtabf;  // exact-Voigt build tables, FP64 and FP32 (built on first use)
   bool voigt_ready = false;
@@ -286,6 +288,7 @@ struct lblx_ctx {
   DevBuf w_cx, w_csp, w_cal, w_cad, w_cfl, w_g, w_jmp, w_wsum, w_bcnt, w_etab, w_jobi,
       w_bat;
   DevBuf w_t9vnu, w_t9sp, w_t9alfa, w_t9epp, w_t9mol, w_t9n, w_t9blk, w_t9bvhi;
+  DevBuf w_cspp, w_t9spp, w_cpl;  // line coupling: SPP of the passing and TAPE9 lines, SPPSP per record
   DevBuf w_dirty;  // blocks whose read panel moved in the last fixpoint step
   DevBuf w_t9s;    // TAPE9-block start tables of the batch walk (six ints per block)
   DevBuf w_near, w_keep, w_kidx, w_blkmax, w_pread, w_changed, w_rec, w_r3, w_r2, w_cnt, w_od, w_absrb;
@@ -325,6 +328,8 @@ struct lblx_ctx {
     L.pshift = d_pshift.as<float>();
     L.mol = d_mol.as<int32_t>();
     L.blk = d_blk.as<int32_t>();
+    L.cidx = ncpl > 0 ? d_cidx.as<int32_t>() : nullptr;
+    L.cco = ncpl > 0 ? d_cco.as<double>() : nullptr;
     return L;
   }
 };
@@ -447,6 +452,7 @@ void lblx_destroy(lblx_ctx *c) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   DevBuf *bufs[] = {&c->d_vnu, &c->d_sp, &c->d_alfa, &c->d_epp, &c->d_hwhm, &c->d_tmpalf,
                     &c->d_pshift, &c->d_mol, &c->d_blk, &c->d_bstart, &c->d_bvmax, &c->d_tabs, &c->d_cont, &c->w_cps,
+                    &c->d_cidx, &c->d_cco, &c->w_cspp, &c->w_t9spp, &c->w_cpl,
                     &c->w_dlay, &c->w_pan, &c->w_win, &c->w_word, &c->w_seg, &c->w_r4, &c->w_margin,
                     &c->w_cx, &c->w_csp, &c->w_cal, &c->w_cad, &c->w_cfl,
                     &c->w_g, &c->w_jmp, &c->w_wsum, &c->w_bcnt, &c->w_etab, &c->w_t9s, &c->w_jobi, &c->w_bat,
@@ -473,18 +479,41 @@ void lblx_destroy(lblx_ctx *c) {
 int lblx_load_lines(lblx_ctx *c, const lblx_lines *L) {
   if (!c || !L || L->nblk < 0) return LBLX_E_ARG;
   HIPCHK(hipSetDevice(c->device));
-  // flatten the records in file order; coupling records are not supported
-  std::vector<int32_t> bstart(L->nblk + 1), blk, mol;
-  std::vector<double> vnu, bvmax(L->nblk);
+  // Flatten the lines in file order.  A coupling record (IFLG < 0) is not
+  // a line (RDLIN's IOUT skips it, oprop.f90:844-850): it is dropped, and
+  // the coefficients it holds for the line before it (IFLG 1 or 3) go to
+  // the coupling table (oprop.f90:789-795, 1039-1048).
+  std::vector<int32_t> bstart(L->nblk + 1), blk, mol, cidx;
+  std::vector<double> vnu, bvmax(L->nblk), cco;
   std::vector<float> sp, alfa, epp, hwhm, tmpalf, pshift;
   int n = 0;
+  int64_t ncpl = 0;
   for (int b = 0; b < L->nblk; b++) {
     if (L->nrec[b] > LBLX_LINREC || L->nrec[b] < 0) return LBLX_E_NREC;
     bstart[b] = n;
     bvmax[b] = L->vmax[b];
     for (int i = 0; i < L->nrec[b]; i++) {
       size_t k = (size_t)b * LBLX_LINREC + i;
-      if (L->iflg[k] != 0) return LBLX_E_UNSUPPORTED;
+      const int f = L->iflg[k];
+      if (f < 0) continue;
+      int ci = -1;
+      if (f == 1 || f == 3) {
+        // the coefficient record must follow in the same block
+        if (i + 1 >= L->nrec[b] || L->iflg[k + 1] >= 0) {
+          set_err("a coupled line (IFLG 1 or 3) lacks its coefficient record");
+          return LBLX_E_UNSUPPORTED;
+        }
+        float am;
+        std::memcpy(&am, &L->mol[k + 1], sizeof am);  // MOL holds a REAL*4
+        ci = 2 * (int)(cco.size() / 8) + (f == 3 ? 1 : 0);
+        const double c8[8] = {L->vnu[k + 1], L->alfa[k + 1], (double)am, L->tmpalf[k + 1],
+                              L->sp[k + 1], L->epp[k + 1], L->hwhm[k + 1], L->pshift[k + 1]};
+        cco.insert(cco.end(), c8, c8 + 8);
+      } else if (f > 0) {
+        ci = -2;
+      }
+      if (f > 0) ncpl++;
+      cidx.push_back(ci);
       vnu.push_back(L->vnu[k]);
       sp.push_back(L->sp[k]);
       alfa.push_back(L->alfa[k]);
@@ -516,7 +545,8 @@ int lblx_load_lines(lblx_ctx *c, const lblx_lines *L) {
       up(c->d_hwhm, hwhm.data(), n * 4) || up(c->d_tmpalf, tmpalf.data(), n * 4) ||
       up(c->d_pshift, pshift.data(), n * 4) || up(c->d_mol, mol.data(), n * 4) ||
       up(c->d_blk, blk.data(), n * 4) || up(c->d_bstart, bstart.data(), (L->nblk + 1) * 4) ||
-      up(c->d_bvmax, bvmax.data(), L->nblk * 8)) {
+      up(c->d_bvmax, bvmax.data(), L->nblk * 8) ||
+      (ncpl > 0 && (up(c->d_cidx, cidx.data(), n * 4) || up(c->d_cco, cco.data(), cco.size() * 8)))) {
     set_err("line table upload failed");
     return LBLX_E_HIP;
   }
@@ -537,6 +567,7 @@ int lblx_load_lines(lblx_ctx *c, const lblx_lines *L) {
   c->max_pshift = mp;
   c->sorted = sorted;
   c->nbrd = 0;
+  c->ncpl = ncpl;
   return LBLX_OK;
 }
 
@@ -899,6 +930,32 @@ static void far_wing_order(const lblx_ctx *c, JobsPrep &J) {
   }
 }
 
+// Temperature interval of the line-coupling coefficients (TEMPLC = 200,
+// 250, 296, 340 K).  LNCOR1 takes the first interval [ILC, ILC+1] with
+// TAVE < TEMPLC(ILC+1), ILC <= 3, anchored at ILC (oprop.f90:981-990);
+// LINF4 the first [ILC-1, ILC] with TAVE <= TEMPLC(ILC), 2 <= ILC <= 4,
+// anchored at ILC (:3077-3082).
+static void lc_interval(double tave, DevLayer &Y) {
+  static const double T[4] = {200.0, 250.0, 296.0, 340.0};
+  int c = 0;  // 0-based ILC - 1
+  for (int il = 0; il < 3; il++) {
+    c = il;
+    if (tave < T[c + 1]) break;
+  }
+  Y.lc_lo[0] = c;
+  Y.lc_an[0] = c;
+  Y.lc_rect[0] = 1.0 / (T[c + 1] - T[c]);
+  Y.lc_dif[0] = tave - T[c];
+  int f = 0;  // 0-based ILC
+  while (f < 4 && !(tave <= T[f])) f++;
+  if (f == 0) f = 1;
+  if (f == 4) f = 3;
+  Y.lc_lo[1] = f - 1;
+  Y.lc_an[1] = f;
+  Y.lc_rect[1] = 1.0 / (T[f] - T[f - 1]);
+  Y.lc_dif[1] = tave - T[f];
+}
+
 // Plan a batch of jobs (layers) on the host.  Reads only the context's line
 // table and settings, so it may run beside launches of another batch.
 // Layer plans are computed here, or handed in (pre, with the layers' JRAD
@@ -908,6 +965,7 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
   if (c->nlines <= 0) return LBLX_E_NOLINES;
   if (njobs <= 0) return LBLX_E_ARG;
   if (run->ibrd > 0 && c->nbrd > 0) return LBLX_E_UNSUPPORTED;  // broadener widths and shifts
+  if (run->voigt && c->ncpl > 0) return LBLX_E_UNSUPPORTED;      // coupling of the exact-Voigt build
   const int n = c->nlines, nblk = c->nblk;
   J.njobs = njobs;
   if (!pre && (int)J.own.size() < njobs) J.own.resize(njobs);  // plans keep their storage
@@ -994,6 +1052,9 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
     }
     Y.nmol = ly->nmol;
     Y.voigt = run->voigt ? 1 : 0;
+    lc_interval(ly->tave, Y);
+    Y.pavp0 = ly->pave / 1013.25;
+    Y.pavp2 = Y.pavp0 * Y.pavp0;
     Y.jrad = P.jrad;
     Y.max1 = P.max1;
     Y.npanel = (int)P.panels.size();
@@ -1348,6 +1409,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   // DPTMIN alone (oprop.f90:1156-1160) and XINT(R4) adds nothing
   const bool nof4 = run->ilblf4 == 0;
   if (tan && voigt) return LBLX_E_UNSUPPORTED;  // the tangent follows the tabulated shapes
+  if (tan && c->ncpl > 0) return LBLX_E_UNSUPPORTED;  // the tangent has no line-coupling terms
   if (tan && nof4) return LBLX_E_UNSUPPORTED;   // the tangent differentiates LBLF4's decisions
   if (voigt) {
     int rv = ensure_voigt(c);
@@ -1359,6 +1421,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_win, win.size() * sizeof(DevWin)) || need(c->w_word, win.size() * 4) || need(c->w_seg, seg.size() * sizeof(DevSeg)) ||
       need(c->w_r4, (size_t)nwin * R4ROW * 8) ||
       need(c->w_cx, NJ * S.line * 8) || need(c->w_csp, NJ * S.line * 8) ||
+      need(c->w_cspp, NJ * S.line * 8) || need(c->w_t9spp, NJ * S.line * 8) ||
+      (c->ncpl > 0 && need(c->w_cpl, NJ * S.line * 8)) ||
       need(c->w_cal, NJ * S.line * 8) || need(c->w_cad, NJ * S.line * 8) ||
       need(c->w_cfl, NJ * S.line) || need(c->w_g, NJ * S.line * 4) ||
       need(c->w_jmp, NJ * S.line * 4 * 3) || need(c->w_wsum, NJ * S.line * 2) ||
@@ -1423,6 +1487,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   ShrinkWork W;
   W.cx = c->w_cx.as<double>();
   W.csp = c->w_csp.as<double>();
+  W.cspp = c->w_cspp.as<double>();
   W.cal = c->w_cal.as<double>();
   W.cad = c->w_cad.as<double>();
   W.cfl = c->w_cfl.as<int8_t>();
@@ -1458,6 +1523,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   T9.sp = c->w_t9sp.as<double>();
   T9.alfa = c->w_t9alfa.as<double>();
   T9.epp = c->w_t9epp.as<double>();
+  T9.spp = c->w_t9spp.as<double>();
   T9.mol = c->w_t9mol.as<int32_t>();
   T9.n = c->w_t9n.as<int32_t>();
   T9.nblk = c->w_t9n.as<int32_t>() + NJ;
@@ -1544,6 +1610,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     recs.r2 = (SRecA *)(rb + nl * sizeof(SRec1));
     recs.r3 = recs.r2 + nl;
     recs.v = voigt ? (SRecV *)(rb + nl * (sizeof(SRec1) + 2 * sizeof(SRecA))) : nullptr;
+    recs.cpl = c->ncpl > 0 ? c->w_cpl.as<double>() : nullptr;
   }
   c->fix_passes = 0;
   c->fix_seq = 0;
@@ -1769,6 +1836,19 @@ int lblx_debug_tape9(lblx_ctx *c, int job, double *vnu, double *sp, double *alfa
     HIPCHK(hipMemcpy(epp, c->w_t9epp.as<double>() + o, k * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(mol, c->w_t9mol.as<int32_t>() + o, k * 4, hipMemcpyDeviceToHost));
   }
+  return LBLX_OK;
+}
+
+int lblx_debug_tape9_spp(lblx_ctx *c, int job, double *spp, int cap, int32_t *n_out) {
+  if (!c || !n_out || job < 0) return LBLX_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int32_t n = 0;
+  HIPCHK(hipMemcpy(&n, c->w_t9n.as<int32_t>() + job, 4, hipMemcpyDeviceToHost));
+  *n_out = n;
+  const int k = n < cap ? n : cap;
+  if (k > 0)
+    HIPCHK(hipMemcpy(spp, c->w_t9spp.as<double>() + job * c->last_line_stride, k * 8, hipMemcpyDeviceToHost));
   return LBLX_OK;
 }
 
@@ -3118,7 +3198,8 @@ int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, con
 // about nline records.
 static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
   const size_t line = ((size_t)nline + 127) / 64 * 64;
-  return line * (32 + 1 + 4 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(SRec1) + 2 * sizeof(SRecA) + sizeof(SRecV)) +
+  return line * (32 + 1 + 4 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(SRec1) + 2 * sizeof(SRecA) + sizeof(SRecV) +
+                 16 + (c->ncpl > 0 ? 8 : 0)) +
          (size_t)(c->nblk + 2) * 4 * (3 + EBLK + 1 + 3 + 6) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
 

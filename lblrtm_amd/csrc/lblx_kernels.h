@@ -19,6 +19,13 @@ struct DevLines {
   const float *sp, *alfa, *epp, *hwhm, *tmpalf, *pshift;
   const int32_t *mol;
   const int32_t *blk;     // block of each record
+  // Line coupling (null when no line has IFLG != 0).  cidx per record:
+  // -1 IFLG = 0; -2 another IFLG > 0 (never rejected, no coefficients);
+  // 2k + (IFLG == 3) for IFLG 1 or 3 with coefficient set k.  cco[8k..8k+7]
+  // = A(1..4), B(1..4) at TEMPLC = 200, 250, 296, 340 K: Y or GAMMA1 in A,
+  // G or GAMMA2 in B (oprop.f90:1039-1048).
+  const int32_t *cidx;
+  const double *cco;
 };
 
 // Per-job constants and offsets (a flattened LayerPlan).
@@ -30,6 +37,12 @@ struct DevLayer {
   double scor[7][10], alfd1[7][10], rhoslf[8], wk[8];
   int32_t nmol, jrad, max1, npanel, nwin;
   int32_t voigt;          // exact-Voigt build (oprop_voigt.f90) tables and CONVF4
+  // line-coupling coefficients at TAVE: value = C(an) + (C(hi) - C(lo)) rect
+  // dif, 0-based indices into the four temperatures; [0] LNCOR1's interval,
+  // [1] LINF4's (oprop.f90:981-990, 3077-3082)
+  int32_t lc_lo[2], lc_an[2];
+  double lc_rect[2], lc_dif[2];
+  double pavp0, pavp2;    // P/P0 and its square
   int64_t g_last, g2max, g3max, n1, n2, n3;
   // continuum grid (ICNTNM): absrb[1..nptabs] on v1abs + (k-1)*dvabs
   double v1abs, v2abs, dvabs;
@@ -73,7 +86,10 @@ struct Strides {
 // SHRINK work arrays (compacted passing lines of LINF4 and the chain tables).
 struct ShrinkWork {
   double *cx, *csp, *cal, *cad;  // centre, strength (0 below 1e-35), widths
-  int8_t *cfl;            // bit 0: CO2, bit 1: positive strength
+  double *cspp;           // coupling strength SPP (0 below 1e-35 or uncoupled)
+  // bit 0: CO2, bit 1: positive strength, bit 2: coupled (SPP != 0, emitted
+  // alone), bits 3-5: molecule
+  int8_t *cfl;
   int32_t *g;             // first-group end from each start
   int32_t *jmp;           // [line] f: next group end after a group end
   int16_t *wsum;          // [line] lines the group after a group end emits
@@ -93,7 +109,7 @@ struct ShrinkWork {
 
 // Shrunk far-wing line list (TAPE9) and its block table, per job.
 struct Tape9 {
-  double *vnu, *sp, *alfa, *epp;
+  double *vnu, *sp, *alfa, *epp, *spp;
   int32_t *mol;
   int32_t *n;            // [job] lines written
   int32_t *bstart, *bcount, *nblk;  // TAPE9 block table
@@ -112,6 +128,7 @@ struct NearHot {
 };
 struct NearCold {
   double sp, zetai;  // corrected strength, zeta
+  double sppsp;      // line coupling SPP/SP (0 uncoupled)
   int32_t izeta, pad;
 };
 
@@ -138,6 +155,10 @@ struct Recs {
   SRec1 *r1;
   SRecA *r2, *r3;
   SRecV *v;  // written and read only in the exact-Voigt build
+  double *cpl;  // SPPSP of the kept lines whose npts carries NPTS_CPL
 };
+// npts flag of a coupled line's scatter records: the tile adds
+// CLC * SPPSP * ZF times the line's value (CNVFNV, oprop.f90:1322-1351)
+constexpr int32_t NPTS_CPL = 1 << 30;
 
 }  // namespace lblx

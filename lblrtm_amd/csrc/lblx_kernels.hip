@@ -78,13 +78,47 @@ __global__ void k_fill_i32(int *p, int64_t n, int v) {
 }
 
 // ---------------------------------------------------------------------
+// Line coupling of record i at the layer's temperature: Y and G (IFLG 1)
+// or GAMMA1 and GAMMA2 (IFLG 3) interpolated linearly between the two
+// TEMPLC nodes of interval w (0: LNCOR1, oprop.f90:1039-1061; 1: LINF4,
+// :3118-3135).  flag: 0 for IFLG 0, 2 for another IFLG > 0.
+struct LcCoef {
+  double yi, gi, g1, g2;
+  int flag;
+};
+__device__ __forceinline__ LcCoef lc_coef(const DevLines &L, const DevLayer &Y, int i, int w) {
+  LcCoef r{0., 0., 0., 0., 0};
+  if (!L.cidx) return r;
+  const int k = L.cidx[i];
+  if (k == -1) return r;
+  if (k < 0) {
+    r.flag = 2;
+    return r;
+  }
+  const double *a = L.cco + (size_t)(k >> 1) * 8, *b = a + 4;
+  const int lo = Y.lc_lo[w], an = Y.lc_an[w];
+  const double va = a[an] + (a[lo + 1] - a[lo]) * Y.lc_rect[w] * Y.lc_dif[w];
+  const double vb = b[an] + (b[lo + 1] - b[lo]) * Y.lc_rect[w] * Y.lc_dif[w];
+  if (k & 1) {
+    r.flag = 3;
+    r.g1 = va;
+    r.g2 = vb;
+  } else {
+    r.flag = 1;
+    r.yi = va;
+    r.gi = vb;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------
 // LINF4 correction of one record (oprop.f90:3093-3235).  Returns whether
 // the record enters SHRINK and fills the corrected centre, strength and
 // widths.  RDLNFL's O2 air->foreign width conversion (oprop.f90:3456-3462)
 // is applied first.
 struct F4Line {
-  double x, sp, al, ad;
-  int co2;
+  double x, sp, al, ad, spp;
+  int co2, m;
 };
 
 // LINF4's tests alone: whether record i enters SHRINK, and its shifted
@@ -115,12 +149,25 @@ __device__ __forceinline__ bool linf4_line(const DevLines &L, const DevLayer &Y,
   double tmpcor = pow(Y.tratio, (double)L.tmpalf[i]);
   double alfa0i = alfa * tmpcor;
   double hwhmsi = hwhm * tmpcor;
+  const LcCoef lc = lc_coef(L, Y, i, 1);
   o.al = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
+  if (lc.flag == 3) o.al = o.al * (1.0 - lc.g1 * Y.pavp0 - lc.g2 * Y.pavp2);
   o.ad = x * Y.alfd1[m - 1][iso - 1];
   o.x = x;
-  o.sp = sui < 1.0e-35 ? 0.0 : sui;  // SHRINK zeroes S < 1e-35 (oprop.f90:3546-3549)
+  // SP = SUI (1 + G P^2), SPP = SUI Y P; SHRINK zeroes both where
+  // SP < 1e-35 (oprop.f90:3224-3225, 3546-3549)
+  const double sp = sui * (1. + lc.gi * Y.pavp2);
+  const bool z = sp < 1.0e-35;
+  o.sp = z ? 0.0 : sp;
+  o.spp = z ? 0.0 : sui * lc.yi * Y.pavp0;
   o.co2 = m == 2;
+  o.m = m;
   return true;
+}
+// SHRINK's flags of a passing line (bit 0 CO2, bit 1 positive strength,
+// bit 2 coupled, bits 3-5 molecule)
+__device__ __forceinline__ int8_t f4_flags(const F4Line &o) {
+  return (int8_t)((o.co2 ? 1 : 0) | (o.sp > 0. ? 2 : 0) | (o.spp != 0. ? 4 : 0) | (o.m << 3));
 }
 
 // ---------------------------------------------------------------------
@@ -147,13 +194,23 @@ constexpr int SB_T = 256;
 
 // Per passing line, cfl: bit 0 CO2, bit 1 positive strength.  The lines a
 // group [a, b] emits: one per class with a positive line.
-__device__ __forceinline__ int em_bits(int f) { return (f & 2) ? 1 << (f & 1) : 0; }
+// A coupled line (bit 2) is emitted on its own and enters neither sum.
+__device__ __forceinline__ int em_bits(int f) { return (f & 4) ? 0 : ((f & 2) ? 1 << (f & 1) : 0); }
 __device__ __forceinline__ int em_of(int m) { return (m & 1) + (m >> 1); }
-// the same over [a, b] by a whole wave (a, b wave-uniform)
+// the same over [a, b] by a whole wave (a, b wave-uniform), coupled lines
+// counted one each
 __device__ __forceinline__ int em_cnt(const int8_t *cfl, int a, int b) {
-  int m = 0;
-  for (int k = a + (int)(threadIdx.x & 63); k <= b; k += 64) m |= em_bits(cfl[k]);
-  return (__ballot(m & 1) ? 1 : 0) + (__ballot(m & 2) ? 1 : 0);
+  int m = 0, c = 0;
+  for (int k = a + (int)(threadIdx.x & 63); k <= b; k += 64) {
+    const int f = cfl[k];
+    m |= em_bits(f);
+    c += (f >> 2) & 1;
+  }
+  if (__ballot(c != 0))
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  else
+    c = 0;
+  return (__ballot(m & 1) ? 1 : 0) + (__ballot(m & 2) ? 1 : 0) + c;
 }
 
 // per TAPE3 block: passing lines, and the block-local position of the
@@ -241,7 +298,7 @@ struct RawCursor {
       if (linf4_line(*L, *Y, r, o)) {
         k++;
         x = o.x;
-        f = (o.co2 ? 1 : 0) | (o.sp > 0. ? 2 : 0);
+        f = f4_flags(o);
         return true;
       }
     }
@@ -274,11 +331,12 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
   const int64_t ob = job * S.blk + bl;
   const int64_t oj = job * S.line;
   const int k0 = W.bp[ob];  // passing index of the block's first passing line
-  const int8_t fl = (int8_t)((o.co2 ? 1 : 0) | (o.sp > 0. ? 2 : 0));
+  const int8_t fl = f4_flags(o);
   if (pass) {
     const int64_t k = oj + k0 + pos;
     W.cx[k] = o.x;
     W.csp[k] = o.sp;
+    W.cspp[k] = o.spp;
     W.cal[k] = o.al;
     W.cad[k] = o.ad;
     W.cfl[k] = fl;
@@ -297,7 +355,7 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
     const int ph_pos = __builtin_popcountll(mh & ((1ull << threadIdx.x) - 1ull));
     if (ph) {
       sx[tot + ph_pos] = oh.x;
-      sf[tot + ph_pos] = (int8_t)((oh.co2 ? 1 : 0) | (oh.sp > 0. ? 2 : 0));
+      sf[tot + ph_pos] = f4_flags(oh);
     }
     if (threadIdx.x == 0) s_hn = __builtin_popcountll(mh);
   }
@@ -322,7 +380,7 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
   };
   // g: first k >= e+1 with x_k > thr; f from e+2 on; the emitted classes
   // of [e+1, f]
-  int k = e + 1, m = 0, g = -1;
+  int k = e + 1, m = 0, g = -1, nc = 0;
   double x = 0.;
   int f = 0;
   for (;; k++) {
@@ -331,9 +389,11 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
       if (g < 0) g = k - 1;
       if (k >= e + 2) break;
       m |= em_bits(f);  // line e+1 is past VLMT but opens the group after e
+      nc += (f >> 2) & 1;
       continue;
     }
     m |= em_bits(f);
+    nc += (f >> 2) & 1;
   }
   if (g < 0) g = k - 1;
   W.g[oj + e] = g;
@@ -343,7 +403,7 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
     return;
   }
   W.jmp[oj + e] = k - 1;
-  W.wsum[oj + e] = (int16_t)em_of(m);
+  W.wsum[oj + e] = (int16_t)(em_of(m) + nc);
 }
 
 // Lines SHRINK emits for the batch [s, t] of passing indices: the first
@@ -680,11 +740,14 @@ __global__ __launch_bounds__(64) void k_shr_bwalk(Strides S, ShrinkWork W) {
   const int64_t o = job * S.line;
   const int s = W.bat_s[o + q], t = W.bat_t[o + q], off = W.bat_off[o + q];
   ChainWalk cw(W, o, W.npass[job], s);
+  const int lane = threadIdx.x & 63;
   auto mark = [&](int gs, int ge, int slot, int em) {
-    if (!lead || em <= 0) return;
-    W.grp[o + slot] = gs;
-    W.gout[o + slot] = ge;
-    if (em == 2) W.grp[o + slot + 1] = -1;
+    if (em <= 0) return;
+    if (lead) {
+      W.grp[o + slot] = gs;
+      W.gout[o + slot] = ge;
+    }
+    for (int k = 1 + lane; k < em; k += 64) W.grp[o + slot + k] = -1;
   };
   if (cw.e1 >= t) {
     mark(s, t, off, em_cnt(cw.cfl, s, t));
@@ -715,9 +778,21 @@ __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9
   const int ge = W.gout[o + qs];
   double sumv = 0., sums = 0., sumal = 0., sumad = 0.;
   double sumv2 = 0., sums2 = 0., sumal2 = 0., sumad2 = 0.;
+  int64_t q = o + qs;
   for (int k = gs; k <= ge; k++) {
     double sp = W.csp[o + k];
-    if (!(W.cfl[o + k] & 1)) {
+    const int fl = W.cfl[o + k];
+    if (fl & 4) {  // a coupled line: emitted as it stands, in line order
+      T9.vnu[q] = W.cx[o + k];
+      T9.sp[q] = sp;
+      T9.alfa[q] = W.cal[o + k];
+      T9.epp[q] = W.cad[o + k];
+      T9.spp[q] = W.cspp[o + k];
+      T9.mol[q] = fl >> 3;
+      q++;
+      continue;
+    }
+    if (!(fl & 1)) {
       sumv = sumv + W.cx[o + k] * sp;
       sums = sums + sp;
       sumal = sumal + sp * W.cal[o + k];
@@ -729,12 +804,12 @@ __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9
       sumad2 = sumad2 + sp * W.cad[o + k];
     }
   }
-  int64_t q = o + qs;
   if (sums > 0.) {
     T9.vnu[q] = sumv / sums;
     T9.sp[q] = sums;
     T9.alfa[q] = sumal / sums;
     T9.epp[q] = sumad / sums;
+    T9.spp[q] = 0.;
     T9.mol[q] = 0;
     q++;
   }
@@ -743,6 +818,7 @@ __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9
     T9.sp[q] = sums2;
     T9.alfa[q] = sumal2 / sums2;
     T9.epp[q] = sumad2 / sums2;
+    T9.spp[q] = 0.;
     T9.mol[q] = 2;
   }
 }
@@ -801,10 +877,12 @@ __device__ __forceinline__ float rl_t(float v, int j) { return rl_f32(v, j); }
 //   near = nA + XMSQ*nB,  far = SIL/(ALFLI2 + XMSQ) + (XMSQ*c1 - c0),
 // with c0 = SUBTR's constant, c1 its XMSQ slope, nA = SIV*A3 - c0,
 // nB = SIV*B3/ALFVI2 + c1, and the branch test XMSQ <= thr = ZSQBND*ALFVI2.
+// A coupled line (SPP != 0) adds XM DPTRAT F4FN to each value, DPTRAT =
+// SPP / (SP ALFAVI) (oprop.f90:3976-4004); cd holds DPTRAT, 0 otherwise.
 template <typename T>
 struct F4Par {
   double xnui;
-  T sil, alfli2, nA, nB, c0, c1, thr;
+  T sil, alfli2, nA, nB, c0, c1, thr, cd;
   int jmin, jmax;
 };
 
@@ -829,6 +907,12 @@ __device__ __forceinline__ T f4_at(const F4Par<T> &p, double xmd) {
   T near = fma(xmsq, p.nB, p.nA);
   T far = fma(p.sil, rcp_nr(p.alfli2 + xmsq), fma(xmsq, p.c1, -p.c0));
   return xmsq <= p.thr ? near : far;
+}
+// with the dispersive term of a coupled line
+template <typename T>
+__device__ __forceinline__ T f4_val(const F4Par<T> &p, double xmd) {
+  const T v = f4_at(p, xmd);
+  return p.cd != (T)0. ? v + ((T)xmd * p.cd) * v : v;
 }
 
 // Five waves per SIMD (registers held to 96): with the 29 KB of LDS of the
@@ -873,10 +957,11 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
   int pos = (s_b < nb9) ? T9.bstart[ot + s_b] : nl;
   unsigned long long nupd = 0;
   // raw TAPE9 fields of the chunk wave 0 resolves next (prefetched)
-  double f_sp = 0., f_al = 0., f_ad = 0., f_v = 0.;
+  double f_sp = 0., f_al = 0., f_ad = 0., f_v = 0., f_spp = 0.;
   int f_mol = 0;
   if (tid < 64 && pos + tid < nl) {
     f_sp = T9.sp[o + pos + tid];
+    f_spp = T9.spp[o + pos + tid];
     f_al = T9.alfa[o + pos + tid];
     f_ad = T9.epp[o + pos + tid];
     f_v = T9.vnu[o + pos + tid];
@@ -902,9 +987,10 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
     p.jmin = 1;
     p.jmax = 0;
     p.xnui = 0.;
+    p.cd = (T)0.;
     T speak = (T)0., partial = (T)0., pkb = (T)0., f4bnd = (T)0.;
     double vnui = 0.;
-    bool nz = false, bnd_ok = false;
+    bool nz = false, bnd_ok = false, cpl = false;
     int jj = 1;
     if (tid < 64 && more) {
       // the in-order decisions are the window's critical path: wave 0 is
@@ -912,7 +998,8 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       __builtin_amdgcn_s_setprio(3);
       if (valid) {
         double sp = f_sp;
-        nz = sp != 0.;
+        cpl = f_spp != 0.;  // a coupled line is never rejected
+        nz = sp != 0. || cpl;
         double alfadi = f_ad, alfali = f_al;
         double zetai = alfali / (alfali + alfadi);
         int iz = fint(100. * zetai + ONEPL_D);
@@ -965,7 +1052,18 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         bnd_ok = sp > 0.;
         const double pkn = siva3;
         const double pkf = sil4 / (alfli2 + zsqbnd * alfvi2 * (1. - 1e-6));
-        pkb = (T)((pkn > pkf ? pkn : pkf) * (1. + 1e-5));
+        double pkd = (pkn > pkf ? pkn : pkf) * (1. + 1e-5);
+        if (cpl) {
+          // |XM| <= BOUND4: the factor 1 + XM DPTRAT lies within
+          // 1 +- BOUND4 |DPTRAT|, and either sign of the shape may reach
+          // max(peak, 2 F4BND) in magnitude
+          const double dptrat = f_spp / (sp * alfavi);
+          p.cd = (T)dptrat;
+          const double mg = (pkd > 2. * f4b ? pkd : 2. * f4b) * (1. + bound4 * fabs(dptrat)) * (1. + 1e-5);
+          pkd = mg;
+          f4bnd = (T)(0.5 * mg);  // the lower bound takes -2 f4bnd
+        }
+        pkb = (T)pkd;
         partial = r4[jj];
       }
     }
@@ -988,6 +1086,10 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         // so the sum's rounding stays within a few ulp of XJJ
         double xmd = (double)(first - 1) * dvr4 - q.xnui;
         int j = first;
+        if (q.cd != (T)0.) {  // a coupled line
+          for (; j <= q.jmax; j += CF4_S, xmd += step) r4[j] = r4[j] + f4_val(q, xmd);
+          continue;
+        }
         if constexpr (sizeof(T) == 4) {
           // FP32 shapes: two points per step, two independent evaluations in
           // flight per thread (a thread's points are distinct, so the order
@@ -1012,7 +1114,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       for (int a = 0; a < nprev; a++) {
         const F4Par<T> q = qn;
         qn = par[cur][a + 1 < nprev ? a + 1 : a];
-        if (valid && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me - q.xnui);
+        if (valid && jj >= q.jmin && jj <= q.jmax) partial += f4_val(q, xjj_me - q.xnui);
       }
       const bool cand = valid && nz;
       // Bounds of the running R4 at this line over the undecided earlier
@@ -1037,8 +1139,8 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       const T slack = (T)Y.dptfc * (fabs(partial) + below + above) * lb_margin + fabs((T)Y.dptmn4) * lb_margin;
       const T thr_lb = (T)Y.dptmn4 + (T)Y.dptfc * (partial - below);
       const T thr_ub = (T)Y.dptmn4 + (T)Y.dptfc * (partial + above);
-      const bool surely_rej = speak < thr_lb - slack;
-      const bool surely_acc = cand && bnd_ok && speak > thr_ub + slack;
+      const bool surely_rej = !cpl && speak < thr_lb - slack;
+      const bool surely_acc = cand && (cpl || (bnd_ok && speak > thr_ub + slack));
       const uint64_t undm = __ballot(cand && !surely_rej && !surely_acc);
       const int lastund = undm ? 63 - __builtin_clzll(undm) : -1;
       const uint64_t head = lastund < 0 ? 0ull : (lastund == 63 ? ~0ull : ((1ull << (lastund + 1)) - 1));
@@ -1057,7 +1159,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         if ((sacc_m >> j) & 1ull) {
           accj = 1;
         } else {
-          bool acc = cand && !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial));
+          bool acc = cand && (cpl || !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial)));
           accj = __builtin_amdgcn_readlane(acc ? 1 : 0, j);
         }
         const int stpj = accj && ((past_m >> j) & 1ull);
@@ -1075,9 +1177,10 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
           q.c0 = rl_t(p.c0, j);
           q.c1 = rl_t(p.c1, j);
           q.thr = rl_t(p.thr, j);
+          q.cd = rl_t(p.cd, j);
           q.jmin = __builtin_amdgcn_readlane(p.jmin, j);
           q.jmax = __builtin_amdgcn_readlane(p.jmax, j);
-          if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_at(q, xjj_me - q.xnui);
+          if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_val(q, xjj_me - q.xnui);
           if (lane == j && p.jmax >= p.jmin) par[nb][nacc] = p;
           if (q.jmax >= q.jmin) {
             nacc++;
@@ -1109,6 +1212,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       const int in = npos + 64 + lane;
       if (!done && in < nl) {
         f_sp = T9.sp[o + in];
+        f_spp = T9.spp[o + in];
         f_al = T9.alfa[o + in];
         f_ad = T9.epp[o + in];
         f_v = T9.vnu[o + in];
@@ -1151,8 +1255,9 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
 // centre, 1/ALFV, corrected strength and the zeta column.  False for a
 // record LNCOR1 skips.
 struct NearCore {
-  double vnu, recalf, sp, zetai;
+  double vnu, recalf, sui, sp, zetai, sppsp;
   int izeta;
+  bool norej;  // IFLG != 0: LNCOR1 never rejects the line
 };
 __device__ __forceinline__ bool lncor1_core(const DevLines &L, const DevLayer &Y, int i, NearCore &c) {
   const int mc = L.mol[i];
@@ -1161,13 +1266,15 @@ __device__ __forceinline__ bool lncor1_core(const DevLines &L, const DevLayer &Y
   double sui = (m <= Y.nmol) ? (double)L.sp[i] * Y.wk[m - 1] : 0.;
   if (Y.jrad == 1) sui = sui * d_radfn(L.vnu[i], Y.xkt);
   if (sui == 0.) return false;
+  const LcCoef lc = lc_coef(L, Y, i, 0);
   double alfa0 = L.alfa[i], hwhm = L.hwhm[i];
   if (m == 7) alfa0 = (alfa0 - 0.21 * hwhm) / (1.0 - 0.21);  // RDLIN (oprop.f90:818-827)
   const double vnu = L.vnu[i] + Y.rhorat * (double)L.pshift[i];
   const double tmpcor = pow(Y.tratio, (double)L.tmpalf[i]);
   const double alfa0i = alfa0 * tmpcor;
   const double hwhmsi = hwhm * tmpcor;
-  const double alfl = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
+  double alfl = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
+  if (lc.flag == 3) alfl = alfl * (1.0 - lc.g1 * Y.pavp0 - lc.g2 * Y.pavp2);
   const double alfad = vnu * Y.alfd1[m - 1][iso - 1];
   const double zeta = alfl / (alfl + alfad);
   const double fzeta = 100. * zeta;
@@ -1182,7 +1289,10 @@ __device__ __forceinline__ bool lncor1_core(const DevLines &L, const DevLayer &Y
   if (epp > -0.999) sui = sui * Y.scor[m - 1][iso - 1] * exp(-epp * Y.betacr_l) * (1. + exp(-vnu / Y.xkt));
   c.vnu = vnu;
   c.recalf = 1. / alfv;
-  c.sp = sui;
+  c.sui = sui;
+  c.sp = sui * (1. + lc.gi * Y.pavp2);   // oprop.f90:1147-1149
+  c.sppsp = (sui * lc.yi * Y.pavp0) / c.sp;
+  c.norej = lc.flag != 0;
   c.zetai = zeta;
   c.izeta = iz;
   return true;
@@ -1205,7 +1315,8 @@ __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPan
   if (lncor1_core(L, Y, i, c)) {
     o.vnu = c.vnu;
     o.recalf = c.recalf;
-    o.speak = c.sp * c.recalf;
+    // SPEAK = SUI * RECALF; lines with IFLG != 0 pass every rejection
+    o.speak = c.norej ? HUGE_VAL : c.sui * c.recalf;
     o.keep = 1;
     // first fit: INT(ZINT_p + BHWDXF + 1.5) <= MAX1
     const double zslope = c.recalf * (Y.dv / 0.002);
@@ -1221,7 +1332,7 @@ __global__ void k_lncor1(DevLines L, const DevLayer *Ys, Strides S, const DevPan
     while (p > 0 && fits(p - 1)) p--;
     while (p < Y.npanel && !fits(p)) p++;
     o.ff = p;  // == npanel: never processed before the last panel
-    cold[job * S.line + t] = NearCold{c.sp, c.zetai, c.izeta, 0};
+    cold[job * S.line + t] = NearCold{c.sp, c.zetai, c.sppsp, c.izeta, 0};
   }
   out[job * S.line + t] = o;
 }
@@ -1482,6 +1593,10 @@ __global__ __launch_bounds__(256) void k_scatter_rec(const NearHot *nh, const Ne
       r2.glo = g1lo - (j2shft + 1800 * P);
       r3.glo = g1lo - (j3shft + 2250 * P);
       r1.npts = r2.npts = r3.npts = np = jmax1 - jmin1 + 1;
+      if (n.sppsp != 0. && np > 0) {  // line coupling: the dispersive term
+        r1.npts = r2.npts = r3.npts = np | NPTS_CPL;
+        recs.cpl[ci] = n.sppsp;
+      }
     }
     recs.r1[ci] = r1;
     recs.r2[ci] = r2;
@@ -1578,10 +1693,10 @@ __device__ __forceinline__ double xint_at(double v1a, double v2a, double dva, co
 template <int G, bool V>
 struct TileLine {
   int64_t glo;
-  double zfl, zslope, str, strd, strver;
+  double zfl, zslope, str, strd, strver, cpl;
   int npts, izm;
   __device__ __forceinline__ void load(const Recs &R, int64_t base, int li, int hi) {
-    strd = strver = 0.;
+    strd = strver = cpl = 0.;
     izm = 1;
     if (li < hi) {
       const int64_t i = base + li;
@@ -1609,6 +1724,7 @@ struct TileLine {
         strd = v.zdif;
         izm = v.izm;
       }
+      if (npts & NPTS_CPL) cpl = R.cpl[i];
     } else {
       npts = 0;
       glo = 0;
@@ -1685,8 +1801,13 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
     if (c1 < hi) nxt.load(recs, rbase, c1 + lane, lane < CH ? hi : c1);
     const int nj = hi - c0 < CH ? hi - c0 : CH;
     for (int j = 0; j < nj; j++) {
-      const int npts = __builtin_amdgcn_readlane(cur.npts, j);
+      const int nraw = __builtin_amdgcn_readlane(cur.npts, j);
+      const int npts = nraw & ~NPTS_CPL;
       if (npts <= 0) continue;
+      // line coupling: CLC * SPPSP, CLC = 4, 16, 64 halfwidths over 2000
+      // table steps for R1, R2, R3 (oprop.f90:1259-1261)
+      const bool cp = (nraw & NPTS_CPL) != 0;
+      const double cz = cp ? (G == 1 ? 4. : (G == 2 ? 16. : 64.)) / 2000. * rl_f64(cur.cpl, j) : 0.;
       const int64_t glo = rl_i64(cur.glo, j);
       const double zfl = rl_f64(cur.zfl, j), zslope = rl_f64(cur.zslope, j);
       const ACC str = (ACC)rl_f64(cur.str, j);
@@ -1710,6 +1831,7 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
         if (V) v = str * ((((ACC)1. - za) * ca[iz]) + (za * ca[iz + NFMX]));
         else if (G == 1) v = fma(str, t1[iz], fma(strd, tg[iz], strver * tv[iz]));
         else v = str * t1[iz];
+        if (cp) v = v + v * (ACC)(cz * zf);
         atomicAdd(&acc[kl], (LACC)v);
       }
     }

@@ -57,6 +57,7 @@ def load():
         "lblx_debug_tape9": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double),
                                        P(C.c_double), P(C.c_double), P(C.c_int32), C.c_int,
                                        P(C.c_int32)]),
+        "lblx_debug_tape9_spp": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), C.c_int, P(C.c_int32)]),
         "lblx_timing": (C.c_int, [C.c_void_p, C.c_int]),
         "lblx_timing_get": (C.c_int, [C.c_void_p, C.c_int, P(C.c_char_p), P(C.c_double),
                                       P(C.c_int64)]),
@@ -703,6 +704,13 @@ class Context:
                                         abi.ptr(mol, C.c_int32), cap, C.byref(n)), "lblx_debug_tape9")
         k = min(n.value, cap)
         return vnu[:k], sp[:k], al[:k], ad[:k], mol[:k]
+
+    def debug_tape9_spp(self, job=0, cap=2000000):
+        spp = np.zeros(cap)
+        n = C.c_int32(0)
+        _chk(self._lib.lblx_debug_tape9_spp(self._h, job, abi.ptr(spp, C.c_double), cap, C.byref(n)),
+             "lblx_debug_tape9_spp")
+        return spp[:min(n.value, cap)]
 
     def timing(self, enable=True):
         _chk(self._lib.lblx_timing(self._h, 1 if enable else 0), "lblx_timing")

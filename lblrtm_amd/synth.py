@@ -34,6 +34,23 @@ def synth_lines(n, v1, v2, mols, seed):
     return LineTable.from_lines(vnu, sp, alfa, epp, mol, hwhm, tmpalf, pshift)
 
 
+def coupled_lines(n, v1, v2, mols, seed, frac1=0.05, frac3=0.02, frac2=0.01):
+    """synth_lines with line coupling: a fraction frac1 of the lines with
+    first-order coefficients (IFLG 1: Y within +-0.1, G within +-0.3 at
+    200, 250, 296, 340 K), frac3 with reduced widths (IFLG 3: GAMMA1,
+    GAMMA2 within +-0.05) and frac2 flagged IFLG 2 (no coefficients)."""
+    t = synth_lines(n, v1, v2, mols, seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 7))
+    keep = np.concatenate([np.arange(b * 250, b * 250 + t.nrec[b]) for b in range(t.nblk)])
+    u = rng.random(n)
+    iflg = np.where(u < frac1, 1, np.where(u < frac1 + frac3, 3, np.where(u < frac1 + frac3 + frac2, 2, 0)))
+    y = np.where((iflg == 1)[:, None], rng.uniform(-0.1, 0.1, (n, 4)), rng.uniform(-0.05, 0.05, (n, 4)))
+    g = np.where((iflg == 1)[:, None], rng.uniform(-0.3, 0.3, (n, 4)), rng.uniform(-0.05, 0.05, (n, 4)))
+    f = {k: getattr(t, k)[keep] for k in ("vnu", "sp", "alfa", "epp", "mol", "hwhm", "tmpalf", "pshift")}
+    return LineTable.from_lines_coupled(f["vnu"], f["sp"], f["alfa"], f["epp"], f["mol"], f["hwhm"],
+                                        f["tmpalf"], f["pshift"], iflg, y, g)
+
+
 def config0_lines(n=20000):
     """20,000 H2O-mass lines in 600-1100 cm-1."""
     return synth_lines(n, 600.0, 1100.0, [1], SEED0 + 0)

@@ -80,6 +80,44 @@ static const double P0 = 1013.25, TEMP0 = 296.;
 
 static int fint(double x) { return (int)x; } /* Fortran INT(): toward zero */
 
+/* Line coupling.  A line with IFLG 1 or 3 carries its coefficients in the
+ * next record of its block: Y (IFLG 1) or GAMMA1 (IFLG 3) at 200, 250,
+ * 296, 340 K in VNU, ALFA, MOL (the bits of a REAL*4) and TMPALF, G or
+ * GAMMA2 in SP, EPP, HWHM and PSHIFT (oprop.f90:789-795, 1039-1048).     */
+static const double TEMPLC[5] = {0., 200.0, 250.0, 296.0, 340.0}; /* 1-based */
+static double mol_as_real(int bits)
+{
+  float f;
+  memcpy(&f, &bits, sizeof f);
+  return (double)f;
+}
+/* LNCOR1 (oprop.f90:981-990): the interval [ILC, ILC+1] with
+ * TAVE < TEMPLC(ILC+1), ILC = 1..3, evaluated from ILC.                */
+static void lc_lncor1(double tave, int *ilc, double *rect, double *dif)
+{
+  int c = 1;
+  for (int il = 1; il <= 3; il++) {
+    c = il;
+    if (tave < TEMPLC[c + 1]) break;
+  }
+  *ilc = c;
+  *rect = 1.0 / (TEMPLC[c + 1] - TEMPLC[c]);
+  *dif = tave - TEMPLC[c];
+}
+/* LINF4 (oprop.f90:3077-3082): the interval [ILC-1, ILC] with
+ * TAVE <= TEMPLC(ILC), ILC = 2..4, evaluated from ILC.                  */
+static void lc_linf4(double tave, int *ilc, double *rect, double *dif)
+{
+  int c;
+  for (c = 1; c <= 4; c++)
+    if (tave <= TEMPLC[c]) break;
+  if (c == 1) c = 2;
+  if (c == 5) c = 4;
+  *ilc = c;
+  *rect = 1.0 / (TEMPLC[c] - TEMPLC[c - 1]);
+  *dif = tave - TEMPLC[c];
+}
+
 /* ------------------------------------------------------------------ */
 /* TIPS_2003 / AtoB: SCOR(m,iso) = Q(296)/Q(T) for molecules 1..7.      */
 static double atob(double aa, const double *A, const double *B, int npt)
@@ -858,7 +896,9 @@ static int linf4(const lblx_lines *L, const lblx_layer *Ly, const molec_t *M,
   const int LIMIN = 1000;
   static shrunk_t S;
   double pavp0 = Ly->pave / P0, pavp2 = pavp0 * pavp0;
-  (void)pavp2;
+  int lc_ilc;
+  double lc_rect, lc_dif;
+  lc_linf4(Ly->tave, &lc_ilc, &lc_rect, &lc_dif);
   double vlo = v1l4, vhi = v2l4;
   double tratio = Ly->tave / TEMP0;
   double rhorat = (Ly->pave / P0) * (TEMP0 / Ly->tave);
@@ -880,7 +920,14 @@ static int linf4(const lblx_lines *L, const lblx_layer *Ly, const molec_t *M,
         line_t ln;
         get_line(L, b, i, &ln);
         if (ln.iflg < 0) continue;           /* IOUT holds IFLG >= 0 only */
-        if (ln.iflg != 0) return LBLX_E_UNSUPPORTED; /* line coupling */
+        if (ln.iflg > 0 && g_voigt) return LBLX_E_UNSUPPORTED;
+        line_t cr;                           /* coupling record */
+        if (ln.iflg == 1 || ln.iflg == 3) {
+          if (i + 1 >= nrec) return LBLX_E_UNSUPPORTED;
+          get_line(L, b, i + 1, &cr);
+          if (cr.iflg >= 0) return LBLX_E_UNSUPPORTED;
+        }
+        double YI = 0., GI = 0., GAMMA1 = 0., GAMMA2 = 0.;
         int m = ln.mol % 100;
         int iso = (ln.mol % 1000) / 100;
         if (m > 7 || m < 1) continue;        /* outside the tabulated set */
@@ -890,6 +937,19 @@ static int linf4(const lblx_lines *L, const lblx_layer *Ly, const molec_t *M,
         if (sui == 0.) continue;
         if (ln.vnu < vlo) continue;
         ij = ij + 1;
+        if (ln.iflg == 1 || ln.iflg == 3) { /* oprop.f90:3118-3135 */
+          double A[5] = {0., cr.vnu, cr.alfa, mol_as_real(cr.mol), cr.tmpalf};
+          double Bc[5] = {0., cr.sp, cr.epp, cr.hwhm, cr.pshift};
+          double SLOPEY = (A[lc_ilc] - A[lc_ilc - 1]) * lc_rect;
+          double SLOPEG = (Bc[lc_ilc] - Bc[lc_ilc - 1]) * lc_rect;
+          if (ln.iflg == 1) {
+            YI = A[lc_ilc] + SLOPEY * lc_dif;
+            GI = Bc[lc_ilc] + SLOPEG * lc_dif;
+          } else {
+            GAMMA1 = A[lc_ilc] + SLOPEY * lc_dif;
+            GAMMA2 = Bc[lc_ilc] + SLOPEG * lc_dif;
+          }
+        }
         S.vnu[ij] = ln.vnu + rhorat * ln.pshift;
         S.alfa[ij] = ln.alfa;
         S.epp[ij] = ln.epp;
@@ -903,10 +963,11 @@ static int linf4(const lblx_lines *L, const lblx_layer *Ly, const molec_t *M,
         double alfa0i = S.alfa[ij] * tmpcor;
         double hwhmsi = ln.hwhm * tmpcor;
         S.alfa[ij] = alfa0i * (rhorat - M->rhoslf[m - 1]) + hwhmsi * M->rhoslf[m - 1];
+        if (ln.iflg == 3) S.alfa[ij] = S.alfa[ij] * (1.0 - GAMMA1 * pavp0 - GAMMA2 * pavp2);
         S.epp[ij] = S.vnu[ij] * M->alfd1[m - 1][iso - 1];
         nlin = nlin + 1;
-        S.sp[ij] = sui * (1. + 0. * pavp2);
-        S.spp[ij] = sui * 0. * pavp0;
+        S.sp[ij] = sui * (1. + GI * pavp2);
+        S.spp[ij] = sui * YI * pavp0;
         if (S.vnu[ij] > vhi) { ieof = 1; goto label60; }
       }
     }
@@ -1025,6 +1086,7 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
       double alfvi2 = alfavi * alfavi;
       double xjj = (double)(jmin - 1) * W->dvr4;
       double f4bnd = sil / (alfli2 + bndsq);
+      double dptrat = spp != 0. ? spp / (sp * alfavi) : 0.; /* line coupling (oprop.f90:3976) */
       double rec_alfvi2 = 1. / alfvi2;
       double siv_a3 = siv * A3;
       double siv_b3 = siv * B3;
@@ -1053,7 +1115,8 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
           double f4fn;
           if (zvsq <= zsqbnd) f4fn = (siv_a3 + zvsq * siv_b3) - fcnt_fn;
           else f4fn = sil / (alfli2 + xmsq) - fcnt_fn;
-          r4[j] = r4[j] + f4fn * 1.0; /* CHI == 1 (oprop.f90:4075) */
+          if (spp != 0.) f4fn = f4fn + xm * dptrat * f4fn;
+          r4[j] = r4[j] + f4fn * 1.0; /* CHI == 1 (oprop.f90:4075); coupled lines add F4FN */
           xjj = xjj + W->dvr4;
         }
       } else {
@@ -1064,6 +1127,7 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
           double f4fn;
           if (zvsq <= zsqbnd) f4fn = (siv_a3 + zvsq * siv_b3) - f4bnd;
           else f4fn = sil / (alfli2 + xmsq) - f4bnd;
+          if (spp != 0.) f4fn = f4fn + xm * dptrat * f4fn;
           r4[j] = r4[j] + f4fn;
           xjj = xjj + W->dvr4;
         }
@@ -1696,7 +1760,11 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
   double HWDXF = HWF1 / DXF1;
   double CONF2 = DV / DVR2;
   double CONF3 = DV / DVR3;
-  (void)NX2; (void)NX3; (void)N2R2;
+  double CLC1 = 4. / (double)(NX1 - 1), CLC2 = 16. / (double)(NX2 - 1), CLC3 = 64. / (double)(NX3 - 1);
+  int LC_ILC;
+  double LC_RECTLC, LC_TMPDIF;
+  lc_lncor1(Ly->tave, &LC_ILC, &LC_RECTLC, &LC_TMPDIF);
+  (void)N2R2;
 
   for (;;) {
     /* label 60: read a block of lines */
@@ -1716,8 +1784,15 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
         B->iflg[i] = ln.iflg; B->vnu[i] = ln.vnu; B->sp[i] = ln.sp;
         B->alfa0[i] = ln.alfa; B->epp[i] = ln.epp; B->mol[i] = ln.mol;
         B->hwhms[i] = ln.hwhm; B->tmpalf[i] = ln.tmpalf; B->pshift[i] = ln.pshift;
-        if (ln.iflg > 0) { rc = LBLX_E_UNSUPPORTED; goto done_hirac; }
+        if (ln.iflg > 0 && g_voigt) { rc = LBLX_E_UNSUPPORTED; goto done_hirac; }
       }
+      /* a coupled line (IFLG 1 or 3) reads its coefficients from the next
+         record of the block (oprop.f90:1039-1048) */
+      for (int i = 1; i <= nrec; i++)
+        if ((B->iflg[i] == 1 || B->iflg[i] == 3) && (i == nrec || B->iflg[i + 1] >= 0)) {
+          rc = LBLX_E_UNSUPPORTED;
+          goto done_hirac;
+        }
       ILO = 1;
       int IJ = 0;
       for (int i = 1; i <= nrec; i++)
@@ -1742,6 +1817,8 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
       int i = B->iout[j];
       int m = B->mol[i] % 100;
       int iso = (B->mol[i] % 1000) / 100;
+      int IFLAG = B->iflg[i];
+      double YI = 0., GI = 0., GAMMA1 = 0., GAMMA2 = 0.;
       double SUI;
       if (m > 7 || m < 1 || iso > LBLX_ISO_MAX[m - 1] || iso < 1) goto rej25;
       B->mol[i] = m;
@@ -1749,12 +1826,29 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
       if (JRAD == 1) SUI = SUI * radfn(B->vnu[i], XKT);
       if (SUI == 0.) goto rej25;
       stats.nlin++;
+      if (IFLAG == 1 || IFLAG == 3) { /* oprop.f90:1039-1061 */
+        double A[5], Bc[5];
+        A[1] = B->vnu[i + 1]; Bc[1] = B->sp[i + 1];
+        A[2] = B->alfa0[i + 1]; Bc[2] = B->epp[i + 1];
+        A[3] = mol_as_real(B->mol[i + 1]); Bc[3] = B->hwhms[i + 1];
+        A[4] = B->tmpalf[i + 1]; Bc[4] = B->pshift[i + 1];
+        double SLOPEA = (A[LC_ILC + 1] - A[LC_ILC]) * LC_RECTLC;
+        double SLOPEB = (Bc[LC_ILC + 1] - Bc[LC_ILC]) * LC_RECTLC;
+        if (IFLAG == 1) {
+          YI = A[LC_ILC] + SLOPEA * LC_TMPDIF;
+          GI = Bc[LC_ILC] + SLOPEB * LC_TMPDIF;
+        } else {
+          GAMMA1 = A[LC_ILC] + SLOPEA * LC_TMPDIF;
+          GAMMA2 = Bc[LC_ILC] + SLOPEB * LC_TMPDIF;
+        }
+      }
       {
         B->vnu[i] = B->vnu[i] + RHORAT * B->pshift[i];
         double TMPCOR = pow(TRATIO, B->tmpalf[i]);
         double ALFA0I = B->alfa0[i] * TMPCOR;
         double HWHMSI = B->hwhms[i] * TMPCOR;
         double ALFL = ALFA0I * (RHORAT - M.rhoslf[m - 1]) + HWHMSI * M.rhoslf[m - 1];
+        if (IFLAG == 3) ALFL = ALFL * (1.0 - GAMMA1 * PAVP0 - GAMMA2 * PAVP2);
         double ALFAD = B->vnu[i] * M.alfd1[m - 1][iso - 1];
         double ZETA = ALFL / (ALFL + ALFAD);
         B->zetai[i] = ZETA;
@@ -1770,17 +1864,19 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
         if (B->epp[i] <= -1.001) B->epp[i] = fabs(B->epp[i]);
         if (B->epp[i] > -0.999)
           SUI = SUI * M.scor[m - 1][iso - 1] * exp(-B->epp[i] * BETACR) * (1. + exp(-B->vnu[i] / XKT));
-        B->sp[i] = SUI * (1. + 0. * PAVP2);
-        double SPPI = SUI * 0. * PAVP0;
+        B->sp[i] = SUI * (1. + GI * PAVP2);
+        double SPPI = SUI * YI * PAVP0;
         B->sppsp[i] = SPPI / B->sp[i];
         double SPEAK = SUI * B->recalf[i];
-        if (DVR4 <= 0.) {
-          if (SPEAK <= DPTMN) goto rej25;
-        } else {
-          int JJ = fint((B->vnu[i] - W->v1r4) / DVR4 + 1.);
-          if (JJ < 1) JJ = 1;
-          if (JJ > W->nptr4) JJ = W->nptr4;
-          if (SPEAK <= (DPTMN + DPTFC * W->r4[JJ]) && B->sppsp[i] == 0.) goto rej25;
+        if (IFLAG == 0) { /* lines with IFLG != 0 are never rejected */
+          if (DVR4 <= 0.) {
+            if (SPEAK <= DPTMN) goto rej25;
+          } else {
+            int JJ = fint((B->vnu[i] - W->v1r4) / DVR4 + 1.);
+            if (JJ < 1) JJ = 1;
+            if (JJ > W->nptr4) JJ = W->nptr4;
+            if (SPEAK <= (DPTMN + DPTFC * W->r4[JJ]) && B->sppsp[i] == 0.) goto rej25;
+          }
         }
         stats.lincnt++;
         log_dec(1, log_blk * NLIMIN + i - 1, log_pan, 1);
@@ -1884,6 +1980,30 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
             R3[J3] = R3[J3] + STRF3 * F3[IZ3];
             R2[J2] = R2[J2] + STRF2 * F2[IZ2];
             R1[J1] = R1[J1] + STRF1 * F1[IZ1] + STRD * FG[IZ1] + STRVER * XVER[IZ1];
+          }
+          if (B->sppsp[i] != 0.) { /* line coupling (oprop.f90:1322-1351) */
+            double DPTRAT = B->sppsp[i];
+            STRF3 = STRF3 * CLC3 * DPTRAT;
+            STRF2 = STRF2 * CLC2 * DPTRAT;
+            STRF1 = STRF1 * CLC1 * DPTRAT;
+            STRD = STRD * CLC1 * DPTRAT;
+            STRVER = STRVER * CLC1 * DPTRAT;
+            double ZF1L = ((double)(JMIN1 - 2) - ZINT) * ZSLOPE;
+            double ZF2L = ((double)(JMIN2 - 2) - ZINT * CONF2) * ZSLOPE;
+            double ZF3L = ((double)(JMIN3 - 2) - ZINT * CONF3) * ZSLOPE;
+            for (int J1 = JMIN1; J1 <= JMAX1; J1++) {
+              int J2 = J1 - J2SHFT;
+              int J3 = J1 - J3SHFT;
+              ZF3L = ZF3L + ZSLOPE;
+              ZF2L = ZF2L + ZSLOPE;
+              ZF1L = ZF1L + ZSLOPE;
+              int IZ3 = fint(fabs(ZF3L) + 1.5);
+              int IZ2 = fint(fabs(ZF2L) + 1.5);
+              int IZ1 = fint(fabs(ZF1L) + 1.5);
+              R3[J3] = R3[J3] + STRF3 * F3[IZ3] * ZF3L;
+              R2[J2] = R2[J2] + STRF2 * F2[IZ2] * ZF2L;
+              R1[J1] = R1[J1] + (STRF1 * F1[IZ1] + STRD * FG[IZ1] + STRVER * XVER[IZ1]) * ZF1L;
+            }
           }
         }
         ILAST = IHI;
@@ -2098,6 +2218,16 @@ void oracle_shape_tables(double *f1, double *f2, double *f3, double *fg, double 
 }
 
 /* SHRINK / LINF4 export: returns the number of shrunk lines written.     */
+/* SPP of the TAPE9 lines of the last oracle_linf4 call (line coupling). */
+static double *g_t9spp = NULL;
+static int g_t9spp_n = 0;
+int oracle_linf4_spp(double *spp, int cap)
+{
+  int n = g_t9spp_n < cap ? g_t9spp_n : cap;
+  for (int i = 0; i < n; i++) spp[i] = g_t9spp[i];
+  return g_t9spp_n;
+}
+
 int oracle_linf4(const lblx_run *run, const lblx_layer *Ly, const lblx_lines *L,
                  double *vnu, double *sp, double *alfa, double *epp, int32_t *mol,
                  int32_t *blk_count, int cap, int blkcap, int *nblk_out)
@@ -2119,6 +2249,10 @@ int oracle_linf4(const lblx_run *run, const lblx_layer *Ly, const lblx_lines *L,
   for (int i = 0; i < n; i++) {
     vnu[i] = T9.vnu[i]; sp[i] = T9.sp[i]; alfa[i] = T9.alfa[i]; epp[i] = T9.epp[i]; mol[i] = T9.mol[i];
   }
+  free(g_t9spp);
+  g_t9spp = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  for (int i = 0; i < n; i++) g_t9spp[i] = T9.spp[i];
+  g_t9spp_n = n;
   int nb = T9.nblk < blkcap ? T9.nblk : blkcap;
   for (int b = 0; b < nb; b++) blk_count[b] = T9.bcount[b];
   *nblk_out = T9.nblk;

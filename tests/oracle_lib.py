@@ -46,6 +46,8 @@ def lib():
             C.c_int, C.c_int, C.POINTER(C.c_int)]
         dp = C.POINTER(C.c_double)
         ip = C.POINTER(C.c_int)
+        L.oracle_linf4_spp.restype = C.c_int
+        L.oracle_linf4_spp.argtypes = [dp, C.c_int]
         L.oracle_panels.restype = C.c_int
         L.oracle_panels.argtypes = [dp, dp, ip, C.c_int]
         L.oracle_planck.restype = C.c_double
@@ -274,3 +276,21 @@ def od_layer_dvout(run, layer, table, dvout, cap=20000000):
 def set_icflg(icflg):
     """CONTNM's /CDERIV/ ICFLG for the following oracle calls (0: off)."""
     lib().oracle_set_icflg(int(icflg))
+
+
+def linf4(run, layer, table, cap=2000000):
+    """The oracle's shrunk far-wing list (LINF4 + SHRINK) for one layer:
+    dict of vnu, sp, alfa, epp, mol, spp and the TAPE9 block counts."""
+    arrs = [np.zeros(cap) for _ in range(4)] + [np.zeros(cap, np.int32)]
+    blk = np.zeros(100000, np.int32)
+    nb = C.c_int(0)
+    n = lib().oracle_linf4(C.byref(run), C.byref(abi.Layer.from_buffer_copy(layer)), C.byref(table.struct()),
+                           *[abi.ptr(a, C.c_double) for a in arrs[:4]], abi.ptr(arrs[4], C.c_int32),
+                           abi.ptr(blk, C.c_int32), cap, len(blk), C.byref(nb))
+    if n < 0:
+        raise RuntimeError("oracle_linf4 rc=%d" % n)
+    k = min(n, cap)
+    spp = np.zeros(max(k, 1))
+    lib().oracle_linf4_spp(abi.ptr(spp, C.c_double), k)
+    return dict(vnu=arrs[0][:k], sp=arrs[1][:k], alfa=arrs[2][:k], epp=arrs[3][:k], mol=arrs[4][:k],
+                spp=spp[:k], bcount=blk[:nb.value])

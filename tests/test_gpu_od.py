@@ -137,10 +137,11 @@ def test_tape3_file_and_errors(gpu_ctx, tmp_path):
     lay = abi.make_layer(p, t, wk, wb, dv[0], 705.0, 755.0)
     od, ref, st, st_r = _run_both(gpu_ctx, run, lay, tab)
     assert_od_close(od, ref, "tape3 file")
-    # line-coupling records are outside the supported subset
+    # a coupled line (IFLG 1) whose next record is a line, not its
+    # coefficient record, is refused
     bad = abi.LineTable(tab.nrec, tab.vmin, tab.vmax, vnu=tab.vnu, sp=tab.sp, alfa=tab.alfa,
                         epp=tab.epp, mol=tab.mol, hwhm=tab.hwhm, tmpalf=tab.tmpalf,
-                        pshift=tab.pshift, iflg=np.where(np.arange(len(tab.iflg)) == 3, -1, 0))
+                        pshift=tab.pshift, iflg=np.where(np.arange(len(tab.iflg)) == 3, 1, 0))
     with pytest.raises(lib.LblxError) as e:
         gpu_ctx.load_lines(bad)
     assert e.value.code == abi.E_UNSUPPORTED
