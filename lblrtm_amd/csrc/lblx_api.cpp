@@ -1421,8 +1421,8 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
       need(c->w_win, win.size() * sizeof(DevWin)) || need(c->w_word, win.size() * 4) || need(c->w_seg, seg.size() * sizeof(DevSeg)) ||
       need(c->w_r4, (size_t)nwin * R4ROW * 8) ||
       need(c->w_cx, NJ * S.line * 8) || need(c->w_csp, NJ * S.line * 8) ||
-      need(c->w_cspp, NJ * S.line * 8) || need(c->w_t9spp, NJ * S.line * 8) ||
-      (c->ncpl > 0 && need(c->w_cpl, NJ * S.line * 8)) ||
+      (c->ncpl > 0 && (need(c->w_cspp, NJ * S.line * 8) || need(c->w_t9spp, NJ * S.line * 8) ||
+                       need(c->w_cpl, NJ * S.line * 8))) ||
       need(c->w_cal, NJ * S.line * 8) || need(c->w_cad, NJ * S.line * 8) ||
       need(c->w_cfl, NJ * S.line) || need(c->w_g, NJ * S.line * 4) ||
       need(c->w_jmp, NJ * S.line * 4 * 3) || need(c->w_wsum, NJ * S.line * 2) ||
@@ -1487,7 +1487,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   ShrinkWork W;
   W.cx = c->w_cx.as<double>();
   W.csp = c->w_csp.as<double>();
-  W.cspp = c->w_cspp.as<double>();
+  W.cspp = c->ncpl > 0 ? c->w_cspp.as<double>() : nullptr;
   W.cal = c->w_cal.as<double>();
   W.cad = c->w_cad.as<double>();
   W.cfl = c->w_cfl.as<int8_t>();
@@ -1523,7 +1523,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   T9.sp = c->w_t9sp.as<double>();
   T9.alfa = c->w_t9alfa.as<double>();
   T9.epp = c->w_t9epp.as<double>();
-  T9.spp = c->w_t9spp.as<double>();
+  T9.spp = c->ncpl > 0 ? c->w_t9spp.as<double>() : nullptr;
   T9.mol = c->w_t9mol.as<int32_t>();
   T9.n = c->w_t9n.as<int32_t>();
   T9.nblk = c->w_t9n.as<int32_t>() + NJ;
@@ -1593,13 +1593,13 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     else if (nof4)
       ;
     else if (split)
-      hipLaunchKernelGGL(k_convf4<float>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
-                         c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ, am, mwords,
-                         ws);
+      hipLaunchKernelGGL((c->ncpl > 0 ? k_convf4<float, true> : k_convf4<float, false>), dim3(nwin), dim3(CF4_T), 0,
+                         s, T9, dY, S, c->w_win.as<DevWin>(), c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(),
+                         c->w_r4.as<double>(), cnt + NJ, am, mwords, ws);
     else
-      hipLaunchKernelGGL(k_convf4<double>, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
-                         c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ, am, mwords,
-                         ws);
+      hipLaunchKernelGGL((c->ncpl > 0 ? k_convf4<double, true> : k_convf4<double, false>), dim3(nwin), dim3(CF4_T),
+                         0, s, T9, dY, S, c->w_win.as<DevWin>(), c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(),
+                         c->w_r4.as<double>(), cnt + NJ, am, mwords, ws);
   }
   c->tend(th);
   Recs recs;
@@ -1695,9 +1695,14 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     double *r3 = c->w_r3.as<double>(), *r2 = c->w_r2.as<double>();
 #define LBLX_TILE_L(GG, OUTT, ACCT, VV, LACCT, VTAB, RIN, ROUT, ODP, ABS)                         \
   do {                                                                                             \
-    HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT, VV, LACCT>,                    \
+    if (!VV && c->ncpl > 0) LBLX_TILE_C(GG, OUTT, ACCT, false, LACCT, true, VTAB, RIN, ROUT, ODP, ABS); \
+    else LBLX_TILE_C(GG, OUTT, ACCT, VV, LACCT, false, VTAB, RIN, ROUT, ODP, ABS);                 \
+  } while (0)
+#define LBLX_TILE_C(GG, OUTT, ACCT, VV, LACCT, CP, VTAB, RIN, ROUT, ODP, ABS)                     \
+  do {                                                                                             \
+    HIPCHK(hipFuncSetAttribute((const void *)k_tile<GG, OUTT, ACCT, VV, LACCT, CP>,                \
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));             \
-    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT, VV, LACCT>), dim3(nt), dim3(TILE_T), shm, s, rec,   \
+    hipLaunchKernelGGL((k_tile<GG, OUTT, ACCT, VV, LACCT, CP>), dim3(nt), dim3(TILE_T), shm, s, rec, \
                        tm[GG - 1], T, CH, tabs, (const ACCT *)(VTAB), dY, S, dp, dw, r4,           \
                        (const double *)(ABS), dg, (const double *)(RIN), (double *)(ROUT),         \
                        (OUTT *)(ODP));                                                            \
@@ -1730,6 +1735,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
 #undef LBLX_TILE
 #undef LBLX_TILE_V
 #undef LBLX_TILE_L
+#undef LBLX_TILE_C
     c->tend(tht);
   }
   HIPCHK(hipGetLastError());
@@ -1847,7 +1853,8 @@ int lblx_debug_tape9_spp(lblx_ctx *c, int job, double *spp, int cap, int32_t *n_
   HIPCHK(hipMemcpy(&n, c->w_t9n.as<int32_t>() + job, 4, hipMemcpyDeviceToHost));
   *n_out = n;
   const int k = n < cap ? n : cap;
-  if (k > 0)
+  if (k > 0 && c->ncpl == 0) std::memset(spp, 0, (size_t)k * 8);  // no coupled lines: SPP is 0 throughout
+  else if (k > 0)
     HIPCHK(hipMemcpy(spp, c->w_t9spp.as<double>() + job * c->last_line_stride, k * 8, hipMemcpyDeviceToHost));
   return LBLX_OK;
 }
@@ -3199,7 +3206,7 @@ int lblx_jacobians_batch_device(lblx_ctx *c, const lblx_run *run, int nprof, con
 static size_t job_bytes(const lblx_ctx *c, int64_t nline) {
   const size_t line = ((size_t)nline + 127) / 64 * 64;
   return line * (32 + 1 + 4 + 4 * 3 + 2 + 12 + 36 + sizeof(NearHot) + sizeof(NearCold) + 8 + sizeof(SRec1) + 2 * sizeof(SRecA) + sizeof(SRecV) +
-                 16 + (c->ncpl > 0 ? 8 : 0)) +
+                 (c->ncpl > 0 ? 24 : 0)) +
          (size_t)(c->nblk + 2) * 4 * (3 + EBLK + 1 + 3 + 6) + (size_t)(c->nlines / 1000 + 16) * 24;
 }
 

@@ -19,6 +19,7 @@
 // the running far-wing array, SHRINK batch boundaries, far-wing windows,
 // the panel at which a line block is read) is reproduced exactly.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <climits>
 #include <cstdint>
@@ -149,17 +150,23 @@ __device__ __forceinline__ bool linf4_line(const DevLines &L, const DevLayer &Y,
   double tmpcor = pow(Y.tratio, (double)L.tmpalf[i]);
   double alfa0i = alfa * tmpcor;
   double hwhmsi = hwhm * tmpcor;
-  const LcCoef lc = lc_coef(L, Y, i, 1);
+  const int ci = L.cidx ? L.cidx[i] : -1;
   o.al = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
-  if (lc.flag == 3) o.al = o.al * (1.0 - lc.g1 * Y.pavp0 - lc.g2 * Y.pavp2);
   o.ad = x * Y.alfd1[m - 1][iso - 1];
   o.x = x;
-  // SP = SUI (1 + G P^2), SPP = SUI Y P; SHRINK zeroes both where
-  // SP < 1e-35 (oprop.f90:3224-3225, 3546-3549)
-  const double sp = sui * (1. + lc.gi * Y.pavp2);
-  const bool z = sp < 1.0e-35;
-  o.sp = z ? 0.0 : sp;
-  o.spp = z ? 0.0 : sui * lc.yi * Y.pavp0;
+  o.spp = 0.;
+  if (ci < 0) {
+    o.sp = sui < 1.0e-35 ? 0.0 : sui;  // SHRINK zeroes S < 1e-35 (oprop.f90:3546-3549)
+  } else {
+    // IFLG 3: reduced width; IFLG 1: SP = SUI (1 + G P^2), SPP = SUI Y P;
+    // SHRINK zeroes both where SP < 1e-35 (oprop.f90:3192-3225, 3546-3549)
+    const LcCoef lc = lc_coef(L, Y, i, 1);
+    if (lc.flag == 3) o.al = o.al * (1.0 - lc.g1 * Y.pavp0 - lc.g2 * Y.pavp2);
+    const double sp = sui * (1. + lc.gi * Y.pavp2);
+    const bool z = sp < 1.0e-35;
+    o.sp = z ? 0.0 : sp;
+    o.spp = z ? 0.0 : sui * lc.yi * Y.pavp0;
+  }
   o.co2 = m == 2;
   o.m = m;
   return true;
@@ -201,15 +208,12 @@ __device__ __forceinline__ int em_of(int m) { return (m & 1) + (m >> 1); }
 // counted one each
 __device__ __forceinline__ int em_cnt(const int8_t *cfl, int a, int b) {
   int m = 0, c = 0;
-  for (int k = a + (int)(threadIdx.x & 63); k <= b; k += 64) {
-    const int f = cfl[k];
+  for (int k0 = a; k0 <= b; k0 += 64) {
+    const int k = k0 + (int)(threadIdx.x & 63);
+    const int f = k <= b ? cfl[k] : 0;
     m |= em_bits(f);
-    c += (f >> 2) & 1;
+    c += __builtin_popcountll(__ballot(f & 4));
   }
-  if (__ballot(c != 0))
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-  else
-    c = 0;
   return (__ballot(m & 1) ? 1 : 0) + (__ballot(m & 2) ? 1 : 0) + c;
 }
 
@@ -336,7 +340,7 @@ __global__ __launch_bounds__(SB_T) void k_shr_compact(DevLines L, const DevLayer
     const int64_t k = oj + k0 + pos;
     W.cx[k] = o.x;
     W.csp[k] = o.sp;
-    W.cspp[k] = o.spp;
+    if (W.cspp) W.cspp[k] = o.spp;
     W.cal[k] = o.al;
     W.cad[k] = o.ad;
     W.cfl[k] = fl;
@@ -779,37 +783,47 @@ __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9
   double sumv = 0., sums = 0., sumal = 0., sumad = 0.;
   double sumv2 = 0., sums2 = 0., sumal2 = 0., sumad2 = 0.;
   int64_t q = o + qs;
-  for (int k = gs; k <= ge; k++) {
-    double sp = W.csp[o + k];
-    const int fl = W.cfl[o + k];
-    if (fl & 4) {  // a coupled line: emitted as it stands, in line order
+  if (W.cspp) {  // the group's coupled lines first, each as it stands, in line order
+    for (int k = gs; k <= ge; k++) {
+      const int fl = W.cfl[o + k];
+      if (!(fl & 4)) continue;
       T9.vnu[q] = W.cx[o + k];
-      T9.sp[q] = sp;
+      T9.sp[q] = W.csp[o + k];
       T9.alfa[q] = W.cal[o + k];
       T9.epp[q] = W.cad[o + k];
       T9.spp[q] = W.cspp[o + k];
       T9.mol[q] = fl >> 3;
       q++;
-      continue;
-    }
-    if (!(fl & 1)) {
-      sumv = sumv + W.cx[o + k] * sp;
-      sums = sums + sp;
-      sumal = sumal + sp * W.cal[o + k];
-      sumad = sumad + sp * W.cad[o + k];
-    } else {
-      sumv2 = sumv2 + W.cx[o + k] * sp;
-      sums2 = sums2 + sp;
-      sumal2 = sumal2 + sp * W.cal[o + k];
-      sumad2 = sumad2 + sp * W.cad[o + k];
     }
   }
+  // the sums over the other lines (built twice so that files without
+  // coupled lines run the plain loop)
+  auto sum_lines = [&](auto coupled) {
+    for (int k = gs; k <= ge; k++) {
+      const int fl = W.cfl[o + k];
+      double sp = W.csp[o + k];
+      if constexpr (decltype(coupled)::value) sp = (fl & 4) ? 0. : sp;  // adds nothing
+      if (!(fl & 1)) {
+        sumv = sumv + W.cx[o + k] * sp;
+        sums = sums + sp;
+        sumal = sumal + sp * W.cal[o + k];
+        sumad = sumad + sp * W.cad[o + k];
+      } else {
+        sumv2 = sumv2 + W.cx[o + k] * sp;
+        sums2 = sums2 + sp;
+        sumal2 = sumal2 + sp * W.cal[o + k];
+        sumad2 = sumad2 + sp * W.cad[o + k];
+      }
+    }
+  };
+  if (W.cspp) sum_lines(std::true_type{});
+  else sum_lines(std::false_type{});
   if (sums > 0.) {
     T9.vnu[q] = sumv / sums;
     T9.sp[q] = sums;
     T9.alfa[q] = sumal / sums;
     T9.epp[q] = sumad / sums;
-    T9.spp[q] = 0.;
+    if (T9.spp) T9.spp[q] = 0.;
     T9.mol[q] = 0;
     q++;
   }
@@ -818,7 +832,7 @@ __global__ void k_shr_emit(const DevLayer *Ys, Strides S, ShrinkWork W, Tape9 T9
     T9.sp[q] = sums2;
     T9.alfa[q] = sumal2 / sums2;
     T9.epp[q] = sumad2 / sums2;
-    T9.spp[q] = 0.;
+    if (T9.spp) T9.spp[q] = 0.;
     T9.mol[q] = 2;
   }
 }
@@ -877,12 +891,10 @@ __device__ __forceinline__ float rl_t(float v, int j) { return rl_f32(v, j); }
 //   near = nA + XMSQ*nB,  far = SIL/(ALFLI2 + XMSQ) + (XMSQ*c1 - c0),
 // with c0 = SUBTR's constant, c1 its XMSQ slope, nA = SIV*A3 - c0,
 // nB = SIV*B3/ALFVI2 + c1, and the branch test XMSQ <= thr = ZSQBND*ALFVI2.
-// A coupled line (SPP != 0) adds XM DPTRAT F4FN to each value, DPTRAT =
-// SPP / (SP ALFAVI) (oprop.f90:3976-4004); cd holds DPTRAT, 0 otherwise.
 template <typename T>
 struct F4Par {
   double xnui;
-  T sil, alfli2, nA, nB, c0, c1, thr, cd;
+  T sil, alfli2, nA, nB, c0, c1, thr;
   int jmin, jmax;
 };
 
@@ -908,23 +920,29 @@ __device__ __forceinline__ T f4_at(const F4Par<T> &p, double xmd) {
   T far = fma(p.sil, rcp_nr(p.alfli2 + xmsq), fma(xmsq, p.c1, -p.c0));
   return xmsq <= p.thr ? near : far;
 }
-// with the dispersive term of a coupled line
-template <typename T>
-__device__ __forceinline__ T f4_val(const F4Par<T> &p, double xmd) {
+// With the dispersive term of a coupled line: SPP != 0 adds XM DPTRAT
+// F4FN, DPTRAT = SPP / (SP ALFAVI) (oprop.f90:3976-4004); cd = DPTRAT, 0
+// for the other lines.  CPL: the line file has coupled lines.
+template <bool CPL, typename T>
+__device__ __forceinline__ T f4_val(const F4Par<T> &p, T cd, double xmd) {
   const T v = f4_at(p, xmd);
-  return p.cd != (T)0. ? v + ((T)xmd * p.cd) * v : v;
+  if constexpr (!CPL) return v;
+  return cd != (T)0. ? v + ((T)xmd * cd) * v : v;
 }
 
 // Five waves per SIMD (registers held to 96): with the 29 KB of LDS of the
 // FP64 build that keeps five windows per compute unit resident, so the
 // 1230 windows of a 50-layer column run in one round.
-template <typename T>
+// CPL: the line file holds coupled lines (a separate build of the kernel,
+// so that files without them run the plain code).
+template <typename T, bool CPL>
 __global__ __launch_bounds__(CF4_T) __attribute__((amdgpu_waves_per_eu(5)))
 void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const DevSeg *segs,
               const int32_t *worder, double *r4out, unsigned long long *k3upd, uint64_t *amask,
               int64_t mwords, int32_t *wspan) {
   __shared__ T r4[R4ROW];
   __shared__ F4Par<T> par[2][64];
+  __shared__ T pcd[CPL ? 2 : 1][CPL ? 64 : 1];  // DPTRAT of the lines in par
   __shared__ int s_nacc2[2], s_done2[2], s_b;
   const int tid = threadIdx.x;
   const int wi = worder[blockIdx.x];
@@ -961,7 +979,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
   int f_mol = 0;
   if (tid < 64 && pos + tid < nl) {
     f_sp = T9.sp[o + pos + tid];
-    f_spp = T9.spp[o + pos + tid];
+    if (CPL) f_spp = T9.spp[o + pos + tid];
     f_al = T9.alfa[o + pos + tid];
     f_ad = T9.epp[o + pos + tid];
     f_v = T9.vnu[o + pos + tid];
@@ -987,7 +1005,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
     p.jmin = 1;
     p.jmax = 0;
     p.xnui = 0.;
-    p.cd = (T)0.;
+    T p_cd = (T)0.;
     T speak = (T)0., partial = (T)0., pkb = (T)0., f4bnd = (T)0.;
     double vnui = 0.;
     bool nz = false, bnd_ok = false, cpl = false;
@@ -998,7 +1016,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       __builtin_amdgcn_s_setprio(3);
       if (valid) {
         double sp = f_sp;
-        cpl = f_spp != 0.;  // a coupled line is never rejected
+        cpl = CPL && f_spp != 0.;  // a coupled line is never rejected
         nz = sp != 0. || cpl;
         double alfadi = f_ad, alfali = f_al;
         double zetai = alfali / (alfali + alfadi);
@@ -1053,12 +1071,12 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         const double pkn = siva3;
         const double pkf = sil4 / (alfli2 + zsqbnd * alfvi2 * (1. - 1e-6));
         double pkd = (pkn > pkf ? pkn : pkf) * (1. + 1e-5);
-        if (cpl) {
+        if (CPL && cpl) {
           // |XM| <= BOUND4: the factor 1 + XM DPTRAT lies within
           // 1 +- BOUND4 |DPTRAT|, and either sign of the shape may reach
           // max(peak, 2 F4BND) in magnitude
           const double dptrat = f_spp / (sp * alfavi);
-          p.cd = (T)dptrat;
+          p_cd = (T)dptrat;
           const double mg = (pkd > 2. * f4b ? pkd : 2. * f4b) * (1. + bound4 * fabs(dptrat)) * (1. + 1e-5);
           pkd = mg;
           f4bnd = (T)(0.5 * mg);  // the lower bound takes -2 f4bnd
@@ -1078,6 +1096,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       for (int a = 0; a < nacc; a++) {
         const F4Par<T> q = qn;
         qn = par[cur][a + 1 < nacc ? a + 1 : a];
+        const T qcd = CPL ? pcd[cur][a] : (T)0.;
         if (ts == 0) nupd += (unsigned long long)(q.jmax - q.jmin + 1);
         int j0 = q.jmin - 1;
         int r = j0 % CF4_S;
@@ -1086,8 +1105,8 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         // so the sum's rounding stays within a few ulp of XJJ
         double xmd = (double)(first - 1) * dvr4 - q.xnui;
         int j = first;
-        if (q.cd != (T)0.) {  // a coupled line
-          for (; j <= q.jmax; j += CF4_S, xmd += step) r4[j] = r4[j] + f4_val(q, xmd);
+        if (CPL && qcd != (T)0.) {  // a coupled line
+          for (; j <= q.jmax; j += CF4_S, xmd += step) r4[j] = r4[j] + f4_val<CPL>(q, qcd, xmd);
           continue;
         }
         if constexpr (sizeof(T) == 4) {
@@ -1114,7 +1133,8 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       for (int a = 0; a < nprev; a++) {
         const F4Par<T> q = qn;
         qn = par[cur][a + 1 < nprev ? a + 1 : a];
-        if (valid && jj >= q.jmin && jj <= q.jmax) partial += f4_val(q, xjj_me - q.xnui);
+        const T qcd = CPL ? pcd[cur][a] : (T)0.;
+        if (valid && jj >= q.jmin && jj <= q.jmax) partial += f4_val<CPL>(q, qcd, xjj_me - q.xnui);
       }
       const bool cand = valid && nz;
       // Bounds of the running R4 at this line over the undecided earlier
@@ -1139,8 +1159,8 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       const T slack = (T)Y.dptfc * (fabs(partial) + below + above) * lb_margin + fabs((T)Y.dptmn4) * lb_margin;
       const T thr_lb = (T)Y.dptmn4 + (T)Y.dptfc * (partial - below);
       const T thr_ub = (T)Y.dptmn4 + (T)Y.dptfc * (partial + above);
-      const bool surely_rej = !cpl && speak < thr_lb - slack;
-      const bool surely_acc = cand && (cpl || (bnd_ok && speak > thr_ub + slack));
+      const bool surely_rej = !(CPL && cpl) && speak < thr_lb - slack;
+      const bool surely_acc = cand && ((CPL && cpl) || (bnd_ok && speak > thr_ub + slack));
       const uint64_t undm = __ballot(cand && !surely_rej && !surely_acc);
       const int lastund = undm ? 63 - __builtin_clzll(undm) : -1;
       const uint64_t head = lastund < 0 ? 0ull : (lastund == 63 ? ~0ull : ((1ull << (lastund + 1)) - 1));
@@ -1159,7 +1179,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         if ((sacc_m >> j) & 1ull) {
           accj = 1;
         } else {
-          bool acc = cand && (cpl || !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial)));
+          bool acc = cand && ((CPL && cpl) || !(speak <= ((T)Y.dptmn4 + (T)Y.dptfc * partial)));
           accj = __builtin_amdgcn_readlane(acc ? 1 : 0, j);
         }
         const int stpj = accj && ((past_m >> j) & 1ull);
@@ -1177,11 +1197,14 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
           q.c0 = rl_t(p.c0, j);
           q.c1 = rl_t(p.c1, j);
           q.thr = rl_t(p.thr, j);
-          q.cd = rl_t(p.cd, j);
+          const T qcd = CPL ? rl_t(p_cd, j) : (T)0.;
           q.jmin = __builtin_amdgcn_readlane(p.jmin, j);
           q.jmax = __builtin_amdgcn_readlane(p.jmax, j);
-          if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_val(q, xjj_me - q.xnui);
-          if (lane == j && p.jmax >= p.jmin) par[nb][nacc] = p;
+          if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_val<CPL>(q, qcd, xjj_me - q.xnui);
+          if (lane == j && p.jmax >= p.jmin) {
+            par[nb][nacc] = p;
+            if (CPL) pcd[nb][nacc] = p_cd;
+          }
           if (q.jmax >= q.jmin) {
             nacc++;
             am |= 1ull << j;
@@ -1198,7 +1221,10 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         }
         const uint64_t addm = rest & __ballot(p.jmax >= p.jmin);
         const uint64_t below_me = lane == 0 ? 0ull : (addm & ((1ull << lane) - 1));
-        if ((addm >> lane) & 1) par[nb][nacc + __builtin_popcountll(below_me)] = p;
+        if ((addm >> lane) & 1) {
+          par[nb][nacc + __builtin_popcountll(below_me)] = p;
+          if (CPL) pcd[nb][nacc + __builtin_popcountll(below_me)] = p_cd;
+        }
         nacc += __builtin_popcountll(addm);
         am |= addm;
       }
@@ -1212,7 +1238,7 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       const int in = npos + 64 + lane;
       if (!done && in < nl) {
         f_sp = T9.sp[o + in];
-        f_spp = T9.spp[o + in];
+        if (CPL) f_spp = T9.spp[o + in];
         f_al = T9.alfa[o + in];
         f_ad = T9.epp[o + in];
         f_v = T9.vnu[o + in];
@@ -1266,7 +1292,7 @@ __device__ __forceinline__ bool lncor1_core(const DevLines &L, const DevLayer &Y
   double sui = (m <= Y.nmol) ? (double)L.sp[i] * Y.wk[m - 1] : 0.;
   if (Y.jrad == 1) sui = sui * d_radfn(L.vnu[i], Y.xkt);
   if (sui == 0.) return false;
-  const LcCoef lc = lc_coef(L, Y, i, 0);
+  const int ci = L.cidx ? L.cidx[i] : -1;
   double alfa0 = L.alfa[i], hwhm = L.hwhm[i];
   if (m == 7) alfa0 = (alfa0 - 0.21 * hwhm) / (1.0 - 0.21);  // RDLIN (oprop.f90:818-827)
   const double vnu = L.vnu[i] + Y.rhorat * (double)L.pshift[i];
@@ -1274,7 +1300,10 @@ __device__ __forceinline__ bool lncor1_core(const DevLines &L, const DevLayer &Y
   const double alfa0i = alfa0 * tmpcor;
   const double hwhmsi = hwhm * tmpcor;
   double alfl = alfa0i * (Y.rhorat - Y.rhoslf[m - 1]) + hwhmsi * Y.rhoslf[m - 1];
-  if (lc.flag == 3) alfl = alfl * (1.0 - lc.g1 * Y.pavp0 - lc.g2 * Y.pavp2);
+  if (ci >= 0 && (ci & 1)) {  // IFLG 3
+    const LcCoef lc = lc_coef(L, Y, i, 0);
+    alfl = alfl * (1.0 - lc.g1 * Y.pavp0 - lc.g2 * Y.pavp2);
+  }
   const double alfad = vnu * Y.alfd1[m - 1][iso - 1];
   const double zeta = alfl / (alfl + alfad);
   const double fzeta = 100. * zeta;
@@ -1290,9 +1319,15 @@ __device__ __forceinline__ bool lncor1_core(const DevLines &L, const DevLayer &Y
   c.vnu = vnu;
   c.recalf = 1. / alfv;
   c.sui = sui;
-  c.sp = sui * (1. + lc.gi * Y.pavp2);   // oprop.f90:1147-1149
-  c.sppsp = (sui * lc.yi * Y.pavp0) / c.sp;
-  c.norej = lc.flag != 0;
+  double yi = 0., gi = 0.;
+  if (ci >= 0 && !(ci & 1)) {  // IFLG 1
+    const LcCoef lc = lc_coef(L, Y, i, 0);
+    yi = lc.yi;
+    gi = lc.gi;
+  }
+  c.sp = sui * (1. + gi * Y.pavp2);   // oprop.f90:1147-1149
+  c.sppsp = (sui * yi * Y.pavp0) / c.sp;
+  c.norej = ci != -1;
   c.zetai = zeta;
   c.izeta = iz;
   return true;
@@ -1693,10 +1728,10 @@ __device__ __forceinline__ double xint_at(double v1a, double v2a, double dva, co
 template <int G, bool V>
 struct TileLine {
   int64_t glo;
-  double zfl, zslope, str, strd, strver, cpl;
+  double zfl, zslope, str, strd, strver;
   int npts, izm;
   __device__ __forceinline__ void load(const Recs &R, int64_t base, int li, int hi) {
-    strd = strver = cpl = 0.;
+    strd = strver = 0.;
     izm = 1;
     if (li < hi) {
       const int64_t i = base + li;
@@ -1724,7 +1759,6 @@ struct TileLine {
         strd = v.zdif;
         izm = v.izm;
       }
-      if (npts & NPTS_CPL) cpl = R.cpl[i];
     } else {
       npts = 0;
       glo = 0;
@@ -1751,7 +1785,8 @@ constexpr int TILE_T = LBLX_TILE_T;
 //   no FG or XVER term.
 //   LACC: type of the LDS accumulator (ACC, or double under FP32 shapes
 //   where LDS float atomics stall).
-template <int G, typename OUT, typename ACC, bool V, typename LACC>
+//   CPL: the line file has coupled lines (their records carry NPTS_CPL).
+template <int G, typename OUT, typename ACC, bool V, typename LACC, bool CPL>
 __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile, int ch,
                                               const double *tabs, const ACC *vtab, const DevLayer *Ys,
                                               Strides S, const DevPanel *pans, const DevWin *wins,
@@ -1802,12 +1837,9 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
     const int nj = hi - c0 < CH ? hi - c0 : CH;
     for (int j = 0; j < nj; j++) {
       const int nraw = __builtin_amdgcn_readlane(cur.npts, j);
-      const int npts = nraw & ~NPTS_CPL;
+      const int npts = CPL ? nraw & ~NPTS_CPL : nraw;
       if (npts <= 0) continue;
-      // line coupling: CLC * SPPSP, CLC = 4, 16, 64 halfwidths over 2000
-      // table steps for R1, R2, R3 (oprop.f90:1259-1261)
-      const bool cp = (nraw & NPTS_CPL) != 0;
-      const double cz = cp ? (G == 1 ? 4. : (G == 2 ? 16. : 64.)) / 2000. * rl_f64(cur.cpl, j) : 0.;
+      const bool cp = CPL && (nraw & NPTS_CPL) != 0;
       const int64_t glo = rl_i64(cur.glo, j);
       const double zfl = rl_f64(cur.zfl, j), zslope = rl_f64(cur.zslope, j);
       const ACC str = (ACC)rl_f64(cur.str, j);
@@ -1822,18 +1854,28 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
       // tile-local point index kl; the line's point offset is kl + d0
       // (|d0| is at most the tile width or the line's point count)
       const int kb = (int)(b - T0), d0 = (int)(T0 - glo);
-      for (int kl = (int)(a - T0) + lane; kl <= kb; kl += 64) {
-        const int off = kl + d0;
-        double zf = zfl + (double)(off + 1) * zslope;
-        int iz = fint(fabs(zf) + 1.5) - 1;
-        iz = iz > NFMX - 1 ? NFMX - 1 : iz;  // |zf| <= 2500 by construction
-        ACC v;
-        if (V) v = str * ((((ACC)1. - za) * ca[iz]) + (za * ca[iz + NFMX]));
-        else if (G == 1) v = fma(str, t1[iz], fma(strd, tg[iz], strver * tv[iz]));
-        else v = str * t1[iz];
-        if (cp) v = v + v * (ACC)(cz * zf);
-        atomicAdd(&acc[kl], (LACC)v);
-      }
+      // the point loop, built twice so that uncoupled lines run it plain
+      auto points = [&](auto coupled) {
+        for (int kl = (int)(a - T0) + lane; kl <= kb; kl += 64) {
+          const int off = kl + d0;
+          double zf = zfl + (double)(off + 1) * zslope;
+          int iz = fint(fabs(zf) + 1.5) - 1;
+          iz = iz > NFMX - 1 ? NFMX - 1 : iz;  // |zf| <= 2500 by construction
+          ACC v;
+          if (V) v = str * ((((ACC)1. - za) * ca[iz]) + (za * ca[iz + NFMX]));
+          else if (G == 1) v = fma(str, t1[iz], fma(strd, tg[iz], strver * tv[iz]));
+          else v = str * t1[iz];
+          if constexpr (decltype(coupled)::value) {
+            // line coupling: CLC * SPPSP * ZF, CLC = 4, 16, 64 halfwidths
+            // over 2000 table steps for R1, R2, R3 (oprop.f90:1259-1261)
+            const double cz = (G == 1 ? 4. : (G == 2 ? 16. : 64.)) / 2000. * recs.cpl[rbase + c0 + j];
+            v = v + v * (ACC)(cz * zf);
+          }
+          atomicAdd(&acc[kl], (LACC)v);
+        }
+      };
+      if (cp) points(std::true_type{});
+      else points(std::false_type{});
     }
     cur = nxt;
   }
