@@ -68,9 +68,9 @@ enum {
  * record after it in the same block (iflg < 0): Y or GAMMA1 in vnu, alfa,
  * mol (the bits of a float) and tmpalf; G or GAMMA2 in sp, epp, hwhm and
  * pshift.  A line with iflg != 0 is never rejected by LNCOR1.  Lines with
- * iflg != 0 are refused (LBLX_E_UNSUPPORTED) by the exact-Voigt build
- * (voigt = 1) and the tangent-linear entry points; a coupled line without
- * its coefficient record is refused by lblx_load_lines.                  */
+ * iflg != 0 are refused (LBLX_E_UNSUPPORTED) by the tangent-linear entry
+ * points; a coupled line without its coefficient record is refused by
+ * lblx_load_lines.                                                       */
 #define LBLX_LINREC 250
 typedef struct lblx_lines {
   int32_t nblk;

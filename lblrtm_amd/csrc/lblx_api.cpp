@@ -965,7 +965,6 @@ static int prep_jobs(const lblx_ctx *c, const lblx_run *run, lblx_layer *layers,
   if (c->nlines <= 0) return LBLX_E_NOLINES;
   if (njobs <= 0) return LBLX_E_ARG;
   if (run->ibrd > 0 && c->nbrd > 0) return LBLX_E_UNSUPPORTED;  // broadener widths and shifts
-  if (run->voigt && c->ncpl > 0) return LBLX_E_UNSUPPORTED;      // coupling of the exact-Voigt build
   const int n = c->nlines, nblk = c->nblk;
   J.njobs = njobs;
   if (!pre && (int)J.own.size() < njobs) J.own.resize(njobs);  // plans keep their storage
@@ -1695,7 +1694,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     double *r3 = c->w_r3.as<double>(), *r2 = c->w_r2.as<double>();
 #define LBLX_TILE_L(GG, OUTT, ACCT, VV, LACCT, VTAB, RIN, ROUT, ODP, ABS)                         \
   do {                                                                                             \
-    if (!VV && c->ncpl > 0) LBLX_TILE_C(GG, OUTT, ACCT, false, LACCT, true, VTAB, RIN, ROUT, ODP, ABS); \
+    if (c->ncpl > 0) LBLX_TILE_C(GG, OUTT, ACCT, VV, LACCT, true, VTAB, RIN, ROUT, ODP, ABS);       \
     else LBLX_TILE_C(GG, OUTT, ACCT, VV, LACCT, false, VTAB, RIN, ROUT, ODP, ABS);                 \
   } while (0)
 #define LBLX_TILE_C(GG, OUTT, ACCT, VV, LACCT, CP, VTAB, RIN, ROUT, ODP, ABS)                     \

@@ -920,7 +920,6 @@ static int linf4(const lblx_lines *L, const lblx_layer *Ly, const molec_t *M,
         line_t ln;
         get_line(L, b, i, &ln);
         if (ln.iflg < 0) continue;           /* IOUT holds IFLG >= 0 only */
-        if (ln.iflg > 0 && g_voigt) return LBLX_E_UNSUPPORTED;
         line_t cr;                           /* coupling record */
         if (ln.iflg == 1 || ln.iflg == 3) {
           if (i + 1 >= nrec) return LBLX_E_UNSUPPORTED;
@@ -1092,6 +1091,7 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
       double siv_b3 = siv * B3;
       *k3upd += jmax - jmin + 1;
       if (g_voigt) { /* src/oprop_voigt.f90:3929-3971: CO2 only through CHI (= 1) */
+        double dptrat_v = spp != 0. ? spp / (sp * alfavi) : 0.;
         double siv_a = siv * a3x, siv_b = siv * b3x;
         double siv_64 = siv * f4_64x * (alfli2 + alfvi2 * zsqbnd);
         double f4bnd_v = siv_64 / (alfli2 + bndsq);
@@ -1102,7 +1102,8 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
           double f4fn;
           if (zvsq <= zsqbnd) f4fn = (siv_a + zvsq * siv_b) - f4bnd_v;
           else f4fn = siv_64 / (alfli2 + xmsq) - f4bnd_v;
-          if (T9->mol[i] == 2) r4[j] = r4[j] + f4fn * 1.0;
+          if (spp != 0.) f4fn = f4fn + xm * dptrat_v * f4fn;
+          if (T9->mol[i] == 2 && spp == 0.) r4[j] = r4[j] + f4fn * 1.0;
           else r4[j] = r4[j] + f4fn;
           xjj = xjj + W->dvr4;
         }
@@ -1784,7 +1785,6 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
         B->iflg[i] = ln.iflg; B->vnu[i] = ln.vnu; B->sp[i] = ln.sp;
         B->alfa0[i] = ln.alfa; B->epp[i] = ln.epp; B->mol[i] = ln.mol;
         B->hwhms[i] = ln.hwhm; B->tmpalf[i] = ln.tmpalf; B->pshift[i] = ln.pshift;
-        if (ln.iflg > 0 && g_voigt) { rc = LBLX_E_UNSUPPORTED; goto done_hirac; }
       }
       /* a coupled line (IFLG 1 or 3) reads its coefficients from the next
          record of the block (oprop.f90:1039-1048) */
@@ -1898,7 +1898,7 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
           double DEPTHI = B->sp[i] * B->recalf[i];
           int IZM = B->izeta[i];
           double ZETDIF = 100. * B->zetai[i] - (double)(IZM - 1);
-          if (g_voigt) { /* src/oprop_voigt.f90:1273-1336 (no line coupling) */
+          if (g_voigt) { /* src/oprop_voigt.f90:1273-1360 */
             double A = ZETDIF;
             double ZSLOPE = B->recalf[i] * WAVDXF;
             double ZINT = (B->vnu[i] - VFT) / DV;
@@ -1923,6 +1923,9 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
             const double *f2a = VF[NZETA + IZM - 1], *f2b = VF[NZETA + IZM];
             const double *f3a = VF[2 * NZETA + IZM - 1], *f3b = VF[2 * NZETA + IZM];
             stats.k1_updates += (int64_t)(JMAX1 - JMIN1 + 1);
+            /* line coupling: x + SPPSP CLC x ZF (:1339-1360) */
+            const int cpl = B->sppsp[i] != 0.;
+            double dptrat1 = B->sppsp[i] * CLC1, dptrat2 = B->sppsp[i] * CLC2, dptrat3 = B->sppsp[i] * CLC3;
             for (int J1 = JMIN1; J1 <= JMAX1; J1++) {
               int J2 = J1 - J2SHFT;
               int J3 = J1 - J3SHFT;
@@ -1935,9 +1938,18 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
               double x3 = DEPTHI * ((1. - A) * f3a[IZ3] + A * f3b[IZ3]);
               double x2 = DEPTHI * ((1. - A) * f2a[IZ2] + A * f2b[IZ2]);
               double x1 = DEPTHI * ((1. - A) * f1a[IZ1] + A * f1b[IZ1]);
-              R3[J3] = R3[J3] + x3;
-              R2[J2] = R2[J2] + x2;
-              R1[J1] = R1[J1] + x1;
+              if (cpl) {
+                double y3 = dptrat3 * x3 * ZF3;
+                double y2 = dptrat2 * x2 * ZF2;
+                double y1 = dptrat1 * x1 * ZF1;
+                R3[J3] = R3[J3] + x3 + y3;
+                R2[J2] = R2[J2] + x2 + y2;
+                R1[J1] = R1[J1] + x1 + y1;
+              } else {
+                R3[J3] = R3[J3] + x3;
+                R2[J2] = R2[J2] + x2;
+                R1[J1] = R1[J1] + x1;
+              }
             }
             continue;
           }

@@ -103,23 +103,26 @@ def test_coefficients_interpolate_linearly_in_temperature(t, iflg):
         assert np.abs(o_w - o_con).max() > 1e-4 * np.abs(o_con).max()
 
 
-def voigt(x, al, ad):
+def voigt_profile(x, al, ad):
     sig = ad / math.sqrt(2.0 * math.log(2.0))
     z = (x + 1j * al) / (sig * math.sqrt(2.0))
     return wofz(z).real / (sig * math.sqrt(2.0 * math.pi))
 
 
+@pytest.mark.parametrize("voigt", [0, 1])
 @pytest.mark.parametrize("p,t,yv,gv", [(P0, T0, 0.05, 0.0), (P0, T0, -0.08, 0.3), (600.0, T0, 0.12, -0.4)])
-def test_dispersive_term_closed_form(p, t, yv, gv):
+def test_dispersive_term_closed_form(p, t, yv, gv, voigt):
     """The coupled minus the uncoupled optical depth (same G) is
     Y (P/P0) x / alpha_V times the reference's cut line: S' (V(x) -
-    c(x) alpha_L / (pi (alpha_L^2 + 625))), c = 2 - x^2/625 for CO2, times
-    RADFN on the grid (JRAD = 0 below V1/kT = 5); at P0 the line is
-    Lorentzian to 1 %, where this is Rosenkranz's profile.  G cancels:
-    SPP = S Y P/P0 carries no G."""
+    c(x) alpha_L / (pi (alpha_L^2 + 625))), c = 2 - x^2/625 for CO2 in the
+    approximate build and 1 in the exact-Voigt build (CO2 only through
+    CHI = 1, src/oprop_voigt.f90:3929-3971), times RADFN on the grid (JRAD
+    = 0 below V1/kT = 5); at P0 the line is Lorentzian to 1 %, where this
+    is Rosenkranz's profile.  G cancels: SPP = S Y P/P0 carries no G."""
     nu0, s, alfa, hwhm, n, epp, shift, mol = LINE
-    o1, nu, jrad = od(table([LINE], [1], [yv] * 4, [gv] * 4), p, t, 2, 660.0, 740.0, dptmin=0.0, dptfac=0.0)
-    o0, _, _ = od(table([LINE], [1], [0.0] * 4, [gv] * 4), p, t, 2, 660.0, 740.0, dptmin=0.0, dptfac=0.0)
+    kw = dict(dptmin=0.0, dptfac=0.0, voigt=voigt)
+    o1, nu, jrad = od(table([LINE], [1], [yv] * 4, [gv] * 4), p, t, 2, 660.0, 740.0, **kw)
+    o0, _, _ = od(table([LINE], [1], [0.0] * 4, [gv] * 4), p, t, 2, 660.0, 740.0, **kw)
     assert jrad == 0
     d = o1 - o0
     x = nu - nu0
@@ -134,8 +137,8 @@ def test_dispersive_term_closed_form(p, t, yv, gv):
     xkt = t / RADCN2
     sprime = float(f32(s)) * w * (1.0 + math.exp(-nu0 / xkt))  # SCOR = 1, E'' term = 1 at 296 K
     assert t == T0
-    cut = (2.0 - x * x / 625.0) * al / (math.pi * (al * al + 625.0))
-    want = (yv * p / P0) * x / av * sprime * (voigt(x, al, ad) - cut) * nu * np.tanh(nu / (2.0 * xkt))
+    cut = (1.0 if voigt else (2.0 - x * x / 625.0)) * al / (math.pi * (al * al + 625.0))
+    want = (yv * p / P0) * x / av * sprime * (voigt_profile(x, al, ad) - cut) * nu * np.tanh(nu / (2.0 * xkt))
     m = (np.abs(x) > 0.5 * al) & (np.abs(x) < 10.0)
     err = np.abs(d[m] - want[m]) / np.abs(want[m])
     # the tabulated decomposition follows the Voigt profile to ~1e-3 in the
@@ -259,12 +262,9 @@ def test_lines_with_iflg_are_not_rejected():
 
 
 def test_refusals():
-    """A coupled line without its coefficient record, and coupling in the
-    exact-Voigt build, are refused."""
+    """A coupled line without its coefficient record is refused."""
     lines = [LINE, (701.0,) + LINE[1:]]
     tab = table(lines, [0, 0], [0.0] * 4, [0.0] * 4)
     tab.iflg[0] = 1  # claims coefficients, but the next record is a line
     with pytest.raises(RuntimeError, match="rc=-5"):
         od(tab, P0, T0, 2, 660.0, 740.0)
-    with pytest.raises(RuntimeError, match="rc=-5"):
-        od(table([LINE], [1], [0.01] * 4, [0.0] * 4), P0, T0, 2, 660.0, 740.0, voigt=1)

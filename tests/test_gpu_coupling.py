@@ -46,19 +46,20 @@ def _both(ctx, tab, run, p, t, wk, wb, v1, v2, tape9=False):
     return od, ref, st, st_r
 
 
+@pytest.mark.parametrize("voigt", [0, 1])
 @pytest.mark.parametrize("iflg,p,t,y,g", [
     (1, P0, T0, [0.05] * 4, [0.0] * 4),
     (1, 600.0, 231.0, [0.03, -0.02, 0.05, 0.01], [0.3, -0.2, 0.1, 0.4]),
     (1, 300.0, 355.0, [-0.06, 0.02, 0.04, -0.01], [0.1, 0.2, -0.1, 0.0]),
     (3, 700.0, 262.0, [0.06, 0.12, -0.03, 0.09], [-0.06, 0.03, 0.12, 0.0]),
     (2, P0, T0, [0.0] * 4, [0.0] * 4)])
-def test_single_coupled_line(gpu_ctx, iflg, p, t, y, g):
+def test_single_coupled_line(gpu_ctx, iflg, p, t, y, g, voigt):
     tab = _one(LINE, iflg, y, g)
     gpu_ctx.load_lines(tab)
     wk, wb = [0.0, 1e20], 1e24
-    print("single line iflg", iflg, "p", p, "t", t, flush=True)
+    print("single line iflg", iflg, "p", p, "t", t, "voigt", voigt, flush=True)
     for kw in (dict(dptmin=0.0, dptfac=0.0), dict()):
-        run = abi.make_run(660.0, 740.0, **kw)
+        run = abi.make_run(660.0, 740.0, voigt=voigt, **kw)
         od, ref, st, st_r = _both(gpu_ctx, tab, run, p, t, wk, wb, 660.0, 740.0, tape9=True)
         assert_od_close(od, ref, "iflg %d" % iflg)
         for k in ("npanel", "npts", "k1_updates", "k3_updates", "nlins4", "lincnt"):
@@ -79,9 +80,9 @@ def test_weak_lines_not_rejected(gpu_ctx):
         assert (np.abs(ref).max() > 0) == (iflg != 0)
 
 
-@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_SPLIT])
+@pytest.mark.parametrize("precision,voigt", [(abi.FP64, 0), (abi.FP32_SPLIT, 0), (abi.FP64, 1)])
 @pytest.mark.parametrize("p,t", [(500.0, 250.0), (900.0, 290.0), (120.0, 215.0)])
-def test_dense_coupled_layer(gpu_ctx, precision, p, t):
+def test_dense_coupled_layer(gpu_ctx, precision, voigt, p, t):
     """20,000 lines over 600-1100 cm-1 (H2O, CO2, O3), 5 % with Y and G, 2 %
     with reduced widths, 1 % IFLG 2: coupled lines sit inside SHRINK's
     groups and LINF4's batches; the TAPE9 list (with SPP), the statistics
@@ -90,8 +91,8 @@ def test_dense_coupled_layer(gpu_ctx, precision, p, t):
     assert (tab.iflg == 1).sum() > 800 and (tab.iflg == -1).sum() == ((tab.iflg == 1) | (tab.iflg == 3)).sum()
     gpu_ctx.load_lines(tab)
     wk, wb = [1e22, 4e21, 1e18], 1.2e25
-    run = abi.make_run(610.0, 1090.0, precision=precision)
-    print("dense coupled layer p", p, "t", t, "precision", precision, flush=True)
+    run = abi.make_run(610.0, 1090.0, precision=precision, voigt=voigt)
+    print("dense coupled layer p", p, "t", t, "precision", precision, "voigt", voigt, flush=True)
     od, ref, st, st_r = _both(gpu_ctx, tab, run, p, t, wk, wb, 610.0, 1090.0, tape9=precision == abi.FP64)
     rel = np.abs(od - ref)[np.abs(ref) > 1e-6] / np.abs(ref)[np.abs(ref) > 1e-6]
     print("  npts", len(od), "max rel", rel.max(), flush=True)
@@ -148,17 +149,13 @@ def test_tape3_round_trip_with_coupling(gpu_ctx, tmp_path):
 
 
 def test_refusals(gpu_ctx):
-    """The exact-Voigt build and the tangent-linear path refuse coupling."""
+    """The tangent-linear path refuses coupling."""
     tab = _one(LINE, 1, [0.05] * 4, [0.0] * 4)
     gpu_ctx.load_lines(tab)
     p, t, wk, wb = P0, T0, [0.0, 1e20], 1e24
-    run = abi.make_run(660.0, 740.0, voigt=1)
+    run = abi.make_run(660.0, 740.0)
     dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
     lay = abi.make_layer(p, t, wk, wb, dv[0], 660.0, 740.0)
-    with pytest.raises(lib.LblxError) as e:
-        gpu_ctx.opdpth(run, abi.Layer.from_buffer_copy(lay))
-    assert e.value.code == abi.E_UNSUPPORTED
-    run = abi.make_run(660.0, 740.0)
     with pytest.raises(lib.LblxError) as e:
         gpu_ctx.od_tangent(run, [abi.Layer.from_buffer_copy(lay)], [2])
     assert e.value.code == abi.E_UNSUPPORTED
