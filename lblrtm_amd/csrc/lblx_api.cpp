@@ -1409,6 +1409,14 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
   const bool nof4 = run->ilblf4 == 0;
   if (tan && voigt) return LBLX_E_UNSUPPORTED;  // the tangent follows the tabulated shapes
   if (tan && c->ncpl > 0) return LBLX_E_UNSUPPORTED;  // the tangent has no line-coupling terms
+  // below 25 cm-1 + the far-wing bound: the VVW mirror in CONVF4 (and RSYM
+  // for a first panel at or below 0 cm-1)
+  bool mir = false, fold = false;
+  for (int jb = 0; jb < njobs; jb++) {
+    mir = mir || (!nof4 && plans[jb]->v1l4 <= 25.);
+    fold = fold || plans[jb]->vft0 <= 0.;
+  }
+  if (tan && (mir || fold)) return LBLX_E_UNSUPPORTED;  // the tangent has neither
   if (tan && nof4) return LBLX_E_UNSUPPORTED;   // the tangent differentiates LBLF4's decisions
   if (voigt) {
     int rv = ensure_voigt(c);
@@ -1591,14 +1599,21 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
                          c->w_r4.as<double>(), (int64_t)nwin * R4ROW, 0.0);
     else if (nof4)
       ;
-    else if (split)
-      hipLaunchKernelGGL((c->ncpl > 0 ? k_convf4<float, true> : k_convf4<float, false>), dim3(nwin), dim3(CF4_T), 0,
-                         s, T9, dY, S, c->w_win.as<DevWin>(), c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(),
-                         c->w_r4.as<double>(), cnt + NJ, am, mwords, ws);
-    else
-      hipLaunchKernelGGL((c->ncpl > 0 ? k_convf4<double, true> : k_convf4<double, false>), dim3(nwin), dim3(CF4_T),
-                         0, s, T9, dY, S, c->w_win.as<DevWin>(), c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(),
-                         c->w_r4.as<double>(), cnt + NJ, am, mwords, ws);
+    else {
+      // coupled lines (CPL) and the mirror of lines below 25 cm-1 (MIR)
+      // each take their own build of the kernel
+      using KF = void (*)(Tape9, const DevLayer *, Strides, const DevWin *, const DevSeg *, const int32_t *,
+                          double *, unsigned long long *, uint64_t *, int64_t, int32_t *);
+      const bool cp = c->ncpl > 0;
+      const KF kd[2][2] = {{k_convf4<double, false, false>, k_convf4<double, false, true>},
+                           {k_convf4<double, true, false>, k_convf4<double, true, true>}};
+      const KF kf[2][2] = {{k_convf4<float, false, false>, k_convf4<float, false, true>},
+                           {k_convf4<float, true, false>, k_convf4<float, true, true>}};
+      const KF k = split ? kf[cp][mir] : kd[cp][mir];
+      hipLaunchKernelGGL(k, dim3(nwin), dim3(CF4_T), 0, s, T9, dY, S, c->w_win.as<DevWin>(),
+                         c->w_seg.as<DevSeg>(), c->w_word.as<int32_t>(), c->w_r4.as<double>(), cnt + NJ, am, mwords,
+                         ws);
+    }
   }
   c->tend(th);
   Recs recs;
@@ -1685,7 +1700,7 @@ static int launch_jobs(lblx_ctx *c, const lblx_run *run, const JobsPrep &J, void
     const size_t esz = split ? sizeof(float) : sizeof(double);
     // FP32 shapes: accumulator in FP64 LDS for the grids in LBLX_FP32_LDS64
     const bool l64 = !split || ((LBLX_FP32_LDS64 >> G) & 1);
-    size_t shm = (size_t)T * (l64 ? 8 : esz) + (size_t)(voigt ? 0 : (G == 1 ? 3 : 1) * 2601) * esz;
+    size_t shm = (size_t)(T + 1) * (l64 ? 8 : esz) + (size_t)(voigt ? 0 : (G == 1 ? 3 : 1) * 2601) * esz;
     const Recs rec = recs;
     const DevPanel *dp = c->w_pan.as<DevPanel>();
     const DevWin *dw = c->w_win.as<DevWin>();

@@ -144,6 +144,7 @@ __device__ __forceinline__ bool linf4_line(const DevLines &L, const DevLayer &Y,
   double sui = (double)L.sp[i] * Y.wk[m - 1];
   double alfa = L.alfa[i], hwhm = L.hwhm[i];
   if (m == 7) alfa = (alfa - 0.21 * hwhm) / (1.0 - 0.21);
+  if (x == 0.) sui = 2. * sui;  // a line at 0 cm-1 counts twice (oprop.f90:3172)
   double epp = L.epp[i];
   if (epp <= -1.001) epp = fabs(epp);
   if (epp > -0.999) sui = sui * Y.scor[m - 1][iso - 1] * exp(-epp * Y.betacr4) * (1. + exp(-x * Y.beta4));
@@ -933,16 +934,20 @@ __device__ __forceinline__ T f4_val(const F4Par<T> &p, T cd, double xmd) {
 // Five waves per SIMD (registers held to 96): with the 29 KB of LDS of the
 // FP64 build that keeps five windows per compute unit resident, so the
 // 1230 windows of a 50-layer column run in one round.
-// CPL: the line file holds coupled lines (a separate build of the kernel,
-// so that files without them run the plain code).
-template <typename T, bool CPL>
+// CPL: the line file holds coupled lines; MIR: some window reaches below
+// 50 cm-1, where lines at 0 < VNU <= 25 add their Van Vleck - Weisskopf
+// mirror at -VNU (oprop.f90:4014-4024), a second entry after the line's
+// own.  Separate builds of the kernel, so that other runs keep the plain
+// code.
+template <typename T, bool CPL, bool MIR>
 __global__ __launch_bounds__(CF4_T) __attribute__((amdgpu_waves_per_eu(5)))
 void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const DevSeg *segs,
               const int32_t *worder, double *r4out, unsigned long long *k3upd, uint64_t *amask,
               int64_t mwords, int32_t *wspan) {
   __shared__ T r4[R4ROW];
-  __shared__ F4Par<T> par[2][64];
-  __shared__ T pcd[CPL ? 2 : 1][CPL ? 64 : 1];  // DPTRAT of the lines in par
+  constexpr int NPAR = MIR ? 128 : 64;  // entries a chunk of 64 lines can add
+  __shared__ F4Par<T> par[2][NPAR];
+  __shared__ T pcd[CPL ? 2 : 1][CPL ? NPAR : 1];  // DPTRAT of the entries in par
   __shared__ int s_nacc2[2], s_done2[2], s_b;
   const int tid = threadIdx.x;
   const int wi = worder[blockIdx.x];
@@ -1001,10 +1006,14 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
     const int i = npos + lane;
     const bool valid = more && i < nl;
     // wave 0's lines of the chunk it decides (unused by waves 1-3)
-    F4Par<T> p;
+    F4Par<T> p, pm;  // the line, and its mirror (MIR)
     p.jmin = 1;
     p.jmax = 0;
     p.xnui = 0.;
+    pm.jmin = 1;
+    pm.jmax = 0;
+    pm.xnui = 0.;
+    bool has_m = false;
     T p_cd = (T)0.;
     T speak = (T)0., partial = (T)0., pkb = (T)0., f4bnd = (T)0.;
     double vnui = 0.;
@@ -1083,6 +1092,17 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
         }
         pkb = (T)pkd;
         partial = r4[jj];
+        if (MIR && vnui > 0. && vnui <= 25. && jmin <= nptr4) {
+          // label 50: the same shape at -VNU, SPP negated (not after GO TO 60)
+          has_m = true;
+          pm = p;
+          pm.xnui = -vnui - W.v1r4;
+          const int mn = fint((pm.xnui - bound4) / dvr4 + 2.);
+          const int mx = fint((pm.xnui + bound4) / dvr4 + 1.);
+          pm.jmin = mn < 1 ? 1 : mn;
+          pm.jmax = mx > nptr4 ? nptr4 : mx;
+          if (mn > nptr4) pm.jmax = 0;
+        }
       }
     }
     __syncthreads();
@@ -1146,6 +1166,10 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
       // (their exact running R4 is needed); the rest are accepted in bulk.
       T neg = cand ? (T)2. * f4bnd : (T)0.;
       T pk = cand ? pkb : (T)0.;
+      if (MIR && has_m) {  // the mirror adds as much again
+        neg = neg + neg;
+        pk = pk + pk;
+      }
       T pre = neg, preu = pk;
       for (int d = 1; d < 64; d <<= 1) {
         T t = __shfl_up(pre, d);
@@ -1209,6 +1233,19 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
             nacc++;
             am |= 1ull << j;
           }
+          if (MIR && __builtin_amdgcn_readlane(has_m ? 1 : 0, j)) {
+            q.xnui = rl_f64(pm.xnui, j);
+            q.jmin = __builtin_amdgcn_readlane(pm.jmin, j);
+            q.jmax = __builtin_amdgcn_readlane(pm.jmax, j);
+            if (lane > j && jj >= q.jmin && jj <= q.jmax) partial += f4_val<CPL>(q, -qcd, xjj_me - q.xnui);
+            if (q.jmax >= q.jmin) {
+              if (lane == j) {
+                par[nb][nacc] = pm;
+                if (CPL) pcd[nb][nacc] = -p_cd;
+              }
+              nacc++;
+            }
+          }
         }
       }
       if (!done) {  // lanes after the last undecided one: accepted in bulk
@@ -1220,12 +1257,19 @@ void k_convf4(Tape9 T9, const DevLayer *Ys, Strides S, const DevWin *wins, const
           done = 1;
         }
         const uint64_t addm = rest & __ballot(p.jmax >= p.jmin);
-        const uint64_t below_me = lane == 0 ? 0ull : (addm & ((1ull << lane) - 1));
+        const uint64_t mirm = MIR ? rest & __ballot(has_m && pm.jmax >= pm.jmin) : 0ull;
+        const uint64_t below = lane == 0 ? 0ull : ((1ull << lane) - 1);
+        const int slot = nacc + __builtin_popcountll(addm & below) + __builtin_popcountll(mirm & below);
         if ((addm >> lane) & 1) {
-          par[nb][nacc + __builtin_popcountll(below_me)] = p;
-          if (CPL) pcd[nb][nacc + __builtin_popcountll(below_me)] = p_cd;
+          par[nb][slot] = p;
+          if (CPL) pcd[nb][slot] = p_cd;
         }
-        nacc += __builtin_popcountll(addm);
+        if (MIR && ((mirm >> lane) & 1)) {  // after the line's own entry
+          const int sm = slot + (int)((addm >> lane) & 1);
+          par[nb][sm] = pm;
+          if (CPL) pcd[nb][sm] = -p_cd;
+        }
+        nacc += __builtin_popcountll(addm) + __builtin_popcountll(mirm);
         am |= addm;
       }
       if (lane == 0) {
@@ -1794,7 +1838,9 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
                                               const DevSeg *segs, const double *rin, double *rout,
                                               OUT *od) {
   extern __shared__ double sm_d[];
-  LACC *acc = reinterpret_cast<LACC *>(sm_d);        // [tile]
+  // [-1, tile): the slot before the tile holds the grid point below VFT
+  // for RSYM on the first tile (R(0) of the reference's RR arrays)
+  LACC *acc = reinterpret_cast<LACC *>(sm_d) + 1;
   ACC *t1 = reinterpret_cast<ACC *>(acc + tile);     // F_G table
   ACC *tg = t1 + NFMX;              // FG   (G = 1)
   ACC *tv = tg + NFMX;              // XVER (G = 1)
@@ -1802,7 +1848,7 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
   const int job = M.job[blockIdx.x];
   const DevLayer &Y = Ys[job];
   const int64_t rbase = job * S.line;
-  for (int k = tid; k < tile; k += TILE_T) acc[k] = (LACC)0.;
+  for (int k = tid - 1; k < tile; k += TILE_T) acc[k] = (LACC)0.;
   if (!V) {
     const double *src = tabs + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NFMX;
     for (int k = tid; k < NFMX; k += TILE_T) t1[k] = (ACC)src[k];
@@ -1816,6 +1862,8 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
   const ACC *vt = vtab + (size_t)(G == 1 ? 0 : (G == 2 ? 1 : 2)) * NZETA * NFMX;
   __syncthreads();
   const int64_t T0 = (int64_t)M.loc[blockIdx.x] * tile;
+  const bool fold = T0 == 0 && Y.vft0 <= 0.;  // RSYM reads the point below VFT too
+  const int64_t tlo = fold ? -1 : T0;
   const int lo = M.lo[blockIdx.x], hi = M.hi[blockIdx.x];
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
@@ -1848,7 +1896,7 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
       const ACC za = V ? (ACC)rl_f64(cur.strd, j) : (ACC)0.;
       const ACC *ca = vt;
       if (V) ca = vt + (size_t)(__builtin_amdgcn_readlane(cur.izm, j) - 1) * NFMX;
-      int64_t a = glo > T0 ? glo : T0;
+      int64_t a = glo > tlo ? glo : tlo;
       int64_t b = glo + npts - 1;
       if (b > T0 + tile - 1) b = T0 + tile - 1;
       // tile-local point index kl; the line's point offset is kl + d0
@@ -1880,6 +1928,46 @@ __global__ __launch_bounds__(TILE_T) void k_tile(Recs recs, TileMap M, int tile,
     cur = nxt;
   }
   __syncthreads();
+  if (fold) {
+    // RSYM (oprop.f90:501-505, 2145-2200): panel 0 starts at or below 0
+    // cm-1, so the lines' values at negative wavenumbers and at their
+    // mirror images are folded onto each other before the coarser grid and
+    // the far wing are added.  Point R(j) = acc[j - 1] (j >= 0); IP is the first
+    // point at or above 0, pair n = (I, K) = (IP + n, IP - 1 - n) for
+    // I <= 2 IP - 1 (at most 34 pairs: VFT >= -32 DV), and each update
+    // reads the values before the fold, so wave 0 reads, then writes.
+    const double dvg = G == 1 ? Y.dv : (G == 2 ? Y.dvr2 : Y.dvr3);
+    const int ip = fint(-Y.vft0 / dvg + 1. - .000001) + 1;
+    double pp = ((double)(ip - 1) + Y.vft0 / dvg) * 2.;
+    const bool low = pp <= 1.;
+    if (!low) pp = pp - 1.;
+    const double wn1 = -pp * (pp - 1.) * (pp - 2.) / 6., w0 = (pp * pp - 1.) * (pp - 2.) / 2.;
+    const double w1 = -pp * (pp + 1.) * (pp - 2.) / 2., w2 = pp * (pp * pp - 1.) / 6.;
+    const int i = ip + tid, k = ip - 1 - tid;
+    double ri = 0., rk = 0.;
+    bool wi = false, wk = false;
+    auto R = [&](int j) { return (double)acc[j - 1]; };
+    if (tid < 64 && i <= ip + ip - 1 && ip + ip + 1 <= tile) {
+      if (low) {  // the walk stops at the first K <= 1
+        if (k > 1) {
+          ri = R(i) + wn1 * R(k + 2) + w0 * R(k + 1) + w1 * R(k) + w2 * R(k - 1);
+          rk = R(k) + wn1 * R(i + 2) + w0 * R(i + 1) + w1 * R(i) + w2 * R(i - 1);
+          wi = wk = true;
+        }
+      } else if (k >= 1) {  // stops at the first K < 1; I untouched for K <= 2
+        rk = R(k) + wn1 * R(i + 1) + w0 * R(i) + w1 * R(i - 1) + w2 * R(i - 2);
+        wk = true;
+        if (k > 2) {
+          ri = R(i) + wn1 * R(k + 1) + w0 * R(k) + w1 * R(k - 1) + w2 * R(k - 2);
+          wi = true;
+        }
+      }
+    }
+    __syncthreads();
+    if (wi) acc[i - 1] = (LACC)ri;
+    if (wk) acc[k - 1] = (LACC)rk;
+    __syncthreads();
+  }
   const double X00 = -7. / 128., X01 = 105. / 128., X02 = 35. / 128., X03 = -5. / 128.;
   const double X10 = -1. / 16., X11 = 9. / 16.;
   const DevPanel *pan = pans + Y.pan0;

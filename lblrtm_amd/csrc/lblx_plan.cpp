@@ -402,7 +402,6 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   P->v2r4st = V2 + 2. * P->dvr4;
   P->v1l4 = P->v1r4st - P->bound4 - P->dvr4;
   P->v2l4 = P->v2r4st + P->bound4 + 2 * P->dvr4;
-  if (run->ilblf4 >= 1 && P->v1l4 <= 25.) return LBLX_E_UNSUPPORTED;  // no VVW mirror / RSYM on device
   // HIRAC1 (oprop.f90:223-315)
   P->dptmn_hirac = run->dptmin;
   if (P->jrad != 1) P->dptmn_hirac = run->dptmin / radfn(V2, Ly->tave / RADCN2);
@@ -422,7 +421,6 @@ int plan_layer(const lblx_run *run, lblx_layer *Ly, LayerPlan *P, int icflg) {
   P->vft0 = V1 - (double)NSHIFT * DV;
   P->vbot = V1 - P->bound;
   P->vtop = V2 + P->bound;
-  if (P->vft0 <= 0.) return LBLX_E_UNSUPPORTED;
   // molecular quantities (MOLEC, LNCOR1 and LINF4 first-call constants)
   P->rhorat = (Ly->pave / P0) * (TEMP0 / Ly->tave);
   P->tratio = Ly->tave / TEMP0;

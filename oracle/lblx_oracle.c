@@ -1072,67 +1072,79 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
       if (speak <= (dptmn + dptfc * r4[jj]) && spp == 0.) { log_dec(0, g_log_win, i, 0); continue; }
       log_dec(0, g_log_win, i, 1);
       *ilin4 += 1;
+      /* label 30 .. 50: the line, then for 0 < VNU <= 25 its Van Vleck -
+         Weisskopf mirror at -VNU with SPP negated (oprop.f90:3919-4024) */
       double vnui = T9->vnu[i];
-      double xnui = vnui - W->v1r4;
-      int jmin = fint((xnui - W->bound4) / W->dvr4 + 2.);
-      if (vnui >= vnulst) { done = 1; break; }
-      if (jmin > W->nptr4) continue;
-      if (jmin < 1) jmin = 1;
-      int jmax = fint((xnui + W->bound4) / W->dvr4 + 1.);
-      if (jmax < jmin) continue; /* label 50: VVW mirror only for 0<vnu<=25 */
-      if (jmax > W->nptr4) jmax = W->nptr4;
-      double alfli2 = alfali * alfali;
-      double alfvi2 = alfavi * alfavi;
-      double xjj = (double)(jmin - 1) * W->dvr4;
-      double f4bnd = sil / (alfli2 + bndsq);
-      double dptrat = spp != 0. ? spp / (sp * alfavi) : 0.; /* line coupling (oprop.f90:3976) */
-      double rec_alfvi2 = 1. / alfvi2;
-      double siv_a3 = siv * A3;
-      double siv_b3 = siv * B3;
-      *k3upd += jmax - jmin + 1;
-      if (g_voigt) { /* src/oprop_voigt.f90:3929-3971: CO2 only through CHI (= 1) */
-        double dptrat_v = spp != 0. ? spp / (sp * alfavi) : 0.;
-        double siv_a = siv * a3x, siv_b = siv * b3x;
-        double siv_64 = siv * f4_64x * (alfli2 + alfvi2 * zsqbnd);
-        double f4bnd_v = siv_64 / (alfli2 + bndsq);
-        for (int j = jmin; j <= jmax; j++) {
-          double xm = (xjj - xnui);
-          double xmsq = xm * xm;
-          double zvsq = xmsq * rec_alfvi2;
-          double f4fn;
-          if (zvsq <= zsqbnd) f4fn = (siv_a + zvsq * siv_b) - f4bnd_v;
-          else f4fn = siv_64 / (alfli2 + xmsq) - f4bnd_v;
-          if (spp != 0.) f4fn = f4fn + xm * dptrat_v * f4fn;
-          if (T9->mol[i] == 2 && spp == 0.) r4[j] = r4[j] + f4fn * 1.0;
-          else r4[j] = r4[j] + f4fn;
-          xjj = xjj + W->dvr4;
+      double sppi = spp;
+      for (;;) {
+        double xnui = vnui - W->v1r4;
+        int jmin = fint((xnui - W->bound4) / W->dvr4 + 2.);
+        if (vnui >= vnulst) { done = 1; break; }
+        if (jmin > W->nptr4) break; /* GO TO 60 */
+        if (jmin < 1) jmin = 1;
+        int jmax = fint((xnui + W->bound4) / W->dvr4 + 1.);
+        if (jmax >= jmin) {
+          if (jmax > W->nptr4) jmax = W->nptr4;
+          double alfli2 = alfali * alfali;
+          double alfvi2 = alfavi * alfavi;
+          double xjj = (double)(jmin - 1) * W->dvr4;
+          double f4bnd = sil / (alfli2 + bndsq);
+          double dptrat = sppi != 0. ? sppi / (sp * alfavi) : 0.; /* line coupling (oprop.f90:3976) */
+          double rec_alfvi2 = 1. / alfvi2;
+          double siv_a3 = siv * A3;
+          double siv_b3 = siv * B3;
+          *k3upd += jmax - jmin + 1;
+          if (g_voigt) { /* src/oprop_voigt.f90:3929-3971: CO2 only through CHI (= 1) */
+            double siv_a = siv * a3x, siv_b = siv * b3x;
+            double siv_64 = siv * f4_64x * (alfli2 + alfvi2 * zsqbnd);
+            double f4bnd_v = siv_64 / (alfli2 + bndsq);
+            for (int j = jmin; j <= jmax; j++) {
+              double xm = (xjj - xnui);
+              double xmsq = xm * xm;
+              double zvsq = xmsq * rec_alfvi2;
+              double f4fn;
+              if (zvsq <= zsqbnd) f4fn = (siv_a + zvsq * siv_b) - f4bnd_v;
+              else f4fn = siv_64 / (alfli2 + xmsq) - f4bnd_v;
+              if (sppi != 0.) f4fn = f4fn + xm * dptrat * f4fn;
+              if (T9->mol[i] == 2 && sppi == 0.) r4[j] = r4[j] + f4fn * 1.0;
+              else r4[j] = r4[j] + f4fn;
+              xjj = xjj + W->dvr4;
+            }
+          } else if (T9->mol[i] == 2) {
+            for (int j = jmin; j <= jmax; j++) {
+              double xm = (xjj - xnui);
+              double xmsq = xm * xm;
+              double zvsq = xmsq * rec_alfvi2;
+              double fcnt_fn = (2. - (xmsq * r_bndsq)) * f4bnd;
+              double f4fn;
+              if (zvsq <= zsqbnd) f4fn = (siv_a3 + zvsq * siv_b3) - fcnt_fn;
+              else f4fn = sil / (alfli2 + xmsq) - fcnt_fn;
+              if (sppi != 0.) f4fn = f4fn + xm * dptrat * f4fn;
+              r4[j] = r4[j] + f4fn * 1.0; /* CHI == 1 (oprop.f90:4075); coupled lines add F4FN */
+              xjj = xjj + W->dvr4;
+            }
+          } else {
+            for (int j = jmin; j <= jmax; j++) {
+              double xm = (xjj - xnui);
+              double xmsq = xm * xm;
+              double zvsq = xmsq * rec_alfvi2;
+              double f4fn;
+              if (zvsq <= zsqbnd) f4fn = (siv_a3 + zvsq * siv_b3) - f4bnd;
+              else f4fn = sil / (alfli2 + xmsq) - f4bnd;
+              if (sppi != 0.) f4fn = f4fn + xm * dptrat * f4fn;
+              r4[j] = r4[j] + f4fn;
+              xjj = xjj + W->dvr4;
+            }
+          }
         }
-      } else if (T9->mol[i] == 2) {
-        for (int j = jmin; j <= jmax; j++) {
-          double xm = (xjj - xnui);
-          double xmsq = xm * xm;
-          double zvsq = xmsq * rec_alfvi2;
-          double fcnt_fn = (2. - (xmsq * r_bndsq)) * f4bnd;
-          double f4fn;
-          if (zvsq <= zsqbnd) f4fn = (siv_a3 + zvsq * siv_b3) - fcnt_fn;
-          else f4fn = sil / (alfli2 + xmsq) - fcnt_fn;
-          if (spp != 0.) f4fn = f4fn + xm * dptrat * f4fn;
-          r4[j] = r4[j] + f4fn * 1.0; /* CHI == 1 (oprop.f90:4075); coupled lines add F4FN */
-          xjj = xjj + W->dvr4;
+        if (vnui > 0. && vnui <= 25.) { /* label 50 */
+          vnui = -T9->vnu[i];
+          sppi = -sppi;
+          continue;
         }
-      } else {
-        for (int j = jmin; j <= jmax; j++) {
-          double xm = (xjj - xnui);
-          double xmsq = xm * xm;
-          double zvsq = xmsq * rec_alfvi2;
-          double f4fn;
-          if (zvsq <= zsqbnd) f4fn = (siv_a3 + zvsq * siv_b3) - f4bnd;
-          else f4fn = sil / (alfli2 + xmsq) - f4bnd;
-          if (spp != 0.) f4fn = f4fn + xm * dptrat * f4fn;
-          r4[j] = r4[j] + f4fn;
-          xjj = xjj + W->dvr4;
-        }
+        break;
       }
+      if (done) break;
     }
   }
   g_log_win++;
@@ -1157,6 +1169,41 @@ static void lblf4(lblf_t *W, const tape9_t *T9, const lblx_layer *Ly, int jrad,
 }
 
 /* ------------------------------------------------------------------ */
+/* RSYM (oprop.f90:2145-2200).  With VFT <= 0 the values of a grid at
+ * negative wavenumbers and at their mirror images are folded onto each
+ * other: point IP is the first at or above 0, P its offset (in half
+ * steps, reduced to [0, 1]) and (WN1, W0, W1, W2) the 4-point Lagrange
+ * weights at P; pair n is I = IP + n with K = IP - 1 - n, up to
+ * I = 2 IP - 1, and every update reads the values before the fold.        */
+static void rsym(double *R /* 1-based */, double dv, double vft)
+{
+  int ip = fint(-vft / dv + 1. - .000001) + 1;
+  double p = ((double)(ip - 1) + vft / dv) * 2.;
+  const double pst = p;
+  if (p > 1.) p = p - 1.;
+  const double wn1 = -p * (p - 1.) * (p - 2.) / 6.;
+  const double w0 = (p * p - 1.) * (p - 2.) / 2.;
+  const double w1 = -p * (p + 1.) * (p - 2.) / 2.;
+  const double w2 = p * (p * p - 1.) / 6.;
+  const int ipmax = ip + ip - 1;
+  double o[160]; /* R(k) before the fold, k = 0 .. ipmax + 2 (R(0): the point below VFT) */
+  if (ipmax + 2 >= 160) return; /* VFT >= -32 DV keeps ip <= 34 */
+  for (int k = 0; k <= ipmax + 2; k++) o[k] = R[k];
+  for (int n = 0; ip + n <= ipmax; n++) {
+    const int i = ip + n, k = ip - 1 - n;
+    if (pst <= 1.) {
+      if (k <= 1) break;
+      R[i] = o[i] + wn1 * o[k + 2] + w0 * o[k + 1] + w1 * o[k] + w2 * o[k - 1];
+      R[k] = o[k] + wn1 * o[i + 2] + w0 * o[i + 1] + w1 * o[i] + w2 * o[i - 1];
+    } else {
+      if (k < 1) break;
+      R[k] = o[k] + wn1 * o[i + 1] + w0 * o[i] + w1 * o[i - 1] + w2 * o[i - 2];
+      if (k <= 2) continue;
+      R[i] = o[i] + wn1 * o[k + 1] + w0 * o[k] + w1 * o[k - 1] + w2 * o[k - 2];
+    }
+  }
+}
+
 static void xint(double v1a, double v2a, double dva, const double *A /*1-based*/, double afact,
                  double vft, double dvr3, double *R3 /*1-based*/, int n1r3, int n2r3)
 {
@@ -2042,7 +2089,11 @@ int oracle_od_layer(const lblx_run *run, lblx_layer *Ly, const lblx_lines *L,
       N2R1 = NLIM1 + NSHIFT - 1;
       N2R2 = NLIM2 + (NSHIFT / 4) - 1 + 4;
       N2R3 = NLIM3 + (NSHIFT / 16) - 1 + 3;
-      if (VFT <= 0.) { rc = LBLX_E_UNSUPPORTED; goto done_hirac; } /* RSYM path */
+      if (VFT <= 0.) { /* oprop.f90:501-505 */
+        rsym(R1, DV, VFT);
+        rsym(R2, DVR2, VFT);
+        rsym(R3, DVR3, VFT);
+      }
       if (ilblf4 >= 1)
         xint(W->v1r4, W->v2r4, DVR4, W->r4, 1.0, VFT, DVR3, R3, N1R3, N2R3);
       if (icntnm >= 1 && absrb1)
