@@ -76,10 +76,10 @@ def compute_view(stage, stats, per):
 
 # stage -> kernel of the FP64 column (names as rocprofv3 lists them)
 STAGE_KERNEL = {
-    "convf4": "k_convf4<double>",
-    "tile_r1": "k_tile<1, double, double, false, double>",
-    "tile_r2": "k_tile<2, double, double, false, double>",
-    "tile_r3": "k_tile<3, double, double, false, double>",
+    "convf4": "k_convf4<double, false, false>",
+    "tile_r1": "k_tile<1, double, double, false, double, false>",
+    "tile_r2": "k_tile<2, double, double, false, double, false>",
+    "tile_r3": "k_tile<3, double, double, false, double, false>",
     "merge": "k_merge_run<double>",
     "scatter_rec": "k_scatter_rec",
     "lncor1": "k_lncor1",
