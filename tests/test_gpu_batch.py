@@ -150,3 +150,43 @@ def test_jacobians_batch_equal_single(gpu_ctx, aj):
         # FP32 ulp
         jk = jb[joff[k]:joff[k] + jsz[k]].reshape(js.shape)
         np.testing.assert_allclose(jk, js, rtol=1e-5, atol=1e-6 * np.abs(js).max())
+
+
+def test_batch_with_coupling_and_far_ir(gpu_ctx):
+    """A batch mixing profiles in the far IR (first panels at or below 0
+    cm-1, the VVW mirror) and mid-IR windows over a line file with coupled
+    lines: each profile equals the same profile run alone."""
+    tab = synth.coupled_lines(60000, 0.3, 1300.0, [1, 2, 3, 7], 57)
+    gpu_ctx.load_lines(tab)
+    profs = synth.batch_profiles(4, seed=79)
+    bands = [(0.1, 60.0), (10.0, 120.0), (600.0, 900.0), (0.0, 40.0)]
+    run = abi.make_run(0.0, 1300.0, icntnm=1)
+    items, runs = [], []
+    for atm, (v1, v2) in zip(profs, bands):
+        atm = dict(atm, v1=v1, v2=v2)
+        r = abi.make_run(v1, v2, icntnm=1)
+        runs.append(r)
+        items.append((_profile_layers(r, atm), atm["tlev"][1:], atm["tlev"][0], 0.98))
+    sizes = [lib.od_size(r, it[0][-1]) for r, it in zip(runs, items)]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    rad = gpu_ctx.alloc(8 * int(sum(sizes)))
+    tra = gpu_ctx.alloc(8 * int(sum(sizes)))
+    v1p = gpu_ctx.spectra_batch_device(run, items, rad.ptr, tra.ptr, offs)
+    rb = rad.to_host(np.float64)
+    tb = tra.to_host(np.float64)
+    for k, it in enumerate(items):
+        rs, ts, v1s, _ = gpu_ctx.spectrum(runs[k], [abi.Layer.from_buffer_copy(x) for x in it[0]], it[1], it[2],
+                                          it[3])
+        print("profile %d: %d layers %.1f-%.1f cm-1, %d points" % (k, len(it[0]), bands[k][0], bands[k][1], len(rs)),
+              flush=True)
+        assert v1p[k] == v1s
+        np.testing.assert_allclose(rb[offs[k]:offs[k] + sizes[k]], rs, rtol=1e-12)
+        # far-IR columns are opaque (T down to 1e-185): compare the total
+        # optical depth -ln T, whose rounding (LDS atomics in another order)
+        # stays ~1e-13 relative
+        tk = tb[offs[k]:offs[k] + sizes[k]]
+        pos = (tk > 0) & (ts > 0)
+        assert np.array_equal(tk > 0, ts > 0)
+        np.testing.assert_allclose(-np.log(tk[pos]), -np.log(ts[pos]), rtol=1e-11, atol=1e-12)
+    rad.free()
+    tra.free()
