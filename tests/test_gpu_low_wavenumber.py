@@ -82,3 +82,18 @@ def test_far_ir_column(gpu_ctx):
         ref, _, _, _ = O.od_layer(run, inp, tab)
         print("far-IR column layer", l, "npts", sizes[l], flush=True)
         assert_od_close(allod[offs[l]:offs[l] + sizes[l]], ref, "far-IR layer %d" % l)
+
+
+def test_tangent_refuses_far_ir(gpu_ctx):
+    """The tangent-linear passes have neither the fold nor the mirror: a
+    band reaching below 25 cm-1 is refused, not differentiated wrongly."""
+    tab = synth.synth_lines(500, 0.3, 60.0, [1], 59)
+    gpu_ctx.load_lines(tab)
+    p, t, wk, wb = 700.0, 260.0, [2e19], 1.5e25
+    for v1, v2 in ((0.1, 40.0), (30.0, 60.0)):
+        run = abi.make_run(v1, v2)
+        dv, _ = lib.plan_dv(run, [p], [t], [wk], [wb])
+        lay = abi.make_layer(p, t, wk, wb, dv[0], v1, v2)
+        with pytest.raises(lib.LblxError) as e:
+            gpu_ctx.od_tangent(run, [lay], [1])
+        assert e.value.code == abi.E_UNSUPPORTED
