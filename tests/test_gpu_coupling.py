@@ -159,3 +159,21 @@ def test_refusals(gpu_ctx):
     with pytest.raises(lib.LblxError) as e:
         gpu_ctx.od_tangent(run, [abi.Layer.from_buffer_copy(lay)], [2])
     assert e.value.code == abi.E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("v1", [610.0, 0.2])
+def test_coupled_lines_without_far_wing(gpu_ctx, v1):
+    """ILBLF4 = 0 with coupled lines: the near wing alone, coupled and IFLG 2
+    lines exempt from the DPTMIN test; in the mid IR and near 0 cm-1."""
+    if v1 < 1.0:
+        tab = synth.coupled_lines(3000, 0.5, 80.0, [1, 3], 61, frac1=0.2, frac3=0.05)
+        v2, wk = 60.0, [2e19, 0.0, 1e17]
+    else:
+        tab = synth.coupled_lines(20000, 600.0, 1100.0, [1, 2, 3], 41)
+        v2, wk = 1090.0, [1e22, 4e21, 1e18]
+    gpu_ctx.load_lines(tab)
+    run = abi.make_run(v1, v2, ilblf4=0, icntnm=1)
+    od, ref, st, st_r = _both(gpu_ctx, tab, run, 500.0, 250.0, wk, 1.2e25, v1, v2)
+    assert_od_close(od, ref, "coupled, no far wing, v1=%g" % v1)
+    for k in ("npanel", "npts", "k1_updates", "lincnt"):
+        assert st[k] == st_r[k], k
